@@ -1,0 +1,302 @@
+// cv_ctx.hip -- context life cycle, device memory, field exchange and the
+// deterministic reductions of the CURVIB fractional-step library (gfx950).
+#include "cv_internal.h"
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+static thread_local char g_err[512];
+
+void cv_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+extern "C" int cv_last_error(char *buf, int len) {
+  if (!buf || len <= 0) return CV_ERR_ARG;
+  strncpy(buf, g_err, (size_t)len - 1);
+  buf[len - 1] = 0;
+  return CV_OK;
+}
+
+int cv_alloc(cv_ctx *c, void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    cv_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    (void)hipGetLastError();
+    return CV_ERR_OOM;
+  }
+  c->allocs.push_back(*p);
+  c->bytes_alloc += bytes;
+  return CV_OK;
+}
+
+__global__ void k_copy(double *dst, const double *src, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+__global__ void k_fill(double *dst, double v, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    dst[t] = v;
+}
+__global__ void k_axpby(double *y, double a, const double *x, double b, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    y[t] = a * x[t] + b * y[t];
+}
+
+extern "C" void cv_default_solver(cv_solver_opts *o) {
+  o->precision = CV_FP64;
+  o->restart = 30;
+  o->max_it = 200;
+  o->rtol = 1e-10;
+  o->atol = 0.0;
+  o->pre_sweeps = 2;
+  o->post_sweeps = 2;
+  o->coarse_sweeps = 40;
+  o->smoother = CV_SMOOTH_JACOBI;
+  o->omega = 0.8;
+}
+
+extern "C" long cv_device_bytes(const cv_ctx *c) { return c ? (long)c->bytes_alloc : 0; }
+
+static int check_desc(const cv_grid_desc *d) {
+  if (!d) return 0;
+  if (d->IM < 4 || d->JM < 4 || d->KM < 4) return 0;
+  for (int q = 0; q < 6; q++)
+    if (d->bctype[q] != 1 && d->bctype[q] != 7) return 0;
+  if (d->bctype[2] == 7 || d->bctype[3] == 7) return 0;            // j-periodic not supported
+  if ((d->bctype[0] == 7) != (d->bctype[1] == 7)) return 0;
+  if ((d->bctype[4] == 7) != (d->bctype[5] == 7)) return 0;
+  if (d->mglevels < 1 || d->mglevels > CV_MAXLEV) return 0;
+  if (d->dt <= 0 || d->ren <= 0) return 0;
+  return 1;
+}
+
+extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out) {
+  if (!out) return CV_ERR_ARG;
+  *out = nullptr;
+  if (!check_desc(desc)) {
+    cv_set_error("cv_ctx_create: unsupported grid description");
+    return CV_ERR_ARG;
+  }
+  CV_CHECK(hipSetDevice(device));
+  cv_ctx *c = new cv_ctx();
+  c->device = device;
+  c->nlev = desc->mglevels;
+  c->IM = desc->IM; c->JM = desc->JM; c->KM = desc->KM;
+  memcpy(c->bctype, desc->bctype, sizeof(c->bctype));
+  c->ren = desc->ren; c->dt = desc->dt; c->st = desc->st > 0 ? desc->st : 1.0;
+  cv_default_solver(&c->opt);
+  // MGDACreate dimensions (init.c:78-99); each coarsening must keep an even
+  // cell count on coarsened axes so that fine cells pair up
+  int im = desc->IM, jm = desc->JM, km = desc->KM;
+  for (int l = c->nlev - 1; l >= 0; l--) {
+    CvLevel &L = c->lev[l];
+    if (l < c->nlev - 1) {
+      if ((!desc->isc && (im - 1) % 2) || (!desc->jsc && (jm - 1) % 2) || (!desc->ksc && (km - 1) % 2)) {
+        delete c;
+        cv_set_error("cv_ctx_create: grid cannot be restricted to %d levels", desc->mglevels);
+        return CV_ERR_ARG;
+      }
+      im = desc->isc ? im : (im + 1) / 2;
+      jm = desc->jsc ? jm : (jm + 1) / 2;
+      km = desc->ksc ? km : (km + 1) / 2;
+    }
+    if (im < 4 || jm < 4 || km < 4) {
+      delete c;
+      cv_set_error("cv_ctx_create: level %d too small (%d %d %d nodes)", l, im, jm, km);
+      return CV_ERR_ARG;
+    }
+    L.d.mx = im + 1; L.d.my = jm + 1; L.d.mz = km + 1;
+    L.d.n = (long)L.d.mx * L.d.my * L.d.mz;
+    L.d.xper = desc->bctype[0] == 7;
+    L.d.zper = desc->bctype[4] == 7;
+    L.d.isc = desc->isc; L.d.jsc = desc->jsc; L.d.ksc = desc->ksc;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    cv_set_error("cv_ctx_create: stream creation failed");
+    return CV_ERR_HIP;
+  }
+  int rc;
+  // per-level operator storage
+  for (int l = 0; l < c->nlev; l++) {
+    CvLevel &L = c->lev[l];
+    long n = L.d.n;
+    void *p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.nvert = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.aj = (double *)p;
+    for (int q = 0; q < 9; q++) { if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.g[q] = (double *)p; }
+    if ((rc = cv_alloc(c, &p, n * sizeof(uint32_t)))) goto fail; L.flags = (uint32_t *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.dinv = (double *)p;
+    for (int q = 0; q < 9; q++) { if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.gf[q] = (float *)p; }
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.ajf = (float *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.dinvf = (float *)p;
+    if ((rc = cv_alloc(c, &p, 3 * n * sizeof(double)))) goto fail; L.coor = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.x = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.b = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.r = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.xf = (float *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.bf = (float *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.rf = (float *)p;
+    CV_CHECK(hipMemsetAsync(L.nvert, 0, n * sizeof(double), c->stream));
+  }
+  {
+    const long n = c->lev[c->nlev - 1].d.n;
+    void *p;
+    CvMetrics &M = c->m;
+    double **vec3[] = {M.csi, M.eta, M.zet, M.icsi, M.ieta, M.izet, M.jcsi, M.jeta, M.jzet,
+                       M.kcsi, M.keta, M.kzet};
+    for (auto v : vec3)
+      for (int q = 0; q < 3; q++) { if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; v[q] = (double *)p; }
+    double **sc[] = {&M.aj, &M.iaj, &M.jaj, &M.kaj};
+    for (auto s : sc) { if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; *s = (double *)p; }
+    M.aj = c->lev[c->nlev - 1].aj ? M.aj : M.aj;
+    double **st3[] = {c->ucont, c->ucont_o, c->ucat, c->rhs, c->rct, c->conv, c->visc};
+    for (auto v : st3)
+      for (int q = 0; q < 3; q++) {
+        if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail;
+        v[q] = (double *)p;
+        CV_CHECK(hipMemsetAsync(v[q], 0, n * sizeof(double), c->stream));
+      }
+    double **st1[] = {&c->p, &c->phi, &c->bvec, &c->w};
+    for (auto s : st1) {
+      if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail;
+      *s = (double *)p;
+      CV_CHECK(hipMemsetAsync(*s, 0, n * sizeof(double), c->stream));
+    }
+    c->red_len = 1 << 16;
+    if ((rc = cv_alloc(c, &p, c->red_len * sizeof(double)))) goto fail; c->red = (double *)p;
+    if ((rc = cv_alloc(c, &p, 128 * sizeof(double)))) goto fail; c->dsc = (double *)p;
+    if ((rc = cv_alloc(c, &p, 34 * 1024 * sizeof(double)))) goto fail; c->part = (double *)p;
+    c->krylov_m = 0;
+    CV_CHECK(hipHostMalloc((void **)&c->red_host, 4096 * sizeof(double), hipHostMallocDefault));
+    c->V = nullptr; c->Z = nullptr;
+  }
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  *out = c;
+  return CV_OK;
+fail:
+  for (void *p : c->allocs) (void)hipFree(p);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return rc;
+}
+
+extern "C" int cv_ctx_destroy(cv_ctx *c) {
+  if (!c) return CV_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->red_host) (void)hipHostFree(c->red_host);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CV_OK;
+}
+
+// ---------------------------------------------------------------------
+// field exchange: host AoS (Cmpnts) <-> device planar
+// ---------------------------------------------------------------------
+static double *const *field_ptr(cv_ctx *c, int f, int level, int *dof) {
+  static thread_local double *tmp[3];
+  CvLevel &L = c->lev[level];
+  CvMetrics &M = c->m;
+  int fine = (level == c->nlev - 1);
+  *dof = 1;
+  switch (f) {
+    case CV_F_UCONT: *dof = 3; return fine ? c->ucont : nullptr;
+    case CV_F_UCONT_O: *dof = 3; return fine ? c->ucont_o : nullptr;
+    case CV_F_UCAT: *dof = 3; return fine ? c->ucat : nullptr;
+    case CV_F_RHS: *dof = 3; return fine ? c->rhs : nullptr;
+    case CV_F_P: tmp[0] = c->p; return fine ? tmp : nullptr;
+    case CV_F_PHI: tmp[0] = c->phi; return fine ? tmp : nullptr;
+    case CV_F_B: tmp[0] = c->bvec; return fine ? tmp : nullptr;
+    case CV_F_NVERT: tmp[0] = L.nvert; return tmp;
+    case CV_F_AJ: tmp[0] = L.aj; return tmp;
+    case CV_F_CSI: *dof = 3; return fine ? M.csi : nullptr;
+    case CV_F_ETA: *dof = 3; return fine ? M.eta : nullptr;
+    case CV_F_ZET: *dof = 3; return fine ? M.zet : nullptr;
+    case CV_F_ICSI: *dof = 3; return fine ? M.icsi : nullptr;
+    case CV_F_IETA: *dof = 3; return fine ? M.ieta : nullptr;
+    case CV_F_IZET: *dof = 3; return fine ? M.izet : nullptr;
+    case CV_F_JCSI: *dof = 3; return fine ? M.jcsi : nullptr;
+    case CV_F_JETA: *dof = 3; return fine ? M.jeta : nullptr;
+    case CV_F_JZET: *dof = 3; return fine ? M.jzet : nullptr;
+    case CV_F_KCSI: *dof = 3; return fine ? M.kcsi : nullptr;
+    case CV_F_KETA: *dof = 3; return fine ? M.keta : nullptr;
+    case CV_F_KZET: *dof = 3; return fine ? M.kzet : nullptr;
+    case CV_F_IAJ: tmp[0] = M.iaj; return fine ? tmp : nullptr;
+    case CV_F_JAJ: tmp[0] = M.jaj; return fine ? tmp : nullptr;
+    case CV_F_KAJ: tmp[0] = M.kaj; return fine ? tmp : nullptr;
+    default: return nullptr;
+  }
+}
+
+__global__ void k_aos_to_planar(const double *aos, double *x, double *y, double *z, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    x[t] = aos[3 * t]; y[t] = aos[3 * t + 1]; z[t] = aos[3 * t + 2];
+  }
+}
+__global__ void k_planar_to_aos(double *aos, const double *x, const double *y, const double *z, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    aos[3 * t] = x[t]; aos[3 * t + 1] = y[t]; aos[3 * t + 2] = z[t];
+  }
+}
+
+extern "C" int cv_get_field(cv_ctx *c, int f, int level, double *out) {
+  if (!c || !out) return CV_ERR_ARG;
+  if (level < 0) level = c->nlev - 1;
+  if (level >= c->nlev) return CV_ERR_ARG;
+  int dof;
+  double *const *ptr = field_ptr(c, f, level, &dof);
+  if (!ptr) { cv_set_error("cv_get_field: field %d not available on level %d", f, level); return CV_ERR_ARG; }
+  long n = c->lev[level].d.n;
+  if (dof == 1) {
+    CV_CHECK(hipMemcpyAsync(out, ptr[0], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    double *tmp = c->w;   // scratch: needs 3n; use rct (3 planar n arrays) contiguous? allocate once
+    void *scratch;
+    int rc = cv_alloc(c, &scratch, 3 * n * sizeof(double));
+    if (rc) return rc;
+    (void)tmp;
+    hipLaunchKernelGGL(k_planar_to_aos, grid1(n, 256), dim3(256), 0, c->stream, (double *)scratch,
+                       ptr[0], ptr[1], ptr[2], n);
+    CV_CHECK(hipMemcpyAsync(out, scratch, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CV_CHECK(hipStreamSynchronize(c->stream));
+    CV_CHECK(hipFree(scratch));
+    c->allocs.pop_back();
+    c->bytes_alloc -= 3 * n * sizeof(double);
+  }
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return CV_OK;
+}
+
+extern "C" int cv_put_field(cv_ctx *c, int f, int level, const double *in) {
+  if (!c || !in) return CV_ERR_ARG;
+  if (level < 0) level = c->nlev - 1;
+  if (level >= c->nlev) return CV_ERR_ARG;
+  int dof;
+  double *const *ptr = field_ptr(c, f, level, &dof);
+  if (!ptr) { cv_set_error("cv_put_field: field %d not available on level %d", f, level); return CV_ERR_ARG; }
+  long n = c->lev[level].d.n;
+  if (dof == 1) {
+    CV_CHECK(hipMemcpyAsync(ptr[0], in, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  } else {
+    void *scratch;
+    int rc = cv_alloc(c, &scratch, 3 * n * sizeof(double));
+    if (rc) return rc;
+    CV_CHECK(hipMemcpyAsync(scratch, in, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_aos_to_planar, grid1(n, 256), dim3(256), 0, c->stream, (const double *)scratch,
+                       ptr[0], ptr[1], ptr[2], n);
+    CV_CHECK(hipStreamSynchronize(c->stream));
+    CV_CHECK(hipFree(scratch));
+    c->allocs.pop_back();
+    c->bytes_alloc -= 3 * n * sizeof(double);
+  }
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return CV_OK;
+}

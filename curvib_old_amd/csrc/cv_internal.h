@@ -1,0 +1,118 @@
+// cv_internal.h -- private declarations of the CURVIB fractional-step
+// library for gfx950 (MI355X).  Device layout: every field is planar
+// (structure of arrays) in DMDA storage order, index i + mx*(j + my*k),
+// i fastest; a Cmpnts field is three planar arrays x|y|z.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <cstdio>
+#include <vector>
+#include <string>
+#include "../../include/curvib_hip.h"
+
+#define CV_MAXLEV 10
+
+#define CV_CHECK(expr)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      cv_set_error("%s:%d %s -> %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); \
+      return CV_ERR_HIP;                                                        \
+    }                                                                           \
+  } while (0)
+
+void cv_set_error(const char *fmt, ...);
+
+// per-cell operator word (see cv_poisson.hip):
+//   bit 0           fluid row
+//   bits 1+5d       face d (upper face of the cell) enabled
+//   bits 2+5d..3+5d cross mode of the first tangential axis of face d
+//   bits 4+5d..5+5d cross mode of the second tangential axis of face d
+// tangential axes of face d: d=0 -> (1,2), d=1 -> (0,2), d=2 -> (0,1)
+enum { XM_CENTRAL = 0, XM_LOWER = 1, XM_UPPER = 2, XM_DROP = 3 };
+
+struct CvDims {
+  int mx, my, mz;
+  long n;
+  int xper, zper;
+  int isc, jsc, ksc;
+};
+
+struct CvLevel {
+  CvDims d;
+  double *nvert;          // storage nvert (level nvert hierarchy)
+  double *aj;             // cell 1/J
+  double *g[9];           // g11 g12 g13 (i faces) g21 g22 g23 (j faces) g31 g32 g33 (k faces)
+  uint32_t *flags;
+  double *dinv;           // 1/diag of the operator row (smoother)
+  float *gf[9], *ajf, *dinvf;  // FP32 copies for the mixed-precision V-cycle
+  double *x, *b, *r;      // FP64 V-cycle work vectors
+  float *xf, *bf, *rf;    // FP32 V-cycle work vectors
+  double *coor;           // node coordinates, planar x|y|z (storage grid)
+};
+
+struct CvMetrics {        // fine-level metrics for the momentum right-hand side
+  double *csi[3], *eta[3], *zet[3], *aj;
+  double *icsi[3], *ieta[3], *izet[3], *iaj;
+  double *jcsi[3], *jeta[3], *jzet[3], *jaj;
+  double *kcsi[3], *keta[3], *kzet[3], *kaj;
+};
+
+struct cv_ctx {
+  int device;
+  hipStream_t stream;
+  int nlev;
+  int IM, JM, KM;
+  int bctype[6];
+  double ren, dt, st;
+  cv_solver_opts opt;
+  CvLevel lev[CV_MAXLEV];
+  CvMetrics m;
+  // fine-level state (planar)
+  double *ucont[3], *ucont_o[3], *ucat[3], *rhs[3], *p, *phi, *bvec;
+  double *rct[3], *conv[3], *visc[3];
+  // Krylov space
+  double *V, *Z, *w;      // (restart+1)*n, restart*n, n
+  double *red;            // reduction scratch (device)
+  double *red_host;       // pinned host mirror
+  int red_len;
+  cv_step_stats last;
+  double nfluid[CV_MAXLEV];   // fluid-row count per level (nullspace mean)
+  double *dsc;                // device scalars (64)
+  double *part;               // reduction partials (33 * CV_RED_GRID)
+  int krylov_m;               // restart length the Krylov space was sized for
+  size_t bytes_alloc;
+  std::vector<void *> allocs;
+  int geometry_set, nvert_set;
+};
+
+// helpers implemented in cv_ctx.hip
+int cv_alloc(cv_ctx *c, void **p, size_t bytes);
+int cv_reduce_sum(cv_ctx *c, const double *const *vecs_a, const double *const *vecs_b, int nvec,
+                  long n, double *out_host);
+
+static inline dim3 grid1(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+// kernels shared across translation units
+__global__ void k_copy(double *dst, const double *src, long n);
+__global__ void k_fill(double *dst, double v, long n);
+__global__ void k_axpby(double *y, double a, const double *x, double b, long n);   // y = a*x + b*y
+
+int cv_metrics_build(cv_ctx *c, const double *xyz_host);
+int cv_levels_setup_nvert(cv_ctx *c, const double *nvert_host);
+int cv_poisson_prepare(cv_ctx *c);
+int cv_contra2cart_dev(cv_ctx *c);
+int cv_ghost_update(cv_ctx *c, int formbcs);
+int cv_poisson_solve_dev(cv_ctx *c, cv_step_stats *st);
+int cv_update_pressure_dev(cv_ctx *c);
+int cv_projection_dev(cv_ctx *c);
+int cv_divergence_dev(cv_ctx *c, double *maxdiv, long *idx);
+int cv_volumeflux_rev_dev(cv_ctx *c, double *fa4);
+int cv_poisson_rhs_dev(cv_ctx *c, double *rhs_sum);
+int cv_ff1_dev(cv_ctx *c);
+int cv_rk_dev(cv_ctx *c);
+
+__device__ inline int cv_wrap(int i, int m, int per) {
+  return per ? (i == m - 1 ? 1 : (i == 0 ? m - 2 : i)) : i;
+}

@@ -1,0 +1,773 @@
+// cv_momentum.hip -- momentum right-hand side and the fractional-step
+// driver on gfx950.
+//
+//  Contra2Cart        rhs.c:25-144
+//  FormBCS            bcs.c:4436-5888  (bctype 1 wall, 7 periodic)
+//  GhostNodeVelocity  bcs.c:3645-4435  (same subset)
+//  FormFunction1      rhs.c:2567-3351 with Convection (QUICK, rhs.c:1463-1945)
+//                     and Viscous (laminar, rhs.c:1947-2565)
+//  RungeKutta         implicitsolver.c:32-119
+//  Flow_Solver        solvers.c:944-1156
+//
+// Face fluxes are recomputed by the two cells that share a face instead of
+// being stored (the reference writes three temporary face-flux vectors per
+// term); the arithmetic per face is the reference's.
+#include "cv_internal.h"
+#include "cv_reduce.h"
+#include <cmath>
+
+#define BS 256
+
+namespace {
+
+struct G {
+  int mx, my, mz, xper, zper;
+  long sj, sk;
+  __device__ long at(int i, int j, int k) const { return (long)i + (long)mx * ((long)j + (long)my * k); }
+  __device__ void dec(long t, int &i, int &j, int &k) const {
+    i = (int)(t % mx);
+    long r = t / mx;
+    j = (int)(r % my);
+    k = (int)(r / my);
+  }
+  // local-array index of the reference's ghosted vectors for the QUICK far
+  // ghosts (-1 -> m-3, m -> 2 on a periodic axis, rhs.c:1557-1628)
+  __device__ int fi(int i) const { return xper ? (i == -1 ? mx - 3 : (i == mx ? 2 : i)) : i; }
+  __device__ int fk(int k) const { return zper ? (k == -1 ? mz - 3 : (k == mz ? 2 : k)) : k; }
+  __device__ long atq(int i, int j, int k) const { return at(fi(i), j, fk(k)); }
+};
+
+struct V3 { double *x, *y, *z; };
+struct C3 { const double *x, *y, *z; };
+
+inline unsigned nblk(long n) {
+  long b = (n + BS - 1) / BS;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// ------------------------------------------------------------------
+// Contra2Cart (rhs.c:25-144)
+// ------------------------------------------------------------------
+__global__ void k_contra2cart(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.dec(t, i, j, k);
+    if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
+    const double nn = nv[t];
+    if (!(nn < 0.1 || nn > 7.)) continue;
+    const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
+    double m00 = 0.5 * (csi.x[ti] + csi.x[t]), m01 = 0.5 * (csi.y[ti] + csi.y[t]), m02 = 0.5 * (csi.z[ti] + csi.z[t]);
+    double m10 = 0.5 * (eta.x[tj] + eta.x[t]), m11 = 0.5 * (eta.y[tj] + eta.y[t]), m12 = 0.5 * (eta.z[tj] + eta.z[t]);
+    double m20 = 0.5 * (zet.x[tk] + zet.x[t]), m21 = 0.5 * (zet.y[tk] + zet.y[t]), m22 = 0.5 * (zet.z[tk] + zet.z[t]);
+    double q0 = 0.5 * (uc.x[ti] + uc.x[t]), q1 = 0.5 * (uc.y[tj] + uc.y[t]), q2 = 0.5 * (uc.z[tk] + uc.z[t]);
+    double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+    double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
+    double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
+    double det2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
+    ucat.x[t] = det0 / det;
+    ucat.y[t] = det1 / det;
+    ucat.z[t] = det2 / det;
+  }
+}
+
+// ------------------------------------------------------------------
+// ghost planes (FormBCS bcs.c:5252-5731, GhostNodeVelocity bcs.c:4015-4415)
+// ------------------------------------------------------------------
+// wall ghosts: ucat_g = 2*ubcs - ucat_inner with ubcs = 0.  full: whole
+// tangential ranges (GhostNodeVelocity); otherwise interior ranges (FormBCS)
+__global__ void k_wall_ghost(G g, int bx, int bz, int full, V3 u) {
+  const int mx = g.mx, my = g.my, mz = g.mz;
+  const int a0 = full ? 0 : 1;
+  const long nyz = (long)my * mz, nxz = (long)mx * mz, nxy = (long)mx * my;
+  const long tot = nyz + nxz + nxy;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+    long lo, hi, ln, in_lo, in_hi;
+    if (t < nyz) {                       // x planes
+      if (bx) continue;
+      int j = (int)(t % my), k = (int)(t / my);
+      if (j < a0 || j >= (full ? my : my - 1) || k < a0 || k >= (full ? mz : mz - 1)) continue;
+      lo = g.at(0, j, k); in_lo = g.at(1, j, k); hi = g.at(mx - 1, j, k); in_hi = g.at(mx - 2, j, k);
+      (void)ln;
+    } else if (t < nyz + nxz) {          // y planes
+      long r = t - nyz;
+      int i = (int)(r % mx), k = (int)(r / mx);
+      if (i < a0 || i >= (full ? mx : mx - 1) || k < a0 || k >= (full ? mz : mz - 1)) continue;
+      lo = g.at(i, 0, k); in_lo = g.at(i, 1, k); hi = g.at(i, my - 1, k); in_hi = g.at(i, my - 2, k);
+    } else {                             // z planes
+      if (bz) continue;
+      long r = t - nyz - nxz;
+      int i = (int)(r % mx), j = (int)(r / mx);
+      if (i < a0 || i >= (full ? mx : mx - 1) || j < a0 || j >= (full ? my : my - 1)) continue;
+      lo = g.at(i, j, 0); in_lo = g.at(i, j, 1); hi = g.at(i, j, mz - 1); in_hi = g.at(i, j, mz - 2);
+    }
+    u.x[lo] = 2 * 0.0 - u.x[in_lo]; u.y[lo] = 2 * 0.0 - u.y[in_lo]; u.z[lo] = 2 * 0.0 - u.z[in_lo];
+    u.x[hi] = 2 * 0.0 - u.x[in_hi]; u.y[hi] = 2 * 0.0 - u.y[in_hi]; u.z[hi] = 2 * 0.0 - u.z[in_hi];
+  }
+}
+
+// periodic plane copies restricted to the interior of the other axes
+// (bcs.c:5325-5431): all four planes read only interior cells
+__global__ void k_periodic_planes(G g, V3 u, double *p, double *nv) {
+  const int mx = g.mx, my = g.my, mz = g.mz;
+  const long nyz = (long)my * mz, nxy = (long)mx * my;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nyz + nxy; t += (long)gridDim.x * blockDim.x) {
+    long d0, s0, d1, s1;
+    if (t < nyz) {
+      if (!g.xper) continue;
+      int j = (int)(t % my), k = (int)(t / my);
+      if (!(j > 0 && k > 0 && j < my - 1 && k < mz - 1)) continue;
+      d0 = g.at(0, j, k); s0 = g.at(mx - 2, j, k); d1 = g.at(mx - 1, j, k); s1 = g.at(1, j, k);
+    } else {
+      if (!g.zper) continue;
+      long r = t - nyz;
+      int i = (int)(r % mx), j = (int)(r / mx);
+      if (!(i > 0 && j > 0 && i < mx - 1 && j < my - 1)) continue;
+      d0 = g.at(i, j, 0); s0 = g.at(i, j, mz - 2); d1 = g.at(i, j, mz - 1); s1 = g.at(i, j, 1);
+    }
+    u.x[d0] = u.x[s0]; u.y[d0] = u.y[s0]; u.z[d0] = u.z[s0];
+    u.x[d1] = u.x[s1]; u.y[d1] = u.y[s1]; u.z[d1] = u.z[s1];
+    if (p) { p[d0] = p[s0]; p[d1] = p[s1]; nv[d0] = nv[s0]; nv[d1] = nv[s1]; }
+  }
+}
+
+// edge averages, twelve sequential sub-sweeps (bcs.c:5440-5566); one block
+__global__ void k_edge_average(G g, V3 u, double *p) {
+  const int mx = g.mx, my = g.my, mz = g.mz;
+  for (int sb = 0; sb < 12; sb++) {
+    int len;
+    int side = (sb < 8) ? (sb / 4) : ((sb - 8) / 2);
+    if (sb < 8) len = ((sb % 4) < 2) ? my : mx;
+    else len = mz;
+    for (int q = threadIdx.x; q < len; q += blockDim.x) {
+      long d, a, b;
+      if (sb < 8) {
+        int k = side ? mz - 1 : 0, kn = side ? mz - 2 : 1;
+        switch (sb % 4) {
+          case 0: d = g.at(0, q, k); a = g.at(0, q, kn); b = g.at(1, q, k); break;
+          case 1: d = g.at(mx - 1, q, k); a = g.at(mx - 1, q, kn); b = g.at(mx - 2, q, k); break;
+          case 2: d = g.at(q, 0, k); a = g.at(q, 0, kn); b = g.at(q, 1, k); break;
+          default: d = g.at(q, my - 1, k); a = g.at(q, my - 1, kn); b = g.at(q, my - 2, k); break;
+        }
+      } else {
+        int j = side ? my - 1 : 0, jn = side ? my - 2 : 1;
+        if ((sb - 8) % 2 == 0) { d = g.at(0, j, q); a = g.at(0, jn, q); b = g.at(1, j, q); }
+        else { d = g.at(mx - 1, j, q); a = g.at(mx - 1, jn, q); b = g.at(mx - 2, j, q); }
+      }
+      u.x[d] = 0.5 * (u.x[a] + u.x[b]);
+      u.y[d] = 0.5 * (u.y[a] + u.y[b]);
+      u.z[d] = 0.5 * (u.z[a] + u.z[b]);
+      if (p) p[d] = 0.5 * (p[a] + p[b]);
+    }
+    __syncthreads();
+  }
+}
+
+// periodic full-plane copies (bcs.c:5567-5731); axis 0 then 2.  x_only:
+// the reference copies only .x on the upper x plane in FormBCS.
+__global__ void k_periodic_full(G g, int axis, int x_only, V3 u, double *p, double *nv) {
+  const int mx = g.mx, my = g.my, mz = g.mz;
+  const long n = axis == 0 ? (long)my * mz : (long)mx * my;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    long d0, s0, d1, s1;
+    if (axis == 0) {
+      int j = (int)(t % my), k = (int)(t / my);
+      d0 = g.at(0, j, k); s0 = g.at(mx - 2, j, k); d1 = g.at(mx - 1, j, k); s1 = g.at(1, j, k);
+    } else {
+      int i = (int)(t % mx), j = (int)(t / mx);
+      d0 = g.at(i, j, 0); s0 = g.at(i, j, mz - 2); d1 = g.at(i, j, mz - 1); s1 = g.at(i, j, 1);
+    }
+    u.x[d0] = u.x[s0]; u.y[d0] = u.y[s0]; u.z[d0] = u.z[s0];
+    u.x[d1] = u.x[s1];
+    if (!(axis == 0 && x_only)) { u.y[d1] = u.y[s1]; u.z[d1] = u.z[s1]; }
+    if (p) { p[d0] = p[s0]; p[d1] = p[s1]; nv[d0] = nv[s0]; nv[d1] = nv[s1]; }
+  }
+}
+
+// zero wall-normal flux of bctype-1 faces after an RK stage (the build's
+// deliberate deviation: the reference never resets it)
+__global__ void k_wall_flux_zero(G g, int bx, int bz, V3 u) {
+  const int mx = g.mx, my = g.my, mz = g.mz;
+  const long nyz = (long)my * mz, nxz = (long)mx * mz, nxy = (long)mx * my;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nyz + nxz + nxy; t += (long)gridDim.x * blockDim.x) {
+    if (t < nyz) {
+      if (!bx) continue;
+      int j = (int)(t % my), k = (int)(t / my);
+      u.x[g.at(0, j, k)] = 0.; u.x[g.at(mx - 2, j, k)] = 0.;
+    } else if (t < nyz + nxz) {
+      long r = t - nyz;
+      int i = (int)(r % mx), k = (int)(r / mx);
+      u.y[g.at(i, 0, k)] = 0.; u.y[g.at(i, my - 2, k)] = 0.;
+    } else {
+      if (!bz) continue;
+      long r = t - nyz - nxz;
+      int i = (int)(r % mx), j = (int)(r / mx);
+      u.z[g.at(i, j, 0)] = 0.; u.z[g.at(i, j, mz - 2)] = 0.;
+    }
+  }
+}
+
+// ------------------------------------------------------------------
+// Convection: QUICK face flux (rhs.c:1636-1897)
+// ------------------------------------------------------------------
+__device__ inline void quick(double um, double up, long a2, long a1, long a0, long am, C3 u, double *f) {
+  const double coef = 0.125;
+  f[0] = um * (coef * (-u.x[a2] - 2. * u.x[a1] + 3. * u.x[a0]) + u.x[a1]) + up * (coef * (-u.x[am] - 2. * u.x[a0] + 3. * u.x[a1]) + u.x[a0]);
+  f[1] = um * (coef * (-u.y[a2] - 2. * u.y[a1] + 3. * u.y[a0]) + u.y[a1]) + up * (coef * (-u.y[am] - 2. * u.y[a0] + 3. * u.y[a1]) + u.y[a0]);
+  f[2] = um * (coef * (-u.z[a2] - 2. * u.z[a1] + 3. * u.z[a0]) + u.z[a1]) + up * (coef * (-u.z[am] - 2. * u.z[a0] + 3. * u.z[a1]) + u.z[a0]);
+}
+
+// convective flux through face (i,j,k) of axis d
+__device__ inline void conv_flux(const G &g, int d, int i, int j, int k, C3 uc, C3 u, const double *nv, double *f) {
+  const double innerblank = 7.;
+  if (d == 0) {
+    const long t = g.at(i, j, k);
+    double ucon = uc.x[t] * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
+    long A2 = g.atq(i + 2, j, k), A1 = g.atq(i + 1, j, k), A0 = t, AM = g.atq(i - 1, j, k);
+    double np1 = nv[g.atq(i + 1, j, k)];
+    double nm1 = (i > 0 || g.xper) ? nv[g.atq(i - 1, j, k)] : 0.0;
+    if (i > 0 && i < g.mx - 2 && (np1 < 0.1 || np1 > innerblank) && (nm1 < 0.1 || nm1 > innerblank)) quick(um, up, A2, A1, A0, AM, u, f);
+    else if (i == 0 || nm1 > 0.1) {
+      if (g.xper && i == 0 && (nm1 < 0.1 && np1 < 0.1)) quick(um, up, A2, A1, A0, AM, u, f);
+      else quick(um, up, A2, A1, A0, A0, u, f);
+    } else if (i == g.mx - 2 || np1 > 0.1) {
+      if (g.xper && i == g.mx - 2 && (nm1 < 0.1 && np1 < 0.1)) quick(um, up, A2, A1, A0, AM, u, f);
+      else quick(um, up, A1, A1, A0, AM, u, f);
+    } else { f[0] = f[1] = f[2] = 0.0; }
+  } else if (d == 1) {
+    const long t = g.at(i, j, k);
+    double ucon = uc.y[t] * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
+    double np1 = nv[t + g.sj];
+    double nm1 = j > 0 ? nv[t - g.sj] : 0.0;
+    long A2 = j + 2 < g.my ? t + 2 * g.sj : t, A1 = t + g.sj, A0 = t, AM = j > 0 ? t - g.sj : t;
+    if (j > 0 && j < g.my - 2 && (np1 < 0.1 || np1 > innerblank) && (nm1 < 0.1 || nm1 > innerblank)) quick(um, up, A2, A1, A0, AM, u, f);
+    else if (j == 0 || nm1 > 0.1) quick(um, up, A2, A1, A0, A0, u, f);
+    else if (j == g.my - 2 || np1 > 0.1) quick(um, up, A1, A1, A0, AM, u, f);
+    else { f[0] = f[1] = f[2] = 0.0; }
+  } else {
+    const long t = g.at(i, j, k);
+    double ucon = uc.z[t] * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
+    long A2 = g.atq(i, j, k + 2), A1 = g.atq(i, j, k + 1), A0 = t, AM = g.atq(i, j, k - 1);
+    double np1 = nv[g.atq(i, j, k + 1)];
+    double nm1 = (k > 0 || g.zper) ? nv[g.atq(i, j, k - 1)] : 0.0;
+    if (k > 0 && k < g.mz - 2 && (np1 < 0.1 || np1 > innerblank) && (nm1 < 0.1 || nm1 > innerblank)) quick(um, up, A2, A1, A0, AM, u, f);
+    else if (k < g.mz - 2 && (k == 0 || nm1 > 0.1)) {
+      if (g.zper && k == 0 && (nm1 < 0.1 && np1 < 0.1)) quick(um, up, A2, A1, A0, AM, u, f);
+      else quick(um, up, A2, A1, A0, A0, u, f);
+    } else if (k > 0 && (k == g.mz - 2 || np1 > 0.1)) {
+      if (g.zper && k == g.mz - 2 && (nm1 < 0.1 && np1 < 0.1)) quick(um, up, A2, A1, A0, AM, u, f);
+      else quick(um, up, A1, A1, A0, AM, u, f);
+    } else { f[0] = f[1] = f[2] = 0.0; }
+  }
+}
+
+// ------------------------------------------------------------------
+// Viscous face flux (rhs.c:2062-2486)
+// ------------------------------------------------------------------
+struct FM { C3 c, e, z; const double *aj; };
+
+__device__ inline void visc_flux(const G &g, int d, int i, int j, int k, C3 u, const double *nv, FM m, double nu, double *f) {
+  const double solid = 0.5, innerblank = 7.;
+  const long t = g.at(i, j, k);
+  const long so[3] = {1, g.sj, g.sk};
+  const long up = t + so[d];
+  double du[3][3];
+  for (int tt = 0; tt < 3; tt++) {
+    if (tt == d) {
+      du[0][tt] = u.x[up] - u.x[t]; du[1][tt] = u.y[up] - u.y[t]; du[2][tt] = u.z[up] - u.z[t];
+      continue;
+    }
+    const long s = so[tt];
+    double n1 = nv[t + s], n2 = nv[up + s], n3 = nv[t - s], n4 = nv[up - s];
+    bool hi = (n1 > solid && n1 < innerblank) || (n2 > solid && n2 < innerblank);
+    bool lo = (n3 > solid && n3 < innerblank) || (n4 > solid && n4 < innerblank);
+    long A, B, C, D; double fac;
+    if (hi) { A = up; B = t; C = up - s; D = t - s; fac = 0.5; }
+    else if (lo) { A = up + s; B = t + s; C = up; D = t; fac = 0.5; }
+    else { A = up + s; B = t + s; C = up - s; D = t - s; fac = 0.25; }
+    du[0][tt] = (u.x[A] + u.x[B] - u.x[C] - u.x[D]) * fac;
+    du[1][tt] = (u.y[A] + u.y[B] - u.y[C] - u.y[D]) * fac;
+    du[2][tt] = (u.z[A] + u.z[B] - u.z[C] - u.z[D]) * fac;
+  }
+  const double csi0 = m.c.x[t], csi1 = m.c.y[t], csi2 = m.c.z[t];
+  const double eta0 = m.e.x[t], eta1 = m.e.y[t], eta2 = m.e.z[t];
+  const double zet0 = m.z.x[t], zet1 = m.z.y[t], zet2 = m.z.z[t];
+  double n0, n1, n2, g11, g21, g31;
+  if (d == 0) {
+    n0 = csi0; n1 = csi1; n2 = csi2;
+    g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2; g21 = eta0 * csi0 + eta1 * csi1 + eta2 * csi2;
+    g31 = zet0 * csi0 + zet1 * csi1 + zet2 * csi2;
+  } else if (d == 1) {
+    n0 = eta0; n1 = eta1; n2 = eta2;
+    g11 = csi0 * eta0 + csi1 * eta1 + csi2 * eta2; g21 = eta0 * eta0 + eta1 * eta1 + eta2 * eta2;
+    g31 = zet0 * eta0 + zet1 * eta1 + zet2 * eta2;
+  } else {
+    n0 = zet0; n1 = zet1; n2 = zet2;
+    g11 = csi0 * zet0 + csi1 * zet1 + csi2 * zet2; g21 = eta0 * zet0 + eta1 * zet1 + eta2 * zet2;
+    g31 = zet0 * zet0 + zet1 * zet1 + zet2 * zet2;
+  }
+  const double dudc = du[0][0], dude = du[0][1], dudz = du[0][2];
+  const double dvdc = du[1][0], dvde = du[1][1], dvdz = du[1][2];
+  const double dwdc = du[2][0], dwde = du[2][1], dwdz = du[2][2];
+  const double r11 = dudc * csi0 + dude * eta0 + dudz * zet0;
+  const double r21 = dvdc * csi0 + dvde * eta0 + dvdz * zet0;
+  const double r31 = dwdc * csi0 + dwde * eta0 + dwdz * zet0;
+  const double r12 = dudc * csi1 + dude * eta1 + dudz * zet1;
+  const double r22 = dvdc * csi1 + dvde * eta1 + dvdz * zet1;
+  const double r32 = dwdc * csi1 + dwde * eta1 + dwdz * zet1;
+  const double r13 = dudc * csi2 + dude * eta2 + dudz * zet2;
+  const double r23 = dvdc * csi2 + dvde * eta2 + dvdz * zet2;
+  const double r33 = dwdc * csi2 + dwde * eta2 + dwdz * zet2;
+  const double ajc = m.aj[t];
+  f[0] = 0; f[1] = 0; f[2] = 0;
+  f[0] += (g11 * dudc + g21 * dude + g31 * dudz + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nu);
+  f[1] += (g11 * dvdc + g21 * dvde + g31 * dvdz + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nu);
+  f[2] += (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nu);
+}
+
+// Rc = Visc - Conv and rct = aj*(avg basis).Rc at interior cells (rhs.c:2663-2703)
+__global__ void __launch_bounds__(BS) k_rct(G g, C3 uc, C3 u, const double *nv, FM mi, FM mj, FM mk, C3 csi, C3 eta, C3 zet,
+                                            const double *aj, double nu, V3 rct, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.dec(t, i, j, k);
+    if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
+    double a[3], b[3], c2[3], d2[3], e[3], f[3];
+    double cv[3], vs[3];
+    conv_flux(g, 0, i, j, k, uc, u, nv, a);
+    conv_flux(g, 0, i - 1, j, k, uc, u, nv, b);
+    conv_flux(g, 1, i, j, k, uc, u, nv, c2);
+    conv_flux(g, 1, i, j - 1, k, uc, u, nv, d2);
+    conv_flux(g, 2, i, j, k, uc, u, nv, e);
+    conv_flux(g, 2, i, j, k - 1, uc, u, nv, f);
+    for (int q = 0; q < 3; q++) cv[q] = a[q] - b[q] + c2[q] - d2[q] + e[q] - f[q];
+    visc_flux(g, 0, i, j, k, u, nv, mi, nu, a);
+    visc_flux(g, 0, i - 1, j, k, u, nv, mi, nu, b);
+    visc_flux(g, 1, i, j, k, u, nv, mj, nu, c2);
+    visc_flux(g, 1, i, j - 1, k, u, nv, mj, nu, d2);
+    visc_flux(g, 2, i, j, k, u, nv, mk, nu, e);
+    visc_flux(g, 2, i, j, k - 1, u, nv, mk, nu, f);
+    for (int q = 0; q < 3; q++) vs[q] = (a[q] - b[q] + c2[q] - d2[q] + e[q] - f[q]);
+    double rcx = -1.0 * cv[0] + vs[0], rcy = -1.0 * cv[1] + vs[1], rcz = -1.0 * cv[2] + vs[2];
+    const double A = aj[t];
+    const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
+    rct.x[t] = A * (0.5 * (csi.x[t] + csi.x[ti]) * rcx + 0.5 * (csi.y[t] + csi.y[ti]) * rcy + 0.5 * (csi.z[t] + csi.z[ti]) * rcz);
+    rct.y[t] = A * (0.5 * (eta.x[t] + eta.x[tj]) * rcx + 0.5 * (eta.y[t] + eta.y[tj]) * rcy + 0.5 * (eta.z[t] + eta.z[tj]) * rcz);
+    rct.z[t] = A * (0.5 * (zet.x[t] + zet.x[tk]) * rcx + 0.5 * (zet.y[t] + zet.y[tk]) * rcy + 0.5 * (zet.z[t] + zet.z[tk]) * rcz);
+  }
+}
+
+// periodic copies of rct planes (DMLocalToLocal + rhs.c:2725-2781)
+__global__ void k_rct_periodic(G g, V3 rct) {
+  const int mx = g.mx, my = g.my, mz = g.mz;
+  const long nyz = (long)my * mz, nxy = (long)mx * my;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nyz + nxy; t += (long)gridDim.x * blockDim.x) {
+    if (t < nyz) {
+      if (!g.xper) continue;
+      int j = (int)(t % my), k = (int)(t / my);
+      if (!(j > 0 && j < my - 1 && k > 0 && k < mz - 1)) continue;
+      rct.x[g.at(0, j, k)] = rct.x[g.at(mx - 2, j, k)];
+      rct.x[g.at(mx - 1, j, k)] = rct.x[g.at(1, j, k)];
+    } else {
+      if (!g.zper) continue;
+      long r = t - nyz;
+      int i = (int)(r % mx), j = (int)(r / mx);
+      if (!(i > 0 && i < mx - 1 && j > 0 && j < my - 1)) continue;
+      rct.z[g.at(i, j, 0)] = rct.z[g.at(i, j, mz - 2)];
+      rct.z[g.at(i, j, mz - 1)] = rct.z[g.at(i, j, 1)];
+    }
+  }
+}
+
+// ------------------------------------------------------------------
+// FormFunction1 face right-hand side (rhs.c:2799-3306)
+// ------------------------------------------------------------------
+__device__ inline int face_mode(const uint32_t *fl, long t, int d, int tt) {
+  int q = (d == 0) ? (tt == 1 ? 0 : 1) : (d == 1) ? (tt == 0 ? 0 : 1) : (tt == 0 ? 0 : 1);
+  return (fl[t] >> (2 + 5 * d + 2 * q)) & 3u;
+}
+
+// tangential derivative of p along tt on face (i,j,k) of axis d by mode
+__device__ inline double ptan(const G &g, const double *p, int d, int tt, long t, int mode) {
+  const long so[3] = {1, g.sj, g.sk};
+  const long o = so[d], f = so[tt];
+  if (mode == XM_LOWER) return (p[t] + p[t + o] - p[t - f] - p[t + o - f]) * 0.5;
+  if (mode == XM_UPPER) return (p[t + f] + p[t + o + f] - p[t] - p[t + o]) * 0.5;
+  return (p[t + f] + p[t + o + f] - p[t - f] - p[t + o - f]) * 0.25;   // CENTRAL
+}
+
+struct FFArgs {
+  G g;
+  const double *p, *nv;
+  const uint32_t *fl;
+  C3 rct;
+  FM mi, mj, mk;
+};
+
+__device__ inline double face_rhs_value(const FFArgs &A, int d, long t, double dpdc, double dpde, double dpdz) {
+  const FM &m = d == 0 ? A.mi : d == 1 ? A.mj : A.mk;
+  const C3 nrm = d == 0 ? m.c : d == 1 ? m.e : m.z;
+  const double *rc = d == 0 ? A.rct.x : d == 1 ? A.rct.y : A.rct.z;
+  const long so[3] = {1, A.g.sj, A.g.sk};
+  double r = 0.5 * (rc[t] + rc[t + so[d]]);
+  r -= (dpdc * (m.c.x[t] * nrm.x[t] + m.c.y[t] * nrm.y[t] + m.c.z[t] * nrm.z[t]) +
+        dpde * (m.e.x[t] * nrm.x[t] + m.e.y[t] * nrm.y[t] + m.e.z[t] * nrm.z[t]) +
+        dpdz * (m.z.x[t] * nrm.x[t] + m.z.y[t] * nrm.y[t] + m.z.z[t] * nrm.z[t])) * m.aj[t];
+  return r;
+}
+
+// the z-face (dpde, dpdz) state a cell leaves behind in loop order, used by
+// the next cell's DROP carry
+__device__ inline void zface_state(const FFArgs &A, long t, double *dpde, double *dpdz) {
+  const G &g = A.g;
+  int me = face_mode(A.fl, t, 2, 1);
+  if (me == XM_DROP) {   // z-face dpde DROP keeps the y-face value p[j+1]-p[j]
+    *dpde = A.p[t + g.sj] - A.p[t];
+  } else {
+    *dpde = ptan(g, A.p, 2, 1, t, me);
+  }
+  *dpdz = A.p[t + g.sk] - A.p[t];
+}
+
+__device__ inline long prev_cell(const G &g, int i, int j, int k) {
+  if (i > 1) return g.at(i - 1, j, k);
+  if (j > 1) return g.at(g.mx - 2, j - 1, k);
+  if (k > 1) return g.at(g.mx - 2, g.my - 2, k - 1);
+  return -1;
+}
+
+__global__ void __launch_bounds__(BS) k_ff1_faces(FFArgs A, V3 rhs, long n) {
+  const G &g = A.g;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.dec(t, i, j, k);
+    if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
+    const double *p = A.p;
+    // carried values from the previous cell in loop order
+    double pe = 0.0, pz = 0.0;
+    int mxe = face_mode(A.fl, t, 0, 1), mxz = face_mode(A.fl, t, 0, 2);
+    if (mxe == XM_DROP || mxz == XM_DROP) {
+      long pc = prev_cell(g, i, j, k);
+      if (pc >= 0) zface_state(A, pc, &pe, &pz);
+    }
+    // x face: dpdc set, dpde rule, dpdz rule
+    double xdc = p[t + 1] - p[t];
+    double xde = mxe == XM_DROP ? pe : ptan(g, p, 0, 1, t, mxe);
+    double xdz = mxz == XM_DROP ? pz : ptan(g, p, 0, 2, t, mxz);
+    double rx = face_rhs_value(A, 0, t, xdc, xde, xdz);
+    // y face: dpdc rule, dpde set, dpdz rule
+    int myc = face_mode(A.fl, t, 1, 0), myz = face_mode(A.fl, t, 1, 2);
+    double ydc = myc == XM_DROP ? xdc : ptan(g, p, 1, 0, t, myc);
+    double yde = p[t + g.sj] - p[t];
+    double ydz = myz == XM_DROP ? xdz : ptan(g, p, 1, 2, t, myz);
+    double ry = face_rhs_value(A, 1, t, ydc, yde, ydz);
+    // z face: dpdc rule, dpde rule, dpdz set
+    int mzc = face_mode(A.fl, t, 2, 0), mze = face_mode(A.fl, t, 2, 1);
+    double zdc = mzc == XM_DROP ? ydc : ptan(g, p, 2, 0, t, mzc);
+    double zde = mze == XM_DROP ? yde : ptan(g, p, 2, 1, t, mze);
+    double zdz = p[t + g.sk] - p[t];
+    double rz = face_rhs_value(A, 2, t, zdc, zde, zdz);
+    // masks (rhs.c:3280-3306)
+    const double *nv = A.nv;
+    if (nv[t] > 0.1) { rx = 0; ry = 0; rz = 0; }
+    if (nv[t + 1] > 0.1) rx = 0;
+    if (nv[t + g.sj] > 0.1) ry = 0;
+    if (nv[t + g.sk] > 0.1) rz = 0;
+    rhs.x[t] = rx; rhs.y[t] = ry; rhs.z[t] = rz;
+  }
+}
+
+// the loop state after the main FormFunction1 sweep: the z-face values of
+// the last interior cell, per variable (dpdc, dpde, dpdz)
+__device__ inline void main_end_state(const FFArgs &A, double *dc, double *de, double *dz) {
+  const G &g = A.g;
+  long t = g.at(g.mx - 2, g.my - 2, g.mz - 2);
+  const double *p = A.p;
+  double pe = 0.0, pz = 0.0;
+  int mxe = face_mode(A.fl, t, 0, 1), mxz = face_mode(A.fl, t, 0, 2);
+  if (mxe == XM_DROP || mxz == XM_DROP) {
+    long pc = prev_cell(g, g.mx - 2, g.my - 2, g.mz - 2);
+    if (pc >= 0) zface_state(A, pc, &pe, &pz);
+  }
+  double xdc = p[t + 1] - p[t];
+  double xdz = mxz == XM_DROP ? pz : ptan(g, p, 0, 2, t, mxz);
+  int myc = face_mode(A.fl, t, 1, 0), myz = face_mode(A.fl, t, 1, 2);
+  double ydc = myc == XM_DROP ? xdc : ptan(g, p, 1, 0, t, myc);
+  double yde = p[t + g.sj] - p[t];
+  (void)myz;
+  int mzc = face_mode(A.fl, t, 2, 0), mze = face_mode(A.fl, t, 2, 1);
+  *dc = mzc == XM_DROP ? ydc : ptan(g, p, 2, 0, t, mzc);
+  *de = mze == XM_DROP ? yde : ptan(g, p, 2, 1, t, mze);
+  *dz = p[t + g.sk] - p[t];
+  (void)pe;
+}
+
+// periodic boundary faces (rhs.c:3032-3278): x faces at i=0 (k outer, j
+// inner), then z faces at k=0 (j outer, i inner).  A DROP keeps the value of
+// the previous face in that order, found by walking back.
+__global__ void k_ff1_periodic(FFArgs A, V3 rhs) {
+  const G &g = A.g;
+  const int ny = g.my - 2, nxk = g.mx - 2, nzk = g.mz - 2;
+  const long nxf = A.g.xper ? (long)ny * nzk : 0, nzf = A.g.zper ? (long)ny * nxk : 0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nxf + nzf; t += (long)gridDim.x * blockDim.x) {
+    const double *p = A.p;
+    if (t < nxf) {
+      int j = 1 + (int)(t % ny), k = 1 + (int)(t / ny);
+      long f = g.at(0, j, k);
+      double dc = p[f + 1] - p[f];
+      double v[2];
+      for (int q = 0; q < 2; q++) {        // q=0: dpde (tt=1), q=1: dpdz (tt=2)
+        int tt = q == 0 ? 1 : 2;
+        int mode = face_mode(A.fl, f, 0, tt);
+        if (mode != XM_DROP) { v[q] = ptan(g, p, 0, tt, f, mode); continue; }
+        // walk back along the periodic x-face sequence
+        long s = t - 1;
+        bool found = false;
+        while (s >= 0) {
+          int jj = 1 + (int)(s % ny), kk = 1 + (int)(s / ny);
+          long ff = g.at(0, jj, kk);
+          int mm = face_mode(A.fl, ff, 0, tt);
+          if (mm != XM_DROP) { v[q] = ptan(g, p, 0, tt, ff, mm); found = true; break; }
+          s--;
+        }
+        if (!found) {
+          double a, b, c2;
+          main_end_state(A, &a, &b, &c2);
+          v[q] = q == 0 ? b : c2;
+        }
+      }
+      rhs.x[f] = face_rhs_value(A, 0, f, dc, v[0], v[1]);
+    } else {
+      long r = t - nxf;
+      int i = 1 + (int)(r % nxk), j = 1 + (int)(r / nxk);
+      long f = g.at(i, j, 0);
+      double dz = p[f + g.sk] - p[f];
+      double v[2];
+      for (int q = 0; q < 2; q++) {        // q=0: dpdc (tt=0), q=1: dpde (tt=1)
+        int tt = q;
+        int mode = face_mode(A.fl, f, 2, tt);
+        if (mode != XM_DROP) { v[q] = ptan(g, p, 2, tt, f, mode); continue; }
+        long s = r - 1;
+        bool found = false;
+        while (s >= 0) {
+          int ii = 1 + (int)(s % nxk), jj = 1 + (int)(s / nxk);
+          long ff = g.at(ii, jj, 0);
+          int mm = face_mode(A.fl, ff, 2, tt);
+          if (mm != XM_DROP) { v[q] = ptan(g, p, 2, tt, ff, mm); found = true; break; }
+          s--;
+        }
+        if (!found) {
+          if (A.g.xper) {   // last periodic x face: dpdc = p[1]-p[0], dpde its rule value
+            long ff = g.at(0, g.my - 2, g.mz - 2);
+            if (q == 0) v[q] = p[ff + 1] - p[ff];
+            else {
+              int mm = face_mode(A.fl, ff, 0, 1);
+              double a, b, c2;
+              main_end_state(A, &a, &b, &c2);
+              v[q] = mm != XM_DROP ? ptan(g, p, 0, 1, ff, mm) : b;
+            }
+          } else {
+            double a, b, c2;
+            main_end_state(A, &a, &b, &c2);
+            v[q] = q == 0 ? a : b;
+          }
+        }
+      }
+      rhs.z[f] = face_rhs_value(A, 2, f, v[0], v[1], dz);
+    }
+  }
+}
+
+__global__ void k_rk_update(double a, C3 uo, C3 r, V3 u, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    u.x[t] = a * r.x[t] + uo.x[t];
+    u.y[t] = a * r.y[t] + uo.y[t];
+    u.z[t] = a * r.z[t] + uo.z[t];
+  }
+}
+
+}  // namespace
+
+// ====================================================================
+// host side
+// ====================================================================
+static G gview(cv_ctx *c) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  G g;
+  g.mx = F.d.mx; g.my = F.d.my; g.mz = F.d.mz; g.xper = F.d.xper; g.zper = F.d.zper;
+  g.sj = g.mx; g.sk = (long)g.mx * g.my;
+  return g;
+}
+static V3 v3(double *const a[3]) { return V3{a[0], a[1], a[2]}; }
+static C3 c3(double *const a[3]) { return C3{a[0], a[1], a[2]}; }
+
+int cv_contra2cart_dev(cv_ctx *c) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  CvMetrics &M = c->m;
+  hipLaunchKernelGGL(k_contra2cart, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, gview(c), c3(c->ucont), c3(M.csi),
+                     c3(M.eta), c3(M.zet), (const double *)F.nvert, v3(c->ucat), F.d.n);
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+// formbcs=1: the three passes of FormBCS (Contra2Cart once: it is
+// idempotent across the passes); formbcs=0: GhostNodeVelocity
+int cv_ghost_update(cv_ctx *c, int formbcs) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  G g = gview(c);
+  const int bxper = F.d.xper, bzper = F.d.zper;
+  const long planes = (long)g.my * g.mz + (long)g.mx * g.mz + (long)g.mx * g.my;
+  const int passes = formbcs ? 3 : 1;
+  if (formbcs) {
+    int rc = cv_contra2cart_dev(c);
+    if (rc) return rc;
+  }
+  double *p = formbcs ? c->p : nullptr, *nv = formbcs ? F.nvert : nullptr;
+  for (int ps = 0; ps < passes; ps++) {
+    hipLaunchKernelGGL(k_wall_ghost, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, bxper, bzper, formbcs ? 0 : 1, v3(c->ucat));
+    if (bxper || bzper)
+      hipLaunchKernelGGL(k_periodic_planes, dim3(nblk((long)g.my * g.mz + (long)g.mx * g.my)), dim3(BS), 0, c->stream, g,
+                         v3(c->ucat), p, nv);
+    hipLaunchKernelGGL(k_edge_average, dim3(1), dim3(BS), 0, c->stream, g, v3(c->ucat), p);
+    if (bxper)
+      hipLaunchKernelGGL(k_periodic_full, dim3(nblk((long)g.my * g.mz)), dim3(BS), 0, c->stream, g, 0, formbcs ? 1 : 0,
+                         v3(c->ucat), p, nv);
+    if (bzper)
+      hipLaunchKernelGGL(k_periodic_full, dim3(nblk((long)g.mx * g.my)), dim3(BS), 0, c->stream, g, 2, 0, v3(c->ucat), p, nv);
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+static FFArgs ffargs(cv_ctx *c) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  CvMetrics &M = c->m;
+  FFArgs A;
+  A.g = gview(c);
+  A.p = c->p; A.nv = F.nvert; A.fl = F.flags;
+  A.rct = c3(c->rct);
+  A.mi = FM{c3(M.icsi), c3(M.ieta), c3(M.izet), M.iaj};
+  A.mj = FM{c3(M.jcsi), c3(M.jeta), c3(M.jzet), M.jaj};
+  A.mk = FM{c3(M.kcsi), c3(M.keta), c3(M.kzet), M.kaj};
+  return A;
+}
+
+// FormFunction1: Contra2Cart is skipped (FormBCS already left Ucat of the
+// same Ucont in place; the reference recomputes the identical values)
+int cv_ff1_dev(cv_ctx *c) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  CvMetrics &M = c->m;
+  FFArgs A = ffargs(c);
+  const double nu = 1. / c->ren;
+  hipLaunchKernelGGL(k_rct, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, c3(c->ucont), c3(c->ucat),
+                     (const double *)F.nvert, A.mi, A.mj, A.mk, c3(M.csi), c3(M.eta), c3(M.zet),
+                     (const double *)M.aj, nu, v3(c->rct), F.d.n);
+  if (F.d.xper || F.d.zper)
+    hipLaunchKernelGGL(k_rct_periodic, dim3(nblk((long)A.g.my * A.g.mz + (long)A.g.mx * A.g.my)), dim3(BS), 0, c->stream,
+                       A.g, v3(c->rct));
+  hipLaunchKernelGGL(k_ff1_faces, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A, v3(c->rhs), F.d.n);
+  if (F.d.xper || F.d.zper) {
+    long nf = (F.d.xper ? (long)(A.g.my - 2) * (A.g.mz - 2) : 0) + (F.d.zper ? (long)(A.g.my - 2) * (A.g.mx - 2) : 0);
+    hipLaunchKernelGGL(k_ff1_periodic, dim3(nblk(nf)), dim3(BS), 0, c->stream, A, v3(c->rhs));
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+int cv_rk_dev(cv_ctx *c) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  G g = gview(c);
+  const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
+  int rc = cv_ghost_update(c, 1);
+  if (rc) return rc;
+  for (int q = 0; q < 3; q++)
+    hipLaunchKernelGGL(k_fill, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, c->rhs[q], 0.0, F.d.n);
+  const long planes = (long)g.my * g.mz + (long)g.mx * g.mz + (long)g.mx * g.my;
+  for (int s = 0; s < 4; s++) {
+    if ((rc = cv_ff1_dev(c))) return rc;
+    double a = alfa[s] * c->dt * c->st;
+    hipLaunchKernelGGL(k_rk_update, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, a, c3(c->ucont_o), c3(c->rhs),
+                       v3(c->ucont), F.d.n);
+    hipLaunchKernelGGL(k_wall_flux_zero, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, c->bctype[0] == 1,
+                       c->bctype[4] == 1, v3(c->ucont));
+    if ((rc = cv_ghost_update(c, 1))) return rc;
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+// ====================================================================
+// C ABI
+// ====================================================================
+static int ready(cv_ctx *c) {
+  if (!c) return CV_ERR_ARG;
+  if (!c->geometry_set || !c->nvert_set) { cv_set_error("cv: geometry and nvert must be set first"); return CV_ERR_STATE; }
+  CV_CHECK(hipSetDevice(c->device));
+  return CV_OK;
+}
+
+extern "C" int cv_contra2cart(cv_ctx *c) {
+  int rc = ready(c);
+  if (rc) return rc;
+  rc = cv_contra2cart_dev(c);
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+extern "C" int cv_form_bcs(cv_ctx *c) {
+  int rc = ready(c);
+  if (rc) return rc;
+  rc = cv_ghost_update(c, 1);
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+extern "C" int cv_form_function1(cv_ctx *c, double *rhs_out) {
+  int rc = ready(c);
+  if (rc) return rc;
+  if ((rc = cv_contra2cart_dev(c))) return rc;
+  if ((rc = cv_ff1_dev(c))) return rc;
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  if (rhs_out) return cv_get_field(c, CV_F_RHS, -1, rhs_out);
+  return CV_OK;
+}
+
+extern "C" int cv_rungekutta(cv_ctx *c) {
+  int rc = ready(c);
+  if (rc) return rc;
+  rc = cv_rk_dev(c);
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+extern "C" int cv_flow_step(cv_ctx *c, cv_step_stats *st) {
+  int rc = ready(c);
+  if (rc) return rc;
+  CvLevel &F = c->lev[c->nlev - 1];
+  cv_step_stats s{};
+  hipEvent_t e[4];
+  for (int q = 0; q < 4; q++) CV_CHECK(hipEventCreate(&e[q]));
+  for (int q = 0; q < 3; q++)      // Ucont_o <- Ucont (main.c:1127 / 1065)
+    hipLaunchKernelGGL(k_copy, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, c->ucont_o[q], (const double *)c->ucont[q], F.d.n);
+  CV_CHECK(hipEventRecord(e[0], c->stream));
+  if ((rc = cv_rk_dev(c))) return rc;
+  CV_CHECK(hipEventRecord(e[1], c->stream));
+  rc = cv_poisson_solve_dev(c, &s);
+  if (rc && rc != CV_ERR_NOT_CONVERGED) return rc;
+  int conv_rc = rc;
+  CV_CHECK(hipEventRecord(e[2], c->stream));
+  if ((rc = cv_update_pressure_dev(c))) return rc;
+  if ((rc = cv_projection_dev(c))) return rc;
+  CV_CHECK(hipEventRecord(e[3], c->stream));
+  if ((rc = cv_divergence_dev(c, &s.maxdiv, &s.maxdiv_idx))) return rc;
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  float ms[3];
+  for (int q = 0; q < 3; q++) CV_CHECK(hipEventElapsedTime(&ms[q], e[q], e[q + 1]));
+  s.t_mom = ms[0] * 1e-3; s.t_p = ms[1] * 1e-3; s.t_pr = ms[2] * 1e-3;
+  for (int q = 0; q < 4; q++) (void)hipEventDestroy(e[q]);
+  c->last = s;
+  if (st) *st = s;
+  if (!std::isfinite(s.maxdiv)) return CV_ERR_NONFINITE;
+  return conv_rc;
+}

@@ -1,0 +1,1171 @@
+// cv_poisson.hip -- pressure Poisson path on gfx950.
+//
+//  * per-cell operator word: face enables and the cross-derivative modes of
+//    PoissonLHSNew (poisson.c:600-1507), built once per nvert change
+//  * matrix-free 19-point apply in flux form, FP64 and FP32
+//  * geometric multigrid with the reference's transfers (MyRestriction
+//    poisson.c:129-226, MyInterpolation poisson.c:228-370) and nvert
+//    hierarchy (MyNvertRestriction poisson.c:1662-1773)
+//  * FGMRES (right preconditioned, classical Gram-Schmidt with DGKS
+//    re-orthogonalisation), FP64 outer loop, FP64 or FP32 V-cycle
+//  * PoissonRHS, VolumeFlux_rev, UpdatePressure, Projection, Divergence
+//
+// All kernels are memory-bound stencils over i-fastest planar arrays; no
+// MFMA.  Reductions are two-pass fixed-order trees (deterministic).
+#include "cv_internal.h"
+#include "cv_reduce.h"
+#include <cmath>
+#include <chrono>
+
+#define BS 256
+
+__global__ void cv_k_final_sum(const double *part, int nb, double *out, int nq) {
+  __shared__ double sh[BS / 64];
+  const int q = blockIdx.x;
+  if (q >= nq) return;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(long)q * nb + b];
+  s = cv_block_sum(s, sh);
+  if (threadIdx.x == 0) out[q] = s;
+}
+
+namespace {
+
+struct Idx {
+  int mx, my, mz;
+  __device__ void dec(long t, int &i, int &j, int &k) const {
+    i = (int)(t % mx);
+    long r = t / mx;
+    j = (int)(r % my);
+    k = (int)(r / my);
+  }
+  __device__ long at(int i, int j, int k) const { return (long)i + (long)mx * ((long)j + (long)my * k); }
+};
+
+// ------------------------------------------------------------------
+// cross-derivative mode (poisson.c:800-837 and siblings)
+// ------------------------------------------------------------------
+__device__ inline int xmode(double hi, double lo, int at_hi, int at_lo, int per) {
+  if ((at_hi && !per) || hi > 0.1) {
+    if (lo < 0.1 && (!at_lo || per)) return XM_LOWER;
+    return XM_DROP;
+  } else if ((at_lo && !per) || lo > 0.1) {
+    if (hi < 0.1) return XM_UPPER;
+    return XM_DROP;
+  }
+  return XM_CENTRAL;
+}
+
+__global__ void k_flags(Idx I, int xper, int zper, const double *__restrict__ nv, uint32_t *__restrict__ fl, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    I.dec(t, i, j, k);
+    int m[3] = {I.mx, I.my, I.mz}, c[3] = {i, j, k}, per[3] = {xper, 0, zper};
+    uint32_t w = 0;
+    bool interior = i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1;
+    if (interior && !(nv[t] > 0.1)) w |= 1u;
+    for (int d = 0; d < 3; d++) {
+      // face (i,j,k) of axis d: lower cell index in [0, m_d-2], others interior
+      bool valid = c[d] <= m[d] - 2;
+      for (int a = 0; a < 3; a++)
+        if (a != d && (c[a] < 1 || c[a] > m[a] - 2)) valid = false;
+      if (!valid) continue;
+      int e[3] = {0, 0, 0}; e[d] = 1;
+      long up = I.at(i + e[0], j + e[1], k + e[2]);
+      bool wall = !per[d] && (c[d] == 0 || c[d] == m[d] - 2);
+      if (nv[t] < 0.1 && nv[up] < 0.1 && !wall) w |= 1u << (1 + 5 * d);
+      int q = 0;
+      for (int tt = 0; tt < 3; tt++) {
+        if (tt == d) continue;
+        int f[3] = {0, 0, 0}; f[tt] = 1;
+        double hi = nv[I.at(i + f[0], j + f[1], k + f[2])] + nv[I.at(i + e[0] + f[0], j + e[1] + f[1], k + e[2] + f[2])];
+        double lo = nv[I.at(i - f[0], j - f[1], k - f[2])] + nv[I.at(i + e[0] - f[0], j + e[1] - f[1], k + e[2] - f[2])];
+        int mode = xmode(hi, lo, c[tt] == m[tt] - 2, c[tt] == 1, per[tt]);
+        w |= (uint32_t)mode << (2 + 5 * d + 2 * q);
+        q++;
+      }
+    }
+    fl[t] = w;
+  }
+}
+
+// ------------------------------------------------------------------
+// operator view and flux-form apply
+// ------------------------------------------------------------------
+template <class T>
+struct OpV {
+  const T *g[9];
+  const T *aj;
+  const T *dinv;
+  const uint32_t *fl;
+  int mx, my, mz, xper, zper;
+};
+
+template <class T>
+__device__ inline long wrapped(const OpV<T> &A, int i, int j, int k) {
+  i = cv_wrap(i, A.mx, A.xper);
+  k = cv_wrap(k, A.mz, A.zper);
+  return (long)i + (long)A.mx * ((long)j + (long)A.my * k);
+}
+
+// flux through face (i,j,k) of axis d (lower cell (i,j,k)); the face word w
+template <class T>
+__device__ inline T face_flux(const OpV<T> &A, const T *__restrict__ x, int d, int i, int j, int k, uint32_t w) {
+  int e[3] = {0, 0, 0}; e[d] = 1;
+  const long lo = wrapped(A, i, j, k);
+  const long up = wrapped(A, i + e[0], j + e[1], k + e[2]);
+  const long fi = (long)i + (long)A.mx * ((long)j + (long)A.my * k);
+  const T xl = x[lo], xu = x[up];
+  T F = A.g[4 * d][fi] * (xu - xl);
+  int q = 0;
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    if (t == d) continue;
+    int mode = (w >> (2 + 5 * d + 2 * q)) & 3u;
+    q++;
+    if (mode == XM_DROP) continue;
+    int f[3] = {0, 0, 0}; f[t] = 1;
+    const T gdt = A.g[3 * d + t][fi];
+    if (mode == XM_CENTRAL) {
+      T s = x[wrapped(A, i + f[0], j + f[1], k + f[2])] + x[wrapped(A, i + e[0] + f[0], j + e[1] + f[1], k + e[2] + f[2])] -
+            x[wrapped(A, i - f[0], j - f[1], k - f[2])] - x[wrapped(A, i + e[0] - f[0], j + e[1] - f[1], k + e[2] - f[2])];
+      F += gdt * (T)0.25 * s;
+    } else if (mode == XM_LOWER) {
+      T s = xl + xu - x[wrapped(A, i - f[0], j - f[1], k - f[2])] - x[wrapped(A, i + e[0] - f[0], j + e[1] - f[1], k + e[2] - f[2])];
+      F += gdt * (T)0.5 * s;
+    } else {
+      T s = x[wrapped(A, i + f[0], j + f[1], k + f[2])] + x[wrapped(A, i + e[0] + f[0], j + e[1] + f[1], k + e[2] + f[2])] - xl - xu;
+      F += gdt * (T)0.5 * s;
+    }
+  }
+  return F;
+}
+
+template <class T>
+__device__ inline T apply_row(const OpV<T> &A, const T *__restrict__ x, int i, int j, int k, long t) {
+  const uint32_t wc = A.fl[t];
+  if (!(wc & 1u)) return x[t];
+  T acc = 0;
+  const uint32_t wi = A.fl[t - 1], wj = A.fl[t - A.mx], wk = A.fl[t - (long)A.mx * A.my];
+  if (wc & (1u << 1)) acc += face_flux(A, x, 0, i, j, k, wc);
+  if (wi & (1u << 1)) acc -= face_flux(A, x, 0, i - 1, j, k, wi);
+  if (wc & (1u << 6)) acc += face_flux(A, x, 1, i, j, k, wc);
+  if (wj & (1u << 6)) acc -= face_flux(A, x, 1, i, j - 1, k, wj);
+  if (wc & (1u << 11)) acc += face_flux(A, x, 2, i, j, k, wc);
+  if (wk & (1u << 11)) acc -= face_flux(A, x, 2, i, j, k - 1, wk);
+  return -A.aj[t] * acc;
+}
+
+template <class T>
+__global__ void __launch_bounds__(BS) k_apply(OpV<T> A, const T *__restrict__ x, T *__restrict__ y, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i = (int)(t % A.mx), j = (int)((t / A.mx) % A.my), k = (int)(t / ((long)A.mx * A.my));
+    y[t] = apply_row(A, x, i, j, k, t);
+  }
+}
+
+// r = b - A x
+template <class T>
+__global__ void __launch_bounds__(BS) k_resid(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i = (int)(t % A.mx), j = (int)((t / A.mx) % A.my), k = (int)(t / ((long)A.mx * A.my));
+    r[t] = b[t] - apply_row(A, x, i, j, k, t);
+  }
+}
+
+// damped Jacobi sweep: xo = xi + w*D^-1*(b - A xi); zero_guess: xi == 0
+template <class T>
+__global__ void __launch_bounds__(BS) k_jacobi(OpV<T> A, const T *__restrict__ xi, const T *__restrict__ b, T *__restrict__ xo,
+                                               T omega, int zero_guess, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    if (zero_guess) { xo[t] = omega * A.dinv[t] * b[t]; continue; }
+    int i = (int)(t % A.mx), j = (int)((t / A.mx) % A.my), k = (int)(t / ((long)A.mx * A.my));
+    T r = b[t] - apply_row(A, xi, i, j, k, t);
+    xo[t] = xi[t] + omega * A.dinv[t] * r;
+  }
+}
+
+// diagonal of the operator row (coefficient of x_c)
+__global__ void k_diag(OpV<double> A, double *dinv, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const uint32_t wc = A.fl[t];
+    if (!(wc & 1u)) { dinv[t] = 1.0; continue; }
+    const long st[3] = {1, A.mx, (long)A.mx * A.my};
+    double dsum = 0;
+    for (int d = 0; d < 3; d++) {
+      // upper face: flux = g_dd (x_u - x_c) + cross; coefficient of x_c
+      uint32_t w = wc;
+      long fi = t;
+      if (w & (1u << (1 + 5 * d))) {
+        double c0 = -A.g[4 * d][fi];
+        int q = 0;
+        for (int tt = 0; tt < 3; tt++) {
+          if (tt == d) continue;
+          int mode = (w >> (2 + 5 * d + 2 * q)) & 3u; q++;
+          if (mode == XM_LOWER) c0 += 0.5 * A.g[3 * d + tt][fi];
+          else if (mode == XM_UPPER) c0 -= 0.5 * A.g[3 * d + tt][fi];
+        }
+        dsum += c0;
+      }
+      // lower face: -flux, x_c is the upper cell of that face
+      w = A.fl[t - st[d]];
+      fi = t - st[d];
+      if (w & (1u << (1 + 5 * d))) {
+        double c1 = A.g[4 * d][fi];   // flux coefficient of x_u is +g_dd
+        int q = 0;
+        for (int tt = 0; tt < 3; tt++) {
+          if (tt == d) continue;
+          int mode = (w >> (2 + 5 * d + 2 * q)) & 3u; q++;
+          if (mode == XM_LOWER) c1 += 0.5 * A.g[3 * d + tt][fi];
+          else if (mode == XM_UPPER) c1 -= 0.5 * A.g[3 * d + tt][fi];
+        }
+        dsum -= c1;
+      }
+    }
+    double diag = -A.aj[t] * dsum;
+    dinv[t] = diag != 0.0 ? 1.0 / diag : 1.0;
+  }
+}
+
+// ------------------------------------------------------------------
+// nvert hierarchy (MyNvertRestriction, poisson.c:1662-1773)
+// ------------------------------------------------------------------
+__global__ void k_nvrestrict1(Idx C, Idx F, int isc, int jsc, int ksc, const double *nf, double *nc, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    C.dec(t, i, j, k);
+    double v = 0.0;
+    if (i > 0 && i < C.mx - 1 && j > 0 && j < C.my - 1 && k > 0 && k < C.mz - 1) {
+      int ia = isc ? 0 : 1, ja = jsc ? 0 : 1, ka = ksc ? 0 : 1;
+      int ih = isc ? i : 2 * i, jh = jsc ? j : 2 * j, kh = ksc ? k : 2 * k;
+      double prod = nf[F.at(ih, jh, kh)] * nf[F.at(ih - ia, jh, kh)] * nf[F.at(ih, jh - ja, kh)] *
+                    nf[F.at(ih, jh, kh - ka)] * nf[F.at(ih - ia, jh - ja, kh)] * nf[F.at(ih, jh - ja, kh - ka)] *
+                    nf[F.at(ih - ia, jh, kh - ka)] * nf[F.at(ih - ia, jh - ja, kh - ka)];
+      if (prod > 0.1) v = fmax(1., v);
+    }
+    nc[t] = v;
+  }
+}
+__global__ void k_nvrestrict2(Idx C, const double *nin, double *nout, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    C.dec(t, i, j, k);
+    double v = nin[t];
+    if (i > 0 && i < C.mx - 1 && j > 0 && j < C.my - 1 && k > 0 && k < C.mz - 1 && v < 0.1) {
+      if (nin[t + 1] + nin[t - 1] > 1.1 && nin[t + C.mx] + nin[t - C.mx] > 1.1 &&
+          nin[t + (long)C.mx * C.my] + nin[t - (long)C.mx * C.my] > 1.1)
+        v = 1.;
+    }
+    nout[t] = v;
+  }
+}
+
+// ------------------------------------------------------------------
+// transfers
+// ------------------------------------------------------------------
+template <class T>
+__global__ void k_restrict(Idx C, Idx F, int isc, int jsc, int ksc, const double *__restrict__ nvf,
+                           const double *__restrict__ nvc, const T *__restrict__ xf, T *__restrict__ xc, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    C.dec(t, i, j, k);
+    T f = 0;
+    if (i > 0 && i < C.mx - 1 && j > 0 && j < C.my - 1 && k > 0 && k < C.mz - 1 && !(nvc[t] > 0.1)) {
+      int ia = isc ? 0 : 1, ja = jsc ? 0 : 1, ka = ksc ? 0 : 1;
+      int ih = isc ? i : 2 * i, jh = jsc ? j : 2 * j, kh = ksc ? k : 2 * k;
+#define RT(a, b, cc) (xf[F.at(a, b, cc)] * (T)fmax(0., 1 - nvf[F.at(a, b, cc)]))
+      f = (T)0.125 * (RT(ih, jh, kh) + RT(ih - ia, jh, kh) + RT(ih, jh - ja, kh) + RT(ih, jh, kh - ka) +
+                      RT(ih - ia, jh - ja, kh) + RT(ih, jh - ja, kh - ka) + RT(ih - ia, jh, kh - ka) +
+                      RT(ih - ia, jh - ja, kh - ka));
+#undef RT
+    }
+    xc[t] = f;
+  }
+}
+
+// xf (+)= P xc ; add=0 overwrites
+template <class T>
+__global__ void k_prolong(Idx F, Idx C, int isc, int jsc, int ksc, const double *__restrict__ nvc,
+                          const double *__restrict__ nvf, const T *__restrict__ xc, T *__restrict__ xf, int add, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    F.dec(t, i, j, k);
+    T f = 0;
+    if (i > 0 && i < F.mx - 1 && j > 0 && j < F.my - 1 && k > 0 && k < F.mz - 1) {
+      int ic, jc, kc, ia, ja, ka;
+      if (isc) { ic = i; ia = 0; } else { ic = (i + 1) / 2; ia = (i - 2 * ic) == 0 ? 1 : -1; if (i == 1 || i == F.mx - 2) ia = 0; }
+      if (jsc) { jc = j; ja = 0; } else { jc = (j + 1) / 2; ja = (j - 2 * jc) == 0 ? 1 : -1; if (j == 1 || j == F.my - 2) ja = 0; }
+      if (ksc) { kc = k; ka = 0; } else { kc = (k + 1) / 2; ka = (k - 2 * kc) == 0 ? 1 : -1; if (k == 1 || k == F.mz - 2) ka = 0; }
+      if (ka == -1 && nvc[C.at(ic, jc, kc - 1)] > 0.1) ka = 0;
+      else if (ka == 1 && nvc[C.at(ic, jc, kc + 1)] > 0.1) ka = 0;
+      if (ja == -1 && nvc[C.at(ic, jc - 1, kc)] > 0.1) ja = 0;
+      else if (ja == 1 && nvc[C.at(ic, jc + 1, kc)] > 0.1) ja = 0;
+      if (ia == -1 && nvc[C.at(ic - 1, jc, kc)] > 0.1) ia = 0;
+      else if (ia == 1 && nvc[C.at(ic + 1, jc, kc)] > 0.1) ia = 0;
+      f = (xc[C.at(ic, jc, kc)] * (T)9 + xc[C.at(ic, jc + ja, kc)] * (T)3 + xc[C.at(ic + ia, jc, kc)] * (T)3 +
+           xc[C.at(ic + ia, jc + ja, kc)]) * (T)(3. / 64.) +
+          (xc[C.at(ic, jc, kc + ka)] * (T)9 + xc[C.at(ic, jc + ja, kc + ka)] * (T)3 +
+           xc[C.at(ic + ia, jc, kc + ka)] * (T)3 + xc[C.at(ic + ia, jc + ja, kc + ka)]) / (T)64.;
+    }
+    if (nvf[t] > 0.1) f = 0;
+    if (add) xf[t] += f; else xf[t] = f;
+  }
+}
+
+// ------------------------------------------------------------------
+// nullspace (PoissonNullSpaceFunction, poisson.c:411-558)
+// ------------------------------------------------------------------
+template <class T>
+__global__ void k_fluid_sum(const T *__restrict__ x, const uint32_t *__restrict__ fl, long n, double *part) {
+  __shared__ double sh[BS / 64];
+  double s = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    if (fl[t] & 1u) s += (double)x[t];
+  s = cv_block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+template <class T>
+__global__ void k_fluid_shift(T *__restrict__ x, const uint32_t *__restrict__ fl, const double *sum, double inv_n, long n) {
+  const double mean = sum[0] * inv_n;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    x[t] = (fl[t] & 1u) ? (T)((double)x[t] - mean) : (T)0;
+}
+
+// ------------------------------------------------------------------
+// Krylov kernels
+// ------------------------------------------------------------------
+#define MAXV 33
+__global__ void __launch_bounds__(BS) k_mdot(const double *__restrict__ w, const double *__restrict__ V, long stride,
+                                             int nv, long n, double *part) {
+  // part[q*gridDim + b] = sum_t w[t]*V[q*stride+t] for q<nv; q==nv -> ||w||^2
+  __shared__ double sh[BS / 64];
+  double acc[MAXV];
+#pragma unroll
+  for (int q = 0; q < MAXV; q++) acc[q] = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const double wt = w[t];
+#pragma unroll
+    for (int q = 0; q < MAXV - 1; q++)
+      if (q < nv) acc[q] += wt * V[q * stride + t];
+    acc[MAXV - 1] += wt * wt;
+  }
+  for (int q = 0; q <= nv; q++) {
+    double v = (q < nv) ? acc[q] : acc[MAXV - 1];
+    double s = cv_block_sum(v, sh);
+    if (threadIdx.x == 0) part[(long)q * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// w -= sum_q h[q] V_q ; part[b] = partial ||w||^2
+__global__ void __launch_bounds__(BS) k_orth_update(double *__restrict__ w, const double *__restrict__ V, long stride,
+                                                    const double *__restrict__ h, int nv, long n, double *part) {
+  __shared__ double sh[BS / 64];
+  __shared__ double hs[MAXV];
+  if (threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  double s = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    double v = w[t];
+    for (int q = 0; q < nv; q++) v -= hs[q] * V[q * stride + t];
+    w[t] = v;
+    s += v * v;
+  }
+  s = cv_block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_scale_to(double *__restrict__ dst, const double *__restrict__ src, const double *inv, long n) {
+  const double a = inv[0];
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = src[t] * a;
+}
+
+// x += sum_q y[q] Z_q
+__global__ void k_xupdate(double *__restrict__ x, const double *__restrict__ Z, long stride, const double *__restrict__ y,
+                          int m, long n) {
+  __shared__ double ys[MAXV];
+  if (threadIdx.x < m) ys[threadIdx.x] = y[threadIdx.x];
+  __syncthreads();
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    double v = x[t];
+    for (int q = 0; q < m; q++) v += ys[q] * Z[q * stride + t];
+    x[t] = v;
+  }
+}
+
+__global__ void k_sumsq(const double *__restrict__ x, long n, double *part) {
+  __shared__ double sh[BS / 64];
+  double s = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) s += x[t] * x[t];
+  s = cv_block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_d2f(float *dst, const double *src, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = (float)src[t];
+}
+__global__ void k_f2d(double *dst, const float *src, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = (double)src[t];
+}
+template <class T>
+__global__ void k_zero(T *dst, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = (T)0;
+}
+
+}  // namespace
+
+// ====================================================================
+// host side
+// ====================================================================
+static inline unsigned nblk(long n) {
+  long b = (n + BS - 1) / BS;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+template <class T>
+static OpV<T> opview(cv_ctx *c, int l);
+template <>
+OpV<double> opview<double>(cv_ctx *c, int l) {
+  CvLevel &L = c->lev[l];
+  OpV<double> A;
+  for (int q = 0; q < 9; q++) A.g[q] = L.g[q];
+  A.aj = L.aj; A.dinv = L.dinv; A.fl = L.flags;
+  A.mx = L.d.mx; A.my = L.d.my; A.mz = L.d.mz; A.xper = L.d.xper; A.zper = L.d.zper;
+  return A;
+}
+template <>
+OpV<float> opview<float>(cv_ctx *c, int l) {
+  CvLevel &L = c->lev[l];
+  OpV<float> A;
+  for (int q = 0; q < 9; q++) A.g[q] = L.gf[q];
+  A.aj = L.ajf; A.dinv = L.dinvf; A.fl = L.flags;
+  A.mx = L.d.mx; A.my = L.d.my; A.mz = L.d.mz; A.xper = L.d.xper; A.zper = L.d.zper;
+  return A;
+}
+static Idx idx_of(const CvLevel &L) { return Idx{L.d.mx, L.d.my, L.d.mz}; }
+
+// sum of part[0..nb) -> dsc[slot] (device)
+static void final_sum(cv_ctx *c, const double *part, int nb, double *out, int nq) {
+  hipLaunchKernelGGL(cv_k_final_sum, dim3(nq), dim3(BS), 0, c->stream, part, nb, out, nq);
+}
+
+int cv_levels_setup_nvert(cv_ctx *c, const double *nvert_host) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  CV_CHECK(hipMemcpyAsync(F.nvert, nvert_host, F.d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  for (int l = c->nlev - 1; l > 0; l--) {
+    CvLevel &H = c->lev[l], &C = c->lev[l - 1];
+    hipLaunchKernelGGL(k_nvrestrict1, dim3(nblk(C.d.n)), dim3(BS), 0, c->stream, idx_of(C), idx_of(H), C.d.isc,
+                       C.d.jsc, C.d.ksc, (const double *)H.nvert, C.r, C.d.n);
+    hipLaunchKernelGGL(k_nvrestrict2, dim3(nblk(C.d.n)), dim3(BS), 0, c->stream, idx_of(C), (const double *)C.r,
+                       C.nvert, C.d.n);
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+__global__ static void k_count_fluid(const uint32_t *fl, long n, double *part) {
+  __shared__ double sh[BS / 64];
+  double s = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    if (fl[t] & 1u) s += 1.0;
+  s = cv_block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+int cv_poisson_prepare(cv_ctx *c) {
+  for (int l = 0; l < c->nlev; l++) {
+    CvLevel &L = c->lev[l];
+    const long n = L.d.n;
+    hipLaunchKernelGGL(k_flags, dim3(nblk(n)), dim3(BS), 0, c->stream, idx_of(L), L.d.xper, L.d.zper,
+                       (const double *)L.nvert, L.flags, n);
+    hipLaunchKernelGGL(k_diag, dim3(nblk(n)), dim3(BS), 0, c->stream, opview<double>(c, l), L.dinv, n);
+    for (int q = 0; q < 9; q++)
+      hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.gf[q], (const double *)L.g[q], n);
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.ajf, (const double *)L.aj, n);
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.dinvf, (const double *)L.dinv, n);
+    hipLaunchKernelGGL(k_count_fluid, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const uint32_t *)L.flags, n, c->part);
+    final_sum(c, c->part, CV_RED_GRID, c->dsc, 1);
+    CV_CHECK(hipMemcpyAsync(c->red_host, c->dsc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CV_CHECK(hipStreamSynchronize(c->stream));
+    c->nfluid[l] = c->red_host[0];
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+extern "C" int cv_set_nvert(cv_ctx *c, const double *nvert) {
+  if (!c || !nvert) return CV_ERR_ARG;
+  CV_CHECK(hipSetDevice(c->device));
+  int rc = cv_levels_setup_nvert(c, nvert);
+  if (rc) return rc;
+  c->nvert_set = 1;
+  if (c->geometry_set) return cv_poisson_prepare(c);
+  return CV_OK;
+}
+
+// ---------------- nullspace projection on level l ------------------
+template <class T>
+static void nullspace_dev(cv_ctx *c, int l, T *x) {
+  CvLevel &L = c->lev[l];
+  const long n = L.d.n;
+  hipLaunchKernelGGL(k_fluid_sum<T>, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const T *)x, (const uint32_t *)L.flags, n, c->part);
+  final_sum(c, c->part, CV_RED_GRID, c->dsc + 100, 1);
+  double inv = c->nfluid[l] > 0 ? 1.0 / c->nfluid[l] : 0.0;
+  hipLaunchKernelGGL(k_fluid_shift<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, x, (const uint32_t *)L.flags,
+                     (const double *)(c->dsc + 100), inv, n);
+}
+
+// ---------------- V-cycle ------------------------------------------
+template <class T>
+struct MgVecs {
+  T *x[CV_MAXLEV], *b[CV_MAXLEV], *r[CV_MAXLEV];
+};
+
+template <class T>
+static MgVecs<T> mgvecs(cv_ctx *c);
+template <>
+MgVecs<double> mgvecs<double>(cv_ctx *c) {
+  MgVecs<double> v;
+  for (int l = 0; l < c->nlev; l++) { v.x[l] = c->lev[l].x; v.b[l] = c->lev[l].b; v.r[l] = c->lev[l].r; }
+  return v;
+}
+template <>
+MgVecs<float> mgvecs<float>(cv_ctx *c) {
+  MgVecs<float> v;
+  for (int l = 0; l < c->nlev; l++) { v.x[l] = c->lev[l].xf; v.b[l] = c->lev[l].bf; v.r[l] = c->lev[l].rf; }
+  return v;
+}
+
+// nsweep Jacobi sweeps on level l; x in v.x[l] (zero_guess: x is 0), r used as ping-pong
+template <class T>
+static void smooth(cv_ctx *c, MgVecs<T> &v, int l, int nsweep, bool zero_guess) {
+  CvLevel &L = c->lev[l];
+  const long n = L.d.n;
+  OpV<T> A = opview<T>(c, l);
+  T om = (T)c->opt.omega;
+  if (zero_guess && nsweep == 0)
+    hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, v.x[l], n);
+  for (int s = 0; s < nsweep; s++) {
+    hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l],
+                       v.r[l], om, (zero_guess && s == 0) ? 1 : 0, n);
+    T *tmp = v.x[l]; v.x[l] = v.r[l]; v.r[l] = tmp;
+  }
+}
+
+template <class T>
+static void vcycle(cv_ctx *c, MgVecs<T> &v, int l) {
+  CvLevel &L = c->lev[l];
+  const long n = L.d.n;
+  if (l == 0) {
+    smooth<T>(c, v, 0, c->opt.coarse_sweeps, true);
+    nullspace_dev<T>(c, 0, v.x[0]);
+    return;
+  }
+  CvLevel &C = c->lev[l - 1];
+  smooth<T>(c, v, l, c->opt.pre_sweeps, true);
+  OpV<T> A = opview<T>(c, l);
+  hipLaunchKernelGGL(k_resid<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l], v.r[l], n);
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.d.n)), dim3(BS), 0, c->stream, idx_of(C), idx_of(L), C.d.isc, C.d.jsc,
+                     C.d.ksc, (const double *)L.nvert, (const double *)C.nvert, (const T *)v.r[l], v.b[l - 1], C.d.n);
+  vcycle<T>(c, v, l - 1);
+  hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, idx_of(L), idx_of(C), L.d.isc, L.d.jsc, L.d.ksc,
+                     (const double *)C.nvert, (const double *)L.nvert, (const T *)v.x[l - 1], v.x[l], 1, n);
+  smooth<T>(c, v, l, c->opt.post_sweeps, false);
+}
+
+// z = M^{-1} v on the finest level (FP64 or FP32 V-cycle), then the nullspace rule
+static void precondition(cv_ctx *c, const double *vin, double *zout) {
+  const int top = c->nlev - 1;
+  CvLevel &F = c->lev[top];
+  const long n = F.d.n;
+  if (c->opt.precision == CV_MIXED) {
+    MgVecs<float> v = mgvecs<float>(c);
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, v.b[top], vin, n);
+    vcycle<float>(c, v, top);
+    hipLaunchKernelGGL(k_f2d, dim3(nblk(n)), dim3(BS), 0, c->stream, zout, (const float *)v.x[top], n);
+    for (int l = 0; l < c->nlev; l++) { c->lev[l].xf = v.x[l]; c->lev[l].rf = v.r[l]; }
+  } else {
+    MgVecs<double> v = mgvecs<double>(c);
+    hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(BS), 0, c->stream, v.b[top], vin, n);
+    vcycle<double>(c, v, top);
+    hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(BS), 0, c->stream, zout, (const double *)v.x[top], n);
+    for (int l = 0; l < c->nlev; l++) { c->lev[l].x = v.x[l]; c->lev[l].r = v.r[l]; }
+  }
+  nullspace_dev<double>(c, top, zout);
+}
+
+static int ensure_krylov(cv_ctx *c) {
+  int m = c->opt.restart;
+  if (m < 1) m = 1;
+  if (m > MAXV - 2) m = MAXV - 2;
+  c->opt.restart = m;
+  if (c->krylov_m >= m) return CV_OK;
+  const long n = c->lev[c->nlev - 1].d.n;
+  void *p;
+  int rc;
+  if ((rc = cv_alloc(c, &p, (size_t)(m + 1) * n * sizeof(double)))) return rc;
+  c->V = (double *)p;
+  if ((rc = cv_alloc(c, &p, (size_t)m * n * sizeof(double)))) return rc;
+  c->Z = (double *)p;
+  c->krylov_m = m;
+  return CV_OK;
+}
+
+// FGMRES(m) for A phi = b on the finest level; phi starts at 0
+static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *rnorm_out, double *bnorm_out) {
+  const int top = c->nlev - 1;
+  CvLevel &F = c->lev[top];
+  const long n = F.d.n;
+  int rc = ensure_krylov(c);
+  if (rc) return rc;
+  const int m = c->opt.restart;
+  double *V = c->V, *Z = c->Z, *w = c->w;
+  double *hd = c->dsc;              // device: h[0..33]
+  double *hh = c->red_host;         // pinned host
+  OpV<double> A = opview<double>(c, top);
+  // ||b||
+  hipLaunchKernelGGL(k_sumsq, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, b, n, c->part);
+  final_sum(c, c->part, CV_RED_GRID, hd + 40, 1);
+  CV_CHECK(hipMemcpyAsync(hh, hd + 40, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  const double bnorm = std::sqrt(hh[0]);
+  *bnorm_out = bnorm;
+  const double tol = std::fmax(c->opt.rtol * bnorm, c->opt.atol);
+  hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(BS), 0, c->stream, x, n);
+  int its = 0;
+  double rnorm = bnorm;
+  std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  // r = b (x = 0)
+  hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(BS), 0, c->stream, w, b, n);
+  double beta = bnorm;
+  while (beta > tol && its < c->opt.max_it) {
+    hh[0] = 1.0 / beta;
+    CV_CHECK(hipMemcpyAsync(hd + 48, hh, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_scale_to, dim3(nblk(n)), dim3(BS), 0, c->stream, V, (const double *)w, (const double *)(hd + 48), n);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int jj = 0;
+    for (; jj < m && its < c->opt.max_it; jj++) {
+      double *vj = V + (size_t)jj * n, *zj = Z + (size_t)jj * n;
+      precondition(c, vj, zj);
+      hipLaunchKernelGGL(k_apply<double>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const double *)zj, w, n);
+      const int nv = jj + 1;
+      // first CGS pass
+      hipLaunchKernelGGL(k_mdot, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, (const double *)V, (long)n, nv, n, c->part);
+      final_sum(c, c->part, CV_RED_GRID, hd, nv + 1);            // hd[0..nv-1] = h, hd[nv] = ||w||^2
+      hipLaunchKernelGGL(k_orth_update, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, w, (const double *)V, (long)n,
+                         (const double *)hd, nv, n, c->part);
+      final_sum(c, c->part, CV_RED_GRID, hd + 36, 1);            // ||w_new||^2
+      CV_CHECK(hipMemcpyAsync(hh, hd, 40 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      CV_CHECK(hipStreamSynchronize(c->stream));
+      double wn2 = hh[nv], hn2 = hh[36];
+      for (int q = 0; q < nv; q++) H[q * m + jj] = hh[q];
+      if (hn2 < 0.25 * wn2) {      // DGKS: second classical pass
+        hipLaunchKernelGGL(k_mdot, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, (const double *)V, (long)n, nv, n, c->part);
+        final_sum(c, c->part, CV_RED_GRID, hd, nv + 1);
+        hipLaunchKernelGGL(k_orth_update, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, w, (const double *)V, (long)n,
+                           (const double *)hd, nv, n, c->part);
+        final_sum(c, c->part, CV_RED_GRID, hd + 36, 1);
+        CV_CHECK(hipMemcpyAsync(hh, hd, 40 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CV_CHECK(hipStreamSynchronize(c->stream));
+        for (int q = 0; q < nv; q++) H[q * m + jj] += hh[q];
+        hn2 = hh[36];
+      }
+      double hn = std::sqrt(std::fmax(hn2, 0.0));
+      H[(jj + 1) * m + jj] = hn;
+      if (!std::isfinite(hn)) return CV_ERR_NONFINITE;
+      if (hn > 0) {
+        hh[0] = 1.0 / hn;
+        CV_CHECK(hipMemcpyAsync(hd + 48, hh, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_scale_to, dim3(nblk(n)), dim3(BS), 0, c->stream, V + (size_t)(jj + 1) * n,
+                           (const double *)w, (const double *)(hd + 48), n);
+      }
+      for (int q = 0; q < jj; q++) {
+        double a = H[q * m + jj], bb = H[(q + 1) * m + jj];
+        H[q * m + jj] = cs[q] * a + sn[q] * bb;
+        H[(q + 1) * m + jj] = -sn[q] * a + cs[q] * bb;
+      }
+      double a = H[jj * m + jj], bb = H[(jj + 1) * m + jj];
+      double den = std::sqrt(a * a + bb * bb);
+      cs[jj] = den == 0 ? 1.0 : a / den;
+      sn[jj] = den == 0 ? 0.0 : bb / den;
+      H[jj * m + jj] = cs[jj] * a + sn[jj] * bb;
+      H[(jj + 1) * m + jj] = 0.0;
+      g[jj + 1] = -sn[jj] * g[jj];
+      g[jj] = cs[jj] * g[jj];
+      its++;
+      rnorm = std::fabs(g[jj + 1]);
+      if (rnorm <= tol || hn == 0) { jj++; break; }
+    }
+    for (int q = jj - 1; q >= 0; q--) {
+      double s = g[q];
+      for (int t = q + 1; t < jj; t++) s -= H[q * m + t] * y[t];
+      y[q] = s / H[q * m + q];
+    }
+    for (int q = 0; q < jj; q++) hh[q] = y[q];
+    CV_CHECK(hipMemcpyAsync(hd, hh, jj * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_xupdate, dim3(nblk(n)), dim3(BS), 0, c->stream, x, (const double *)Z, (long)n, (const double *)hd, jj, n);
+    // true residual
+    hipLaunchKernelGGL(k_resid<double>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const double *)x, b, w, n);
+    hipLaunchKernelGGL(k_sumsq, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, n, c->part);
+    final_sum(c, c->part, CV_RED_GRID, hd + 40, 1);
+    CV_CHECK(hipMemcpyAsync(hh, hd + 40, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CV_CHECK(hipStreamSynchronize(c->stream));
+    beta = std::sqrt(hh[0]);
+    rnorm = beta;
+    if (!std::isfinite(beta)) return CV_ERR_NONFINITE;
+  }
+  CV_CHECK(hipGetLastError());
+  *its_out = its;
+  *rnorm_out = rnorm;
+  return rnorm <= tol ? CV_OK : CV_ERR_NOT_CONVERGED;
+}
+
+// ====================================================================
+// PoissonRHS (poisson.c:1509-1576)
+// ====================================================================
+__global__ static void k_poisson_rhs(Idx I, const double *ux, const double *uy, const double *uz, const double *aj,
+                                     const double *nv, double *b, double dt, double st, long n, double *part) {
+  __shared__ double sh[BS / 64];
+  double s = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    I.dec(t, i, j, k);
+    double r;
+    if (i == 0 || i == I.mx - 1 || j == 0 || j == I.my - 1 || k == 0 || k == I.mz - 1) r = 0.;
+    else if (nv[t] > 0.1) r = 0;
+    else
+      r = -(ux[t] - ux[t - 1] + uy[t] - uy[t - I.mx] + uz[t] - uz[t - (long)I.mx * I.my]) / dt * aj[t] / st * 1.5;
+    b[t] = r;
+    s += r / aj[t] * dt / 1.5;
+  }
+  s = cv_block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+int cv_poisson_rhs_dev(cv_ctx *c, double *rhs_sum) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  hipLaunchKernelGGL(k_poisson_rhs, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, idx_of(F), (const double *)c->ucont[0],
+                     (const double *)c->ucont[1], (const double *)c->ucont[2], (const double *)F.aj,
+                     (const double *)F.nvert, c->bvec, c->dt, c->st, F.d.n, c->part);
+  final_sum(c, c->part, CV_RED_GRID, c->dsc + 104, 1);
+  if (rhs_sum) {
+    CV_CHECK(hipMemcpyAsync(c->red_host + 100, c->dsc + 104, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CV_CHECK(hipStreamSynchronize(c->stream));
+    *rhs_sum = c->red_host[100];
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+// ====================================================================
+// VolumeFlux_rev (poisson.c:3712-3944)
+// ====================================================================
+__device__ static inline double vnorm(const double *x, const double *y, const double *z, long t) {
+  return sqrt(x[t] * x[t] + y[t] * y[t] + z[t] * z[t]);
+}
+
+// mode 0: sum flux/area into part[2*b], mode 1: apply correction dsc[slot]
+__global__ static void k_ibm_flux(Idx I, const double *nv, double *ux, double *uy, double *uz, const double *cx,
+                                  const double *cy, const double *cz, const double *ex, const double *ey, const double *ez,
+                                  const double *zx, const double *zy, const double *zz, int mode, const double *corr_p,
+                                  long n, double *part) {
+  __shared__ double sh[BS / 64];
+  const double ibmval = 1.5;
+  const double corr = mode ? corr_p[0] : 0.0;
+  double f = 0.0, a = 0.0;
+  const long sj = I.mx, sk = (long)I.mx * I.my;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    I.dec(t, i, j, k);
+    if (!(i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1)) continue;
+    const double n0 = nv[t];
+    const long nb[3] = {t + 1, t + sj, t + sk};
+    const bool lim[3] = {i < I.mx - 2, j < I.my - 2, k < I.mz - 2};
+    double *u[3] = {ux, uy, uz};
+    const double *vx[3] = {cx, ex, zx}, *vy[3] = {cy, ey, zy}, *vz[3] = {cz, ez, zz};
+    for (int d = 0; d < 3; d++) {
+      const double nn = nv[nb[d]];
+      if (n0 < 0.1 && nn > ibmval - 0.4 && nn < ibmval && lim[d]) {
+        double ar = vnorm(vx[d], vy[d], vz[d], t);
+        if (mode) u[d][t] -= ar * corr; else { f += u[d][t]; a += ar; }
+      }
+    }
+    if (n0 > ibmval - 0.4 && n0 < ibmval) {
+      for (int d = 0; d < 3; d++) {
+        if (nv[nb[d]] < 0.1 && lim[d]) {
+          double ar = vnorm(vx[d], vy[d], vz[d], t);
+          if (mode) u[d][t] += ar * corr; else { f -= u[d][t]; a += ar; }
+        }
+      }
+    }
+  }
+  if (mode) return;
+  double sf = cv_block_sum(f, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = sf;
+  double sa = cv_block_sum(a, sh);
+  if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = sa;
+}
+
+__global__ static void k_ibm_corr(const double *fa, double *corr) {
+  corr[0] = fa[1] > 1.e-15 ? (fa[0] + 0.0) / fa[1] : 0.0;   // FluxIntpSum = 0
+}
+
+int cv_volumeflux_rev_dev(cv_ctx *c, double *fa4) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  CvMetrics &M = c->m;
+  for (int pass = 0; pass < 3; pass++) {
+    int mode = pass == 1 ? 1 : 0;
+    hipLaunchKernelGGL(k_ibm_flux, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, idx_of(F), (const double *)F.nvert,
+                       c->ucont[0], c->ucont[1], c->ucont[2], (const double *)M.csi[0], (const double *)M.csi[1],
+                       (const double *)M.csi[2], (const double *)M.eta[0], (const double *)M.eta[1],
+                       (const double *)M.eta[2], (const double *)M.zet[0], (const double *)M.zet[1],
+                       (const double *)M.zet[2], mode, (const double *)(c->dsc + 108), F.d.n, c->part);
+    if (!mode) {
+      final_sum(c, c->part, CV_RED_GRID, c->dsc + (pass == 0 ? 112 : 114), 2);
+      if (pass == 0) hipLaunchKernelGGL(k_ibm_corr, dim3(1), dim3(1), 0, c->stream, (const double *)(c->dsc + 112), c->dsc + 108);
+    }
+  }
+  if (fa4) {
+    CV_CHECK(hipMemcpyAsync(c->red_host + 104, c->dsc + 112, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CV_CHECK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 4; q++) fa4[q] = c->red_host[104 + q];
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+// ====================================================================
+// PoissonSolver_MG (poisson.c:1775-2017)
+// ====================================================================
+int cv_poisson_solve_dev(cv_ctx *c, cv_step_stats *st) {
+  if (!c->geometry_set || !c->nvert_set) { cv_set_error("geometry/nvert not set"); return CV_ERR_STATE; }
+  double fa[4];
+  int rc = cv_volumeflux_rev_dev(c, fa);
+  if (rc) return rc;
+  double rs = 0;
+  rc = cv_poisson_rhs_dev(c, &rs);
+  if (rc) return rc;
+  int its = 0;
+  double rn = 0, bn = 0;
+  rc = fgmres(c, c->bvec, c->phi, &its, &rn, &bn);
+  if (rc && rc != CV_ERR_NOT_CONVERGED) return rc;
+  nullspace_dev<double>(c, c->nlev - 1, c->phi);
+  CV_CHECK(hipGetLastError());
+  if (st) {
+    st->ksp_its = its; st->ksp_rnorm = rn; st->ksp_bnorm = bn; st->rhs_sum = rs;
+    st->ibm_flux0 = fa[0]; st->ibm_area0 = fa[1]; st->ibm_flux1 = fa[2]; st->ibm_area1 = fa[3];
+    st->converged = rc == CV_OK;
+  }
+  return rc;
+}
+
+// ====================================================================
+// UpdatePressure (poisson.c:2908-3248)
+// ====================================================================
+__global__ static void k_p_add(Idx I, double *p, const double *phi, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    I.dec(t, i, j, k);
+    if (i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1) p[t] += phi[t];
+  }
+}
+// x planes: i=0 <- mx-2, i=mx-1 <- 1 over (j,k) in [j0,j1)x[k0,k1)
+__global__ static void k_pplane_x(Idx I, double *a, double *b, int j0, int j1, int k0, int k1) {
+  long nj = j1 - j0, nk = k1 - k0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nj * nk; t += (long)gridDim.x * blockDim.x) {
+    int j = j0 + (int)(t % nj), k = k0 + (int)(t / nj);
+    long p0 = I.at(0, j, k), pm = I.at(I.mx - 2, j, k), q0 = I.at(I.mx - 1, j, k), q1 = I.at(1, j, k);
+    a[p0] = a[pm]; a[q0] = a[q1];
+    if (b) { b[p0] = b[pm]; b[q0] = b[q1]; }
+  }
+}
+__global__ static void k_pplane_z(Idx I, double *a, double *b, int i0, int i1, int j0, int j1) {
+  long ni = i1 - i0, nj = j1 - j0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < ni * nj; t += (long)gridDim.x * blockDim.x) {
+    int i = i0 + (int)(t % ni), j = j0 + (int)(t / ni);
+    long p0 = I.at(i, j, 0), pm = I.at(i, j, I.mz - 2), q0 = I.at(i, j, I.mz - 1), q1 = I.at(i, j, 1);
+    a[p0] = a[pm]; a[q0] = a[q1];
+    if (b) { b[p0] = b[pm]; b[q0] = b[q1]; }
+  }
+}
+
+int cv_update_pressure_dev(cv_ctx *c) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  Idx I = idx_of(F);
+  hipLaunchKernelGGL(k_p_add, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, I, c->p, (const double *)c->phi, F.d.n);
+  const int mx = F.d.mx, my = F.d.my, mz = F.d.mz;
+  if (F.d.xper)
+    hipLaunchKernelGGL(k_pplane_x, dim3(nblk((long)my * mz)), dim3(BS), 0, c->stream, I, c->p, c->phi, 1, my - 1, 1, mz - 1);
+  if (F.d.zper)
+    hipLaunchKernelGGL(k_pplane_z, dim3(nblk((long)mx * my)), dim3(BS), 0, c->stream, I, c->p, c->phi, 1, mx - 1, 1, my - 1);
+  if (F.d.xper)
+    hipLaunchKernelGGL(k_pplane_x, dim3(nblk((long)my * mz)), dim3(BS), 0, c->stream, I, c->p, c->phi, 0, my, 0, mz);
+  if (F.d.zper)
+    hipLaunchKernelGGL(k_pplane_z, dim3(nblk((long)mx * my)), dim3(BS), 0, c->stream, I, c->p, c->phi, 0, mx, 0, my);
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+// ====================================================================
+// Projection (poisson.c:2281-2906), face corrections only; the caller
+// follows with Contra2Cart and GhostNodeVelocity
+// ====================================================================
+struct FaceMet {
+  const double *c[3], *e[3], *z[3], *aj;   // face metric vectors of one family
+};
+
+__device__ static inline double tang_p(const double *p, Idx I, int d, int t, int i, int j, int k, int mode) {
+  int o[3] = {0, 0, 0}; o[d] = 1;
+  int f[3] = {0, 0, 0}; f[t] = 1;
+#define PP(a, b, cc) p[I.at(i + (a), j + (b), k + (cc))]
+  if (mode == XM_LOWER)
+    return (PP(0, 0, 0) + PP(o[0], o[1], o[2]) - PP(-f[0], -f[1], -f[2]) - PP(o[0] - f[0], o[1] - f[1], o[2] - f[2])) * 0.5;
+  if (mode == XM_UPPER)
+    return (PP(f[0], f[1], f[2]) + PP(o[0] + f[0], o[1] + f[1], o[2] + f[2]) - PP(0, 0, 0) - PP(o[0], o[1], o[2])) * 0.5;
+  if (mode == XM_CENTRAL)
+    return (PP(f[0], f[1], f[2]) + PP(o[0] + f[0], o[1] + f[1], o[2] + f[2]) - PP(-f[0], -f[1], -f[2]) -
+            PP(o[0] - f[0], o[1] - f[1], o[2] - f[2])) * 0.25;
+#undef PP
+  return 0.0;
+}
+
+__device__ static inline void project_face(Idx I, int d, int i, int j, int k, const double *phi, const double *nv,
+                                           const uint32_t *fl, FaceMet M, double *u, double dt, double st) {
+  long t = I.at(i, j, k);
+  int o[3] = {0, 0, 0}; o[d] = 1;
+  long up = I.at(i + o[0], j + o[1], k + o[2]);
+  if (nv[t] + nv[up] != 0.0) return;
+  uint32_t w = fl[t];
+  double dp[3];
+  int q = 0;
+  for (int tt = 0; tt < 3; tt++) {
+    if (tt == d) { dp[tt] = phi[up] - phi[t]; continue; }
+    int mode = (w >> (2 + 5 * d + 2 * q)) & 3u; q++;
+    dp[tt] = tang_p(phi, I, d, tt, i, j, k, mode);
+  }
+  const double *nx = d == 0 ? M.c[0] : d == 1 ? M.e[0] : M.z[0];
+  const double *ny = d == 0 ? M.c[1] : d == 1 ? M.e[1] : M.z[1];
+  const double *nz = d == 0 ? M.c[2] : d == 1 ? M.e[2] : M.z[2];
+  double a = M.aj[t];
+  double g0 = (M.c[0][t] * nx[t] + M.c[1][t] * ny[t] + M.c[2][t] * nz[t]) * a;
+  double g1 = (M.e[0][t] * nx[t] + M.e[1][t] * ny[t] + M.e[2][t] * nz[t]) * a;
+  double g2 = (M.z[0][t] * nx[t] + M.z[1][t] * ny[t] + M.z[2][t] * nz[t]) * a;
+  u[t] -= (dp[0] * g0 + dp[1] * g1 + dp[2] * g2) * dt * st / 1.5;
+}
+
+__global__ static void k_projection(Idx I, int xper, int zper, const double *phi, const double *nv, const uint32_t *fl,
+                                    FaceMet Mi, FaceMet Mj, FaceMet Mk, double *ux, double *uy, double *uz, double dt,
+                                    double st, long n) {
+  const int i_end = xper ? I.mx - 1 : I.mx - 2, j_end = I.my - 2, k_end = zper ? I.mz - 1 : I.mz - 2;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    I.dec(t, i, j, k);
+    bool jk = j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1, ij = i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1;
+    if (jk && i > 0 && i < I.mx - 1) {
+      if (i < i_end) project_face(I, 0, i, j, k, phi, nv, fl, Mi, ux, dt, st);
+      if (j < j_end) project_face(I, 1, i, j, k, phi, nv, fl, Mj, uy, dt, st);
+      if (k < k_end) project_face(I, 2, i, j, k, phi, nv, fl, Mk, uz, dt, st);
+    }
+    if (xper && i == 0 && jk) project_face(I, 0, 0, j, k, phi, nv, fl, Mi, ux, dt, st);
+    if (zper && k == 0 && ij) project_face(I, 2, i, j, 0, phi, nv, fl, Mk, uz, dt, st);
+  }
+}
+
+int cv_projection_dev(cv_ctx *c) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  CvMetrics &M = c->m;
+  FaceMet Mi{{M.icsi[0], M.icsi[1], M.icsi[2]}, {M.ieta[0], M.ieta[1], M.ieta[2]}, {M.izet[0], M.izet[1], M.izet[2]}, M.iaj};
+  FaceMet Mj{{M.jcsi[0], M.jcsi[1], M.jcsi[2]}, {M.jeta[0], M.jeta[1], M.jeta[2]}, {M.jzet[0], M.jzet[1], M.jzet[2]}, M.jaj};
+  FaceMet Mk{{M.kcsi[0], M.kcsi[1], M.kcsi[2]}, {M.keta[0], M.keta[1], M.keta[2]}, {M.kzet[0], M.kzet[1], M.kzet[2]}, M.kaj};
+  hipLaunchKernelGGL(k_projection, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, idx_of(F), F.d.xper, F.d.zper,
+                     (const double *)c->phi, (const double *)F.nvert, (const uint32_t *)F.flags, Mi, Mj, Mk,
+                     c->ucont[0], c->ucont[1], c->ucont[2], c->dt, c->st, F.d.n);
+  CV_CHECK(hipGetLastError());
+  int rc = cv_contra2cart_dev(c);
+  if (rc) return rc;
+  return cv_ghost_update(c, 0);
+}
+
+// ====================================================================
+// Divergence (main.c:469-593): max and first index of the maximum
+// ====================================================================
+__global__ static void k_divmax(Idx I, const double *ux, const double *uy, const double *uz, const double *aj,
+                                const double *nv, long n, double *pv, double *pi) {
+  __shared__ double sv[BS];
+  __shared__ long si[BS];
+  double best = -1.0;
+  long bi = 0;
+  const long sj = I.mx, sk = (long)I.mx * I.my;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    I.dec(t, i, j, k);
+    double dv = 0.0;
+    if (i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1) {
+      dv = fabs((ux[t] - ux[t - 1] + uy[t] - uy[t - sj] + uz[t] - uz[t - sk]) * aj[t]);
+      if (nv[t] + nv[t + sk] + nv[t - sk] + nv[t + sj] + nv[t - sj] + nv[t + 1] + nv[t - 1] > 0.1) dv = 0.;
+    }
+    if (dv > best) { best = dv; bi = t; }
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double v2 = sv[threadIdx.x + s];
+      long i2 = si[threadIdx.x + s];
+      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { pv[blockIdx.x] = sv[0]; pi[blockIdx.x] = (double)si[0]; }
+}
+
+int cv_divergence_dev(cv_ctx *c, double *maxdiv, long *idx) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  const int nb = CV_RED_GRID;
+  hipLaunchKernelGGL(k_divmax, dim3(nb), dim3(BS), 0, c->stream, idx_of(F), (const double *)c->ucont[0],
+                     (const double *)c->ucont[1], (const double *)c->ucont[2], (const double *)F.aj,
+                     (const double *)F.nvert, F.d.n, c->part, c->part + nb);
+  std::vector<double> hv(2 * nb);
+  CV_CHECK(hipMemcpyAsync(hv.data(), c->part, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  double best = -1.0;
+  long bi = 0;
+  for (int b = 0; b < nb; b++) {
+    long ii = (long)hv[nb + b];
+    if (hv[b] > best || (hv[b] == best && ii < bi)) { best = hv[b]; bi = ii; }
+  }
+  if (maxdiv) *maxdiv = best;
+  if (idx) *idx = bi;
+  return CV_OK;
+}
+
+// ====================================================================
+// C ABI
+// ====================================================================
+static int need_ready(cv_ctx *c) {
+  if (!c) return CV_ERR_ARG;
+  if (!c->geometry_set || !c->nvert_set) { cv_set_error("cv: geometry and nvert must be set first"); return CV_ERR_STATE; }
+  CV_CHECK(hipSetDevice(c->device));
+  return CV_OK;
+}
+
+extern "C" int cv_set_solver(cv_ctx *c, const cv_solver_opts *o) {
+  if (!c || !o) return CV_ERR_ARG;
+  if (o->restart < 1 || o->restart > MAXV - 2 || o->max_it < 1 || o->rtol < 0 || o->pre_sweeps < 0 ||
+      o->post_sweeps < 0 || o->coarse_sweeps < 1 || (o->precision != CV_FP64 && o->precision != CV_MIXED) ||
+      o->omega <= 0) {
+    cv_set_error("cv_set_solver: invalid options");
+    return CV_ERR_ARG;
+  }
+  c->opt = *o;
+  return CV_OK;
+}
+
+extern "C" int cv_poisson_rhs(cv_ctx *c, double *b_out) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  double rs;
+  if ((rc = cv_poisson_rhs_dev(c, &rs))) return rc;
+  c->last.rhs_sum = rs;
+  if (b_out) return cv_get_field(c, CV_F_B, -1, b_out);
+  return CV_OK;
+}
+
+extern "C" int cv_poisson_apply(cv_ctx *c, int level, const double *x, double *y) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  if (level < 0) level = c->nlev - 1;
+  if (level >= c->nlev || !x || !y) return CV_ERR_ARG;
+  CvLevel &L = c->lev[level];
+  CV_CHECK(hipMemcpyAsync(L.b, x, L.d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_apply<double>, dim3(nblk(L.d.n)), dim3(BS), 0, c->stream, opview<double>(c, level),
+                     (const double *)L.b, L.r, L.d.n);
+  CV_CHECK(hipGetLastError());
+  CV_CHECK(hipMemcpyAsync(y, L.r, L.d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return CV_OK;
+}
+
+extern "C" int cv_poisson_solve(cv_ctx *c, cv_step_stats *st) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  cv_step_stats s{};
+  rc = cv_poisson_solve_dev(c, &s);
+  c->last.ksp_its = s.ksp_its; c->last.ksp_rnorm = s.ksp_rnorm; c->last.ksp_bnorm = s.ksp_bnorm;
+  c->last.rhs_sum = s.rhs_sum; c->last.converged = s.converged;
+  if (st) *st = s;
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+extern "C" int cv_volumeflux_rev(cv_ctx *c, double *fa4) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  double tmp[4];
+  rc = cv_volumeflux_rev_dev(c, fa4 ? fa4 : tmp);
+  return rc;
+}
+
+extern "C" int cv_update_pressure(cv_ctx *c) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  rc = cv_update_pressure_dev(c);
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+extern "C" int cv_projection(cv_ctx *c) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  rc = cv_projection_dev(c);
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+extern "C" int cv_divergence(cv_ctx *c, double *maxdiv, long *idx) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  return cv_divergence_dev(c, maxdiv, idx);
+}
+
+extern "C" int cv_mg_restrict(cv_ctx *c, int lc, const double *xf, double *xc) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  if (lc < 0 || lc >= c->nlev - 1 || !xf || !xc) return CV_ERR_ARG;
+  CvLevel &C = c->lev[lc], &F = c->lev[lc + 1];
+  CV_CHECK(hipMemcpyAsync(F.r, xf, F.d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(C.d.n)), dim3(BS), 0, c->stream, idx_of(C), idx_of(F), C.d.isc,
+                     C.d.jsc, C.d.ksc, (const double *)F.nvert, (const double *)C.nvert, (const double *)F.r, C.b, C.d.n);
+  CV_CHECK(hipMemcpyAsync(xc, C.b, C.d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return CV_OK;
+}
+
+extern "C" int cv_mg_prolong(cv_ctx *c, int lf, const double *xc, double *xf) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  if (lf < 1 || lf >= c->nlev || !xf || !xc) return CV_ERR_ARG;
+  CvLevel &C = c->lev[lf - 1], &F = c->lev[lf];
+  CV_CHECK(hipMemcpyAsync(C.r, xc, C.d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_prolong<double>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, idx_of(F), idx_of(C), F.d.isc,
+                     F.d.jsc, F.d.ksc, (const double *)C.nvert, (const double *)F.nvert, (const double *)C.r, F.b, 0, F.d.n);
+  CV_CHECK(hipMemcpyAsync(xf, F.b, F.d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return CV_OK;
+}
+
+extern "C" int cv_nullspace_project(cv_ctx *c, int level, double *x) {
+  int rc = need_ready(c);
+  if (rc) return rc;
+  if (level < 0) level = c->nlev - 1;
+  if (level >= c->nlev || !x) return CV_ERR_ARG;
+  CvLevel &L = c->lev[level];
+  CV_CHECK(hipMemcpyAsync(L.b, x, L.d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  nullspace_dev<double>(c, level, L.b);
+  CV_CHECK(hipMemcpyAsync(x, L.b, L.d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return CV_OK;
+}

@@ -1,0 +1,205 @@
+"""Host-side mirror of the reference's fractional-step call sites on top of
+the gfx950 library.
+
+``UserCtx`` stands in for the reference's per-block ``UserCtx``
+(variables.h:155-304) plus ``UserMG`` (variables.h:306-319): it owns one
+device context holding the multigrid hierarchy, metrics and flow fields.
+Its methods carry the reference routine names and run them on the GPU:
+
+    FormFunction1   rhs.c:2567          PoissonSolver_MG  poisson.c:1775
+    RungeKutta      implicitsolver.c:32 UpdatePressure    poisson.c:2908
+    FormBCS         bcs.c:4436          Projection        poisson.c:2281
+    Contra2Cart     rhs.c:25            Divergence        main.c:469
+    PoissonRHS      poisson.c:1509      Flow_Solver       solvers.c:944
+
+Host arrays use the DMDA global layout: scalar (mz, my, mx), Cmpnts
+(mz, my, mx, 3).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class UserCtx:
+    def __init__(self, nodes, bctype, mglevels, ren, dt, st=1.0, semi=(0, 0, 0), device=0):
+        L = _lib.load()
+        d = _lib.GridDesc()
+        d.IM, d.JM, d.KM = (int(v) for v in nodes)
+        for q in range(6):
+            d.bctype[q] = int(bctype[q])
+        d.mglevels = int(mglevels)
+        d.isc, d.jsc, d.ksc = (int(v) for v in semi)
+        d.ren, d.dt, d.st = float(ren), float(dt), float(st)
+        h = C.c_void_p()
+        check(L.cv_ctx_create(C.byref(d), int(device), C.byref(h)), "cv_ctx_create")
+        self.h = h
+        self.nodes = tuple(int(v) for v in nodes)
+        self.mglevels = mglevels
+        self.storage = tuple(v + 1 for v in self.nodes)
+        self.opts = _lib.SolverOpts()
+        L.cv_default_solver(C.byref(self.opts))
+
+    # -- life cycle ----------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().cv_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_bytes(self):
+        return _lib.load().cv_device_bytes(self.h)
+
+    def set_solver(self, **kw):
+        for k, v in kw.items():
+            if k == "precision" and isinstance(v, str):
+                v = {"fp64": _lib.CV_FP64, "mixed": _lib.CV_MIXED}[v]
+            if k == "smoother" and isinstance(v, str):
+                v = {"jacobi": _lib.CV_SMOOTH_JACOBI, "rb": _lib.CV_SMOOTH_RB, "yline": _lib.CV_SMOOTH_YLINE}[v]
+            setattr(self.opts, k, v)
+        check(_lib.load().cv_set_solver(self.h, C.byref(self.opts)), "cv_set_solver")
+
+    # -- inputs (grid.dat, nvert, restart fields) -----------------------
+    def set_grid(self, xyz):
+        a = np.ascontiguousarray(xyz, dtype=np.float64)
+        IM, JM, KM = self.nodes
+        if a.size != IM * JM * KM * 3:
+            raise ValueError("node coordinate array has the wrong size")
+        check(_lib.load().cv_set_geometry(self.h, _dp(a)), "cv_set_geometry")
+
+    def set_nvert(self, nv):
+        a = np.ascontiguousarray(nv, dtype=np.float64)
+        if a.size != np.prod(self.storage):
+            raise ValueError("nvert array has the wrong size")
+        check(_lib.load().cv_set_nvert(self.h, _dp(a)), "cv_set_nvert")
+
+    def _shape(self, name, level):
+        if level < 0:
+            mx, my, mz = self.storage
+        else:
+            mx, my, mz = self.level_storage(level)
+        return (mz, my, mx, 3) if name in _lib.VEC_FIELDS else (mz, my, mx)
+
+    def level_storage(self, level):
+        im, jm, km = self.nodes
+        for _ in range(self.mglevels - 1 - level):
+            im, jm, km = (im + 1) // 2, (jm + 1) // 2, (km + 1) // 2
+        return im + 1, jm + 1, km + 1
+
+    def get(self, name, level=-1):
+        out = np.empty(self._shape(name, level))
+        check(_lib.load().cv_get_field(self.h, _lib.FIELD_ID[name], level, _dp(out)), f"get {name}")
+        return out
+
+    def put(self, name, arr, level=-1):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != self._shape(name, level):
+            raise ValueError(f"{name}: expected shape {self._shape(name, level)}, got {a.shape}")
+        check(_lib.load().cv_put_field(self.h, _lib.FIELD_ID[name], level, _dp(a)), f"put {name}")
+
+    # -- reference routines ----------------------------------------------
+    def Contra2Cart(self):
+        check(_lib.load().cv_contra2cart(self.h), "Contra2Cart")
+
+    def FormBCS(self):
+        check(_lib.load().cv_form_bcs(self.h), "FormBCS")
+
+    def FormFunction1(self):
+        out = np.empty(self._shape("rhs", -1))
+        check(_lib.load().cv_form_function1(self.h, _dp(out)), "FormFunction1")
+        return out
+
+    def RungeKutta(self):
+        check(_lib.load().cv_rungekutta(self.h), "RungeKutta")
+
+    def VolumeFlux_rev(self):
+        out = np.zeros(4)
+        check(_lib.load().cv_volumeflux_rev(self.h, _dp(out)), "VolumeFlux_rev")
+        return out
+
+    def PoissonRHS(self):
+        out = np.empty(self._shape("b", -1))
+        check(_lib.load().cv_poisson_rhs(self.h, _dp(out)), "PoissonRHS")
+        return out
+
+    def PoissonApply(self, x, level=-1):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty_like(a)
+        check(_lib.load().cv_poisson_apply(self.h, level, _dp(a), _dp(out)), "PoissonApply")
+        return out
+
+    def PoissonSolver_MG(self, allow_unconverged=False):
+        st = _lib.StepStats()
+        rc = _lib.load().cv_poisson_solve(self.h, C.byref(st))
+        if not (rc == _lib.CV_OK or (allow_unconverged and rc == _lib.CV_ERR_NOT_CONVERGED)):
+            check(rc, "PoissonSolver_MG")
+        return st.as_dict()
+
+    def UpdatePressure(self):
+        check(_lib.load().cv_update_pressure(self.h), "UpdatePressure")
+
+    def Projection(self):
+        check(_lib.load().cv_projection(self.h), "Projection")
+
+    def Divergence(self):
+        v = C.c_double()
+        i = C.c_long()
+        check(_lib.load().cv_divergence(self.h, C.byref(v), C.byref(i)), "Divergence")
+        return v.value, i.value
+
+    def Flow_Solver(self, allow_unconverged=False):
+        st = _lib.StepStats()
+        rc = _lib.load().cv_flow_step(self.h, C.byref(st))
+        if not (rc == _lib.CV_OK or (allow_unconverged and rc == _lib.CV_ERR_NOT_CONVERGED)):
+            check(rc, "Flow_Solver")
+        return st.as_dict()
+
+    # operator plug-in points (tests)
+    def MyRestriction(self, coarse_level, xf):
+        shp = self._shape("b", coarse_level)
+        out = np.empty(shp)
+        a = np.ascontiguousarray(xf, dtype=np.float64)
+        check(_lib.load().cv_mg_restrict(self.h, coarse_level, _dp(a), _dp(out)), "MyRestriction")
+        return out
+
+    def MyInterpolation(self, fine_level, xc):
+        shp = self._shape("b", fine_level)
+        out = np.empty(shp)
+        a = np.ascontiguousarray(xc, dtype=np.float64)
+        check(_lib.load().cv_mg_prolong(self.h, fine_level, _dp(a), _dp(out)), "MyInterpolation")
+        return out
+
+    def PoissonNullSpaceFunction(self, x, level=-1):
+        a = np.ascontiguousarray(x, dtype=np.float64).copy()
+        check(_lib.load().cv_nullspace_project(self.h, level, _dp(a)), "PoissonNullSpaceFunction")
+        return a
+
+
+def from_case(case, mglevels=3, device=0, semi=(0, 0, 0)):
+    """UserCtx loaded with a curvib_old_amd.cases.ChannelCase."""
+    from . import cases
+    xyz, nv, uc = cases.build(case)
+    u = UserCtx(case.nodes, case.bctype, mglevels, case.re, case.dt, semi=semi, device=device)
+    u.set_grid(xyz)
+    u.set_nvert(nv)
+    u.put("ucont", uc)
+    return u, (xyz, nv, uc)

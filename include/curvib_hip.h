@@ -1,0 +1,138 @@
+/*
+ * curvib_hip.h -- C ABI of the MI355X (gfx950) fractional-step core of
+ * CURVIB.  Plain pointers and sizes only; every entry point returns an int
+ * status (CV_OK = 0).
+ *
+ * Array layout at this boundary is the reference's PETSc DMDA global layout:
+ * storage extents mx = IM+1, my = JM+1, mz = KM+1 (init.c:161-165); scalar
+ * fields are mx*my*mz doubles, Cmpnts fields 3*mx*my*mz doubles interleaved
+ * (x,y,z) (variables.h:17-19); natural order i + mx*(j + my*k)
+ * (poisson.c:95).  Host-pointer entry points are synchronous: they copy in,
+ * run on the context's HIP stream and copy out.  All device state (metrics,
+ * operator coefficients, multigrid hierarchy, Krylov space, flow fields) is
+ * allocated once by cv_ctx_create/cv_set_geometry and owned by the context.
+ *
+ * Threading: calls on one context are serialised by the caller; distinct
+ * contexts may be driven from distinct host threads.
+ */
+#ifndef CURVIB_HIP_H
+#define CURVIB_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference returns PetscErrorCode and never checks
+ * convergence; poisson.c:1828-1983) */
+#define CV_OK                 0
+#define CV_ERR_ARG           -1
+#define CV_ERR_OOM           -2
+#define CV_ERR_NOT_CONVERGED -3
+#define CV_ERR_NONFINITE     -4
+#define CV_ERR_HIP           -5
+#define CV_ERR_STATE         -6
+
+/* Poisson preconditioner precision */
+#define CV_FP64   0   /* FP64 V-cycle */
+#define CV_MIXED  1   /* FP32 V-cycle inside the FP64 FGMRES */
+
+/* multigrid smoothers */
+#define CV_SMOOTH_JACOBI 0    /* damped point Jacobi                     */
+#define CV_SMOOTH_RB     1    /* red-black ordered Gauss-Seidel sweeps   */
+#define CV_SMOOTH_YLINE  2    /* y-line (wall-normal) Jacobi, Thomas solves */
+
+typedef struct cv_ctx cv_ctx;
+
+/* Grid and run parameters: grid.dat sizes, bcs.dat, the MG options of
+ * MG_Initial (init.c:296-321, 350-358) and FormInitialize (init.c:52-62). */
+typedef struct {
+  int IM, JM, KM;          /* grid nodes per axis */
+  int bctype[6];           /* i-, i+, j-, j+, k-, k+ ; supported: 1 wall, 7 periodic (i, k) */
+  int mglevels;            /* -mg_level */
+  int isc, jsc, ksc;       /* -mg_{i,j,k}_semi */
+  double ren, dt, st;      /* -ren, -dt, Strouhal number */
+} cv_grid_desc;
+
+/* Poisson solver options (control.dat~:69-87 prefix ps_) */
+typedef struct {
+  int precision;           /* CV_FP64 / CV_MIXED */
+  int restart;             /* FGMRES restart (-ps_ksp_gmres_restart, default 30) */
+  int max_it;              /* -ps_ksp_max_it */
+  double rtol, atol;       /* -ps_ksp_rtol, -ps_ksp_atol: stop when ||b-Ax|| <= max(rtol*||b||, atol) */
+  int pre_sweeps, post_sweeps, coarse_sweeps;
+  int smoother;            /* CV_SMOOTH_* */
+  double omega;            /* smoother damping */
+} cv_solver_opts;
+
+/* per-step record (the reference's diagnostics: "Summation RHS"
+ * poisson.c:1567, "IBMFlux REV" poisson.c:3806/3933, "Maxdiv" main.c:575,
+ * CPU_TIME t_mom t_p t_pr solvers.c:1130-1142) */
+typedef struct {
+  int ksp_its;
+  int converged;
+  double ksp_rnorm, ksp_bnorm;
+  double rhs_sum;
+  double maxdiv;
+  long maxdiv_idx;
+  double ibm_flux0, ibm_area0, ibm_flux1, ibm_area1;
+  double t_mom, t_p, t_pr;  /* seconds, device-event timed */
+} cv_step_stats;
+
+/* field identifiers for cv_get_field / cv_put_field */
+enum {
+  CV_F_UCONT = 0, CV_F_UCAT, CV_F_P, CV_F_PHI, CV_F_RHS, CV_F_B, CV_F_NVERT,
+  CV_F_AJ, CV_F_CSI, CV_F_ETA, CV_F_ZET,
+  CV_F_ICSI, CV_F_IETA, CV_F_IZET, CV_F_IAJ,
+  CV_F_JCSI, CV_F_JETA, CV_F_JZET, CV_F_JAJ,
+  CV_F_KCSI, CV_F_KETA, CV_F_KZET, CV_F_KAJ,
+  CV_F_UCONT_O, CV_F_NFIELDS
+};
+
+/* ---- life cycle: MG_Initial (init.c:296-1060) / MG_Finalize (init.c:1063) ---- */
+int  cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out);
+int  cv_ctx_destroy(cv_ctx *ctx);
+int  cv_set_solver(cv_ctx *ctx, const cv_solver_opts *opts);
+void cv_default_solver(cv_solver_opts *opts);
+int  cv_last_error(char *buf, int len);
+long cv_device_bytes(const cv_ctx *ctx);
+
+/* Node coordinates, IM*JM*KM*3 doubles (x,y,z per node, node (i,j,k) at
+ * i + IM*(j + JM*k)), the content of grid.dat (init.c:689-720); builds
+ * FormMetrics + Update_Metrics_PBC on every level (metrics.c:4-1229,
+ * implicitsolver.c:2174-2573) on the device in FP64. */
+int cv_set_geometry(cv_ctx *ctx, const double *xyz_nodes);
+
+/* Fine-level nvert (mx*my*mz), the output of the immersed-body search
+ * (ibm.c:1074-1079); coarse levels follow MyNvertRestriction
+ * (poisson.c:1662-1773). */
+int cv_set_nvert(cv_ctx *ctx, const double *nvert);
+
+/* field exchange (Ucont_P_Binary_Input/Output, main.c:75-467, in memory) */
+int cv_get_field(cv_ctx *ctx, int field, int level, double *out);
+int cv_put_field(cv_ctx *ctx, int field, int level, const double *in);
+
+/* ---- hot path: one entry point per reference routine ---- */
+int cv_contra2cart(cv_ctx *ctx);                         /* Contra2Cart        rhs.c:25-144      */
+int cv_form_bcs(cv_ctx *ctx);                            /* FormBCS            bcs.c:4436-5888   */
+int cv_form_function1(cv_ctx *ctx, double *rhs_out);     /* FormFunction1      rhs.c:2567-3351   */
+int cv_rungekutta(cv_ctx *ctx);                          /* RungeKutta         implicitsolver.c:32-119 */
+int cv_volumeflux_rev(cv_ctx *ctx, double *flux_area4);  /* VolumeFlux_rev     poisson.c:3712-3944 */
+int cv_poisson_rhs(cv_ctx *ctx, double *b_out);          /* PoissonRHS         poisson.c:1509-1576 */
+int cv_poisson_apply(cv_ctx *ctx, int level, const double *x, double *y); /* MatMult(A) of PoissonLHSNew poisson.c:600-1507 */
+int cv_poisson_solve(cv_ctx *ctx, cv_step_stats *st);    /* PoissonSolver_MG   poisson.c:1775-2017
+                                                            and the PoissonSolver_Hypre call site
+                                                            solvers.c:1053-1056 */
+int cv_update_pressure(cv_ctx *ctx);                     /* UpdatePressure     poisson.c:2908-3248 */
+int cv_projection(cv_ctx *ctx);                          /* Projection         poisson.c:2281-2906 */
+int cv_divergence(cv_ctx *ctx, double *maxdiv, long *idx); /* Divergence       main.c:469-593    */
+int cv_flow_step(cv_ctx *ctx, cv_step_stats *st);        /* Flow_Solver        solvers.c:944-1156 + main.c:1123-1145 */
+
+/* operator plug-in points of the solve, exported for tests (MatShell
+ * callbacks poisson.c:1863-1875, nullspace callback poisson.c:1950-1954) */
+int cv_mg_restrict(cv_ctx *ctx, int coarse_level, const double *xf, double *xc);  /* MyRestriction  poisson.c:129-226 */
+int cv_mg_prolong(cv_ctx *ctx, int fine_level, const double *xc, double *xf);     /* MyInterpolation poisson.c:265-370 */
+int cv_nullspace_project(cv_ctx *ctx, int level, double *x);                      /* PoissonNullSpaceFunction poisson.c:372-563 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

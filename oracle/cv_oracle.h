@@ -1,0 +1,114 @@
+/*
+ * cv_oracle.h -- CPU FP64 restatement of the CURVIB fractional-step core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the
+ * CPU baseline; it is never linked into, loaded by, or called from the HIP
+ * product path (curvib_old_amd/csrc).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it.
+ *
+ * Every routine restates one reference routine (file:line cited at each
+ * definition in the .c files).  Arrays follow the reference's PETSc DMDA
+ * conventions: storage mx=IM+1 (likewise my, mz), index [k][j][i] with i
+ * fastest, Cmpnts fields interleaved (x,y,z).  Internally every field is a
+ * "local" array padded by G=3 ghost layers, mirroring a ghosted DMDA local
+ * vector (stencil width s=3 when any axis is periodic); ghost layers of a
+ * periodic axis are filled with PETSc's wrap (local index l -> global
+ * l mod m) by o_g2l(), the DMGlobalToLocal of a single-rank run.
+ *
+ * Parity status per function is listed in the header comment of each .c.
+ */
+#ifndef CV_ORACLE_H
+#define CV_ORACLE_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OG 3          /* ghost width of the padded local arrays */
+#define O_MAXLEV 10
+
+typedef struct { double x, y, z; } ocmp;
+
+typedef struct {
+  int m[3];           /* mx, my, mz (storage extents incl. boundary planes) */
+  int p[3];           /* padded extents m+2*OG */
+  int dof;            /* 1 or 3 */
+  double *a;          /* p[0]*p[1]*p[2]*dof doubles, zero-initialised */
+} ofield;
+
+typedef struct {
+  int IM, JM, KM;     /* grid nodes per axis */
+  int mx, my, mz;     /* storage = nodes + 1 (init.c:161-165) */
+  int isc, jsc, ksc;  /* semi-coarsening flags of this level (init.c:82-100) */
+  int bctype[6];
+  int per[3];         /* axis periodic (bctype 7) */
+  int thislevel;
+  ofield coor;        /* node coordinates, dof 3 */
+  ofield csi, eta, zet, aj;
+  ofield icsi, ieta, izet, iaj;
+  ofield jcsi, jeta, jzet, jaj;
+  ofield kcsi, keta, kzet, kaj;
+  ofield cent, centx, centy, centz, gs;
+  ofield nvert;
+  /* assembled Poisson operator (PoissonLHSNew), CSR over mx*my*mz rows */
+  int nrows;
+  int *rowptr, *col;
+  double *val;
+  int multinullspace; int *kske;
+} olevel;
+
+typedef struct {
+  int nlev;           /* level nlev-1 is the finest (reference convention) */
+  olevel lev[O_MAXLEV];
+  double ren, dt, st;
+  /* fine-level state */
+  ofield ucont, ucont_o, ucat, p, phi, nvert_o, rhs, ubcs;
+  ofield b;           /* Poisson right-hand side */
+  /* diagnostics */
+  double rhs_sum;     /* "Summation RHS" (poisson.c:1554-1567) */
+  double ibm_flux0, ibm_area0, ibm_flux1, ibm_area1;  /* VolumeFlux_rev */
+  double maxdiv; long maxdiv_idx;
+  int ksp_its; double ksp_rnorm, ksp_bnorm;
+  long drop_count;    /* DROP cross-term modes met on unmasked faces */
+  int nthreads;
+  /* solver options (reference: control.dat~:69-87, poisson.c:1828-1983) */
+  int restart, max_it; double rtol, atol;
+  int smooth_it;      /* Richardson iterations per level smoother */
+  double t_mom, t_p, t_pr;  /* CPU_TIME-style phase times (solvers.c:1130-1142) */
+} octx;
+
+/* ---- life cycle / data exchange (host arrays in DMDA global layout) ---- */
+octx *o_create(int IM, int JM, int KM, const int bctype[6], int nlev,
+               int isc, int jsc, int ksc, double ren, double dt);
+void  o_destroy(octx *c);
+void  o_set_coords(octx *c, const double *xyz_nodes); /* IM*JM*KM*3, grid.dat order */
+void  o_form_metrics_all(octx *c);                   /* FormMetrics (+PBC) every level */
+void  o_set_nvert(octx *c, const double *nvert);      /* fine level, mx*my*mz */
+void  o_nvert_restrict_all(octx *c);                  /* MyNvertRestriction chain */
+void  o_get_field(const ofield *f, double *out);      /* storage region -> dense */
+void  o_put_field(ofield *f, const double *in);       /* dense -> storage region */
+void  o_g2l(ofield *f, const int per[3]);             /* DMGlobalToLocal ghost fill */
+
+/* ---- hot-path routines ---- */
+void o_contra2cart(octx *c);
+void o_formbcs(octx *c);
+void o_formfunction1(octx *c, ofield *rhs);
+void o_rungekutta(octx *c);
+void o_volumeflux_rev(octx *c);
+void o_poisson_rhs(octx *c, ofield *b);
+void o_poisson_lhs(octx *c, int level);
+void o_nullspace(octx *c, int level, double *x_dense);
+void o_restrict(octx *c, int lc, const double *xf, double *xc);
+void o_interp(octx *c, int lf, const double *xc, double *xf);
+int  o_poisson_solve(octx *c);
+void o_update_pressure(octx *c);
+void o_projection(octx *c);
+void o_ghostnodevelocity(octx *c);
+double o_divergence(octx *c);
+void o_flow_step(octx *c);
+void o_csr_matvec(const olevel *L, const double *x, double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,651 @@
+/*
+ * cv_oracle_momentum.c -- TEST INFRASTRUCTURE (CPU oracle): momentum right-
+ * hand side and the step driver.
+ *
+ *   o_contra2cart       Contra2Cart          rhs.c:25-144
+ *   convection          Convection (QUICK)   rhs.c:1463-1945
+ *   viscous             Viscous (laminar)    rhs.c:1947-2565
+ *   o_formfunction1     FormFunction1        rhs.c:2567-3351
+ *   o_formbcs           FormBCS, bctype {1,7} subset   bcs.c:4436-5888
+ *   o_ghostnodevelocity GhostNodeVelocity    bcs.c:3645-4435 (subset)
+ *   o_rungekutta        RungeKutta           implicitsolver.c:32-119
+ *   o_flow_step         Flow_Solver          solvers.c:944-1156 (+ the level
+ *                       rotation Ucont_o <- Ucont of main.c:1123-1145)
+ *
+ * Parity pins (tests/test_oracle_momentum.py): free-stream preservation on a
+ * warped grid (RHS equals the GCL error), Poiseuille flow on a uniform grid
+ * (viscous term exact for a quadratic profile, convection zero), QUICK
+ * exactness for linear fields, Contra2Cart round trip, RK update linearity
+ * in a constant RHS.  The stale dpdc/dpde/dpdz carry of FormFunction1
+ * (rhs.c:2803-2861) is reproduced in loop order; its first read is of an
+ * uninitialised variable in the reference and is 0 here.
+ *
+ * Deliberate deviations (each also in DESIGN.md):
+ *  - no-slip walls (bctype 1): the reference never resets the wall-normal
+ *    face flux, so FormFunction1's interior loop drives the flux through the
+ *    upper wall face (j=my-2 etc.) while the lower one keeps its initial
+ *    value, which makes the Poisson right-hand side incompatible.  After each
+ *    RK stage the build sets the wall-normal flux of bctype-1 faces to the
+ *    wall value 0, as FormBCS does for the slip wall (bcs.c:4745-4853).
+ *  - GhostNodeVelocity's two periodic copies that read ".x" from k+2
+ *    (bcs.c:4099, 4159) are taken as full copies from the wrapped plane.
+ */
+#include "cv_oracle_int.h"
+#include <stdio.h>
+#include <time.h>
+
+int o_xmode(double hi_pair, double lo_pair, int at_hi, int at_lo, int per);
+
+static double now_s(void) {
+  struct timespec ts; clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* ------------------------------------------------------------------ */
+/* Contra2Cart (rhs.c:25-144)                                           */
+/* ------------------------------------------------------------------ */
+void o_contra2cart(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  const ofield *csi = &L->csi, *eta = &L->eta, *zet = &L->zet, *nv = &L->nvert;
+  const ofield *uc = &c->ucont;
+  ofield *ucat = &c->ucat;
+  const double innerblank = 7.;
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        double n = S(nv, k, j, i);
+        if (!(n < 0.1 || n > innerblank)) continue;
+        double mat[3][3], q[3];
+        mat[0][0] = 0.5 * (V(csi,k,j,i-1).x + V(csi,k,j,i).x);
+        mat[0][1] = 0.5 * (V(csi,k,j,i-1).y + V(csi,k,j,i).y);
+        mat[0][2] = 0.5 * (V(csi,k,j,i-1).z + V(csi,k,j,i).z);
+        mat[1][0] = 0.5 * (V(eta,k,j-1,i).x + V(eta,k,j,i).x);
+        mat[1][1] = 0.5 * (V(eta,k,j-1,i).y + V(eta,k,j,i).y);
+        mat[1][2] = 0.5 * (V(eta,k,j-1,i).z + V(eta,k,j,i).z);
+        mat[2][0] = 0.5 * (V(zet,k-1,j,i).x + V(zet,k,j,i).x);
+        mat[2][1] = 0.5 * (V(zet,k-1,j,i).y + V(zet,k,j,i).y);
+        mat[2][2] = 0.5 * (V(zet,k-1,j,i).z + V(zet,k,j,i).z);
+        q[0] = 0.5 * (V(uc,k,j,i-1).x + V(uc,k,j,i).x);
+        q[1] = 0.5 * (V(uc,k,j-1,i).y + V(uc,k,j,i).y);
+        q[2] = 0.5 * (V(uc,k-1,j,i).z + V(uc,k,j,i).z);
+        double det = mat[0][0] * (mat[1][1] * mat[2][2] - mat[1][2] * mat[2][1]) -
+                     mat[0][1] * (mat[1][0] * mat[2][2] - mat[1][2] * mat[2][0]) +
+                     mat[0][2] * (mat[1][0] * mat[2][1] - mat[1][1] * mat[2][0]);
+        double det0 = q[0] * (mat[1][1] * mat[2][2] - mat[1][2] * mat[2][1]) -
+                      q[1] * (mat[0][1] * mat[2][2] - mat[0][2] * mat[2][1]) +
+                      q[2] * (mat[0][1] * mat[1][2] - mat[0][2] * mat[1][1]);
+        double det1 = -q[0] * (mat[1][0] * mat[2][2] - mat[1][2] * mat[2][0]) +
+                      q[1] * (mat[0][0] * mat[2][2] - mat[0][2] * mat[2][0]) -
+                      q[2] * (mat[0][0] * mat[1][2] - mat[0][2] * mat[1][0]);
+        double det2 = q[0] * (mat[1][0] * mat[2][1] - mat[1][1] * mat[2][0]) -
+                      q[1] * (mat[0][0] * mat[2][1] - mat[0][1] * mat[2][0]) +
+                      q[2] * (mat[0][0] * mat[1][1] - mat[0][1] * mat[1][0]);
+        V(ucat,k,j,i).x = det0 / det;
+        V(ucat,k,j,i).y = det1 / det;
+        V(ucat,k,j,i).z = det2 / det;
+      }
+  o_g2l(ucat, L->per);
+}
+
+/* ------------------------------------------------------------------ */
+/* ghost planes of Ucat/P/Nvert (FormBCS and GhostNodeVelocity)         */
+/* ------------------------------------------------------------------ */
+/* wall ghost: ucat_g = 2*ubcs - ucat_inner (bcs.c:5252-5314);
+ * full=1 covers the full tangential ranges (GhostNodeVelocity, bcs.c:4015-4081),
+ * full=0 the interior ranges (FormBCS). */
+static void wall_ghosts(octx *c, int full) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  ofield *u = &c->ucat; const ofield *ub = &c->ubcs;
+  int a0 = full ? 0 : 1, ax = full ? mx : mx - 1, ay = full ? my : my - 1, az = full ? mz : mz - 1;
+#define WG(kk, jj, ii, kn, jn, in)                                  \
+  do {                                                              \
+    V(u,kk,jj,ii).x = 2 * V(ub,kk,jj,ii).x - V(u,kn,jn,in).x;        \
+    V(u,kk,jj,ii).y = 2 * V(ub,kk,jj,ii).y - V(u,kn,jn,in).y;        \
+    V(u,kk,jj,ii).z = 2 * V(ub,kk,jj,ii).z - V(u,kn,jn,in).z;        \
+  } while (0)
+  if (L->bctype[0] != 7) {
+    for (int k = a0; k < az; k++) for (int j = a0; j < ay; j++) WG(k, j, 0, k, j, 1);
+    for (int k = a0; k < az; k++) for (int j = a0; j < ay; j++) WG(k, j, mx - 1, k, j, mx - 2);
+  }
+  if (L->bctype[2] != 7) {
+    for (int k = a0; k < az; k++) for (int i = a0; i < ax; i++) WG(k, 0, i, k, 1, i);
+    for (int k = a0; k < az; k++) for (int i = a0; i < ax; i++) WG(k, my - 1, i, k, my - 2, i);
+  }
+  if (L->bctype[4] != 7) {
+    for (int j = a0; j < ay; j++) for (int i = a0; i < ax; i++) WG(0, j, i, 1, j, i);
+    for (int j = a0; j < ay; j++) for (int i = a0; i < ax; i++) WG(mz - 1, j, i, mz - 2, j, i);
+  }
+#undef WG
+}
+
+/* periodic plane copies restricted to the interior of the other axes
+ * (bcs.c:5325-5431) */
+static void periodic_planes(octx *c, int with_p) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz, IM = L->IM, JM = L->JM, KM = L->KM;
+  if (!(L->bctype[0] == 7 || L->bctype[4] == 7)) return;
+  ofield *u = &c->ucat, *p = &c->p, *nv = &L->nvert;
+  o_g2l(u, L->per); if (with_p) { o_g2l(p, L->per); }
+  /* the copies read the local (pre-copy) arrays */
+  ofield lu, lp, ln;
+  of_alloc(&lu, mx, my, mz, 3); of_copy(&lu, u);
+  of_alloc(&lp, mx, my, mz, 1); of_copy(&lp, p);
+  of_alloc(&ln, mx, my, mz, 1); of_copy(&ln, nv);
+  if (L->bctype[0] == 7)
+    for (int k = 0; k < mz; k++) for (int j = 0; j < my; j++)
+      if (j > 0 && k > 0 && j < JM && k < KM) {
+        V(u,k,j,0) = V(&lu,k,j,-2);
+        if (with_p) { S(p,k,j,0) = S(&lp,k,j,-2); S(nv,k,j,0) = S(&ln,k,j,-2); }
+      }
+  if (L->bctype[4] == 7)
+    for (int j = 0; j < my; j++) for (int i = 0; i < mx; i++)
+      if (i > 0 && j > 0 && i < IM && j < JM) {
+        V(u,0,j,i) = V(&lu,-2,j,i);
+        if (with_p) { S(p,0,j,i) = S(&lp,-2,j,i); S(nv,0,j,i) = S(&ln,-2,j,i); }
+      }
+  if (L->bctype[0] == 7)
+    for (int k = 0; k < mz; k++) for (int j = 0; j < my; j++)
+      if (j > 0 && k > 0 && j < JM && k < KM) {
+        V(u,k,j,mx-1) = V(&lu,k,j,mx+1);
+        if (with_p) { S(p,k,j,mx-1) = S(&lp,k,j,mx+1); S(nv,k,j,mx-1) = S(&ln,k,j,mx+1); }
+      }
+  if (L->bctype[4] == 7)
+    for (int j = 0; j < my; j++) for (int i = 0; i < mx; i++)
+      if (i > 0 && j > 0 && i < IM && j < JM) {
+        V(u,mz-1,j,i) = V(&lu,mz+1,j,i);
+        if (with_p) { S(p,mz-1,j,i) = S(&lp,mz+1,j,i); S(nv,mz-1,j,i) = S(&ln,mz+1,j,i); }
+      }
+  of_free(&lu); of_free(&lp); of_free(&ln);
+  o_g2l(u, L->per); o_g2l(p, L->per); o_g2l(nv, L->per);
+}
+
+/* edge values: average of the two adjacent ghost-plane values
+ * ("0 velocity on the corner point", bcs.c:5440-5566), in the reference's
+ * sequential order */
+static void edge_average(octx *c, int with_p) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  ofield *u = &c->ucat, *p = &c->p;
+#define EA(k0, j0, i0, ka, ja, ia, kb, jb, ib)                                   \
+  do {                                                                           \
+    V(u,k0,j0,i0).x = 0.5 * (V(u,ka,ja,ia).x + V(u,kb,jb,ib).x);                  \
+    V(u,k0,j0,i0).y = 0.5 * (V(u,ka,ja,ia).y + V(u,kb,jb,ib).y);                  \
+    V(u,k0,j0,i0).z = 0.5 * (V(u,ka,ja,ia).z + V(u,kb,jb,ib).z);                  \
+    if (with_p) S(p,k0,j0,i0) = 0.5 * (S(p,ka,ja,ia) + S(p,kb,jb,ib));           \
+  } while (0)
+  for (int side = 0; side < 2; side++) {
+    int k = side ? mz - 1 : 0, kn = side ? mz - 2 : 1;
+    for (int j = 0; j < my; j++) EA(k, j, 0, kn, j, 0, k, j, 1);
+    for (int j = 0; j < my; j++) EA(k, j, mx - 1, kn, j, mx - 1, k, j, mx - 2);
+    for (int i = 0; i < mx; i++) EA(k, 0, i, kn, 0, i, k, 1, i);
+    for (int i = 0; i < mx; i++) EA(k, my - 1, i, kn, my - 1, i, k, my - 2, i);
+  }
+  for (int side = 0; side < 2; side++) {
+    int j = side ? my - 1 : 0, jn = side ? my - 2 : 1;
+    for (int k = 0; k < mz; k++) EA(k, j, 0, k, jn, 0, k, j, 1);
+    for (int k = 0; k < mz; k++) EA(k, j, mx - 1, k, jn, mx - 1, k, j, mx - 2);
+  }
+#undef EA
+  o_g2l(u, L->per); o_g2l(p, L->per);
+}
+
+/* periodic full-plane copies (bcs.c:5567-5731); the reference copies only
+ * .x on the i=mx-1 plane here (bcs.c:5613) */
+static void periodic_corners(octx *c, int with_p, int x_only_upper) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  if (!(L->bctype[0] == 7 || L->bctype[4] == 7)) return;
+  ofield *u = &c->ucat, *p = &c->p, *nv = &L->nvert;
+  if (L->bctype[0] == 7) {
+    for (int k = 0; k < mz; k++) for (int j = 0; j < my; j++) {
+      V(u,k,j,0) = V(u,k,j,-2);
+      if (with_p) { S(p,k,j,0) = S(p,k,j,-2); S(nv,k,j,0) = S(nv,k,j,-2); }
+    }
+    for (int k = 0; k < mz; k++) for (int j = 0; j < my; j++) {
+      if (x_only_upper) V(u,k,j,mx-1).x = V(u,k,j,mx+1).x;
+      else V(u,k,j,mx-1) = V(u,k,j,mx+1);
+      if (with_p) { S(p,k,j,mx-1) = S(p,k,j,mx+1); S(nv,k,j,mx-1) = S(nv,k,j,mx+1); }
+    }
+    o_g2l(u, L->per); o_g2l(p, L->per); o_g2l(nv, L->per);
+  }
+  if (L->bctype[4] == 7) {
+    for (int j = 0; j < my; j++) for (int i = 0; i < mx; i++) {
+      V(u,0,j,i) = V(u,-2,j,i);
+      if (with_p) { S(p,0,j,i) = S(p,-2,j,i); S(nv,0,j,i) = S(nv,-2,j,i); }
+    }
+    for (int j = 0; j < my; j++) for (int i = 0; i < mx; i++) {
+      V(u,mz-1,j,i) = V(u,mz+1,j,i);
+      if (with_p) { S(p,mz-1,j,i) = S(p,mz+1,j,i); S(nv,mz-1,j,i) = S(nv,mz+1,j,i); }
+    }
+    o_g2l(u, L->per); o_g2l(p, L->per); o_g2l(nv, L->per);
+  }
+}
+
+/* zero the wall-normal flux of bctype-1 faces (deviation, see header) */
+static void wall_flux_zero(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  ofield *u = &c->ucont;
+  if (L->bctype[0] == 1)
+    for (int k = 0; k < mz; k++) for (int j = 0; j < my; j++) { V(u,k,j,0).x = 0.; V(u,k,j,mx-2).x = 0.; }
+  if (L->bctype[2] == 1)
+    for (int k = 0; k < mz; k++) for (int i = 0; i < mx; i++) { V(u,k,0,i).y = 0.; V(u,k,my-2,i).y = 0.; }
+  if (L->bctype[4] == 1)
+    for (int j = 0; j < my; j++) for (int i = 0; i < mx; i++) { V(u,0,j,i).z = 0.; V(u,mz-2,j,i).z = 0.; }
+  o_g2l(u, L->per);
+}
+
+/* FormBCS for bctype {1,7} (bcs.c:4436-5888): three passes of
+ * Contra2Cart -> wall ghosts -> periodic planes -> edge averages ->
+ * periodic full planes. */
+void o_formbcs(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  for (int ttemp = 0; ttemp < 3; ttemp++) {
+    o_g2l(&c->ucont, L->per);
+    o_contra2cart(c);
+    wall_ghosts(c, 0);
+    periodic_planes(c, 1);
+    edge_average(c, 1);
+    periodic_corners(c, 1, 1);
+  }
+  o_g2l(&c->p, L->per); o_g2l(&c->ucat, L->per); o_g2l(&c->ucont, L->per);
+}
+
+/* GhostNodeVelocity (bcs.c:3645-4435) for bctype {1,7} */
+void o_ghostnodevelocity(octx *c) {
+  wall_ghosts(c, 1);
+  periodic_planes(c, 0);
+  edge_average(c, 0);
+  periodic_corners(c, 0, 0);
+}
+
+/* ------------------------------------------------------------------ */
+/* Convection (rhs.c:1463-1945), QUICK (central/LES branches omitted)   */
+/* ------------------------------------------------------------------ */
+static inline void quick(double um, double up, const ocmp *a2, const ocmp *a1, const ocmp *a0,
+                         const ocmp *am, ocmp *fp) {
+  /* fp = um*(coef*(-u[+2] - 2u[+1] + 3u[0]) + u[+1]) + up*(coef*(-u[-1] - 2u[0] + 3u[+1]) + u[0])
+   * with a2=u[+2], a1=u[+1], a0=u[0], am=u[-1] */
+  const double coef = 0.125;
+  fp->x = um * (coef * (-a2->x - 2. * a1->x + 3. * a0->x) + a1->x) +
+          up * (coef * (-am->x - 2. * a0->x + 3. * a1->x) + a0->x);
+  fp->y = um * (coef * (-a2->y - 2. * a1->y + 3. * a0->y) + a1->y) +
+          up * (coef * (-am->y - 2. * a0->y + 3. * a1->y) + a0->y);
+  fp->z = um * (coef * (-a2->z - 2. * a1->z + 3. * a0->z) + a1->z) +
+          up * (coef * (-am->z - 2. * a0->z + 3. * a1->z) + a0->z);
+}
+
+static void convection(octx *c, ofield *conv) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  ofield *u = &c->ucat, *nv = &L->nvert; const ofield *uc = &c->ucont;
+  const double innerblank = 7.;
+  const int sw = (L->per[0] || L->per[1] || L->per[2]) ? 3 : 2;
+  const int gxs = L->per[0] ? -sw : 0, gxe = L->per[0] ? mx + sw : mx;
+  const int gys = L->per[1] ? -sw : 0, gye = L->per[1] ? my + sw : my;
+  const int gzs = L->per[2] ? -sw : 0, gze = L->per[2] ? mz + sw : mz;
+  /* periodic far ghosts for QUICK (rhs.c:1557-1628) */
+  if (L->bctype[0] == 7 || L->bctype[1] == 7) {
+    for (int k = gzs; k < gze; k++) for (int j = gys; j < gye; j++) {
+      V(u,k,j,-1) = V(u,k,j,-3); S(nv,k,j,-1) = S(nv,k,j,-3);
+    }
+    for (int k = gzs; k < gze; k++) for (int j = gys; j < gye; j++) {
+      V(u,k,j,mx) = V(u,k,j,mx+2); S(nv,k,j,mx) = S(nv,k,j,mx+2);
+    }
+  }
+  if (L->bctype[4] == 7 || L->bctype[5] == 7) {
+    for (int j = gys; j < gye; j++) for (int i = gxs; i < gxe; i++) {
+      V(u,-1,j,i) = V(u,-3,j,i); S(nv,-1,j,i) = S(nv,-3,j,i);
+    }
+    for (int j = gys; j < gye; j++) for (int i = gxs; i < gxe; i++) {
+      V(u,mz,j,i) = V(u,mz+2,j,i); S(nv,mz,j,i) = S(nv,mz+2,j,i);
+    }
+  }
+  ofield fp1, fp2, fp3;
+  of_alloc(&fp1, mx, my, mz, 3); of_alloc(&fp2, mx, my, mz, 3); of_alloc(&fp3, mx, my, mz, 3);
+#define FL(n) ((n) < 0.1 || (n) > innerblank)
+  /* i direction (rhs.c:1636-1706) */
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 0; i < mx - 1; i++) {
+        double ucon = V(uc,k,j,i).x * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
+        ocmp *f = &V(&fp1,k,j,i);
+        if (i > 0 && i < mx - 2 && FL(S(nv,k,j,i+1)) && FL(S(nv,k,j,i-1)))
+          quick(um, up, &V(u,k,j,i+2), &V(u,k,j,i+1), &V(u,k,j,i), &V(u,k,j,i-1), f);
+        else if (i == 0 || S(nv,k,j,i-1) > 0.1) {
+          if (L->bctype[0] == 7 && i == 0 && (S(nv,k,j,i-1) < 0.1 && S(nv,k,j,i+1) < 0.1))
+            quick(um, up, &V(u,k,j,i+2), &V(u,k,j,i+1), &V(u,k,j,i), &V(u,k,j,i-1), f);
+          else
+            quick(um, up, &V(u,k,j,i+2), &V(u,k,j,i+1), &V(u,k,j,i), &V(u,k,j,i), f);
+        } else if (i == mx - 2 || S(nv,k,j,i+1) > 0.1) {
+          if (L->bctype[0] == 7 && i == mx - 2 && (S(nv,k,j,i-1) < 0.1 && S(nv,k,j,i+1) < 0.1))
+            quick(um, up, &V(u,k,j,i+2), &V(u,k,j,i+1), &V(u,k,j,i), &V(u,k,j,i-1), f);
+          else
+            quick(um, up, &V(u,k,j,i+1), &V(u,k,j,i+1), &V(u,k,j,i), &V(u,k,j,i-1), f);
+        }
+      }
+  /* j direction (rhs.c:1709-1800) */
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 0; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        double ucon = V(uc,k,j,i).y * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
+        ocmp *f = &V(&fp2,k,j,i);
+        if (j > 0 && j < my - 2 && FL(S(nv,k,j+1,i)) && FL(S(nv,k,j-1,i)))
+          quick(um, up, &V(u,k,j+2,i), &V(u,k,j+1,i), &V(u,k,j,i), &V(u,k,j-1,i), f);
+        else if (j == 0 || S(nv,k,j-1,i) > 0.1)
+          quick(um, up, &V(u,k,j+2,i), &V(u,k,j+1,i), &V(u,k,j,i), &V(u,k,j,i), f);
+        else if (j == my - 2 || S(nv,k,j+1,i) > 0.1)
+          quick(um, up, &V(u,k,j+1,i), &V(u,k,j+1,i), &V(u,k,j,i), &V(u,k,j-1,i), f);
+      }
+  /* k direction (rhs.c:1803-1897) */
+  for (int k = 0; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        double ucon = V(uc,k,j,i).z * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
+        ocmp *f = &V(&fp3,k,j,i);
+        if (k > 0 && k < mz - 2 && FL(S(nv,k+1,j,i)) && FL(S(nv,k-1,j,i)))
+          quick(um, up, &V(u,k+2,j,i), &V(u,k+1,j,i), &V(u,k,j,i), &V(u,k-1,j,i), f);
+        else if (k < mz - 2 && (k == 0 || S(nv,k-1,j,i) > 0.1)) {
+          if (L->bctype[4] == 7 && k == 0 && (S(nv,k-1,j,i) < 0.1 && S(nv,k+1,j,i) < 0.1))
+            quick(um, up, &V(u,k+2,j,i), &V(u,k+1,j,i), &V(u,k,j,i), &V(u,k-1,j,i), f);
+          else
+            quick(um, up, &V(u,k+2,j,i), &V(u,k+1,j,i), &V(u,k,j,i), &V(u,k,j,i), f);
+        } else if (k > 0 && (k == mz - 2 || S(nv,k+1,j,i) > 0.1)) {
+          if (L->bctype[4] == 7 && k == mz - 2 && (S(nv,k-1,j,i) < 0.1 && S(nv,k+1,j,i) < 0.1))
+            quick(um, up, &V(u,k+2,j,i), &V(u,k+1,j,i), &V(u,k,j,i), &V(u,k-1,j,i), f);
+          else
+            quick(um, up, &V(u,k+1,j,i), &V(u,k+1,j,i), &V(u,k,j,i), &V(u,k-1,j,i), f);
+        }
+      }
+#undef FL
+  of_zero(conv);
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        ocmp *cv = &V(conv,k,j,i);
+        cv->x = V(&fp1,k,j,i).x - V(&fp1,k,j,i-1).x + V(&fp2,k,j,i).x - V(&fp2,k,j-1,i).x +
+                V(&fp3,k,j,i).x - V(&fp3,k-1,j,i).x;
+        cv->y = V(&fp1,k,j,i).y - V(&fp1,k,j,i-1).y + V(&fp2,k,j,i).y - V(&fp2,k,j-1,i).y +
+                V(&fp3,k,j,i).y - V(&fp3,k-1,j,i).y;
+        cv->z = V(&fp1,k,j,i).z - V(&fp1,k,j,i-1).z + V(&fp2,k,j,i).z - V(&fp2,k,j-1,i).z +
+                V(&fp3,k,j,i).z - V(&fp3,k-1,j,i).z;
+      }
+  of_free(&fp1); of_free(&fp2); of_free(&fp3);
+}
+
+/* ------------------------------------------------------------------ */
+/* Viscous (rhs.c:1947-2565), laminar                                   */
+/* ------------------------------------------------------------------ */
+/* viscous flux through one face given the nine velocity gradients and the
+ * face metrics (a = csi, e = eta, z = zet of the face), normal axis d:
+ * fp = (g1*du/dc + g2*du/de + g3*du/dz + r.n) * ajc * nu  (rhs.c:2148-2165) */
+static void visc_flux(int d, const double du[3][3], const ocmp *a, const ocmp *e, const ocmp *z,
+                      double ajc, double nu, ocmp *fp) {
+  double csi0 = a->x, csi1 = a->y, csi2 = a->z;
+  double eta0 = e->x, eta1 = e->y, eta2 = e->z;
+  double zet0 = z->x, zet1 = z->y, zet2 = z->z;
+  double n0, n1, n2, g11, g21, g31;
+  if (d == 0) { n0 = csi0; n1 = csi1; n2 = csi2;
+    g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2; g21 = eta0 * csi0 + eta1 * csi1 + eta2 * csi2;
+    g31 = zet0 * csi0 + zet1 * csi1 + zet2 * csi2; }
+  else if (d == 1) { n0 = eta0; n1 = eta1; n2 = eta2;
+    g11 = csi0 * eta0 + csi1 * eta1 + csi2 * eta2; g21 = eta0 * eta0 + eta1 * eta1 + eta2 * eta2;
+    g31 = zet0 * eta0 + zet1 * eta1 + zet2 * eta2; }
+  else { n0 = zet0; n1 = zet1; n2 = zet2;
+    g11 = csi0 * zet0 + csi1 * zet1 + csi2 * zet2; g21 = eta0 * zet0 + eta1 * zet1 + eta2 * zet2;
+    g31 = zet0 * zet0 + zet1 * zet1 + zet2 * zet2; }
+  /* du[comp][dir]: comp u,v,w; dir c,e,z */
+  double dudc = du[0][0], dude = du[0][1], dudz = du[0][2];
+  double dvdc = du[1][0], dvde = du[1][1], dvdz = du[1][2];
+  double dwdc = du[2][0], dwde = du[2][1], dwdz = du[2][2];
+  double r11 = dudc * csi0 + dude * eta0 + dudz * zet0;
+  double r21 = dvdc * csi0 + dvde * eta0 + dvdz * zet0;
+  double r31 = dwdc * csi0 + dwde * eta0 + dwdz * zet0;
+  double r12 = dudc * csi1 + dude * eta1 + dudz * zet1;
+  double r22 = dvdc * csi1 + dvde * eta1 + dvdz * zet1;
+  double r32 = dwdc * csi1 + dwde * eta1 + dwdz * zet1;
+  double r13 = dudc * csi2 + dude * eta2 + dudz * zet2;
+  double r23 = dvdc * csi2 + dvde * eta2 + dvdz * zet2;
+  double r33 = dwdc * csi2 + dwde * eta2 + dwdz * zet2;
+  fp->x = 0; fp->y = 0; fp->z = 0;
+  fp->x += (g11 * dudc + g21 * dude + g31 * dudz + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nu);
+  fp->y += (g11 * dvdc + g21 * dvde + g31 * dvdz + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nu);
+  fp->z += (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nu);
+}
+
+/* tangential velocity derivative on a face (rhs.c:2070-2135): one-sided
+ * toward the other pair when a neighbour cell is an IB/solid node */
+static void tangent_grad(const ofield *u, const ofield *nv, int d, int t, int i, int j, int k,
+                         double out[3]) {
+  const double solid = 0.5, innerblank = 7.;
+  int o[3] = {0, 0, 0}; o[d] = 1;
+  int up[3] = {0, 0, 0}, dn[3] = {0, 0, 0}; up[t] = 1; dn[t] = -1;
+#define NV(di, dj, dk) S(nv, k + (dk), j + (dj), i + (di))
+#define UU(di, dj, dk) V(u, k + (dk), j + (dj), i + (di))
+#define SB(n) ((n) > solid && (n) < innerblank)
+  /* pairs: the two cells of the face at tangential offset +1 and -1 */
+  int hi = SB(NV(up[0], up[1], up[2])) || SB(NV(up[0] + o[0], up[1] + o[1], up[2] + o[2]));
+  int lo = SB(NV(dn[0], dn[1], dn[2])) || SB(NV(dn[0] + o[0], dn[1] + o[1], dn[2] + o[2]));
+  const ocmp *A, *B, *C, *D; double f;
+  if (hi) {        /* (u[0]+u[o] - u[-1]-u[o-1]) * 0.5 */
+    A = &UU(o[0], o[1], o[2]); B = &UU(0, 0, 0);
+    C = &UU(o[0] + dn[0], o[1] + dn[1], o[2] + dn[2]); D = &UU(dn[0], dn[1], dn[2]); f = 0.5;
+  } else if (lo) { /* (u[+1]+u[o+1] - u[0]-u[o]) * 0.5 */
+    A = &UU(o[0] + up[0], o[1] + up[1], o[2] + up[2]); B = &UU(up[0], up[1], up[2]);
+    C = &UU(o[0], o[1], o[2]); D = &UU(0, 0, 0); f = 0.5;
+  } else {         /* (u[+1]+u[o+1] - u[-1]-u[o-1]) * 0.25 */
+    A = &UU(o[0] + up[0], o[1] + up[1], o[2] + up[2]); B = &UU(up[0], up[1], up[2]);
+    C = &UU(o[0] + dn[0], o[1] + dn[1], o[2] + dn[2]); D = &UU(dn[0], dn[1], dn[2]); f = 0.25;
+  }
+  out[0] = (A->x + B->x - C->x - D->x) * f;
+  out[1] = (A->y + B->y - C->y - D->y) * f;
+  out[2] = (A->z + B->z - C->z - D->z) * f;
+#undef NV
+#undef UU
+#undef SB
+}
+
+static void viscous(octx *c, ofield *visc) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  const ofield *u = &c->ucat, *nv = &L->nvert;
+  const double nu = 1. / c->ren;
+  ofield fp[3];
+  for (int d = 0; d < 3; d++) of_alloc(&fp[d], mx, my, mz, 3);
+  const ofield *fa[3][3] = {{&L->icsi, &L->ieta, &L->izet}, {&L->jcsi, &L->jeta, &L->jzet},
+                            {&L->kcsi, &L->keta, &L->kzet}};
+  const ofield *faj[3] = {&L->iaj, &L->jaj, &L->kaj};
+  for (int d = 0; d < 3; d++) {
+    int k0 = d == 2 ? 0 : 1, j0 = d == 1 ? 0 : 1, i0 = d == 0 ? 0 : 1;
+    for (int k = k0; k < mz - 1; k++)
+      for (int j = j0; j < my - 1; j++)
+        for (int i = i0; i < mx - 1; i++) {
+          double du[3][3];   /* du[comp][dir] */
+          int o[3] = {0, 0, 0}; o[d] = 1;
+          for (int t = 0; t < 3; t++) {
+            double g[3];
+            if (t == d) {
+              const ocmp *a = &V(u, k + o[2], j + o[1], i + o[0]), *b = &V(u, k, j, i);
+              g[0] = a->x - b->x; g[1] = a->y - b->y; g[2] = a->z - b->z;
+            } else tangent_grad(u, nv, d, t, i, j, k, g);
+            du[0][t] = g[0]; du[1][t] = g[1]; du[2][t] = g[2];
+          }
+          visc_flux(d, du, &V(fa[d][0], k, j, i), &V(fa[d][1], k, j, i), &V(fa[d][2], k, j, i),
+                    S(faj[d], k, j, i), nu, &V(&fp[d], k, j, i));
+        }
+  }
+  of_zero(visc);
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        ocmp *v = &V(visc,k,j,i);
+        v->x = (V(&fp[0],k,j,i).x - V(&fp[0],k,j,i-1).x + V(&fp[1],k,j,i).x - V(&fp[1],k,j-1,i).x +
+                V(&fp[2],k,j,i).x - V(&fp[2],k-1,j,i).x);
+        v->y = (V(&fp[0],k,j,i).y - V(&fp[0],k,j,i-1).y + V(&fp[1],k,j,i).y - V(&fp[1],k,j-1,i).y +
+                V(&fp[2],k,j,i).y - V(&fp[2],k-1,j,i).y);
+        v->z = (V(&fp[0],k,j,i).z - V(&fp[0],k,j,i-1).z + V(&fp[1],k,j,i).z - V(&fp[1],k,j-1,i).z +
+                V(&fp[2],k,j,i).z - V(&fp[2],k-1,j,i).z);
+      }
+  for (int d = 0; d < 3; d++) of_free(&fp[d]);
+}
+
+/* ------------------------------------------------------------------ */
+/* FormFunction1 (rhs.c:2567-3351)                                      */
+/* ------------------------------------------------------------------ */
+typedef struct { double dpdc, dpde, dpdz; long drops; } ff1_carry;
+
+/* tangential pressure derivative with the FF1 semantics: DROP keeps *v */
+static void ff1_tangent(const olevel *L, const ofield *p, int d, int t, int i, int j, int k,
+                        double *v, long *drops) {
+  const ofield *nv = &L->nvert;
+  int o[3] = {0, 0, 0}; o[d] = 1;
+  int up[3] = {0, 0, 0}, dn[3] = {0, 0, 0}; up[t] = 1; dn[t] = -1;
+  int m_ax[3] = {L->mx, L->my, L->mz}, ijk[3] = {i, j, k};
+  double hi = S(nv, k + up[2], j + up[1], i + up[0]) + S(nv, k + up[2] + o[2], j + up[1] + o[1], i + up[0] + o[0]);
+  double lo = S(nv, k + dn[2], j + dn[1], i + dn[0]) + S(nv, k + dn[2] + o[2], j + dn[1] + o[1], i + dn[0] + o[0]);
+  int per = (t == 0) ? (L->bctype[0] == 7) : (t == 1) ? (L->bctype[2] == 7) : (L->bctype[4] == 7);
+  int mode = o_xmode(hi, lo, ijk[t] == m_ax[t] - 2, ijk[t] == 1, per);
+#define PP(di, dj, dk) S(p, k + (dk), j + (dj), i + (di))
+  if (mode == 1)
+    *v = (PP(0,0,0) + PP(o[0],o[1],o[2]) - PP(dn[0],dn[1],dn[2]) - PP(o[0]+dn[0],o[1]+dn[1],o[2]+dn[2])) * 0.5;
+  else if (mode == 2)
+    *v = (PP(up[0],up[1],up[2]) + PP(o[0]+up[0],o[1]+up[1],o[2]+up[2]) - PP(0,0,0) - PP(o[0],o[1],o[2])) * 0.5;
+  else if (mode == 0)
+    *v = (PP(up[0],up[1],up[2]) + PP(o[0]+up[0],o[1]+up[1],o[2]+up[2]) - PP(dn[0],dn[1],dn[2]) - PP(o[0]+dn[0],o[1]+dn[1],o[2]+dn[2])) * 0.25;
+  else {
+    /* DROP: the variable keeps its previous value; count it when the face
+     * is not masked afterwards (both cells fluid) */
+    if (S(nv, k, j, i) <= 0.1 && S(nv, k + o[2], j + o[1], i + o[0]) <= 0.1) (*drops)++;
+  }
+#undef PP
+}
+
+static void ff1_face(octx *c, int d, int i, int j, int k, const ofield *rct, ofield *rhs, ff1_carry *cr) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const ofield *p = &c->p;
+  int o[3] = {0, 0, 0}; o[d] = 1;
+  double *dv[3] = {&cr->dpdc, &cr->dpde, &cr->dpdz};
+  for (int t = 0; t < 3; t++) {
+    if (t == d) *dv[t] = S(p, k + o[2], j + o[1], i + o[0]) - S(p, k, j, i);
+    else ff1_tangent(L, p, d, t, i, j, k, dv[t], &cr->drops);
+  }
+  const ocmp *cs, *et, *ze; double aj;
+  const ocmp *nrm;
+  if (d == 0) { cs = &V(&L->icsi,k,j,i); et = &V(&L->ieta,k,j,i); ze = &V(&L->izet,k,j,i); aj = S(&L->iaj,k,j,i); nrm = cs; }
+  else if (d == 1) { cs = &V(&L->jcsi,k,j,i); et = &V(&L->jeta,k,j,i); ze = &V(&L->jzet,k,j,i); aj = S(&L->jaj,k,j,i); nrm = et; }
+  else { cs = &V(&L->kcsi,k,j,i); et = &V(&L->keta,k,j,i); ze = &V(&L->kzet,k,j,i); aj = S(&L->kaj,k,j,i); nrm = ze; }
+  const double *rc0 = &V(rct, k, j, i).x, *rc1 = &V(rct, k + o[2], j + o[1], i + o[0]).x;
+  double r = 0.5 * (rc0[d] + rc1[d]);
+  r -= (cr->dpdc * (cs->x * nrm->x + cs->y * nrm->y + cs->z * nrm->z) +
+        cr->dpde * (et->x * nrm->x + et->y * nrm->y + et->z * nrm->z) +
+        cr->dpdz * (ze->x * nrm->x + ze->y * nrm->y + ze->z * nrm->z)) * aj;
+  (&V(rhs, k, j, i).x)[d] = r;
+}
+
+void o_formfunction1(octx *c, ofield *rhs) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  ofield conv, visc, rct;
+  of_alloc(&conv, mx, my, mz, 3); of_alloc(&visc, mx, my, mz, 3); of_alloc(&rct, mx, my, mz, 3);
+  o_contra2cart(c);
+  convection(c, &conv);
+  viscous(c, &visc);
+  const ofield *csi = &L->csi, *eta = &L->eta, *zet = &L->zet, *aj = &L->aj;
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        ocmp rc;   /* Rc = Visc - Conv (VecWAXPY, rhs.c:2663) */
+        rc.x = -1.0 * V(&conv,k,j,i).x + V(&visc,k,j,i).x;
+        rc.y = -1.0 * V(&conv,k,j,i).y + V(&visc,k,j,i).y;
+        rc.z = -1.0 * V(&conv,k,j,i).z + V(&visc,k,j,i).z;
+        double a = S(aj, k, j, i);
+        V(&rct,k,j,i).x = a * (0.5 * (V(csi,k,j,i).x + V(csi,k,j,i-1).x) * rc.x +
+                               0.5 * (V(csi,k,j,i).y + V(csi,k,j,i-1).y) * rc.y +
+                               0.5 * (V(csi,k,j,i).z + V(csi,k,j,i-1).z) * rc.z);
+        V(&rct,k,j,i).y = a * (0.5 * (V(eta,k,j,i).x + V(eta,k,j-1,i).x) * rc.x +
+                               0.5 * (V(eta,k,j,i).y + V(eta,k,j-1,i).y) * rc.y +
+                               0.5 * (V(eta,k,j,i).z + V(eta,k,j-1,i).z) * rc.z);
+        V(&rct,k,j,i).z = a * (0.5 * (V(zet,k,j,i).x + V(zet,k-1,j,i).x) * rc.x +
+                               0.5 * (V(zet,k,j,i).y + V(zet,k-1,j,i).y) * rc.y +
+                               0.5 * (V(zet,k,j,i).z + V(zet,k-1,j,i).z) * rc.z);
+      }
+  o_g2l(&rct, L->per);      /* DMLocalToLocal (rhs.c:2720) */
+  if (L->bctype[0] == 7)
+    for (int k = 1; k < mz - 1; k++) for (int j = 1; j < my - 1; j++) {
+      V(&rct,k,j,0).x = V(&rct,k,j,-2).x; V(&rct,k,j,mx-1).x = V(&rct,k,j,mx+1).x;
+    }
+  if (L->bctype[4] == 7)
+    for (int j = 1; j < my - 1; j++) for (int i = 1; i < mx - 1; i++) {
+      V(&rct,0,j,i).z = V(&rct,-2,j,i).z; V(&rct,mz-1,j,i).z = V(&rct,mz+1,j,i).z;
+    }
+  ff1_carry cr = {0., 0., 0., 0};
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        ff1_face(c, 0, i, j, k, &rct, rhs, &cr);
+        ff1_face(c, 1, i, j, k, &rct, rhs, &cr);
+        ff1_face(c, 2, i, j, k, &rct, rhs, &cr);
+      }
+  if (L->bctype[0] == 7)
+    for (int k = 1; k < mz - 1; k++) for (int j = 1; j < my - 1; j++) ff1_face(c, 0, 0, j, k, &rct, rhs, &cr);
+  if (L->bctype[4] == 7)
+    for (int j = 1; j < my - 1; j++) for (int i = 1; i < mx - 1; i++) ff1_face(c, 2, i, j, 0, &rct, rhs, &cr);
+  c->drop_count += cr.drops;
+  /* masks (rhs.c:3280-3306) */
+  const ofield *nv = &L->nvert;
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        ocmp *r = &V(rhs,k,j,i);
+        if (S(nv,k,j,i) > 0.1) { r->x = 0; r->y = 0; r->z = 0; }
+        if (S(nv,k,j,i+1) > 0.1) r->x = 0;
+        if (S(nv,k,j+1,i) > 0.1) r->y = 0;
+        if (S(nv,k+1,j,i) > 0.1) r->z = 0;
+      }
+  of_free(&conv); of_free(&visc); of_free(&rct);
+}
+
+/* ------------------------------------------------------------------ */
+/* RungeKutta (implicitsolver.c:32-119)                                 */
+/* ------------------------------------------------------------------ */
+void o_rungekutta(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
+  o_formbcs(c);
+  of_zero(&c->rhs);
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  for (int istage = 0; istage < 4; istage++) {
+    o_formfunction1(c, &c->rhs);
+    double a = alfa[istage] * c->dt * c->st;
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+          ocmp *u = &V(&c->ucont,k,j,i); const ocmp *uo = &V(&c->ucont_o,k,j,i), *r = &V(&c->rhs,k,j,i);
+          u->x = a * r->x + uo->x; u->y = a * r->y + uo->y; u->z = a * r->z + uo->z;
+        }
+    wall_flux_zero(c);
+    o_g2l(&c->ucont, L->per);
+    o_formbcs(c);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* one fractional step (solvers.c:944-1156)                             */
+/* ------------------------------------------------------------------ */
+void o_flow_step(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  of_copy(&c->ucont_o, &c->ucont);          /* main.c:1127 / 1065 */
+  double t0 = now_s();
+  o_rungekutta(c);
+  double t1 = now_s();
+  o_poisson_solve(c);
+  double t2 = now_s();
+  o_update_pressure(c);
+  o_projection(c);
+  o_g2l(&c->p, L->per);
+  double t3 = now_s();
+  o_divergence(c);
+  c->t_mom = t1 - t0; c->t_p = t2 - t1; c->t_pr = t3 - t2;
+}

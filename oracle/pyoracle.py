@@ -1,0 +1,194 @@
+"""ctypes driver for the CPU oracle library (oracle/_build/libcv_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libcv_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.o_create.restype = vp
+        L.o_create.argtypes = [C.c_int, C.c_int, C.c_int, ip, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_double, C.c_double]
+        L.o_destroy.argtypes = [vp]
+        L.o_set_coords.argtypes = [vp, dp]
+        L.o_form_metrics_all.argtypes = [vp]
+        L.o_set_nvert.argtypes = [vp, dp]
+        L.o_nvert_restrict_all.argtypes = [vp]
+        for nm in ("o_get", "o_put", "o_get_padded"):
+            getattr(L, nm).argtypes = [vp, C.c_char_p, C.c_int, dp]
+            getattr(L, nm).restype = C.c_int
+        L.o_dims.argtypes = [vp, C.c_int, ip]
+        L.o_set_solver.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.o_csr_nnz.argtypes = [vp, C.c_int]
+        L.o_csr_nnz.restype = C.c_int
+        L.o_csr_get.argtypes = [vp, C.c_int, ip, ip, dp]
+        L.o_stats.argtypes = [vp, dp]
+        for nm in ("o_rhs_into_b", "o_ff1_into_rhs", "o_lhs_all", "o_contra2cart", "o_formbcs",
+                   "o_rungekutta", "o_volumeflux_rev", "o_update_pressure", "o_projection",
+                   "o_ghostnodevelocity", "o_flow_step"):
+            getattr(L, nm).argtypes = [vp]
+        L.o_poisson_solve.argtypes = [vp]
+        L.o_poisson_solve.restype = C.c_int
+        L.o_divergence.argtypes = [vp]
+        L.o_divergence.restype = C.c_double
+        L.o_poisson_lhs.argtypes = [vp, C.c_int]
+        L.o_nullspace.argtypes = [vp, C.c_int, dp]
+        L.o_restrict.argtypes = [vp, C.c_int, dp, dp]
+        L.o_interp.argtypes = [vp, C.c_int, dp, dp]
+        L.o_matvec_level.argtypes = [vp, C.c_int, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """One oracle context: a grid hierarchy plus the fine-level state."""
+
+    def __init__(self, nodes, bctype, nlev, re, dt, semi=(0, 0, 0)):
+        L = lib()
+        bc = (C.c_int * 6)(*bctype)
+        self.h = L.o_create(int(nodes[0]), int(nodes[1]), int(nodes[2]), bc, int(nlev),
+                            int(semi[0]), int(semi[1]), int(semi[2]), float(re), float(dt))
+        if not self.h:
+            raise ValueError("oracle: unsupported configuration")
+        self.nlev = nlev
+
+    def close(self):
+        if self.h:
+            lib().o_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def dims(self, level=-1):
+        out = (C.c_int * 6)()
+        lib().o_dims(self.h, level, out)
+        return tuple(out)
+
+    def set_grid(self, xyz):
+        a = np.ascontiguousarray(xyz, dtype=np.float64)
+        lib().o_set_coords(self.h, _dp(a))
+        lib().o_form_metrics_all(self.h)
+
+    def set_nvert(self, nv):
+        a = np.ascontiguousarray(nv, dtype=np.float64)
+        lib().o_set_nvert(self.h, _dp(a))
+        lib().o_nvert_restrict_all(self.h)
+
+    def get(self, name, level=-1, dof=None):
+        mx, my, mz = self.dims(level)[:3]
+        if dof is None:
+            dof = 1 if name in ("aj", "iaj", "jaj", "kaj", "p", "phi", "b", "nvert") else 3
+        out = np.empty((mz, my, mx, dof) if dof == 3 else (mz, my, mx))
+        if lib().o_get(self.h, name.encode(), level, _dp(out)) != 0:
+            raise KeyError(name)
+        return out
+
+    def get_padded(self, name, level=-1, dof=None):
+        mx, my, mz = self.dims(level)[:3]
+        if dof is None:
+            dof = 1 if name in ("aj", "iaj", "jaj", "kaj", "p", "phi", "b", "nvert") else 3
+        shp = (mz + 6, my + 6, mx + 6) + ((3,) if dof == 3 else ())
+        out = np.empty(shp)
+        if lib().o_get_padded(self.h, name.encode(), level, _dp(out)) != 0:
+            raise KeyError(name)
+        return out
+
+    def put(self, name, arr, level=-1):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if lib().o_put(self.h, name.encode(), level, _dp(a)) != 0:
+            raise KeyError(name)
+
+    def set_solver(self, restart=30, max_it=500, rtol=1e-10, atol=0.0, smooth_it=2):
+        lib().o_set_solver(self.h, restart, max_it, rtol, atol, smooth_it)
+
+    def stats(self):
+        out = np.zeros(16)
+        lib().o_stats(self.h, _dp(out))
+        keys = ("its", "rnorm", "bnorm", "rhs_sum", "maxdiv", "maxdiv_idx", "ibm_flux0", "ibm_area0",
+                "ibm_flux1", "ibm_area1", "drops", "t_mom", "t_p", "t_pr", "threads", "multinull")
+        return dict(zip(keys, out.tolist()))
+
+    def csr(self, level):
+        mx, my, mz = self.dims(level)[:3]
+        n = mx * my * mz
+        nnz = lib().o_csr_nnz(self.h, level)
+        rp = np.empty(n + 1, dtype=np.int32)
+        col = np.empty(nnz, dtype=np.int32)
+        val = np.empty(nnz)
+        lib().o_csr_get(self.h, level, rp.ctypes.data_as(C.POINTER(C.c_int)),
+                        col.ctypes.data_as(C.POINTER(C.c_int)), _dp(val))
+        import scipy.sparse as sp
+        return sp.csr_matrix((val, col, rp), shape=(n, n))
+
+    # -- reference routines -------------------------------------------------
+    def call(self, name):
+        return getattr(lib(), name)(self.h)
+
+    def poisson_lhs(self, level):
+        lib().o_poisson_lhs(self.h, level)
+
+    def nullspace(self, level, x):
+        a = np.ascontiguousarray(x, dtype=np.float64).copy()
+        lib().o_nullspace(self.h, level, _dp(a))
+        return a
+
+    def restrict(self, lc, xf):
+        mx, my, mz = self.dims(lc)[:3]
+        out = np.empty((mz, my, mx))
+        a = np.ascontiguousarray(xf, dtype=np.float64)
+        lib().o_restrict(self.h, lc, _dp(a), _dp(out))
+        return out
+
+    def interp(self, lf, xc):
+        mx, my, mz = self.dims(lf)[:3]
+        out = np.empty((mz, my, mx))
+        a = np.ascontiguousarray(xc, dtype=np.float64)
+        lib().o_interp(self.h, lf, _dp(a), _dp(out))
+        return out
+
+    def matvec(self, level, x):
+        mx, my, mz = self.dims(level)[:3]
+        out = np.empty((mz, my, mx))
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        lib().o_matvec_level(self.h, level, _dp(a), _dp(out))
+        return out
+
+
+def from_case(case, nlev=3, semi=(0, 0, 0)):
+    """Oracle context loaded with a curvib_old_amd.cases.ChannelCase."""
+    from curvib_old_amd import cases
+    xyz, nv, uc = cases.build(case)
+    o = Oracle(case.nodes, case.bctype, nlev, case.re, case.dt, semi)
+    o.set_grid(xyz)
+    o.set_nvert(nv)
+    o.put("ucont", uc)
+    return o, (xyz, nv, uc)
