@@ -137,6 +137,14 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.ajf = (float *)p;
     if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.dinvf = (float *)p;
     if ((rc = cv_alloc(c, &p, 3 * n * sizeof(double)))) goto fail; L.coor = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.ylo = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.yip = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.ycp = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.ylof = (float *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.yipf = (float *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.ycpf = (float *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.ltmp = (double *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.ltmpf = (float *)p;
     if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.x = (double *)p;
     if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.b = (double *)p;
     if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.r = (double *)p;

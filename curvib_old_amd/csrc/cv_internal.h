@@ -47,6 +47,9 @@ struct CvLevel {
   uint32_t *flags;
   double *dinv;           // 1/diag of the operator row (smoother)
   float *gf[9], *ajf, *dinvf;  // FP32 copies for the mixed-precision V-cycle
+  double *ylo, *yip, *ycp;     // y-line smoother: lower coefficient, Thomas 1/pivot and c'
+  float *ylof, *yipf, *ycpf;
+  double *ltmp; float *ltmpf;  // line-solve forward-sweep scratch
   double *x, *b, *r;      // FP64 V-cycle work vectors
   float *xf, *bf, *rf;    // FP32 V-cycle work vectors
   double *coor;           // node coordinates, planar x|y|z (storage grid)

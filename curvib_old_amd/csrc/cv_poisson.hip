@@ -98,6 +98,7 @@ struct OpV {
   const T *aj;
   const T *dinv;
   const uint32_t *fl;
+  const T *ylo, *yip, *ycp;
   int mx, my, mz, xper, zper;
 };
 
@@ -108,14 +109,15 @@ __device__ inline long wrapped(const OpV<T> &A, int i, int j, int k) {
   return (long)i + (long)A.mx * ((long)j + (long)A.my * k);
 }
 
-// flux through face (i,j,k) of axis d (lower cell (i,j,k)); the face word w
-template <class T>
-__device__ inline T face_flux(const OpV<T> &A, const T *__restrict__ x, int d, int i, int j, int k, uint32_t w) {
+// flux through face (i,j,k) of axis d (lower cell (i,j,k)); the face word w.
+// X(q) returns the vector entry at storage index q.
+template <class T, class XA>
+__device__ inline T face_flux(const OpV<T> &A, const XA &X, int d, int i, int j, int k, uint32_t w) {
   int e[3] = {0, 0, 0}; e[d] = 1;
   const long lo = wrapped(A, i, j, k);
   const long up = wrapped(A, i + e[0], j + e[1], k + e[2]);
   const long fi = (long)i + (long)A.mx * ((long)j + (long)A.my * k);
-  const T xl = x[lo], xu = x[up];
+  const T xl = X(lo), xu = X(up);
   T F = A.g[4 * d][fi] * (xu - xl);
   int q = 0;
 #pragma unroll
@@ -127,33 +129,49 @@ __device__ inline T face_flux(const OpV<T> &A, const T *__restrict__ x, int d, i
     int f[3] = {0, 0, 0}; f[t] = 1;
     const T gdt = A.g[3 * d + t][fi];
     if (mode == XM_CENTRAL) {
-      T s = x[wrapped(A, i + f[0], j + f[1], k + f[2])] + x[wrapped(A, i + e[0] + f[0], j + e[1] + f[1], k + e[2] + f[2])] -
-            x[wrapped(A, i - f[0], j - f[1], k - f[2])] - x[wrapped(A, i + e[0] - f[0], j + e[1] - f[1], k + e[2] - f[2])];
+      T s = X(wrapped(A, i + f[0], j + f[1], k + f[2])) + X(wrapped(A, i + e[0] + f[0], j + e[1] + f[1], k + e[2] + f[2])) -
+            X(wrapped(A, i - f[0], j - f[1], k - f[2])) - X(wrapped(A, i + e[0] - f[0], j + e[1] - f[1], k + e[2] - f[2]));
       F += gdt * (T)0.25 * s;
     } else if (mode == XM_LOWER) {
-      T s = xl + xu - x[wrapped(A, i - f[0], j - f[1], k - f[2])] - x[wrapped(A, i + e[0] - f[0], j + e[1] - f[1], k + e[2] - f[2])];
+      T s = xl + xu - X(wrapped(A, i - f[0], j - f[1], k - f[2])) - X(wrapped(A, i + e[0] - f[0], j + e[1] - f[1], k + e[2] - f[2]));
       F += gdt * (T)0.5 * s;
     } else {
-      T s = x[wrapped(A, i + f[0], j + f[1], k + f[2])] + x[wrapped(A, i + e[0] + f[0], j + e[1] + f[1], k + e[2] + f[2])] - xl - xu;
+      T s = X(wrapped(A, i + f[0], j + f[1], k + f[2])) + X(wrapped(A, i + e[0] + f[0], j + e[1] + f[1], k + e[2] + f[2])) - xl - xu;
       F += gdt * (T)0.5 * s;
     }
   }
   return F;
 }
 
-template <class T>
-__device__ inline T apply_row(const OpV<T> &A, const T *__restrict__ x, int i, int j, int k, long t) {
+template <class T, class XA>
+__device__ inline T apply_row_x(const OpV<T> &A, const XA &X, int i, int j, int k, long t) {
   const uint32_t wc = A.fl[t];
-  if (!(wc & 1u)) return x[t];
+  if (!(wc & 1u)) return X(t);
   T acc = 0;
   const uint32_t wi = A.fl[t - 1], wj = A.fl[t - A.mx], wk = A.fl[t - (long)A.mx * A.my];
-  if (wc & (1u << 1)) acc += face_flux(A, x, 0, i, j, k, wc);
-  if (wi & (1u << 1)) acc -= face_flux(A, x, 0, i - 1, j, k, wi);
-  if (wc & (1u << 6)) acc += face_flux(A, x, 1, i, j, k, wc);
-  if (wj & (1u << 6)) acc -= face_flux(A, x, 1, i, j - 1, k, wj);
-  if (wc & (1u << 11)) acc += face_flux(A, x, 2, i, j, k, wc);
-  if (wk & (1u << 11)) acc -= face_flux(A, x, 2, i, j, k - 1, wk);
+  if (wc & (1u << 1)) acc += face_flux(A, X, 0, i, j, k, wc);
+  if (wi & (1u << 1)) acc -= face_flux(A, X, 0, i - 1, j, k, wi);
+  if (wc & (1u << 6)) acc += face_flux(A, X, 1, i, j, k, wc);
+  if (wj & (1u << 6)) acc -= face_flux(A, X, 1, i, j - 1, k, wj);
+  if (wc & (1u << 11)) acc += face_flux(A, X, 2, i, j, k, wc);
+  if (wk & (1u << 11)) acc -= face_flux(A, X, 2, i, j, k - 1, wk);
   return -A.aj[t] * acc;
+}
+
+template <class T>
+struct PtrX {
+  const T *__restrict__ p;
+  __device__ T operator()(long q) const { return p[q]; }
+};
+template <class T>
+struct UnitX {          // the unit vector at storage index q0
+  long q0;
+  __device__ T operator()(long q) const { return q == q0 ? (T)1 : (T)0; }
+};
+
+template <class T>
+__device__ inline T apply_row(const OpV<T> &A, const T *__restrict__ x, int i, int j, int k, long t) {
+  return apply_row_x(A, PtrX<T>{x}, i, j, k, t);
 }
 
 template <class T>
@@ -224,6 +242,141 @@ __global__ void k_diag(OpV<double> A, double *dinv, long n) {
     }
     double diag = -A.aj[t] * dsum;
     dinv[t] = diag != 0.0 ? 1.0 / diag : 1.0;
+  }
+}
+
+// ------------------------------------------------------------------
+// y-line (wall-normal) smoother: per row the coefficients of x[c-e_j],
+// x[c], x[c+e_j] taken from the operator itself, the Thomas factors of
+// every (i,k) line, and a fused residual + line-solve sweep
+// ------------------------------------------------------------------
+__global__ void k_linecoef(OpV<double> A, double *lo, double *dg, double *up, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i = (int)(t % A.mx), j = (int)((t / A.mx) % A.my), k = (int)(t / ((long)A.mx * A.my));
+    if (!(A.fl[t] & 1u)) { lo[t] = 0.0; dg[t] = 1.0; up[t] = 0.0; continue; }
+    lo[t] = apply_row_x(A, UnitX<double>{t - A.mx}, i, j, k, t);
+    dg[t] = apply_row_x(A, UnitX<double>{t}, i, j, k, t);
+    up[t] = apply_row_x(A, UnitX<double>{t + A.mx}, i, j, k, t);
+  }
+}
+
+// Thomas factors along j=1..my-2 of every (i,k) line: ip = 1/pivot,
+// cp = up/pivot; rows j=0 and my-1 are identity rows
+__global__ void k_linefactor(int mx, int my, int mz, const double *lo, const double *dg, const double *up,
+                             double *ip, double *cp, double *dinv) {
+  const long nl = (long)mx * mz;
+  for (long ln = blockIdx.x * (long)blockDim.x + threadIdx.x; ln < nl; ln += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(ln % mx), k = (int)(ln / mx);
+    const long base = (long)i + (long)mx * my * k;
+    double cprev = 0.0;
+    for (int j = 1; j < my - 1; j++) {
+      const long t = base + (long)mx * j;
+      double piv = dg[t] - lo[t] * cprev;
+      double r = piv != 0.0 ? 1.0 / piv : 1.0;
+      ip[t] = r;
+      cprev = up[t] * r;
+      cp[t] = cprev;
+      dinv[t] = dg[t] != 0.0 ? 1.0 / dg[t] : 1.0;
+    }
+    ip[base] = 1.0; cp[base] = 0.0; dinv[base] = 1.0;
+    ip[base + (long)mx * (my - 1)] = 1.0; cp[base + (long)mx * (my - 1)] = 0.0; dinv[base + (long)mx * (my - 1)] = 1.0;
+  }
+}
+
+// one damped y-line Jacobi sweep: xo = xi + w * T^{-1} (b - A xi); the
+// forward elimination keeps y in tmp
+template <class T>
+__global__ void __launch_bounds__(BS) k_yline(OpV<T> A, const T *__restrict__ xi, const T *__restrict__ b,
+                                              T *__restrict__ xo, T *__restrict__ tmp, const T *__restrict__ lo,
+                                              const T *__restrict__ ip, const T *__restrict__ cp, T omega,
+                                              int zero_guess) {
+  const long nl = (long)A.mx * A.mz;
+  const long sj = A.mx;
+  for (long ln = blockIdx.x * (long)blockDim.x + threadIdx.x; ln < nl; ln += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(ln % A.mx), k = (int)(ln / A.mx);
+    const long base = (long)i + (long)A.mx * A.my * k;
+    T yprev = 0;
+    for (int j = 1; j < A.my - 1; j++) {
+      const long t = base + sj * j;
+      T r = zero_guess ? b[t] : b[t] - apply_row(A, xi, i, j, k, t);
+      T y = (r - lo[t] * yprev) * ip[t];
+      tmp[t] = y;
+      yprev = y;
+    }
+    T enext = 0;
+    for (int j = A.my - 2; j >= 1; j--) {
+      const long t = base + sj * j;
+      T e = tmp[t] - cp[t] * enext;
+      xo[t] = (zero_guess ? (T)0 : xi[t]) + omega * e;
+      enext = e;
+    }
+    const long t0 = base, t1 = base + sj * (A.my - 1);
+    xo[t0] = zero_guess ? (T)0 : xi[t0];
+    xo[t1] = zero_guess ? (T)0 : xi[t1];
+  }
+}
+
+// chunked line solve: block (64 lines in i) x (YL_NC chunks along j); the
+// two recurrences of the factored Thomas solve (forward y_j = b_j + a_j
+// y_{j-1}, backward e_j = y_j - cp_j e_{j+1}) are composed per chunk as
+// affine maps, the chunk carries resolved through LDS, then each chunk is
+// swept again from its true carry.  r: residual (or b for a zero guess).
+#define YL_NC 4
+template <class T>
+__global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int mz, const T *r,
+                                                           const T *__restrict__ lo, const T *__restrict__ ip,
+                                                           const T *__restrict__ cp, const T *__restrict__ xi,
+                                                           T *__restrict__ xo, T *tmp, T omega,
+                                                           int zero_guess) {
+  __shared__ T SA[YL_NC][64], SB[YL_NC][64];
+  const int ti = threadIdx.x, tc = threadIdx.y;
+  const int i = blockIdx.x * 64 + ti, k = blockIdx.y;
+  const bool act = i < mx;
+  const long sj = mx;
+  const long base = (long)i + (long)mx * my * k;
+  const int nr = my - 2, cs = (nr + YL_NC - 1) / YL_NC;
+  const int j0 = 1 + tc * cs, j1 = min(1 + (tc + 1) * cs, my - 1);
+  // forward: composite of the chunk
+  T Aa = 1, Bb = 0;
+  if (act)
+    for (int j = j0; j < j1; j++) {
+      const long t = base + sj * j;
+      const T a = -lo[t] * ip[t], bb = r[t] * ip[t];
+      Aa = a * Aa;
+      Bb = a * Bb + bb;
+    }
+  SA[tc][ti] = Aa; SB[tc][ti] = Bb;
+  __syncthreads();
+  T y = 0;
+  for (int q = 0; q < tc; q++) y = SA[q][ti] * y + SB[q][ti];
+  if (act)
+    for (int j = j0; j < j1; j++) {
+      const long t = base + sj * j;
+      y = r[t] * ip[t] - lo[t] * ip[t] * y;
+      tmp[t] = y;
+    }
+  __syncthreads();
+  // backward: composite of the chunk, processed from its top row down
+  Aa = 1; Bb = 0;
+  if (act)
+    for (int j = j1 - 1; j >= j0; j--) {
+      const long t = base + sj * j;
+      const T a = -cp[t];
+      Aa = a * Aa;
+      Bb = a * Bb + tmp[t];
+    }
+  SA[tc][ti] = Aa; SB[tc][ti] = Bb;
+  __syncthreads();
+  T e = 0;
+  for (int q = YL_NC - 1; q > tc; q--) e = SA[q][ti] * e + SB[q][ti];
+  if (act) {
+    for (int j = j1 - 1; j >= j0; j--) {
+      const long t = base + sj * j;
+      e = tmp[t] - cp[t] * e;
+      xo[t] = (zero_guess ? (T)0 : xi[t]) + omega * e;
+    }
+    if (tc == 0) xo[base] = zero_guess ? (T)0 : xi[base];
+    if (tc == YL_NC - 1) xo[base + sj * (my - 1)] = zero_guess ? (T)0 : xi[base + sj * (my - 1)];
   }
 }
 
@@ -431,6 +584,7 @@ OpV<double> opview<double>(cv_ctx *c, int l) {
   OpV<double> A;
   for (int q = 0; q < 9; q++) A.g[q] = L.g[q];
   A.aj = L.aj; A.dinv = L.dinv; A.fl = L.flags;
+  A.ylo = L.ylo; A.yip = L.yip; A.ycp = L.ycp;
   A.mx = L.d.mx; A.my = L.d.my; A.mz = L.d.mz; A.xper = L.d.xper; A.zper = L.d.zper;
   return A;
 }
@@ -440,6 +594,7 @@ OpV<float> opview<float>(cv_ctx *c, int l) {
   OpV<float> A;
   for (int q = 0; q < 9; q++) A.g[q] = L.gf[q];
   A.aj = L.ajf; A.dinv = L.dinvf; A.fl = L.flags;
+  A.ylo = L.ylof; A.yip = L.yipf; A.ycp = L.ycpf;
   A.mx = L.d.mx; A.my = L.d.my; A.mz = L.d.mz; A.xper = L.d.xper; A.zper = L.d.zper;
   return A;
 }
@@ -479,7 +634,13 @@ int cv_poisson_prepare(cv_ctx *c) {
     const long n = L.d.n;
     hipLaunchKernelGGL(k_flags, dim3(nblk(n)), dim3(BS), 0, c->stream, idx_of(L), L.d.xper, L.d.zper,
                        (const double *)L.nvert, L.flags, n);
-    hipLaunchKernelGGL(k_diag, dim3(nblk(n)), dim3(BS), 0, c->stream, opview<double>(c, l), L.dinv, n);
+    // line coefficients: lo -> ylo, diag -> r (scratch), up -> b (scratch)
+    hipLaunchKernelGGL(k_linecoef, dim3(nblk(n)), dim3(BS), 0, c->stream, opview<double>(c, l), L.ylo, L.r, L.b, n);
+    hipLaunchKernelGGL(k_linefactor, dim3(nblk((long)L.d.mx * L.d.mz)), dim3(BS), 0, c->stream, L.d.mx, L.d.my, L.d.mz,
+                       (const double *)L.ylo, (const double *)L.r, (const double *)L.b, L.yip, L.ycp, L.dinv);
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.ylof, (const double *)L.ylo, n);
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.yipf, (const double *)L.yip, n);
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.ycpf, (const double *)L.ycp, n);
     for (int q = 0; q < 9; q++)
       hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.gf[q], (const double *)L.g[q], n);
     hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.ajf, (const double *)L.aj, n);
@@ -537,6 +698,14 @@ MgVecs<float> mgvecs<float>(cv_ctx *c) {
   return v;
 }
 
+// scratch for the forward elimination of the line solve, per precision
+template <class T>
+static T *line_tmp(cv_ctx *c, int l);
+template <>
+double *line_tmp<double>(cv_ctx *c, int l) { return c->lev[l].ltmp; }
+template <>
+float *line_tmp<float>(cv_ctx *c, int l) { return c->lev[l].ltmpf; }
+
 // nsweep Jacobi sweeps on level l; x in v.x[l] (zero_guess: x is 0), r used as ping-pong
 template <class T>
 static void smooth(cv_ctx *c, MgVecs<T> &v, int l, int nsweep, bool zero_guess) {
@@ -547,8 +716,24 @@ static void smooth(cv_ctx *c, MgVecs<T> &v, int l, int nsweep, bool zero_guess) 
   if (zero_guess && nsweep == 0)
     hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, v.x[l], n);
   for (int s = 0; s < nsweep; s++) {
-    hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l],
-                       v.r[l], om, (zero_guess && s == 0) ? 1 : 0, n);
+    if (c->opt.smoother == CV_SMOOTH_YLINE) {
+      // residual into tmp, then the chunked line solve reads it and keeps
+      // its forward sweep in the same array
+      T *tmpv = line_tmp<T>(c, l);
+      const bool zg = zero_guess && s == 0;
+      const T *rr = (const T *)v.b[l];
+      if (!zg) {
+        hipLaunchKernelGGL(k_resid<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l],
+                           tmpv, n);
+        rr = tmpv;
+      }
+      dim3 gb((unsigned)((L.d.mx + 63) / 64), (unsigned)L.d.mz), bb(64, YL_NC);
+      hipLaunchKernelGGL(k_yline_solve<T>, gb, bb, 0, c->stream, L.d.mx, L.d.my, L.d.mz, rr, A.ylo, A.yip, A.ycp,
+                         (const T *)v.x[l], v.r[l], tmpv, om, zg ? 1 : 0);
+    } else {
+      hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l],
+                         v.r[l], om, (zero_guess && s == 0) ? 1 : 0, n);
+    }
     T *tmp = v.x[l]; v.x[l] = v.r[l]; v.r[l] = tmp;
   }
 }
