@@ -1,0 +1,270 @@
+#!/usr/bin/env python3
+"""Fractional-step benchmark of the gfx950 CURVIB core on one MI355X.
+
+Workload: the warped periodic channel with an immersed sphere at
+512x256x256 cells (33.5 M cells, 34.2 M DMDA storage points), wall
+amplitude 0.1, sphere diameter 0.25, Re 500, tanh wall stretching beta 2,
+seeded +-1e-2 perturbations (curvib_old_amd.cases.config_case(2)), with the
+explicit-RK stable dt of that grid.  One step is the reference's
+Flow_Solver (solvers.c:944-1156): 4-stage RK momentum with FormFunction1,
+PoissonRHS, the Poisson solve to relative residual 1e-9, UpdatePressure,
+Projection and Divergence.
+
+Two contexts run the same steps from the same initial state: the FP64 path
+(FP64 V-cycle) and the mixed path (FP32 V-cycle inside the FP64 FGMRES).
+``value`` is the mixed path's steps/s; the FP64 figures, their ratio, the
+per-step pressure error of the mixed path against the FP64 path and the
+per-step Poisson residuals sit beside it.  All inputs are resident in HBM
+before the timed region starts; every step call returns after a
+synchronisation of the library's stream.
+
+``roofline``: the fine-level operator residual kernel of the V-cycle (the
+most launched stencil of the step), its algorithmic bytes per launch over
+its average device-event duration on the library stream, measured during
+the timed steps.  ``traffic``: HBM bytes per launch of the same kernel from
+the rocprofv3 FETCH_SIZE/WRITE_SIZE summary under profiles/, when present.
+
+``cpu_baseline``: the C oracle (the reference algorithm restated: assembled
+19-point CSR operator, FGMRES(30) with a 3-level V-cycle and block-Jacobi
+ILU(0) Richardson smoothing, RK momentum) timed for one fractional step of
+the same workload at 192x96x96 cells, scaled by cell count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fractional steps/s at 512x256x256 (FP64 vs mixed) + p50/p99 ensemble latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cells_of(s):
+    v = tuple(int(x) for x in s.lower().split("x"))
+    if len(v) != 3:
+        raise ValueError(s)
+    return v
+
+
+def workload_case(cells):
+    from curvib_old_amd import cases
+    full = cases.config_case(2)
+    dt = cases.stable_dt(full)
+    return cases.config_case(2, cells=tuple(cells), dt=dt)
+
+
+SOLVER = dict(smoother="yline", omega=0.8, pre_sweeps=2, post_sweeps=2, coarse_sweeps=10,
+              restart=30, max_it=200, rtol=1e-9, atol=0.0)
+
+
+def make_ctx(case, arrays, levels, semi, precision):
+    from curvib_old_amd import solver
+    xyz, nv, uc = arrays
+    u = solver.UserCtx(case.nodes, case.bctype, levels, case.re, case.dt, semi=semi)
+    u.set_solver(precision=precision, **SOLVER)
+    u.set_grid(xyz)
+    u.set_nvert(nv)
+    u.put("ucont", uc)
+    return u
+
+
+def reset(u, arrays):
+    _, _, uc = arrays
+    u.put("ucont", uc)
+    z = np.zeros(u._shape("p", -1))
+    u.put("p", z)
+    u.put("phi", z)
+
+
+def timed_steps(u, tag, warmup, steps, prof=None):
+    for s in range(warmup):
+        st = u.Flow_Solver()
+        log(f"[{tag}] warmup {s}: its={st['ksp_its']} t_p={st['t_p']:.4f}s")
+    if prof is not None:
+        u.prof_select(*prof)
+    its = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        st = u.Flow_Solver()
+        its.append(st["ksp_its"])
+    t1 = time.perf_counter()
+    pr = u.prof_read() if prof is not None else None
+    if prof is not None:
+        u.prof_select("none")
+    log(f"[{tag}] {steps} steps in {t1 - t0:.3f}s its={its}")
+    return t1 - t0, its, pr
+
+
+def accuracy_pass(u64, umx, arrays, steps):
+    reset(u64, arrays)
+    reset(umx, arrays)
+    rows = []
+    for s in range(steps):
+        a = u64.Flow_Solver()
+        b = umx.Flow_Solver()
+        p64 = u64.get("p")
+        pmx = umx.get("p")
+        den = float(np.linalg.norm(p64))
+        err = float(np.linalg.norm(pmx - p64)) / den if den > 0 else 0.0
+        rows.append(dict(step=s, p_rel_l2=err,
+                         fp64_its=a["ksp_its"], mixed_its=b["ksp_its"],
+                         fp64_rel_res=a["ksp_rnorm"] / a["ksp_bnorm"] if a["ksp_bnorm"] else 0.0,
+                         mixed_rel_res=b["ksp_rnorm"] / b["ksp_bnorm"] if b["ksp_bnorm"] else 0.0,
+                         fp64_maxdiv=a["maxdiv"], mixed_maxdiv=b["maxdiv"]))
+        log(f"[accuracy] step {s}: p rel L2 {err:.3e} its {a['ksp_its']}/{b['ksp_its']}")
+    return rows
+
+
+def traffic_from_profiles(kernel_tag):
+    """HBM bytes per launch recorded by the rocprofv3 counter summary of this
+    workload's kernel, or None."""
+    p = os.path.join(HERE, "profiles", "traffic_stencil_resid_512x256x256.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = d.get(kernel_tag)
+    return float(e["hbm_bytes_per_launch"]) if e and "hbm_bytes_per_launch" in e else None
+
+
+def cpu_baseline(sample_cells, full_cells, threads):
+    """One fractional step of the oracle on a reduced version of the
+    workload; steps/s at the full size scaled by cell count."""
+    if threads:
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    from curvib_old_amd import cases
+    import oracle.pyoracle as po
+    case = workload_case(sample_cells)
+    o, _ = po.from_case(case, nlev=3)
+    o.set_solver(restart=30, max_it=500, rtol=SOLVER["rtol"], atol=0.0, smooth_it=2)
+    t0 = time.perf_counter()
+    o.call("o_flow_step")
+    dt = time.perf_counter() - t0
+    st = o.stats()
+    o.close()
+    ncell = float(np.prod(sample_cells))
+    nfull = float(np.prod(full_cells))
+    cups = ncell / dt
+    log(f"[cpu] oracle step at {sample_cells}: {dt:.2f}s its={int(st['its'])} threads={int(st['threads'])}")
+    return {
+        "value": cups / nfull,
+        "unit": "steps/s",
+        "cores": int(st["threads"]),
+        "kind": "port",
+        "sample": (f"one fractional step of the {'x'.join(map(str, sample_cells))} version of the workload "
+                   f"({ncell / 1e6:.2f} M cells, {int(st['its'])} FGMRES its to rtol {SOLVER['rtol']:g}) "
+                   f"on the C oracle in {dt:.2f} s = {cups / 1e6:.3f} M cell-updates/s, scaled to "
+                   f"{nfull / 1e6:.1f} M cells"),
+        "cell_updates_per_s": cups,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", default="512x256x256")
+    ap.add_argument("--levels", type=int, default=7)
+    ap.add_argument("--semi", default="0,1,0")
+    ap.add_argument("--cpu-sample", default="192x96x96")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; runs on more than one GPU are not supported "
+              "(the single-case path would shard by k-slabs with halo exchange, not built here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    cells = cells_of(args.cells)
+    semi = tuple(int(v) for v in args.semi.split(","))
+    from curvib_old_amd import cases
+    case = workload_case(cells)
+    t0 = time.perf_counter()
+    arrays = cases.build(case)
+    log(f"[setup] case {cells} dt={case.dt:g} built in {time.perf_counter() - t0:.1f}s")
+
+    u64 = make_ctx(case, arrays, args.levels, semi, "fp64")
+    umx = make_ctx(case, arrays, args.levels, semi, "mixed")
+    log(f"[setup] device bytes per context: {u64.device_bytes() / 2**30:.1f} GiB")
+
+    t64, its64, _ = timed_steps(u64, "fp64", args.warmup, args.steps)
+    tmx, itsmx, pr = timed_steps(umx, "mixed", args.warmup, args.steps, prof=("resid", -1, 1))
+
+    acc = [] if args.no_accuracy else accuracy_pass(u64, umx, arrays, args.steps)
+
+    ncell = float(np.prod(cells))
+    npts = float(np.prod([c + 2 for c in cells]))
+    sps_mx = args.steps / tmx
+    sps_64 = args.steps / t64
+
+    launches, ms, byts = pr
+    achieved = (byts / (ms * 1e-3)) / 1e9 if ms > 0 else 0.0
+    roof = {
+        "bound": "hbm",
+        "kernel": "k_stencil<float, 1> (fine-level operator residual r = b - A x of the FP32 V-cycle)",
+        "launches": launches,
+        "avg_us": 1e3 * ms / launches if launches else None,
+        "bytes_per_launch": byts / launches if launches else None,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic_from_profiles("k_stencil<float, 1>"),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cells_of(args.cpu_sample), cells, args.cpu_threads)
+
+    out = {
+        "metric": METRIC,
+        "value": sps_mx,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * tmx / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64/f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"warped channel + immersed sphere {'x'.join(map(str, cells))} cells, Re {case.re:g}, "
+                        f"dt {case.dt:g}, Poisson rtol {SOLVER['rtol']:g}",
+            "cells": int(ncell), "storage_points": int(npts),
+            "mg_levels": args.levels, "semi_coarsening": list(semi), **SOLVER,
+        },
+        "fp64": {"steps_per_s": sps_64, "ms_per_step": 1e3 * t64 / args.steps, "its": its64,
+                 "cell_updates_per_s": ncell * sps_64},
+        "mixed": {"steps_per_s": sps_mx, "ms_per_step": 1e3 * tmx / args.steps, "its": itsmx,
+                  "cell_updates_per_s": ncell * sps_mx},
+        "mixed_over_fp64": sps_mx / sps_64,
+        "accuracy": acc,
+        "ensemble_latency_ms": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "gpu_mixed_over_cpu": (sps_mx / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

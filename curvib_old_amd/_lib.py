@@ -36,8 +36,10 @@ SYMBOLS = [
     "cv_contra2cart", "cv_form_bcs", "cv_form_function1", "cv_rungekutta", "cv_volumeflux_rev",
     "cv_poisson_rhs", "cv_poisson_apply", "cv_poisson_solve", "cv_update_pressure",
     "cv_projection", "cv_divergence", "cv_flow_step", "cv_mg_restrict", "cv_mg_prolong",
-    "cv_nullspace_project",
+    "cv_nullspace_project", "cv_prof_select", "cv_prof_read",
 ]
+
+CV_PROF = {"none": 0, "resid": 1, "apply": 2, "line": 3, "rct": 4, "ff1": 5, "orth": 6}
 
 
 class GridDesc(C.Structure):
@@ -101,6 +103,8 @@ def load():
     L.cv_mg_restrict.argtypes = [vp, C.c_int, dp, dp]
     L.cv_mg_prolong.argtypes = [vp, C.c_int, dp, dp]
     L.cv_nullspace_project.argtypes = [vp, C.c_int, dp]
+    L.cv_prof_select.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.cv_prof_read.argtypes = [vp, C.POINTER(C.c_long), dp, dp, C.c_int]
     for s in SYMBOLS:
         f = getattr(L, s)
         if s not in ("cv_default_solver", "cv_device_bytes", "cv_last_error"):

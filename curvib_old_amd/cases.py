@@ -223,6 +223,23 @@ def initial_ucont(case: ChannelCase, xyz: np.ndarray, nvert: np.ndarray) -> np.n
     return uc
 
 
+def stable_dt(case: ChannelCase, safety: float = 0.4) -> float:
+    """Largest dt the explicit 4-stage RK momentum step tolerates on the
+    case's grid, the viscous limit dt <= safety * dy_min^2 * Re (dy_min: the
+    first cell of the tanh-stretched wall-normal spacing), capped at the
+    case's own dt.  The reference's RK driver (implicitsolver.c:32-119) has
+    no such check."""
+    JM = case.nodes[1]
+    eta = np.arange(JM) / (JM - 1)
+    b = case.beta
+    y = 0.5 * (1.0 + np.tanh(b * (2 * eta - 1)) / np.tanh(b)) if b > 0 else eta
+    dy = float(np.min(np.diff(y))) * case.ly
+    lim = safety * dy * dy * case.re
+    e = np.floor(np.log10(lim))
+    lim = float(f"{int(np.floor(lim / 10 ** e))}e{int(e)}")   # one significant digit, rounded down
+    return float(min(case.dt, lim))
+
+
 def build(case: ChannelCase):
     """Return (xyz_nodes (KM,JM,IM,3), nvert (mz,my,mx), ucont (mz,my,mx,3))."""
     xyz = node_coords(case)

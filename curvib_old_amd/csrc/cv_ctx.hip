@@ -164,7 +164,7 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     double **sc[] = {&M.aj, &M.iaj, &M.jaj, &M.kaj};
     for (auto s : sc) { if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; *s = (double *)p; }
     M.aj = c->lev[c->nlev - 1].aj ? M.aj : M.aj;
-    double **st3[] = {c->ucont, c->ucont_o, c->ucat, c->rhs, c->rct, c->conv, c->visc};
+    double **st3[] = {c->ucont, c->ucont_o, c->ucat, c->rhs, c->rct, c->fface, c->fface + 3, c->fface + 6};
     for (auto v : st3)
       for (int q = 0; q < 3; q++) {
         if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail;
@@ -201,6 +201,7 @@ extern "C" int cv_ctx_destroy(cv_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->red_host) (void)hipHostFree(c->red_host);
+  for (auto e : c->prof.ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return CV_OK;
@@ -306,5 +307,70 @@ extern "C" int cv_put_field(cv_ctx *c, int f, int level, const double *in) {
     c->bytes_alloc -= 3 * n * sizeof(double);
   }
   CV_CHECK(hipStreamSynchronize(c->stream));
+  return CV_OK;
+}
+
+// ---------------------------------------------------------------------
+// kernel timing (device events on the context stream)
+// ---------------------------------------------------------------------
+#define CV_PROF_POOL 256
+
+CvProfScope::CvProfScope(cv_ctx *c_, int kind, int level, int prec, double bytes) : c(c_), on(false) {
+  CvProf &P = c->prof;
+  if (P.kind == CV_PROF_NONE || P.kind != kind) return;
+  const int lev = P.level < 0 ? c->nlev - 1 : P.level;
+  if (lev != level || (P.prec >= 0 && P.prec != prec)) return;
+  if (P.pending == CV_PROF_POOL && cv_prof_drain(c) != CV_OK) return;
+  if (hipEventRecord(P.ev[2 * P.pending], c->stream) != hipSuccess) return;
+  on = true;
+  P.bytes += bytes;
+}
+
+CvProfScope::~CvProfScope() {
+  if (!on) return;
+  CvProf &P = c->prof;
+  (void)hipEventRecord(P.ev[2 * P.pending + 1], c->stream);
+  P.pending++;
+  P.launches++;
+}
+
+int cv_prof_drain(cv_ctx *c) {
+  CvProf &P = c->prof;
+  if (P.pending == 0) return CV_OK;
+  CV_CHECK(hipEventSynchronize(P.ev[2 * P.pending - 1]));
+  for (int q = 0; q < P.pending; q++) {
+    float ms = 0.f;
+    CV_CHECK(hipEventElapsedTime(&ms, P.ev[2 * q], P.ev[2 * q + 1]));
+    P.ms += ms;
+  }
+  P.pending = 0;
+  return CV_OK;
+}
+
+extern "C" int cv_prof_select(cv_ctx *c, int kernel, int level, int precision) {
+  if (!c || kernel < CV_PROF_NONE || kernel > CV_PROF_ORTH || level >= c->nlev) return CV_ERR_ARG;
+  CV_CHECK(hipSetDevice(c->device));
+  int rc = cv_prof_drain(c);
+  if (rc) return rc;
+  CvProf &P = c->prof;
+  if (P.ev.empty()) {
+    P.ev.resize(2 * CV_PROF_POOL);
+    for (auto &e : P.ev) CV_CHECK(hipEventCreate(&e));
+  }
+  P.kind = kernel; P.level = level; P.prec = precision;
+  P.launches = 0; P.ms = 0.0; P.bytes = 0.0;
+  return CV_OK;
+}
+
+extern "C" int cv_prof_read(cv_ctx *c, long *launches, double *total_ms, double *bytes, int reset) {
+  if (!c) return CV_ERR_ARG;
+  CV_CHECK(hipSetDevice(c->device));
+  int rc = cv_prof_drain(c);
+  if (rc) return rc;
+  CvProf &P = c->prof;
+  if (launches) *launches = P.launches;
+  if (total_ms) *total_ms = P.ms;
+  if (bytes) *bytes = P.bytes;
+  if (reset) { P.launches = 0; P.ms = 0.0; P.bytes = 0.0; }
   return CV_OK;
 }

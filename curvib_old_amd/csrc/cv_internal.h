@@ -62,6 +62,15 @@ struct CvMetrics {        // fine-level metrics for the momentum right-hand side
   double *kcsi[3], *keta[3], *kzet[3], *kaj;
 };
 
+// per-launch device-event timing of one selected kernel class
+struct CvProf {
+  int kind = CV_PROF_NONE, level = -1, prec = -1;
+  std::vector<hipEvent_t> ev;   // start/end pairs of pending launches
+  int pending = 0;
+  long launches = 0;
+  double ms = 0.0, bytes = 0.0;
+};
+
 struct cv_ctx {
   int device;
   hipStream_t stream;
@@ -74,7 +83,8 @@ struct cv_ctx {
   CvMetrics m;
   // fine-level state (planar)
   double *ucont[3], *ucont_o[3], *ucat[3], *rhs[3], *p, *phi, *bvec;
-  double *rct[3], *conv[3], *visc[3];
+  double *rct[3];
+  double *fface[9];       // Visc - Conv flux of the i, j, k face families (x|y|z each)
   // Krylov space
   double *V, *Z, *w;      // (restart+1)*n, restart*n, n
   double *red;            // reduction scratch (device)
@@ -88,7 +98,19 @@ struct cv_ctx {
   size_t bytes_alloc;
   std::vector<void *> allocs;
   int geometry_set, nvert_set;
+  CvProf prof;
 };
+
+// Records device events around one launch when (kind, level, precision)
+// matches the selection; prec 0 = FP64, 1 = FP32.  bytes: algorithmic bytes
+// the launch moves.
+struct CvProfScope {
+  cv_ctx *c;
+  bool on;
+  CvProfScope(cv_ctx *c, int kind, int level, int prec, double bytes);
+  ~CvProfScope();
+};
+int cv_prof_drain(cv_ctx *c);
 
 // helpers implemented in cv_ctx.hip
 int cv_alloc(cv_ctx *c, void **p, size_t bytes);

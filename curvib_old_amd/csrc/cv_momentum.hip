@@ -9,9 +9,9 @@
 //  RungeKutta         implicitsolver.c:32-119
 //  Flow_Solver        solvers.c:944-1156
 //
-// Face fluxes are recomputed by the two cells that share a face instead of
-// being stored (the reference writes three temporary face-flux vectors per
-// term); the arithmetic per face is the reference's.
+// Each face flux (Visc - Conv) is evaluated once per face into a face
+// family array, as the reference's Fp1/Fp2/Fp3 temporaries; the arithmetic
+// per face is the reference's.
 #include "cv_internal.h"
 #include "cv_reduce.h"
 #include <cmath>
@@ -326,32 +326,44 @@ __device__ inline void visc_flux(const G &g, int d, int i, int j, int k, C3 u, c
   f[2] += (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nu);
 }
 
-// Rc = Visc - Conv and rct = aj*(avg basis).Rc at interior cells (rhs.c:2663-2703)
-__global__ void __launch_bounds__(BS) k_rct(G g, C3 uc, C3 u, const double *nv, FM mi, FM mj, FM mk, C3 csi, C3 eta, C3 zet,
-                                            const double *aj, double nu, V3 rct, long n) {
+// Visc - Conv flux through every face of family D: face index along D in
+// [0, m_D - 2], the other two indices interior (the faces rhs.c:1636-1897
+// and rhs.c:2062-2486 evaluate into Fp1/Fp2/Fp3)
+template <int D>
+__global__ void __launch_bounds__(BS) k_face_rc(G g, C3 uc, C3 u, const double *nv, FM m, double nu, V3 F, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.dec(t, i, j, k);
+    const int c[3] = {i, j, k}, mm[3] = {g.mx, g.my, g.mz};
+    bool valid = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      if (a == D) valid = valid && c[a] >= 0 && c[a] <= mm[a] - 2;
+      else valid = valid && c[a] >= 1 && c[a] <= mm[a] - 2;
+    }
+    if (!valid) continue;
+    double cf[3], vf[3];
+    conv_flux(g, D, i, j, k, uc, u, nv, cf);
+    visc_flux(g, D, i, j, k, u, nv, m, nu, vf);
+    F.x[t] = vf[0] - cf[0];
+    F.y[t] = vf[1] - cf[1];
+    F.z[t] = vf[2] - cf[2];
+  }
+}
+
+// Rc = Visc - Conv as the divergence of the face fluxes, and
+// rct = aj*(avg basis).Rc at interior cells (rhs.c:2663-2703)
+__global__ void __launch_bounds__(BS) k_rct(G g, C3 f0, C3 f1, C3 f2, C3 csi, C3 eta, C3 zet, const double *aj, V3 rct,
+                                            long n) {
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     int i, j, k;
     g.dec(t, i, j, k);
     if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
-    double a[3], b[3], c2[3], d2[3], e[3], f[3];
-    double cv[3], vs[3];
-    conv_flux(g, 0, i, j, k, uc, u, nv, a);
-    conv_flux(g, 0, i - 1, j, k, uc, u, nv, b);
-    conv_flux(g, 1, i, j, k, uc, u, nv, c2);
-    conv_flux(g, 1, i, j - 1, k, uc, u, nv, d2);
-    conv_flux(g, 2, i, j, k, uc, u, nv, e);
-    conv_flux(g, 2, i, j, k - 1, uc, u, nv, f);
-    for (int q = 0; q < 3; q++) cv[q] = a[q] - b[q] + c2[q] - d2[q] + e[q] - f[q];
-    visc_flux(g, 0, i, j, k, u, nv, mi, nu, a);
-    visc_flux(g, 0, i - 1, j, k, u, nv, mi, nu, b);
-    visc_flux(g, 1, i, j, k, u, nv, mj, nu, c2);
-    visc_flux(g, 1, i, j - 1, k, u, nv, mj, nu, d2);
-    visc_flux(g, 2, i, j, k, u, nv, mk, nu, e);
-    visc_flux(g, 2, i, j, k - 1, u, nv, mk, nu, f);
-    for (int q = 0; q < 3; q++) vs[q] = (a[q] - b[q] + c2[q] - d2[q] + e[q] - f[q]);
-    double rcx = -1.0 * cv[0] + vs[0], rcy = -1.0 * cv[1] + vs[1], rcz = -1.0 * cv[2] + vs[2];
-    const double A = aj[t];
     const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
+    const double rcx = (f0.x[t] - f0.x[ti]) + (f1.x[t] - f1.x[tj]) + (f2.x[t] - f2.x[tk]);
+    const double rcy = (f0.y[t] - f0.y[ti]) + (f1.y[t] - f1.y[tj]) + (f2.y[t] - f2.y[tk]);
+    const double rcz = (f0.z[t] - f0.z[ti]) + (f1.z[t] - f1.z[tj]) + (f2.z[t] - f2.z[tk]);
+    const double A = aj[t];
     rct.x[t] = A * (0.5 * (csi.x[t] + csi.x[ti]) * rcx + 0.5 * (csi.y[t] + csi.y[ti]) * rcy + 0.5 * (csi.z[t] + csi.z[ti]) * rcz);
     rct.y[t] = A * (0.5 * (eta.x[t] + eta.x[tj]) * rcx + 0.5 * (eta.y[t] + eta.y[tj]) * rcy + 0.5 * (eta.z[t] + eta.z[tj]) * rcz);
     rct.z[t] = A * (0.5 * (zet.x[t] + zet.x[tk]) * rcx + 0.5 * (zet.y[t] + zet.y[tk]) * rcy + 0.5 * (zet.z[t] + zet.z[tk]) * rcz);
@@ -660,9 +672,19 @@ int cv_ff1_dev(cv_ctx *c) {
   CvMetrics &M = c->m;
   FFArgs A = ffargs(c);
   const double nu = 1. / c->ren;
-  hipLaunchKernelGGL(k_rct, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, c3(c->ucont), c3(c->ucat),
-                     (const double *)F.nvert, A.mi, A.mj, A.mk, c3(M.csi), c3(M.eta), c3(M.zet),
-                     (const double *)M.aj, nu, v3(c->rct), F.d.n);
+  const C3 uc = c3(c->ucont), u = c3(c->ucat);
+  const V3 f0 = v3(c->fface), f1 = v3(c->fface + 3), f2 = v3(c->fface + 6);
+  hipLaunchKernelGGL(k_face_rc<0>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, A.mi,
+                     nu, f0, F.d.n);
+  hipLaunchKernelGGL(k_face_rc<1>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, A.mj,
+                     nu, f1, F.d.n);
+  hipLaunchKernelGGL(k_face_rc<2>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, A.mk,
+                     nu, f2, F.d.n);
+  {
+    CvProfScope ps(c, CV_PROF_RCT, c->nlev - 1, 0, (double)F.d.n * 8.0 * (9 + 9 + 1 + 3));
+    hipLaunchKernelGGL(k_rct, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, c3(c->fface), c3(c->fface + 3),
+                       c3(c->fface + 6), c3(M.csi), c3(M.eta), c3(M.zet), (const double *)M.aj, v3(c->rct), F.d.n);
+  }
   if (F.d.xper || F.d.zper)
     hipLaunchKernelGGL(k_rct_periodic, dim3(nblk((long)A.g.my * A.g.mz + (long)A.g.mx * A.g.my)), dim3(BS), 0, c->stream,
                        A.g, v3(c->rct));

@@ -174,23 +174,6 @@ __device__ inline T apply_row(const OpV<T> &A, const T *__restrict__ x, int i, i
   return apply_row_x(A, PtrX<T>{x}, i, j, k, t);
 }
 
-template <class T>
-__global__ void __launch_bounds__(BS) k_apply(OpV<T> A, const T *__restrict__ x, T *__restrict__ y, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int i = (int)(t % A.mx), j = (int)((t / A.mx) % A.my), k = (int)(t / ((long)A.mx * A.my));
-    y[t] = apply_row(A, x, i, j, k, t);
-  }
-}
-
-// r = b - A x
-template <class T>
-__global__ void __launch_bounds__(BS) k_resid(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int i = (int)(t % A.mx), j = (int)((t / A.mx) % A.my), k = (int)(t / ((long)A.mx * A.my));
-    r[t] = b[t] - apply_row(A, x, i, j, k, t);
-  }
-}
-
 // damped Jacobi sweep: xo = xi + w*D^-1*(b - A xi); zero_guess: xi == 0
 template <class T>
 __global__ void __launch_bounds__(BS) k_jacobi(OpV<T> A, const T *__restrict__ xi, const T *__restrict__ b, T *__restrict__ xo,
@@ -203,45 +186,146 @@ __global__ void __launch_bounds__(BS) k_jacobi(OpV<T> A, const T *__restrict__ x
   }
 }
 
-// diagonal of the operator row (coefficient of x_c)
-__global__ void k_diag(OpV<double> A, double *dinv, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    const uint32_t wc = A.fl[t];
-    if (!(wc & 1u)) { dinv[t] = 1.0; continue; }
-    const long st[3] = {1, A.mx, (long)A.mx * A.my};
-    double dsum = 0;
-    for (int d = 0; d < 3; d++) {
-      // upper face: flux = g_dd (x_u - x_c) + cross; coefficient of x_c
-      uint32_t w = wc;
-      long fi = t;
-      if (w & (1u << (1 + 5 * d))) {
-        double c0 = -A.g[4 * d][fi];
-        int q = 0;
-        for (int tt = 0; tt < 3; tt++) {
-          if (tt == d) continue;
-          int mode = (w >> (2 + 5 * d + 2 * q)) & 3u; q++;
-          if (mode == XM_LOWER) c0 += 0.5 * A.g[3 * d + tt][fi];
-          else if (mode == XM_UPPER) c0 -= 0.5 * A.g[3 * d + tt][fi];
-        }
-        dsum += c0;
-      }
-      // lower face: -flux, x_c is the upper cell of that face
-      w = A.fl[t - st[d]];
-      fi = t - st[d];
-      if (w & (1u << (1 + 5 * d))) {
-        double c1 = A.g[4 * d][fi];   // flux coefficient of x_u is +g_dd
-        int q = 0;
-        for (int tt = 0; tt < 3; tt++) {
-          if (tt == d) continue;
-          int mode = (w >> (2 + 5 * d + 2 * q)) & 3u; q++;
-          if (mode == XM_LOWER) c1 += 0.5 * A.g[3 * d + tt][fi];
-          else if (mode == XM_UPPER) c1 -= 0.5 * A.g[3 * d + tt][fi];
-        }
-        dsum -= c1;
+// ------------------------------------------------------------------
+// 2.5D-tiled flux-form stencil.  A block of 64 x ST_TY threads owns 63
+// cells along i (lane L <-> cell i0-1+L; lane 0 only supplies the flux of
+// face i0-1) and ST_TY rows along j, and marches k over one chunk.  The x
+// planes k-1, k, k+1 of the tile (+1 halo, periodic wrap applied at load)
+// live in LDS; every face flux is formed once: the lower i-face comes from
+// the neighbouring lane (shuffle), the lower j-face from the row below
+// (LDS), the lower k-face from the previous plane (register).  The
+// arithmetic per face and per row is the same as face_flux/apply_row_x.
+//   MODE 0: y = A x      MODE 1: y = b - A x
+//   MODE 2: y = x + omega * dinv * (b - A x)   (damped Jacobi sweep)
+// ------------------------------------------------------------------
+#define ST_TY 8
+#define ST_TC 66
+#define ST_TR (ST_TY + 2)
+
+// one face flux from its two cells (xl, xu) and the four neighbours of
+// each tangential axis: (+f, +e+f, -f, +e-f)
+template <class T>
+__device__ inline T tile_flux(T xl, T xu, T gdd, uint32_t w, int sh, T g0, T a0, T b0, T c0, T d0, T g1, T a1, T b1,
+                              T c1, T d1) {
+  T F = gdd * (xu - xl);
+  const int m0 = (w >> sh) & 3u, m1 = (w >> (sh + 2)) & 3u;
+  if (m0 == XM_CENTRAL) F += g0 * (T)0.25 * (a0 + b0 - c0 - d0);
+  else if (m0 == XM_LOWER) F += g0 * (T)0.5 * (xl + xu - c0 - d0);
+  else if (m0 == XM_UPPER) F += g0 * (T)0.5 * (a0 + b0 - xl - xu);
+  if (m1 == XM_CENTRAL) F += g1 * (T)0.25 * (a1 + b1 - c1 - d1);
+  else if (m1 == XM_LOWER) F += g1 * (T)0.5 * (xl + xu - c1 - d1);
+  else if (m1 == XM_UPPER) F += g1 * (T)0.5 * (a1 + b1 - xl - xu);
+  return F;
+}
+
+// k-face flux between planes P (k) and Q (k+1) at LDS column c, row r
+template <class T>
+__device__ inline T tile_flux_k(const OpV<T> &A, const T *P, const T *Q, int c, int r, uint32_t w, long t) {
+  if (!(w & (1u << 11))) return (T)0;
+  const int o = r * ST_TC + c;
+  return tile_flux<T>(P[o], Q[o], A.g[8][t], w, 12,
+                      A.g[6][t], P[o + 1], Q[o + 1], P[o - 1], Q[o - 1],
+                      A.g[7][t], P[o + ST_TC], Q[o + ST_TC], P[o - ST_TC], Q[o - ST_TC]);
+}
+// j-face flux between rows r and r+1 of plane P (M = k-1, Q = k+1)
+template <class T>
+__device__ inline T tile_flux_j(const OpV<T> &A, const T *M, const T *P, const T *Q, int c, int r, uint32_t w, long t) {
+  if (!(w & (1u << 6))) return (T)0;
+  const int o = r * ST_TC + c, u = o + ST_TC;
+  return tile_flux<T>(P[o], P[u], A.g[4][t], w, 7,
+                      A.g[3][t], P[o + 1], P[u + 1], P[o - 1], P[u - 1],
+                      A.g[5][t], Q[o], Q[u], M[o], M[u]);
+}
+// i-face flux between columns c and c+1 of plane P
+template <class T>
+__device__ inline T tile_flux_i(const OpV<T> &A, const T *M, const T *P, const T *Q, int c, int r, uint32_t w, long t) {
+  if (!(w & (1u << 1))) return (T)0;
+  const int o = r * ST_TC + c;
+  return tile_flux<T>(P[o], P[o + 1], A.g[0][t], w, 2,
+                      A.g[1][t], P[o + ST_TC], P[o + ST_TC + 1], P[o - ST_TC], P[o - ST_TC + 1],
+                      A.g[2][t], Q[o], Q[o + 1], M[o], M[o + 1]);
+}
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(64 * ST_TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
+                                                      T *__restrict__ y, const T *__restrict__ dinv, T omega,
+                                                      int kchunk) {
+  __shared__ T X[3][ST_TR * ST_TC];
+  __shared__ T FJ[ST_TY][64];
+  const int L = threadIdx.x, R = threadIdx.y;
+  const int tid = R * 64 + L;
+  const int mx = A.mx, my = A.my, mz = A.mz;
+  const int i0 = 63 * blockIdx.x, j0 = ST_TY * blockIdx.y;
+  const int k0 = kchunk * blockIdx.z, k1 = min(k0 + kchunk, mz);
+  const int gi = i0 - 1 + L, gj = j0 + R;
+  const bool in_i = gi >= 0 && gi < mx, in_j = gj < my;
+  const bool out = L >= 1 && in_i && in_j;
+  const long pl = (long)mx * my;
+  const int c = L + 1, r = R + 1;
+
+  // fill plane gk (wrapped) into slot s
+  auto load_plane = [&](int gk, int s) {
+    const bool kin = gk >= 0 && gk < mz;
+    const long kb = kin ? pl * cv_wrap(gk, mz, A.zper) : 0;
+    for (int e = tid; e < ST_TR * ST_TC; e += 64 * ST_TY) {
+      const int rr = e / ST_TC, cc = e - rr * ST_TC;
+      const int ii = i0 - 2 + cc, jj = j0 - 1 + rr;
+      T v = (T)0;
+      if (kin && ii >= 0 && ii < mx && jj >= 0 && jj < my) v = x[kb + (long)mx * jj + cv_wrap(ii, mx, A.xper)];
+      X[s][e] = v;
+    }
+  };
+  auto slot = [](int gk) { return (gk + 3) % 3; };
+
+  // prologue: flux of the k-face below the chunk
+  load_plane(k0 - 1, slot(k0 - 1));
+  load_plane(k0, slot(k0));
+  __syncthreads();
+  T fk_lo = (T)0;
+  if (out && k0 >= 1) {
+    const long t = (long)gi + (long)mx * gj + pl * (k0 - 1);
+    fk_lo = tile_flux_k<T>(A, X[slot(k0 - 1)], X[slot(k0)], c, r, A.fl[t], t);
+  }
+  for (int gk = k0; gk < k1; gk++) {
+    load_plane(gk + 1, slot(gk + 1));
+    __syncthreads();
+    const T *Pm = X[slot(gk - 1)], *P = X[slot(gk)], *Q = X[slot(gk + 1)];
+    const long t = (long)gi + (long)mx * gj + pl * gk;
+    const uint32_t w = (in_i && in_j) ? A.fl[t] : 0u;
+    const T fi = (in_i && in_j) ? tile_flux_i<T>(A, Pm, P, Q, c, r, w, t) : (T)0;
+    T fj = (T)0, fk = (T)0, fj_lo = (T)0;
+    if (out) {
+      fj = tile_flux_j<T>(A, Pm, P, Q, c, r, w, t);
+      fk = tile_flux_k<T>(A, P, Q, c, r, w, t);
+      if (R == 0 && gj >= 1) {
+        const long tb = t - mx;
+        fj_lo = tile_flux_j<T>(A, Pm, P, Q, c, r - 1, A.fl[tb], tb);
       }
     }
-    double diag = -A.aj[t] * dsum;
-    dinv[t] = diag != 0.0 ? 1.0 / diag : 1.0;
+    FJ[R][L] = fj;
+    const T fi_lo = __shfl_up(fi, 1, 64);
+    __syncthreads();
+    if (R > 0) fj_lo = FJ[R - 1][L];
+    if (out) {
+      T res;
+      if (!(w & 1u)) {
+        const T xv = x[t];
+        if (MODE == 0) res = xv;
+        else if (MODE == 1) res = b[t] - xv;
+        else res = xv + omega * dinv[t] * (b[t] - xv);
+      } else {
+        T acc = 0;
+        acc += fi; acc -= fi_lo;
+        acc += fj; acc -= fj_lo;
+        acc += fk; acc -= fk_lo;
+        const T ax = -A.aj[t] * acc;
+        if (MODE == 0) res = ax;
+        else if (MODE == 1) res = b[t] - ax;
+        else res = P[r * ST_TC + c] + omega * dinv[t] * (b[t] - ax);
+      }
+      y[t] = res;
+    }
+    fk_lo = fk;
   }
 }
 
@@ -280,39 +364,6 @@ __global__ void k_linefactor(int mx, int my, int mz, const double *lo, const dou
     }
     ip[base] = 1.0; cp[base] = 0.0; dinv[base] = 1.0;
     ip[base + (long)mx * (my - 1)] = 1.0; cp[base + (long)mx * (my - 1)] = 0.0; dinv[base + (long)mx * (my - 1)] = 1.0;
-  }
-}
-
-// one damped y-line Jacobi sweep: xo = xi + w * T^{-1} (b - A xi); the
-// forward elimination keeps y in tmp
-template <class T>
-__global__ void __launch_bounds__(BS) k_yline(OpV<T> A, const T *__restrict__ xi, const T *__restrict__ b,
-                                              T *__restrict__ xo, T *__restrict__ tmp, const T *__restrict__ lo,
-                                              const T *__restrict__ ip, const T *__restrict__ cp, T omega,
-                                              int zero_guess) {
-  const long nl = (long)A.mx * A.mz;
-  const long sj = A.mx;
-  for (long ln = blockIdx.x * (long)blockDim.x + threadIdx.x; ln < nl; ln += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(ln % A.mx), k = (int)(ln / A.mx);
-    const long base = (long)i + (long)A.mx * A.my * k;
-    T yprev = 0;
-    for (int j = 1; j < A.my - 1; j++) {
-      const long t = base + sj * j;
-      T r = zero_guess ? b[t] : b[t] - apply_row(A, xi, i, j, k, t);
-      T y = (r - lo[t] * yprev) * ip[t];
-      tmp[t] = y;
-      yprev = y;
-    }
-    T enext = 0;
-    for (int j = A.my - 2; j >= 1; j--) {
-      const long t = base + sj * j;
-      T e = tmp[t] - cp[t] * enext;
-      xo[t] = (zero_guess ? (T)0 : xi[t]) + omega * e;
-      enext = e;
-    }
-    const long t0 = base, t1 = base + sj * (A.my - 1);
-    xo[t0] = zero_guess ? (T)0 : xi[t0];
-    xo[t1] = zero_guess ? (T)0 : xi[t1];
   }
 }
 
@@ -600,6 +651,35 @@ OpV<float> opview<float>(cv_ctx *c, int l) {
 }
 static Idx idx_of(const CvLevel &L) { return Idx{L.d.mx, L.d.my, L.d.mz}; }
 
+// k extent of one stencil block (tunable through CV_STENCIL_KCHUNK)
+static int stencil_kchunk() {
+  static int kc = 0;
+  if (!kc) {
+    const char *e = getenv("CV_STENCIL_KCHUNK");
+    kc = e ? atoi(e) : 16;
+    if (kc < 1) kc = 1;
+  }
+  return kc;
+}
+
+// algorithmic bytes per storage point of the stencil modes: x, 9 g, aj (+b)
+// (+dinv), y in T, and the 4-byte operator word
+template <class T>
+static double stencil_bytes_pt(int mode) {
+  return (double)(mode == 0 ? 12 : mode == 1 ? 13 : 14) * sizeof(T) + 4.0;
+}
+
+template <class T, int MODE>
+static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T)0) {
+  CvLevel &L = c->lev[l];
+  OpV<T> A = opview<T>(c, l);
+  const int kc = stencil_kchunk();
+  dim3 g((unsigned)((L.d.mx + 62) / 63), (unsigned)((L.d.my + ST_TY - 1) / ST_TY), (unsigned)((L.d.mz + kc - 1) / kc));
+  CvProfScope ps(c, MODE == 0 ? CV_PROF_APPLY : CV_PROF_RESID, l, sizeof(T) == 4 ? 1 : 0,
+                 (double)L.d.n * stencil_bytes_pt<T>(MODE));
+  hipLaunchKernelGGL((k_stencil<T, MODE>), g, dim3(64, ST_TY), 0, c->stream, A, x, b, y, A.dinv, omega, kc);
+}
+
 // sum of part[0..nb) -> dsc[slot] (device)
 static void final_sum(cv_ctx *c, const double *part, int nb, double *out, int nq) {
   hipLaunchKernelGGL(cv_k_final_sum, dim3(nq), dim3(BS), 0, c->stream, part, nb, out, nq);
@@ -723,16 +803,18 @@ static void smooth(cv_ctx *c, MgVecs<T> &v, int l, int nsweep, bool zero_guess) 
       const bool zg = zero_guess && s == 0;
       const T *rr = (const T *)v.b[l];
       if (!zg) {
-        hipLaunchKernelGGL(k_resid<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l],
-                           tmpv, n);
+        stencil<T, 1>(c, l, (const T *)v.x[l], (const T *)v.b[l], tmpv);
         rr = tmpv;
       }
       dim3 gb((unsigned)((L.d.mx + 63) / 64), (unsigned)L.d.mz), bb(64, YL_NC);
       hipLaunchKernelGGL(k_yline_solve<T>, gb, bb, 0, c->stream, L.d.mx, L.d.my, L.d.mz, rr, A.ylo, A.yip, A.ycp,
                          (const T *)v.x[l], v.r[l], tmpv, om, zg ? 1 : 0);
     } else {
-      hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l],
-                         v.r[l], om, (zero_guess && s == 0) ? 1 : 0, n);
+      if (zero_guess && s == 0)
+        hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l],
+                           v.r[l], om, 1, n);
+      else
+        stencil<T, 2>(c, l, (const T *)v.x[l], (const T *)v.b[l], v.r[l], om);
     }
     T *tmp = v.x[l]; v.x[l] = v.r[l]; v.r[l] = tmp;
   }
@@ -749,8 +831,7 @@ static void vcycle(cv_ctx *c, MgVecs<T> &v, int l) {
   }
   CvLevel &C = c->lev[l - 1];
   smooth<T>(c, v, l, c->opt.pre_sweeps, true);
-  OpV<T> A = opview<T>(c, l);
-  hipLaunchKernelGGL(k_resid<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l], v.r[l], n);
+  stencil<T, 1>(c, l, (const T *)v.x[l], (const T *)v.b[l], v.r[l]);
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.d.n)), dim3(BS), 0, c->stream, idx_of(C), idx_of(L), C.d.isc, C.d.jsc,
                      C.d.ksc, (const double *)L.nvert, (const double *)C.nvert, (const T *)v.r[l], v.b[l - 1], C.d.n);
   vcycle<T>(c, v, l - 1);
@@ -808,7 +889,6 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
   double *V = c->V, *Z = c->Z, *w = c->w;
   double *hd = c->dsc;              // device: h[0..33]
   double *hh = c->red_host;         // pinned host
-  OpV<double> A = opview<double>(c, top);
   // ||b||
   hipLaunchKernelGGL(k_sumsq, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, b, n, c->part);
   final_sum(c, c->part, CV_RED_GRID, hd + 40, 1);
@@ -834,7 +914,7 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
     for (; jj < m && its < c->opt.max_it; jj++) {
       double *vj = V + (size_t)jj * n, *zj = Z + (size_t)jj * n;
       precondition(c, vj, zj);
-      hipLaunchKernelGGL(k_apply<double>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const double *)zj, w, n);
+      stencil<double, 0>(c, top, (const double *)zj, (const double *)nullptr, w);
       const int nv = jj + 1;
       // first CGS pass
       hipLaunchKernelGGL(k_mdot, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, (const double *)V, (long)n, nv, n, c->part);
@@ -892,7 +972,7 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
     CV_CHECK(hipMemcpyAsync(hd, hh, jj * sizeof(double), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_xupdate, dim3(nblk(n)), dim3(BS), 0, c->stream, x, (const double *)Z, (long)n, (const double *)hd, jj, n);
     // true residual
-    hipLaunchKernelGGL(k_resid<double>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const double *)x, b, w, n);
+    stencil<double, 1>(c, top, (const double *)x, b, w);
     hipLaunchKernelGGL(k_sumsq, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, n, c->part);
     final_sum(c, c->part, CV_RED_GRID, hd + 40, 1);
     CV_CHECK(hipMemcpyAsync(hh, hd + 40, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1266,8 +1346,7 @@ extern "C" int cv_poisson_apply(cv_ctx *c, int level, const double *x, double *y
   if (level >= c->nlev || !x || !y) return CV_ERR_ARG;
   CvLevel &L = c->lev[level];
   CV_CHECK(hipMemcpyAsync(L.b, x, L.d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_apply<double>, dim3(nblk(L.d.n)), dim3(BS), 0, c->stream, opview<double>(c, level),
-                     (const double *)L.b, L.r, L.d.n);
+  stencil<double, 0>(c, level, (const double *)L.b, (const double *)nullptr, L.r);
   CV_CHECK(hipGetLastError());
   CV_CHECK(hipMemcpyAsync(y, L.r, L.d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CV_CHECK(hipStreamSynchronize(c->stream));

@@ -44,6 +44,7 @@ class UserCtx:
         self.h = h
         self.nodes = tuple(int(v) for v in nodes)
         self.mglevels = mglevels
+        self.semi = tuple(int(v) for v in semi)
         self.storage = tuple(v + 1 for v in self.nodes)
         self.opts = _lib.SolverOpts()
         L.cv_default_solver(C.byref(self.opts))
@@ -100,9 +101,15 @@ class UserCtx:
         return (mz, my, mx, 3) if name in _lib.VEC_FIELDS else (mz, my, mx)
 
     def level_storage(self, level):
+        """Storage extents of MG level ``level`` (MGDACreate, init.c:78-99):
+        a coarsened axis keeps (nodes + 1) // 2 nodes, a semi-coarsened one
+        keeps its size."""
         im, jm, km = self.nodes
+        si, sj, sk = self.semi
         for _ in range(self.mglevels - 1 - level):
-            im, jm, km = (im + 1) // 2, (jm + 1) // 2, (km + 1) // 2
+            im = im if si else (im + 1) // 2
+            jm = jm if sj else (jm + 1) // 2
+            km = km if sk else (km + 1) // 2
         return im + 1, jm + 1, km + 1
 
     def get(self, name, level=-1):
@@ -172,6 +179,21 @@ class UserCtx:
         if not (rc == _lib.CV_OK or (allow_unconverged and rc == _lib.CV_ERR_NOT_CONVERGED)):
             check(rc, "Flow_Solver")
         return st.as_dict()
+
+    # -- instrumentation ---------------------------------------------------
+    def prof_select(self, kernel, level=-1, precision=-1):
+        """Time every launch of one kernel class (``_lib.CV_PROF`` name) on
+        one level with device events on the context stream."""
+        k = _lib.CV_PROF[kernel] if isinstance(kernel, str) else int(kernel)
+        check(_lib.load().cv_prof_select(self.h, k, int(level), int(precision)), "cv_prof_select")
+
+    def prof_read(self, reset=False):
+        """(launches, device ms summed, algorithmic bytes summed)."""
+        n = C.c_long()
+        ms = C.c_double()
+        by = C.c_double()
+        check(_lib.load().cv_prof_read(self.h, C.byref(n), C.byref(ms), C.byref(by), int(reset)), "cv_prof_read")
+        return n.value, ms.value, by.value
 
     # operator plug-in points (tests)
     def MyRestriction(self, coarse_level, xf):

@@ -132,6 +132,24 @@ int cv_mg_restrict(cv_ctx *ctx, int coarse_level, const double *xf, double *xc);
 int cv_mg_prolong(cv_ctx *ctx, int fine_level, const double *xc, double *xf);     /* MyInterpolation poisson.c:265-370 */
 int cv_nullspace_project(cv_ctx *ctx, int level, double *x);                      /* PoissonNullSpaceFunction poisson.c:372-563 */
 
+/* ---- instrumentation (no reference counterpart; the reference's only
+ * timing hook is the per-phase CPU_TIME log, solvers.c:1130-1142) ----
+ * Device-event timing of every launch of one kernel class on one level,
+ * on the context's stream, while the context runs normally.  level -1 =
+ * finest; precision -1 = either, 0 = FP64, 1 = FP32.  cv_prof_read returns
+ * the launch count, their summed device time in ms and the summed
+ * algorithmic bytes (compulsory HBM traffic of the launch's inputs and
+ * outputs); reset clears the counters. */
+#define CV_PROF_NONE   0
+#define CV_PROF_RESID  1   /* 19-point operator residual r = b - A x       */
+#define CV_PROF_APPLY  2   /* 19-point operator apply y = A x (Krylov)     */
+#define CV_PROF_LINE   3   /* y-line solve of the line smoother            */
+#define CV_PROF_RCT    4   /* momentum cell kernel (Rc -> rct)             */
+#define CV_PROF_FF1    5   /* momentum face kernel (rct, p -> Rhs)         */
+#define CV_PROF_ORTH   6   /* Gram-Schmidt dot products                    */
+int cv_prof_select(cv_ctx *ctx, int kernel, int level, int precision);
+int cv_prof_read(cv_ctx *ctx, long *launches, double *total_ms, double *bytes, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,57 @@
+"""C ABI: the gfx950 library loads (no GPU needed) and exports every entry
+point include/curvib_hip.h declares; the Python binding lists the same set.
+No compute call is made here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "curvib_hip.h")
+SO = os.path.join(ROOT, "curvib_old_amd", "libcurvib_hip.so")
+
+
+def declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void|long)\s+(cv_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for must in ("cv_ctx_create", "cv_form_function1", "cv_poisson_solve", "cv_flow_step",
+                 "cv_projection", "cv_divergence", "cv_update_pressure", "cv_poisson_rhs"):
+        assert must in names
+
+
+@pytest.mark.skipif(not os.path.exists(SO), reason="library not built")
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(SO)
+    missing = [s for s in declared() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_binding_symbol_list_matches_header():
+    from curvib_old_amd import _lib
+    assert sorted(_lib.SYMBOLS) == declared()
+
+
+@pytest.mark.skipif(not os.path.exists(SO), reason="library not built")
+def test_error_paths_without_gpu():
+    """Argument errors are reported before any device call."""
+    from curvib_old_amd import _lib
+    L = _lib.load()
+    out = ctypes.c_void_p()
+    assert L.cv_ctx_create(None, 0, ctypes.byref(out)) == _lib.CV_ERR_ARG
+    d = _lib.GridDesc()
+    d.IM = d.JM = d.KM = 9
+    for q, b in enumerate((7, 7, 7, 7, 7, 7)):       # j-periodic is rejected
+        d.bctype[q] = b
+    d.mglevels, d.ren, d.dt, d.st = 2, 100.0, 1e-3, 1.0
+    assert L.cv_ctx_create(ctypes.byref(d), 0, ctypes.byref(out)) == _lib.CV_ERR_ARG
+    assert "unsupported" in _lib.last_error()
+    assert L.cv_flow_step(None, None) == _lib.CV_ERR_ARG
+    o = _lib.SolverOpts()
+    L.cv_default_solver(ctypes.byref(o))
+    assert o.restart == 30 and o.precision == _lib.CV_FP64
