@@ -191,16 +191,21 @@ __global__ void __launch_bounds__(BS) k_jacobi(OpV<T> A, const T *__restrict__ x
 // cells along i (lane L <-> cell i0-1+L; lane 0 only supplies the flux of
 // face i0-1) and ST_TY rows along j, and marches k over one chunk.  The x
 // planes k-1, k, k+1 of the tile (+1 halo, periodic wrap applied at load)
-// live in LDS; every face flux is formed once: the lower i-face comes from
-// the neighbouring lane (shuffle), the lower j-face from the row below
-// (LDS), the lower k-face from the previous plane (register).  The
-// arithmetic per face and per row is the same as face_flux/apply_row_x.
+// live in LDS (four rotating slots); every face flux is formed once: the
+// lower i-face comes from the neighbouring lane (shuffle), the lower j-face
+// from the row below (LDS, double buffered), the lower k-face from the
+// previous plane (register).  One barrier per plane; the x plane two
+// ahead and the per-cell coefficients one plane ahead are loaded into
+// registers before the current plane is computed.  The arithmetic per face
+// and per row is that of face_flux/apply_row_x.
 //   MODE 0: y = A x      MODE 1: y = b - A x
 //   MODE 2: y = x + omega * dinv * (b - A x)   (damped Jacobi sweep)
 // ------------------------------------------------------------------
 #define ST_TY 8
 #define ST_TC 66
 #define ST_TR (ST_TY + 2)
+#define ST_NT (ST_TR * ST_TC)
+#define ST_THREADS (64 * ST_TY)
 
 // one face flux from its two cells (xl, xu) and the four neighbours of
 // each tangential axis: (+f, +e+f, -f, +e-f)
@@ -218,40 +223,45 @@ __device__ inline T tile_flux(T xl, T xu, T gdd, uint32_t w, int sh, T g0, T a0,
   return F;
 }
 
-// k-face flux between planes P (k) and Q (k+1) at LDS column c, row r
+// k-face flux between planes P (k) and Q (k+1) at LDS column c, row r;
+// g6, g7, g8 of the face
 template <class T>
-__device__ inline T tile_flux_k(const OpV<T> &A, const T *P, const T *Q, int c, int r, uint32_t w, long t) {
+__device__ inline T tile_flux_k(const T *P, const T *Q, int c, int r, uint32_t w, T g6, T g7, T g8) {
   if (!(w & (1u << 11))) return (T)0;
   const int o = r * ST_TC + c;
-  return tile_flux<T>(P[o], Q[o], A.g[8][t], w, 12,
-                      A.g[6][t], P[o + 1], Q[o + 1], P[o - 1], Q[o - 1],
-                      A.g[7][t], P[o + ST_TC], Q[o + ST_TC], P[o - ST_TC], Q[o - ST_TC]);
+  return tile_flux<T>(P[o], Q[o], g8, w, 12, g6, P[o + 1], Q[o + 1], P[o - 1], Q[o - 1],
+                      g7, P[o + ST_TC], Q[o + ST_TC], P[o - ST_TC], Q[o - ST_TC]);
 }
 // j-face flux between rows r and r+1 of plane P (M = k-1, Q = k+1)
 template <class T>
-__device__ inline T tile_flux_j(const OpV<T> &A, const T *M, const T *P, const T *Q, int c, int r, uint32_t w, long t) {
+__device__ inline T tile_flux_j(const T *M, const T *P, const T *Q, int c, int r, uint32_t w, T g3, T g4, T g5) {
   if (!(w & (1u << 6))) return (T)0;
   const int o = r * ST_TC + c, u = o + ST_TC;
-  return tile_flux<T>(P[o], P[u], A.g[4][t], w, 7,
-                      A.g[3][t], P[o + 1], P[u + 1], P[o - 1], P[u - 1],
-                      A.g[5][t], Q[o], Q[u], M[o], M[u]);
+  return tile_flux<T>(P[o], P[u], g4, w, 7, g3, P[o + 1], P[u + 1], P[o - 1], P[u - 1],
+                      g5, Q[o], Q[u], M[o], M[u]);
 }
 // i-face flux between columns c and c+1 of plane P
 template <class T>
-__device__ inline T tile_flux_i(const OpV<T> &A, const T *M, const T *P, const T *Q, int c, int r, uint32_t w, long t) {
+__device__ inline T tile_flux_i(const T *M, const T *P, const T *Q, int c, int r, uint32_t w, T g0, T g1, T g2) {
   if (!(w & (1u << 1))) return (T)0;
   const int o = r * ST_TC + c;
-  return tile_flux<T>(P[o], P[o + 1], A.g[0][t], w, 2,
-                      A.g[1][t], P[o + ST_TC], P[o + ST_TC + 1], P[o - ST_TC], P[o - ST_TC + 1],
-                      A.g[2][t], Q[o], Q[o + 1], M[o], M[o + 1]);
+  return tile_flux<T>(P[o], P[o + 1], g0, w, 2, g1, P[o + ST_TC], P[o + ST_TC + 1], P[o - ST_TC], P[o - ST_TC + 1],
+                      g2, Q[o], Q[o + 1], M[o], M[o + 1]);
 }
 
+// per-cell inputs of one plane, loaded a plane ahead
+template <class T>
+struct StCell {
+  uint32_t w, wb;        // operator words of the cell and (row 0) of the cell below the tile
+  T g[9], gb[3], aj, b, dv;
+};
+
 template <class T, int MODE>
-__global__ void __launch_bounds__(64 * ST_TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
+__global__ void __launch_bounds__(ST_THREADS) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
                                                       T *__restrict__ y, const T *__restrict__ dinv, T omega,
                                                       int kchunk) {
-  __shared__ T X[3][ST_TR * ST_TC];
-  __shared__ T FJ[ST_TY][64];
+  __shared__ T X[4][ST_NT];
+  __shared__ T FJ[2][ST_TY][64];
   const int L = threadIdx.x, R = threadIdx.y;
   const int tid = R * 64 + L;
   const int mx = A.mx, my = A.my, mz = A.mz;
@@ -259,73 +269,118 @@ __global__ void __launch_bounds__(64 * ST_TY) k_stencil(OpV<T> A, const T *__res
   const int k0 = kchunk * blockIdx.z, k1 = min(k0 + kchunk, mz);
   const int gi = i0 - 1 + L, gj = j0 + R;
   const bool in_i = gi >= 0 && gi < mx, in_j = gj < my;
-  const bool out = L >= 1 && in_i && in_j;
+  const bool cell = in_i && in_j;
+  const bool out = L >= 1 && cell;
+  const bool row0 = out && R == 0 && gj >= 1;
   const long pl = (long)mx * my;
   const int c = L + 1, r = R + 1;
+  const long tij = (long)gi + (long)mx * gj;
 
-  // fill plane gk (wrapped) into slot s
-  auto load_plane = [&](int gk, int s) {
+  // this thread's share of a tile plane: entries e0 = tid and e1 = tid + ST_THREADS
+  const int e1 = tid + ST_THREADS;
+  long off0 = 0, off1 = 0;
+  bool ok0, ok1;
+  {
+    int rr = tid / ST_TC, cc = tid - rr * ST_TC;
+    int ii = i0 - 2 + cc, jj = j0 - 1 + rr;
+    ok0 = ii >= 0 && ii < mx && jj >= 0 && jj < my;
+    if (ok0) off0 = (long)mx * jj + cv_wrap(ii, mx, A.xper);
+    rr = e1 / ST_TC; cc = e1 - rr * ST_TC;
+    ii = i0 - 2 + cc; jj = j0 - 1 + rr;
+    ok1 = e1 < ST_NT && ii >= 0 && ii < mx && jj >= 0 && jj < my;
+    if (ok1) off1 = (long)mx * jj + cv_wrap(ii, mx, A.xper);
+  }
+  auto fetch_x = [&](int gk, T &v0, T &v1) {
     const bool kin = gk >= 0 && gk < mz;
     const long kb = kin ? pl * cv_wrap(gk, mz, A.zper) : 0;
-    for (int e = tid; e < ST_TR * ST_TC; e += 64 * ST_TY) {
-      const int rr = e / ST_TC, cc = e - rr * ST_TC;
-      const int ii = i0 - 2 + cc, jj = j0 - 1 + rr;
-      T v = (T)0;
-      if (kin && ii >= 0 && ii < mx && jj >= 0 && jj < my) v = x[kb + (long)mx * jj + cv_wrap(ii, mx, A.xper)];
-      X[s][e] = v;
+    v0 = (kin && ok0) ? x[kb + off0] : (T)0;
+    v1 = (kin && ok1) ? x[kb + off1] : (T)0;
+  };
+  auto store_x = [&](int gk, T v0, T v1) {
+    T *S = X[(gk + 4) & 3];
+    S[tid] = v0;
+    if (e1 < ST_NT) S[e1] = v1;
+  };
+  auto fetch_cell = [&](int gk, StCell<T> &C) {
+    C.w = 0u; C.wb = 0u;
+#pragma unroll
+    for (int q = 0; q < 9; q++) C.g[q] = (T)0;
+    C.gb[0] = C.gb[1] = C.gb[2] = (T)0;
+    C.aj = C.b = C.dv = (T)0;
+    if (!cell) return;
+    const long t = tij + pl * gk;
+    C.w = A.fl[t];
+#pragma unroll
+    for (int q = 0; q < 9; q++) C.g[q] = A.g[q][t];
+    if (out) {
+      C.aj = A.aj[t];
+      if (MODE >= 1) C.b = b[t];
+      if (MODE == 2) C.dv = dinv[t];
+    }
+    if (row0) {
+      C.wb = A.fl[t - mx];
+      C.gb[0] = A.g[3][t - mx]; C.gb[1] = A.g[4][t - mx]; C.gb[2] = A.g[5][t - mx];
     }
   };
-  auto slot = [](int gk) { return (gk + 3) % 3; };
 
-  // prologue: flux of the k-face below the chunk
-  load_plane(k0 - 1, slot(k0 - 1));
-  load_plane(k0, slot(k0));
+  // prologue: planes k0-1 .. k0+1 in LDS, plane k0+2 and the cells of k0 in registers
+  T xa, xb;
+  fetch_x(k0 - 1, xa, xb); store_x(k0 - 1, xa, xb);
+  fetch_x(k0, xa, xb); store_x(k0, xa, xb);
+  fetch_x(k0 + 1, xa, xb); store_x(k0 + 1, xa, xb);
+  T xn0, xn1;
+  fetch_x(k0 + 2, xn0, xn1);
+  StCell<T> cur;
+  fetch_cell(k0, cur);
   __syncthreads();
   T fk_lo = (T)0;
   if (out && k0 >= 1) {
-    const long t = (long)gi + (long)mx * gj + pl * (k0 - 1);
-    fk_lo = tile_flux_k<T>(A, X[slot(k0 - 1)], X[slot(k0)], c, r, A.fl[t], t);
+    const long t = tij + pl * (k0 - 1);
+    fk_lo = tile_flux_k<T>(X[(k0 - 1 + 4) & 3], X[(k0 + 4) & 3], c, r, A.fl[t], A.g[6][t], A.g[7][t], A.g[8][t]);
   }
   for (int gk = k0; gk < k1; gk++) {
-    load_plane(gk + 1, slot(gk + 1));
-    __syncthreads();
-    const T *Pm = X[slot(gk - 1)], *P = X[slot(gk)], *Q = X[slot(gk + 1)];
-    const long t = (long)gi + (long)mx * gj + pl * gk;
-    const uint32_t w = (in_i && in_j) ? A.fl[t] : 0u;
-    const T fi = (in_i && in_j) ? tile_flux_i<T>(A, Pm, P, Q, c, r, w, t) : (T)0;
+    T xq0, xq1;
+    fetch_x(gk + 3, xq0, xq1);
+    StCell<T> nxt;
+    if (gk + 1 < k1) fetch_cell(gk + 1, nxt);
+    const T *Pm = X[(gk - 1 + 4) & 3], *P = X[(gk + 4) & 3], *Q = X[(gk + 1 + 4) & 3];
+    const uint32_t w = cur.w;
+    const T fi = cell ? tile_flux_i<T>(Pm, P, Q, c, r, w, cur.g[0], cur.g[1], cur.g[2]) : (T)0;
     T fj = (T)0, fk = (T)0, fj_lo = (T)0;
     if (out) {
-      fj = tile_flux_j<T>(A, Pm, P, Q, c, r, w, t);
-      fk = tile_flux_k<T>(A, P, Q, c, r, w, t);
-      if (R == 0 && gj >= 1) {
-        const long tb = t - mx;
-        fj_lo = tile_flux_j<T>(A, Pm, P, Q, c, r - 1, A.fl[tb], tb);
-      }
+      fj = tile_flux_j<T>(Pm, P, Q, c, r, w, cur.g[3], cur.g[4], cur.g[5]);
+      fk = tile_flux_k<T>(P, Q, c, r, w, cur.g[6], cur.g[7], cur.g[8]);
+      if (row0) fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, cur.wb, cur.gb[0], cur.gb[1], cur.gb[2]);
     }
-    FJ[R][L] = fj;
+    const T xc = P[r * ST_TC + c];
+    FJ[gk & 1][R][L] = fj;
+    store_x(gk + 2, xn0, xn1);
     const T fi_lo = __shfl_up(fi, 1, 64);
     __syncthreads();
-    if (R > 0) fj_lo = FJ[R - 1][L];
+    if (R > 0) fj_lo = FJ[gk & 1][R - 1][L];
     if (out) {
+      const long t = tij + pl * gk;
       T res;
       if (!(w & 1u)) {
         const T xv = x[t];
         if (MODE == 0) res = xv;
-        else if (MODE == 1) res = b[t] - xv;
-        else res = xv + omega * dinv[t] * (b[t] - xv);
+        else if (MODE == 1) res = cur.b - xv;
+        else res = xv + omega * cur.dv * (cur.b - xv);
       } else {
         T acc = 0;
         acc += fi; acc -= fi_lo;
         acc += fj; acc -= fj_lo;
         acc += fk; acc -= fk_lo;
-        const T ax = -A.aj[t] * acc;
+        const T ax = -cur.aj * acc;
         if (MODE == 0) res = ax;
-        else if (MODE == 1) res = b[t] - ax;
-        else res = P[r * ST_TC + c] + omega * dinv[t] * (b[t] - ax);
+        else if (MODE == 1) res = cur.b - ax;
+        else res = xc + omega * cur.dv * (cur.b - ax);
       }
       y[t] = res;
     }
     fk_lo = fk;
+    cur = nxt;
+    xn0 = xq0; xn1 = xq1;
   }
 }
 
@@ -345,8 +400,9 @@ __global__ void k_linecoef(OpV<double> A, double *lo, double *dg, double *up, lo
 }
 
 // Thomas factors along j=1..my-2 of every (i,k) line: ip = 1/pivot,
-// cp = up/pivot; rows j=0 and my-1 are identity rows
-__global__ void k_linefactor(int mx, int my, int mz, const double *lo, const double *dg, const double *up,
+// cp = up/pivot, and lo replaced by a = -lo/pivot; rows j=0 and my-1 are
+// identity rows
+__global__ void k_linefactor(int mx, int my, int mz, double *lo, const double *dg, const double *up,
                              double *ip, double *cp, double *dinv) {
   const long nl = (long)mx * mz;
   for (long ln = blockIdx.x * (long)blockDim.x + threadIdx.x; ln < nl; ln += (long)gridDim.x * blockDim.x) {
@@ -358,6 +414,7 @@ __global__ void k_linefactor(int mx, int my, int mz, const double *lo, const dou
       double piv = dg[t] - lo[t] * cprev;
       double r = piv != 0.0 ? 1.0 / piv : 1.0;
       ip[t] = r;
+      lo[t] = -lo[t] * r;      // stored as the forward multiplier a = -lo/pivot
       cprev = up[t] * r;
       cp[t] = cprev;
       dinv[t] = dg[t] != 0.0 ? 1.0 / dg[t] : 1.0;
@@ -367,7 +424,8 @@ __global__ void k_linefactor(int mx, int my, int mz, const double *lo, const dou
   }
 }
 
-// chunked line solve: block (64 lines in i) x (YL_NC chunks along j); the
+// chunked line solve for lines longer than the register-resident kernel
+// takes (lo holds a = -lo/pivot): block (64 lines in i) x (YL_NC chunks along j); the
 // two recurrences of the factored Thomas solve (forward y_j = b_j + a_j
 // y_{j-1}, backward e_j = y_j - cp_j e_{j+1}) are composed per chunk as
 // affine maps, the chunk carries resolved through LDS, then each chunk is
@@ -392,7 +450,7 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
   if (act)
     for (int j = j0; j < j1; j++) {
       const long t = base + sj * j;
-      const T a = -lo[t] * ip[t], bb = r[t] * ip[t];
+      const T a = lo[t], bb = r[t] * ip[t];
       Aa = a * Aa;
       Bb = a * Bb + bb;
     }
@@ -403,7 +461,7 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
   if (act)
     for (int j = j0; j < j1; j++) {
       const long t = base + sj * j;
-      y = r[t] * ip[t] - lo[t] * ip[t] * y;
+      y = r[t] * ip[t] + lo[t] * y;
       tmp[t] = y;
     }
   __syncthreads();
@@ -429,6 +487,71 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
     if (tc == 0) xo[base] = zero_guess ? (T)0 : xi[base];
     if (tc == YL_NC - 1) xo[base + sj * (my - 1)] = zero_guess ? (T)0 : xi[base + sj * (my - 1)];
   }
+}
+
+// register-resident line solve: block 64 lanes (i) x NC chunks along j,
+// each thread keeps the CS rows of its chunk in registers, so r, the line
+// factors and x are each read once and x written once.  Forward recurrence
+// y_j = r_j ip_j + a_j y_{j-1} (a = -lo ip, stored), backward
+// e_j = y_j - cp_j e_{j+1}; chunk carries resolved through LDS.
+template <class T, int CS>
+__global__ void __launch_bounds__(1024) k_yline_reg(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
+                                                   const T *__restrict__ ip, const T *__restrict__ cp,
+                                                   const T *xi, T *xo, T omega, int zero_guess) {
+  __shared__ T SA[16][64], SB[16][64];
+  const int ti = threadIdx.x, tc = threadIdx.y, nc = blockDim.y;
+  const int i = blockIdx.x * 64 + ti, k = blockIdx.y;
+  const bool act = i < mx;
+  const long sj = mx;
+  const long base = (long)i + (long)mx * my * k;
+  const int j0 = 1 + tc * CS;
+  const int nrow = act ? max(0, min(CS, my - 1 - j0)) : 0;
+  T av[CS], yv[CS];
+  // forward composite of the chunk
+  T Aa = 1, Bb = 0;
+#pragma unroll
+  for (int q = 0; q < CS; q++) {
+    if (q < nrow) {
+      const long t = base + sj * (j0 + q);
+      av[q] = ya[t];
+      yv[q] = r[t] * ip[t];
+      Aa = av[q] * Aa;
+      Bb = av[q] * Bb + yv[q];
+    }
+  }
+  SA[tc][ti] = Aa; SB[tc][ti] = Bb;
+  __syncthreads();
+  T y = 0;
+  for (int q = 0; q < tc; q++) y = SA[q][ti] * y + SB[q][ti];
+#pragma unroll
+  for (int q = 0; q < CS; q++)
+    if (q < nrow) { y = yv[q] + av[q] * y; yv[q] = y; }
+  __syncthreads();
+  // backward composite, from the chunk's top row down
+  Aa = 1; Bb = 0;
+#pragma unroll
+  for (int q = CS - 1; q >= 0; q--) {
+    if (q < nrow) {
+      av[q] = -cp[base + sj * (j0 + q)];
+      Aa = av[q] * Aa;
+      Bb = av[q] * Bb + yv[q];
+    }
+  }
+  SA[tc][ti] = Aa; SB[tc][ti] = Bb;
+  __syncthreads();
+  T e = 0;
+  for (int q = nc - 1; q > tc; q--) e = SA[q][ti] * e + SB[q][ti];
+  if (!act) return;
+#pragma unroll
+  for (int q = CS - 1; q >= 0; q--) {
+    if (q < nrow) {
+      const long t = base + sj * (j0 + q);
+      e = yv[q] + av[q] * e;
+      xo[t] = (zero_guess ? (T)0 : xi[t]) + omega * e;
+    }
+  }
+  if (tc == 0) xo[base] = zero_guess ? (T)0 : xi[base];
+  if (tc == nc - 1) xo[base + sj * (my - 1)] = zero_guess ? (T)0 : xi[base + sj * (my - 1)];
 }
 
 // ------------------------------------------------------------------
@@ -539,44 +662,55 @@ __global__ void k_fluid_shift(T *__restrict__ x, const uint32_t *__restrict__ fl
 // Krylov kernels
 // ------------------------------------------------------------------
 #define MAXV 33
+// part[q*gridDim + b] = sum_t w[t]*V[q*stride+t] for q < NV; q == NV -> ||w||^2
+template <int NV>
 __global__ void __launch_bounds__(BS) k_mdot(const double *__restrict__ w, const double *__restrict__ V, long stride,
-                                             int nv, long n, double *part) {
-  // part[q*gridDim + b] = sum_t w[t]*V[q*stride+t] for q<nv; q==nv -> ||w||^2
+                                             long n, double *part) {
   __shared__ double sh[BS / 64];
-  double acc[MAXV];
+  double acc[NV + 1];
 #pragma unroll
-  for (int q = 0; q < MAXV; q++) acc[q] = 0.0;
+  for (int q = 0; q <= NV; q++) acc[q] = 0.0;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     const double wt = w[t];
 #pragma unroll
-    for (int q = 0; q < MAXV - 1; q++)
-      if (q < nv) acc[q] += wt * V[q * stride + t];
-    acc[MAXV - 1] += wt * wt;
+    for (int q = 0; q < NV; q++) acc[q] += wt * V[q * stride + t];
+    acc[NV] += wt * wt;
   }
-  for (int q = 0; q <= nv; q++) {
-    double v = (q < nv) ? acc[q] : acc[MAXV - 1];
-    double s = cv_block_sum(v, sh);
+#pragma unroll
+  for (int q = 0; q <= NV; q++) {
+    const double s = cv_block_sum(acc[q], sh);
     if (threadIdx.x == 0) part[(long)q * gridDim.x + blockIdx.x] = s;
   }
 }
 
 // w -= sum_q h[q] V_q ; part[b] = partial ||w||^2
+template <int NV>
 __global__ void __launch_bounds__(BS) k_orth_update(double *__restrict__ w, const double *__restrict__ V, long stride,
-                                                    const double *__restrict__ h, int nv, long n, double *part) {
+                                                    const double *__restrict__ h, long n, double *part) {
   __shared__ double sh[BS / 64];
-  __shared__ double hs[MAXV];
-  if (threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
-  __syncthreads();
+  double hq[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) hq[q] = h[q];
   double s = 0.0;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     double v = w[t];
-    for (int q = 0; q < nv; q++) v -= hs[q] * V[q * stride + t];
+#pragma unroll
+    for (int q = 0; q < NV; q++) v -= hq[q] * V[q * stride + t];
     w[t] = v;
     s += v * v;
   }
   s = cv_block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
+
+typedef void (*MdotFn)(const double *, const double *, long, long, double *);
+typedef void (*OrthFn)(double *, const double *, long, const double *, long, double *);
+#define CV_NV_LIST(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+  X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
+#define CV_MDOT_ENTRY(N) k_mdot<N>,
+#define CV_ORTH_ENTRY(N) k_orth_update<N>,
+static const MdotFn mdot_tbl[] = {CV_NV_LIST(CV_MDOT_ENTRY)};
+static const OrthFn orth_tbl[] = {CV_NV_LIST(CV_ORTH_ENTRY)};
 
 __global__ void k_scale_to(double *__restrict__ dst, const double *__restrict__ src, const double *inv, long n) {
   const double a = inv[0];
@@ -717,7 +851,7 @@ int cv_poisson_prepare(cv_ctx *c) {
     // line coefficients: lo -> ylo, diag -> r (scratch), up -> b (scratch)
     hipLaunchKernelGGL(k_linecoef, dim3(nblk(n)), dim3(BS), 0, c->stream, opview<double>(c, l), L.ylo, L.r, L.b, n);
     hipLaunchKernelGGL(k_linefactor, dim3(nblk((long)L.d.mx * L.d.mz)), dim3(BS), 0, c->stream, L.d.mx, L.d.my, L.d.mz,
-                       (const double *)L.ylo, (const double *)L.r, (const double *)L.b, L.yip, L.ycp, L.dinv);
+                       L.ylo, (const double *)L.r, (const double *)L.b, L.yip, L.ycp, L.dinv);
     hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.ylof, (const double *)L.ylo, n);
     hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.yipf, (const double *)L.yip, n);
     hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.ycpf, (const double *)L.ycp, n);
@@ -786,6 +920,35 @@ double *line_tmp<double>(cv_ctx *c, int l) { return c->lev[l].ltmp; }
 template <>
 float *line_tmp<float>(cv_ctx *c, int l) { return c->lev[l].ltmpf; }
 
+// xo = xi + omega * T^{-1} r over every y-line of level l (T: the
+// j-tridiagonal part of the operator); tmp: scratch of the long-line kernel
+template <class T>
+static void line_solve(cv_ctx *c, int l, const T *r, const T *xi, T *xo, T *tmp, T om, bool zg) {
+  CvLevel &L = c->lev[l];
+  OpV<T> A = opview<T>(c, l);
+  const int rows = L.d.my - 2;
+  const dim3 g((unsigned)((L.d.mx + 63) / 64), (unsigned)L.d.mz);
+  CvProfScope ps(c, CV_PROF_LINE, l, sizeof(T) == 4 ? 1 : 0, (double)L.d.n * 6.0 * sizeof(T));
+  // FP64 keeps at most 16 rows per thread (32 would spill at 1024 threads)
+  const int csmax = sizeof(T) == 8 ? 16 : 32;
+  if (rows <= 16 * csmax) {
+    const int cs = rows <= 16 * 8 ? 8 : rows <= 16 * 16 ? 16 : 32;
+    const dim3 b(64, (unsigned)((rows + cs - 1) / cs));
+    if (cs == 8)
+      hipLaunchKernelGGL((k_yline_reg<T, 8>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip, A.ycp, xi,
+                         xo, om, zg ? 1 : 0);
+    else if (cs == 16)
+      hipLaunchKernelGGL((k_yline_reg<T, 16>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip, A.ycp, xi,
+                         xo, om, zg ? 1 : 0);
+    else
+      hipLaunchKernelGGL((k_yline_reg<T, sizeof(T) == 8 ? 16 : 32>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r,
+                         A.ylo, A.yip, A.ycp, xi, xo, om, zg ? 1 : 0);
+  } else {
+    hipLaunchKernelGGL(k_yline_solve<T>, g, dim3(64, YL_NC), 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
+                       A.ycp, xi, xo, tmp, om, zg ? 1 : 0);
+  }
+}
+
 // nsweep Jacobi sweeps on level l; x in v.x[l] (zero_guess: x is 0), r used as ping-pong
 template <class T>
 static void smooth(cv_ctx *c, MgVecs<T> &v, int l, int nsweep, bool zero_guess) {
@@ -806,9 +969,7 @@ static void smooth(cv_ctx *c, MgVecs<T> &v, int l, int nsweep, bool zero_guess) 
         stencil<T, 1>(c, l, (const T *)v.x[l], (const T *)v.b[l], tmpv);
         rr = tmpv;
       }
-      dim3 gb((unsigned)((L.d.mx + 63) / 64), (unsigned)L.d.mz), bb(64, YL_NC);
-      hipLaunchKernelGGL(k_yline_solve<T>, gb, bb, 0, c->stream, L.d.mx, L.d.my, L.d.mz, rr, A.ylo, A.yip, A.ycp,
-                         (const T *)v.x[l], v.r[l], tmpv, om, zg ? 1 : 0);
+      line_solve<T>(c, l, rr, (const T *)v.x[l], v.r[l], tmpv, om, zg);
     } else {
       if (zero_guess && s == 0)
         hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, A, (const T *)v.x[l], (const T *)v.b[l],
@@ -878,6 +1039,20 @@ static int ensure_krylov(cv_ctx *c) {
   return CV_OK;
 }
 
+// one classical Gram-Schmidt pass of w against V_0..V_{nv-1}: h and ||w||^2
+// to dsc[0..nv], then w -= V h and ||w||^2 to dsc[36]
+static void gram_schmidt_pass(cv_ctx *c, double *w, const double *V, int nv, long n) {
+  double *hd = c->dsc;
+  {
+    CvProfScope ps(c, CV_PROF_ORTH, c->nlev - 1, 0, (double)n * 8.0 * (nv + 1));
+    hipLaunchKernelGGL(mdot_tbl[nv - 1], dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, V, n, n, c->part);
+  }
+  final_sum(c, c->part, CV_RED_GRID, hd, nv + 1);
+  hipLaunchKernelGGL(orth_tbl[nv - 1], dim3(CV_RED_GRID), dim3(BS), 0, c->stream, w, V, n, (const double *)hd, n,
+                     c->part);
+  final_sum(c, c->part, CV_RED_GRID, hd + 36, 1);
+}
+
 // FGMRES(m) for A phi = b on the finest level; phi starts at 0
 static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *rnorm_out, double *bnorm_out) {
   const int top = c->nlev - 1;
@@ -917,21 +1092,13 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
       stencil<double, 0>(c, top, (const double *)zj, (const double *)nullptr, w);
       const int nv = jj + 1;
       // first CGS pass
-      hipLaunchKernelGGL(k_mdot, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, (const double *)V, (long)n, nv, n, c->part);
-      final_sum(c, c->part, CV_RED_GRID, hd, nv + 1);            // hd[0..nv-1] = h, hd[nv] = ||w||^2
-      hipLaunchKernelGGL(k_orth_update, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, w, (const double *)V, (long)n,
-                         (const double *)hd, nv, n, c->part);
-      final_sum(c, c->part, CV_RED_GRID, hd + 36, 1);            // ||w_new||^2
+      gram_schmidt_pass(c, w, V, nv, n);
       CV_CHECK(hipMemcpyAsync(hh, hd, 40 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       CV_CHECK(hipStreamSynchronize(c->stream));
       double wn2 = hh[nv], hn2 = hh[36];
       for (int q = 0; q < nv; q++) H[q * m + jj] = hh[q];
-      if (hn2 < 0.25 * wn2) {      // DGKS: second classical pass
-        hipLaunchKernelGGL(k_mdot, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, (const double *)V, (long)n, nv, n, c->part);
-        final_sum(c, c->part, CV_RED_GRID, hd, nv + 1);
-        hipLaunchKernelGGL(k_orth_update, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, w, (const double *)V, (long)n,
-                           (const double *)hd, nv, n, c->part);
-        final_sum(c, c->part, CV_RED_GRID, hd + 36, 1);
+      if (hn2 < 1e-2 * wn2) {      // DGKS: second classical pass when ||w|| fell below 0.1 of itself
+        gram_schmidt_pass(c, w, V, nv, n);
         CV_CHECK(hipMemcpyAsync(hh, hd, 40 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         CV_CHECK(hipStreamSynchronize(c->stream));
         for (int q = 0; q < nv; q++) H[q * m + jj] += hh[q];
