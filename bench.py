@@ -27,7 +27,7 @@ the rocprofv3 FETCH_SIZE/WRITE_SIZE summary under profiles/, when present.
 ``cpu_baseline``: the C oracle (the reference algorithm restated: assembled
 19-point CSR operator, FGMRES(30) with a 3-level V-cycle and block-Jacobi
 ILU(0) Richardson smoothing, RK momentum) timed for one fractional step of
-the same workload at 192x96x96 cells, scaled by cell count.
+the same workload at 256x128x128 cells, scaled by cell count.
 """
 from __future__ import annotations
 
@@ -180,7 +180,7 @@ def main():
     ap.add_argument("--cells", default="512x256x256")
     ap.add_argument("--levels", type=int, default=7)
     ap.add_argument("--semi", default="0,1,0")
-    ap.add_argument("--cpu-sample", default="192x96x96")
+    ap.add_argument("--cpu-sample", default="256x128x128")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")

@@ -141,3 +141,21 @@ int cv_rk_dev(cv_ctx *c);
 __device__ inline int cv_wrap(int i, int m, int per) {
   return per ? (i == m - 1 ? 1 : (i == 0 ? m - 2 : i)) : i;
 }
+
+// XCD-aware tile order.  Work-groups are handed to the 8 XCDs of the chip
+// round-robin by linear id; this maps the blocks one XCD receives onto a
+// contiguous run of tiles (x fastest), so tiles that share halo lines share
+// that XCD's L2.  A bijection on [0, gridDim.x*gridDim.y*gridDim.z).
+#define CV_NXCD 8
+__device__ inline void cv_xcd_tile(int &tx, int &ty, int &tz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned nb = gx * gy * gridDim.z;
+  const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned q = nb / CV_NXCD, rem = nb % CV_NXCD;
+  const unsigned x = b % CV_NXCD, l = b / CV_NXCD;
+  const unsigned t = (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + l;
+  tx = (int)(t % gx);
+  ty = (int)((t / gx) % gy);
+  tz = (int)(t / (gx * gy));
+}
+

@@ -187,9 +187,9 @@ __global__ void __launch_bounds__(BS) k_jacobi(OpV<T> A, const T *__restrict__ x
 }
 
 // ------------------------------------------------------------------
-// 2.5D-tiled flux-form stencil.  A block of 64 x ST_TY threads owns 63
+// 2.5D-tiled flux-form stencil.  A block of 64 x TY threads owns 63
 // cells along i (lane L <-> cell i0-1+L; lane 0 only supplies the flux of
-// face i0-1) and ST_TY rows along j, and marches k over one chunk.  The x
+// face i0-1) and TY rows along j, and marches k over one chunk.  The x
 // planes k-1, k, k+1 of the tile (+1 halo, periodic wrap applied at load)
 // live in LDS (four rotating slots); every face flux is formed once: the
 // lower i-face comes from the neighbouring lane (shuffle), the lower j-face
@@ -201,11 +201,7 @@ __global__ void __launch_bounds__(BS) k_jacobi(OpV<T> A, const T *__restrict__ x
 //   MODE 0: y = A x      MODE 1: y = b - A x
 //   MODE 2: y = x + omega * dinv * (b - A x)   (damped Jacobi sweep)
 // ------------------------------------------------------------------
-#define ST_TY 8
 #define ST_TC 66
-#define ST_TR (ST_TY + 2)
-#define ST_NT (ST_TR * ST_TC)
-#define ST_THREADS (64 * ST_TY)
 
 // one face flux from its two cells (xl, xu) and the four neighbours of
 // each tangential axis: (+f, +e+f, -f, +e-f)
@@ -249,24 +245,33 @@ __device__ inline T tile_flux_i(const T *M, const T *P, const T *Q, int c, int r
                       g2, Q[o], Q[o + 1], M[o], M[o + 1]);
 }
 
-// per-cell inputs of one plane, loaded a plane ahead
+// operator word and face coefficients of one cell, loaded a plane ahead;
+// in FP32 (where registers allow it) also aj, b and dinv
 template <class T>
 struct StCell {
   uint32_t w, wb;        // operator words of the cell and (row 0) of the cell below the tile
-  T g[9], gb[3], aj, b, dv;
+  T g[9], gb[3];
+  T aj, b, dv;
 };
 
-template <class T, int MODE>
-__global__ void __launch_bounds__(ST_THREADS) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
-                                                      T *__restrict__ y, const T *__restrict__ dinv, T omega,
-                                                      int kchunk) {
-  __shared__ T X[4][ST_NT];
-  __shared__ T FJ[2][ST_TY][64];
+// TY: rows per block (block 64 x TY); PFB: prefetch the row-0 face below
+// the tile with the cells (keeps the row-0 wave off the barrier's critical
+// path at the cost of registers); remap: XCD-aware tile order
+template <class T, int MODE, int TY, bool PFB>
+__global__ void __launch_bounds__(64 * TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
+                                                   T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk,
+                                                   int remap) {
+  constexpr int TR = TY + 2, NT = TR * ST_TC, NTH = 64 * TY;
+  static_assert(2 * NTH >= NT, "two tile entries per thread must cover a plane");
+  __shared__ T X[4][NT];
+  __shared__ T FJ[2][TY][64];
   const int L = threadIdx.x, R = threadIdx.y;
   const int tid = R * 64 + L;
   const int mx = A.mx, my = A.my, mz = A.mz;
-  const int i0 = 63 * blockIdx.x, j0 = ST_TY * blockIdx.y;
-  const int k0 = kchunk * blockIdx.z, k1 = min(k0 + kchunk, mz);
+  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  if (remap) cv_xcd_tile(bx, by, bz);
+  const int i0 = 63 * bx, j0 = TY * by;
+  const int k0 = kchunk * bz, k1 = min(k0 + kchunk, mz);
   const int gi = i0 - 1 + L, gj = j0 + R;
   const bool in_i = gi >= 0 && gi < mx, in_j = gj < my;
   const bool cell = in_i && in_j;
@@ -276,18 +281,18 @@ __global__ void __launch_bounds__(ST_THREADS) k_stencil(OpV<T> A, const T *__res
   const int c = L + 1, r = R + 1;
   const long tij = (long)gi + (long)mx * gj;
 
-  // this thread's share of a tile plane: entries e0 = tid and e1 = tid + ST_THREADS
-  const int e1 = tid + ST_THREADS;
+  // this thread's share of a tile plane: entries tid and e1 = tid + NTH
+  const int e1 = tid + NTH;
   long off0 = 0, off1 = 0;
   bool ok0, ok1;
   {
     int rr = tid / ST_TC, cc = tid - rr * ST_TC;
     int ii = i0 - 2 + cc, jj = j0 - 1 + rr;
-    ok0 = ii >= 0 && ii < mx && jj >= 0 && jj < my;
+    ok0 = tid < NT && ii >= 0 && ii < mx && jj >= 0 && jj < my;
     if (ok0) off0 = (long)mx * jj + cv_wrap(ii, mx, A.xper);
     rr = e1 / ST_TC; cc = e1 - rr * ST_TC;
     ii = i0 - 2 + cc; jj = j0 - 1 + rr;
-    ok1 = e1 < ST_NT && ii >= 0 && ii < mx && jj >= 0 && jj < my;
+    ok1 = e1 < NT && ii >= 0 && ii < mx && jj >= 0 && jj < my;
     if (ok1) off1 = (long)mx * jj + cv_wrap(ii, mx, A.xper);
   }
   auto fetch_x = [&](int gk, T &v0, T &v1) {
@@ -298,26 +303,28 @@ __global__ void __launch_bounds__(ST_THREADS) k_stencil(OpV<T> A, const T *__res
   };
   auto store_x = [&](int gk, T v0, T v1) {
     T *S = X[(gk + 4) & 3];
-    S[tid] = v0;
-    if (e1 < ST_NT) S[e1] = v1;
+    if (tid < NT) S[tid] = v0;
+    if (e1 < NT) S[e1] = v1;
   };
+  constexpr bool PF = sizeof(T) == 4;      // prefetch aj, b, dinv with the cell
   auto fetch_cell = [&](int gk, StCell<T> &C) {
-    C.w = 0u; C.wb = 0u;
+    C.w = 0u;
 #pragma unroll
     for (int q = 0; q < 9; q++) C.g[q] = (T)0;
-    C.gb[0] = C.gb[1] = C.gb[2] = (T)0;
     C.aj = C.b = C.dv = (T)0;
+    C.wb = 0u;
+    C.gb[0] = C.gb[1] = C.gb[2] = (T)0;
     if (!cell) return;
     const long t = tij + pl * gk;
     C.w = A.fl[t];
 #pragma unroll
     for (int q = 0; q < 9; q++) C.g[q] = A.g[q][t];
-    if (out) {
+    if (PF && out) {
       C.aj = A.aj[t];
       if (MODE >= 1) C.b = b[t];
       if (MODE == 2) C.dv = dinv[t];
     }
-    if (row0) {
+    if (PFB && row0) {
       C.wb = A.fl[t - mx];
       C.gb[0] = A.g[3][t - mx]; C.gb[1] = A.g[4][t - mx]; C.gb[2] = A.g[5][t - mx];
     }
@@ -343,6 +350,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_stencil(OpV<T> A, const T *__res
     fetch_x(gk + 3, xq0, xq1);
     StCell<T> nxt;
     if (gk + 1 < k1) fetch_cell(gk + 1, nxt);
+    const long t = tij + pl * gk;
     const T *Pm = X[(gk - 1 + 4) & 3], *P = X[(gk + 4) & 3], *Q = X[(gk + 1 + 4) & 3];
     const uint32_t w = cur.w;
     const T fi = cell ? tile_flux_i<T>(Pm, P, Q, c, r, w, cur.g[0], cur.g[1], cur.g[2]) : (T)0;
@@ -350,7 +358,21 @@ __global__ void __launch_bounds__(ST_THREADS) k_stencil(OpV<T> A, const T *__res
     if (out) {
       fj = tile_flux_j<T>(Pm, P, Q, c, r, w, cur.g[3], cur.g[4], cur.g[5]);
       fk = tile_flux_k<T>(P, Q, c, r, w, cur.g[6], cur.g[7], cur.g[8]);
-      if (row0) fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, cur.wb, cur.gb[0], cur.gb[1], cur.gb[2]);
+      if (row0) {
+        if (PFB) {
+          fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, cur.wb, cur.gb[0], cur.gb[1], cur.gb[2]);
+        } else {
+          const long tb = t - mx;
+          fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, A.fl[tb], A.g[3][tb], A.g[4][tb], A.g[5][tb]);
+        }
+      }
+    }
+    // operands of the output (FP64: loaded here, before the barrier)
+    T ajc = cur.aj, bc = cur.b, dvc = cur.dv;
+    if (!PF && out) {
+      ajc = A.aj[t];
+      if (MODE >= 1) bc = b[t];
+      if (MODE == 2) dvc = dinv[t];
     }
     const T xc = P[r * ST_TC + c];
     FJ[gk & 1][R][L] = fj;
@@ -359,22 +381,21 @@ __global__ void __launch_bounds__(ST_THREADS) k_stencil(OpV<T> A, const T *__res
     __syncthreads();
     if (R > 0) fj_lo = FJ[gk & 1][R - 1][L];
     if (out) {
-      const long t = tij + pl * gk;
       T res;
       if (!(w & 1u)) {
         const T xv = x[t];
         if (MODE == 0) res = xv;
-        else if (MODE == 1) res = cur.b - xv;
-        else res = xv + omega * cur.dv * (cur.b - xv);
+        else if (MODE == 1) res = bc - xv;
+        else res = xv + omega * dvc * (bc - xv);
       } else {
         T acc = 0;
         acc += fi; acc -= fi_lo;
         acc += fj; acc -= fj_lo;
         acc += fk; acc -= fk_lo;
-        const T ax = -cur.aj * acc;
+        const T ax = -ajc * acc;
         if (MODE == 0) res = ax;
-        else if (MODE == 1) res = cur.b - ax;
-        else res = xc + omega * cur.dv * (cur.b - ax);
+        else if (MODE == 1) res = bc - ax;
+        else res = xc + omega * dvc * (bc - ax);
       }
       y[t] = res;
     }
@@ -500,7 +521,9 @@ __global__ void __launch_bounds__(1024) k_yline_reg(int mx, int my, int mz, cons
                                                    const T *xi, T *xo, T omega, int zero_guess) {
   __shared__ T SA[16][64], SB[16][64];
   const int ti = threadIdx.x, tc = threadIdx.y, nc = blockDim.y;
-  const int i = blockIdx.x * 64 + ti, k = blockIdx.y;
+  int bx, bk, bz;
+  cv_xcd_tile(bx, bk, bz);
+  const int i = bx * 64 + ti, k = bk;
   const bool act = i < mx;
   const long sj = mx;
   const long base = (long)i + (long)mx * my * k;
@@ -796,6 +819,30 @@ static int stencil_kchunk() {
   return kc;
 }
 
+// rows per stencil block (4 or 8; CV_STENCIL_ROWS64 / CV_STENCIL_ROWS32) and
+// XCD-aware tile order (CV_STENCIL_REMAP, 1 = on); the defaults are the
+// fastest measured at 512x256x256
+static int env_int(const char *name, int dflt) {
+  const char *e = getenv(name);
+  return e ? atoi(e) : dflt;
+}
+template <class T>
+static int stencil_rows() {
+  static int ty = 0;
+  if (!ty) ty = env_int(sizeof(T) == 8 ? "CV_STENCIL_ROWS64" : "CV_STENCIL_ROWS32", 4) == 8 ? 8 : 4;
+  return ty;
+}
+static int stencil_pfb64() {
+  static int v = -1;
+  if (v < 0) v = env_int("CV_STENCIL_PFB64", 1) != 0;
+  return v;
+}
+static int stencil_remap() {
+  static int rm = -1;
+  if (rm < 0) rm = env_int("CV_STENCIL_REMAP", 0) != 0;
+  return rm;
+}
+
 // algorithmic bytes per storage point of the stencil modes: x, 9 g, aj (+b)
 // (+dinv), y in T, and the 4-byte operator word
 template <class T>
@@ -808,10 +855,19 @@ static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T
   CvLevel &L = c->lev[l];
   OpV<T> A = opview<T>(c, l);
   const int kc = stencil_kchunk();
-  dim3 g((unsigned)((L.d.mx + 62) / 63), (unsigned)((L.d.my + ST_TY - 1) / ST_TY), (unsigned)((L.d.mz + kc - 1) / kc));
+  const int ty = stencil_rows<T>(), remap = stencil_remap();
+  dim3 g((unsigned)((L.d.mx + 62) / 63), (unsigned)((L.d.my + ty - 1) / ty), (unsigned)((L.d.mz + kc - 1) / kc));
   CvProfScope ps(c, MODE == 0 ? CV_PROF_APPLY : CV_PROF_RESID, l, sizeof(T) == 4 ? 1 : 0,
                  (double)L.d.n * stencil_bytes_pt<T>(MODE));
-  hipLaunchKernelGGL((k_stencil<T, MODE>), g, dim3(64, ST_TY), 0, c->stream, A, x, b, y, A.dinv, omega, kc);
+  const bool pfb = sizeof(T) == 4 || stencil_pfb64();
+  if (ty == 4 && pfb)
+    hipLaunchKernelGGL((k_stencil<T, MODE, 4, true>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap);
+  else if (ty == 4)
+    hipLaunchKernelGGL((k_stencil<T, MODE, 4, false>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap);
+  else if (pfb)
+    hipLaunchKernelGGL((k_stencil<T, MODE, 8, true>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap);
+  else
+    hipLaunchKernelGGL((k_stencil<T, MODE, 8, false>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap);
 }
 
 // sum of part[0..nb) -> dsc[slot] (device)
