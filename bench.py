@@ -65,7 +65,7 @@ def workload_case(cells):
     return cases.config_case(2, cells=tuple(cells), dt=dt)
 
 
-SOLVER = dict(smoother="yline", omega=0.8, pre_sweeps=2, post_sweeps=2, coarse_sweeps=10,
+SOLVER = dict(smoother="yline", omega=0.9, pre_sweeps=4, post_sweeps=4, coarse_sweeps=10,
               restart=30, max_it=200, rtol=1e-9, atol=0.0)
 
 
@@ -184,7 +184,15 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--restart", type=int, default=SOLVER["restart"])
+    ap.add_argument("--pre", type=int, default=SOLVER["pre_sweeps"])
+    ap.add_argument("--post", type=int, default=SOLVER["post_sweeps"])
+    ap.add_argument("--coarse", type=int, default=SOLVER["coarse_sweeps"])
+    ap.add_argument("--omega", type=float, default=SOLVER["omega"])
+    ap.add_argument("--smoother", default=SOLVER["smoother"], choices=("jacobi", "rb", "yline"))
     args = ap.parse_args()
+    SOLVER.update(restart=args.restart, pre_sweeps=args.pre, post_sweeps=args.post, coarse_sweeps=args.coarse,
+                  omega=args.omega, smoother=args.smoother)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not supported "
