@@ -172,6 +172,43 @@ def cpu_baseline(sample_cells, full_cells, threads):
     }
 
 
+def serving_workloads(n_serve, n_ens, seed, log_workers):
+    """Requests of the serving stream (Poisson arrivals, lam 50/s, grids
+    32^3..192^3) and of the ensemble (grids 32^3..128x96x96), inputs built
+    in spawned worker processes before any GPU work."""
+    from curvib_old_amd import serving
+    rng = np.random.default_rng(seed)
+    serve = [serving.sample_case(rng, lo=(32, 32, 32), hi=(192, 192, 192)) for _ in range(n_serve)]
+    arrivals = serving.poisson_arrivals(n_serve, 50.0, rng)
+    ens = [serving.sample_case(rng, lo=(32, 32, 32), hi=(128, 96, 96)) for _ in range(n_ens)]
+    t0 = time.perf_counter()
+    arrays = serving.build_inputs(serve + ens, workers=log_workers)
+    log(f"[serving] built {len(arrays)} case inputs in {time.perf_counter() - t0:.1f}s")
+    return list(zip(serve, arrays[:n_serve])), arrivals, list(zip(ens, arrays[n_serve:]))
+
+
+def run_serving(serve_reqs, arrivals, ens_reqs, nsteps, ens_steps):
+    from curvib_old_amd import serving
+    opts = {k: v for k, v in SOLVER.items()}
+    out = {}
+    for prec in ("fp64", "mixed"):
+        r = serving.run_stream(serve_reqs, arrivals, nsteps, prec, budget_bytes=200e9, slots=16, log=None, **opts)
+        r.pop("records")
+        out[f"serving_{prec}"] = r
+        log(f"[serving {prec}] {r['cases']} cases x {nsteps} steps: step p50 {r['step_latency'].get('p50_ms', 0):.1f} ms "
+            f"p99 {r['step_latency'].get('p99_ms', 0):.1f} ms, {r['cell_updates_per_s'] / 1e6:.1f} M cell-updates/s, "
+            f"peak HBM {r['bytes_peak'] / 2**30:.1f} GiB, failed {r['failed']}")
+    for prec in ("fp64", "mixed"):
+        r = serving.run_stream(ens_reqs, np.zeros(len(ens_reqs)), ens_steps, prec, budget_bytes=200e9, slots=16,
+                               **opts)
+        r.pop("records")
+        out[f"ensemble_{prec}"] = r
+        log(f"[ensemble {prec}] {r['cases']} cases x {ens_steps} steps: makespan {r['makespan_s']:.2f} s, step p50 "
+            f"{r['step_latency'].get('p50_ms', 0):.1f} ms p99 {r['step_latency'].get('p99_ms', 0):.1f} ms, "
+            f"{r['cell_updates_per_s'] / 1e6:.1f} M cell-updates/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +227,12 @@ def main():
     ap.add_argument("--coarse", type=int, default=SOLVER["coarse_sweeps"])
     ap.add_argument("--omega", type=float, default=SOLVER["omega"])
     ap.add_argument("--smoother", default=SOLVER["smoother"], choices=("jacobi", "rb", "yline"))
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=50)
+    ap.add_argument("--serving-steps", type=int, default=20)
+    ap.add_argument("--ensemble-cases", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=5)
+    ap.add_argument("--build-workers", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     SOLVER.update(restart=args.restart, pre_sweeps=args.pre, post_sweeps=args.post, coarse_sweeps=args.coarse,
                   omega=args.omega, smoother=args.smoother)
@@ -199,6 +242,10 @@ def main():
               "(the single-case path would shard by k-slabs with halo exchange, not built here)",
               file=sys.stderr)
         sys.exit(2)
+
+    serve_in = None
+    if not args.no_serving:
+        serve_in = serving_workloads(args.serving_requests, args.ensemble_cases, 2026, args.build_workers)
 
     cells = cells_of(args.cells)
     semi = tuple(int(v) for v in args.semi.split(","))
@@ -216,6 +263,14 @@ def main():
     tmx, itsmx, pr = timed_steps(umx, "mixed", args.warmup, args.steps, prof=("resid", -1, 1))
 
     acc = [] if args.no_accuracy else accuracy_pass(u64, umx, arrays, args.steps)
+    u64.close()
+    umx.close()
+    del arrays
+
+    srv = None
+    if serve_in is not None:
+        srv = run_serving(*serve_in, args.serving_steps, args.ensemble_steps)
+        serve_in = None
 
     ncell = float(np.prod(cells))
     npts = float(np.prod([c + 2 for c in cells]))
@@ -266,7 +321,15 @@ def main():
                   "cell_updates_per_s": ncell * sps_mx},
         "mixed_over_fp64": sps_mx / sps_64,
         "accuracy": acc,
-        "ensemble_latency_ms": None,
+        "ensemble_latency_ms": (None if srv is None else {
+            "serving_mixed_p50": srv["serving_mixed"]["step_latency"].get("p50_ms"),
+            "serving_mixed_p99": srv["serving_mixed"]["step_latency"].get("p99_ms"),
+            "serving_fp64_p50": srv["serving_fp64"]["step_latency"].get("p50_ms"),
+            "serving_fp64_p99": srv["serving_fp64"]["step_latency"].get("p99_ms"),
+            "ensemble_mixed_p50": srv["ensemble_mixed"]["step_latency"].get("p50_ms"),
+            "ensemble_mixed_p99": srv["ensemble_mixed"]["step_latency"].get("p99_ms"),
+        }),
+        "serving": srv,
         "roofline": roof,
         "cpu_baseline": cpu,
         "gpu_mixed_over_cpu": (sps_mx / cpu["value"]) if cpu else None,

@@ -37,7 +37,10 @@ SYMBOLS = [
     "cv_poisson_rhs", "cv_poisson_apply", "cv_poisson_solve", "cv_update_pressure",
     "cv_projection", "cv_divergence", "cv_flow_step", "cv_mg_restrict", "cv_mg_prolong",
     "cv_nullspace_project", "cv_prof_select", "cv_prof_read",
+    "cv_ctx_bytes", "cv_server_create", "cv_server_submit", "cv_server_wait", "cv_server_step_ms",
+    "cv_server_fetch", "cv_server_release", "cv_server_status", "cv_server_destroy",
 ]
+CV_CASE_QUEUED, CV_CASE_RUNNING, CV_CASE_DONE, CV_CASE_FAILED = 0, 1, 2, 3
 
 CV_PROF = {"none": 0, "resid": 1, "apply": 2, "line": 3, "rct": 4, "ff1": 5, "orth": 6}
 
@@ -64,6 +67,17 @@ class StepStats(C.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class CaseInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("rc", C.c_int), ("steps_done", C.c_int), ("bytes", C.c_long),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("last", StepStats)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("queued", C.c_long), ("running", C.c_long), ("done", C.c_long), ("bytes_in_use", C.c_long),
+                ("bytes_peak", C.c_long), ("budget", C.c_long), ("time", C.c_double)]
 
 
 _lib = None
@@ -105,9 +119,20 @@ def load():
     L.cv_nullspace_project.argtypes = [vp, C.c_int, dp]
     L.cv_prof_select.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.cv_prof_read.argtypes = [vp, C.POINTER(C.c_long), dp, dp, C.c_int]
+    L.cv_ctx_bytes.argtypes = [C.POINTER(GridDesc), C.c_int]
+    L.cv_ctx_bytes.restype = C.c_long
+    L.cv_server_create.argtypes = [C.c_int, C.c_long, C.c_int, C.POINTER(C.c_void_p)]
+    L.cv_server_submit.argtypes = [vp, C.POINTER(GridDesc), C.POINTER(SolverOpts), dp, dp, dp, C.c_int,
+                                   C.POINTER(C.c_long)]
+    L.cv_server_wait.argtypes = [vp, C.c_long, C.POINTER(CaseInfo)]
+    L.cv_server_step_ms.argtypes = [vp, C.c_long, dp, C.POINTER(C.c_int), C.c_int]
+    L.cv_server_fetch.argtypes = [vp, C.c_long, C.c_int, dp]
+    L.cv_server_release.argtypes = [vp, C.c_long]
+    L.cv_server_status.argtypes = [vp, C.POINTER(ServerInfo)]
+    L.cv_server_destroy.argtypes = [vp]
     for s in SYMBOLS:
         f = getattr(L, s)
-        if s not in ("cv_default_solver", "cv_device_bytes", "cv_last_error"):
+        if s not in ("cv_default_solver", "cv_device_bytes", "cv_last_error", "cv_ctx_bytes"):
             f.restype = C.c_int
     L.cv_last_error.restype = C.c_int
     _lib = L

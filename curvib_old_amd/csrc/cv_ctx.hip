@@ -35,6 +35,17 @@ int cv_alloc(cv_ctx *c, void **p, size_t bytes) {
   return CV_OK;
 }
 
+void cv_free(cv_ctx *c, void *p) {
+  for (size_t q = 0; q < c->allocs.size(); q++)
+    if (c->allocs[q] == p) {
+      c->allocs.erase(c->allocs.begin() + q);
+      break;
+    }
+  size_t sz = 0;
+  if (hipMemPtrGetInfo(p, &sz) == hipSuccess) c->bytes_alloc -= sz;
+  (void)hipFree(p);
+}
+
 __global__ void k_copy(double *dst, const double *src, long n) {
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
     dst[t] = src[t];
@@ -76,6 +87,60 @@ static int check_desc(const cv_grid_desc *d) {
   return 1;
 }
 
+// MGDACreate dimensions (init.c:78-99), level 0 coarsest; each coarsening
+// must keep an even cell count on coarsened axes so that fine cells pair
+// up.  Returns 0 or -1 with the error text set.
+static int level_dims(const cv_grid_desc *desc, CvDims *dims) {
+  int im = desc->IM, jm = desc->JM, km = desc->KM;
+  for (int l = desc->mglevels - 1; l >= 0; l--) {
+    if (l < desc->mglevels - 1) {
+      if ((!desc->isc && (im - 1) % 2) || (!desc->jsc && (jm - 1) % 2) || (!desc->ksc && (km - 1) % 2)) {
+        cv_set_error("cv_ctx_create: grid cannot be restricted to %d levels", desc->mglevels);
+        return -1;
+      }
+      im = desc->isc ? im : (im + 1) / 2;
+      jm = desc->jsc ? jm : (jm + 1) / 2;
+      km = desc->ksc ? km : (km + 1) / 2;
+    }
+    if (im < 4 || jm < 4 || km < 4) {
+      cv_set_error("cv_ctx_create: level %d too small (%d %d %d nodes)", l, im, jm, km);
+      return -1;
+    }
+    CvDims &D = dims[l];
+    D.mx = im + 1; D.my = jm + 1; D.mz = km + 1;
+    D.n = (long)D.mx * D.my * D.mz;
+    D.xper = desc->bctype[0] == 7;
+    D.zper = desc->bctype[4] == 7;
+    D.isc = desc->isc; D.jsc = desc->jsc; D.ksc = desc->ksc;
+  }
+  return 0;
+}
+
+// device bytes of one context: per level (FP64, 22 arrays: nvert, aj, 9 g, dinv, 3
+// coordinates, 3 line factors, line scratch, x b r; FP32: 9 g, aj, dinv, 3
+// line factors, line scratch, x b r (18 arrays); the operator word), the finest-level
+// metrics (40 doubles), state (28 doubles) and the 3n exchange staging,
+// reduction scratch and the
+// Krylov space of FGMRES(restart)
+static size_t plan_bytes(const CvDims *dims, int nlev, int restart) {
+  size_t b = 0;
+  for (int l = 0; l < nlev; l++) b += (size_t)dims[l].n * (22 * 8 + 18 * 4 + 4);
+  const size_t n = (size_t)dims[nlev - 1].n;
+  b += n * 8 * (40 + 28 + 3);
+  b += (size_t)(1 << 16) * 8 + 128 * 8 + 34 * 1024 * 8;
+  if (restart < 1) restart = 1;
+  if (restart > 31) restart = 31;
+  b += n * 8 * (size_t)(2 * restart + 1);
+  return b;
+}
+
+extern "C" long cv_ctx_bytes(const cv_grid_desc *desc, int restart) {
+  if (!check_desc(desc)) return -1;
+  CvDims dims[CV_MAXLEV];
+  if (level_dims(desc, dims)) return -1;
+  return (long)plan_bytes(dims, desc->mglevels, restart);
+}
+
 extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out) {
   if (!out) return CV_ERR_ARG;
   *out = nullptr;
@@ -83,6 +148,8 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     cv_set_error("cv_ctx_create: unsupported grid description");
     return CV_ERR_ARG;
   }
+  CvDims dims[CV_MAXLEV];
+  if (level_dims(desc, dims)) return CV_ERR_ARG;
   CV_CHECK(hipSetDevice(device));
   cv_ctx *c = new cv_ctx();
   c->device = device;
@@ -91,32 +158,7 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
   memcpy(c->bctype, desc->bctype, sizeof(c->bctype));
   c->ren = desc->ren; c->dt = desc->dt; c->st = desc->st > 0 ? desc->st : 1.0;
   cv_default_solver(&c->opt);
-  // MGDACreate dimensions (init.c:78-99); each coarsening must keep an even
-  // cell count on coarsened axes so that fine cells pair up
-  int im = desc->IM, jm = desc->JM, km = desc->KM;
-  for (int l = c->nlev - 1; l >= 0; l--) {
-    CvLevel &L = c->lev[l];
-    if (l < c->nlev - 1) {
-      if ((!desc->isc && (im - 1) % 2) || (!desc->jsc && (jm - 1) % 2) || (!desc->ksc && (km - 1) % 2)) {
-        delete c;
-        cv_set_error("cv_ctx_create: grid cannot be restricted to %d levels", desc->mglevels);
-        return CV_ERR_ARG;
-      }
-      im = desc->isc ? im : (im + 1) / 2;
-      jm = desc->jsc ? jm : (jm + 1) / 2;
-      km = desc->ksc ? km : (km + 1) / 2;
-    }
-    if (im < 4 || jm < 4 || km < 4) {
-      delete c;
-      cv_set_error("cv_ctx_create: level %d too small (%d %d %d nodes)", l, im, jm, km);
-      return CV_ERR_ARG;
-    }
-    L.d.mx = im + 1; L.d.my = jm + 1; L.d.mz = km + 1;
-    L.d.n = (long)L.d.mx * L.d.my * L.d.mz;
-    L.d.xper = desc->bctype[0] == 7;
-    L.d.zper = desc->bctype[4] == 7;
-    L.d.isc = desc->isc; L.d.jsc = desc->jsc; L.d.ksc = desc->ksc;
-  }
+  for (int l = 0; l < c->nlev; l++) c->lev[l].d = dims[l];
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     cv_set_error("cv_ctx_create: stream creation failed");
@@ -171,6 +213,8 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
         v[q] = (double *)p;
         CV_CHECK(hipMemsetAsync(v[q], 0, n * sizeof(double), c->stream));
       }
+    if ((rc = cv_alloc(c, &p, 3 * n * sizeof(double)))) goto fail;
+    c->stage = (double *)p;
     double **st1[] = {&c->p, &c->phi, &c->bvec, &c->w};
     for (auto s : st1) {
       if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail;
@@ -267,18 +311,8 @@ extern "C" int cv_get_field(cv_ctx *c, int f, int level, double *out) {
   if (dof == 1) {
     CV_CHECK(hipMemcpyAsync(out, ptr[0], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   } else {
-    double *tmp = c->w;   // scratch: needs 3n; use rct (3 planar n arrays) contiguous? allocate once
-    void *scratch;
-    int rc = cv_alloc(c, &scratch, 3 * n * sizeof(double));
-    if (rc) return rc;
-    (void)tmp;
-    hipLaunchKernelGGL(k_planar_to_aos, grid1(n, 256), dim3(256), 0, c->stream, (double *)scratch,
-                       ptr[0], ptr[1], ptr[2], n);
-    CV_CHECK(hipMemcpyAsync(out, scratch, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CV_CHECK(hipStreamSynchronize(c->stream));
-    CV_CHECK(hipFree(scratch));
-    c->allocs.pop_back();
-    c->bytes_alloc -= 3 * n * sizeof(double);
+    hipLaunchKernelGGL(k_planar_to_aos, grid1(n, 256), dim3(256), 0, c->stream, c->stage, ptr[0], ptr[1], ptr[2], n);
+    CV_CHECK(hipMemcpyAsync(out, c->stage, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }
   CV_CHECK(hipStreamSynchronize(c->stream));
   return CV_OK;
@@ -295,16 +329,9 @@ extern "C" int cv_put_field(cv_ctx *c, int f, int level, const double *in) {
   if (dof == 1) {
     CV_CHECK(hipMemcpyAsync(ptr[0], in, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   } else {
-    void *scratch;
-    int rc = cv_alloc(c, &scratch, 3 * n * sizeof(double));
-    if (rc) return rc;
-    CV_CHECK(hipMemcpyAsync(scratch, in, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_aos_to_planar, grid1(n, 256), dim3(256), 0, c->stream, (const double *)scratch,
+    CV_CHECK(hipMemcpyAsync(c->stage, in, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_aos_to_planar, grid1(n, 256), dim3(256), 0, c->stream, (const double *)c->stage,
                        ptr[0], ptr[1], ptr[2], n);
-    CV_CHECK(hipStreamSynchronize(c->stream));
-    CV_CHECK(hipFree(scratch));
-    c->allocs.pop_back();
-    c->bytes_alloc -= 3 * n * sizeof(double);
   }
   CV_CHECK(hipStreamSynchronize(c->stream));
   return CV_OK;

@@ -84,6 +84,7 @@ struct cv_ctx {
   // fine-level state (planar)
   double *ucont[3], *ucont_o[3], *ucat[3], *rhs[3], *p, *phi, *bvec;
   double *rct[3];
+  double *stage;          // 3n staging of the host field exchange (Cmpnts AoS)
   double *fface[9];       // Visc - Conv flux of the i, j, k face families (x|y|z each)
   // Krylov space
   double *V, *Z, *w;      // (restart+1)*n, restart*n, n
@@ -114,6 +115,7 @@ int cv_prof_drain(cv_ctx *c);
 
 // helpers implemented in cv_ctx.hip
 int cv_alloc(cv_ctx *c, void **p, size_t bytes);
+void cv_free(cv_ctx *c, void *p);
 int cv_reduce_sum(cv_ctx *c, const double *const *vecs_a, const double *const *vecs_b, int nvec,
                   long n, double *out_host);
 

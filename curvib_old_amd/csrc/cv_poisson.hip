@@ -1087,6 +1087,8 @@ static int ensure_krylov(cv_ctx *c) {
   const long n = c->lev[c->nlev - 1].d.n;
   void *p;
   int rc;
+  if (c->V) { cv_free(c, c->V); c->V = nullptr; }
+  if (c->Z) { cv_free(c, c->Z); c->Z = nullptr; }
   if ((rc = cv_alloc(c, &p, (size_t)(m + 1) * n * sizeof(double)))) return rc;
   c->V = (double *)p;
   if ((rc = cv_alloc(c, &p, (size_t)m * n * sizeof(double)))) return rc;

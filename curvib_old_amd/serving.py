@@ -1,0 +1,226 @@
+"""Continuous-batching serving of independent channel cases on one GPU.
+
+The server (``cv_server_*`` in the library) admits queued requests when a
+slot is free and the case's device context fits the HBM budget beside the
+cases in flight.  Each admitted case keeps its fields resident for all its
+steps on its own HIP stream, so the cases in flight overlap on the GPU.
+This module samples the ensemble and serving workloads, drives the server
+with a Poisson arrival stream, and summarises per-step latency and
+aggregate cell-updates/s.
+
+Workloads:
+  ensemble  n cases submitted at once, grids log-uniform from 32^3 to
+            128x96x96 cells, wall amplitude U[0, 0.1], Re U[100, 2000],
+            a sphere of diameter U[0.1, 0.3] in half of the cases
+  serving   Poisson arrivals (rate lam per second) of cases with grids
+            log-uniform from 32^3 to 192^3 cells, geometry and Re as above,
+            nsteps fractional steps each, an HBM admission budget
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import multiprocessing as mp
+import time
+
+import numpy as np
+
+from . import _lib, cases
+from ._lib import check
+
+SEMI = (0, 1, 0)
+
+
+def levels_for(cells, semi=SEMI, max_levels=7, min_cells=4):
+    """Most MG levels the grid allows: every coarsened axis keeps an even
+    cell count down to at least ``min_cells`` cells."""
+    lev = 1
+    while lev < max_levels:
+        ok = True
+        for c, s in zip(cells, semi):
+            if s:
+                continue
+            f = 2 ** lev
+            if c % f or c // f < min_cells:
+                ok = False
+        if not ok:
+            break
+        lev += 1
+    return lev
+
+
+def sample_case(rng, lo=(32, 32, 32), hi=(192, 192, 192), round_to=8):
+    """One case of the ensemble/serving distributions."""
+    t = rng.uniform()
+    cells = tuple(int(round_to * max(1, round(l * (h / l) ** t / round_to))) for l, h in zip(lo, hi))
+    amp = float(rng.uniform(0.0, 0.1))
+    re = float(rng.uniform(100.0, 2000.0))
+    sphere = None
+    if rng.uniform() < 0.5:
+        sphere = (0.5, 0.5, 0.5, float(rng.uniform(0.1, 0.3)))
+    seed = int(rng.integers(0, 2 ** 31 - 1))
+    c = cases.ChannelCase(cells=cells, amp=amp, sphere=sphere, re=re, seed=seed)
+    return dataclasses.replace(c, dt=cases.stable_dt(c))
+
+
+def _build_arrays(case):
+    return cases.build(case)
+
+
+def build_inputs(case_list, workers=8):
+    """Node coordinates, nvert and initial fluxes of every case, built in a
+    spawn-started process pool (call before any GPU work in this process)."""
+    if workers <= 1 or len(case_list) <= 1:
+        return [cases.build(c) for c in case_list]
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(workers) as pool:
+        return pool.map(_build_arrays, case_list, chunksize=1)
+
+
+def grid_desc(case, mglevels, semi=SEMI):
+    d = _lib.GridDesc()
+    d.IM, d.JM, d.KM = case.nodes
+    for q in range(6):
+        d.bctype[q] = int(case.bctype[q])
+    d.mglevels = int(mglevels)
+    d.isc, d.jsc, d.ksc = (int(v) for v in semi)
+    d.ren, d.dt, d.st = float(case.re), float(case.dt), 1.0
+    return d
+
+
+def solver_opts(**kw):
+    o = _lib.SolverOpts()
+    _lib.load().cv_default_solver(C.byref(o))
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = {"fp64": _lib.CV_FP64, "mixed": _lib.CV_MIXED}[v]
+        if k == "smoother" and isinstance(v, str):
+            v = {"jacobi": _lib.CV_SMOOTH_JACOBI, "rb": _lib.CV_SMOOTH_RB, "yline": _lib.CV_SMOOTH_YLINE}[v]
+        setattr(o, k, v)
+    return o
+
+
+def ctx_bytes(case, mglevels, restart=30, semi=SEMI):
+    return _lib.load().cv_ctx_bytes(C.byref(grid_desc(case, mglevels, semi)), int(restart))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Server:
+    """ctypes handle of one cv_server."""
+
+    def __init__(self, budget_bytes=200e9, slots=16, device=0):
+        L = _lib.load()
+        h = C.c_void_p()
+        check(L.cv_server_create(int(device), int(budget_bytes), int(slots), C.byref(h)), "cv_server_create")
+        self.h = h
+
+    def submit(self, case, arrays, nsteps, mglevels=None, semi=SEMI, **solver):
+        xyz, nv, uc = (np.ascontiguousarray(a, dtype=np.float64) for a in arrays)
+        if mglevels is None:
+            mglevels = levels_for(case.cells, semi)
+        d = grid_desc(case, mglevels, semi)
+        o = solver_opts(**solver)
+        cid = C.c_long()
+        check(_lib.load().cv_server_submit(self.h, C.byref(d), C.byref(o), _dp(xyz), _dp(nv), _dp(uc),
+                                           int(nsteps), C.byref(cid)), "cv_server_submit")
+        return cid.value
+
+    def wait(self, cid):
+        info = _lib.CaseInfo()
+        rc = _lib.load().cv_server_wait(self.h, int(cid), C.byref(info))
+        return rc, {"status": info.status, "rc": info.rc, "steps_done": info.steps_done, "bytes": info.bytes,
+                    "t_submit": info.t_submit, "t_admit": info.t_admit, "t_done": info.t_done,
+                    "last": info.last.as_dict()}
+
+    def step_ms(self, cid, nmax=100000):
+        ms = np.zeros(nmax)
+        its = np.zeros(nmax, dtype=np.int32)
+        n = _lib.load().cv_server_step_ms(self.h, int(cid), _dp(ms), its.ctypes.data_as(C.POINTER(C.c_int)), nmax)
+        if n < 0:
+            check(n, "cv_server_step_ms")
+        return ms[:n], its[:n]
+
+    def fetch(self, cid, field, shape):
+        out = np.empty(shape)
+        check(_lib.load().cv_server_fetch(self.h, int(cid), _lib.FIELD_ID[field], _dp(out)), "cv_server_fetch")
+        return out
+
+    def release(self, cid):
+        check(_lib.load().cv_server_release(self.h, int(cid)), "cv_server_release")
+
+    def status(self):
+        s = _lib.ServerInfo()
+        check(_lib.load().cv_server_status(self.h, C.byref(s)), "cv_server_status")
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().cv_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def latency_summary(ms):
+    ms = np.asarray(ms, dtype=np.float64)
+    if ms.size == 0:
+        return {"n": 0}
+    return {"n": int(ms.size), "p50_ms": float(np.percentile(ms, 50)), "p99_ms": float(np.percentile(ms, 99)),
+            "mean_ms": float(ms.mean()), "max_ms": float(ms.max())}
+
+
+def run_stream(requests, arrivals, nsteps, precision, budget_bytes=200e9, slots=16, device=0, log=None,
+               **solver):
+    """Submit ``requests`` [(case, arrays)] at the wall-clock offsets
+    ``arrivals`` (seconds, increasing), wait for all, and return per-case
+    records plus the aggregate figures.  Every case runs ``nsteps`` steps."""
+    srv = Server(budget_bytes, slots, device)
+    ids = []
+    t0 = time.perf_counter()
+    try:
+        for (case, arrays), ta in zip(requests, arrivals):
+            dt = ta - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append((case, srv.submit(case, arrays, nsteps, precision=precision, **solver)))
+        recs = []
+        for case, cid in ids:
+            rc, info = srv.wait(cid)
+            ms, its = srv.step_ms(cid)
+            recs.append({"cells": int(np.prod(case.cells)), "rc": rc, "info": info, "step_ms": ms, "its": its})
+            if log:
+                log(f"[serve {precision}] case {cid} {case.cells} rc={rc} steps={len(ms)} "
+                    f"p50={np.median(ms) if len(ms) else 0:.1f}ms its={int(np.median(its)) if len(its) else 0}")
+        wall = time.perf_counter() - t0
+        st = srv.status()
+    finally:
+        srv.close()
+    all_ms = np.concatenate([r["step_ms"] for r in recs]) if recs else np.zeros(0)
+    cell_updates = float(sum(r["cells"] * len(r["step_ms"]) for r in recs))
+    first = min(r["info"]["t_submit"] for r in recs)
+    last = max(r["info"]["t_done"] for r in recs)
+    return {
+        "precision": precision,
+        "cases": len(recs),
+        "failed": sum(1 for r in recs if r["rc"] != 0),
+        "steps": int(all_ms.size),
+        "step_latency": latency_summary(all_ms),
+        "queue_wait_s": latency_summary(1e3 * np.array([r["info"]["t_admit"] - r["info"]["t_submit"]
+                                                        for r in recs])),
+        "makespan_s": last - first,
+        "wall_s": wall,
+        "cell_updates_per_s": cell_updates / (last - first) if last > first else 0.0,
+        "bytes_peak": st["bytes_peak"],
+        "records": recs,
+    }
+
+
+def poisson_arrivals(n, lam, rng):
+    return np.cumsum(rng.exponential(1.0 / lam, size=n))

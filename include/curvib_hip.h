@@ -132,6 +132,48 @@ int cv_mg_restrict(cv_ctx *ctx, int coarse_level, const double *xf, double *xc);
 int cv_mg_prolong(cv_ctx *ctx, int fine_level, const double *xc, double *xf);     /* MyInterpolation poisson.c:265-370 */
 int cv_nullspace_project(cv_ctx *ctx, int level, double *x);                      /* PoissonNullSpaceFunction poisson.c:372-563 */
 
+/* ---- serving: continuous batching of independent cases on one GPU (no
+ * reference counterpart; the reference runs one case per process,
+ * main.c:1094-1170) ----
+ * Requests queue in arrival order; a request is admitted when one of the
+ * server's slots is free and its context (cv_ctx_bytes) fits the HBM budget
+ * beside the cases in flight.  An admitted case keeps all its fields
+ * resident on the device for its nsteps fractional steps (cv_flow_step),
+ * on its own HIP stream, so the cases in flight overlap on the GPU.  The
+ * server copies the inputs at submit; results (p, Ucont) and per-step
+ * wall-clock latencies stay with the server until cv_server_release.
+ * cv_server_destroy finishes the cases in flight and drops queued ones. */
+#define CV_CASE_QUEUED  0
+#define CV_CASE_RUNNING 1
+#define CV_CASE_DONE    2
+#define CV_CASE_FAILED  3
+
+typedef struct cv_server cv_server;
+typedef struct {
+  int status, rc, steps_done;
+  long bytes;                          /* device bytes of the case's context */
+  double t_submit, t_admit, t_done;    /* seconds since cv_server_create */
+  cv_step_stats last;                  /* statistics of the last step */
+} cv_case_info;
+typedef struct {
+  long queued, running, done;
+  long bytes_in_use, bytes_peak, budget;
+  double time;                         /* seconds since cv_server_create */
+} cv_server_info;
+
+/* device bytes one context for this grid and FGMRES restart length holds
+ * (-1 for an unsupported description) */
+long cv_ctx_bytes(const cv_grid_desc *desc, int restart);
+int  cv_server_create(int device, long hbm_budget_bytes, int slots, cv_server **out);
+int  cv_server_submit(cv_server *s, const cv_grid_desc *grid, const cv_solver_opts *opts, const double *xyz_nodes,
+                      const double *nvert, const double *ucont, int nsteps, long *case_id);
+int  cv_server_wait(cv_server *s, long case_id, cv_case_info *info);      /* blocks until done or failed */
+int  cv_server_step_ms(cv_server *s, long case_id, double *ms, int *its, int max);  /* returns the count */
+int  cv_server_fetch(cv_server *s, long case_id, int field /* CV_F_P, CV_F_UCONT */, double *out);
+int  cv_server_release(cv_server *s, long case_id);
+int  cv_server_status(cv_server *s, cv_server_info *info);
+int  cv_server_destroy(cv_server *s);
+
 /* ---- instrumentation (no reference counterpart; the reference's only
  * timing hook is the per-phase CPU_TIME log, solvers.c:1130-1142) ----
  * Device-event timing of every launch of one kernel class on one level,

@@ -55,3 +55,27 @@ def test_error_paths_without_gpu():
     o = _lib.SolverOpts()
     L.cv_default_solver(ctypes.byref(o))
     assert o.restart == 30 and o.precision == _lib.CV_FP64
+
+
+@pytest.mark.skipif(not os.path.exists(SO), reason="library not built")
+def test_ctx_bytes_estimate_without_gpu():
+    """cv_ctx_bytes is host arithmetic: grows with the grid and the restart
+    length, rejects unsupported descriptions."""
+    from curvib_old_amd import cases, serving
+    a = cases.ChannelCase(cells=(32, 32, 32))
+    b = cases.ChannelCase(cells=(64, 32, 32))
+    la = serving.levels_for(a.cells)
+    assert serving.ctx_bytes(a, la, 30) > serving.ctx_bytes(a, la, 10) > 0
+    assert serving.ctx_bytes(b, la, 30) > serving.ctx_bytes(a, la, 30)
+    # one Krylov vector more costs 2 fine-level vectors (V and Z)
+    n = 34 * 34 * 34
+    assert serving.ctx_bytes(a, la, 11) - serving.ctx_bytes(a, la, 10) == 2 * 8 * n
+    bad = cases.ChannelCase(cells=(32, 32, 32), bctype=(7, 7, 7, 7, 7, 7))
+    assert serving.ctx_bytes(bad, la, 30) == -1
+
+
+def test_levels_for_semi_coarsening():
+    from curvib_old_amd import serving
+    assert serving.levels_for((32, 32, 32)) == 4          # x, z: 32 -> 4 cells
+    assert serving.levels_for((512, 256, 256)) == 7
+    assert serving.levels_for((96, 40, 96)) == 5

@@ -1,0 +1,86 @@
+"""Serving layer on the GPU: continuous batching over the C ABI.
+
+A case's result is the same alone and in the mix: each admitted case owns
+its context and stream and every reduction has a fixed order, so the
+tolerance is exact equality (0).  Admission respects the HBM budget, and a
+case larger than the budget is refused at submit."""
+import numpy as np
+import pytest
+
+from curvib_old_amd import _lib, cases, serving
+
+pytestmark = pytest.mark.gpu
+
+OPTS = dict(precision="mixed", smoother="yline", omega=0.9, pre_sweeps=4, post_sweeps=4, coarse_sweeps=10,
+            restart=30, rtol=1e-9, max_it=200)
+
+
+def small_cases(n, seed):
+    rng = np.random.default_rng(seed)
+    return [serving.sample_case(rng, lo=(16, 16, 16), hi=(48, 40, 40)) for _ in range(n)]
+
+
+@pytest.fixture(scope="module")
+def mix():
+    cs = small_cases(6, 11)
+    return [(c, cases.build(c)) for c in cs]
+
+
+def run(reqs, slots, budget=50e9, nsteps=3):
+    srv = serving.Server(budget, slots)
+    try:
+        ids = [srv.submit(c, a, nsteps, **OPTS) for c, a in reqs]
+        out = []
+        for (c, _), cid in zip(reqs, ids):
+            rc, info = srv.wait(cid)
+            assert rc == 0, (c.cells, rc, _lib.last_error())
+            mx, my, mz = c.storage
+            out.append((srv.fetch(cid, "p", (mz, my, mx)), srv.fetch(cid, "ucont", (mz, my, mx, 3)), info))
+        return out, srv.status()
+    finally:
+        srv.close()
+
+
+def test_alone_equals_mix(mix):
+    alone, _ = run(mix[:1], slots=1)
+    together, st = run(mix, slots=4)
+    p0, u0, i0 = alone[0]
+    p1, u1, i1 = together[0]
+    assert i0["steps_done"] == i1["steps_done"] == 3
+    assert np.array_equal(p0, p1)
+    assert np.array_equal(u0, u1)
+    # the mix really overlapped: more than one case was resident at once
+    assert st["bytes_peak"] > max(i["bytes"] for _, _, i in together)
+
+
+def test_budget_bounds_residency(mix):
+    b = sorted(serving.ctx_bytes(c, serving.levels_for(c.cells), OPTS["restart"]) for c, _ in mix)
+    budget = b[-1] + b[-2]           # at most two of the largest fit together
+    out, st = run(mix, slots=6, budget=budget, nsteps=1)
+    assert st["bytes_peak"] <= budget
+    assert all(i["steps_done"] == 1 for _, _, i in out)
+
+
+def test_oversize_case_refused(mix):
+    c, a = mix[0]
+    srv = serving.Server(budget_bytes=1 << 20, slots=1)
+    try:
+        with pytest.raises(_lib.CvError) as e:
+            srv.submit(c, a, 1, **OPTS)
+        assert e.value.code == _lib.CV_ERR_OOM
+    finally:
+        srv.close()
+
+
+def test_ctx_bytes_matches_allocation(mix):
+    from curvib_old_amd.solver import UserCtx
+    c, (xyz, nv, uc) = mix[1]
+    lev = serving.levels_for(c.cells)
+    u = UserCtx(c.nodes, c.bctype, lev, c.re, c.dt, semi=serving.SEMI)
+    u.set_solver(**OPTS)
+    u.set_grid(xyz)
+    u.set_nvert(nv)
+    u.put("ucont", uc)
+    u.Flow_Solver()
+    assert u.device_bytes() == serving.ctx_bytes(c, lev, OPTS["restart"])
+    u.close()
