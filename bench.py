@@ -65,14 +65,14 @@ def workload_case(cells):
     return cases.config_case(2, cells=tuple(cells), dt=dt)
 
 
-SOLVER = dict(smoother="yline", omega=0.9, pre_sweeps=4, post_sweeps=4, coarse_sweeps=10,
+SOLVER = dict(smoother="yline", omega=0.9, pre_sweeps=3, post_sweeps=3, coarse_sweeps=10,
               restart=30, max_it=200, rtol=1e-9, atol=0.0)
 
 
-def make_ctx(case, arrays, levels, semi, precision):
+def make_ctx(case, arrays, levels, semi, precision, coarsen_auto):
     from curvib_old_amd import solver
     xyz, nv, uc = arrays
-    u = solver.UserCtx(case.nodes, case.bctype, levels, case.re, case.dt, semi=semi)
+    u = solver.UserCtx(case.nodes, case.bctype, levels, case.re, case.dt, semi=semi, coarsen_auto=coarsen_auto)
     u.set_solver(precision=precision, **SOLVER)
     u.set_grid(xyz)
     u.set_nvert(nv)
@@ -215,7 +215,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", default="512x256x256")
-    ap.add_argument("--levels", type=int, default=7)
+    ap.add_argument("--levels", type=int, default=8)
+    ap.add_argument("--coarsen-auto", type=int, default=1)
     ap.add_argument("--semi", default="0,1,0")
     ap.add_argument("--cpu-sample", default="256x128x128")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -255,8 +256,8 @@ def main():
     arrays = cases.build(case)
     log(f"[setup] case {cells} dt={case.dt:g} built in {time.perf_counter() - t0:.1f}s")
 
-    u64 = make_ctx(case, arrays, args.levels, semi, "fp64")
-    umx = make_ctx(case, arrays, args.levels, semi, "mixed")
+    u64 = make_ctx(case, arrays, args.levels, semi, "fp64", args.coarsen_auto)
+    umx = make_ctx(case, arrays, args.levels, semi, "mixed", args.coarsen_auto)
     log(f"[setup] device bytes per context: {u64.device_bytes() / 2**30:.1f} GiB")
 
     t64, its64, _ = timed_steps(u64, "fp64", args.warmup, args.steps)
@@ -313,7 +314,7 @@ def main():
             "workload": f"warped channel + immersed sphere {'x'.join(map(str, cells))} cells, Re {case.re:g}, "
                         f"dt {case.dt:g}, Poisson rtol {SOLVER['rtol']:g}",
             "cells": int(ncell), "storage_points": int(npts),
-            "mg_levels": args.levels, "semi_coarsening": list(semi), **SOLVER,
+            "mg_levels": args.levels, "semi_coarsening": list(semi), "coarsen_auto": args.coarsen_auto, **SOLVER,
         },
         "fp64": {"steps_per_s": sps_64, "ms_per_step": 1e3 * t64 / args.steps, "its": its64,
                  "cell_updates_per_s": ncell * sps_64},

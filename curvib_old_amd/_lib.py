@@ -48,7 +48,7 @@ CV_PROF = {"none": 0, "resid": 1, "apply": 2, "line": 3, "rct": 4, "ff1": 5, "or
 class GridDesc(C.Structure):
     _fields_ = [("IM", C.c_int), ("JM", C.c_int), ("KM", C.c_int), ("bctype", C.c_int * 6),
                 ("mglevels", C.c_int), ("isc", C.c_int), ("jsc", C.c_int), ("ksc", C.c_int),
-                ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double)]
+                ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double), ("coarsen_auto", C.c_int)]
 
 
 class SolverOpts(C.Structure):

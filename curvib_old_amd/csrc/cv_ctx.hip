@@ -1,6 +1,7 @@
 // cv_ctx.hip -- context life cycle, device memory, field exchange and the
 // deterministic reductions of the CURVIB fractional-step library (gfx950).
 #include "cv_internal.h"
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
@@ -89,18 +90,33 @@ static int check_desc(const cv_grid_desc *d) {
 
 // MGDACreate dimensions (init.c:78-99), level 0 coarsest; each coarsening
 // must keep an even cell count on coarsened axes so that fine cells pair
-// up.  Returns 0 or -1 with the error text set.
+// up.  With coarsen_auto a transition coarsens, among the axes the semi
+// flags allow, only those with the most cells; each level stores the flags
+// of the transition that produced it.  Returns 0 or -1 with the error text
+// set.
 static int level_dims(const cv_grid_desc *desc, CvDims *dims) {
   int im = desc->IM, jm = desc->JM, km = desc->KM;
+  const bool autoc = desc->coarsen_auto != 0;
   for (int l = desc->mglevels - 1; l >= 0; l--) {
+    int isc = desc->isc, jsc = desc->jsc, ksc = desc->ksc;
+    if (autoc && l < desc->mglevels - 1) {
+      // coarsen, among the axes allowed to coarsen, only those with the most cells
+      int mxc = 0;
+      if (!isc) mxc = std::max(mxc, im - 1);
+      if (!jsc) mxc = std::max(mxc, jm - 1);
+      if (!ksc) mxc = std::max(mxc, km - 1);
+      if (!isc && im - 1 < mxc) isc = 1;
+      if (!jsc && jm - 1 < mxc) jsc = 1;
+      if (!ksc && km - 1 < mxc) ksc = 1;
+    }
     if (l < desc->mglevels - 1) {
-      if ((!desc->isc && (im - 1) % 2) || (!desc->jsc && (jm - 1) % 2) || (!desc->ksc && (km - 1) % 2)) {
+      if ((!isc && (im - 1) % 2) || (!jsc && (jm - 1) % 2) || (!ksc && (km - 1) % 2)) {
         cv_set_error("cv_ctx_create: grid cannot be restricted to %d levels", desc->mglevels);
         return -1;
       }
-      im = desc->isc ? im : (im + 1) / 2;
-      jm = desc->jsc ? jm : (jm + 1) / 2;
-      km = desc->ksc ? km : (km + 1) / 2;
+      im = isc ? im : (im + 1) / 2;
+      jm = jsc ? jm : (jm + 1) / 2;
+      km = ksc ? km : (km + 1) / 2;
     }
     if (im < 4 || jm < 4 || km < 4) {
       cv_set_error("cv_ctx_create: level %d too small (%d %d %d nodes)", l, im, jm, km);
@@ -111,7 +127,7 @@ static int level_dims(const cv_grid_desc *desc, CvDims *dims) {
     D.n = (long)D.mx * D.my * D.mz;
     D.xper = desc->bctype[0] == 7;
     D.zper = desc->bctype[4] == 7;
-    D.isc = desc->isc; D.jsc = desc->jsc; D.ksc = desc->ksc;
+    D.isc = isc; D.jsc = jsc; D.ksc = ksc;
   }
   return 0;
 }

@@ -1052,7 +1052,7 @@ static void vcycle(cv_ctx *c, MgVecs<T> &v, int l) {
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.d.n)), dim3(BS), 0, c->stream, idx_of(C), idx_of(L), C.d.isc, C.d.jsc,
                      C.d.ksc, (const double *)L.nvert, (const double *)C.nvert, (const T *)v.r[l], v.b[l - 1], C.d.n);
   vcycle<T>(c, v, l - 1);
-  hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, idx_of(L), idx_of(C), L.d.isc, L.d.jsc, L.d.ksc,
+  hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, idx_of(L), idx_of(C), C.d.isc, C.d.jsc, C.d.ksc,
                      (const double *)C.nvert, (const double *)L.nvert, (const T *)v.x[l - 1], v.x[l], 1, n);
   smooth<T>(c, v, l, c->opt.post_sweeps, false);
 }
@@ -1639,8 +1639,8 @@ extern "C" int cv_mg_prolong(cv_ctx *c, int lf, const double *xc, double *xf) {
   if (lf < 1 || lf >= c->nlev || !xf || !xc) return CV_ERR_ARG;
   CvLevel &C = c->lev[lf - 1], &F = c->lev[lf];
   CV_CHECK(hipMemcpyAsync(C.r, xc, C.d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_prolong<double>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, idx_of(F), idx_of(C), F.d.isc,
-                     F.d.jsc, F.d.ksc, (const double *)C.nvert, (const double *)F.nvert, (const double *)C.r, F.b, 0, F.d.n);
+  hipLaunchKernelGGL(k_prolong<double>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, idx_of(F), idx_of(C), C.d.isc,
+                     C.d.jsc, C.d.ksc, (const double *)C.nvert, (const double *)F.nvert, (const double *)C.r, F.b, 0, F.d.n);
   CV_CHECK(hipMemcpyAsync(xf, F.b, F.d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CV_CHECK(hipStreamSynchronize(c->stream));
   return CV_OK;

@@ -31,22 +31,23 @@ from ._lib import check
 SEMI = (0, 1, 0)
 
 
-def levels_for(cells, semi=SEMI, max_levels=7, min_cells=4):
-    """Most MG levels the grid allows: every coarsened axis keeps an even
-    cell count down to at least ``min_cells`` cells."""
-    lev = 1
-    while lev < max_levels:
-        ok = True
-        for c, s in zip(cells, semi):
-            if s:
-                continue
-            f = 2 ** lev
-            if c % f or c // f < min_cells:
-                ok = False
-        if not ok:
+COARSEN_AUTO = 1
+
+
+def levels_for(cells, semi=SEMI, max_levels=8, min_cells=4, coarsen_auto=COARSEN_AUTO):
+    """Most MG levels the grid allows (at most ``max_levels``): every
+    coarsening halves an even cell count and the coarsest level keeps at
+    least ``min_cells`` cells on the coarsened axes."""
+    from .solver import level_cells
+    best = 1
+    for lev in range(2, max_levels + 1):
+        h = level_cells(cells, semi, lev, coarsen_auto)
+        if h is None:
             break
-        lev += 1
-    return lev
+        if any(h[0][a] < min_cells for a in range(3) if not semi[a]):
+            break
+        best = lev
+    return best
 
 
 def sample_case(rng, lo=(32, 32, 32), hi=(192, 192, 192), round_to=8):
@@ -77,8 +78,9 @@ def build_inputs(case_list, workers=8):
         return pool.map(_build_arrays, case_list, chunksize=1)
 
 
-def grid_desc(case, mglevels, semi=SEMI):
+def grid_desc(case, mglevels, semi=SEMI, coarsen_auto=COARSEN_AUTO):
     d = _lib.GridDesc()
+    d.coarsen_auto = int(coarsen_auto)
     d.IM, d.JM, d.KM = case.nodes
     for q in range(6):
         d.bctype[q] = int(case.bctype[q])

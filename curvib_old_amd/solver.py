@@ -30,7 +30,7 @@ def _dp(a):
 
 
 class UserCtx:
-    def __init__(self, nodes, bctype, mglevels, ren, dt, st=1.0, semi=(0, 0, 0), device=0):
+    def __init__(self, nodes, bctype, mglevels, ren, dt, st=1.0, semi=(0, 0, 0), device=0, coarsen_auto=0):
         L = _lib.load()
         d = _lib.GridDesc()
         d.IM, d.JM, d.KM = (int(v) for v in nodes)
@@ -39,12 +39,14 @@ class UserCtx:
         d.mglevels = int(mglevels)
         d.isc, d.jsc, d.ksc = (int(v) for v in semi)
         d.ren, d.dt, d.st = float(ren), float(dt), float(st)
+        d.coarsen_auto = int(coarsen_auto)
         h = C.c_void_p()
         check(L.cv_ctx_create(C.byref(d), int(device), C.byref(h)), "cv_ctx_create")
         self.h = h
         self.nodes = tuple(int(v) for v in nodes)
         self.mglevels = mglevels
         self.semi = tuple(int(v) for v in semi)
+        self.coarsen_auto = int(coarsen_auto)
         self.storage = tuple(v + 1 for v in self.nodes)
         self.opts = _lib.SolverOpts()
         L.cv_default_solver(C.byref(self.opts))
@@ -104,13 +106,8 @@ class UserCtx:
         """Storage extents of MG level ``level`` (MGDACreate, init.c:78-99):
         a coarsened axis keeps (nodes + 1) // 2 nodes, a semi-coarsened one
         keeps its size."""
-        im, jm, km = self.nodes
-        si, sj, sk = self.semi
-        for _ in range(self.mglevels - 1 - level):
-            im = im if si else (im + 1) // 2
-            jm = jm if sj else (jm + 1) // 2
-            km = km if sk else (km + 1) // 2
-        return im + 1, jm + 1, km + 1
+        cells = level_cells(tuple(n - 1 for n in self.nodes), self.semi, self.mglevels, self.coarsen_auto)[level]
+        return tuple(c + 2 for c in cells)
 
     def get(self, name, level=-1):
         out = np.empty(self._shape(name, level))
@@ -216,11 +213,34 @@ class UserCtx:
         return a
 
 
-def from_case(case, mglevels=3, device=0, semi=(0, 0, 0)):
+def level_cells(cells, semi, nlev, coarsen_auto=0):
+    """Cell counts of every MG level, coarsest first (the hierarchy
+    cv_ctx_create builds); None where a level cannot be formed."""
+    c = list(cells)
+    out = [tuple(c)]
+    for _ in range(nlev - 1):
+        allowed = [a for a in range(3) if not semi[a]]
+        if not allowed:
+            return None
+        most = max(c[a] for a in allowed)
+        for a in allowed:
+            if coarsen_auto and c[a] < most:
+                continue
+            if c[a] % 2:
+                return None
+            c[a] //= 2
+        out.append(tuple(c))
+    if min(min(x) for x in out) < 3:
+        return None
+    return out[::-1]
+
+
+def from_case(case, mglevels=3, device=0, semi=(0, 0, 0), coarsen_auto=0):
     """UserCtx loaded with a curvib_old_amd.cases.ChannelCase."""
     from . import cases
     xyz, nv, uc = cases.build(case)
-    u = UserCtx(case.nodes, case.bctype, mglevels, case.re, case.dt, semi=semi, device=device)
+    u = UserCtx(case.nodes, case.bctype, mglevels, case.re, case.dt, semi=semi, device=device,
+                coarsen_auto=coarsen_auto)
     u.set_grid(xyz)
     u.set_nvert(nv)
     u.put("ucont", uc)

@@ -50,6 +50,9 @@ typedef struct {
   int mglevels;            /* -mg_level */
   int isc, jsc, ksc;       /* -mg_{i,j,k}_semi */
   double ren, dt, st;      /* -ren, -dt, Strouhal number */
+  int coarsen_auto;        /* 0: every level coarsens the axes the semi flags allow
+                              (MGDACreate); 1: a level coarsens, among those axes,
+                              only the ones with the most cells (anisotropic grids) */
 } cv_grid_desc;
 
 /* Poisson solver options (control.dat~:69-87 prefix ps_) */

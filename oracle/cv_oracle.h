@@ -79,7 +79,7 @@ typedef struct {
 
 /* ---- life cycle / data exchange (host arrays in DMDA global layout) ---- */
 octx *o_create(int IM, int JM, int KM, const int bctype[6], int nlev,
-               int isc, int jsc, int ksc, double ren, double dt);
+               int isc, int jsc, int ksc, double ren, double dt, int coarsen_auto);
 void  o_destroy(octx *c);
 void  o_set_coords(octx *c, const double *xyz_nodes); /* IM*JM*KM*3, grid.dat order */
 void  o_form_metrics_all(octx *c);                   /* FormMetrics (+PBC) every level */

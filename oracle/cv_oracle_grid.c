@@ -103,8 +103,11 @@ static void level_free(olevel *L) {
   free(L->rowptr); free(L->col); free(L->val); free(L->kske);
 }
 
+/* coarsen_auto: at each level transition coarsen, among the axes the semi
+ * flags allow, only those with the most cells (per-level flags, stored on
+ * the coarser level of each transition); 0: MGDACreate's fixed flags */
 octx *o_create(int IM, int JM, int KM, const int bctype[6], int nlev,
-               int isc, int jsc, int ksc, double ren, double dt) {
+               int isc, int jsc, int ksc, double ren, double dt, int coarsen_auto) {
   if (nlev < 1 || nlev > O_MAXLEV) return NULL;
   if (bctype[2] == 7 || bctype[3] == 7) return NULL;  /* j-periodic not restated */
   for (int q = 0; q < 6; q++) if (bctype[q] != 1 && bctype[q] != 7) return NULL;
@@ -117,11 +120,21 @@ octx *o_create(int IM, int JM, int KM, const int bctype[6], int nlev,
   int im = IM, jm = JM, km = KM;
   for (int l = nlev - 1; l >= 0; l--) {
     olevel *L = &c->lev[l];
-    L->isc = isc; L->jsc = jsc; L->ksc = ksc;
+    int si = isc, sj = jsc, sk = ksc;
+    if (coarsen_auto && l < nlev - 1) {
+      int most = 0;
+      if (!si && im - 1 > most) most = im - 1;
+      if (!sj && jm - 1 > most) most = jm - 1;
+      if (!sk && km - 1 > most) most = km - 1;
+      if (!si && im - 1 < most) si = 1;
+      if (!sj && jm - 1 < most) sj = 1;
+      if (!sk && km - 1 < most) sk = 1;
+    }
+    L->isc = si; L->jsc = sj; L->ksc = sk;
     if (l < nlev - 1) {          /* MGDACreate, init.c:78-99 */
-      im = isc ? im : (im + 1) / 2;
-      jm = jsc ? jm : (jm + 1) / 2;
-      km = ksc ? km : (km + 1) / 2;
+      im = si ? im : (im + 1) / 2;
+      jm = sj ? jm : (jm + 1) / 2;
+      km = sk ? km : (km + 1) / 2;
     }
     L->IM = im; L->JM = jm; L->KM = km;
     L->mx = im + 1; L->my = jm + 1; L->mz = km + 1;

@@ -356,11 +356,11 @@ void o_interp(octx *c, int lf, const double *xc, double *xf) {
         double f = 0.;
         if (i > 0 && i < mx - 1 && j > 0 && j < my - 1 && k > 0 && k < mz - 1) {
           int ic, jc, kc, ia, ja, ka;
-          if (F->isc) { ic = i; ia = 0; }
+          if (C->isc) { ic = i; ia = 0; }
           else { ic = (i + 1) / 2; ia = (i - 2 * ic) == 0 ? 1 : -1; if (i == 1 || i == mx - 2) ia = 0; }
-          if (F->jsc) { jc = j; ja = 0; }
+          if (C->jsc) { jc = j; ja = 0; }
           else { jc = (j + 1) / 2; ja = (j - 2 * jc) == 0 ? 1 : -1; if (j == 1 || j == my - 2) ja = 0; }
-          if (F->ksc) { kc = k; ka = 0; }
+          if (C->ksc) { kc = k; ka = 0; }
           else { kc = (k + 1) / 2; ka = (k - 2 * kc) == 0 ? 1 : -1; if (k == 1 || k == mz - 2) ka = 0; }
           if (ka == -1 && S(nvc, kc - 1, jc, ic) > 0.1) ka = 0;
           else if (ka == 1 && S(nvc, kc + 1, jc, ic) > 0.1) ka = 0;

@@ -29,7 +29,7 @@ def lib():
         vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
         L.o_create.restype = vp
         L.o_create.argtypes = [C.c_int, C.c_int, C.c_int, ip, C.c_int, C.c_int, C.c_int, C.c_int,
-                               C.c_double, C.c_double]
+                               C.c_double, C.c_double, C.c_int]
         L.o_destroy.argtypes = [vp]
         L.o_set_coords.argtypes = [vp, dp]
         L.o_form_metrics_all.argtypes = [vp]
@@ -68,11 +68,11 @@ def _dp(a):
 class Oracle:
     """One oracle context: a grid hierarchy plus the fine-level state."""
 
-    def __init__(self, nodes, bctype, nlev, re, dt, semi=(0, 0, 0)):
+    def __init__(self, nodes, bctype, nlev, re, dt, semi=(0, 0, 0), coarsen_auto=0):
         L = lib()
         bc = (C.c_int * 6)(*bctype)
         self.h = L.o_create(int(nodes[0]), int(nodes[1]), int(nodes[2]), bc, int(nlev),
-                            int(semi[0]), int(semi[1]), int(semi[2]), float(re), float(dt))
+                            int(semi[0]), int(semi[1]), int(semi[2]), float(re), float(dt), int(coarsen_auto))
         if not self.h:
             raise ValueError("oracle: unsupported configuration")
         self.nlev = nlev
@@ -183,11 +183,11 @@ class Oracle:
         return out
 
 
-def from_case(case, nlev=3, semi=(0, 0, 0)):
+def from_case(case, nlev=3, semi=(0, 0, 0), coarsen_auto=0):
     """Oracle context loaded with a curvib_old_amd.cases.ChannelCase."""
     from curvib_old_amd import cases
     xyz, nv, uc = cases.build(case)
-    o = Oracle(case.nodes, case.bctype, nlev, case.re, case.dt, semi)
+    o = Oracle(case.nodes, case.bctype, nlev, case.re, case.dt, semi, coarsen_auto)
     o.set_grid(xyz)
     o.set_nvert(nv)
     o.put("ucont", uc)

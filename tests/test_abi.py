@@ -76,6 +76,12 @@ def test_ctx_bytes_estimate_without_gpu():
 
 def test_levels_for_semi_coarsening():
     from curvib_old_amd import serving
-    assert serving.levels_for((32, 32, 32)) == 4          # x, z: 32 -> 4 cells
-    assert serving.levels_for((512, 256, 256)) == 7
-    assert serving.levels_for((96, 40, 96)) == 5
+    from curvib_old_amd.solver import level_cells
+    fixed = dict(coarsen_auto=0)
+    assert serving.levels_for((32, 32, 32), **fixed) == 4          # x, z: 32 -> 4 cells
+    assert serving.levels_for((512, 256, 256), **fixed) == 7
+    assert serving.levels_for((96, 40, 96), **fixed) == 5
+    # auto coarsening halves only the longest coarsenable axes first
+    h = level_cells((512, 256, 256), (0, 1, 0), 8, 1)
+    assert h[-1] == (512, 256, 256) and h[-2] == (256, 256, 256) and h[0] == (4, 256, 4)
+    assert serving.levels_for((512, 256, 256), coarsen_auto=1) == 8

@@ -10,12 +10,17 @@ from curvib_old_amd import cases
 
 pytestmark = pytest.mark.gpu
 
+# name: (case, MG levels, semi-coarsening flags, coarsen_auto)
 CASES = {
-    "channel": cases.ChannelCase(cells=(16, 16, 16), amp=0.05, beta=2.0, sphere=None, re=180.0, seed=1),
-    "sphere": cases.ChannelCase(cells=(32, 24, 16), amp=0.1, beta=2.0, sphere=(0.5, 0.5, 0.5, 0.4),
-                                re=500.0, seed=2),
-    "walls": cases.ChannelCase(cells=(16, 16, 16), amp=0.05, beta=1.0, sphere=(0.5, 0.5, 0.5, 0.35),
-                               re=300.0, seed=3, bctype=(1, 1, 1, 1, 1, 1)),
+    "channel": (cases.ChannelCase(cells=(16, 16, 16), amp=0.05, beta=2.0, sphere=None, re=180.0, seed=1),
+                3, (0, 0, 0), 0),
+    "sphere": (cases.ChannelCase(cells=(32, 24, 16), amp=0.1, beta=2.0, sphere=(0.5, 0.5, 0.5, 0.4),
+                                 re=500.0, seed=2), 3, (0, 0, 0), 0),
+    "walls": (cases.ChannelCase(cells=(16, 16, 16), amp=0.05, beta=1.0, sphere=(0.5, 0.5, 0.5, 0.35),
+                                re=300.0, seed=3, bctype=(1, 1, 1, 1, 1, 1)), 3, (0, 0, 0), 0),
+    # y kept, x coarsened alone first, then x and z together
+    "semi_auto": (cases.ChannelCase(cells=(48, 20, 24), amp=0.1, beta=2.0, sphere=(0.5, 0.5, 0.5, 0.3),
+                                    re=800.0, seed=4, dt=2e-4), 3, (0, 1, 0), 1),
 }
 
 
@@ -29,9 +34,9 @@ def rel(a, b):
 def pair(request):
     from oracle.pyoracle import from_case as ofc
     from curvib_old_amd.solver import from_case as gfc
-    case = CASES[request.param]
-    o, data = ofc(case, nlev=3)
-    u, _ = gfc(case, mglevels=3)
+    case, nlev, semi, auto = CASES[request.param]
+    o, data = ofc(case, nlev=nlev, semi=semi, coarsen_auto=auto)
+    u, _ = gfc(case, mglevels=nlev, semi=semi, coarsen_auto=auto)
     yield request.param, case, o, u
     u.close()
     o.close()
@@ -48,6 +53,7 @@ def test_metrics(pair):
 def test_nvert_levels(pair):
     name, case, o, u = pair
     for l in range(3):
+        assert u.get("nvert", l).shape == tuple(reversed(o.dims(l)[:3])), (name, l)
         assert np.array_equal(u.get("nvert", l), o.get("nvert", l)), (name, l)
 
 

@@ -11,7 +11,7 @@ from curvib_old_amd import _lib, cases, serving
 
 pytestmark = pytest.mark.gpu
 
-OPTS = dict(precision="mixed", smoother="yline", omega=0.9, pre_sweeps=4, post_sweeps=4, coarse_sweeps=10,
+OPTS = dict(precision="mixed", smoother="yline", omega=0.9, pre_sweeps=3, post_sweeps=3, coarse_sweeps=10,
             restart=30, rtol=1e-9, max_it=200)
 
 
@@ -76,7 +76,7 @@ def test_ctx_bytes_matches_allocation(mix):
     from curvib_old_amd.solver import UserCtx
     c, (xyz, nv, uc) = mix[1]
     lev = serving.levels_for(c.cells)
-    u = UserCtx(c.nodes, c.bctype, lev, c.re, c.dt, semi=serving.SEMI)
+    u = UserCtx(c.nodes, c.bctype, lev, c.re, c.dt, semi=serving.SEMI, coarsen_auto=serving.COARSEN_AUTO)
     u.set_solver(**OPTS)
     u.set_grid(xyz)
     u.set_nvert(nv)
