@@ -85,6 +85,7 @@ static int check_desc(const cv_grid_desc *d) {
   if ((d->bctype[4] == 7) != (d->bctype[5] == 7)) return 0;
   if (d->mglevels < 1 || d->mglevels > CV_MAXLEV) return 0;
   if (d->dt <= 0 || d->ren <= 0) return 0;
+  if ((long)(d->IM + 1) * (d->JM + 1) * (d->KM + 1) >= (1L << 31)) return 0;   // 32-bit storage indices in kernels
   return 1;
 }
 

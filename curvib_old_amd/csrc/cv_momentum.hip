@@ -24,11 +24,13 @@ struct G {
   int mx, my, mz, xper, zper;
   long sj, sk;
   __device__ long at(int i, int j, int k) const { return (long)i + (long)mx * ((long)j + (long)my * k); }
+  // storage index -> (i, j, k); storage sizes stay below 2^31 (cv_ctx_create)
   __device__ void dec(long t, int &i, int &j, int &k) const {
-    i = (int)(t % mx);
-    long r = t / mx;
-    j = (int)(r % my);
-    k = (int)(r / my);
+    unsigned u = (unsigned)t;
+    i = (int)(u % (unsigned)mx);
+    u /= (unsigned)mx;
+    j = (int)(u % (unsigned)my);
+    k = (int)(u / (unsigned)my);
   }
   // local-array index of the reference's ghosted vectors for the QUICK far
   // ghosts (-1 -> m-3, m -> 2 on a periodic axis, rhs.c:1557-1628)
@@ -414,18 +416,18 @@ struct FFArgs {
   const double *p, *nv;
   const uint32_t *fl;
   C3 rct;
-  FM mi, mj, mk;
+  const double *gc[9];   // face coefficients (xi_d . xi_t) / J of the fine level, g[3d + t]
 };
 
+// rhs = avg(rct) - (dpdc g_d0 + dpde g_d1 + dpdz g_d2) on face t of axis d;
+// g_dt = (xi_d . xi_t) * aj_face are the coefficients the Poisson operator
+// uses (rhs.c:2869-2878 multiplies the same dot products by iaj after the
+// sum; the two orders agree to rounding)
 __device__ inline double face_rhs_value(const FFArgs &A, int d, long t, double dpdc, double dpde, double dpdz) {
-  const FM &m = d == 0 ? A.mi : d == 1 ? A.mj : A.mk;
-  const C3 nrm = d == 0 ? m.c : d == 1 ? m.e : m.z;
   const double *rc = d == 0 ? A.rct.x : d == 1 ? A.rct.y : A.rct.z;
   const long so[3] = {1, A.g.sj, A.g.sk};
   double r = 0.5 * (rc[t] + rc[t + so[d]]);
-  r -= (dpdc * (m.c.x[t] * nrm.x[t] + m.c.y[t] * nrm.y[t] + m.c.z[t] * nrm.z[t]) +
-        dpde * (m.e.x[t] * nrm.x[t] + m.e.y[t] * nrm.y[t] + m.e.z[t] * nrm.z[t]) +
-        dpdz * (m.z.x[t] * nrm.x[t] + m.z.y[t] * nrm.y[t] + m.z.z[t] * nrm.z[t])) * m.aj[t];
+  r -= dpdc * A.gc[3 * d][t] + dpde * A.gc[3 * d + 1][t] + dpdz * A.gc[3 * d + 2][t];
   return r;
 }
 
@@ -654,14 +656,11 @@ int cv_ghost_update(cv_ctx *c, int formbcs) {
 
 static FFArgs ffargs(cv_ctx *c) {
   CvLevel &F = c->lev[c->nlev - 1];
-  CvMetrics &M = c->m;
   FFArgs A;
   A.g = gview(c);
   A.p = c->p; A.nv = F.nvert; A.fl = F.flags;
   A.rct = c3(c->rct);
-  A.mi = FM{c3(M.icsi), c3(M.ieta), c3(M.izet), M.iaj};
-  A.mj = FM{c3(M.jcsi), c3(M.jeta), c3(M.jzet), M.jaj};
-  A.mk = FM{c3(M.kcsi), c3(M.keta), c3(M.kzet), M.kaj};
+  for (int q = 0; q < 9; q++) A.gc[q] = F.g[q];
   return A;
 }
 
@@ -673,12 +672,15 @@ int cv_ff1_dev(cv_ctx *c) {
   FFArgs A = ffargs(c);
   const double nu = 1. / c->ren;
   const C3 uc = c3(c->ucont), u = c3(c->ucat);
+  const FM mi{c3(M.icsi), c3(M.ieta), c3(M.izet), M.iaj};
+  const FM mj{c3(M.jcsi), c3(M.jeta), c3(M.jzet), M.jaj};
+  const FM mk{c3(M.kcsi), c3(M.keta), c3(M.kzet), M.kaj};
   const V3 f0 = v3(c->fface), f1 = v3(c->fface + 3), f2 = v3(c->fface + 6);
-  hipLaunchKernelGGL(k_face_rc<0>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, A.mi,
+  hipLaunchKernelGGL(k_face_rc<0>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, mi,
                      nu, f0, F.d.n);
-  hipLaunchKernelGGL(k_face_rc<1>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, A.mj,
+  hipLaunchKernelGGL(k_face_rc<1>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, mj,
                      nu, f1, F.d.n);
-  hipLaunchKernelGGL(k_face_rc<2>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, A.mk,
+  hipLaunchKernelGGL(k_face_rc<2>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, mk,
                      nu, f2, F.d.n);
   {
     CvProfScope ps(c, CV_PROF_RCT, c->nlev - 1, 0, (double)F.d.n * 8.0 * (9 + 9 + 1 + 3));

@@ -33,11 +33,13 @@ namespace {
 
 struct Idx {
   int mx, my, mz;
+  // storage index -> (i, j, k); storage sizes stay below 2^31 (cv_ctx_create)
   __device__ void dec(long t, int &i, int &j, int &k) const {
-    i = (int)(t % mx);
-    long r = t / mx;
-    j = (int)(r % my);
-    k = (int)(r / my);
+    unsigned u = (unsigned)t;
+    i = (int)(u % (unsigned)mx);
+    u /= (unsigned)mx;
+    j = (int)(u % (unsigned)my);
+    k = (int)(u / (unsigned)my);
   }
   __device__ long at(int i, int j, int k) const { return (long)i + (long)mx * ((long)j + (long)my * k); }
 };
@@ -1403,7 +1405,7 @@ int cv_update_pressure_dev(cv_ctx *c) {
 // follows with Contra2Cart and GhostNodeVelocity
 // ====================================================================
 struct FaceMet {
-  const double *c[3], *e[3], *z[3], *aj;   // face metric vectors of one family
+  const double *g[3];   // (xi_d . xi_t) / J of one face family d, t = 0..2 (the operator's g[3d + t])
 };
 
 __device__ static inline double tang_p(const double *p, Idx I, int d, int t, int i, int j, int k, int mode) {
@@ -1435,13 +1437,9 @@ __device__ static inline void project_face(Idx I, int d, int i, int j, int k, co
     int mode = (w >> (2 + 5 * d + 2 * q)) & 3u; q++;
     dp[tt] = tang_p(phi, I, d, tt, i, j, k, mode);
   }
-  const double *nx = d == 0 ? M.c[0] : d == 1 ? M.e[0] : M.z[0];
-  const double *ny = d == 0 ? M.c[1] : d == 1 ? M.e[1] : M.z[1];
-  const double *nz = d == 0 ? M.c[2] : d == 1 ? M.e[2] : M.z[2];
-  double a = M.aj[t];
-  double g0 = (M.c[0][t] * nx[t] + M.c[1][t] * ny[t] + M.c[2][t] * nz[t]) * a;
-  double g1 = (M.e[0][t] * nx[t] + M.e[1][t] * ny[t] + M.e[2][t] * nz[t]) * a;
-  double g2 = (M.z[0][t] * nx[t] + M.z[1][t] * ny[t] + M.z[2][t] * nz[t]) * a;
+  // the coefficients of poisson.c:2420-2431 (dot products of the face
+  // metrics times the face 1/J) are the operator's g of this face
+  const double g0 = M.g[0][t], g1 = M.g[1][t], g2 = M.g[2][t];
   u[t] -= (dp[0] * g0 + dp[1] * g1 + dp[2] * g2) * dt * st / 1.5;
 }
 
@@ -1465,10 +1463,9 @@ __global__ static void k_projection(Idx I, int xper, int zper, const double *phi
 
 int cv_projection_dev(cv_ctx *c) {
   CvLevel &F = c->lev[c->nlev - 1];
-  CvMetrics &M = c->m;
-  FaceMet Mi{{M.icsi[0], M.icsi[1], M.icsi[2]}, {M.ieta[0], M.ieta[1], M.ieta[2]}, {M.izet[0], M.izet[1], M.izet[2]}, M.iaj};
-  FaceMet Mj{{M.jcsi[0], M.jcsi[1], M.jcsi[2]}, {M.jeta[0], M.jeta[1], M.jeta[2]}, {M.jzet[0], M.jzet[1], M.jzet[2]}, M.jaj};
-  FaceMet Mk{{M.kcsi[0], M.kcsi[1], M.kcsi[2]}, {M.keta[0], M.keta[1], M.keta[2]}, {M.kzet[0], M.kzet[1], M.kzet[2]}, M.kaj};
+  FaceMet Mi{{F.g[0], F.g[1], F.g[2]}};
+  FaceMet Mj{{F.g[3], F.g[4], F.g[5]}};
+  FaceMet Mk{{F.g[6], F.g[7], F.g[8]}};
   hipLaunchKernelGGL(k_projection, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, idx_of(F), F.d.xper, F.d.zper,
                      (const double *)c->phi, (const double *)F.nvert, (const uint32_t *)F.flags, Mi, Mj, Mk,
                      c->ucont[0], c->ucont[1], c->ucont[2], c->dt, c->st, F.d.n);
