@@ -49,13 +49,31 @@ inline unsigned nblk(long n) {
   return (unsigned)b;
 }
 
+// k-marching launch shape for the cell and face kernels: a block of
+// 64 x MARCH_TY threads owns an (i, j) tile and walks MARCH_KC planes of k,
+// so the k-1 .. k+2 neighbours a cell reads were loaded by the same block a
+// few iterations earlier and are still in the cache
+#define MARCH_TY 4
+#define MARCH_KC 16
+#define CV_MARCH(g)                                                                          \
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * MARCH_TY + threadIdx.y;      \
+  const int k0_ = blockIdx.z * MARCH_KC, k1_ = min(k0_ + MARCH_KC, (g).mz);                 \
+  if (i < (g).mx && j < (g).my)                                                              \
+    for (int k = k0_; k < k1_; k++)
+
+inline dim3 march_grid(int mx, int my, int mz) {
+  return dim3((unsigned)((mx + 63) / 64), (unsigned)((my + MARCH_TY - 1) / MARCH_TY),
+              (unsigned)((mz + MARCH_KC - 1) / MARCH_KC));
+}
+#define MARCH_BLOCK dim3(64, MARCH_TY)
+
 // ------------------------------------------------------------------
 // Contra2Cart (rhs.c:25-144)
 // ------------------------------------------------------------------
 __global__ void k_contra2cart(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int i, j, k;
-    g.dec(t, i, j, k);
+  (void)n;
+  CV_MARCH(g) {
+    const long t = g.at(i, j, k);
     if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
     const double nn = nv[t];
     if (!(nn < 0.1 || nn > 7.)) continue;
@@ -333,9 +351,9 @@ __device__ inline void visc_flux(const G &g, int d, int i, int j, int k, C3 u, c
 // and rhs.c:2062-2486 evaluate into Fp1/Fp2/Fp3)
 template <int D>
 __global__ void __launch_bounds__(BS) k_face_rc(G g, C3 uc, C3 u, const double *nv, FM m, double nu, V3 F, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int i, j, k;
-    g.dec(t, i, j, k);
+  (void)n;
+  CV_MARCH(g) {
+    const long t = g.at(i, j, k);
     const int c[3] = {i, j, k}, mm[3] = {g.mx, g.my, g.mz};
     bool valid = true;
 #pragma unroll
@@ -357,9 +375,9 @@ __global__ void __launch_bounds__(BS) k_face_rc(G g, C3 uc, C3 u, const double *
 // rct = aj*(avg basis).Rc at interior cells (rhs.c:2663-2703)
 __global__ void __launch_bounds__(BS) k_rct(G g, C3 f0, C3 f1, C3 f2, C3 csi, C3 eta, C3 zet, const double *aj, V3 rct,
                                             long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int i, j, k;
-    g.dec(t, i, j, k);
+  (void)n;
+  CV_MARCH(g) {
+    const long t = g.at(i, j, k);
     if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
     const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
     const double rcx = (f0.x[t] - f0.x[ti]) + (f1.x[t] - f1.x[tj]) + (f2.x[t] - f2.x[tk]);
@@ -453,9 +471,9 @@ __device__ inline long prev_cell(const G &g, int i, int j, int k) {
 
 __global__ void __launch_bounds__(BS) k_ff1_faces(FFArgs A, V3 rhs, long n) {
   const G &g = A.g;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int i, j, k;
-    g.dec(t, i, j, k);
+  (void)n;
+  CV_MARCH(g) {
+    const long t = g.at(i, j, k);
     if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
     const double *p = A.p;
     // carried values from the previous cell in loop order
@@ -619,7 +637,7 @@ static C3 c3(double *const a[3]) { return C3{a[0], a[1], a[2]}; }
 int cv_contra2cart_dev(cv_ctx *c) {
   CvLevel &F = c->lev[c->nlev - 1];
   CvMetrics &M = c->m;
-  hipLaunchKernelGGL(k_contra2cart, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, gview(c), c3(c->ucont), c3(M.csi),
+  hipLaunchKernelGGL(k_contra2cart, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, gview(c), c3(c->ucont), c3(M.csi),
                      c3(M.eta), c3(M.zet), (const double *)F.nvert, v3(c->ucat), F.d.n);
   CV_CHECK(hipGetLastError());
   return CV_OK;
@@ -676,21 +694,21 @@ int cv_ff1_dev(cv_ctx *c) {
   const FM mj{c3(M.jcsi), c3(M.jeta), c3(M.jzet), M.jaj};
   const FM mk{c3(M.kcsi), c3(M.keta), c3(M.kzet), M.kaj};
   const V3 f0 = v3(c->fface), f1 = v3(c->fface + 3), f2 = v3(c->fface + 6);
-  hipLaunchKernelGGL(k_face_rc<0>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, mi,
+  hipLaunchKernelGGL(k_face_rc<0>, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A.g, uc, u, (const double *)F.nvert, mi,
                      nu, f0, F.d.n);
-  hipLaunchKernelGGL(k_face_rc<1>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, mj,
+  hipLaunchKernelGGL(k_face_rc<1>, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A.g, uc, u, (const double *)F.nvert, mj,
                      nu, f1, F.d.n);
-  hipLaunchKernelGGL(k_face_rc<2>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, uc, u, (const double *)F.nvert, mk,
+  hipLaunchKernelGGL(k_face_rc<2>, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A.g, uc, u, (const double *)F.nvert, mk,
                      nu, f2, F.d.n);
   {
     CvProfScope ps(c, CV_PROF_RCT, c->nlev - 1, 0, (double)F.d.n * 8.0 * (9 + 9 + 1 + 3));
-    hipLaunchKernelGGL(k_rct, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A.g, c3(c->fface), c3(c->fface + 3),
+    hipLaunchKernelGGL(k_rct, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A.g, c3(c->fface), c3(c->fface + 3),
                        c3(c->fface + 6), c3(M.csi), c3(M.eta), c3(M.zet), (const double *)M.aj, v3(c->rct), F.d.n);
   }
   if (F.d.xper || F.d.zper)
     hipLaunchKernelGGL(k_rct_periodic, dim3(nblk((long)A.g.my * A.g.mz + (long)A.g.mx * A.g.my)), dim3(BS), 0, c->stream,
                        A.g, v3(c->rct));
-  hipLaunchKernelGGL(k_ff1_faces, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, A, v3(c->rhs), F.d.n);
+  hipLaunchKernelGGL(k_ff1_faces, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A, v3(c->rhs), F.d.n);
   if (F.d.xper || F.d.zper) {
     long nf = (F.d.xper ? (long)(A.g.my - 2) * (A.g.mz - 2) : 0) + (F.d.zper ? (long)(A.g.my - 2) * (A.g.mx - 2) : 0);
     hipLaunchKernelGGL(k_ff1_periodic, dim3(nblk(nf)), dim3(BS), 0, c->stream, A, v3(c->rhs));
