@@ -1,0 +1,117 @@
+"""GPU parity of single routines and multi-step runs against the oracle,
+the mixed-precision path against the FP64 path, the smoother options and
+the not-converged status.
+
+Tolerances: FP64 routine outputs 1e-12 relative; fields after converged
+steps (both sides solved to rtol 1e-12) 1e-9 relative L2; mixed against
+FP64 pressure at rtol 1e-9: 1e-8 relative L2."""
+import numpy as np
+import pytest
+
+from curvib_old_amd import _lib, cases
+from curvib_old_amd.solver import from_case as gpu_case
+from oracle.pyoracle import from_case as oracle_case
+
+pytestmark = pytest.mark.gpu
+
+SPHERE = cases.ChannelCase(cells=(32, 24, 16), amp=0.1, beta=2.0, sphere=(0.5, 0.5, 0.5, 0.45), re=500.0, seed=21,
+                           dt=2e-4)
+CHANNEL = cases.ChannelCase(cells=(24, 16, 16), amp=0.05, beta=2.0, sphere=None, re=180.0, seed=22, dt=4e-4)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_rhs_volumeflux_divergence():
+    u, _ = gpu_case(SPHERE, mglevels=3)
+    o, _ = oracle_case(SPHERE, nlev=3)
+    try:
+        # Divergence of the initial fluxes (main.c:469-593): value and first argmax
+        dv, di = u.Divergence()
+        do = o.call("o_divergence")
+        assert abs(dv - do) <= 1e-12 * abs(do)
+        assert di == int(o.stats()["maxdiv_idx"])
+        # VolumeFlux_rev (poisson.c:3712-3944): corrected fluxes and the IB flux/area sums
+        fa = u.VolumeFlux_rev()
+        o.call("o_volumeflux_rev")
+        so = o.stats()
+        assert rel(u.get("ucont"), o.get("ucont")) < 1e-13
+        for q, k in enumerate(("ibm_flux0", "ibm_area0", "ibm_flux1", "ibm_area1")):
+            assert abs(fa[q] - so[k]) <= 1e-12 * max(abs(so[k]), 1e-12), (k, fa[q], so[k])
+        # PoissonRHS (poisson.c:1509-1576)
+        b = u.PoissonRHS()
+        o.call("o_rhs_into_b")
+        assert rel(b, o.get("b")) < 1e-12
+    finally:
+        u.close()
+        o.close()
+
+
+def test_three_steps_track_the_oracle():
+    u, _ = gpu_case(CHANNEL, mglevels=3)
+    o, _ = oracle_case(CHANNEL, nlev=3)
+    u.set_solver(rtol=1e-12, max_it=300, smoother="yline", pre_sweeps=3, post_sweeps=3, omega=0.9)
+    o.set_solver(rtol=1e-12, max_it=300)
+    try:
+        for s in range(3):
+            u.Flow_Solver()
+            o.call("o_flow_step")
+            for f in ("ucont", "p", "ucat"):
+                e = rel(u.get(f), o.get(f))
+                assert e < 1e-9, (s, f, e)
+    finally:
+        u.close()
+        o.close()
+
+
+def test_mixed_pressure_matches_fp64():
+    case = cases.ChannelCase(cells=(64, 48, 32), amp=0.1, beta=2.0, sphere=(0.5, 0.5, 0.5, 0.25), re=500.0,
+                             seed=23)
+    case = cases.ChannelCase(**{**case.__dict__, "dt": cases.stable_dt(case)})
+    opts = dict(rtol=1e-9, max_it=200, smoother="yline", pre_sweeps=3, post_sweeps=3, omega=0.9)
+    a, _ = gpu_case(case, mglevels=4, semi=(0, 1, 0), coarsen_auto=1)
+    b, _ = gpu_case(case, mglevels=4, semi=(0, 1, 0), coarsen_auto=1)
+    a.set_solver(precision="fp64", **opts)
+    b.set_solver(precision="mixed", **opts)
+    try:
+        for s in range(3):
+            sa, sb = a.Flow_Solver(), b.Flow_Solver()
+            assert sa["ksp_rnorm"] <= 1e-9 * sa["ksp_bnorm"] and sb["ksp_rnorm"] <= 1e-9 * sb["ksp_bnorm"]
+            e = rel(b.get("p"), a.get("p"))
+            assert e < 1e-8, (s, e)
+    finally:
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("smoother", ["jacobi", "yline"])
+def test_smoothers_reach_the_same_solution(smoother):
+    u, _ = gpu_case(SPHERE, mglevels=3)
+    o, _ = oracle_case(SPHERE, nlev=3)
+    u.set_solver(rtol=1e-12, max_it=500, smoother=smoother, pre_sweeps=2, post_sweeps=2,
+                 omega=0.8 if smoother == "jacobi" else 0.9)
+    o.set_solver(rtol=1e-12, max_it=500)
+    try:
+        u.FormBCS()
+        o.call("o_formbcs")
+        st = u.PoissonSolver_MG()
+        assert o.call("o_poisson_solve") == 0
+        assert st["converged"] == 1
+        assert rel(u.get("phi"), o.get("phi")) < 1e-8
+    finally:
+        u.close()
+        o.close()
+
+
+def test_not_converged_status():
+    u, _ = gpu_case(SPHERE, mglevels=3)
+    u.set_solver(rtol=1e-14, max_it=2)
+    try:
+        with pytest.raises(_lib.CvError) as e:
+            u.Flow_Solver()
+        assert e.value.code == _lib.CV_ERR_NOT_CONVERGED
+        st = u.Flow_Solver(allow_unconverged=True)
+        assert st["ksp_its"] == 2 and st["converged"] == 0
+    finally:
+        u.close()
