@@ -37,6 +37,7 @@ SYMBOLS = [
     "cv_poisson_rhs", "cv_poisson_apply", "cv_poisson_solve", "cv_update_pressure",
     "cv_projection", "cv_divergence", "cv_flow_step", "cv_mg_restrict", "cv_mg_prolong",
     "cv_nullspace_project", "cv_prof_select", "cv_prof_read",
+    "cv_set_momentum", "cv_default_momentum", "cv_imprk", "cv_calcrhs",
     "cv_ctx_bytes", "cv_server_create", "cv_server_submit", "cv_server_wait", "cv_server_step_ms",
     "cv_server_fetch", "cv_server_release", "cv_server_status", "cv_server_destroy",
 ]
@@ -63,10 +64,19 @@ class StepStats(C.Structure):
                 ("ksp_bnorm", C.c_double), ("rhs_sum", C.c_double), ("maxdiv", C.c_double),
                 ("maxdiv_idx", C.c_long), ("ibm_flux0", C.c_double), ("ibm_area0", C.c_double),
                 ("ibm_flux1", C.c_double), ("ibm_area1", C.c_double), ("t_mom", C.c_double),
-                ("t_p", C.c_double), ("t_pr", C.c_double)]
+                ("t_p", C.c_double), ("t_pr", C.c_double), ("imp_its", C.c_int), ("imp_lambda", C.c_double),
+                ("imp_normdU", C.c_double), ("imp_reldU", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+CV_MOM_RK, CV_MOM_IMPRK = 0, 1
+
+
+class MomentumOpts(C.Structure):
+    _fields_ = [("scheme", C.c_int), ("imp_max_it", C.c_int), ("imp_atol", C.c_double), ("imp_rtol", C.c_double),
+                ("cfl", C.c_double)]
 
 
 class CaseInfo(C.Structure):
@@ -119,6 +129,11 @@ def load():
     L.cv_nullspace_project.argtypes = [vp, C.c_int, dp]
     L.cv_prof_select.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.cv_prof_read.argtypes = [vp, C.POINTER(C.c_long), dp, dp, C.c_int]
+    L.cv_set_momentum.argtypes = [vp, C.POINTER(MomentumOpts)]
+    L.cv_default_momentum.argtypes = [C.POINTER(MomentumOpts)]
+    L.cv_default_momentum.restype = None
+    L.cv_imprk.argtypes = [vp, C.POINTER(StepStats)]
+    L.cv_calcrhs.argtypes = [vp, dp]
     L.cv_ctx_bytes.argtypes = [C.POINTER(GridDesc), C.c_int]
     L.cv_ctx_bytes.restype = C.c_long
     L.cv_server_create.argtypes = [C.c_int, C.c_long, C.c_int, C.POINTER(C.c_void_p)]
@@ -132,7 +147,8 @@ def load():
     L.cv_server_destroy.argtypes = [vp]
     for s in SYMBOLS:
         f = getattr(L, s)
-        if s not in ("cv_default_solver", "cv_device_bytes", "cv_last_error", "cv_ctx_bytes"):
+        if s not in ("cv_default_solver", "cv_default_momentum", "cv_device_bytes", "cv_last_error",
+                     "cv_ctx_bytes"):
             f.restype = C.c_int
     L.cv_last_error.restype = C.c_int
     _lib = L

@@ -75,6 +75,21 @@ extern "C" void cv_default_solver(cv_solver_opts *o) {
 
 extern "C" long cv_device_bytes(const cv_ctx *c) { return c ? (long)c->bytes_alloc : 0; }
 
+extern "C" void cv_default_momentum(cv_momentum_opts *o) {
+  o->scheme = CV_MOM_RK;
+  o->imp_max_it = 50;
+  o->imp_atol = 1e-7;
+  o->imp_rtol = 1e-4;
+  o->cfl = 0.2;
+}
+
+extern "C" int cv_set_momentum(cv_ctx *c, const cv_momentum_opts *o) {
+  if (!c || !o || (o->scheme != CV_MOM_RK && o->scheme != CV_MOM_IMPRK) || o->imp_max_it < 1 || o->cfl <= 0)
+    return CV_ERR_ARG;
+  c->mom = *o;
+  return CV_OK;
+}
+
 static int check_desc(const cv_grid_desc *d) {
   if (!d) return 0;
   if (d->IM < 4 || d->JM < 4 || d->KM < 4) return 0;
@@ -136,14 +151,14 @@ static int level_dims(const cv_grid_desc *desc, CvDims *dims) {
 // device bytes of one context: per level (FP64, 22 arrays: nvert, aj, 9 g, dinv, 3
 // coordinates, 3 line factors, line scratch, x b r; FP32: 9 g, aj, dinv, 3
 // line factors, line scratch, x b r (18 arrays); the operator word), the finest-level
-// metrics (40 doubles), state (28 doubles) and the 3n exchange staging,
+// metrics (40 doubles), state (34 doubles) and the 3n exchange staging,
 // reduction scratch and the
 // Krylov space of FGMRES(restart)
 static size_t plan_bytes(const CvDims *dims, int nlev, int restart) {
   size_t b = 0;
   for (int l = 0; l < nlev; l++) b += (size_t)dims[l].n * (22 * 8 + 18 * 4 + 4);
   const size_t n = (size_t)dims[nlev - 1].n;
-  b += n * 8 * (40 + 28 + 3);
+  b += n * 8 * (40 + 34 + 3);
   b += (size_t)(1 << 16) * 8 + 128 * 8 + 34 * 1024 * 8;
   if (restart < 1) restart = 1;
   if (restart > 31) restart = 31;
@@ -175,6 +190,7 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
   memcpy(c->bctype, desc->bctype, sizeof(c->bctype));
   c->ren = desc->ren; c->dt = desc->dt; c->st = desc->st > 0 ? desc->st : 1.0;
   cv_default_solver(&c->opt);
+  cv_default_momentum(&c->mom);
   for (int l = 0; l < c->nlev; l++) c->lev[l].d = dims[l];
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -223,7 +239,8 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     double **sc[] = {&M.aj, &M.iaj, &M.jaj, &M.kaj};
     for (auto s : sc) { if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; *s = (double *)p; }
     M.aj = c->lev[c->nlev - 1].aj ? M.aj : M.aj;
-    double **st3[] = {c->ucont, c->ucont_o, c->ucat, c->rhs, c->rct, c->fface, c->fface + 3, c->fface + 6};
+    double **st3[] = {c->ucont, c->ucont_o, c->ucat, c->rhs, c->rct, c->fface, c->fface + 3, c->fface + 6,
+                      c->ucont_rm1, c->pucont};
     for (auto v : st3)
       for (int q = 0; q < 3; q++) {
         if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail;
@@ -337,6 +354,7 @@ extern "C" int cv_get_field(cv_ctx *c, int f, int level, double *out) {
 
 extern "C" int cv_put_field(cv_ctx *c, int f, int level, const double *in) {
   if (!c || !in) return CV_ERR_ARG;
+  if (f == CV_F_UCONT) c->steps_done = 0;     // loaded fluxes start a run (BDF1 first)
   if (level < 0) level = c->nlev - 1;
   if (level >= c->nlev) return CV_ERR_ARG;
   int dof;

@@ -83,6 +83,9 @@ struct cv_ctx {
   CvMetrics m;
   // fine-level state (planar)
   double *ucont[3], *ucont_o[3], *ucat[3], *rhs[3], *p, *phi, *bvec;
+  double *ucont_rm1[3], *pucont[3];   // ImpRK: U^{n-1} and the pseudo-time base
+  int steps_done;                      // steps since Ucont was last loaded (BDF1 on the first)
+  cv_momentum_opts mom;
   double *rct[3];
   double *stage;          // 3n staging of the host field exchange (Cmpnts AoS)
   double *fface[9];       // Visc - Conv flux of the i, j, k face families (x|y|z each)
@@ -139,6 +142,8 @@ int cv_volumeflux_rev_dev(cv_ctx *c, double *fa4);
 int cv_poisson_rhs_dev(cv_ctx *c, double *rhs_sum);
 int cv_ff1_dev(cv_ctx *c);
 int cv_rk_dev(cv_ctx *c);
+int cv_imprk_dev(cv_ctx *c, cv_step_stats *st);
+int cv_calcrhs_dev(cv_ctx *c);
 
 __device__ inline int cv_wrap(int i, int m, int per) {
   return per ? (i == m - 1 ? 1 : (i == 0 ? m - 2 : i)) : i;

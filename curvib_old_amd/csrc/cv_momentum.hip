@@ -619,6 +619,91 @@ __global__ void k_rk_update(double a, C3 uo, C3 r, V3 u, long n) {
   }
 }
 
+
+// CalcRHS (implicitsolver.c:6449-6741) after FormFunction1: Rhs += (-1/dt)
+// du with du = 1.5 u - 2 u_o + 0.5 u_rm1 (BDF2) or u - u_o (first step),
+// at every storage point, then the zero set of the reference's interior and
+// boundary-plane masks (all of them only zero entries, so they commute)
+__global__ void __launch_bounds__(BS) k_calcrhs(G g, int bdf2, double idt, C3 u, C3 uo, C3 um, const double *nv,
+                                                V3 r, int yper) {
+  const int mx = g.mx, my = g.my, mz = g.mz, xper = g.xper, zper = g.zper;
+  CV_MARCH(g) {
+    const long t = g.at(i, j, k);
+    double d0, d1, d2;
+    if (bdf2) {
+      d0 = u.x[t] * 1.5; d0 = d0 + (-2.) * uo.x[t]; d0 = d0 + 0.5 * um.x[t];
+      d1 = u.y[t] * 1.5; d1 = d1 + (-2.) * uo.y[t]; d1 = d1 + 0.5 * um.y[t];
+      d2 = u.z[t] * 1.5; d2 = d2 + (-2.) * uo.z[t]; d2 = d2 + 0.5 * um.z[t];
+    } else {
+      d0 = -1. * uo.x[t] + u.x[t];
+      d1 = -1. * uo.y[t] + u.y[t];
+      d2 = -1. * uo.z[t] + u.z[t];
+    }
+    double rx = r.x[t] + idt * d0, ry = r.y[t] + idt * d1, rz = r.z[t] + idt * d2;
+    bool zx = false, zy = false, zz = false;
+    const long sj = g.sj, sk = g.sk;
+    auto nvx = [&]() { return nv[t] + nv[t + 1] > 0.1; };
+    auto nvy = [&]() { return nv[t] + nv[t + sj] > 0.1; };
+    auto nvz = [&]() { return nv[t] + nv[t + sk] > 0.1; };
+    // interior cells (6524-6538)
+    if (i >= 1 && i <= mx - 2 && j >= 1 && j <= my - 2 && k >= 1 && k <= mz - 2) {
+      if (nvx() || (i > mx - 3 && !xper)) zx = true;
+      if (nvy() || (j > my - 3 && !yper)) zy = true;
+      if (nvz() || (k > mz - 3 && !zper)) zz = true;
+    }
+    // i planes (6541-6583): j, k in [0, m-2]
+    if (j <= my - 2 && k <= mz - 2) {
+      if (i == 0) {
+        if (!xper) zx = zy = zz = true;
+        else { if (nvx()) zx = true; if (nvy()) zy = true; if (nvz()) zz = true; }
+      }
+      if (i == mx - 2 && !xper) zx = true;
+      if (i == mx - 1) zx = zy = zz = true;
+    }
+    // j planes (6587-6636): i, k in [0, m-2]
+    if (i <= mx - 2 && k <= mz - 2) {
+      if (j == 0) {
+        if (!yper) zx = zy = zz = true;
+        else { if (nvx()) zx = true; if (nvy()) zy = true; if (nvz()) zz = true; }
+      }
+      if (j == my - 2 && !yper) zy = true;
+      if (j == my - 1) zx = zy = zz = true;
+    }
+    // k planes (6639-6690): i, j in [0, m-2]
+    if (i <= mx - 2 && j <= my - 2) {
+      if (k == 0) {
+        if (!zper) zx = zy = zz = true;
+        else { if (nvx()) zx = true; if (nvy()) zy = true; if (nvz()) zz = true; }
+      }
+      if (k == mz - 2 && !zper) zz = true;
+      if (k == mz - 1) zx = zy = zz = true;
+    }
+    r.x[t] = zx ? 0. : rx;
+    r.y[t] = zy ? 0. : ry;
+    r.z[t] = zz ? 0. : rz;
+  }
+}
+
+// max |a - b| (diff) or max |a| over three planar arrays; part[block]
+__global__ void __launch_bounds__(BS) k_absmax3(C3 a, C3 b, int diff, long n, double *part) {
+  __shared__ double sh[BS / 64];
+  double m = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    double v0 = a.x[t], v1 = a.y[t], v2 = a.z[t];
+    if (diff) { v0 = b.x[t] - v0; v1 = b.y[t] - v1; v2 = b.z[t] - v2; }
+    m = fmax(m, fmax(fabs(v0), fmax(fabs(v1), fabs(v2))));
+  }
+  m = cv_block_max(m, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ void k_final_max(const double *part, int nb, double *out) {
+  __shared__ double sh[BS / 64];
+  double m = 0.0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) m = fmax(m, part[q]);
+  m = cv_block_max(m, sh);
+  if (threadIdx.x == 0) out[0] = m;
+}
 }  // namespace
 
 // ====================================================================
@@ -739,6 +824,90 @@ int cv_rk_dev(cv_ctx *c) {
   return CV_OK;
 }
 
+// CalcRHS: FormFunction1 into Rhs, then the time derivative and the masks
+int cv_calcrhs_dev(cv_ctx *c) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  G g = gview(c);
+  int rc = cv_ff1_dev(c);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_calcrhs, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, g,
+                     c->steps_done > 0 ? 1 : 0, -1. / c->dt, c3(c->ucont), c3(c->ucont_o), c3(c->ucont_rm1),
+                     (const double *)F.nvert, v3(c->rhs), c->bctype[2] == 7 ? 1 : 0);
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+// max |a - b| (b non-null) or max |a| over the fine level, to the host
+static int absmax3(cv_ctx *c, double *const a[3], double *const b[3], double *out) {
+  const long n = c->lev[c->nlev - 1].d.n;
+  const C3 ca = c3(a), cb = b ? c3(b) : ca;
+  hipLaunchKernelGGL(k_absmax3, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, ca, cb, b ? 1 : 0, n, c->part);
+  hipLaunchKernelGGL(k_final_max, dim3(1), dim3(BS), 0, c->stream, (const double *)c->part, CV_RED_GRID, c->dsc + 120);
+  CV_CHECK(hipMemcpyAsync(c->red_host, c->dsc + 120, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  *out = c->red_host[0];
+  return CV_OK;
+}
+
+static void copy3(cv_ctx *c, double *const dst[3], double *const src[3]) {
+  const long n = c->lev[c->nlev - 1].d.n;
+  for (int q = 0; q < 3; q++)
+    hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(BS), 0, c->stream, dst[q], (const double *)src[q], n);
+}
+
+// ImpRK (implicitsolver.c:7094-7256), one block: pseudo-time RK stages on
+// CalcRHS from the base pUcont, stop on ||pU - U||_inf (absolute or
+// relative to the first pseudo iteration), halve lambda and reject the
+// iteration when both ||Rhs||_inf and ||dU||_inf grew
+int cv_imprk_dev(cv_ctx *c, cv_step_stats *st) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
+  int rc = cv_ghost_update(c, 1);   // InflowFlux/OutflowFlux have nothing to do for bctype 1 and 7
+  if (rc) return rc;
+  for (int q = 0; q < 3; q++)
+    hipLaunchKernelGGL(k_fill, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, c->rhs[q], 0.0, F.d.n);
+  copy3(c, c->pucont, c->ucont);
+  if ((rc = cv_calcrhs_dev(c))) return rc;
+  double normF0, normF1 = 1000., normdU1 = 1000.;
+  if ((rc = absmax3(c, c->rhs, nullptr, &normF0))) return rc;
+  double lambda = c->mom.cfl;
+  double normdU = 10., reldU = 1., normdU0 = 0., normdU_bk = 0., reldU_bk = 1., normF_bk = 0.;
+  int pseudot = 0;
+  while (((normdU > c->mom.imp_atol && reldU > c->mom.imp_rtol) || pseudot < 1) && pseudot < c->mom.imp_max_it) {
+    pseudot++;
+    for (int s = 0; s < 4; s++) {
+      const double a = lambda * alfa[s] * c->dt * c->st;
+      hipLaunchKernelGGL(k_rk_update, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, a, c3(c->pucont), c3(c->rhs),
+                         v3(c->ucont), F.d.n);
+      if ((rc = cv_ghost_update(c, 1))) return rc;
+      if ((rc = cv_calcrhs_dev(c))) return rc;
+    }
+    normdU1 = normdU_bk;
+    if ((rc = absmax3(c, c->ucont, c->pucont, &normdU_bk))) return rc;   // dU = pU - U
+    if (pseudot == 1) { normdU0 = normdU_bk; reldU_bk = 1.; }
+    else if (normdU0 > 1.e-10) reldU_bk = normdU_bk / normdU0;
+    if (pseudot > 1) normF1 = normF_bk;
+    if ((rc = absmax3(c, c->rhs, nullptr, &normF_bk))) return rc;
+    if (pseudot == 1) normF0 = normF_bk;
+    normdU = normdU_bk;
+    reldU = reldU_bk;
+    if (!std::isfinite(normdU) || !std::isfinite(normF_bk)) return CV_ERR_NONFINITE;
+    if (pseudot > 1 && normF_bk > normF1 && normdU_bk > normdU1 && lambda > 0.001) {
+      lambda = 0.5 * lambda;
+      pseudot--;
+      copy3(c, c->ucont, c->pucont);
+    } else {
+      copy3(c, c->pucont, c->ucont);
+      normF1 = normF_bk;
+      normdU1 = normdU_bk;
+    }
+  }
+  (void)normF0;
+  if (st) { st->imp_its = pseudot; st->imp_lambda = lambda; st->imp_normdU = normdU; st->imp_reldU = reldU; }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
 // ====================================================================
 // C ABI
 // ====================================================================
@@ -790,10 +959,13 @@ extern "C" int cv_flow_step(cv_ctx *c, cv_step_stats *st) {
   cv_step_stats s{};
   hipEvent_t e[4];
   for (int q = 0; q < 4; q++) CV_CHECK(hipEventCreate(&e[q]));
-  for (int q = 0; q < 3; q++)      // Ucont_o <- Ucont (main.c:1127 / 1065)
-    hipLaunchKernelGGL(k_copy, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, c->ucont_o[q], (const double *)c->ucont[q], F.d.n);
+  // Ucont_rm1 <- Ucont_o <- Ucont (main.c:1131-1132), done at the step start
+  if (c->steps_done > 0) copy3(c, c->ucont_rm1, c->ucont_o);
+  copy3(c, c->ucont_o, c->ucont);
   CV_CHECK(hipEventRecord(e[0], c->stream));
-  if ((rc = cv_rk_dev(c))) return rc;
+  if (c->mom.scheme == CV_MOM_IMPRK) rc = cv_imprk_dev(c, &s);
+  else rc = cv_rk_dev(c);
+  if (rc) return rc;
   CV_CHECK(hipEventRecord(e[1], c->stream));
   rc = cv_poisson_solve_dev(c, &s);
   if (rc && rc != CV_ERR_NOT_CONVERGED) return rc;
@@ -809,7 +981,27 @@ extern "C" int cv_flow_step(cv_ctx *c, cv_step_stats *st) {
   s.t_mom = ms[0] * 1e-3; s.t_p = ms[1] * 1e-3; s.t_pr = ms[2] * 1e-3;
   for (int q = 0; q < 4; q++) (void)hipEventDestroy(e[q]);
   c->last = s;
+  c->steps_done++;
   if (st) *st = s;
   if (!std::isfinite(s.maxdiv)) return CV_ERR_NONFINITE;
   return conv_rc;
+}
+
+extern "C" int cv_imprk(cv_ctx *c, cv_step_stats *st) {
+  int rc = ready(c);
+  if (rc) return rc;
+  cv_step_stats s{};
+  rc = cv_imprk_dev(c, &s);
+  if (rc) return rc;
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  if (st) *st = s;
+  return CV_OK;
+}
+
+extern "C" int cv_calcrhs(cv_ctx *c, double *rhs_out) {
+  int rc = ready(c);
+  if (rc) return rc;
+  if ((rc = cv_calcrhs_dev(c))) return rc;
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return rhs_out ? cv_get_field(c, CV_F_RHS, -1, rhs_out) : CV_OK;
 }

@@ -26,5 +26,26 @@ __device__ inline double cv_block_sum(double v, double *sh) {
   return s;
 }
 
+__device__ inline double cv_wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// max over the block; result valid in thread 0.  sh: >= blockDim/64 doubles.
+__device__ inline double cv_block_max(double v, double *sh) {
+  v = cv_wave_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double m = v;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int q = 0; q < nw; q++) m = fmax(m, sh[q]);
+  }
+  return m;
+}
+
 // final pass: out[q] = sum_b part[q*nb + b], one block per q, fixed order
 __global__ void cv_k_final_sum(const double *part, int nb, double *out, int nq);

@@ -81,6 +81,15 @@ class UserCtx:
             setattr(self.opts, k, v)
         check(_lib.load().cv_set_solver(self.h, C.byref(self.opts)), "cv_set_solver")
 
+    def set_momentum(self, scheme="rk", imp_max_it=50, imp_atol=1e-7, imp_rtol=1e-4, cfl=0.2):
+        """Momentum scheme of Flow_Solver (solvers.c:1022-1040): "rk" explicit
+        RungeKutta or "imprk" dual time stepping, with the -imp_MAX_IT,
+        -imp_atol, -imp_rtol and -cfl options."""
+        o = _lib.MomentumOpts()
+        o.scheme = {"rk": _lib.CV_MOM_RK, "imprk": _lib.CV_MOM_IMPRK}[scheme]
+        o.imp_max_it, o.imp_atol, o.imp_rtol, o.cfl = int(imp_max_it), float(imp_atol), float(imp_rtol), float(cfl)
+        check(_lib.load().cv_set_momentum(self.h, C.byref(o)), "cv_set_momentum")
+
     # -- inputs (grid.dat, nvert, restart fields) -----------------------
     def set_grid(self, xyz):
         a = np.ascontiguousarray(xyz, dtype=np.float64)
@@ -134,6 +143,16 @@ class UserCtx:
 
     def RungeKutta(self):
         check(_lib.load().cv_rungekutta(self.h), "RungeKutta")
+
+    def ImpRK(self):
+        st = _lib.StepStats()
+        check(_lib.load().cv_imprk(self.h, C.byref(st)), "ImpRK")
+        return st.as_dict()
+
+    def CalcRHS(self):
+        out = np.empty(self._shape("rhs", -1))
+        check(_lib.load().cv_calcrhs(self.h, _dp(out)), "CalcRHS")
+        return out
 
     def VolumeFlux_rev(self):
         out = np.zeros(4)

@@ -78,7 +78,22 @@ typedef struct {
   long maxdiv_idx;
   double ibm_flux0, ibm_area0, ibm_flux1, ibm_area1;
   double t_mom, t_p, t_pr;  /* seconds, device-event timed */
+  int imp_its;              /* ImpRK: pseudo-time iterations taken ("!!norm of dU", implicitsolver.c:7190) */
+  double imp_lambda;        /* ImpRK: final pseudo-time step factor */
+  double imp_normdU, imp_reldU;  /* ImpRK: last ||dU||_inf and its ratio to the first */
 } cv_step_stats;
+
+/* momentum scheme of the fractional step (-imp, main.c; Flow_Solver
+ * solvers.c:1022-1040) */
+#define CV_MOM_RK    0   /* explicit 4-stage RungeKutta, implicitsolver.c:32-119        */
+#define CV_MOM_IMPRK 1   /* dual time stepping ImpRK, implicitsolver.c:7094-7256 with
+                            CalcRHS 6449-6741 (BDF2 after the first step of a run)      */
+typedef struct {
+  int scheme;              /* CV_MOM_* */
+  int imp_max_it;          /* -imp_MAX_IT (default 50, main.c:21) */
+  double imp_atol, imp_rtol;  /* -imp_atol 1e-7, -imp_rtol 1e-4 (main.c:26) */
+  double cfl;              /* initial pseudo-time factor lambda (-cfl, default 0.2, init.c:54) */
+} cv_momentum_opts;
 
 /* field identifiers for cv_get_field / cv_put_field */
 enum {
@@ -95,6 +110,8 @@ int  cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out);
 int  cv_ctx_destroy(cv_ctx *ctx);
 int  cv_set_solver(cv_ctx *ctx, const cv_solver_opts *opts);
 void cv_default_solver(cv_solver_opts *opts);
+int  cv_set_momentum(cv_ctx *ctx, const cv_momentum_opts *opts);
+void cv_default_momentum(cv_momentum_opts *opts);
 int  cv_last_error(char *buf, int len);
 long cv_device_bytes(const cv_ctx *ctx);
 
@@ -118,6 +135,8 @@ int cv_contra2cart(cv_ctx *ctx);                         /* Contra2Cart        r
 int cv_form_bcs(cv_ctx *ctx);                            /* FormBCS            bcs.c:4436-5888   */
 int cv_form_function1(cv_ctx *ctx, double *rhs_out);     /* FormFunction1      rhs.c:2567-3351   */
 int cv_rungekutta(cv_ctx *ctx);                          /* RungeKutta         implicitsolver.c:32-119 */
+int cv_imprk(cv_ctx *ctx, cv_step_stats *st);            /* ImpRK              implicitsolver.c:7094-7256 */
+int cv_calcrhs(cv_ctx *ctx, double *rhs_out);            /* CalcRHS            implicitsolver.c:6449-6741 */
 int cv_volumeflux_rev(cv_ctx *ctx, double *flux_area4);  /* VolumeFlux_rev     poisson.c:3712-3944 */
 int cv_poisson_rhs(cv_ctx *ctx, double *b_out);          /* PoissonRHS         poisson.c:1509-1576 */
 int cv_poisson_apply(cv_ctx *ctx, int level, const double *x, double *y); /* MatMult(A) of PoissonLHSNew poisson.c:600-1507 */
@@ -127,7 +146,8 @@ int cv_poisson_solve(cv_ctx *ctx, cv_step_stats *st);    /* PoissonSolver_MG   p
 int cv_update_pressure(cv_ctx *ctx);                     /* UpdatePressure     poisson.c:2908-3248 */
 int cv_projection(cv_ctx *ctx);                          /* Projection         poisson.c:2281-2906 */
 int cv_divergence(cv_ctx *ctx, double *maxdiv, long *idx); /* Divergence       main.c:469-593    */
-int cv_flow_step(cv_ctx *ctx, cv_step_stats *st);        /* Flow_Solver        solvers.c:944-1156 + main.c:1123-1145 */
+int cv_flow_step(cv_ctx *ctx, cv_step_stats *st);        /* Flow_Solver        solvers.c:944-1156 + main.c:1123-1145;
+                                                            momentum by the scheme of cv_set_momentum */
 
 /* operator plug-in points of the solve, exported for tests (MatShell
  * callbacks poisson.c:1863-1875, nullspace callback poisson.c:1950-1954) */

@@ -63,6 +63,12 @@ typedef struct {
   double ren, dt, st;
   /* fine-level state */
   ofield ucont, ucont_o, ucat, p, phi, nvert_o, rhs, ubcs;
+  ofield ucont_rm1, pucont;   /* ImpRK: U^{n-1} (BDF2) and the pseudo-time base */
+  int steps_done;             /* steps since the fluxes were last loaded (BDF1 on the first) */
+  /* momentum scheme: 0 RungeKutta (implicitsolver.c:32), 1 ImpRK (7094);
+   * ImpRK options -imp_MAX_IT -imp_atol -imp_rtol -cfl (main.c:21-26, init.c:54) */
+  int mom_scheme, imp_max_it; double imp_atol, imp_rtol, cfl;
+  int imp_its; double imp_lambda, imp_normdU, imp_reldU;
   ofield b;           /* Poisson right-hand side */
   /* diagnostics */
   double rhs_sum;     /* "Summation RHS" (poisson.c:1554-1567) */
@@ -94,6 +100,11 @@ void o_contra2cart(octx *c);
 void o_formbcs(octx *c);
 void o_formfunction1(octx *c, ofield *rhs);
 void o_rungekutta(octx *c);
+void o_calcrhs(octx *c);
+void o_imprk(octx *c);
+void o_set_momentum(octx *c, int scheme, int imp_max_it, double imp_atol, double imp_rtol, double cfl);
+void o_imp_stats(octx *c, double *out);
+void o_set_steps_done(octx *c, int n);
 void o_volumeflux_rev(octx *c);
 void o_poisson_rhs(octx *c, ofield *b);
 void o_poisson_lhs(octx *c, int level);

@@ -10,7 +10,7 @@ static ofield *lookup(octx *c, const char *name, int level) {
   if (level >= c->nlev) return NULL;
   olevel *L = &c->lev[level];
 #define F(nm, ptr) if (!strcmp(name, nm)) return ptr;
-  F("ucont", &c->ucont) F("ucont_o", &c->ucont_o) F("ucat", &c->ucat) F("p", &c->p)
+  F("ucont", &c->ucont) F("ucont_o", &c->ucont_o) F("ucont_rm1", &c->ucont_rm1) F("ucat", &c->ucat) F("p", &c->p)
   F("phi", &c->phi) F("rhs", &c->rhs) F("b", &c->b) F("ubcs", &c->ubcs)
   F("coor", &L->coor) F("csi", &L->csi) F("eta", &L->eta) F("zet", &L->zet) F("aj", &L->aj)
   F("icsi", &L->icsi) F("ieta", &L->ieta) F("izet", &L->izet) F("iaj", &L->iaj)
@@ -33,6 +33,7 @@ int o_put(octx *c, const char *name, int level, const double *in) {
   ofield *f = lookup(c, name, level);
   if (!f) return -1;
   o_put_field(f, in);
+  if (f == &c->ucont) c->steps_done = 0;          /* loaded fluxes start a run: BDF1 first */
   olevel *L = &c->lev[level < 0 ? c->nlev - 1 : level];
   o_g2l(f, L->per);
   return 0;
@@ -70,6 +71,16 @@ int o_csr_get(octx *c, int level, int *rowptr, int *col, double *val) {
   memcpy(col, L->col, L->rowptr[L->nrows] * sizeof(int));
   memcpy(val, L->val, L->rowptr[L->nrows] * sizeof(double));
   return 0;
+}
+
+void o_set_momentum(octx *c, int scheme, int imp_max_it, double imp_atol, double imp_rtol, double cfl) {
+  c->mom_scheme = scheme; c->imp_max_it = imp_max_it; c->imp_atol = imp_atol; c->imp_rtol = imp_rtol; c->cfl = cfl;
+}
+
+void o_set_steps_done(octx *c, int n) { c->steps_done = n; }
+
+void o_imp_stats(octx *c, double *out) {
+  out[0] = c->imp_its; out[1] = c->imp_lambda; out[2] = c->imp_normdU; out[3] = c->imp_reldU; out[4] = c->steps_done;
 }
 
 void o_stats(octx *c, double *out) {

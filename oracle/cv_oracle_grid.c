@@ -144,8 +144,9 @@ octx *o_create(int IM, int JM, int KM, const int bctype[6], int nlev,
     level_alloc(L);
   }
   olevel *F = &c->lev[nlev - 1];
-  ofield *v3[] = {&c->ucont, &c->ucont_o, &c->ucat, &c->rhs, &c->ubcs};
-  for (size_t q = 0; q < 5; q++) of_alloc(v3[q], F->mx, F->my, F->mz, 3);
+  ofield *v3[] = {&c->ucont, &c->ucont_o, &c->ucat, &c->rhs, &c->ubcs, &c->ucont_rm1, &c->pucont};
+  for (size_t q = 0; q < 7; q++) of_alloc(v3[q], F->mx, F->my, F->mz, 3);
+  c->mom_scheme = 0; c->imp_max_it = 50; c->imp_atol = 1e-7; c->imp_rtol = 1e-4; c->cfl = 0.2;
   of_alloc(&c->p, F->mx, F->my, F->mz, 1);
   of_alloc(&c->phi, F->mx, F->my, F->mz, 1);
   of_alloc(&c->nvert_o, F->mx, F->my, F->mz, 1);
@@ -157,7 +158,7 @@ void o_destroy(octx *c) {
   if (!c) return;
   for (int l = 0; l < c->nlev; l++) level_free(&c->lev[l]);
   ofield *v[] = {&c->ucont, &c->ucont_o, &c->ucat, &c->rhs, &c->ubcs, &c->p, &c->phi,
-                 &c->nvert_o, &c->b};
+                 &c->nvert_o, &c->b, &c->ucont_rm1, &c->pucont};
   for (size_t q = 0; q < sizeof(v) / sizeof(v[0]); q++) of_free(v[q]);
   free(c);
 }

@@ -632,13 +632,170 @@ void o_rungekutta(octx *c) {
 }
 
 /* ------------------------------------------------------------------ */
+/* CalcRHS (implicitsolver.c:6449-6741): FormFunction1 minus the BDF    */
+/* time derivative, then the explicit boundary masks                     */
+/* ------------------------------------------------------------------ */
+void o_calcrhs(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  const int xper = L->bctype[0] == 7, yper = L->bctype[2] == 7, zper = L->bctype[4] == 7;
+  ofield *r = &c->rhs, *nvf = &L->nvert;
+  o_formfunction1(c, r);
+  /* du = 1.5 u - 2 u_o + 0.5 u_rm1 except on the first step of a run
+   * (COEF_TIME_ACCURACY = 1.5), then Rhs += (-1/dt) du (6497-6511) */
+  const int bdf2 = c->steps_done > 0;
+  const double idt = -1. / c->dt;
+  for (int k = 0; k < mz; k++)
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) {
+        const ocmp *u = &V(&c->ucont, k, j, i), *uo = &V(&c->ucont_o, k, j, i), *um = &V(&c->ucont_rm1, k, j, i);
+        ocmp *rr = &V(r, k, j, i);
+        double d[3];
+        const double uu[3] = {u->x, u->y, u->z}, oo[3] = {uo->x, uo->y, uo->z}, mm[3] = {um->x, um->y, um->z};
+        for (int q = 0; q < 3; q++) {
+          if (bdf2) { d[q] = uu[q] * COEF_TIME_ACCURACY; d[q] = d[q] + (-2.) * oo[q]; d[q] = d[q] + 0.5 * mm[q]; }
+          else d[q] = -1. * oo[q] + uu[q];
+        }
+        rr->x = rr->x + idt * d[0]; rr->y = rr->y + idt * d[1]; rr->z = rr->z + idt * d[2];
+      }
+#define NV(kk, jj, ii) S(nvf, kk, jj, ii)
+  /* interior faces (6524-6538): IB neighbours and, on a non-periodic axis,
+   * the last face (i > mx-3 etc.; mz_end = mz-3 without the outlet type 8) */
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        ocmp *rr = &V(r, k, j, i);
+        if (NV(k, j, i) + NV(k, j, i + 1) > 0.1 || (i > mx - 3 && !xper)) rr->x = 0.;
+        if (NV(k, j, i) + NV(k, j + 1, i) > 0.1 || (j > my - 3 && !yper)) rr->y = 0.;
+        if (NV(k, j, i) + NV(k + 1, j, i) > 0.1 || (k > mz - 3 && !zper)) rr->z = 0.;
+      }
+  /* boundary planes (6541-6690); every branch only zeroes entries, so the
+   * passes commute.  In the periodic upper-plane branches the reference
+   * indexes with a stale loop variable: the first pass repeats the lower
+   * plane's masks, the rest act on the plane that is zeroed right after. */
+  for (int k = 0; k < mz - 1; k++)
+    for (int j = 0; j < my - 1; j++) {
+      ocmp *a = &V(r, k, j, 0);
+      if (!xper) { a->x = a->y = a->z = 0.; }
+      else {
+        if (NV(k, j, 0) + NV(k, j, 1) > 0.1) a->x = 0.;
+        if (NV(k, j, 0) + NV(k, j + 1, 0) > 0.1) a->y = 0.;
+        if (NV(k, j, 0) + NV(k + 1, j, 0) > 0.1) a->z = 0.;
+      }
+      if (!xper) V(r, k, j, mx - 2).x = 0.;
+      ocmp *b = &V(r, k, j, mx - 1);
+      b->x = b->y = b->z = 0.;
+    }
+  for (int k = 0; k < mz - 1; k++)
+    for (int i = 0; i < mx - 1; i++) {
+      ocmp *a = &V(r, k, 0, i);
+      if (!yper) { a->x = a->y = a->z = 0.; }
+      else {
+        if (NV(k, 0, i) + NV(k, 0, i + 1) > 0.1) a->x = 0.;
+        if (NV(k, 0, i) + NV(k, 1, i) > 0.1) a->y = 0.;
+        if (NV(k, 0, i) + NV(k + 1, 0, i) > 0.1) a->z = 0.;
+      }
+      if (!yper) V(r, k, my - 2, i).y = 0.;
+      ocmp *b = &V(r, k, my - 1, i);
+      b->x = b->y = b->z = 0.;
+    }
+  for (int j = 0; j < my - 1; j++)
+    for (int i = 0; i < mx - 1; i++) {
+      ocmp *a = &V(r, 0, j, i);
+      if (!zper) { a->x = a->y = a->z = 0.; }
+      else {
+        if (NV(0, j, i) + NV(0, j, i + 1) > 0.1) a->x = 0.;
+        if (NV(0, j, i) + NV(0, j + 1, i) > 0.1) a->y = 0.;
+        if (NV(0, j, i) + NV(1, j, i) > 0.1) a->z = 0.;
+      }
+      if (!zper) V(r, mz - 2, j, i).z = 0.;
+      ocmp *b = &V(r, mz - 1, j, i);
+      b->x = b->y = b->z = 0.;
+    }
+#undef NV
+  o_g2l(r, L->per);
+}
+
+/* max |a - b| over the storage points of two Cmpnts fields (b NULL: |a|) */
+static double absmax3(const octx *c, const ofield *a, const ofield *b) {
+  const olevel *L = &c->lev[c->nlev - 1];
+  double m = 0.;
+  for (int k = 0; k < L->mz; k++)
+    for (int j = 0; j < L->my; j++)
+      for (int i = 0; i < L->mx; i++) {
+        const ocmp *p = &V(a, k, j, i);
+        double v[3] = {p->x, p->y, p->z};
+        if (b) { const ocmp *q = &V(b, k, j, i); v[0] = q->x - p->x; v[1] = q->y - p->y; v[2] = q->z - p->z; }
+        for (int t = 0; t < 3; t++) if (fabs(v[t]) > m) m = fabs(v[t]);
+      }
+  return m;
+}
+
+/* ImpRK (implicitsolver.c:7094-7256): dual time stepping, four pseudo-time
+ * RK stages per pseudo iteration on CalcRHS, stopped on the infinity norm
+ * of the pseudo-time change; lambda (initially -cfl) halves and the
+ * iteration is rejected when both the residual and the change grow */
+void o_imprk(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
+  o_formbcs(c);                     /* InflowFlux/OutflowFlux: nothing for bctype 1 and 7 */
+  of_zero(&c->rhs);
+  of_copy(&c->pucont, &c->ucont);
+  o_calcrhs(c);
+  double normF0 = absmax3(c, &c->rhs, NULL), normF1 = 1000., normdU1 = 1000.;
+  double lambda = c->cfl;
+  double normdU = 10., reldU = 1., normdU0 = 0., normdU_bk = 0., reldU_bk = 1., normF_bk = 0.;
+  (void)normF0;
+  int pseudot = 0;
+  while (((normdU > c->imp_atol && reldU > c->imp_rtol) || pseudot < 1) && pseudot < c->imp_max_it) {
+    pseudot++;
+    for (int s = 0; s < 4; s++) {
+      const double a = lambda * alfa[s] * c->dt * c->st;
+      for (int k = 0; k < L->mz; k++)
+        for (int j = 0; j < L->my; j++)
+          for (int i = 0; i < L->mx; i++) {
+            ocmp *u = &V(&c->ucont, k, j, i);
+            const ocmp *pu = &V(&c->pucont, k, j, i), *r = &V(&c->rhs, k, j, i);
+            u->x = a * r->x + pu->x; u->y = a * r->y + pu->y; u->z = a * r->z + pu->z;
+          }
+      o_g2l(&c->ucont, L->per);
+      o_formbcs(c);
+      o_calcrhs(c);
+    }
+    normdU1 = normdU_bk;
+    normdU_bk = absmax3(c, &c->ucont, &c->pucont);   /* dU = pU - U */
+    if (pseudot == 1) { normdU0 = normdU_bk; reldU_bk = 1.; }
+    else if (normdU0 > 1.e-10) reldU_bk = normdU_bk / normdU0;
+    if (pseudot > 1) normF1 = normF_bk;
+    normF_bk = absmax3(c, &c->rhs, NULL);
+    if (pseudot == 1) normF0 = normF_bk;
+    normdU = normdU_bk;
+    reldU = reldU_bk;
+    if (pseudot > 1 && normF_bk > normF1 && normdU_bk > normdU1 && lambda > 0.001) {
+      lambda = 0.5 * lambda;
+      pseudot--;
+      of_copy(&c->ucont, &c->pucont);
+      o_g2l(&c->ucont, L->per);
+    } else {
+      of_copy(&c->pucont, &c->ucont);
+      normF1 = normF_bk;
+      normdU1 = normdU_bk;
+    }
+  }
+  c->imp_its = pseudot; c->imp_lambda = lambda; c->imp_normdU = normdU; c->imp_reldU = reldU;
+}
+
+/* ------------------------------------------------------------------ */
 /* one fractional step (solvers.c:944-1156)                             */
 /* ------------------------------------------------------------------ */
 void o_flow_step(octx *c) {
   olevel *L = &c->lev[c->nlev - 1];
+  /* U_rm1 <- U_o <- U (main.c:1131-1132), done here at the step start */
+  if (c->steps_done > 0) of_copy(&c->ucont_rm1, &c->ucont_o);
   of_copy(&c->ucont_o, &c->ucont);          /* main.c:1127 / 1065 */
   double t0 = now_s();
-  o_rungekutta(c);
+  if (c->mom_scheme == 1) o_imprk(c);
+  else o_rungekutta(c);
   double t1 = now_s();
   o_poisson_solve(c);
   double t2 = now_s();
@@ -648,4 +805,5 @@ void o_flow_step(octx *c) {
   double t3 = now_s();
   o_divergence(c);
   c->t_mom = t1 - t0; c->t_p = t2 - t1; c->t_pr = t3 - t2;
+  c->steps_done++;
 }

@@ -57,6 +57,11 @@ def lib():
         L.o_restrict.argtypes = [vp, C.c_int, dp, dp]
         L.o_interp.argtypes = [vp, C.c_int, dp, dp]
         L.o_matvec_level.argtypes = [vp, C.c_int, dp, dp]
+        L.o_set_momentum.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.o_imp_stats.argtypes = [vp, dp]
+        L.o_set_steps_done.argtypes = [vp, C.c_int]
+        for nm in ("o_calcrhs", "o_imprk"):
+            getattr(L, nm).argtypes = [vp]
         _lib = L
     return _lib
 
@@ -136,6 +141,20 @@ class Oracle:
         keys = ("its", "rnorm", "bnorm", "rhs_sum", "maxdiv", "maxdiv_idx", "ibm_flux0", "ibm_area0",
                 "ibm_flux1", "ibm_area1", "drops", "t_mom", "t_p", "t_pr", "threads", "multinull")
         return dict(zip(keys, out.tolist()))
+
+    def set_momentum(self, scheme="rk", imp_max_it=50, imp_atol=1e-7, imp_rtol=1e-4, cfl=0.2):
+        """scheme "rk" (RungeKutta) or "imprk" (ImpRK dual time stepping)."""
+        lib().o_set_momentum(self.h, {"rk": 0, "imprk": 1}[scheme], int(imp_max_it), float(imp_atol),
+                             float(imp_rtol), float(cfl))
+
+    def lib_set_steps(self, n):
+        """Steps since the fluxes were loaded (0: BDF1 in CalcRHS)."""
+        lib().o_set_steps_done(self.h, int(n))
+
+    def imp_stats(self):
+        out = np.zeros(5)
+        lib().o_imp_stats(self.h, _dp(out))
+        return dict(zip(("its", "lambda", "normdU", "reldU", "steps_done"), out.tolist()))
 
     def csr(self, level):
         mx, my, mz = self.dims(level)[:3]

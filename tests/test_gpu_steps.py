@@ -159,3 +159,32 @@ def test_input_deck_matches_direct_setup(tmp_path):
     finally:
         d.close()
         u.close()
+
+
+@pytest.mark.parametrize("name", ["channel", "sphere"])
+def test_imprk_tracks_the_oracle(name):
+    """ImpRK (implicitsolver.c:7094-7256) as the momentum scheme of two
+    fractional steps (BDF1, then BDF2): same pseudo-iteration counts and
+    lambda as the oracle, fields to 1e-9."""
+    case = {"channel": CHANNEL, "sphere": SPHERE}[name]
+    u, _ = gpu_case(case, mglevels=3)
+    o, _ = oracle_case(case, nlev=3)
+    imp = dict(imp_max_it=30, imp_atol=1e-7, imp_rtol=1e-4, cfl=0.3)
+    u.set_momentum("imprk", **imp)
+    o.set_momentum("imprk", **imp)
+    u.set_solver(rtol=1e-12, max_it=300)
+    o.set_solver(rtol=1e-12, max_it=300)
+    try:
+        for s in range(2):
+            st = u.Flow_Solver()
+            o.call("o_flow_step")
+            so = o.imp_stats()
+            assert st["imp_its"] == int(so["its"]), (s, st["imp_its"], so)
+            assert st["imp_lambda"] == so["lambda"]
+            assert abs(st["imp_normdU"] - so["normdU"]) <= 1e-9 * so["normdU"]
+            for f in ("ucont", "p", "ucat"):
+                e = rel(u.get(f), o.get(f))
+                assert e < 1e-9, (s, f, e)
+    finally:
+        u.close()
+        o.close()

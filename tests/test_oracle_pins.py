@@ -14,6 +14,10 @@ ships no tests, golden vectors or fixtures for this path):
                 exact (-8 nu dy dz), convection vanishes
   Projection    one fractional step leaves a divergence at the residual
                 scale
+  CalcRHS       -du/dt exact for fluxes linear in time (BDF1 and BDF2),
+                BDF2 coefficients, the boundary zero set
+  ImpRK         the pseudo-time iteration meets its stopping test and
+                reduces its own residual
 
 Each check is chosen so that a dropped term, a wrong sign or index or a
 transposed operand fails it."""
@@ -236,3 +240,89 @@ def test_projection_leaves_residual_scale_divergence():
     assert st["rnorm"] <= 1e-12 * st["bnorm"] * 1.0001
     # residual of the Poisson equation in divergence units: |b| scale * rtol
     assert st["maxdiv"] < 1e-6
+
+
+def _ff1_and_calcrhs(o):
+    o.call("o_formbcs")
+    o.call("o_ff1_into_rhs")
+    ff1 = o.get("rhs")
+    o.call("o_calcrhs")
+    return ff1, o.get("rhs")
+
+
+@pytest.mark.parametrize("bdf2", [0, 1])
+def test_calcrhs_time_derivative_exact_for_linear_time(bdf2):
+    """CalcRHS (implicitsolver.c:6449-6741) adds -du/dt: with fluxes linear
+    in time (U = U0 + d, U_o = U0, U_rm1 = U0 - d) both BDF1 and BDF2 give
+    du/dt = d/dt exactly, so CalcRHS - FormFunction1 = -d/dt on every face
+    the masks leave open."""
+    case = cases.ChannelCase(cells=(12, 10, 8), amp=0.05, seed=9, dt=3e-4)
+    o, nv = oracle_for(case)
+    u0 = o.get("ucont")
+    d = 1e-3 * np.random.default_rng(3).standard_normal(u0.shape)
+    o.put("ucont_o", u0)
+    o.put("ucont_rm1", u0 - d)
+    o.put("ucont", u0 + d)
+    o.lib_set_steps(bdf2)
+    ff1, cr = _ff1_and_calcrhs(o)
+    delta = cr - ff1
+    inner = (slice(1, -1), slice(1, -2), slice(1, -1))     # interior, away from the top wall faces
+    sel = (cr[inner] != 0.0)
+    np.testing.assert_allclose(delta[inner][sel], (-d / case.dt)[inner][sel], rtol=1e-9, atol=1e-12)
+    assert sel.mean() > 0.9
+
+
+def test_calcrhs_bdf2_coefficients():
+    """A step U0 -> U0 + d after U0 - e -> U0: BDF2 gives
+    (1.5 (U0 + d) - 2 U0 + 0.5 (U0 - e)) / dt = (1.5 d - 0.5 e) / dt."""
+    case = cases.ChannelCase(cells=(12, 10, 8), amp=0.05, seed=10, dt=3e-4)
+    o, nv = oracle_for(case)
+    u0 = o.get("ucont")
+    rng = np.random.default_rng(4)
+    d = 1e-3 * rng.standard_normal(u0.shape)
+    e = 1e-3 * rng.standard_normal(u0.shape)
+    o.put("ucont_o", u0)
+    o.put("ucont_rm1", u0 - e)
+    o.put("ucont", u0 + d)
+    o.lib_set_steps(1)
+    ff1, cr = _ff1_and_calcrhs(o)
+    inner = (slice(1, -1), slice(1, -2), slice(1, -1))
+    sel = cr[inner] != 0.0
+    np.testing.assert_allclose((cr - ff1)[inner][sel], (-(1.5 * d - 0.5 * e) / case.dt)[inner][sel],
+                               rtol=1e-9, atol=1e-12)
+
+
+def test_calcrhs_boundary_zero_set():
+    case = cases.ChannelCase(cells=(12, 10, 8), amp=0.05, seed=11, dt=3e-4)
+    o, nv = oracle_for(case)
+    u0 = o.get("ucont")
+    o.put("ucont_o", u0 - 1e-3)
+    o.put("ucont", u0)
+    _, cr = _ff1_and_calcrhs(o)
+    mz, my, mx, _ = cr.shape
+    assert np.all(cr[:mz - 1, :my - 1, mx - 1] == 0.0)     # i = mx-1
+    assert np.all(cr[:mz - 1, 0, :mx - 1] == 0.0)          # lower wall plane
+    assert np.all(cr[:mz - 1, my - 1, :mx - 1] == 0.0)     # upper ghost plane
+    assert np.all(cr[:mz - 1, my - 2, :mx - 1, 1] == 0.0)  # upper wall faces, normal component
+    assert np.all(cr[mz - 1, :my - 1, :mx - 1] == 0.0)     # k = mz-1
+    # periodic i = 0 plane without a body keeps its x-faces (FormFunction1's periodic faces)
+    assert np.count_nonzero(cr[1:mz - 1, 1:my - 2, 0, 0]) > 0
+
+
+def test_imprk_reduces_its_residual():
+    """ImpRK's pseudo-time iteration converges toward the implicit step:
+    the stopping test is met before -imp_MAX_IT and the final CalcRHS
+    residual is far below the first one."""
+    case = cases.ChannelCase(cells=(16, 12, 8), amp=0.05, sphere=(0.5, 0.5, 0.5, 0.4), seed=12, dt=2e-3, re=100.0)
+    o, nv = oracle_for(case, nlev=2)
+    o.set_momentum("imprk", imp_max_it=200, imp_atol=1e-9, imp_rtol=1e-6, cfl=0.2)
+    o.put("ucont_o", o.get("ucont"))
+    o.lib_set_steps(0)
+    o.call("o_formbcs")
+    o.call("o_calcrhs")
+    r0 = np.abs(o.get("rhs")).max()
+    o.call("o_imprk")
+    st = o.imp_stats()
+    assert st["its"] < 200
+    assert st["normdU"] <= 1e-9 or st["reldU"] <= 1e-6
+    assert np.abs(o.get("rhs")).max() < 1e-2 * r0
