@@ -264,6 +264,14 @@ def main():
     tmx, itsmx, pr = timed_steps(umx, "mixed", args.warmup, args.steps, prof=("resid", -1, 1))
 
     acc = [] if args.no_accuracy else accuracy_pass(u64, umx, arrays, args.steps)
+    # host-array exchange of a drop-in caller that hands Ucont in and takes
+    # Ucont and P back every step (pageable numpy buffers, PCIe)
+    t0 = time.perf_counter()
+    uc = umx.get("ucont")
+    umx.put("ucont", uc)
+    pp = umx.get("p")
+    xchg_ms = 1e3 * (time.perf_counter() - t0)
+    del uc, pp
     u64.close()
     umx.close()
     del arrays
@@ -321,6 +329,8 @@ def main():
         "mixed": {"steps_per_s": sps_mx, "ms_per_step": 1e3 * tmx / args.steps, "its": itsmx,
                   "cell_updates_per_s": ncell * sps_mx},
         "mixed_over_fp64": sps_mx / sps_64,
+        "host_exchange_ms_per_step": xchg_ms,
+        "pcie_inclusive_mixed_ms_per_step": 1e3 * tmx / args.steps + xchg_ms,
         "accuracy": acc,
         "ensemble_latency_ms": (None if srv is None else {
             "serving_mixed_p50": srv["serving_mixed"]["step_latency"].get("p50_ms"),
