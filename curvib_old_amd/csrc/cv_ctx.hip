@@ -151,14 +151,14 @@ static int level_dims(const cv_grid_desc *desc, CvDims *dims) {
 // device bytes of one context: per level (FP64, 22 arrays: nvert, aj, 9 g, dinv, 3
 // coordinates, 3 line factors, line scratch, x b r; FP32: 9 g, aj, dinv, 3
 // line factors, line scratch, x b r (18 arrays); the operator word), the finest-level
-// metrics (40 doubles), state (34 doubles) and the 3n exchange staging,
-// reduction scratch and the
-// Krylov space of FGMRES(restart)
+// metrics (40 doubles), state (25 doubles: 7 vectors of 3, p phi b w) and
+// the 3n exchange staging, reduction scratch and the Krylov space of
+// FGMRES(restart)
 static size_t plan_bytes(const CvDims *dims, int nlev, int restart) {
   size_t b = 0;
   for (int l = 0; l < nlev; l++) b += (size_t)dims[l].n * (22 * 8 + 18 * 4 + 4);
   const size_t n = (size_t)dims[nlev - 1].n;
-  b += n * 8 * (40 + 34 + 3);
+  b += n * 8 * (40 + 25 + 3);
   b += (size_t)(1 << 16) * 8 + 128 * 8 + 34 * 1024 * 8;
   if (restart < 1) restart = 1;
   if (restart > 31) restart = 31;
@@ -239,8 +239,7 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     double **sc[] = {&M.aj, &M.iaj, &M.jaj, &M.kaj};
     for (auto s : sc) { if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; *s = (double *)p; }
     M.aj = c->lev[c->nlev - 1].aj ? M.aj : M.aj;
-    double **st3[] = {c->ucont, c->ucont_o, c->ucat, c->rhs, c->rct, c->fface, c->fface + 3, c->fface + 6,
-                      c->ucont_rm1, c->pucont};
+    double **st3[] = {c->ucont, c->ucont_o, c->ucat, c->rhs, c->rct, c->ucont_rm1, c->pucont};
     for (auto v : st3)
       for (int q = 0; q < 3; q++) {
         if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail;

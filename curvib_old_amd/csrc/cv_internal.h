@@ -88,7 +88,6 @@ struct cv_ctx {
   cv_momentum_opts mom;
   double *rct[3];
   double *stage;          // 3n staging of the host field exchange (Cmpnts AoS)
-  double *fface[9];       // Visc - Conv flux of the i, j, k face families (x|y|z each)
   // Krylov space
   double *V, *Z, *w;      // (restart+1)*n, restart*n, n
   double *red;            // reduction scratch (device)
@@ -154,6 +153,24 @@ __device__ inline int cv_wrap(int i, int m, int per) {
 // contiguous run of tiles (x fastest), so tiles that share halo lines share
 // that XCD's L2.  A bijection on [0, gridDim.x*gridDim.y*gridDim.z).
 #define CV_NXCD 8
+// Interleaved runs: tile t belongs to run t / R; run r goes to XCD r mod 8,
+// so every XCD holds runs of R consecutive tiles (which share halo lines in
+// its L2) spread evenly over the whole grid.  Blocks past the last full
+// round of 8 runs keep their own index.  A bijection on the block range.
+__device__ inline void cv_xcd_runs(int R, int &tx, int &ty, int &tz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned nb = gx * gy * gridDim.z;
+  const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned full = (nb / (CV_NXCD * (unsigned)R)) * CV_NXCD * (unsigned)R;
+  unsigned t = b;
+  if (b < full) {
+    const unsigned x = b % CV_NXCD, l = b / CV_NXCD;
+    t = (x + CV_NXCD * (l / (unsigned)R)) * (unsigned)R + l % (unsigned)R;
+  }
+  tx = (int)(t % gx);
+  ty = (int)((t / gx) % gy);
+  tz = (int)(t / (gx * gy));
+}
 __device__ inline void cv_xcd_tile(int &tx, int &ty, int &tz) {
   const unsigned gx = gridDim.x, gy = gridDim.y;
   const unsigned nb = gx * gy * gridDim.z;

@@ -229,96 +229,144 @@ __global__ void k_wall_flux_zero(G g, int bx, int bz, V3 u) {
 }
 
 // ------------------------------------------------------------------
-// Convection: QUICK face flux (rhs.c:1636-1897)
+// Face fluxes: QUICK convection (rhs.c:1636-1897) and the viscous flux
+// (rhs.c:2062-2486).  The flux rules are written once over an accessor
+// that returns ucat and the nvert classes at offsets from the face's cell,
+// so the same arithmetic runs on global loads and on LDS tiles.
 // ------------------------------------------------------------------
-__device__ inline void quick(double um, double up, long a2, long a1, long a0, long am, C3 u, double *f) {
-  const double coef = 0.125;
-  f[0] = um * (coef * (-u.x[a2] - 2. * u.x[a1] + 3. * u.x[a0]) + u.x[a1]) + up * (coef * (-u.x[am] - 2. * u.x[a0] + 3. * u.x[a1]) + u.x[a0]);
-  f[1] = um * (coef * (-u.y[a2] - 2. * u.y[a1] + 3. * u.y[a0]) + u.y[a1]) + up * (coef * (-u.y[am] - 2. * u.y[a0] + 3. * u.y[a1]) + u.y[a0]);
-  f[2] = um * (coef * (-u.z[a2] - 2. * u.z[a1] + 3. * u.z[a0]) + u.z[a1]) + up * (coef * (-u.z[am] - 2. * u.z[a0] + 3. * u.z[a1]) + u.z[a0]);
+// nvert classes the face rules test
+#define NVB_LT01 1u   // nv < 0.1
+#define NVB_GT01 2u   // nv > 0.1
+#define NVB_GT7 4u    // nv > 7 (inner blank)
+#define NVB_SOL 8u    // 0.5 < nv < 7 (solid)
+__device__ inline unsigned nv_bits(double v) {
+  return (v < 0.1 ? NVB_LT01 : 0u) | (v > 0.1 ? NVB_GT01 : 0u) | (v > 7. ? NVB_GT7 : 0u) |
+         ((v > 0.5 && v < 7.) ? NVB_SOL : 0u);
 }
 
-// convective flux through face (i,j,k) of axis d
-__device__ inline void conv_flux(const G &g, int d, int i, int j, int k, C3 uc, C3 u, const double *nv, double *f) {
-  const double innerblank = 7.;
-  if (d == 0) {
-    const long t = g.at(i, j, k);
-    double ucon = uc.x[t] * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
-    long A2 = g.atq(i + 2, j, k), A1 = g.atq(i + 1, j, k), A0 = t, AM = g.atq(i - 1, j, k);
-    double np1 = nv[g.atq(i + 1, j, k)];
-    double nm1 = (i > 0 || g.xper) ? nv[g.atq(i - 1, j, k)] : 0.0;
-    if (i > 0 && i < g.mx - 2 && (np1 < 0.1 || np1 > innerblank) && (nm1 < 0.1 || nm1 > innerblank)) quick(um, up, A2, A1, A0, AM, u, f);
-    else if (i == 0 || nm1 > 0.1) {
-      if (g.xper && i == 0 && (nm1 < 0.1 && np1 < 0.1)) quick(um, up, A2, A1, A0, AM, u, f);
-      else quick(um, up, A2, A1, A0, A0, u, f);
-    } else if (i == g.mx - 2 || np1 > 0.1) {
-      if (g.xper && i == g.mx - 2 && (nm1 < 0.1 && np1 < 0.1)) quick(um, up, A2, A1, A0, AM, u, f);
-      else quick(um, up, A1, A1, A0, AM, u, f);
-    } else { f[0] = f[1] = f[2] = 0.0; }
-  } else if (d == 1) {
-    const long t = g.at(i, j, k);
-    double ucon = uc.y[t] * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
-    double np1 = nv[t + g.sj];
-    double nm1 = j > 0 ? nv[t - g.sj] : 0.0;
-    long A2 = j + 2 < g.my ? t + 2 * g.sj : t, A1 = t + g.sj, A0 = t, AM = j > 0 ? t - g.sj : t;
-    if (j > 0 && j < g.my - 2 && (np1 < 0.1 || np1 > innerblank) && (nm1 < 0.1 || nm1 > innerblank)) quick(um, up, A2, A1, A0, AM, u, f);
-    else if (j == 0 || nm1 > 0.1) quick(um, up, A2, A1, A0, A0, u, f);
-    else if (j == g.my - 2 || np1 > 0.1) quick(um, up, A1, A1, A0, AM, u, f);
-    else { f[0] = f[1] = f[2] = 0.0; }
-  } else {
-    const long t = g.at(i, j, k);
-    double ucon = uc.z[t] * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
-    long A2 = g.atq(i, j, k + 2), A1 = g.atq(i, j, k + 1), A0 = t, AM = g.atq(i, j, k - 1);
-    double np1 = nv[g.atq(i, j, k + 1)];
-    double nm1 = (k > 0 || g.zper) ? nv[g.atq(i, j, k - 1)] : 0.0;
-    if (k > 0 && k < g.mz - 2 && (np1 < 0.1 || np1 > innerblank) && (nm1 < 0.1 || nm1 > innerblank)) quick(um, up, A2, A1, A0, AM, u, f);
-    else if (k < g.mz - 2 && (k == 0 || nm1 > 0.1)) {
-      if (g.zper && k == 0 && (nm1 < 0.1 && np1 < 0.1)) quick(um, up, A2, A1, A0, AM, u, f);
-      else quick(um, up, A2, A1, A0, A0, u, f);
-    } else if (k > 0 && (k == g.mz - 2 || np1 > 0.1)) {
-      if (g.zper && k == g.mz - 2 && (nm1 < 0.1 && np1 < 0.1)) quick(um, up, A2, A1, A0, AM, u, f);
-      else quick(um, up, A1, A1, A0, AM, u, f);
-    } else { f[0] = f[1] = f[2] = 0.0; }
+// unit offset along axis a, scaled by s
+#define EOFS(a, s) ((a) == 0 ? (s) : 0), ((a) == 1 ? (s) : 0), ((a) == 2 ? (s) : 0)
+
+// ucat and nvert at offsets of the face's cell, from global memory
+struct GAcc {
+  const G *g;
+  C3 u;
+  const double *nv;
+  int i, j, k;
+  __device__ long idx(int di, int dj, int dk) const { return g->atq(i + di, j + dj, k + dk); }
+  __device__ void u3(int di, int dj, int dk, double *v) const {
+    const long p = idx(di, dj, dk);
+    v[0] = u.x[p]; v[1] = u.y[p]; v[2] = u.z[p];
   }
+  __device__ unsigned nb(int di, int dj, int dk) const { return nv_bits(nv[idx(di, dj, dk)]); }
+};
+
+// the same from an LDS tile of planes k-1, k, k+1 (row pitch TW); the
+// k+2 values of the face's own column come in registers
+template <int TW>
+struct TAcc {
+  const double *P[3][3];          // [plane k-1, k, k+1][x, y, z]
+  const unsigned char *N[3];      // nvert classes of the planes
+  int o;                          // tile offset of the face's cell
+  double kp2[3];                  // ucat at (i, j, k+2)
+  __device__ void u3(int di, int dj, int dk, double *v) const {
+    if (dk == 2) { v[0] = kp2[0]; v[1] = kp2[1]; v[2] = kp2[2]; return; }
+    const int e = o + dj * TW + di;
+    v[0] = P[dk + 1][0][e]; v[1] = P[dk + 1][1][e]; v[2] = P[dk + 1][2][e];
+  }
+  __device__ unsigned nb(int di, int dj, int dk) const { return N[dk + 1][o + dj * TW + di]; }
+};
+
+__device__ inline void quick3(double um, double up, const double *w2, const double *v1, const double *v0,
+                              const double *wm, double *f) {
+  const double coef = 0.125;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    f[c] = um * (coef * (-w2[c] - 2. * v1[c] + 3. * v0[c]) + v1[c]) + up * (coef * (-wm[c] - 2. * v0[c] + 3. * v1[c]) + v0[c]);
 }
 
-// ------------------------------------------------------------------
-// Viscous face flux (rhs.c:2062-2486)
-// ------------------------------------------------------------------
+// convective flux through face (i,j,k) of axis D; ucd is the face's
+// contravariant flux.  The far upwind points are +2 (or +1 / the cell
+// itself near walls and blanked cells) and -1 (or the cell itself).
+template <int D, class Acc>
+__device__ inline void conv_acc(const G &g, int i, int j, int k, double ucd, const Acc &a, double *f) {
+  const double ucon = ucd * 0.5, up = ucon + fabs(ucon), um = ucon - fabs(ucon);
+  const int c = D == 0 ? i : D == 1 ? j : k;
+  const int m = D == 0 ? g.mx : D == 1 ? g.my : g.mz;
+  const bool per = D == 0 ? g.xper != 0 : D == 2 ? g.zper != 0 : false;
+  const unsigned np1 = a.nb(EOFS(D, 1));
+  const unsigned nm1 = (c > 0 || per) ? a.nb(EOFS(D, -1)) : NVB_LT01;
+  const bool fr1 = (np1 & (NVB_LT01 | NVB_GT7)) != 0, frm = (nm1 & (NVB_LT01 | NVB_GT7)) != 0;
+  int s2, sm;   // far points: s2 in {2, 1, 0}, sm in {-1, 0}
+  if (D == 1) {
+    const int A2 = j + 2 < g.my ? 2 : 0, AM = j > 0 ? -1 : 0;
+    if (j > 0 && j < g.my - 2 && fr1 && frm) { s2 = A2; sm = AM; }
+    else if (j == 0 || (nm1 & NVB_GT01)) { s2 = A2; sm = 0; }
+    else if (j == g.my - 2 || (np1 & NVB_GT01)) { s2 = 1; sm = AM; }
+    else { f[0] = f[1] = f[2] = 0.0; return; }
+  } else {
+    const bool both = (nm1 & NVB_LT01) && (np1 & NVB_LT01);
+    if (c > 0 && c < m - 2 && fr1 && frm) { s2 = 2; sm = -1; }
+    else if ((D == 0 || c < m - 2) && (c == 0 || (nm1 & NVB_GT01))) {
+      s2 = 2; sm = (per && c == 0 && both) ? -1 : 0;
+    } else if ((D == 0 || c > 0) && (c == m - 2 || (np1 & NVB_GT01))) {
+      const bool wrap = per && c == m - 2 && both;
+      s2 = wrap ? 2 : 1; sm = -1;
+    } else { f[0] = f[1] = f[2] = 0.0; return; }
+  }
+  double v1[3], v0[3], w2[3], wm[3];
+  a.u3(EOFS(D, 1), v1);
+  a.u3(0, 0, 0, v0);
+  if (s2 == 2) a.u3(EOFS(D, 2), w2);
+  else for (int q = 0; q < 3; q++) w2[q] = s2 == 1 ? v1[q] : v0[q];
+  if (sm == -1) a.u3(EOFS(D, -1), wm);
+  else for (int q = 0; q < 3; q++) wm[q] = v0[q];
+  quick3(um, up, w2, v1, v0, wm, f);
+}
+
 struct FM { C3 c, e, z; const double *aj; };
 
-__device__ inline void visc_flux(const G &g, int d, int i, int j, int k, C3 u, const double *nv, FM m, double nu, double *f) {
-  const double solid = 0.5, innerblank = 7.;
-  const long t = g.at(i, j, k);
-  const long so[3] = {1, g.sj, g.sk};
-  const long up = t + so[d];
-  double du[3][3];
+// viscous flux through face t of axis D; tangential differences switch to
+// one-sided next to solid cells
+template <int D, class Acc>
+__device__ inline void visc_acc(long t, const Acc &a, const FM &m, double nu, double *f) {
+  double du[3][3], ut[3], uu[3];
+  a.u3(0, 0, 0, ut);
+  a.u3(EOFS(D, 1), uu);
+#pragma unroll
   for (int tt = 0; tt < 3; tt++) {
-    if (tt == d) {
-      du[0][tt] = u.x[up] - u.x[t]; du[1][tt] = u.y[up] - u.y[t]; du[2][tt] = u.z[up] - u.z[t];
+    if (tt == D) {
+      for (int q = 0; q < 3; q++) du[q][tt] = uu[q] - ut[q];
       continue;
     }
-    const long s = so[tt];
-    double n1 = nv[t + s], n2 = nv[up + s], n3 = nv[t - s], n4 = nv[up - s];
-    bool hi = (n1 > solid && n1 < innerblank) || (n2 > solid && n2 < innerblank);
-    bool lo = (n3 > solid && n3 < innerblank) || (n4 > solid && n4 < innerblank);
-    long A, B, C, D; double fac;
-    if (hi) { A = up; B = t; C = up - s; D = t - s; fac = 0.5; }
-    else if (lo) { A = up + s; B = t + s; C = up; D = t; fac = 0.5; }
-    else { A = up + s; B = t + s; C = up - s; D = t - s; fac = 0.25; }
-    du[0][tt] = (u.x[A] + u.x[B] - u.x[C] - u.x[D]) * fac;
-    du[1][tt] = (u.y[A] + u.y[B] - u.y[C] - u.y[D]) * fac;
-    du[2][tt] = (u.z[A] + u.z[B] - u.z[C] - u.z[D]) * fac;
+    const unsigned n1 = a.nb(EOFS(tt, 1)), n3 = a.nb(EOFS(tt, -1));
+    const unsigned n2 = a.nb((D == 0) + (tt == 0), (D == 1) + (tt == 1), (D == 2) + (tt == 2));
+    const unsigned n4 = a.nb((D == 0) - (tt == 0), (D == 1) - (tt == 1), (D == 2) - (tt == 2));
+    const bool hi = ((n1 | n2) & NVB_SOL) != 0, lo = ((n3 | n4) & NVB_SOL) != 0;
+    double tp[3], tm[3], pp[3], pm[3];   // t+s, t-s, up+s, up-s
+    if (!hi) {
+      a.u3(EOFS(tt, 1), tp);
+      a.u3((D == 0) + (tt == 0), (D == 1) + (tt == 1), (D == 2) + (tt == 2), pp);
+    }
+    if (hi || !lo) {
+      a.u3(EOFS(tt, -1), tm);
+      a.u3((D == 0) - (tt == 0), (D == 1) - (tt == 1), (D == 2) - (tt == 2), pm);
+    }
+    for (int q = 0; q < 3; q++) {
+      if (hi) du[q][tt] = (uu[q] + ut[q] - pm[q] - tm[q]) * 0.5;
+      else if (lo) du[q][tt] = (pp[q] + tp[q] - uu[q] - ut[q]) * 0.5;
+      else du[q][tt] = (pp[q] + tp[q] - pm[q] - tm[q]) * 0.25;
+    }
   }
   const double csi0 = m.c.x[t], csi1 = m.c.y[t], csi2 = m.c.z[t];
   const double eta0 = m.e.x[t], eta1 = m.e.y[t], eta2 = m.e.z[t];
   const double zet0 = m.z.x[t], zet1 = m.z.y[t], zet2 = m.z.z[t];
   double n0, n1, n2, g11, g21, g31;
-  if (d == 0) {
+  if (D == 0) {
     n0 = csi0; n1 = csi1; n2 = csi2;
     g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2; g21 = eta0 * csi0 + eta1 * csi1 + eta2 * csi2;
     g31 = zet0 * csi0 + zet1 * csi1 + zet2 * csi2;
-  } else if (d == 1) {
+  } else if (D == 1) {
     n0 = eta0; n1 = eta1; n2 = eta2;
     g11 = csi0 * eta0 + csi1 * eta1 + csi2 * eta2; g21 = eta0 * eta0 + eta1 * eta1 + eta2 * eta2;
     g31 = zet0 * eta0 + zet1 * eta1 + zet2 * eta2;
@@ -346,47 +394,179 @@ __device__ inline void visc_flux(const G &g, int d, int i, int j, int k, C3 u, c
   f[2] += (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nu);
 }
 
-// Visc - Conv flux through every face of family D: face index along D in
-// [0, m_D - 2], the other two indices interior (the faces rhs.c:1636-1897
-// and rhs.c:2062-2486 evaluate into Fp1/Fp2/Fp3)
-template <int D>
-__global__ void __launch_bounds__(BS) k_face_rc(G g, C3 uc, C3 u, const double *nv, FM m, double nu, V3 F, long n) {
-  (void)n;
-  CV_MARCH(g) {
+// Visc - Conv flux through face (i,j,k) of family D (the faces
+// rhs.c:1636-1897 and rhs.c:2062-2486 evaluate into Fp1/Fp2/Fp3)
+template <int D, class Acc>
+__device__ inline void face_rc(const G &g, int i, int j, int k, long t, double ucd, const Acc &a, const FM &m, double nu,
+                               double *F) {
+  double cf[3], vf[3];
+  conv_acc<D>(g, i, j, k, ucd, a, cf);
+  visc_acc<D>(t, a, m, nu, vf);
+  F[0] = vf[0] - cf[0];
+  F[1] = vf[1] - cf[1];
+  F[2] = vf[2] - cf[2];
+}
+
+// rct = aj*(avg basis).Rc of interior cell t from the face-flux
+// differences along i, j and k (rhs.c:2663-2703)
+__device__ inline void rct_out(const G &g, long t, const double *d0, const double *d1, const double *d2, C3 csi, C3 eta,
+                               C3 zet, const double *aj, V3 rct) {
+  const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
+  const double rcx = d0[0] + d1[0] + d2[0];
+  const double rcy = d0[1] + d1[1] + d2[1];
+  const double rcz = d0[2] + d1[2] + d2[2];
+  const double A = aj[t];
+  rct.x[t] = A * (0.5 * (csi.x[t] + csi.x[ti]) * rcx + 0.5 * (csi.y[t] + csi.y[ti]) * rcy + 0.5 * (csi.z[t] + csi.z[ti]) * rcz);
+  rct.y[t] = A * (0.5 * (eta.x[t] + eta.x[tj]) * rcx + 0.5 * (eta.y[t] + eta.y[tj]) * rcy + 0.5 * (eta.z[t] + eta.z[tj]) * rcz);
+  rct.z[t] = A * (0.5 * (zet.x[t] + zet.x[tk]) * rcx + 0.5 * (zet.y[t] + zet.y[tk]) * rcy + 0.5 * (zet.z[t] + zet.z[tk]) * rcz);
+}
+
+// Rc = Visc - Conv as the divergence of the three face families and rct
+// at the interior cells, in one pass with no face arrays in HBM.  Block
+// 64 x TY owns an (i, j) tile and marches RC_KC planes of k: lane 0 is the
+// halo column i0-1 (its i-face reaches lane 1 through a lane shuffle), row
+// 0 the halo row j0-1 (its j-face reaches row 1 through LDS), and the
+// k-face of plane k-1 stays in registers.  63 x (TY-1) cells per block and
+// plane.  This variant gathers ucat and nvert from global memory.
+#define RC_KC 16
+template <int TY>
+__global__ void __launch_bounds__(64 * TY) k_rc_fused(G g, C3 uc, C3 u, const double *nv, FM mi, FM mj, FM mk, double nu,
+                                                    C3 csi, C3 eta, C3 zet, const double *aj, V3 rct) {
+  __shared__ double S[3][TY][64];
+  const int L = threadIdx.x, R = threadIdx.y;
+  const int mx = g.mx, my = g.my, mz = g.mz;
+  const int i = 63 * blockIdx.x - 1 + L, j = (TY - 1) * blockIdx.y - 1 + R;
+  const int k0 = RC_KC * blockIdx.z, k1 = min(k0 + RC_KC, mz);
+  const bool in_i = i >= 1 && i <= mx - 2, in_j = j >= 1 && j <= my - 2;
+  const bool out = L >= 1 && R >= 1 && in_i && in_j;
+  const bool do0 = R >= 1 && in_j && i >= 0 && i <= mx - 2;   // i-face (i,j,k)
+  const bool do1 = L >= 1 && in_i && j >= 0 && j <= my - 2;   // j-face (i,j,k)
+  double p2[3] = {0., 0., 0.};                               // k-face (i,j,k-1)
+  if (out && k0 >= 1) {
+    const long t = g.at(i, j, k0 - 1);
+    GAcc a{&g, u, nv, i, j, k0 - 1};
+    face_rc<2>(g, i, j, k0 - 1, t, uc.z[t], a, mk, nu, p2);
+  }
+  for (int k = k0; k < k1; k++) {
+    const bool kin = k >= 1 && k <= mz - 2;
     const long t = g.at(i, j, k);
-    const int c[3] = {i, j, k}, mm[3] = {g.mx, g.my, g.mz};
-    bool valid = true;
+    const GAcc a{&g, u, nv, i, j, k};
+    double d0[3], a1[3] = {0., 0., 0.}, a2[3] = {0., 0., 0.};
+    {
+      double a0[3] = {0., 0., 0.};
+      if (do0 && kin) face_rc<0>(g, i, j, k, t, uc.x[t], a, mi, nu, a0);
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-      if (a == D) valid = valid && c[a] >= 0 && c[a] <= mm[a] - 2;
-      else valid = valid && c[a] >= 1 && c[a] <= mm[a] - 2;
+      for (int c = 0; c < 3; c++) d0[c] = a0[c] - __shfl_up(a0[c], 1, 64);
     }
-    if (!valid) continue;
-    double cf[3], vf[3];
-    conv_flux(g, D, i, j, k, uc, u, nv, cf);
-    visc_flux(g, D, i, j, k, u, nv, m, nu, vf);
-    F.x[t] = vf[0] - cf[0];
-    F.y[t] = vf[1] - cf[1];
-    F.z[t] = vf[2] - cf[2];
+    if (do1 && kin) face_rc<1>(g, i, j, k, t, uc.y[t], a, mj, nu, a1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) S[c][R][L] = a1[c];
+    if (out && k <= mz - 2) face_rc<2>(g, i, j, k, t, uc.z[t], a, mk, nu, a2);
+    __syncthreads();
+    if (out && kin) {
+      double d1[3], d2[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) { d1[c] = a1[c] - S[c][R - 1][L]; d2[c] = a2[c] - p2[c]; }
+      rct_out(g, t, d0, d1, d2, csi, eta, zet, aj, rct);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; c++) p2[c] = a2[c];
   }
 }
 
-// Rc = Visc - Conv as the divergence of the face fluxes, and
-// rct = aj*(avg basis).Rc at interior cells (rhs.c:2663-2703)
-__global__ void __launch_bounds__(BS) k_rct(G g, C3 f0, C3 f1, C3 f2, C3 csi, C3 eta, C3 zet, const double *aj, V3 rct,
-                                            long n) {
-  (void)n;
-  CV_MARCH(g) {
+// The same pass with ucat and the nvert classes staged through LDS: a ring
+// of three k-planes of the tile i0-2 .. i0+64 x j0-2 .. j0+TY (every point
+// the faces of the block read in those planes), one plane loaded per step
+// with coalesced rows.  Barrier A publishes the new plane; barrier B
+// publishes the j-faces and ends every read of the oldest plane before the
+// next step overwrites its slot.  Points outside the grid are loaded
+// through the periodic ghost map on x and z or left zero: no face rule
+// reads them.
+template <int TY>
+__global__ void __launch_bounds__(64 * TY) k_rc_tiled(G g, C3 uc, C3 u, const double *nv, FM mi, FM mj, FM mk, double nu,
+                                                    C3 csi, C3 eta, C3 zet, const double *aj, V3 rct) {
+  constexpr int TW = 67, TH = TY + 3, NP = TW * TH, NT = 64 * TY;
+  __shared__ double SU[3][3][NP];
+  __shared__ unsigned char SN[3][NP];
+  __shared__ double SF[3][TY][64];
+  const int L = threadIdx.x, R = threadIdx.y, tid = R * 64 + L;
+  const int mx = g.mx, my = g.my, mz = g.mz;
+  const int i0 = 63 * blockIdx.x, j0 = (TY - 1) * blockIdx.y;
+  const int i = i0 - 1 + L, j = j0 - 1 + R;
+  const int k0 = RC_KC * blockIdx.z, k1 = min(k0 + RC_KC, mz);
+  const bool in_i = i >= 1 && i <= mx - 2, in_j = j >= 1 && j <= my - 2;
+  const bool out = L >= 1 && R >= 1 && in_i && in_j;
+  const bool do0 = R >= 1 && in_j && i >= 0 && i <= mx - 2;
+  const bool do1 = L >= 1 && in_i && j >= 0 && j <= my - 2;
+  const int ks = k0 > 0 ? k0 - 1 : 0;   // the k-face of plane k0-1 starts the k difference
+  auto load_plane = [&](int kk) {
+    const int sl = (kk + 3) % 3;
+    int kq = kk;
+    bool kok = kk >= 0 && kk < mz;
+    if (!kok && g.zper && (kk == -1 || kk == mz)) { kq = kk == -1 ? mz - 3 : 2; kok = true; }
+    for (int e = tid; e < NP; e += NT) {
+      const int row = e / TW, col = e - row * TW;
+      int gi = i0 - 2 + col;
+      const int gj = j0 - 2 + row;
+      bool ok = kok && gj >= 0 && gj < my;
+      if (gi < 0 || gi >= mx) {
+        if (g.xper && (gi == -1 || gi == mx)) gi = gi == -1 ? mx - 3 : 2;
+        else ok = false;
+      }
+      double x = 0., y = 0., z = 0., w = 0.;
+      if (ok) {
+        const long p = g.at(gi, gj, kq);
+        x = u.x[p]; y = u.y[p]; z = u.z[p]; w = nv[p];
+      }
+      SU[sl][0][e] = x; SU[sl][1][e] = y; SU[sl][2][e] = z;
+      SN[sl][e] = (unsigned char)nv_bits(w);
+    }
+  };
+  load_plane(ks - 1);
+  load_plane(ks);
+  double p2[3] = {0., 0., 0.};
+  for (int k = ks; k < k1; k++) {
+    load_plane(k + 1);
+    __syncthreads();   // A
+    const bool kin = k >= k0 && k >= 1 && k <= mz - 2;
     const long t = g.at(i, j, k);
-    if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
-    const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
-    const double rcx = (f0.x[t] - f0.x[ti]) + (f1.x[t] - f1.x[tj]) + (f2.x[t] - f2.x[tk]);
-    const double rcy = (f0.y[t] - f0.y[ti]) + (f1.y[t] - f1.y[tj]) + (f2.y[t] - f2.y[tk]);
-    const double rcz = (f0.z[t] - f0.z[ti]) + (f1.z[t] - f1.z[tj]) + (f2.z[t] - f2.z[tk]);
-    const double A = aj[t];
-    rct.x[t] = A * (0.5 * (csi.x[t] + csi.x[ti]) * rcx + 0.5 * (csi.y[t] + csi.y[ti]) * rcy + 0.5 * (csi.z[t] + csi.z[ti]) * rcz);
-    rct.y[t] = A * (0.5 * (eta.x[t] + eta.x[tj]) * rcx + 0.5 * (eta.y[t] + eta.y[tj]) * rcy + 0.5 * (eta.z[t] + eta.z[tj]) * rcz);
-    rct.z[t] = A * (0.5 * (zet.x[t] + zet.x[tk]) * rcx + 0.5 * (zet.y[t] + zet.y[tk]) * rcy + 0.5 * (zet.z[t] + zet.z[tk]) * rcz);
+    TAcc<TW> a;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const int sl = (k - 1 + q + 3) % 3;
+      a.P[q][0] = SU[sl][0]; a.P[q][1] = SU[sl][1]; a.P[q][2] = SU[sl][2];
+      a.N[q] = SN[sl];
+    }
+    a.o = (R + 1) * TW + L + 1;
+    a.kp2[0] = a.kp2[1] = a.kp2[2] = 0.;
+    double d0[3], a1[3] = {0., 0., 0.}, a2[3] = {0., 0., 0.};
+    {
+      double a0[3] = {0., 0., 0.};
+      if (do0 && kin) face_rc<0>(g, i, j, k, t, uc.x[t], a, mi, nu, a0);
+#pragma unroll
+      for (int c = 0; c < 3; c++) d0[c] = a0[c] - __shfl_up(a0[c], 1, 64);
+    }
+    if (do1 && kin) face_rc<1>(g, i, j, k, t, uc.y[t], a, mj, nu, a1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) SF[c][R][L] = a1[c];
+    if (out && k <= mz - 2) {
+      const int kk = k + 2;
+      if (kk < mz || (g.zper && kk == mz)) {
+        const long p = g.at(i, j, kk < mz ? kk : 2);
+        a.kp2[0] = u.x[p]; a.kp2[1] = u.y[p]; a.kp2[2] = u.z[p];
+      }
+      face_rc<2>(g, i, j, k, t, uc.z[t], a, mk, nu, a2);
+    }
+    __syncthreads();   // B
+    if (out && kin) {
+      double d1[3], d2[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) { d1[c] = a1[c] - SF[c][R - 1][L]; d2[c] = a2[c] - p2[c]; }
+      rct_out(g, t, d0, d1, d2, csi, eta, zet, aj, rct);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) p2[c] = a2[c];
   }
 }
 
@@ -757,6 +937,24 @@ int cv_ghost_update(cv_ctx *c, int formbcs) {
   return CV_OK;
 }
 
+// rows per face-flux block (4 or 8, CV_RC_ROWS; row 0 is halo); 8 is the
+// faster at 512x256x256
+static int rc_rows() {
+  static int ty = 0;
+  if (!ty) {
+    const char *e = getenv("CV_RC_ROWS");
+    ty = (e && atoi(e) == 4) ? 4 : 8;
+  }
+  return ty;
+}
+
+// face-flux pass variant (CV_RC_KERNEL: 1 LDS-tiled, 0 global gathers),
+// read at every call so one process can run both
+static int rc_kernel() {
+  const char *e = getenv("CV_RC_KERNEL");
+  return e ? atoi(e) != 0 : 1;
+}
+
 static FFArgs ffargs(cv_ctx *c) {
   CvLevel &F = c->lev[c->nlev - 1];
   FFArgs A;
@@ -778,17 +976,16 @@ int cv_ff1_dev(cv_ctx *c) {
   const FM mi{c3(M.icsi), c3(M.ieta), c3(M.izet), M.iaj};
   const FM mj{c3(M.jcsi), c3(M.jeta), c3(M.jzet), M.jaj};
   const FM mk{c3(M.kcsi), c3(M.keta), c3(M.kzet), M.kaj};
-  const V3 f0 = v3(c->fface), f1 = v3(c->fface + 3), f2 = v3(c->fface + 6);
-  hipLaunchKernelGGL(k_face_rc<0>, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A.g, uc, u, (const double *)F.nvert, mi,
-                     nu, f0, F.d.n);
-  hipLaunchKernelGGL(k_face_rc<1>, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A.g, uc, u, (const double *)F.nvert, mj,
-                     nu, f1, F.d.n);
-  hipLaunchKernelGGL(k_face_rc<2>, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A.g, uc, u, (const double *)F.nvert, mk,
-                     nu, f2, F.d.n);
   {
-    CvProfScope ps(c, CV_PROF_RCT, c->nlev - 1, 0, (double)F.d.n * 8.0 * (9 + 9 + 1 + 3));
-    hipLaunchKernelGGL(k_rct, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A.g, c3(c->fface), c3(c->fface + 3),
-                       c3(c->fface + 6), c3(M.csi), c3(M.eta), c3(M.zet), (const double *)M.aj, v3(c->rct), F.d.n);
+    // algorithmic bytes per point: 30 face metrics, 9 cell metrics + aj,
+    // ucont, ucat, nvert, rct out
+    CvProfScope ps(c, CV_PROF_RCT, c->nlev - 1, 0, (double)F.d.n * 8.0 * (30 + 10 + 3 + 3 + 1 + 3));
+    const int ty = rc_rows(), tiled = rc_kernel();
+    const dim3 grid((unsigned)((F.d.mx + 62) / 63), (unsigned)((F.d.my - 1 + ty - 2) / (ty - 1)),
+                    (unsigned)((F.d.mz + RC_KC - 1) / RC_KC));
+    auto kern = tiled ? (ty == 8 ? k_rc_tiled<8> : k_rc_tiled<4>) : (ty == 8 ? k_rc_fused<8> : k_rc_fused<4>);
+    hipLaunchKernelGGL(kern, grid, dim3(64, ty), 0, c->stream, A.g, uc, u, (const double *)F.nvert, mi, mj, mk, nu,
+                       c3(M.csi), c3(M.eta), c3(M.zet), (const double *)M.aj, v3(c->rct));
   }
   if (F.d.xper || F.d.zper)
     hipLaunchKernelGGL(k_rct_periodic, dim3(nblk((long)A.g.my * A.g.mz + (long)A.g.mx * A.g.my)), dim3(BS), 0, c->stream,

@@ -271,7 +271,8 @@ __global__ void __launch_bounds__(64 * TY) k_stencil(OpV<T> A, const T *__restri
   const int tid = R * 64 + L;
   const int mx = A.mx, my = A.my, mz = A.mz;
   int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
-  if (remap) cv_xcd_tile(bx, by, bz);
+  if (remap == 1) cv_xcd_tile(bx, by, bz);
+  else if (remap > 1) cv_xcd_runs(remap, bx, by, bz);
   const int i0 = 63 * bx, j0 = TY * by;
   const int k0 = kchunk * bz, k1 = min(k0 + kchunk, mz);
   const int gi = i0 - 1 + L, gj = j0 + R;
@@ -822,8 +823,8 @@ static int stencil_kchunk() {
 }
 
 // rows per stencil block (4 or 8; CV_STENCIL_ROWS64 / CV_STENCIL_ROWS32) and
-// XCD-aware tile order (CV_STENCIL_REMAP, 1 = on); the defaults are the
-// fastest measured at 512x256x256
+// XCD-aware tile order (CV_STENCIL_REMAP); the defaults are the fastest
+// measured at 512x256x256
 static int env_int(const char *name, int dflt) {
   const char *e = getenv(name);
   return e ? atoi(e) : dflt;
@@ -839,10 +840,13 @@ static int stencil_pfb64() {
   if (v < 0) v = env_int("CV_STENCIL_PFB64", 1) != 0;
   return v;
 }
-static int stencil_remap() {
-  static int rm = -1;
-  if (rm < 0) rm = env_int("CV_STENCIL_REMAP", 0) != 0;
-  return rm;
+// tile order: 0 launch order, 1 one contiguous block range per XCD, R > 1
+// interleaved runs of R tiles; default (-1 here) runs of two rows of x-tiles,
+// the measured best at 512x256x256 (4-5 % over launch order)
+static int stencil_remap(unsigned gx) {
+  static int rm = -2;
+  if (rm == -2) rm = env_int("CV_STENCIL_REMAP", -1);
+  return rm >= 0 ? rm : (int)(2 * gx);
 }
 
 // algorithmic bytes per storage point of the stencil modes: x, 9 g, aj (+b)
@@ -857,8 +861,9 @@ static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T
   CvLevel &L = c->lev[l];
   OpV<T> A = opview<T>(c, l);
   const int kc = stencil_kchunk();
-  const int ty = stencil_rows<T>(), remap = stencil_remap();
+  const int ty = stencil_rows<T>();
   dim3 g((unsigned)((L.d.mx + 62) / 63), (unsigned)((L.d.my + ty - 1) / ty), (unsigned)((L.d.mz + kc - 1) / kc));
+  const int remap = stencil_remap(g.x);
   CvProfScope ps(c, MODE == 0 ? CV_PROF_APPLY : CV_PROF_RESID, l, sizeof(T) == 4 ? 1 : 0,
                  (double)L.d.n * stencil_bytes_pt<T>(MODE));
   const bool pfb = sizeof(T) == 4 || stencil_pfb64();
