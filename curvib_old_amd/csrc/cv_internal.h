@@ -143,6 +143,8 @@ int cv_ff1_dev(cv_ctx *c);
 int cv_rk_dev(cv_ctx *c);
 int cv_imprk_dev(cv_ctx *c, cv_step_stats *st);
 int cv_calcrhs_dev(cv_ctx *c);
+int cv_step_pre_dev(cv_ctx *c, cv_step_stats *st);
+int cv_step_post_dev(cv_ctx *c);
 
 __device__ inline int cv_wrap(int i, int m, int per) {
   return per ? (i == m - 1 ? 1 : (i == 0 ? m - 2 : i)) : i;

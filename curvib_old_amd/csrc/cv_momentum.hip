@@ -1149,27 +1149,35 @@ extern "C" int cv_rungekutta(cv_ctx *c) {
   return rc;
 }
 
+// the part of a fractional step before the pressure solve: the level
+// rotation Ucont_rm1 <- Ucont_o <- Ucont (main.c:1131-1132, done at the step
+// start) and the momentum scheme
+int cv_step_pre_dev(cv_ctx *c, cv_step_stats *s) {
+  if (c->steps_done > 0) copy3(c, c->ucont_rm1, c->ucont_o);
+  copy3(c, c->ucont_o, c->ucont);
+  return c->mom.scheme == CV_MOM_IMPRK ? cv_imprk_dev(c, s) : cv_rk_dev(c);
+}
+
+// the part after it: UpdatePressure and Projection (solvers.c:1066-1068)
+int cv_step_post_dev(cv_ctx *c) {
+  int rc = cv_update_pressure_dev(c);
+  return rc ? rc : cv_projection_dev(c);
+}
+
 extern "C" int cv_flow_step(cv_ctx *c, cv_step_stats *st) {
   int rc = ready(c);
   if (rc) return rc;
-  CvLevel &F = c->lev[c->nlev - 1];
   cv_step_stats s{};
   hipEvent_t e[4];
   for (int q = 0; q < 4; q++) CV_CHECK(hipEventCreate(&e[q]));
-  // Ucont_rm1 <- Ucont_o <- Ucont (main.c:1131-1132), done at the step start
-  if (c->steps_done > 0) copy3(c, c->ucont_rm1, c->ucont_o);
-  copy3(c, c->ucont_o, c->ucont);
   CV_CHECK(hipEventRecord(e[0], c->stream));
-  if (c->mom.scheme == CV_MOM_IMPRK) rc = cv_imprk_dev(c, &s);
-  else rc = cv_rk_dev(c);
-  if (rc) return rc;
+  if ((rc = cv_step_pre_dev(c, &s))) return rc;
   CV_CHECK(hipEventRecord(e[1], c->stream));
   rc = cv_poisson_solve_dev(c, &s);
   if (rc && rc != CV_ERR_NOT_CONVERGED) return rc;
   int conv_rc = rc;
   CV_CHECK(hipEventRecord(e[2], c->stream));
-  if ((rc = cv_update_pressure_dev(c))) return rc;
-  if ((rc = cv_projection_dev(c))) return rc;
+  if ((rc = cv_step_post_dev(c))) return rc;
   CV_CHECK(hipEventRecord(e[3], c->stream));
   if ((rc = cv_divergence_dev(c, &s.maxdiv, &s.maxdiv_idx))) return rc;
   CV_CHECK(hipStreamSynchronize(c->stream));

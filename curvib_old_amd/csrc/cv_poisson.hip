@@ -19,14 +19,19 @@
 
 #define BS 256
 
-__global__ void cv_k_final_sum(const double *part, int nb, double *out, int nq) {
+// out[q] = sum_b part[q*nb + b] in a fixed order (one block per q)
+__device__ inline void final_sum_body(const double *part, int nb, double *out, int q) {
   __shared__ double sh[BS / 64];
-  const int q = blockIdx.x;
-  if (q >= nq) return;
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(long)q * nb + b];
   s = cv_block_sum(s, sh);
   if (threadIdx.x == 0) out[q] = s;
+}
+
+__global__ void cv_k_final_sum(const double *part, int nb, double *out, int nq) {
+  const int q = blockIdx.x;
+  if (q >= nq) return;
+  final_sum_body(part, nb, out, q);
 }
 
 namespace {
@@ -259,10 +264,11 @@ struct StCell {
 // TY: rows per block (block 64 x TY); PFB: prefetch the row-0 face below
 // the tile with the cells (keeps the row-0 wave off the barrier's critical
 // path at the cost of registers); remap: XCD-aware tile order
+// one tile (bx, by, bz) of the stencil: 63 x TY cells of kchunk planes
 template <class T, int MODE, int TY, bool PFB>
-__global__ void __launch_bounds__(64 * TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
-                                                   T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk,
-                                                   int remap) {
+__device__ inline void stencil_tile(const OpV<T> &A, const T *__restrict__ x, const T *__restrict__ b,
+                                    T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk, int bx,
+                                    int by, int bz) {
   constexpr int TR = TY + 2, NT = TR * ST_TC, NTH = 64 * TY;
   static_assert(2 * NTH >= NT, "two tile entries per thread must cover a plane");
   __shared__ T X[4][NT];
@@ -270,9 +276,6 @@ __global__ void __launch_bounds__(64 * TY) k_stencil(OpV<T> A, const T *__restri
   const int L = threadIdx.x, R = threadIdx.y;
   const int tid = R * 64 + L;
   const int mx = A.mx, my = A.my, mz = A.mz;
-  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
-  if (remap == 1) cv_xcd_tile(bx, by, bz);
-  else if (remap > 1) cv_xcd_runs(remap, bx, by, bz);
   const int i0 = 63 * bx, j0 = TY * by;
   const int k0 = kchunk * bz, k1 = min(k0 + kchunk, mz);
   const int gi = i0 - 1 + L, gj = j0 + R;
@@ -408,6 +411,16 @@ __global__ void __launch_bounds__(64 * TY) k_stencil(OpV<T> A, const T *__restri
   }
 }
 
+template <class T, int MODE, int TY, bool PFB>
+__global__ void __launch_bounds__(64 * TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
+                                                   T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk,
+                                                   int remap) {
+  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  if (remap == 1) cv_xcd_tile(bx, by, bz);
+  else if (remap > 1) cv_xcd_runs(remap, bx, by, bz);
+  stencil_tile<T, MODE, TY, PFB>(A, x, b, y, dinv, omega, kchunk, bx, by, bz);
+}
+
 // ------------------------------------------------------------------
 // y-line (wall-normal) smoother: per row the coefficients of x[c-e_j],
 // x[c], x[c+e_j] taken from the operator itself, the Thomas factors of
@@ -518,14 +531,15 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
 // factors and x are each read once and x written once.  Forward recurrence
 // y_j = r_j ip_j + a_j y_{j-1} (a = -lo ip, stored), backward
 // e_j = y_j - cp_j e_{j+1}; chunk carries resolved through LDS.
+// lines (bx*64 .. bx*64+63, k) of the register-resident line solve; the
+// chunks beyond the line (blockDim.y larger than the line needs) are
+// identity maps, so the result does not depend on blockDim.y
 template <class T, int CS>
-__global__ void __launch_bounds__(1024) k_yline_reg(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
-                                                   const T *__restrict__ ip, const T *__restrict__ cp,
-                                                   const T *xi, T *xo, T omega, int zero_guess) {
+__device__ inline void yline_reg_tile(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
+                                      const T *__restrict__ ip, const T *__restrict__ cp, const T *xi, T *xo,
+                                      T omega, int zero_guess, int bx, int bk) {
   __shared__ T SA[16][64], SB[16][64];
   const int ti = threadIdx.x, tc = threadIdx.y, nc = blockDim.y;
-  int bx, bk, bz;
-  cv_xcd_tile(bx, bk, bz);
   const int i = bx * 64 + ti, k = bk;
   const bool act = i < mx;
   const long sj = mx;
@@ -580,6 +594,15 @@ __global__ void __launch_bounds__(1024) k_yline_reg(int mx, int my, int mz, cons
   if (tc == nc - 1) xo[base + sj * (my - 1)] = zero_guess ? (T)0 : xi[base + sj * (my - 1)];
 }
 
+template <class T, int CS>
+__global__ void __launch_bounds__(1024) k_yline_reg(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
+                                                   const T *__restrict__ ip, const T *__restrict__ cp,
+                                                   const T *xi, T *xo, T omega, int zero_guess) {
+  int bx, bk, bz;
+  cv_xcd_tile(bx, bk, bz);
+  yline_reg_tile<T, CS>(mx, my, mz, r, ya, ip, cp, xi, xo, omega, zero_guess, bx, bk);
+}
+
 // ------------------------------------------------------------------
 // nvert hierarchy (MyNvertRestriction, poisson.c:1662-1773)
 // ------------------------------------------------------------------
@@ -616,10 +639,12 @@ __global__ void k_nvrestrict2(Idx C, const double *nin, double *nout, long n) {
 // ------------------------------------------------------------------
 // transfers
 // ------------------------------------------------------------------
+// restriction of the coarse points t0, t0+step, ...
 template <class T>
-__global__ void k_restrict(Idx C, Idx F, int isc, int jsc, int ksc, const double *__restrict__ nvf,
-                           const double *__restrict__ nvc, const T *__restrict__ xf, T *__restrict__ xc, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+__device__ inline void restrict_body(Idx C, Idx F, int isc, int jsc, int ksc, const double *__restrict__ nvf,
+                                     const double *__restrict__ nvc, const T *__restrict__ xf, T *__restrict__ xc,
+                                     long n, long t0, long step) {
+  for (long t = t0; t < n; t += step) {
     int i, j, k;
     C.dec(t, i, j, k);
     T f = 0;
@@ -635,12 +660,19 @@ __global__ void k_restrict(Idx C, Idx F, int isc, int jsc, int ksc, const double
     xc[t] = f;
   }
 }
+template <class T>
+__global__ void k_restrict(Idx C, Idx F, int isc, int jsc, int ksc, const double *__restrict__ nvf,
+                           const double *__restrict__ nvc, const T *__restrict__ xf, T *__restrict__ xc, long n) {
+  restrict_body<T>(C, F, isc, jsc, ksc, nvf, nvc, xf, xc, n, blockIdx.x * (long)blockDim.x + threadIdx.x,
+                   (long)gridDim.x * blockDim.x);
+}
 
 // xf (+)= P xc ; add=0 overwrites
 template <class T>
-__global__ void k_prolong(Idx F, Idx C, int isc, int jsc, int ksc, const double *__restrict__ nvc,
-                          const double *__restrict__ nvf, const T *__restrict__ xc, T *__restrict__ xf, int add, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+__device__ inline void prolong_body(Idx F, Idx C, int isc, int jsc, int ksc, const double *__restrict__ nvc,
+                                    const double *__restrict__ nvf, const T *__restrict__ xc, T *__restrict__ xf,
+                                    int add, long n, long t0, long step) {
+  for (long t = t0; t < n; t += step) {
     int i, j, k;
     F.dec(t, i, j, k);
     T f = 0;
@@ -664,24 +696,41 @@ __global__ void k_prolong(Idx F, Idx C, int isc, int jsc, int ksc, const double 
     if (add) xf[t] += f; else xf[t] = f;
   }
 }
+template <class T>
+__global__ void k_prolong(Idx F, Idx C, int isc, int jsc, int ksc, const double *__restrict__ nvc,
+                          const double *__restrict__ nvf, const T *__restrict__ xc, T *__restrict__ xf, int add, long n) {
+  prolong_body<T>(F, C, isc, jsc, ksc, nvc, nvf, xc, xf, add, n, blockIdx.x * (long)blockDim.x + threadIdx.x,
+                  (long)gridDim.x * blockDim.x);
+}
 
 // ------------------------------------------------------------------
 // nullspace (PoissonNullSpaceFunction, poisson.c:411-558)
 // ------------------------------------------------------------------
+// partition of a reduction: block bid of nb, BS threads each
+#define RED_LOOP for (long t = (long)bid * BS + threadIdx.x; t < n; t += (long)nb * BS)
 template <class T>
-__global__ void k_fluid_sum(const T *__restrict__ x, const uint32_t *__restrict__ fl, long n, double *part) {
+__device__ inline void fluid_sum_body(const T *__restrict__ x, const uint32_t *__restrict__ fl, long n, double *part,
+                                      int bid, int nb) {
   __shared__ double sh[BS / 64];
   double s = 0.0;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+  RED_LOOP
     if (fl[t] & 1u) s += (double)x[t];
   s = cv_block_sum(s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part[bid] = s;
+}
+template <class T>
+__global__ void k_fluid_sum(const T *__restrict__ x, const uint32_t *__restrict__ fl, long n, double *part) {
+  fluid_sum_body<T>(x, fl, n, part, blockIdx.x, gridDim.x);
+}
+template <class T>
+__device__ inline void fluid_shift_body(T *__restrict__ x, const uint32_t *__restrict__ fl, const double *sum,
+                                        double inv_n, long n, long t0, long step) {
+  const double mean = sum[0] * inv_n;
+  for (long t = t0; t < n; t += step) x[t] = (fl[t] & 1u) ? (T)((double)x[t] - mean) : (T)0;
 }
 template <class T>
 __global__ void k_fluid_shift(T *__restrict__ x, const uint32_t *__restrict__ fl, const double *sum, double inv_n, long n) {
-  const double mean = sum[0] * inv_n;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
-    x[t] = (fl[t] & 1u) ? (T)((double)x[t] - mean) : (T)0;
+  fluid_shift_body<T>(x, fl, sum, inv_n, n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
 }
 
 // ------------------------------------------------------------------
@@ -690,13 +739,13 @@ __global__ void k_fluid_shift(T *__restrict__ x, const uint32_t *__restrict__ fl
 #define MAXV 33
 // part[q*gridDim + b] = sum_t w[t]*V[q*stride+t] for q < NV; q == NV -> ||w||^2
 template <int NV>
-__global__ void __launch_bounds__(BS) k_mdot(const double *__restrict__ w, const double *__restrict__ V, long stride,
-                                             long n, double *part) {
+__device__ inline void mdot_body(const double *__restrict__ w, const double *__restrict__ V, long stride, long n,
+                                 double *part, int bid, int nb) {
   __shared__ double sh[BS / 64];
   double acc[NV + 1];
 #pragma unroll
   for (int q = 0; q <= NV; q++) acc[q] = 0.0;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+  RED_LOOP {
     const double wt = w[t];
 #pragma unroll
     for (int q = 0; q < NV; q++) acc[q] += wt * V[q * stride + t];
@@ -705,20 +754,25 @@ __global__ void __launch_bounds__(BS) k_mdot(const double *__restrict__ w, const
 #pragma unroll
   for (int q = 0; q <= NV; q++) {
     const double s = cv_block_sum(acc[q], sh);
-    if (threadIdx.x == 0) part[(long)q * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0) part[(long)q * nb + bid] = s;
   }
+}
+template <int NV>
+__global__ void __launch_bounds__(BS) k_mdot(const double *__restrict__ w, const double *__restrict__ V, long stride,
+                                             long n, double *part) {
+  mdot_body<NV>(w, V, stride, n, part, blockIdx.x, gridDim.x);
 }
 
 // w -= sum_q h[q] V_q ; part[b] = partial ||w||^2
 template <int NV>
-__global__ void __launch_bounds__(BS) k_orth_update(double *__restrict__ w, const double *__restrict__ V, long stride,
-                                                    const double *__restrict__ h, long n, double *part) {
+__device__ inline void orth_body(double *__restrict__ w, const double *__restrict__ V, long stride,
+                                 const double *__restrict__ h, long n, double *part, int bid, int nb) {
   __shared__ double sh[BS / 64];
   double hq[NV];
 #pragma unroll
   for (int q = 0; q < NV; q++) hq[q] = h[q];
   double s = 0.0;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+  RED_LOOP {
     double v = w[t];
 #pragma unroll
     for (int q = 0; q < NV; q++) v -= hq[q] * V[q * stride + t];
@@ -726,7 +780,12 @@ __global__ void __launch_bounds__(BS) k_orth_update(double *__restrict__ w, cons
     s += v * v;
   }
   s = cv_block_sum(s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part[bid] = s;
+}
+template <int NV>
+__global__ void __launch_bounds__(BS) k_orth_update(double *__restrict__ w, const double *__restrict__ V, long stride,
+                                                    const double *__restrict__ h, long n, double *part) {
+  orth_body<NV>(w, V, stride, h, n, part, blockIdx.x, gridDim.x);
 }
 
 typedef void (*MdotFn)(const double *, const double *, long, long, double *);
@@ -738,31 +797,39 @@ typedef void (*OrthFn)(double *, const double *, long, const double *, long, dou
 static const MdotFn mdot_tbl[] = {CV_NV_LIST(CV_MDOT_ENTRY)};
 static const OrthFn orth_tbl[] = {CV_NV_LIST(CV_ORTH_ENTRY)};
 
+__device__ inline void scale_to_body(double *__restrict__ dst, const double *__restrict__ src, double a, long n,
+                                     long t0, long step) {
+  for (long t = t0; t < n; t += step) dst[t] = src[t] * a;
+}
 __global__ void k_scale_to(double *__restrict__ dst, const double *__restrict__ src, const double *inv, long n) {
-  const double a = inv[0];
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = src[t] * a;
+  scale_to_body(dst, src, inv[0], n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
 }
 
 // x += sum_q y[q] Z_q
-__global__ void k_xupdate(double *__restrict__ x, const double *__restrict__ Z, long stride, const double *__restrict__ y,
-                          int m, long n) {
+__device__ inline void xupdate_body(double *__restrict__ x, const double *__restrict__ Z, long stride,
+                                    const double *__restrict__ y, int m, long n, long t0, long step) {
   __shared__ double ys[MAXV];
   if (threadIdx.x < m) ys[threadIdx.x] = y[threadIdx.x];
   __syncthreads();
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+  for (long t = t0; t < n; t += step) {
     double v = x[t];
     for (int q = 0; q < m; q++) v += ys[q] * Z[q * stride + t];
     x[t] = v;
   }
 }
+__global__ void k_xupdate(double *__restrict__ x, const double *__restrict__ Z, long stride, const double *__restrict__ y,
+                          int m, long n) {
+  xupdate_body(x, Z, stride, y, m, n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
+}
 
-__global__ void k_sumsq(const double *__restrict__ x, long n, double *part) {
+__device__ inline void sumsq_body(const double *__restrict__ x, long n, double *part, int bid, int nb) {
   __shared__ double sh[BS / 64];
   double s = 0.0;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) s += x[t] * x[t];
+  RED_LOOP s += x[t] * x[t];
   s = cv_block_sum(s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part[bid] = s;
 }
+__global__ void k_sumsq(const double *__restrict__ x, long n, double *part) { sumsq_body(x, n, part, blockIdx.x, gridDim.x); }
 
 __global__ void k_d2f(float *dst, const double *src, long n) {
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = (float)src[t];
@@ -810,6 +877,16 @@ OpV<float> opview<float>(cv_ctx *c, int l) {
   return A;
 }
 static Idx idx_of(const CvLevel &L) { return Idx{L.d.mx, L.d.my, L.d.mz}; }
+
+// blocks of a Poisson-solve reduction over n points: about 8 points per
+// thread, at most CV_RED_GRID; the same partition serves one case alone and
+// in a batch, so the sums agree bit for bit
+static int nred(long n) {
+  long b = (n + (long)BS * 8 - 1) / ((long)BS * 8);
+  if (b > CV_RED_GRID) b = CV_RED_GRID;
+  if (b < 1) b = 1;
+  return (int)b;
+}
 
 // k extent of one stencil block (tunable through CV_STENCIL_KCHUNK)
 static int stencil_kchunk() {
@@ -947,8 +1024,9 @@ template <class T>
 static void nullspace_dev(cv_ctx *c, int l, T *x) {
   CvLevel &L = c->lev[l];
   const long n = L.d.n;
-  hipLaunchKernelGGL(k_fluid_sum<T>, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const T *)x, (const uint32_t *)L.flags, n, c->part);
-  final_sum(c, c->part, CV_RED_GRID, c->dsc + 100, 1);
+  const int nb = nred(n);
+  hipLaunchKernelGGL(k_fluid_sum<T>, dim3(nb), dim3(BS), 0, c->stream, (const T *)x, (const uint32_t *)L.flags, n, c->part);
+  final_sum(c, c->part, nb, c->dsc + 100, 1);
   double inv = c->nfluid[l] > 0 ? 1.0 / c->nfluid[l] : 0.0;
   hipLaunchKernelGGL(k_fluid_shift<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, x, (const uint32_t *)L.flags,
                      (const double *)(c->dsc + 100), inv, n);
@@ -1110,12 +1188,12 @@ static void gram_schmidt_pass(cv_ctx *c, double *w, const double *V, int nv, lon
   double *hd = c->dsc;
   {
     CvProfScope ps(c, CV_PROF_ORTH, c->nlev - 1, 0, (double)n * 8.0 * (nv + 1));
-    hipLaunchKernelGGL(mdot_tbl[nv - 1], dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, V, n, n, c->part);
+    hipLaunchKernelGGL(mdot_tbl[nv - 1], dim3(nred(n)), dim3(BS), 0, c->stream, (const double *)w, V, n, n, c->part);
   }
-  final_sum(c, c->part, CV_RED_GRID, hd, nv + 1);
-  hipLaunchKernelGGL(orth_tbl[nv - 1], dim3(CV_RED_GRID), dim3(BS), 0, c->stream, w, V, n, (const double *)hd, n,
+  final_sum(c, c->part, nred(n), hd, nv + 1);
+  hipLaunchKernelGGL(orth_tbl[nv - 1], dim3(nred(n)), dim3(BS), 0, c->stream, w, V, n, (const double *)hd, n,
                      c->part);
-  final_sum(c, c->part, CV_RED_GRID, hd + 36, 1);
+  final_sum(c, c->part, nred(n), hd + 36, 1);
 }
 
 // FGMRES(m) for A phi = b on the finest level; phi starts at 0
@@ -1130,8 +1208,8 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
   double *hd = c->dsc;              // device: h[0..33]
   double *hh = c->red_host;         // pinned host
   // ||b||
-  hipLaunchKernelGGL(k_sumsq, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, b, n, c->part);
-  final_sum(c, c->part, CV_RED_GRID, hd + 40, 1);
+  hipLaunchKernelGGL(k_sumsq, dim3(nred(n)), dim3(BS), 0, c->stream, b, n, c->part);
+  final_sum(c, c->part, nred(n), hd + 40, 1);
   CV_CHECK(hipMemcpyAsync(hh, hd + 40, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CV_CHECK(hipStreamSynchronize(c->stream));
   const double bnorm = std::sqrt(hh[0]);
@@ -1205,8 +1283,8 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
     hipLaunchKernelGGL(k_xupdate, dim3(nblk(n)), dim3(BS), 0, c->stream, x, (const double *)Z, (long)n, (const double *)hd, jj, n);
     // true residual
     stencil<double, 1>(c, top, (const double *)x, b, w);
-    hipLaunchKernelGGL(k_sumsq, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const double *)w, n, c->part);
-    final_sum(c, c->part, CV_RED_GRID, hd + 40, 1);
+    hipLaunchKernelGGL(k_sumsq, dim3(nred(n)), dim3(BS), 0, c->stream, (const double *)w, n, c->part);
+    final_sum(c, c->part, nred(n), hd + 40, 1);
     CV_CHECK(hipMemcpyAsync(hh, hd + 40, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CV_CHECK(hipStreamSynchronize(c->stream));
     beta = std::sqrt(hh[0]);
@@ -1660,3 +1738,5 @@ extern "C" int cv_nullspace_project(cv_ctx *c, int level, double *x) {
   CV_CHECK(hipStreamSynchronize(c->stream));
   return CV_OK;
 }
+
+#include "cv_batch_solve.h"

@@ -226,3 +226,50 @@ def run_stream(requests, arrivals, nsteps, precision, budget_bytes=200e9, slots=
 
 def poisson_arrivals(n, lam, rng):
     return np.cumsum(rng.exponential(1.0 / lam, size=n))
+
+
+def run_batched(requests, nsteps, precision, warmup=1, device=0, log=None, **solver):
+    """The ensemble as one cv_batch: every case [(case, arrays)] resident in
+    its own context, all advanced together ``warmup`` untimed steps and then
+    ``nsteps`` timed steps.  Every case finishes a step when the batch step
+    returns, so each step's wall time is every case's step latency."""
+    from .solver import Batch, UserCtx
+    ctxs = []
+    b = None
+    try:
+        for case, (xyz, nv, uc) in requests:
+            u = UserCtx(case.nodes, case.bctype, levels_for(case.cells), case.re, case.dt, semi=SEMI, device=device,
+                        coarsen_auto=COARSEN_AUTO)
+            u.set_grid(xyz)
+            u.set_nvert(nv)
+            u.put("ucont", uc)
+            u.set_solver(precision=precision, **solver)
+            ctxs.append(u)
+        b = Batch(ctxs)
+        for _ in range(warmup):
+            b.step()
+        ms, its = [], []
+        for s in range(nsteps):
+            t0 = time.perf_counter()
+            st = b.step()
+            ms.append(1e3 * (time.perf_counter() - t0))
+            its.append(max(x["ksp_its"] for x in st))
+            if log:
+                log(f"[batch {precision}] step {s}: {ms[-1]:.1f} ms, max its {its[-1]}")
+        cells = float(sum(np.prod(c.cells) for c, _ in requests))
+        wall = sum(ms) * 1e-3
+        return {
+            "precision": precision,
+            "cases": len(requests),
+            "steps": nsteps,
+            "step_latency": latency_summary(ms),
+            "max_its": its,
+            "cell_updates_per_s": cells * nsteps / wall if wall > 0 else 0.0,
+            "makespan_s": wall,
+            "device_bytes": int(sum(u.device_bytes() for u in ctxs)),
+        }
+    finally:
+        if b is not None:
+            b.close()
+        for u in ctxs:
+            u.close()

@@ -264,3 +264,51 @@ def from_case(case, mglevels=3, device=0, semi=(0, 0, 0), coarsen_auto=0):
     u.set_nvert(nv)
     u.put("ucont", uc)
     return u, (xyz, nv, uc)
+
+
+class Batch:
+    """A batch of UserCtx cases advanced together (``cv_batch_*``): one
+    kernel launch per multigrid depth and operation for the pressure solve
+    of every case, the rest of the step per case on its own stream.  The
+    cases must share solver options (y-line smoother) and stay open while
+    the batch exists."""
+
+    def __init__(self, ctxs):
+        self.ctxs = list(ctxs)
+        arr = (C.c_void_p * len(self.ctxs))(*[u.h.value if isinstance(u.h, C.c_void_p) else u.h
+                                              for u in self.ctxs])
+        h = C.c_void_p()
+        check(_lib.load().cv_batch_create(arr, len(self.ctxs), C.byref(h)), "cv_batch_create")
+        self.h = h
+
+    def _run(self, fn, name, allow_unconverged):
+        st = (_lib.StepStats * len(self.ctxs))()
+        rc = fn(self.h, st)
+        if not (rc == _lib.CV_OK or (allow_unconverged and rc == _lib.CV_ERR_NOT_CONVERGED)):
+            check(rc, name)
+        return [s.as_dict() for s in st]
+
+    def step(self, allow_unconverged=False):
+        """One fractional step of every case; per-case statistics."""
+        return self._run(_lib.load().cv_batch_step, "cv_batch_step", allow_unconverged)
+
+    def poisson_solve(self, allow_unconverged=False):
+        """VolumeFlux_rev, PoissonRHS and the pressure solve of every case."""
+        return self._run(_lib.load().cv_batch_poisson_solve, "cv_batch_poisson_solve", allow_unconverged)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().cv_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

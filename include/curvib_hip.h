@@ -197,6 +197,27 @@ int  cv_server_release(cv_server *s, long case_id);
 int  cv_server_status(cv_server *s, cv_server_info *info);
 int  cv_server_destroy(cv_server *s);
 
+/* ---- batched ensemble (no reference counterpart: the reference solves one
+ * case per run; each case's PoissonSolver_MG, poisson.c:1775-2017, and
+ * Flow_Solver step, solvers.c:944-1156, are what a batch runs for many
+ * cases at once) ----
+ * A batch groups up to 64 contexts of one device that share solver options
+ * (y-line smoother; at most 512 rows per y-line in mixed precision, 256 in
+ * FP64).  cv_batch_step advances every case by one fractional step: the
+ * momentum and projection parts run per case on each case's own stream, the
+ * pressure solve runs once for all cases, one kernel launch per multigrid
+ * depth and operation covering every case, FGMRES in lockstep with one host
+ * synchronisation per iteration, the preconditioner and apply replayed from
+ * a hipGraph.  Each case's fields are bit-identical to cv_flow_step on that
+ * case alone.  stats has one entry per case (in the order given); the call
+ * returns CV_ERR_NOT_CONVERGED when any case missed its tolerance.  The
+ * contexts must not be destroyed or re-optioned while the batch exists. */
+typedef struct cv_batch cv_batch;
+int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out);
+int cv_batch_step(cv_batch *b, cv_step_stats *stats);
+int cv_batch_poisson_solve(cv_batch *b, cv_step_stats *stats);   /* VolumeFlux_rev + PoissonRHS + solve */
+int cv_batch_destroy(cv_batch *b);
+
 /* ---- instrumentation (no reference counterpart; the reference's only
  * timing hook is the per-phase CPU_TIME log, solvers.c:1130-1142) ----
  * Device-event timing of every launch of one kernel class on one level,

@@ -1,0 +1,83 @@
+"""Batched ensemble (cv_batch_*): every case of a batch of uneven grids --
+different multigrid depths and y-line lengths -- ends each fractional step
+with the same bits in every field, and the same FGMRES iteration count, as
+the same case stepped alone by cv_flow_step (Flow_Solver, solvers.c:944-1156),
+in mixed and FP64 precision, with the captured hipGraph and with direct
+launches.  Tolerance: exact equality."""
+import numpy as np
+import pytest
+
+from curvib_old_amd import _lib, cases, serving
+from curvib_old_amd.solver import Batch, from_case
+
+pytestmark = pytest.mark.gpu
+
+SOLVER = dict(rtol=1e-9, max_it=200, smoother="yline", pre_sweeps=3, post_sweeps=3, coarse_sweeps=10, omega=0.9,
+              restart=30)
+
+
+def _cases():
+    rng = np.random.default_rng(77)
+    cl = [serving.sample_case(rng, lo=(24, 24, 24), hi=(72, 64, 64)) for _ in range(4)]
+    # a long y-line (160 rows: the 16-row chunks of the line solve) beside the short ones
+    c = cases.ChannelCase(cells=(32, 160, 24), amp=0.08, sphere=(0.5, 0.5, 0.5, 0.2), re=700.0, seed=5)
+    cl.append(cases.ChannelCase(**{**c.__dict__, "dt": cases.stable_dt(c)}))
+    return cl
+
+
+def _ctx(case, precision):
+    u, _ = from_case(case, mglevels=serving.levels_for(case.cells), semi=serving.SEMI,
+                     coarsen_auto=serving.COARSEN_AUTO)
+    u.set_solver(precision=precision, **SOLVER)
+    return u
+
+
+@pytest.mark.parametrize("precision,graph", [("mixed", "1"), ("fp64", "1"), ("mixed", "0")])
+def test_batch_matches_each_case_alone(monkeypatch, precision, graph):
+    monkeypatch.setenv("CV_BATCH_GRAPH", graph)
+    cl = _cases()
+    levels = {serving.levels_for(c.cells) for c in cl}
+    assert len(levels) > 1, "the batch must mix multigrid depths"
+    inb = [_ctx(c, precision) for c in cl]
+    alone = [_ctx(c, precision) for c in cl]
+    b = Batch(inb)
+    try:
+        for s in range(2):
+            sb = b.step()
+            for q, u in enumerate(alone):
+                sa = u.Flow_Solver()
+                assert sb[q]["ksp_its"] == sa["ksp_its"], (s, q, sb[q]["ksp_its"], sa["ksp_its"])
+                assert sb[q]["ksp_rnorm"] == sa["ksp_rnorm"] and sb[q]["maxdiv"] == sa["maxdiv"], (s, q)
+                assert sb[q]["converged"] == 1
+            for q, (ua, ub) in enumerate(zip(inb, alone)):
+                for f in ("p", "phi", "ucont", "ucat"):
+                    assert np.array_equal(ua.get(f), ub.get(f)), (s, q, f)
+    finally:
+        b.close()
+        for u in inb + alone:
+            u.close()
+
+
+def test_batch_rejects_mixed_options():
+    cl = _cases()[:2]
+    a, b = _ctx(cl[0], "mixed"), _ctx(cl[1], "fp64")
+    try:
+        with pytest.raises(_lib.CvError) as e:
+            Batch([a, b])
+        assert e.value.code == _lib.CV_ERR_ARG
+        b.set_solver(precision="mixed", **{**SOLVER, "smoother": "jacobi"})
+        a.set_solver(precision="mixed", **{**SOLVER, "smoother": "jacobi"})
+        with pytest.raises(_lib.CvError) as e:
+            Batch([a, b])
+        assert e.value.code == _lib.CV_ERR_ARG
+        a.set_solver(precision="mixed", **SOLVER)
+        b.set_solver(precision="mixed", **SOLVER)
+        bt = Batch([a, b])
+        a.set_solver(precision="mixed", **{**SOLVER, "rtol": 1e-8})
+        with pytest.raises(_lib.CvError) as e:
+            bt.step()
+        assert e.value.code == _lib.CV_ERR_STATE
+        bt.close()
+    finally:
+        a.close()
+        b.close()
