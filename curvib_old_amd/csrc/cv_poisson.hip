@@ -20,7 +20,7 @@
 #define BS 256
 
 // out[q] = sum_b part[q*nb + b] in a fixed order (one block per q)
-__device__ inline void final_sum_body(const double *part, int nb, double *out, int q) {
+__device__ __forceinline__ void final_sum_body(const double *part, int nb, double *out, int q) {
   __shared__ double sh[BS / 64];
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(long)q * nb + b];
@@ -266,7 +266,7 @@ struct StCell {
 // path at the cost of registers); remap: XCD-aware tile order
 // one tile (bx, by, bz) of the stencil: 63 x TY cells of kchunk planes
 template <class T, int MODE, int TY, bool PFB>
-__device__ inline void stencil_tile(const OpV<T> &A, const T *__restrict__ x, const T *__restrict__ b,
+__device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restrict__ x, const T *__restrict__ b,
                                     T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk, int bx,
                                     int by, int bz) {
   constexpr int TR = TY + 2, NT = TR * ST_TC, NTH = 64 * TY;
@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
 // chunks beyond the line (blockDim.y larger than the line needs) are
 // identity maps, so the result does not depend on blockDim.y
 template <class T, int CS>
-__device__ inline void yline_reg_tile(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
+__device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
                                       const T *__restrict__ ip, const T *__restrict__ cp, const T *xi, T *xo,
                                       T omega, int zero_guess, int bx, int bk) {
   __shared__ T SA[16][64], SB[16][64];
@@ -641,7 +641,7 @@ __global__ void k_nvrestrict2(Idx C, const double *nin, double *nout, long n) {
 // ------------------------------------------------------------------
 // restriction of the coarse points t0, t0+step, ...
 template <class T>
-__device__ inline void restrict_body(Idx C, Idx F, int isc, int jsc, int ksc, const double *__restrict__ nvf,
+__device__ __forceinline__ void restrict_body(Idx C, Idx F, int isc, int jsc, int ksc, const double *__restrict__ nvf,
                                      const double *__restrict__ nvc, const T *__restrict__ xf, T *__restrict__ xc,
                                      long n, long t0, long step) {
   for (long t = t0; t < n; t += step) {
@@ -669,7 +669,7 @@ __global__ void k_restrict(Idx C, Idx F, int isc, int jsc, int ksc, const double
 
 // xf (+)= P xc ; add=0 overwrites
 template <class T>
-__device__ inline void prolong_body(Idx F, Idx C, int isc, int jsc, int ksc, const double *__restrict__ nvc,
+__device__ __forceinline__ void prolong_body(Idx F, Idx C, int isc, int jsc, int ksc, const double *__restrict__ nvc,
                                     const double *__restrict__ nvf, const T *__restrict__ xc, T *__restrict__ xf,
                                     int add, long n, long t0, long step) {
   for (long t = t0; t < n; t += step) {
@@ -709,7 +709,7 @@ __global__ void k_prolong(Idx F, Idx C, int isc, int jsc, int ksc, const double 
 // partition of a reduction: block bid of nb, BS threads each
 #define RED_LOOP for (long t = (long)bid * BS + threadIdx.x; t < n; t += (long)nb * BS)
 template <class T>
-__device__ inline void fluid_sum_body(const T *__restrict__ x, const uint32_t *__restrict__ fl, long n, double *part,
+__device__ __forceinline__ void fluid_sum_body(const T *__restrict__ x, const uint32_t *__restrict__ fl, long n, double *part,
                                       int bid, int nb) {
   __shared__ double sh[BS / 64];
   double s = 0.0;
@@ -723,7 +723,7 @@ __global__ void k_fluid_sum(const T *__restrict__ x, const uint32_t *__restrict_
   fluid_sum_body<T>(x, fl, n, part, blockIdx.x, gridDim.x);
 }
 template <class T>
-__device__ inline void fluid_shift_body(T *__restrict__ x, const uint32_t *__restrict__ fl, const double *sum,
+__device__ __forceinline__ void fluid_shift_body(T *__restrict__ x, const uint32_t *__restrict__ fl, const double *sum,
                                         double inv_n, long n, long t0, long step) {
   const double mean = sum[0] * inv_n;
   for (long t = t0; t < n; t += step) x[t] = (fl[t] & 1u) ? (T)((double)x[t] - mean) : (T)0;
@@ -739,7 +739,7 @@ __global__ void k_fluid_shift(T *__restrict__ x, const uint32_t *__restrict__ fl
 #define MAXV 33
 // part[q*gridDim + b] = sum_t w[t]*V[q*stride+t] for q < NV; q == NV -> ||w||^2
 template <int NV>
-__device__ inline void mdot_body(const double *__restrict__ w, const double *__restrict__ V, long stride, long n,
+__device__ __forceinline__ void mdot_body(const double *__restrict__ w, const double *__restrict__ V, long stride, long n,
                                  double *part, int bid, int nb) {
   __shared__ double sh[BS / 64];
   double acc[NV + 1];
@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(BS) k_mdot(const double *__restrict__ w, const
 
 // w -= sum_q h[q] V_q ; part[b] = partial ||w||^2
 template <int NV>
-__device__ inline void orth_body(double *__restrict__ w, const double *__restrict__ V, long stride,
+__device__ __forceinline__ void orth_body(double *__restrict__ w, const double *__restrict__ V, long stride,
                                  const double *__restrict__ h, long n, double *part, int bid, int nb) {
   __shared__ double sh[BS / 64];
   double hq[NV];
@@ -797,7 +797,7 @@ typedef void (*OrthFn)(double *, const double *, long, const double *, long, dou
 static const MdotFn mdot_tbl[] = {CV_NV_LIST(CV_MDOT_ENTRY)};
 static const OrthFn orth_tbl[] = {CV_NV_LIST(CV_ORTH_ENTRY)};
 
-__device__ inline void scale_to_body(double *__restrict__ dst, const double *__restrict__ src, double a, long n,
+__device__ __forceinline__ void scale_to_body(double *__restrict__ dst, const double *__restrict__ src, double a, long n,
                                      long t0, long step) {
   for (long t = t0; t < n; t += step) dst[t] = src[t] * a;
 }
@@ -806,7 +806,7 @@ __global__ void k_scale_to(double *__restrict__ dst, const double *__restrict__ 
 }
 
 // x += sum_q y[q] Z_q
-__device__ inline void xupdate_body(double *__restrict__ x, const double *__restrict__ Z, long stride,
+__device__ __forceinline__ void xupdate_body(double *__restrict__ x, const double *__restrict__ Z, long stride,
                                     const double *__restrict__ y, int m, long n, long t0, long step) {
   __shared__ double ys[MAXV];
   if (threadIdx.x < m) ys[threadIdx.x] = y[threadIdx.x];
@@ -822,7 +822,7 @@ __global__ void k_xupdate(double *__restrict__ x, const double *__restrict__ Z, 
   xupdate_body(x, Z, stride, y, m, n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
 }
 
-__device__ inline void sumsq_body(const double *__restrict__ x, long n, double *part, int bid, int nb) {
+__device__ __forceinline__ void sumsq_body(const double *__restrict__ x, long n, double *part, int bid, int nb) {
   __shared__ double sh[BS / 64];
   double s = 0.0;
   RED_LOOP s += x[t] * x[t];
