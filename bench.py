@@ -206,6 +206,17 @@ def run_serving(serve_reqs, arrivals, ens_reqs, nsteps, ens_steps):
         log(f"[ensemble {prec}] {r['cases']} cases x {ens_steps} steps: makespan {r['makespan_s']:.2f} s, step p50 "
             f"{r['step_latency'].get('p50_ms', 0):.1f} ms p99 {r['step_latency'].get('p99_ms', 0):.1f} ms, "
             f"{r['cell_updates_per_s'] / 1e6:.1f} M cell-updates/s")
+    # the serving stream again through the batched server: every admitted case
+    # advances in one cv_batch, rebuilt as cases arrive and finish
+    for prec in ("fp64", "mixed"):
+        r = serving.run_stream(serve_reqs, arrivals, nsteps, prec, budget_bytes=200e9, slots=64, log=None,
+                               mode="batched", **opts)
+        r.pop("records")
+        out[f"serving_batched_{prec}"] = r
+        log(f"[serving batched {prec}] {r['cases']} cases x {nsteps} steps: step p50 "
+            f"{r['step_latency'].get('p50_ms', 0):.1f} ms p99 {r['step_latency'].get('p99_ms', 0):.1f} ms, "
+            f"{r['cell_updates_per_s'] / 1e6:.1f} M cell-updates/s, peak HBM {r['bytes_peak'] / 2**30:.1f} GiB, "
+            f"failed {r['failed']}")
     # the same ensemble as one batch: one launch per multigrid depth for all cases
     for prec in ("fp64", "mixed"):
         r = serving.run_batched(ens_reqs, ens_steps, prec, **opts)
@@ -346,6 +357,10 @@ def main():
             "serving_fp64_p99": srv["serving_fp64"]["step_latency"].get("p99_ms"),
             "ensemble_mixed_p50": srv["ensemble_mixed"]["step_latency"].get("p50_ms"),
             "ensemble_mixed_p99": srv["ensemble_mixed"]["step_latency"].get("p99_ms"),
+            "serving_batched_mixed_p50": srv["serving_batched_mixed"]["step_latency"].get("p50_ms"),
+            "serving_batched_mixed_p99": srv["serving_batched_mixed"]["step_latency"].get("p99_ms"),
+            "serving_batched_fp64_p50": srv["serving_batched_fp64"]["step_latency"].get("p50_ms"),
+            "serving_batched_fp64_p99": srv["serving_batched_fp64"]["step_latency"].get("p99_ms"),
             "ensemble_batched_mixed_p50": srv["ensemble_batched_mixed"]["step_latency"].get("p50_ms"),
             "ensemble_batched_mixed_p99": srv["ensemble_batched_mixed"]["step_latency"].get("p99_ms"),
             "ensemble_batched_fp64_p50": srv["ensemble_batched_fp64"]["step_latency"].get("p50_ms"),

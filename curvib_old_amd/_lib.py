@@ -40,8 +40,9 @@ SYMBOLS = [
     "cv_set_momentum", "cv_default_momentum", "cv_imprk", "cv_calcrhs",
     "cv_ctx_bytes", "cv_server_create", "cv_server_submit", "cv_server_wait", "cv_server_step_ms",
     "cv_server_fetch", "cv_server_release", "cv_server_status", "cv_server_destroy",
-    "cv_batch_create", "cv_batch_step", "cv_batch_poisson_solve", "cv_batch_destroy",
+    "cv_batch_create", "cv_batch_step", "cv_batch_poisson_solve", "cv_batch_destroy", "cv_server_create_mode",
 ]
+CV_SERVE_STREAMS, CV_SERVE_BATCHED = 0, 1
 CV_CASE_QUEUED, CV_CASE_RUNNING, CV_CASE_DONE, CV_CASE_FAILED = 0, 1, 2, 3
 
 CV_PROF = {"none": 0, "resid": 1, "apply": 2, "line": 3, "rct": 4, "ff1": 5, "orth": 6}
@@ -142,6 +143,7 @@ def load():
     L.cv_batch_poisson_solve.argtypes = [vp, C.POINTER(StepStats)]
     L.cv_batch_destroy.argtypes = [vp]
     L.cv_server_create.argtypes = [C.c_int, C.c_long, C.c_int, C.POINTER(C.c_void_p)]
+    L.cv_server_create_mode.argtypes = [C.c_int, C.c_long, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     L.cv_server_submit.argtypes = [vp, C.POINTER(GridDesc), C.POINTER(SolverOpts), dp, dp, dp, C.c_int,
                                    C.POINTER(C.c_long)]
     L.cv_server_wait.argtypes = [vp, C.c_long, C.POINTER(CaseInfo)]

@@ -6,16 +6,24 @@
 // step count.  Requests queue in arrival order.  A request is admitted when
 // a slot is free and its context fits the HBM budget next to the cases in
 // flight; its fields then stay resident on the device until its last step.
-// Every admitted case owns one context and one HIP stream, so the kernels
-// of all cases in flight overlap on the GPU, and a case's results do not
-// depend on what else runs (each context's arithmetic and reduction order
-// is fixed).  Each step's wall-clock latency is recorded.
+// Every admitted case owns one context and one HIP stream.  Two modes:
+//  * streams: one worker thread per slot steps its case with cv_flow_step,
+//    so the kernels of all cases in flight overlap on the GPU;
+//  * batched: one scheduler thread advances all admitted cases together,
+//    one cv_batch per group of cases with equal solver options, rebuilt
+//    whenever cases join or leave; a case the batch cannot take is stepped
+//    alone in the same tick.
+// A case's results do not depend on what else runs (each context's
+// arithmetic and reduction order is fixed, and a batch reproduces the
+// single-case bits).  Each step's wall-clock latency is recorded.
 #include "cv_internal.h"
 #include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <map>
 #include <memory>
+#include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <thread>
 
@@ -43,6 +51,7 @@ struct Case {
 
 struct cv_server {
   int device = 0;
+  int mode = CV_SERVE_STREAMS;
   long budget = 0, in_use = 0, peak = 0;
   int slots = 0, running = 0;
   bool stop = false;
@@ -57,7 +66,8 @@ struct cv_server {
   double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
 };
 
-static int run_case(cv_server *S, Case *cs) {
+// the case's device context from its inputs (which are then dropped)
+static int open_case(cv_server *S, Case *cs, cv_ctx **out) {
   (void)hipSetDevice(S->device);
   cv_ctx *c = nullptr;
   int rc = cv_ctx_create(&cs->g, S->device, &c);
@@ -68,6 +78,26 @@ static int run_case(cv_server *S, Case *cs) {
   std::vector<double>().swap(cs->xyz);
   std::vector<double>().swap(cs->nv);
   std::vector<double>().swap(cs->uc);
+  *out = c;
+  return rc;
+}
+
+// the final p and Ucont to the case record, then the context goes
+static int close_case(Case *cs, cv_ctx *c, int rc) {
+  if (rc == CV_OK && c) {
+    const long n = (long)(cs->g.IM + 1) * (cs->g.JM + 1) * (cs->g.KM + 1);
+    cs->p.resize(n);
+    cs->ucont.resize(3 * n);
+    rc = cv_get_field(c, CV_F_P, -1, cs->p.data());
+    if (rc == CV_OK) rc = cv_get_field(c, CV_F_UCONT, -1, cs->ucont.data());
+  }
+  if (c) cv_ctx_destroy(c);
+  return rc;
+}
+
+static int run_case(cv_server *S, Case *cs) {
+  cv_ctx *c = nullptr;
+  int rc = open_case(S, cs, &c);
   for (int s = 0; rc == CV_OK && s < cs->nsteps; s++) {
     cv_step_stats st{};
     const auto a = clk::now();
@@ -80,15 +110,32 @@ static int run_case(cv_server *S, Case *cs) {
     cs->last = st;
     if (r == CV_ERR_NOT_CONVERGED) cs->rc = r;
   }
-  if (rc == CV_OK) {
-    const long n = (long)(cs->g.IM + 1) * (cs->g.JM + 1) * (cs->g.KM + 1);
-    cs->p.resize(n);
-    cs->ucont.resize(3 * n);
-    rc = cv_get_field(c, CV_F_P, -1, cs->p.data());
-    if (rc == CV_OK) rc = cv_get_field(c, CV_F_UCONT, -1, cs->ucont.data());
-  }
-  if (c) cv_ctx_destroy(c);
-  return rc;
+  return close_case(cs, c, rc);
+}
+
+// bookkeeping of a finished case (lock held by the caller)
+static void retire(cv_server *S, Case *cs, int rc) {
+  S->in_use -= cs->bytes;
+  S->running--;
+  S->done++;
+  if (rc != CV_OK) cs->rc = rc;
+  cs->t_done = S->now();
+  cs->status = rc == CV_OK ? CV_CASE_DONE : CV_CASE_FAILED;
+}
+
+// admission of the head of the queue (lock held): a free slot and room in
+// the budget
+static Case *admit_one(cv_server *S) {
+  if (S->pending.empty() || S->running >= S->slots) return nullptr;
+  Case *h = S->cases[S->pending.front()].get();
+  if (S->in_use + h->bytes > S->budget) return nullptr;
+  S->pending.pop_front();
+  S->in_use += h->bytes;
+  if (S->in_use > S->peak) S->peak = S->in_use;
+  S->running++;
+  h->status = CV_CASE_RUNNING;
+  h->t_admit = S->now();
+  return h;
 }
 
 static void worker(cv_server *S) {
@@ -115,30 +162,158 @@ static void worker(cv_server *S) {
     const int rc = run_case(S, cs);
     {
       std::lock_guard<std::mutex> lk(S->mu);
-      S->in_use -= cs->bytes;
-      S->running--;
-      S->done++;
-      if (rc != CV_OK) cs->rc = rc;
-      cs->t_done = S->now();
-      cs->status = rc == CV_OK ? CV_CASE_DONE : CV_CASE_FAILED;
+      retire(S, cs, rc);
     }
     S->cv_done.notify_all();
     S->cv_work.notify_all();
   }
 }
 
-extern "C" int cv_server_create(int device, long hbm_budget_bytes, int slots, cv_server **out) {
+// ---------------- batched mode -----------------------------------------
+namespace {
+struct Live {
+  Case *cs;
+  cv_ctx *c;
+  int steps = 0;
+};
+struct Group {
+  cv_solver_opts o{};
+  std::vector<Live *> members;
+  cv_batch *b = nullptr;
+  bool batchable = true;
+};
+bool same_solver(const cv_solver_opts &a, const cv_solver_opts &b) { return memcmp(&a, &b, sizeof(a)) == 0; }
+}  // namespace
+
+// record one step of a case (lock taken here)
+static void record(cv_server *S, Live *l, double ms, const cv_step_stats &st, int r) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  l->cs->step_ms.push_back(ms);
+  l->cs->step_its.push_back(st.ksp_its);
+  l->cs->last = st;
+  if (r == CV_ERR_NOT_CONVERGED) l->cs->rc = r;
+  l->steps++;
+}
+
+static void scheduler(cv_server *S) {
+  std::vector<std::unique_ptr<Live>> live;
+  std::vector<Group> groups;
+  bool changed = false;
+  for (;;) {
+    std::vector<Case *> admitted;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_work.wait(lk, [&] {
+        if (S->stop || !live.empty()) return true;
+        if (S->pending.empty()) return false;
+        return S->in_use + S->cases[S->pending.front()]->bytes <= S->budget;
+      });
+      if (S->stop && live.empty()) break;
+      if (!S->stop)   // at destroy: finish the cases in flight, admit no more
+        while (Case *h = admit_one(S)) admitted.push_back(h);
+    }
+    for (Case *cs : admitted) {
+      cv_ctx *c = nullptr;
+      const int rc = open_case(S, cs, &c);
+      if (rc != CV_OK) {
+        close_case(cs, c, rc);
+        std::lock_guard<std::mutex> lk(S->mu);
+        retire(S, cs, rc);
+        S->cv_done.notify_all();
+        continue;
+      }
+      live.push_back(std::make_unique<Live>(Live{cs, c, 0}));
+      changed = true;
+    }
+    if (changed) {   // regroup by solver options and rebuild the batches
+      for (auto &g : groups)
+        if (g.b) cv_batch_destroy(g.b);
+      groups.clear();
+      for (auto &l : live) {
+        Group *g = nullptr;
+        for (auto &x : groups)
+          if (same_solver(x.o, l->cs->o) && x.members.size() < 64) { g = &x; break; }
+        if (!g) { groups.emplace_back(); g = &groups.back(); g->o = l->cs->o; }
+        g->members.push_back(l.get());
+      }
+      for (auto &g : groups) {
+        std::vector<cv_ctx *> cs;
+        for (Live *l : g.members) cs.push_back(l->c);
+        g.batchable = cv_batch_create(cs.data(), (int)cs.size(), &g.b) == CV_OK;
+        if (!g.batchable) g.b = nullptr;
+      }
+      changed = false;
+    }
+    // one step of every live case
+    std::vector<Live *> failed;
+    for (auto &g : groups) {
+      if (g.batchable) {
+        std::vector<cv_step_stats> st(g.members.size());
+        const auto a = clk::now();
+        const int r = cv_batch_step(g.b, st.data());
+        const double ms = 1e3 * std::chrono::duration<double>(clk::now() - a).count();
+        if (r != CV_OK && r != CV_ERR_NOT_CONVERGED) {
+          for (Live *l : g.members) { l->cs->rc = r; failed.push_back(l); }
+          continue;
+        }
+        for (size_t q = 0; q < g.members.size(); q++)
+          record(S, g.members[q], ms, st[q], st[q].converged ? CV_OK : CV_ERR_NOT_CONVERGED);
+      } else {
+        for (Live *l : g.members) {
+          cv_step_stats st{};
+          const auto a = clk::now();
+          const int r = cv_flow_step(l->c, &st);
+          const double ms = 1e3 * std::chrono::duration<double>(clk::now() - a).count();
+          if (r != CV_OK && r != CV_ERR_NOT_CONVERGED) { l->cs->rc = r; failed.push_back(l); continue; }
+          record(S, l, ms, st, r);
+        }
+      }
+    }
+    // retire finished and failed cases
+    for (size_t q = 0; q < live.size();) {
+      Live *l = live[q].get();
+      const bool bad = std::find(failed.begin(), failed.end(), l) != failed.end();
+      if (!bad && l->steps < l->cs->nsteps) { q++; continue; }
+      const int rc = close_case(l->cs, l->c, bad ? l->cs->rc : CV_OK);
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        retire(S, l->cs, bad ? (rc != CV_OK ? rc : l->cs->rc) : rc);
+      }
+      S->cv_done.notify_all();
+      live.erase(live.begin() + q);
+      changed = true;
+    }
+    if (changed)
+      for (auto &g : groups) {   // contexts may be gone: drop the batches now
+        if (g.b) cv_batch_destroy(g.b);
+        g.b = nullptr;
+      }
+  }
+  for (auto &g : groups)
+    if (g.b) cv_batch_destroy(g.b);
+  S->cv_done.notify_all();
+}
+
+extern "C" int cv_server_create_mode(int device, long hbm_budget_bytes, int slots, int mode, cv_server **out) {
   if (!out || hbm_budget_bytes <= 0 || slots < 1 || slots > 64) return CV_ERR_ARG;
+  if (mode != CV_SERVE_STREAMS && mode != CV_SERVE_BATCHED) return CV_ERR_ARG;
   *out = nullptr;
   CV_CHECK(hipSetDevice(device));
   cv_server *S = new cv_server();
   S->device = device;
+  S->mode = mode;
   S->budget = hbm_budget_bytes;
   S->slots = slots;
   S->t0 = clk::now();
-  for (int q = 0; q < slots; q++) S->workers.emplace_back(worker, S);
+  if (mode == CV_SERVE_BATCHED) S->workers.emplace_back(scheduler, S);
+  else
+    for (int q = 0; q < slots; q++) S->workers.emplace_back(worker, S);
   *out = S;
   return CV_OK;
+}
+
+extern "C" int cv_server_create(int device, long hbm_budget_bytes, int slots, cv_server **out) {
+  return cv_server_create_mode(device, hbm_budget_bytes, slots, CV_SERVE_STREAMS, out);
 }
 
 extern "C" int cv_server_submit(cv_server *S, const cv_grid_desc *g, const cv_solver_opts *o, const double *xyz,

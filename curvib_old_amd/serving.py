@@ -113,10 +113,12 @@ def _dp(a):
 class Server:
     """ctypes handle of one cv_server."""
 
-    def __init__(self, budget_bytes=200e9, slots=16, device=0):
+    def __init__(self, budget_bytes=200e9, slots=16, device=0, mode="streams"):
         L = _lib.load()
         h = C.c_void_p()
-        check(L.cv_server_create(int(device), int(budget_bytes), int(slots), C.byref(h)), "cv_server_create")
+        m = {"streams": _lib.CV_SERVE_STREAMS, "batched": _lib.CV_SERVE_BATCHED}[mode]
+        check(L.cv_server_create_mode(int(device), int(budget_bytes), int(slots), m, C.byref(h)),
+              "cv_server_create_mode")
         self.h = h
 
     def submit(self, case, arrays, nsteps, mglevels=None, semi=SEMI, **solver):
@@ -179,11 +181,13 @@ def latency_summary(ms):
 
 
 def run_stream(requests, arrivals, nsteps, precision, budget_bytes=200e9, slots=16, device=0, log=None,
-               **solver):
+               mode="streams", **solver):
     """Submit ``requests`` [(case, arrays)] at the wall-clock offsets
     ``arrivals`` (seconds, increasing), wait for all, and return per-case
-    records plus the aggregate figures.  Every case runs ``nsteps`` steps."""
-    srv = Server(budget_bytes, slots, device)
+    records plus the aggregate figures.  Every case runs ``nsteps`` steps.
+    mode: "streams" (a worker and stream per slot) or "batched" (all
+    admitted cases advance together as one batch)."""
+    srv = Server(budget_bytes, slots, device, mode)
     ids = []
     t0 = time.perf_counter()
     try:
@@ -210,6 +214,7 @@ def run_stream(requests, arrivals, nsteps, precision, budget_bytes=200e9, slots=
     last = max(r["info"]["t_done"] for r in recs)
     return {
         "precision": precision,
+        "mode": mode,
         "cases": len(recs),
         "failed": sum(1 for r in recs if r["rc"] != 0),
         "steps": int(all_ms.size),

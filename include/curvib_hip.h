@@ -187,7 +187,14 @@ typedef struct {
 /* device bytes one context for this grid and FGMRES restart length holds
  * (-1 for an unsupported description) */
 long cv_ctx_bytes(const cv_grid_desc *desc, int restart);
-int  cv_server_create(int device, long hbm_budget_bytes, int slots, cv_server **out);
+int  cv_server_create(int device, long hbm_budget_bytes, int slots, cv_server **out);   /* streams mode */
+/* mode CV_SERVE_STREAMS: one worker thread and stream per slot, each case
+ * stepped by cv_flow_step; CV_SERVE_BATCHED: one scheduler advances all
+ * admitted cases together, one cv_batch per group of equal solver options,
+ * rebuilt when cases join or leave (slots bounds the cases in flight) */
+#define CV_SERVE_STREAMS 0
+#define CV_SERVE_BATCHED 1
+int  cv_server_create_mode(int device, long hbm_budget_bytes, int slots, int mode, cv_server **out);
 int  cv_server_submit(cv_server *s, const cv_grid_desc *grid, const cv_solver_opts *opts, const double *xyz_nodes,
                       const double *nvert, const double *ucont, int nsteps, long *case_id);
 int  cv_server_wait(cv_server *s, long case_id, cv_case_info *info);      /* blocks until done or failed */

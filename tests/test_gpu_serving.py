@@ -26,10 +26,15 @@ def mix():
     return [(c, cases.build(c)) for c in cs]
 
 
-def run(reqs, slots, budget=50e9, nsteps=3):
-    srv = serving.Server(budget, slots)
+def run(reqs, slots, budget=50e9, nsteps=3, mode="streams", stagger=0.0):
+    import time
+    srv = serving.Server(budget, slots, mode=mode)
     try:
-        ids = [srv.submit(c, a, nsteps, **OPTS) for c, a in reqs]
+        ids = []
+        for q, (c, a) in enumerate(reqs):
+            if stagger and q:
+                time.sleep(stagger)
+            ids.append(srv.submit(c, a, nsteps, **OPTS))
         out = []
         for (c, _), cid in zip(reqs, ids):
             rc, info = srv.wait(cid)
@@ -84,3 +89,24 @@ def test_ctx_bytes_matches_allocation(mix):
     u.Flow_Solver()
     assert u.device_bytes() == serving.ctx_bytes(c, lev, OPTS["restart"])
     u.close()
+
+
+def test_batched_mode_equals_streams(mix):
+    """The batched server (all admitted cases advance as one cv_batch, rebuilt
+    as cases join and leave) gives every case the bits it gets in the
+    streams server; staggered arrivals change the batch mid-run."""
+    ref, _ = run(mix, slots=6, nsteps=6)
+    got, st = run(mix, slots=4, mode="batched", nsteps=6, stagger=0.005)
+    for (p0, u0, i0), (p1, u1, i1) in zip(ref, got):
+        assert i1["steps_done"] == 6
+        assert np.array_equal(p0, p1)
+        assert np.array_equal(u0, u1)
+    assert st["bytes_peak"] > max(i["bytes"] for _, _, i in got)
+
+
+def test_batched_mode_budget(mix):
+    b = sorted(serving.ctx_bytes(c, serving.levels_for(c.cells), OPTS["restart"]) for c, _ in mix)
+    budget = b[-1] + b[-2]
+    out, st = run(mix, slots=6, budget=budget, nsteps=1, mode="batched")
+    assert st["bytes_peak"] <= budget
+    assert all(i["steps_done"] == 1 for _, _, i in out)
