@@ -43,6 +43,11 @@ struct BCase {
   double *part, *dsc, *bsc;
   long n;
   int nlev;
+  // the step around the solve: Ucont, the cell face-area vectors, p
+  double *uc[3];
+  const double *csi[3], *eta[3], *zet[3];
+  double *p;
+  double dt, st;
 };
 
 struct BCtl {
@@ -228,6 +233,62 @@ typedef void (*BRedFn)(const BCase *, const uint32_t *, unsigned long long, cons
 static const BRedFn bmdot_tbl[] = {CV_NV_LIST(CV_BMDOT_ENTRY)};
 static const BRedFn borth_tbl[] = {CV_NV_LIST(CV_BORTH_ENTRY)};
 
+// ---- the step around the solve (no FGMRES mask) ----
+#define BK2_ARGS const BCase *__restrict__ cs, const uint32_t *__restrict__ tab, unsigned long long pm
+#define BK2_HEAD                                                   \
+  const uint32_t e_ = tab[blockIdx.x];                             \
+  const int c = (int)(e_ >> 24), lb = (int)(e_ & 0xffffffu);       \
+  if (!((pm >> c) & 1ull)) return;                                 \
+  const BCase &C = cs[c];
+#define BK2_GS (long)lb * BS + threadIdx.x, (long)C.lev[0].nvb * BS
+
+// VolumeFlux_rev: mode 0 sums IB flux and area, mode 1 applies dsc[108]
+__global__ void kb_ibm_flux(BK2_ARGS, int mode) {
+  BK2_HEAD
+  const BLev &L = C.lev[0];
+  ibm_flux_body(L.I, L.nv, C.uc[0], C.uc[1], C.uc[2], C.csi[0], C.csi[1], C.csi[2], C.eta[0], C.eta[1], C.eta[2],
+                C.zet[0], C.zet[1], C.zet[2], mode, C.dsc + 108, C.n, C.part, lb, L.nrd);
+}
+__global__ void kb_ibm_corr(const BCase *__restrict__ cs, unsigned long long pm) {
+  const int c = blockIdx.x;
+  if (!((pm >> c) & 1ull) || threadIdx.x) return;
+  ibm_corr_body(cs[c].dsc + 112, cs[c].dsc + 108);
+}
+__global__ void kb_poisson_rhs(BK2_ARGS) {
+  BK2_HEAD
+  const BLev &L = C.lev[0];
+  poisson_rhs_body(L.I, C.uc[0], C.uc[1], C.uc[2], L.Ad.aj, L.nv, C.rhs, C.dt, C.st, C.n, C.part, lb, L.nrd);
+}
+// dsc[slot + q] = sum of the fine-level partials, grid (cases, quantities)
+__global__ void kb_final2(const BCase *__restrict__ cs, unsigned long long pm, int slot) {
+  const int c = blockIdx.x;
+  if (!((pm >> c) & 1ull)) return;
+  const BCase &C = cs[c];
+  final_sum_body(C.part, C.lev[0].nrd, C.dsc + slot, blockIdx.y);
+}
+__global__ void kb_p_add(BK2_ARGS) {
+  BK2_HEAD
+  p_add_body(C.lev[0].I, C.p, C.phi, C.n, BK2_GS);
+}
+// UpdatePressure's periodic planes: axis 0 (x) or 2 (z), interior or full range
+__global__ void kb_pplane(BK2_ARGS, int axis, int full) {
+  BK2_HEAD
+  const Idx I = C.lev[0].I;
+  if (axis == 0)
+    pplane_x_body(I, C.p, C.phi, full ? 0 : 1, full ? I.my : I.my - 1, full ? 0 : 1, full ? I.mz : I.mz - 1, BK2_GS);
+  else
+    pplane_z_body(I, C.p, C.phi, full ? 0 : 1, full ? I.mx : I.mx - 1, full ? 0 : 1, full ? I.my : I.my - 1, BK2_GS);
+}
+__global__ void kb_projection(BK2_ARGS) {
+  BK2_HEAD
+  const BLev &L = C.lev[0];
+  const FaceMet Mi{{L.Ad.g[0], L.Ad.g[1], L.Ad.g[2]}};
+  const FaceMet Mj{{L.Ad.g[3], L.Ad.g[4], L.Ad.g[5]}};
+  const FaceMet Mk{{L.Ad.g[6], L.Ad.g[7], L.Ad.g[8]}};
+  projection_body(L.I, L.Ad.xper, L.Ad.zper, C.phi, L.nv, L.fl, Mi, Mj, Mk, C.uc[0], C.uc[1], C.uc[2], C.dt, C.st,
+                  C.n, BK2_GS);
+}
+
 // dsc[0..BSC) of every case -> out[c*BSC ..]
 __global__ void kb_gather(const BCase *__restrict__ cs, double *out) {
   const int c = blockIdx.x;
@@ -268,6 +329,10 @@ struct cv_batch {
   unsigned long long all = 0, has[CV_MAXLEV], inner[CV_MAXLEV], coarse[CV_MAXLEV], lgm[CV_MAXLEV][3];
   hipGraphExec_t gexec = nullptr;
   int use_graph = 1;
+  std::vector<hipGraphExec_t> gpre, gpost;   // per-case graphs of the work around the solve
+  CvMomBatch *mom = nullptr;                  // the momentum of all cases on the batch stream
+  int use_mom = 1;
+  unsigned long long xper = 0, zper = 0;
   std::vector<BFg> fg;
   std::vector<int> rc;
   std::vector<hipEvent_t> ev_case;
@@ -594,6 +659,11 @@ static void b_release(cv_batch *B) {
   (void)hipSetDevice(B->device);
   if (B->stream) (void)hipStreamSynchronize(B->stream);
   if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+  if (B->mom) cv_mom_batch_destroy(B->mom);
+  for (auto g : B->gpre)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (auto g : B->gpost)
+    if (g) (void)hipGraphExecDestroy(g);
   if (B->dcases) (void)hipFree(B->dcases);
   if (B->dctl) (void)hipFree(B->dctl);
   if (B->dbsc) (void)hipFree(B->dbsc);
@@ -644,6 +714,8 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
   B->kchunk = stencil_kchunk();
   const char *ge = getenv("CV_BATCH_GRAPH");
   B->use_graph = ge ? atoi(ge) != 0 : 1;
+  const char *gm = getenv("CV_BATCH_MOM");
+  B->use_mom = gm ? atoi(gm) != 0 : 1;
   int rc = CV_OK;
   std::vector<BCase> hc(ncases);
   std::vector<uint32_t> tabs;
@@ -700,9 +772,18 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
       B->lnc[d][g] = std::max(B->lnc[d][g], (rows + b.cs - 1) / b.cs);
     }
     CvLevel &F = x->lev[x->nlev - 1];
+    if (F.d.xper) B->xper |= bit(c);
+    if (F.d.zper) B->zper |= bit(c);
     bc.V = x->V; bc.Z = x->Z; bc.w = x->w; bc.phi = x->phi; bc.rhs = x->bvec;
     bc.part = x->part; bc.dsc = x->dsc;
     bc.n = F.d.n;
+    for (int q = 0; q < 3; q++) {
+      bc.uc[q] = x->ucont[q];
+      bc.csi[q] = x->m.csi[q]; bc.eta[q] = x->m.eta[q]; bc.zet[q] = x->m.zet[q];
+    }
+    bc.p = x->p;
+    bc.dt = x->dt;
+    bc.st = x->st;
     B->all |= bit(c);
   }
   for (int d = 0; d < B->maxlev; d++) {
@@ -737,6 +818,9 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
   CV_CHECK(hipMemcpy(B->dcases, hc.data(), sizeof(BCase) * ncases, hipMemcpyHostToDevice));
   if (!tabs.empty()) CV_CHECK(hipMemcpy(B->dtab, tabs.data(), sizeof(uint32_t) * tabs.size(), hipMemcpyHostToDevice));
   CV_CHECK(hipMemcpy(B->dbsc, B->hbsc, sizeof(double) * BSC * ncases, hipMemcpyHostToDevice));
+  if ((rc = cv_mom_batch_create(cases, ncases, B->stream, &B->mom))) { b_release(B); return rc; }
+  B->gpre.assign(ncases, nullptr);
+  B->gpost.assign(ncases, nullptr);
   B->ev_case.resize(ncases);
   for (int c = 0; c < ncases; c++) CV_CHECK(hipEventCreateWithFlags(&B->ev_case[c], hipEventDisableTiming));
   CV_CHECK(hipEventCreateWithFlags(&B->ev_solved, hipEventDisableTiming));
@@ -764,19 +848,12 @@ static int b_check(cv_batch *B) {
   return CV_OK;
 }
 
-// VolumeFlux_rev and PoissonRHS of every case on its own stream, the batched
-// solve on the batch stream, then each case's stream waits for the solve
-static int b_solve_all(cv_batch *B, double *t_p) {
-  int rc;
-  for (int c = 0; c < B->n; c++) {
-    cv_ctx *x = B->cs[c];
-    if ((rc = cv_volumeflux_rev_dev(x, nullptr))) return rc;
-    if ((rc = cv_poisson_rhs_dev(x, nullptr))) return rc;
-    CV_CHECK(hipEventRecord(B->ev_case[c], x->stream));
-    CV_CHECK(hipStreamWaitEvent(B->stream, B->ev_case[c], 0));
-  }
+// the batched solve on the batch stream once every case's stream has
+// recorded ev_case; then each case's stream waits for the solve
+static int b_solve(cv_batch *B, double *t_p) {
+  for (int c = 0; c < B->n; c++) CV_CHECK(hipStreamWaitEvent(B->stream, B->ev_case[c], 0));
   CV_CHECK(hipEventRecord(B->ev_t0, B->stream));
-  rc = b_fgmres(B);
+  int rc = b_fgmres(B);
   if (rc) return rc;
   CV_CHECK(hipEventRecord(B->ev_t1, B->stream));
   CV_CHECK(hipEventRecord(B->ev_solved, B->stream));
@@ -788,6 +865,41 @@ static int b_solve_all(cv_batch *B, double *t_p) {
     *t_p = ms * 1e-3;
   }
   return CV_OK;
+}
+
+// VolumeFlux_rev and PoissonRHS of a case (statistics stay on the device)
+static int b_rhs_work(cv_ctx *x) {
+  int rc = cv_volumeflux_rev_dev(x, nullptr);
+  return rc ? rc : cv_poisson_rhs_dev(x, nullptr);
+}
+
+// a case's kernels of `work` captured once on its stream into *g, then
+// replayed; the first call captures and launches
+template <class W>
+static int b_graph_run(cv_ctx *x, hipGraphExec_t *g, W &&work) {
+  if (!*g) {
+    hipGraph_t gg = nullptr;
+    CV_CHECK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeRelaxed));
+    const int rc = work();
+    const hipError_t e = hipStreamEndCapture(x->stream, &gg);
+    if (rc) {
+      if (e == hipSuccess && gg) (void)hipGraphDestroy(gg);
+      return rc;
+    }
+    if (e != hipSuccess) { cv_set_error("hipStreamEndCapture: %s", hipGetErrorString(e)); return CV_ERR_HIP; }
+    const hipError_t ei = hipGraphInstantiate(g, gg, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gg);
+    if (ei != hipSuccess) { *g = nullptr; cv_set_error("hipGraphInstantiate: %s", hipGetErrorString(ei)); return CV_ERR_HIP; }
+  }
+  CV_CHECK(hipGraphLaunch(*g, x->stream));
+  return CV_OK;
+}
+
+// the per-case work before the solve is a fixed kernel sequence once the
+// BDF history exists (steps_done > 0) for explicit RK with no kernel timing;
+// ImpRK synchronises with the host and always runs eagerly
+static bool b_pre_graphable(const cv_batch *B, const cv_ctx *x) {
+  return B->use_graph && x->steps_done > 0 && x->mom.scheme == CV_MOM_RK && x->prof.kind == CV_PROF_NONE;
 }
 
 // statistics of the solve (and of the step) of every case, read back after
@@ -819,11 +931,82 @@ extern "C" int cv_batch_poisson_solve(cv_batch *B, cv_step_stats *st) {
   int rc = b_check(B);
   if (rc) return rc;
   for (int c = 0; c < B->n; c++) st[c] = cv_step_stats{};
+  for (int c = 0; c < B->n; c++) {
+    if ((rc = b_rhs_work(B->cs[c]))) return rc;
+    CV_CHECK(hipEventRecord(B->ev_case[c], B->cs[c]->stream));
+  }
   double tp = 0;
-  if ((rc = b_solve_all(B, &tp))) return rc;
+  if ((rc = b_solve(B, &tp))) return rc;
   if ((rc = b_stats(B, st, tp))) return rc;
   for (int c = 0; c < B->n; c++)
     if (B->rc[c]) return B->rc[c];
+  return CV_OK;
+}
+
+static int b_finish(cv_batch *B, cv_step_stats *st, double tp);
+
+// every case runs explicit RK with kernel timing off: the whole step
+// around the solve can then run on the batch stream, one launch per kernel
+static bool b_mom_batchable(const cv_batch *B) {
+  if (!B->use_mom || !B->mom) return false;
+  for (const cv_ctx *x : B->cs)
+    if (x->mom.scheme != CV_MOM_RK || x->prof.kind != CV_PROF_NONE) return false;
+  return true;
+}
+
+#define B2_LAUNCH(kern, t, pm, ...)                                                                          \
+  do {                                                                                                      \
+    if ((t).len > 0 && (pm))                                                                                \
+      hipLaunchKernelGGL(kern, dim3((unsigned)(t).len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,   \
+                         (const uint32_t *)BTAB(t), (unsigned long long)(pm), ##__VA_ARGS__);               \
+  } while (0)
+
+// rotation + RK (cv_step_pre_dev), VolumeFlux_rev and PoissonRHS of every
+// case on the batch stream
+static int b_pre_batched(cv_batch *B) {
+  unsigned long long rot = 0;
+  for (int c = 0; c < B->n; c++) {   // the batch stream follows each case's earlier work
+    CV_CHECK(hipEventRecord(B->ev_case[c], B->cs[c]->stream));
+    CV_CHECK(hipStreamWaitEvent(B->stream, B->ev_case[c], 0));
+    if (B->cs[c]->steps_done > 0) rot |= bit(c);
+  }
+  int rc = cv_mom_batch_rk(B->mom, rot);
+  if (rc) return rc;
+  const unsigned long long all = B->all;
+  const unsigned nc = (unsigned)B->n;
+  for (int pass = 0; pass < 3; pass++) {
+    const int mode = pass == 1 ? 1 : 0;
+    B2_LAUNCH(kb_ibm_flux, B->rd[0], all, mode);
+    if (!mode) {
+      hipLaunchKernelGGL(kb_final2, dim3(nc, 2), dim3(BS), 0, B->stream, (const BCase *)B->dcases, all,
+                         pass == 0 ? 112 : 114);
+      if (pass == 0) hipLaunchKernelGGL(kb_ibm_corr, dim3(nc), dim3(64), 0, B->stream, (const BCase *)B->dcases, all);
+    }
+  }
+  B2_LAUNCH(kb_poisson_rhs, B->rd[0], all);
+  hipLaunchKernelGGL(kb_final2, dim3(nc, 1), dim3(BS), 0, B->stream, (const BCase *)B->dcases, all, 104);
+  CV_CHECK(hipGetLastError());
+  CV_CHECK(hipEventRecord(B->ev_t0, B->stream));
+  rc = b_fgmres(B);
+  if (rc) return rc;
+  CV_CHECK(hipEventRecord(B->ev_t1, B->stream));
+  return CV_OK;
+}
+
+// UpdatePressure and Projection (with Contra2Cart and the ghost update) of
+// every case on the batch stream; each case's stream then follows
+static int b_post_batched(cv_batch *B) {
+  const unsigned long long all = B->all;
+  B2_LAUNCH(kb_p_add, B->vc[0], all);
+  B2_LAUNCH(kb_pplane, B->vc[0], B->xper, 0, 0);
+  B2_LAUNCH(kb_pplane, B->vc[0], B->zper, 2, 0);
+  B2_LAUNCH(kb_pplane, B->vc[0], B->xper, 0, 1);
+  B2_LAUNCH(kb_pplane, B->vc[0], B->zper, 2, 1);
+  B2_LAUNCH(kb_projection, B->vc[0], all);
+  int rc = cv_mom_batch_ghost(B->mom, 0, 1);
+  if (rc) return rc;
+  CV_CHECK(hipEventRecord(B->ev_solved, B->stream));
+  for (int c = 0; c < B->n; c++) CV_CHECK(hipStreamWaitEvent(B->cs[c]->stream, B->ev_solved, 0));
   return CV_OK;
 }
 
@@ -833,12 +1016,42 @@ extern "C" int cv_batch_step(cv_batch *B, cv_step_stats *st) {
   int rc = b_check(B);
   if (rc) return rc;
   for (int c = 0; c < B->n; c++) st[c] = cv_step_stats{};
-  for (int c = 0; c < B->n; c++)
-    if ((rc = cv_step_pre_dev(B->cs[c], &st[c]))) return rc;
   double tp = 0;
-  if ((rc = b_solve_all(B, &tp))) return rc;
-  for (int c = 0; c < B->n; c++)
-    if ((rc = cv_step_post_dev(B->cs[c]))) return rc;
+  if (b_mom_batchable(B)) {
+    if ((rc = b_pre_batched(B))) return rc;
+    if ((rc = b_post_batched(B))) return rc;
+    CV_CHECK(hipEventSynchronize(B->ev_t1));
+    float ms = 0;
+    CV_CHECK(hipEventElapsedTime(&ms, B->ev_t0, B->ev_t1));
+    tp = ms * 1e-3;
+    return b_finish(B, st, tp);
+  }
+  // momentum, VolumeFlux_rev and PoissonRHS per case on its own stream
+  for (int c = 0; c < B->n; c++) {
+    cv_ctx *x = B->cs[c];
+    cv_step_stats *sc = &st[c];
+    auto pre = [x, sc]() {
+      const int r = cv_step_pre_dev(x, sc);
+      return r ? r : b_rhs_work(x);
+    };
+    rc = b_pre_graphable(B, x) ? b_graph_run(x, &B->gpre[c], pre) : pre();
+    if (rc) return rc;
+    CV_CHECK(hipEventRecord(B->ev_case[c], x->stream));
+  }
+  if ((rc = b_solve(B, &tp))) return rc;
+  // UpdatePressure and Projection per case
+  for (int c = 0; c < B->n; c++) {
+    cv_ctx *x = B->cs[c];
+    auto post = [x]() { return cv_step_post_dev(x); };
+    rc = (B->use_graph && x->prof.kind == CV_PROF_NONE) ? b_graph_run(x, &B->gpost[c], post) : post();
+    if (rc) return rc;
+  }
+  return b_finish(B, st, tp);
+}
+
+// statistics, Divergence and the step count of every case
+static int b_finish(cv_batch *B, cv_step_stats *st, double tp) {
+  int rc;
   if ((rc = b_stats(B, st, tp))) return rc;
   int conv_rc = CV_OK;
   for (int c = 0; c < B->n; c++) {

@@ -146,6 +146,13 @@ int cv_calcrhs_dev(cv_ctx *c);
 int cv_step_pre_dev(cv_ctx *c, cv_step_stats *st);
 int cv_step_post_dev(cv_ctx *c);
 
+// batched momentum over many contexts on one stream (cv_momentum.hip)
+struct CvMomBatch;
+int cv_mom_batch_create(cv_ctx *const *cases, int n, hipStream_t s, CvMomBatch **out);
+void cv_mom_batch_destroy(CvMomBatch *mb);
+int cv_mom_batch_rk(CvMomBatch *mb, unsigned long long rot);
+int cv_mom_batch_ghost(CvMomBatch *mb, int formbcs, int cart);
+
 __device__ inline int cv_wrap(int i, int m, int per) {
   return per ? (i == m - 1 ? 1 : (i == 0 ? m - 2 : i)) : i;
 }

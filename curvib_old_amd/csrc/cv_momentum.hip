@@ -13,6 +13,8 @@
 // family array, as the reference's Fp1/Fp2/Fp3 temporaries; the arithmetic
 // per face is the reference's.
 #include "cv_internal.h"
+#include <cstring>
+#include <algorithm>
 #include "cv_reduce.h"
 #include <cmath>
 
@@ -61,6 +63,19 @@ inline unsigned nblk(long n) {
   if (i < (g).mx && j < (g).my)                                                              \
     for (int k = k0_; k < k1_; k++)
 
+// the same launch shape with the block given explicitly (batched launches)
+#define CV_MARCH_B(g, bx_, by_, bz_)                                                          \
+  const int i = (bx_) * 64 + threadIdx.x, j = (by_) * MARCH_TY + threadIdx.y;               \
+  const int k0_ = (bz_) * MARCH_KC, k1_ = min(k0_ + MARCH_KC, (g).mz);                      \
+  if (i < (g).mx && j < (g).my)                                                              \
+    for (int k = k0_; k < k1_; k++)
+
+// a grid-stride partition: first index and stride of this thread
+struct GS {
+  long t0, step;
+};
+#define GS_SELF GS{blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x}
+
 inline dim3 march_grid(int mx, int my, int mz) {
   return dim3((unsigned)((mx + 63) / 64), (unsigned)((my + MARCH_TY - 1) / MARCH_TY),
               (unsigned)((mz + MARCH_KC - 1) / MARCH_KC));
@@ -70,9 +85,9 @@ inline dim3 march_grid(int mx, int my, int mz) {
 // ------------------------------------------------------------------
 // Contra2Cart (rhs.c:25-144)
 // ------------------------------------------------------------------
-__global__ void k_contra2cart(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n) {
+__device__ __forceinline__ void contra2cart_body(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n, int bx, int by, int bz) {
   (void)n;
-  CV_MARCH(g) {
+  CV_MARCH_B(g, bx, by, bz) {
     const long t = g.at(i, j, k);
     if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
     const double nn = nv[t];
@@ -91,18 +106,20 @@ __global__ void k_contra2cart(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *
     ucat.z[t] = det2 / det;
   }
 }
+__global__ void k_contra2cart(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n) { contra2cart_body(g, uc, csi, eta, zet, nv, ucat, n, blockIdx.x, blockIdx.y, blockIdx.z); }
+
 
 // ------------------------------------------------------------------
 // ghost planes (FormBCS bcs.c:5252-5731, GhostNodeVelocity bcs.c:4015-4415)
 // ------------------------------------------------------------------
 // wall ghosts: ucat_g = 2*ubcs - ucat_inner with ubcs = 0.  full: whole
 // tangential ranges (GhostNodeVelocity); otherwise interior ranges (FormBCS)
-__global__ void k_wall_ghost(G g, int bx, int bz, int full, V3 u) {
+__device__ __forceinline__ void wall_ghost_body(G g, int bx, int bz, int full, V3 u, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   const int a0 = full ? 0 : 1;
   const long nyz = (long)my * mz, nxz = (long)mx * mz, nxy = (long)mx * my;
   const long tot = nyz + nxz + nxy;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+  for (long t = gs.t0; t < tot; t += gs.step) {
     long lo, hi, ln, in_lo, in_hi;
     if (t < nyz) {                       // x planes
       if (bx) continue;
@@ -126,13 +143,15 @@ __global__ void k_wall_ghost(G g, int bx, int bz, int full, V3 u) {
     u.x[hi] = 2 * 0.0 - u.x[in_hi]; u.y[hi] = 2 * 0.0 - u.y[in_hi]; u.z[hi] = 2 * 0.0 - u.z[in_hi];
   }
 }
+__global__ void k_wall_ghost(G g, int bx, int bz, int full, V3 u) { wall_ghost_body(g, bx, bz, full, u, GS_SELF); }
+
 
 // periodic plane copies restricted to the interior of the other axes
 // (bcs.c:5325-5431): all four planes read only interior cells
-__global__ void k_periodic_planes(G g, V3 u, double *p, double *nv) {
+__device__ __forceinline__ void periodic_planes_body(G g, V3 u, double *p, double *nv, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   const long nyz = (long)my * mz, nxy = (long)mx * my;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nyz + nxy; t += (long)gridDim.x * blockDim.x) {
+  for (long t = gs.t0; t < nyz + nxy; t += gs.step) {
     long d0, s0, d1, s1;
     if (t < nyz) {
       if (!g.xper) continue;
@@ -151,9 +170,11 @@ __global__ void k_periodic_planes(G g, V3 u, double *p, double *nv) {
     if (p) { p[d0] = p[s0]; p[d1] = p[s1]; nv[d0] = nv[s0]; nv[d1] = nv[s1]; }
   }
 }
+__global__ void k_periodic_planes(G g, V3 u, double *p, double *nv) { periodic_planes_body(g, u, p, nv, GS_SELF); }
+
 
 // edge averages, twelve sequential sub-sweeps (bcs.c:5440-5566); one block
-__global__ void k_edge_average(G g, V3 u, double *p) {
+__device__ __forceinline__ void edge_average_body(G g, V3 u, double *p) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   for (int sb = 0; sb < 12; sb++) {
     int len;
@@ -183,13 +204,14 @@ __global__ void k_edge_average(G g, V3 u, double *p) {
     __syncthreads();
   }
 }
+__global__ void k_edge_average(G g, V3 u, double *p) { edge_average_body(g, u, p); }
 
 // periodic full-plane copies (bcs.c:5567-5731); axis 0 then 2.  x_only:
 // the reference copies only .x on the upper x plane in FormBCS.
-__global__ void k_periodic_full(G g, int axis, int x_only, V3 u, double *p, double *nv) {
+__device__ __forceinline__ void periodic_full_body(G g, int axis, int x_only, V3 u, double *p, double *nv, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   const long n = axis == 0 ? (long)my * mz : (long)mx * my;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+  for (long t = gs.t0; t < n; t += gs.step) {
     long d0, s0, d1, s1;
     if (axis == 0) {
       int j = (int)(t % my), k = (int)(t / my);
@@ -204,13 +226,15 @@ __global__ void k_periodic_full(G g, int axis, int x_only, V3 u, double *p, doub
     if (p) { p[d0] = p[s0]; p[d1] = p[s1]; nv[d0] = nv[s0]; nv[d1] = nv[s1]; }
   }
 }
+__global__ void k_periodic_full(G g, int axis, int x_only, V3 u, double *p, double *nv) { periodic_full_body(g, axis, x_only, u, p, nv, GS_SELF); }
+
 
 // zero wall-normal flux of bctype-1 faces after an RK stage (the build's
 // deliberate deviation: the reference never resets it)
-__global__ void k_wall_flux_zero(G g, int bx, int bz, V3 u) {
+__device__ __forceinline__ void wall_flux_zero_body(G g, int bx, int bz, V3 u, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   const long nyz = (long)my * mz, nxz = (long)mx * mz, nxy = (long)mx * my;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nyz + nxz + nxy; t += (long)gridDim.x * blockDim.x) {
+  for (long t = gs.t0; t < nyz + nxz + nxy; t += gs.step) {
     if (t < nyz) {
       if (!bx) continue;
       int j = (int)(t % my), k = (int)(t / my);
@@ -227,6 +251,8 @@ __global__ void k_wall_flux_zero(G g, int bx, int bz, V3 u) {
     }
   }
 }
+__global__ void k_wall_flux_zero(G g, int bx, int bz, V3 u) { wall_flux_zero_body(g, bx, bz, u, GS_SELF); }
+
 
 // ------------------------------------------------------------------
 // Face fluxes: QUICK convection (rhs.c:1636-1897) and the viscous flux
@@ -484,17 +510,18 @@ __global__ void __launch_bounds__(64 * TY) k_rc_fused(G g, C3 uc, C3 u, const do
 // through the periodic ghost map on x and z or left zero: no face rule
 // reads them.
 template <int TY>
-__global__ void __launch_bounds__(64 * TY) k_rc_tiled(G g, C3 uc, C3 u, const double *nv, FM mi, FM mj, FM mk, double nu,
-                                                    C3 csi, C3 eta, C3 zet, const double *aj, V3 rct) {
+__device__ __forceinline__ void rc_tiled_body(const G &g, C3 uc, C3 u, const double *nv, const FM &mi, const FM &mj,
+                                              const FM &mk, double nu, C3 csi, C3 eta, C3 zet, const double *aj, V3 rct,
+                                              int bx, int by, int bz) {
   constexpr int TW = 67, TH = TY + 3, NP = TW * TH, NT = 64 * TY;
   __shared__ double SU[3][3][NP];
   __shared__ unsigned char SN[3][NP];
   __shared__ double SF[3][TY][64];
   const int L = threadIdx.x, R = threadIdx.y, tid = R * 64 + L;
   const int mx = g.mx, my = g.my, mz = g.mz;
-  const int i0 = 63 * blockIdx.x, j0 = (TY - 1) * blockIdx.y;
+  const int i0 = 63 * bx, j0 = (TY - 1) * by;
   const int i = i0 - 1 + L, j = j0 - 1 + R;
-  const int k0 = RC_KC * blockIdx.z, k1 = min(k0 + RC_KC, mz);
+  const int k0 = RC_KC * bz, k1 = min(k0 + RC_KC, mz);
   const bool in_i = i >= 1 && i <= mx - 2, in_j = j >= 1 && j <= my - 2;
   const bool out = L >= 1 && R >= 1 && in_i && in_j;
   const bool do0 = R >= 1 && in_j && i >= 0 && i <= mx - 2;
@@ -570,11 +597,17 @@ __global__ void __launch_bounds__(64 * TY) k_rc_tiled(G g, C3 uc, C3 u, const do
   }
 }
 
+template <int TY>
+__global__ void __launch_bounds__(64 * TY) k_rc_tiled(G g, C3 uc, C3 u, const double *nv, FM mi, FM mj, FM mk, double nu,
+                                                    C3 csi, C3 eta, C3 zet, const double *aj, V3 rct) {
+  rc_tiled_body<TY>(g, uc, u, nv, mi, mj, mk, nu, csi, eta, zet, aj, rct, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 // periodic copies of rct planes (DMLocalToLocal + rhs.c:2725-2781)
-__global__ void k_rct_periodic(G g, V3 rct) {
+__device__ __forceinline__ void rct_periodic_body(G g, V3 rct, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   const long nyz = (long)my * mz, nxy = (long)mx * my;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nyz + nxy; t += (long)gridDim.x * blockDim.x) {
+  for (long t = gs.t0; t < nyz + nxy; t += gs.step) {
     if (t < nyz) {
       if (!g.xper) continue;
       int j = (int)(t % my), k = (int)(t / my);
@@ -591,6 +624,8 @@ __global__ void k_rct_periodic(G g, V3 rct) {
     }
   }
 }
+__global__ void k_rct_periodic(G g, V3 rct) { rct_periodic_body(g, rct, GS_SELF); }
+
 
 // ------------------------------------------------------------------
 // FormFunction1 face right-hand side (rhs.c:2799-3306)
@@ -649,10 +684,10 @@ __device__ inline long prev_cell(const G &g, int i, int j, int k) {
   return -1;
 }
 
-__global__ void __launch_bounds__(BS) k_ff1_faces(FFArgs A, V3 rhs, long n) {
+__device__ __forceinline__ void ff1_faces_body(FFArgs A, V3 rhs, long n, int bx, int by, int bz) {
   const G &g = A.g;
   (void)n;
-  CV_MARCH(g) {
+  CV_MARCH_B(g, bx, by, bz) {
     const long t = g.at(i, j, k);
     if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
     const double *p = A.p;
@@ -689,6 +724,8 @@ __global__ void __launch_bounds__(BS) k_ff1_faces(FFArgs A, V3 rhs, long n) {
     rhs.x[t] = rx; rhs.y[t] = ry; rhs.z[t] = rz;
   }
 }
+__global__ void __launch_bounds__(BS) k_ff1_faces(FFArgs A, V3 rhs, long n) { ff1_faces_body(A, rhs, n, blockIdx.x, blockIdx.y, blockIdx.z); }
+
 
 // the loop state after the main FormFunction1 sweep: the z-face values of
 // the last interior cell, per variable (dpdc, dpde, dpdz)
@@ -718,11 +755,11 @@ __device__ inline void main_end_state(const FFArgs &A, double *dc, double *de, d
 // periodic boundary faces (rhs.c:3032-3278): x faces at i=0 (k outer, j
 // inner), then z faces at k=0 (j outer, i inner).  A DROP keeps the value of
 // the previous face in that order, found by walking back.
-__global__ void k_ff1_periodic(FFArgs A, V3 rhs) {
+__device__ __forceinline__ void ff1_periodic_body(FFArgs A, V3 rhs, GS gs) {
   const G &g = A.g;
   const int ny = g.my - 2, nxk = g.mx - 2, nzk = g.mz - 2;
   const long nxf = A.g.xper ? (long)ny * nzk : 0, nzf = A.g.zper ? (long)ny * nxk : 0;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nxf + nzf; t += (long)gridDim.x * blockDim.x) {
+  for (long t = gs.t0; t < nxf + nzf; t += gs.step) {
     const double *p = A.p;
     if (t < nxf) {
       int j = 1 + (int)(t % ny), k = 1 + (int)(t / ny);
@@ -790,14 +827,18 @@ __global__ void k_ff1_periodic(FFArgs A, V3 rhs) {
     }
   }
 }
+__global__ void k_ff1_periodic(FFArgs A, V3 rhs) { ff1_periodic_body(A, rhs, GS_SELF); }
 
-__global__ void k_rk_update(double a, C3 uo, C3 r, V3 u, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+
+__device__ __forceinline__ void rk_update_body(double a, C3 uo, C3 r, V3 u, long n, GS gs) {
+  for (long t = gs.t0; t < n; t += gs.step) {
     u.x[t] = a * r.x[t] + uo.x[t];
     u.y[t] = a * r.y[t] + uo.y[t];
     u.z[t] = a * r.z[t] + uo.z[t];
   }
 }
+__global__ void k_rk_update(double a, C3 uo, C3 r, V3 u, long n) { rk_update_body(a, uo, r, u, n, GS_SELF); }
+
 
 
 // CalcRHS (implicitsolver.c:6449-6741) after FormFunction1: Rhs += (-1/dt)
@@ -1209,4 +1250,251 @@ extern "C" int cv_calcrhs(cv_ctx *c, double *rhs_out) {
   if ((rc = cv_calcrhs_dev(c))) return rc;
   CV_CHECK(hipStreamSynchronize(c->stream));
   return rhs_out ? cv_get_field(c, CV_F_RHS, -1, rhs_out) : CV_OK;
+}
+
+// ====================================================================
+// batched momentum: the level rotation, the explicit RK step and the
+// ghost updates of many cases, one launch per kernel covering every case.
+// A block table maps each block of a launch to (case, block of that case's
+// own launch shape) and the block runs the single-case body, so each case
+// gets the bits cv_rk_dev gives it alone.
+// ====================================================================
+namespace {
+
+struct MCase {
+  G g;
+  double *uc[3], *uo[3], *urm1[3], *uca[3], *rhs[3], *rct[3];
+  double *p, *nv;
+  FFArgs ff;
+  FM mi, mj, mk;
+  C3 csi, eta, zet;
+  const double *aj;
+  double nu, dt, st;
+  int wx, wz;               // bctype[0] == 1, bctype[4] == 1 (wall-normal flux reset)
+  long n;
+  int mgx, mgy, mgz;        // k-marching blocks
+  int rgx, rgy, rgz;        // face-flux tiles (8 rows)
+  int nvb, npb;             // blocks of an elementwise and of a plane pass
+};
+
+#define MB_ARGS const MCase *__restrict__ cs, const uint32_t *__restrict__ tab, unsigned long long pm
+#define MB_HEAD                                            \
+  const uint32_t e_ = tab[blockIdx.x];                     \
+  const int c = (int)(e_ >> 24), lb = (int)(e_ & 0xffffffu); \
+  if (!((pm >> c) & 1ull)) return;                         \
+  const MCase &C = cs[c];
+#define MB_GSV GS{(long)lb * BS + threadIdx.x, (long)C.nvb * BS}
+#define MB_GSP GS{(long)lb * BS + threadIdx.x, (long)C.npb * BS}
+
+__device__ inline C3 mc3(double *const a[3]) { return C3{a[0], a[1], a[2]}; }
+__device__ inline V3 mv3(double *const a[3]) { return V3{a[0], a[1], a[2]}; }
+
+// which 0: Ucont_rm1 <- Ucont_o; 1: Ucont_o <- Ucont; 2: Rhs <- 0
+__global__ void kbm_vec3(MB_ARGS, int which) {
+  MB_HEAD
+  const GS gs = MB_GSV;
+  for (int q = 0; q < 3; q++) {
+    double *d = which == 0 ? C.urm1[q] : which == 1 ? C.uo[q] : C.rhs[q];
+    const double *s = which == 0 ? C.uo[q] : C.uc[q];
+    for (long t = gs.t0; t < C.n; t += gs.step) d[t] = which == 2 ? 0.0 : s[t];
+  }
+}
+
+__global__ void __launch_bounds__(64 * MARCH_TY) kbm_contra2cart(MB_ARGS) {
+  MB_HEAD
+  const int bx = lb % C.mgx, by = (lb / C.mgx) % C.mgy, bz = lb / (C.mgx * C.mgy);
+  contra2cart_body(C.g, mc3(C.uc), C.csi, C.eta, C.zet, C.nv, mv3(C.uca), C.n, bx, by, bz);
+}
+
+__global__ void kbm_wall_ghost(MB_ARGS, int full) {
+  MB_HEAD
+  wall_ghost_body(C.g, C.g.xper, C.g.zper, full, mv3(C.uca), MB_GSP);
+}
+
+__global__ void kbm_periodic_planes(MB_ARGS, int formbcs) {
+  MB_HEAD
+  periodic_planes_body(C.g, mv3(C.uca), formbcs ? C.p : nullptr, formbcs ? C.nv : nullptr, MB_GSP);
+}
+
+__global__ void kbm_edge_average(MB_ARGS, int formbcs) {
+  MB_HEAD
+  (void)lb;
+  edge_average_body(C.g, mv3(C.uca), formbcs ? C.p : nullptr);
+}
+
+__global__ void kbm_periodic_full(MB_ARGS, int axis, int formbcs) {
+  MB_HEAD
+  periodic_full_body(C.g, axis, axis == 0 ? formbcs : 0, mv3(C.uca), formbcs ? C.p : nullptr,
+                     formbcs ? C.nv : nullptr, MB_GSP);
+}
+
+__global__ void __launch_bounds__(64 * 8) kbm_rc_tiled(MB_ARGS) {
+  MB_HEAD
+  const int bx = lb % C.rgx, by = (lb / C.rgx) % C.rgy, bz = lb / (C.rgx * C.rgy);
+  rc_tiled_body<8>(C.g, mc3(C.uc), mc3(C.uca), C.nv, C.mi, C.mj, C.mk, C.nu, C.csi, C.eta, C.zet, C.aj, mv3(C.rct),
+                   bx, by, bz);
+}
+
+__global__ void kbm_rct_periodic(MB_ARGS) {
+  MB_HEAD
+  rct_periodic_body(C.g, mv3(C.rct), MB_GSP);
+}
+
+__global__ void __launch_bounds__(BS) kbm_ff1_faces(MB_ARGS) {
+  MB_HEAD
+  const int bx = lb % C.mgx, by = (lb / C.mgx) % C.mgy, bz = lb / (C.mgx * C.mgy);
+  ff1_faces_body(C.ff, mv3(C.rhs), C.n, bx, by, bz);
+}
+
+__global__ void kbm_ff1_periodic(MB_ARGS) {
+  MB_HEAD
+  ff1_periodic_body(C.ff, mv3(C.rhs), MB_GSP);
+}
+
+__global__ void kbm_rk_update(MB_ARGS, double alfa) {
+  MB_HEAD
+  const double a = alfa * C.dt * C.st;
+  rk_update_body(a, mc3(C.uo), mc3(C.rhs), mv3(C.uc), C.n, MB_GSV);
+}
+
+__global__ void kbm_wall_flux_zero(MB_ARGS) {
+  MB_HEAD
+  wall_flux_zero_body(C.g, C.wx, C.wz, mv3(C.uc), MB_GSP);
+}
+
+}  // namespace
+
+struct CvMomBatch {
+  int n = 0;
+  hipStream_t s = nullptr;
+  std::vector<cv_ctx *> cs;
+  MCase *d = nullptr;
+  uint32_t *tab = nullptr;
+  long t_march = 0, t_tile = 0, t_vec = 0, t_plane = 0, t_one = 0;
+  int n_march = 0, n_tile = 0, n_vec = 0, n_plane = 0, n_one = 0;
+  unsigned long long all = 0, xper = 0, zper = 0, per = 0;
+};
+
+void cv_mom_batch_destroy(CvMomBatch *mb) {
+  if (!mb) return;
+  if (mb->d) (void)hipFree(mb->d);
+  if (mb->tab) (void)hipFree(mb->tab);
+  delete mb;
+}
+
+int cv_mom_batch_create(cv_ctx *const *cases, int n, hipStream_t s, CvMomBatch **out) {
+  *out = nullptr;
+  if (n < 1 || n > 64) return CV_ERR_ARG;
+  CvMomBatch *mb = new CvMomBatch();
+  mb->n = n;
+  mb->s = s;
+  mb->cs.assign(cases, cases + n);
+  std::vector<MCase> hc(n);
+  std::vector<uint32_t> tabs;
+  std::vector<int> cnt[5];
+  for (int c = 0; c < n; c++) {
+    cv_ctx *x = cases[c];
+    CvLevel &F = x->lev[x->nlev - 1];
+    CvMetrics &M = x->m;
+    MCase &m = hc[c];
+    memset(&m, 0, sizeof(m));
+    m.g = gview(x);
+    for (int q = 0; q < 3; q++) {
+      m.uc[q] = x->ucont[q]; m.uo[q] = x->ucont_o[q]; m.urm1[q] = x->ucont_rm1[q];
+      m.uca[q] = x->ucat[q]; m.rhs[q] = x->rhs[q]; m.rct[q] = x->rct[q];
+    }
+    m.p = x->p;
+    m.nv = F.nvert;
+    m.ff = ffargs(x);
+    m.mi = FM{c3(M.icsi), c3(M.ieta), c3(M.izet), M.iaj};
+    m.mj = FM{c3(M.jcsi), c3(M.jeta), c3(M.jzet), M.jaj};
+    m.mk = FM{c3(M.kcsi), c3(M.keta), c3(M.kzet), M.kaj};
+    m.csi = c3(M.csi); m.eta = c3(M.eta); m.zet = c3(M.zet);
+    m.aj = M.aj;
+    m.nu = 1. / x->ren;
+    m.dt = x->dt;
+    m.st = x->st;
+    m.wx = x->bctype[0] == 1;
+    m.wz = x->bctype[4] == 1;
+    m.n = F.d.n;
+    const dim3 mg = march_grid(F.d.mx, F.d.my, F.d.mz);
+    m.mgx = (int)mg.x; m.mgy = (int)mg.y; m.mgz = (int)mg.z;
+    m.rgx = (F.d.mx + 62) / 63; m.rgy = (F.d.my - 1 + 6) / 7; m.rgz = (F.d.mz + RC_KC - 1) / RC_KC;
+    m.nvb = (int)std::min<long>((F.d.n + BS - 1) / BS, 1024);
+    const long planes = (long)F.d.my * F.d.mz + (long)F.d.mx * F.d.mz + (long)F.d.mx * F.d.my;
+    m.npb = (int)std::min<long>((planes + BS - 1) / BS, 256);
+    cnt[0].push_back(m.mgx * m.mgy * m.mgz);
+    cnt[1].push_back(m.rgx * m.rgy * m.rgz);
+    cnt[2].push_back(m.nvb);
+    cnt[3].push_back(m.npb);
+    cnt[4].push_back(1);
+    mb->all |= 1ull << c;
+    if (F.d.xper) mb->xper |= 1ull << c;
+    if (F.d.zper) mb->zper |= 1ull << c;
+  }
+  mb->per = mb->xper | mb->zper;
+  long *off[5] = {&mb->t_march, &mb->t_tile, &mb->t_vec, &mb->t_plane, &mb->t_one};
+  int *len[5] = {&mb->n_march, &mb->n_tile, &mb->n_vec, &mb->n_plane, &mb->n_one};
+  for (int k = 0; k < 5; k++) {
+    *off[k] = (long)tabs.size();
+    for (int c = 0; c < n; c++)
+      for (int b = 0; b < cnt[k][c]; b++) tabs.push_back(((uint32_t)c << 24) | (uint32_t)b);
+    *len[k] = (int)((long)tabs.size() - *off[k]);
+  }
+  hipError_t e = hipMalloc((void **)&mb->d, sizeof(MCase) * n);
+  if (e == hipSuccess) e = hipMalloc((void **)&mb->tab, sizeof(uint32_t) * tabs.size());
+  if (e == hipSuccess) e = hipMemcpy(mb->d, hc.data(), sizeof(MCase) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(mb->tab, tabs.data(), sizeof(uint32_t) * tabs.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    cv_set_error("cv_mom_batch_create: %s", hipGetErrorString(e));
+    cv_mom_batch_destroy(mb);
+    return CV_ERR_HIP;
+  }
+  *out = mb;
+  return CV_OK;
+}
+
+#define MB_LAUNCH(kern, blk, t, nb, pm, ...)                                                                      \
+  do {                                                                                                          \
+    if ((nb) > 0 && (pm))                                                                                       \
+      hipLaunchKernelGGL(kern, dim3((unsigned)(nb)), blk, 0, mb->s, (const MCase *)mb->d,                       \
+                         (const uint32_t *)(mb->tab + (t)), (unsigned long long)(pm), ##__VA_ARGS__);            \
+  } while (0)
+
+// Contra2Cart (formbcs) and the ghost passes of cv_ghost_update for every case
+int cv_mom_batch_ghost(CvMomBatch *mb, int formbcs, int cart) {
+  if (cart)
+    MB_LAUNCH(kbm_contra2cart, dim3(64, MARCH_TY), mb->t_march, mb->n_march, mb->all);
+  const int passes = formbcs ? 3 : 1;
+  for (int ps = 0; ps < passes; ps++) {
+    MB_LAUNCH(kbm_wall_ghost, dim3(BS), mb->t_plane, mb->n_plane, mb->all, formbcs ? 0 : 1);
+    MB_LAUNCH(kbm_periodic_planes, dim3(BS), mb->t_plane, mb->n_plane, mb->per, formbcs);
+    MB_LAUNCH(kbm_edge_average, dim3(BS), mb->t_one, mb->n_one, mb->all, formbcs);
+    MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_plane, mb->n_plane, mb->xper, 0, formbcs);
+    MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_plane, mb->n_plane, mb->zper, 2, formbcs);
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+// the level rotation (rot: the cases with a step history) and the explicit
+// RK step of cv_rk_dev for every case
+int cv_mom_batch_rk(CvMomBatch *mb, unsigned long long rot) {
+  MB_LAUNCH(kbm_vec3, dim3(BS), mb->t_vec, mb->n_vec, rot, 0);
+  MB_LAUNCH(kbm_vec3, dim3(BS), mb->t_vec, mb->n_vec, mb->all, 1);
+  int rc = cv_mom_batch_ghost(mb, 1, 1);
+  if (rc) return rc;
+  MB_LAUNCH(kbm_vec3, dim3(BS), mb->t_vec, mb->n_vec, mb->all, 2);
+  const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
+  for (int s = 0; s < 4; s++) {
+    MB_LAUNCH(kbm_rc_tiled, dim3(64, 8), mb->t_tile, mb->n_tile, mb->all);
+    MB_LAUNCH(kbm_rct_periodic, dim3(BS), mb->t_plane, mb->n_plane, mb->per);
+    MB_LAUNCH(kbm_ff1_faces, dim3(64, MARCH_TY), mb->t_march, mb->n_march, mb->all);
+    MB_LAUNCH(kbm_ff1_periodic, dim3(BS), mb->t_plane, mb->n_plane, mb->per);
+    MB_LAUNCH(kbm_rk_update, dim3(BS), mb->t_vec, mb->n_vec, mb->all, alfa[s]);
+    MB_LAUNCH(kbm_wall_flux_zero, dim3(BS), mb->t_plane, mb->n_plane, mb->all);
+    if ((rc = cv_mom_batch_ghost(mb, 1, 1))) return rc;
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
 }

@@ -1300,11 +1300,11 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
 // ====================================================================
 // PoissonRHS (poisson.c:1509-1576)
 // ====================================================================
-__global__ static void k_poisson_rhs(Idx I, const double *ux, const double *uy, const double *uz, const double *aj,
-                                     const double *nv, double *b, double dt, double st, long n, double *part) {
+__device__ __forceinline__ static void poisson_rhs_body(Idx I, const double *ux, const double *uy, const double *uz, const double *aj,
+                                     const double *nv, double *b, double dt, double st, long n, double *part, int bid, int nb) {
   __shared__ double sh[BS / 64];
   double s = 0.0;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+  for (long t = (long)bid * BS + threadIdx.x; t < n; t += (long)nb * BS) {
     int i, j, k;
     I.dec(t, i, j, k);
     double r;
@@ -1316,15 +1316,19 @@ __global__ static void k_poisson_rhs(Idx I, const double *ux, const double *uy, 
     s += r / aj[t] * dt / 1.5;
   }
   s = cv_block_sum(s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part[bid] = s;
 }
+__global__ static void k_poisson_rhs(Idx I, const double *ux, const double *uy, const double *uz, const double *aj,
+                                     const double *nv, double *b, double dt, double st, long n, double *part) { poisson_rhs_body(I, ux, uy, uz, aj, nv, b, dt, st, n, part, blockIdx.x, gridDim.x); }
+
 
 int cv_poisson_rhs_dev(cv_ctx *c, double *rhs_sum) {
   CvLevel &F = c->lev[c->nlev - 1];
-  hipLaunchKernelGGL(k_poisson_rhs, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, idx_of(F), (const double *)c->ucont[0],
+  const int nb = nred(F.d.n);
+  hipLaunchKernelGGL(k_poisson_rhs, dim3(nb), dim3(BS), 0, c->stream, idx_of(F), (const double *)c->ucont[0],
                      (const double *)c->ucont[1], (const double *)c->ucont[2], (const double *)F.aj,
                      (const double *)F.nvert, c->bvec, c->dt, c->st, F.d.n, c->part);
-  final_sum(c, c->part, CV_RED_GRID, c->dsc + 104, 1);
+  final_sum(c, c->part, nb, c->dsc + 104, 1);
   if (rhs_sum) {
     CV_CHECK(hipMemcpyAsync(c->red_host + 100, c->dsc + 104, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CV_CHECK(hipStreamSynchronize(c->stream));
@@ -1342,16 +1346,16 @@ __device__ static inline double vnorm(const double *x, const double *y, const do
 }
 
 // mode 0: sum flux/area into part[2*b], mode 1: apply correction dsc[slot]
-__global__ static void k_ibm_flux(Idx I, const double *nv, double *ux, double *uy, double *uz, const double *cx,
+__device__ __forceinline__ static void ibm_flux_body(Idx I, const double *nv, double *ux, double *uy, double *uz, const double *cx,
                                   const double *cy, const double *cz, const double *ex, const double *ey, const double *ez,
                                   const double *zx, const double *zy, const double *zz, int mode, const double *corr_p,
-                                  long n, double *part) {
+                                  long n, double *part, int bid, int nb) {
   __shared__ double sh[BS / 64];
   const double ibmval = 1.5;
   const double corr = mode ? corr_p[0] : 0.0;
   double f = 0.0, a = 0.0;
   const long sj = I.mx, sk = (long)I.mx * I.my;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+  for (long t = (long)bid * BS + threadIdx.x; t < n; t += (long)nb * BS) {
     int i, j, k;
     I.dec(t, i, j, k);
     if (!(i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1)) continue;
@@ -1378,27 +1382,34 @@ __global__ static void k_ibm_flux(Idx I, const double *nv, double *ux, double *u
   }
   if (mode) return;
   double sf = cv_block_sum(f, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = sf;
+  if (threadIdx.x == 0) part[bid] = sf;
   double sa = cv_block_sum(a, sh);
-  if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = sa;
+  if (threadIdx.x == 0) part[nb + bid] = sa;
 }
+__global__ static void k_ibm_flux(Idx I, const double *nv, double *ux, double *uy, double *uz, const double *cx,
+                                  const double *cy, const double *cz, const double *ex, const double *ey, const double *ez,
+                                  const double *zx, const double *zy, const double *zz, int mode, const double *corr_p,
+                                  long n, double *part) { ibm_flux_body(I, nv, ux, uy, uz, cx, cy, cz, ex, ey, ez, zx, zy, zz, mode, corr_p, n, part, blockIdx.x, gridDim.x); }
 
-__global__ static void k_ibm_corr(const double *fa, double *corr) {
+
+__device__ __forceinline__ static void ibm_corr_body(const double *fa, double *corr) {
   corr[0] = fa[1] > 1.e-15 ? (fa[0] + 0.0) / fa[1] : 0.0;   // FluxIntpSum = 0
 }
+__global__ static void k_ibm_corr(const double *fa, double *corr) { ibm_corr_body(fa, corr); }
 
 int cv_volumeflux_rev_dev(cv_ctx *c, double *fa4) {
   CvLevel &F = c->lev[c->nlev - 1];
   CvMetrics &M = c->m;
+  const int nb = nred(F.d.n);
   for (int pass = 0; pass < 3; pass++) {
     int mode = pass == 1 ? 1 : 0;
-    hipLaunchKernelGGL(k_ibm_flux, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, idx_of(F), (const double *)F.nvert,
+    hipLaunchKernelGGL(k_ibm_flux, dim3(nb), dim3(BS), 0, c->stream, idx_of(F), (const double *)F.nvert,
                        c->ucont[0], c->ucont[1], c->ucont[2], (const double *)M.csi[0], (const double *)M.csi[1],
                        (const double *)M.csi[2], (const double *)M.eta[0], (const double *)M.eta[1],
                        (const double *)M.eta[2], (const double *)M.zet[0], (const double *)M.zet[1],
                        (const double *)M.zet[2], mode, (const double *)(c->dsc + 108), F.d.n, c->part);
     if (!mode) {
-      final_sum(c, c->part, CV_RED_GRID, c->dsc + (pass == 0 ? 112 : 114), 2);
+      final_sum(c, c->part, nb, c->dsc + (pass == 0 ? 112 : 114), 2);
       if (pass == 0) hipLaunchKernelGGL(k_ibm_corr, dim3(1), dim3(1), 0, c->stream, (const double *)(c->dsc + 112), c->dsc + 108);
     }
   }
@@ -1439,32 +1450,38 @@ int cv_poisson_solve_dev(cv_ctx *c, cv_step_stats *st) {
 // ====================================================================
 // UpdatePressure (poisson.c:2908-3248)
 // ====================================================================
-__global__ static void k_p_add(Idx I, double *p, const double *phi, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+__device__ __forceinline__ static void p_add_body(Idx I, double *p, const double *phi, long n, long t0_, long st_) {
+  for (long t = t0_; t < n; t += st_) {
     int i, j, k;
     I.dec(t, i, j, k);
     if (i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1) p[t] += phi[t];
   }
 }
+__global__ static void k_p_add(Idx I, double *p, const double *phi, long n) { p_add_body(I, p, phi, n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x); }
+
 // x planes: i=0 <- mx-2, i=mx-1 <- 1 over (j,k) in [j0,j1)x[k0,k1)
-__global__ static void k_pplane_x(Idx I, double *a, double *b, int j0, int j1, int k0, int k1) {
+__device__ __forceinline__ static void pplane_x_body(Idx I, double *a, double *b, int j0, int j1, int k0, int k1, long t0_, long st_) {
   long nj = j1 - j0, nk = k1 - k0;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nj * nk; t += (long)gridDim.x * blockDim.x) {
+  for (long t = t0_; t < nj * nk; t += st_) {
     int j = j0 + (int)(t % nj), k = k0 + (int)(t / nj);
     long p0 = I.at(0, j, k), pm = I.at(I.mx - 2, j, k), q0 = I.at(I.mx - 1, j, k), q1 = I.at(1, j, k);
     a[p0] = a[pm]; a[q0] = a[q1];
     if (b) { b[p0] = b[pm]; b[q0] = b[q1]; }
   }
 }
-__global__ static void k_pplane_z(Idx I, double *a, double *b, int i0, int i1, int j0, int j1) {
+__global__ static void k_pplane_x(Idx I, double *a, double *b, int j0, int j1, int k0, int k1) { pplane_x_body(I, a, b, j0, j1, k0, k1, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x); }
+
+__device__ __forceinline__ static void pplane_z_body(Idx I, double *a, double *b, int i0, int i1, int j0, int j1, long t0_, long st_) {
   long ni = i1 - i0, nj = j1 - j0;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < ni * nj; t += (long)gridDim.x * blockDim.x) {
+  for (long t = t0_; t < ni * nj; t += st_) {
     int i = i0 + (int)(t % ni), j = j0 + (int)(t / ni);
     long p0 = I.at(i, j, 0), pm = I.at(i, j, I.mz - 2), q0 = I.at(i, j, I.mz - 1), q1 = I.at(i, j, 1);
     a[p0] = a[pm]; a[q0] = a[q1];
     if (b) { b[p0] = b[pm]; b[q0] = b[q1]; }
   }
 }
+__global__ static void k_pplane_z(Idx I, double *a, double *b, int i0, int i1, int j0, int j1) { pplane_z_body(I, a, b, i0, i1, j0, j1, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x); }
+
 
 int cv_update_pressure_dev(cv_ctx *c) {
   CvLevel &F = c->lev[c->nlev - 1];
@@ -1526,11 +1543,11 @@ __device__ static inline void project_face(Idx I, int d, int i, int j, int k, co
   u[t] -= (dp[0] * g0 + dp[1] * g1 + dp[2] * g2) * dt * st / 1.5;
 }
 
-__global__ static void k_projection(Idx I, int xper, int zper, const double *phi, const double *nv, const uint32_t *fl,
+__device__ __forceinline__ static void projection_body(Idx I, int xper, int zper, const double *phi, const double *nv, const uint32_t *fl,
                                     FaceMet Mi, FaceMet Mj, FaceMet Mk, double *ux, double *uy, double *uz, double dt,
-                                    double st, long n) {
+                                    double st, long n, long t0_, long st_) {
   const int i_end = xper ? I.mx - 1 : I.mx - 2, j_end = I.my - 2, k_end = zper ? I.mz - 1 : I.mz - 2;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+  for (long t = t0_; t < n; t += st_) {
     int i, j, k;
     I.dec(t, i, j, k);
     bool jk = j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1, ij = i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1;
@@ -1543,6 +1560,10 @@ __global__ static void k_projection(Idx I, int xper, int zper, const double *phi
     if (zper && k == 0 && ij) project_face(I, 2, i, j, 0, phi, nv, fl, Mk, uz, dt, st);
   }
 }
+__global__ static void k_projection(Idx I, int xper, int zper, const double *phi, const double *nv, const uint32_t *fl,
+                                    FaceMet Mi, FaceMet Mj, FaceMet Mk, double *ux, double *uy, double *uz, double dt,
+                                    double st, long n) { projection_body(I, xper, zper, phi, nv, fl, Mi, Mj, Mk, ux, uy, uz, dt, st, n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x); }
+
 
 int cv_projection_dev(cv_ctx *c) {
   CvLevel &F = c->lev[c->nlev - 1];

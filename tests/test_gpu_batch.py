@@ -3,7 +3,8 @@ different multigrid depths and y-line lengths -- ends each fractional step
 with the same bits in every field, and the same FGMRES iteration count, as
 the same case stepped alone by cv_flow_step (Flow_Solver, solvers.c:944-1156),
 in mixed and FP64 precision, with the captured hipGraph and with direct
-launches.  Tolerance: exact equality."""
+launches, with the momentum batched and per case.  Tolerance: exact
+equality."""
 import numpy as np
 import pytest
 
@@ -32,9 +33,13 @@ def _ctx(case, precision):
     return u
 
 
-@pytest.mark.parametrize("precision,graph", [("mixed", "1"), ("fp64", "1"), ("mixed", "0")])
-def test_batch_matches_each_case_alone(monkeypatch, precision, graph):
+# mom "1": the momentum and the rest of the step batched on the batch stream;
+# "0": per case on each case's stream (graph "1": replayed from per-case graphs)
+@pytest.mark.parametrize("precision,graph,mom", [("mixed", "1", "1"), ("fp64", "1", "1"), ("mixed", "1", "0"),
+                                                 ("mixed", "0", "0")])
+def test_batch_matches_each_case_alone(monkeypatch, precision, graph, mom):
     monkeypatch.setenv("CV_BATCH_GRAPH", graph)
+    monkeypatch.setenv("CV_BATCH_MOM", mom)
     cl = _cases()
     levels = {serving.levels_for(c.cells) for c in cl}
     assert len(levels) > 1, "the batch must mix multigrid depths"
@@ -42,7 +47,9 @@ def test_batch_matches_each_case_alone(monkeypatch, precision, graph):
     alone = [_ctx(c, precision) for c in cl]
     b = Batch(inb)
     try:
-        for s in range(2):
+        # three steps: the per-case graphs of the work around the solve are
+        # captured on the first eligible step and replayed after it
+        for s in range(3):
             sb = b.step()
             for q, u in enumerate(alone):
                 sa = u.Flow_Solver()
