@@ -187,6 +187,20 @@ def serving_workloads(n_serve, n_ens, seed, log_workers):
     return list(zip(serve, arrays[:n_serve])), arrivals, list(zip(ens, arrays[n_serve:]))
 
 
+def batch_ensemble_only(args):
+    """Only the batched-ensemble leg (the same 64 seeded cases as the full
+    run): one untimed step, then --ensemble-steps timed steps; one JSON line
+    with its latency and throughput (for profiling that leg alone)."""
+    from curvib_old_amd import serving
+    rng = np.random.default_rng(2026)
+    [serving.sample_case(rng, lo=(32, 32, 32), hi=(192, 192, 192)) for _ in range(args.serving_requests)]
+    serving.poisson_arrivals(args.serving_requests, 50.0, rng)
+    ens = [serving.sample_case(rng, lo=(32, 32, 32), hi=(128, 96, 96)) for _ in range(args.ensemble_cases)]
+    arrays = serving.build_inputs(ens, workers=args.build_workers)
+    r = serving.run_batched(list(zip(ens, arrays)), args.ensemble_steps, args.batch_ensemble_only, log=log, **SOLVER)
+    print(json.dumps({"workload": f"{len(ens)}-case ensemble, one cv_batch", **r}), flush=True)
+
+
 def run_serving(serve_reqs, arrivals, ens_reqs, nsteps, ens_steps):
     from curvib_old_amd import serving
     opts = {k: v for k, v in SOLVER.items()}
@@ -247,6 +261,8 @@ def main():
     ap.add_argument("--omega", type=float, default=SOLVER["omega"])
     ap.add_argument("--smoother", default=SOLVER["smoother"], choices=("jacobi", "rb", "yline"))
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batch-ensemble-only", choices=("mixed", "fp64"), default=None,
+                    help="run only the batched-ensemble leg in this precision")
     ap.add_argument("--serving-requests", type=int, default=50)
     ap.add_argument("--serving-steps", type=int, default=20)
     ap.add_argument("--ensemble-cases", type=int, default=64)
@@ -255,6 +271,9 @@ def main():
     args = ap.parse_args()
     SOLVER.update(restart=args.restart, pre_sweeps=args.pre, post_sweeps=args.post, coarse_sweeps=args.coarse,
                   omega=args.omega, smoother=args.smoother)
+    if args.batch_ensemble_only:
+        batch_ensemble_only(args)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not supported "

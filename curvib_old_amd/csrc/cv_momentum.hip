@@ -740,7 +740,6 @@ __device__ inline void main_end_state(const FFArgs &A, double *dc, double *de, d
     if (pc >= 0) zface_state(A, pc, &pe, &pz);
   }
   double xdc = p[t + 1] - p[t];
-  double xdz = mxz == XM_DROP ? pz : ptan(g, p, 0, 2, t, mxz);
   int myc = face_mode(A.fl, t, 1, 0), myz = face_mode(A.fl, t, 1, 2);
   double ydc = myc == XM_DROP ? xdc : ptan(g, p, 1, 0, t, myc);
   double yde = p[t + g.sj] - p[t];
