@@ -325,6 +325,8 @@ struct cv_batch {
   double *dbsc = nullptr, *hbsc = nullptr, *dgat = nullptr, *hgat = nullptr;
   uint32_t *dtab = nullptr;
   BTab st[CV_MAXLEV], vc[CV_MAXLEV], rd[CV_MAXLEV], ln[CV_MAXLEV][3];
+  // the same tables restricted to the cases that go deeper [0] or bottom out [1] at the depth
+  BTab st_v[CV_MAXLEV][2], vc_v[CV_MAXLEV][2], rd_v[CV_MAXLEV][2], ln_v[CV_MAXLEV][2][3];
   int lnc[CV_MAXLEV][3];
   unsigned long long all = 0, has[CV_MAXLEV], inner[CV_MAXLEV], coarse[CV_MAXLEV], lgm[CV_MAXLEV][3];
   hipGraphExec_t gexec = nullptr;
@@ -356,11 +358,23 @@ static int b_gather(cv_batch *B) {
 }
 
 #define BTAB(t) B->dtab + (t).off
+
+// which block table a launch at depth d with case mask pm needs: -1 every
+// case of the depth, 0 only cases that go deeper, 1 only cases whose
+// coarsest level is d (fewer idle blocks in the coarse sweeps)
+static int b_variant(const cv_batch *B, int d, unsigned long long pm) {
+  const unsigned long long m = pm & B->has[d];
+  if (m == B->has[d]) return -1;
+  if ((m & ~B->inner[d]) == 0) return 0;
+  if ((m & ~B->coarse[d]) == 0) return 1;
+  return -1;
+}
+#define BPICK(full, var, d, pm) (b_variant(B, d, pm) < 0 ? (full)[d] : (var)[d][b_variant(B, d, pm)])
 #define BLAUNCH_OK(t, pm) ((t).len > 0 && (pm) != 0)
 
 template <class T, int MODE>
 static void b_stencil(cv_batch *B, int d, unsigned long long pm, int sx, int sb, int sy) {
-  const BTab &t = B->st[d];
+  const BTab &t = BPICK(B->st, B->st_v, d, pm);
   if (!BLAUNCH_OK(t, pm)) return;
   hipLaunchKernelGGL((kb_stencil<T, MODE>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream, (const BCase *)B->dcases,
                      (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, d, sx, sb, sy, B->kchunk);
@@ -368,8 +382,9 @@ static void b_stencil(cv_batch *B, int d, unsigned long long pm, int sx, int sb,
 
 template <class T>
 static void b_line(cv_batch *B, int d, unsigned long long pm, int sr, T om, bool zg) {
+  const int var = b_variant(B, d, pm);
   for (int g = 0; g < 3; g++) {
-    const BTab &t = B->ln[d][g];
+    const BTab &t = var < 0 ? B->ln[d][g] : B->ln_v[d][var][g];
     const unsigned long long m = pm & B->lgm[d][g];
     if (!BLAUNCH_OK(t, m)) continue;
     const dim3 blk(64, (unsigned)B->lnc[d][g]);
@@ -388,7 +403,7 @@ static void b_line(cv_batch *B, int d, unsigned long long pm, int sr, T om, bool
 
 template <class T>
 static void b_zero(cv_batch *B, int d, unsigned long long pm, int sx) {
-  const BTab &t = B->vc[d];
+  const BTab &t = BPICK(B->vc, B->vc_v, d, pm);
   if (!BLAUNCH_OK(t, pm)) return;
   hipLaunchKernelGGL(kb_zero<T>, dim3((unsigned)t.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
                      (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, d, sx);
@@ -401,12 +416,13 @@ static void b_final(cv_batch *B, int d, unsigned long long pm, int slot, int nq)
 
 template <class T>
 static void b_nullspace(cv_batch *B, int d, unsigned long long pm, int sx) {
-  if (!BLAUNCH_OK(B->rd[d], pm)) return;
-  hipLaunchKernelGGL(kb_fluid_sum<T>, dim3((unsigned)B->rd[d].len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
-                     (const uint32_t *)BTAB(B->rd[d]), pm, (const BCtl *)B->dctl, d, sx);
+  const BTab &tr = BPICK(B->rd, B->rd_v, d, pm), &tv = BPICK(B->vc, B->vc_v, d, pm);
+  if (!BLAUNCH_OK(tr, pm)) return;
+  hipLaunchKernelGGL(kb_fluid_sum<T>, dim3((unsigned)tr.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
+                     (const uint32_t *)BTAB(tr), pm, (const BCtl *)B->dctl, d, sx);
   b_final(B, d, pm, 100, 1);
-  hipLaunchKernelGGL(kb_fluid_shift<T>, dim3((unsigned)B->vc[d].len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
-                     (const uint32_t *)BTAB(B->vc[d]), pm, (const BCtl *)B->dctl, d, sx);
+  hipLaunchKernelGGL(kb_fluid_shift<T>, dim3((unsigned)tv.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
+                     (const uint32_t *)BTAB(tv), pm, (const BCtl *)B->dctl, d, sx);
 }
 
 // the V-cycle of depth d for every case that has it; the cases whose
@@ -431,8 +447,11 @@ static void b_vcycle(cv_batch *B, int d) {
   hipLaunchKernelGGL(kb_restrict<T>, dim3((unsigned)B->vc[d + 1].len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
                      (const uint32_t *)BTAB(B->vc[d + 1]), in, (const BCtl *)B->dctl, d);
   b_vcycle<T>(B, d + 1);
-  hipLaunchKernelGGL(kb_prolong<T>, dim3((unsigned)B->vc[d].len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
-                     (const uint32_t *)BTAB(B->vc[d]), in, (const BCtl *)B->dctl, d);
+  {
+    const BTab &tp = BPICK(B->vc, B->vc_v, d, in);
+    hipLaunchKernelGGL(kb_prolong<T>, dim3((unsigned)tp.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
+                       (const uint32_t *)BTAB(tp), in, (const BCtl *)B->dctl, d);
+  }
   for (int s = 0; s < npost; s++) {
     b_stencil<T, 1>(B, d, in, BX, BB, BR);
     b_line<T>(B, d, in, BR, om, false);
@@ -787,20 +806,24 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
     B->all |= bit(c);
   }
   for (int d = 0; d < B->maxlev; d++) {
-    std::vector<std::pair<int, int>> st, vc, rd, ln[3];
-    for (int c = 0; c < ncases; c++) {
-      if (!(B->has[d] & bit(c))) continue;
-      const BLev &b = hc[c].lev[d];
-      st.push_back({c, b.sgx * b.sgy * b.sgz});
-      vc.push_back({c, b.nvb});
-      rd.push_back({c, b.nrd});
-      const int g = b.cs == 8 ? 0 : b.cs == 16 ? 1 : 2;
-      ln[g].push_back({c, b.lgx * b.I.mz});
+    // v = -1: every case of the depth; 0: the cases that go deeper; 1: those that bottom out here
+    for (int v = -1; v < 2; v++) {
+      std::vector<std::pair<int, int>> st, vc, rd, ln[3];
+      const unsigned long long m = v < 0 ? B->has[d] : v == 0 ? B->inner[d] : B->coarse[d];
+      for (int c = 0; c < ncases; c++) {
+        if (!(m & bit(c))) continue;
+        const BLev &b = hc[c].lev[d];
+        st.push_back({c, b.sgx * b.sgy * b.sgz});
+        vc.push_back({c, b.nvb});
+        rd.push_back({c, b.nrd});
+        const int g = b.cs == 8 ? 0 : b.cs == 16 ? 1 : 2;
+        ln[g].push_back({c, b.lgx * b.I.mz});
+      }
+      add_tab(v < 0 ? B->st[d] : B->st_v[d][v], st);
+      add_tab(v < 0 ? B->vc[d] : B->vc_v[d][v], vc);
+      add_tab(v < 0 ? B->rd[d] : B->rd_v[d][v], rd);
+      for (int g = 0; g < 3; g++) add_tab(v < 0 ? B->ln[d][g] : B->ln_v[d][v][g], ln[g]);
     }
-    add_tab(B->st[d], st);
-    add_tab(B->vc[d], vc);
-    add_tab(B->rd[d], rd);
-    for (int g = 0; g < 3; g++) add_tab(B->ln[d][g], ln[g]);
   }
   for (const uint32_t e : tabs)
     if ((e & 0xffffffu) >= (1u << 24)) { cv_set_error("cv_batch_create: case too large"); b_release(B); return CV_ERR_ARG; }
