@@ -33,6 +33,7 @@ struct BLev {
   long n;
   double inv_nf;
   int sgx, sgy, sgz;   // stencil tiles in i, j, k
+  int kc;              // k planes per stencil tile
   int lgx, cs;         // line-solve blocks in i; rows per thread of the line solve
   int nvb, nrd;        // blocks of an elementwise pass and of a reduction
 };
@@ -104,13 +105,13 @@ __device__ inline bool bblock(const uint32_t *tab, unsigned long long pm, const 
   const BCase &C = cs[c];
 
 template <class T, int MODE>
-__global__ void __launch_bounds__(256) kb_stencil(BK_ARGS, int d, int sx, int sb, int sy, int kchunk) {
+__global__ void __launch_bounds__(256) kb_stencil(BK_ARGS, int d, int sx, int sb, int sy) {
   BK_HEAD
   const OpV<T> A = bop<T>(C, d);
   const int gx = C.lev[d].sgx, gy = C.lev[d].sgy, jj = ctl->jj;
   const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
   stencil_tile<T, MODE, 4, true>(A, bvec<T>(C, d, sx, jj), MODE >= 1 ? bvec<T>(C, d, sb, jj) : (const T *)nullptr,
-                                 bvec<T>(C, d, sy, jj), A.dinv, (T)0, kchunk, bx, by, bz);
+                                 bvec<T>(C, d, sy, jj), A.dinv, (T)0, C.lev[d].kc, bx, by, bz);
 }
 
 // x += omega T^-1 r in place (zg: x = omega T^-1 r)
@@ -318,7 +319,6 @@ struct cv_batch {
   int n = 0;
   std::vector<cv_ctx *> cs;
   cv_solver_opts opt{};
-  int kchunk = 16;
   int maxlev = 0;
   BCase *dcases = nullptr;
   BCtl *dctl = nullptr, *hctl = nullptr;
@@ -377,7 +377,7 @@ static void b_stencil(cv_batch *B, int d, unsigned long long pm, int sx, int sb,
   const BTab &t = BPICK(B->st, B->st_v, d, pm);
   if (!BLAUNCH_OK(t, pm)) return;
   hipLaunchKernelGGL((kb_stencil<T, MODE>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream, (const BCase *)B->dcases,
-                     (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, d, sx, sb, sy, B->kchunk);
+                     (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, d, sx, sb, sy);
 }
 
 template <class T>
@@ -730,7 +730,6 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
   B->n = ncases;
   B->cs.assign(cases, cases + ncases);
   B->opt = o;
-  B->kchunk = stencil_kchunk();
   const char *ge = getenv("CV_BATCH_GRAPH");
   B->use_graph = ge ? atoi(ge) != 0 : 1;
   const char *gm = getenv("CV_BATCH_MOM");
@@ -770,7 +769,8 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
       b.isc = L.d.isc; b.jsc = L.d.jsc; b.ksc = L.d.ksc;
       b.n = L.d.n;
       b.inv_nf = x->nfluid[l] > 0 ? 1.0 / x->nfluid[l] : 0.0;
-      b.sgx = (L.d.mx + 62) / 63; b.sgy = (L.d.my + 3) / 4; b.sgz = (L.d.mz + B->kchunk - 1) / B->kchunk;
+      b.kc = stencil_kchunk(L.d.mz);
+      b.sgx = (L.d.mx + 62) / 63; b.sgy = (L.d.my + 3) / 4; b.sgz = (L.d.mz + b.kc - 1) / b.kc;
       b.lgx = (L.d.mx + 63) / 64;
       const int rows = L.d.my - 2;
       const int rmax = o.precision == CV_MIXED ? 512 : 256;

@@ -889,14 +889,15 @@ static int nred(long n) {
 }
 
 // k extent of one stencil block (tunable through CV_STENCIL_KCHUNK)
-static int stencil_kchunk() {
-  static int kc = 0;
-  if (!kc) {
+static int stencil_kchunk(int mz) {
+  static int kc = -1;
+  if (kc < 0) {
     const char *e = getenv("CV_STENCIL_KCHUNK");
-    kc = e ? atoi(e) : 16;
-    if (kc < 1) kc = 1;
+    kc = e ? std::max(1, atoi(e)) : 0;
   }
-  return kc;
+  // default: 12 planes on deep levels, 8 on shallower ones (more blocks);
+  // the fastest measured at 512x256x256
+  return kc ? kc : (mz >= 192 ? 12 : 8);
 }
 
 // rows per stencil block (4 or 8; CV_STENCIL_ROWS64 / CV_STENCIL_ROWS32) and
@@ -937,7 +938,7 @@ template <class T, int MODE>
 static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T)0) {
   CvLevel &L = c->lev[l];
   OpV<T> A = opview<T>(c, l);
-  const int kc = stencil_kchunk();
+  const int kc = stencil_kchunk(L.d.mz);
   const int ty = stencil_rows<T>();
   dim3 g((unsigned)((L.d.mx + 62) / 63), (unsigned)((L.d.my + ty - 1) / ty), (unsigned)((L.d.mz + kc - 1) / kc));
   const int remap = stencil_remap(g.x);
