@@ -457,12 +457,12 @@ __device__ inline void rct_out(const G &g, long t, const double *d0, const doubl
 #define RC_KC 16
 template <int TY>
 __global__ void __launch_bounds__(64 * TY) k_rc_fused(G g, C3 uc, C3 u, const double *nv, FM mi, FM mj, FM mk, double nu,
-                                                    C3 csi, C3 eta, C3 zet, const double *aj, V3 rct) {
+                                                    C3 csi, C3 eta, C3 zet, const double *aj, V3 rct, int kc) {
   __shared__ double S[3][TY][64];
   const int L = threadIdx.x, R = threadIdx.y;
   const int mx = g.mx, my = g.my, mz = g.mz;
   const int i = 63 * blockIdx.x - 1 + L, j = (TY - 1) * blockIdx.y - 1 + R;
-  const int k0 = RC_KC * blockIdx.z, k1 = min(k0 + RC_KC, mz);
+  const int k0 = kc * blockIdx.z, k1 = min(k0 + kc, mz);
   const bool in_i = i >= 1 && i <= mx - 2, in_j = j >= 1 && j <= my - 2;
   const bool out = L >= 1 && R >= 1 && in_i && in_j;
   const bool do0 = R >= 1 && in_j && i >= 0 && i <= mx - 2;   // i-face (i,j,k)
@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(64 * TY) k_rc_fused(G g, C3 uc, C3 u, const do
 template <int TY>
 __device__ __forceinline__ void rc_tiled_body(const G &g, C3 uc, C3 u, const double *nv, const FM &mi, const FM &mj,
                                               const FM &mk, double nu, C3 csi, C3 eta, C3 zet, const double *aj, V3 rct,
-                                              int bx, int by, int bz) {
+                                              int kc, int bx, int by, int bz) {
   constexpr int TW = 67, TH = TY + 3, NP = TW * TH, NT = 64 * TY;
   __shared__ double SU[3][3][NP];
   __shared__ unsigned char SN[3][NP];
@@ -521,7 +521,7 @@ __device__ __forceinline__ void rc_tiled_body(const G &g, C3 uc, C3 u, const dou
   const int mx = g.mx, my = g.my, mz = g.mz;
   const int i0 = 63 * bx, j0 = (TY - 1) * by;
   const int i = i0 - 1 + L, j = j0 - 1 + R;
-  const int k0 = RC_KC * bz, k1 = min(k0 + RC_KC, mz);
+  const int k0 = kc * bz, k1 = min(k0 + kc, mz);
   const bool in_i = i >= 1 && i <= mx - 2, in_j = j >= 1 && j <= my - 2;
   const bool out = L >= 1 && R >= 1 && in_i && in_j;
   const bool do0 = R >= 1 && in_j && i >= 0 && i <= mx - 2;
@@ -599,8 +599,8 @@ __device__ __forceinline__ void rc_tiled_body(const G &g, C3 uc, C3 u, const dou
 
 template <int TY>
 __global__ void __launch_bounds__(64 * TY) k_rc_tiled(G g, C3 uc, C3 u, const double *nv, FM mi, FM mj, FM mk, double nu,
-                                                    C3 csi, C3 eta, C3 zet, const double *aj, V3 rct) {
-  rc_tiled_body<TY>(g, uc, u, nv, mi, mj, mk, nu, csi, eta, zet, aj, rct, blockIdx.x, blockIdx.y, blockIdx.z);
+                                                    C3 csi, C3 eta, C3 zet, const double *aj, V3 rct, int kc) {
+  rc_tiled_body<TY>(g, uc, u, nv, mi, mj, mk, nu, csi, eta, zet, aj, rct, kc, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // periodic copies of rct planes (DMLocalToLocal + rhs.c:2725-2781)
@@ -988,6 +988,16 @@ static int rc_rows() {
   return ty;
 }
 
+// k planes per face-flux block (CV_RC_KC)
+static int rc_kchunk() {
+  static int kc = 0;
+  if (!kc) {
+    const char *e = getenv("CV_RC_KC");
+    kc = e ? std::max(1, atoi(e)) : RC_KC;
+  }
+  return kc;
+}
+
 // face-flux pass variant (CV_RC_KERNEL: 1 LDS-tiled, 0 global gathers),
 // read at every call so one process can run both
 static int rc_kernel() {
@@ -1021,11 +1031,12 @@ int cv_ff1_dev(cv_ctx *c) {
     // ucont, ucat, nvert, rct out
     CvProfScope ps(c, CV_PROF_RCT, c->nlev - 1, 0, (double)F.d.n * 8.0 * (30 + 10 + 3 + 3 + 1 + 3));
     const int ty = rc_rows(), tiled = rc_kernel();
+    const int kc = rc_kchunk();
     const dim3 grid((unsigned)((F.d.mx + 62) / 63), (unsigned)((F.d.my - 1 + ty - 2) / (ty - 1)),
-                    (unsigned)((F.d.mz + RC_KC - 1) / RC_KC));
+                    (unsigned)((F.d.mz + kc - 1) / kc));
     auto kern = tiled ? (ty == 8 ? k_rc_tiled<8> : k_rc_tiled<4>) : (ty == 8 ? k_rc_fused<8> : k_rc_fused<4>);
     hipLaunchKernelGGL(kern, grid, dim3(64, ty), 0, c->stream, A.g, uc, u, (const double *)F.nvert, mi, mj, mk, nu,
-                       c3(M.csi), c3(M.eta), c3(M.zet), (const double *)M.aj, v3(c->rct));
+                       c3(M.csi), c3(M.eta), c3(M.zet), (const double *)M.aj, v3(c->rct), kc);
   }
   if (F.d.xper || F.d.zper)
     hipLaunchKernelGGL(k_rct_periodic, dim3(nblk((long)A.g.my * A.g.mz + (long)A.g.mx * A.g.my)), dim3(BS), 0, c->stream,
@@ -1272,7 +1283,7 @@ struct MCase {
   int wx, wz;               // bctype[0] == 1, bctype[4] == 1 (wall-normal flux reset)
   long n;
   int mgx, mgy, mgz;        // k-marching blocks
-  int rgx, rgy, rgz;        // face-flux tiles (8 rows)
+  int rgx, rgy, rgz, rkc;   // face-flux tiles (8 rows) and their k planes
   int nvb, npb;             // blocks of an elementwise and of a plane pass
 };
 
@@ -1331,7 +1342,7 @@ __global__ void __launch_bounds__(64 * 8) kbm_rc_tiled(MB_ARGS) {
   MB_HEAD
   const int bx = lb % C.rgx, by = (lb / C.rgx) % C.rgy, bz = lb / (C.rgx * C.rgy);
   rc_tiled_body<8>(C.g, mc3(C.uc), mc3(C.uca), C.nv, C.mi, C.mj, C.mk, C.nu, C.csi, C.eta, C.zet, C.aj, mv3(C.rct),
-                   bx, by, bz);
+                   C.rkc, bx, by, bz);
 }
 
 __global__ void kbm_rct_periodic(MB_ARGS) {
@@ -1418,7 +1429,8 @@ int cv_mom_batch_create(cv_ctx *const *cases, int n, hipStream_t s, CvMomBatch *
     m.n = F.d.n;
     const dim3 mg = march_grid(F.d.mx, F.d.my, F.d.mz);
     m.mgx = (int)mg.x; m.mgy = (int)mg.y; m.mgz = (int)mg.z;
-    m.rgx = (F.d.mx + 62) / 63; m.rgy = (F.d.my - 1 + 6) / 7; m.rgz = (F.d.mz + RC_KC - 1) / RC_KC;
+    m.rkc = rc_kchunk();
+    m.rgx = (F.d.mx + 62) / 63; m.rgy = (F.d.my - 1 + 6) / 7; m.rgz = (F.d.mz + m.rkc - 1) / m.rkc;
     m.nvb = (int)std::min<long>((F.d.n + BS - 1) / BS, 1024);
     const long planes = (long)F.d.my * F.d.mz + (long)F.d.mx * F.d.mz + (long)F.d.mx * F.d.my;
     m.npb = (int)std::min<long>((planes + BS - 1) / BS, 256);
