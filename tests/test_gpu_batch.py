@@ -88,3 +88,29 @@ def test_batch_rejects_mixed_options():
     finally:
         a.close()
         b.close()
+
+
+def test_batch_with_imprk_matches_alone():
+    """Cases on ImpRK (implicitsolver.c:7094-7256; pseudo iterations decided
+    on the host) take the per-case path around the batched solve and still
+    end each step with the bits they get alone."""
+    cl = _cases()[:3]
+    imp = dict(imp_max_it=30, imp_atol=1e-7, imp_rtol=1e-4, cfl=0.3)
+    inb = [_ctx(c, "mixed") for c in cl]
+    alone = [_ctx(c, "mixed") for c in cl]
+    for u in inb + alone:
+        u.set_momentum("imprk", **imp)
+    b = Batch(inb)
+    try:
+        for s in range(2):
+            sb = b.step()
+            for q, u in enumerate(alone):
+                sa = u.Flow_Solver()
+                assert sb[q]["ksp_its"] == sa["ksp_its"] and sb[q]["imp_its"] == sa["imp_its"], (s, q)
+            for q, (ua, ub) in enumerate(zip(inb, alone)):
+                for f in ("p", "ucont", "ucat"):
+                    assert np.array_equal(ua.get(f), ub.get(f)), (s, q, f)
+    finally:
+        b.close()
+        for u in inb + alone:
+            u.close()
