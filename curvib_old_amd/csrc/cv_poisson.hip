@@ -268,7 +268,7 @@ struct StCell {
 template <class T, int MODE, int TY, bool PFB>
 __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restrict__ x, const T *__restrict__ b,
                                     T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk, int bx,
-                                    int by, int bz) {
+                                    int by, int bz, int color = -1) {
   constexpr int TR = TY + 2, NT = TR * ST_TC, NTH = 64 * TY;
   static_assert(2 * NTH >= NT, "two tile entries per thread must cover a plane");
   __shared__ T X[4][NT];
@@ -388,11 +388,15 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
     if (R > 0) fj_lo = FJ[gk & 1][R - 1][L];
     if (out) {
       T res;
+      // MODE 2 with a colour: cells of the other (i+j+k) parity keep x
+      const bool keep = MODE == 2 && color >= 0 && ((gi + gj + gk) & 1) != color;
       if (!(w & 1u)) {
         const T xv = x[t];
         if (MODE == 0) res = xv;
         else if (MODE == 1) res = bc - xv;
-        else res = xv + omega * dvc * (bc - xv);
+        else res = keep ? xv : xv + omega * dvc * (bc - xv);
+      } else if (keep) {
+        res = xc;
       } else {
         T acc = 0;
         acc += fi; acc -= fi_lo;
@@ -414,11 +418,11 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
 template <class T, int MODE, int TY, bool PFB>
 __global__ void __launch_bounds__(64 * TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
                                                    T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk,
-                                                   int remap) {
+                                                   int remap, int color) {
   int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
   if (remap == 1) cv_xcd_tile(bx, by, bz);
   else if (remap > 1) cv_xcd_runs(remap, bx, by, bz);
-  stencil_tile<T, MODE, TY, PFB>(A, x, b, y, dinv, omega, kchunk, bx, by, bz);
+  stencil_tile<T, MODE, TY, PFB>(A, x, b, y, dinv, omega, kchunk, bx, by, bz, color);
 }
 
 // ------------------------------------------------------------------
@@ -934,8 +938,10 @@ static double stencil_bytes_pt(int mode) {
   return (double)(mode == 0 ? 12 : mode == 1 ? 13 : 14) * sizeof(T) + 4.0;
 }
 
+// MODE 0: y = A x; 1: y = b - A x; 2: y = x + omega D^-1 (b - A x), on the
+// cells of one (i+j+k) parity when color is 0 or 1 (the others keep x)
 template <class T, int MODE>
-static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T)0) {
+static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T)0, int color = -1) {
   CvLevel &L = c->lev[l];
   OpV<T> A = opview<T>(c, l);
   const int kc = stencil_kchunk(L.d.mz);
@@ -946,13 +952,17 @@ static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T
                  (double)L.d.n * stencil_bytes_pt<T>(MODE));
   const bool pfb = sizeof(T) == 4 || stencil_pfb64();
   if (ty == 4 && pfb)
-    hipLaunchKernelGGL((k_stencil<T, MODE, 4, true>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap);
+    hipLaunchKernelGGL((k_stencil<T, MODE, 4, true>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
+                       color);
   else if (ty == 4)
-    hipLaunchKernelGGL((k_stencil<T, MODE, 4, false>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap);
+    hipLaunchKernelGGL((k_stencil<T, MODE, 4, false>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
+                       color);
   else if (pfb)
-    hipLaunchKernelGGL((k_stencil<T, MODE, 8, true>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap);
+    hipLaunchKernelGGL((k_stencil<T, MODE, 8, true>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
+                       color);
   else
-    hipLaunchKernelGGL((k_stencil<T, MODE, 8, false>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap);
+    hipLaunchKernelGGL((k_stencil<T, MODE, 8, false>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
+                       color);
 }
 
 // sum of part[0..nb) -> dsc[slot] (device)
@@ -1091,16 +1101,24 @@ static void line_solve(cv_ctx *c, int l, const T *r, const T *xi, T *xo, T *tmp,
   }
 }
 
-// nsweep Jacobi sweeps on level l; x in v.x[l] (zero_guess: x is 0), r used as ping-pong
+// nsweep smoothing sweeps on level l; x in v.x[l] (zero_guess: x is 0), r
+// used as ping-pong.  Red-black: each sweep is two damped half-sweeps, the
+// cells of even (i+j+k) then the odd ones from the updated even ones; out of
+// place, because the 19-point cross terms couple cells of one colour
 template <class T>
 static void smooth(cv_ctx *c, MgVecs<T> &v, int l, int nsweep, bool zero_guess) {
   CvLevel &L = c->lev[l];
   const long n = L.d.n;
   OpV<T> A = opview<T>(c, l);
   T om = (T)c->opt.omega;
-  if (zero_guess && nsweep == 0)
+  if (zero_guess && (nsweep == 0 || c->opt.smoother == CV_SMOOTH_RB))
     hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, v.x[l], n);
   for (int s = 0; s < nsweep; s++) {
+    if (c->opt.smoother == CV_SMOOTH_RB) {
+      stencil<T, 2>(c, l, (const T *)v.x[l], (const T *)v.b[l], v.r[l], om, 0);
+      stencil<T, 2>(c, l, (const T *)v.r[l], (const T *)v.b[l], v.x[l], om, 1);
+      continue;
+    }
     if (c->opt.smoother == CV_SMOOTH_YLINE) {
       // residual into tmp, then the chunked line solve reads it and keeps
       // its forward sweep in the same array
@@ -1648,7 +1666,7 @@ extern "C" int cv_set_solver(cv_ctx *c, const cv_solver_opts *o) {
   if (!c || !o) return CV_ERR_ARG;
   if (o->restart < 1 || o->restart > MAXV - 2 || o->max_it < 1 || o->rtol < 0 || o->pre_sweeps < 0 ||
       o->post_sweeps < 0 || o->coarse_sweeps < 1 || (o->precision != CV_FP64 && o->precision != CV_MIXED) ||
-      o->omega <= 0) {
+      o->omega <= 0 || (o->smoother != CV_SMOOTH_JACOBI && o->smoother != CV_SMOOTH_RB && o->smoother != CV_SMOOTH_YLINE)) {
     cv_set_error("cv_set_solver: invalid options");
     return CV_ERR_ARG;
   }

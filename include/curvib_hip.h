@@ -37,7 +37,7 @@ extern "C" {
 
 /* multigrid smoothers */
 #define CV_SMOOTH_JACOBI 0    /* damped point Jacobi                     */
-#define CV_SMOOTH_RB     1    /* red-black ordered Gauss-Seidel sweeps   */
+#define CV_SMOOTH_RB     1    /* red-black: damped half-sweeps over even, then odd (i+j+k) cells */
 #define CV_SMOOTH_YLINE  2    /* y-line (wall-normal) Jacobi, Thomas solves */
 
 typedef struct cv_ctx cv_ctx;

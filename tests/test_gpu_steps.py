@@ -85,12 +85,12 @@ def test_mixed_pressure_matches_fp64():
         b.close()
 
 
-@pytest.mark.parametrize("smoother", ["jacobi", "yline"])
+@pytest.mark.parametrize("smoother", ["jacobi", "rb", "yline"])
 def test_smoothers_reach_the_same_solution(smoother):
     u, _ = gpu_case(SPHERE, mglevels=3)
     o, _ = oracle_case(SPHERE, nlev=3)
     u.set_solver(rtol=1e-12, max_it=500, smoother=smoother, pre_sweeps=2, post_sweeps=2,
-                 omega=0.8 if smoother == "jacobi" else 0.9)
+                 omega={"jacobi": 0.8, "rb": 1.0, "yline": 0.9}[smoother])
     o.set_solver(rtol=1e-12, max_it=500)
     try:
         u.FormBCS()
