@@ -1041,7 +1041,11 @@ int cv_ff1_dev(cv_ctx *c) {
   if (F.d.xper || F.d.zper)
     hipLaunchKernelGGL(k_rct_periodic, dim3(nblk((long)A.g.my * A.g.mz + (long)A.g.mx * A.g.my)), dim3(BS), 0, c->stream,
                        A.g, v3(c->rct));
-  hipLaunchKernelGGL(k_ff1_faces, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A, v3(c->rhs), F.d.n);
+  {
+    // algorithmic bytes per point: p, rct, 9 face coefficients, nvert, Rhs out, the operator word
+    CvProfScope ps(c, CV_PROF_FF1, c->nlev - 1, 0, (double)F.d.n * (8.0 * (1 + 3 + 9 + 1 + 3) + 4.0));
+    hipLaunchKernelGGL(k_ff1_faces, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, A, v3(c->rhs), F.d.n);
+  }
   if (F.d.xper || F.d.zper) {
     long nf = (F.d.xper ? (long)(A.g.my - 2) * (A.g.mz - 2) : 0) + (F.d.zper ? (long)(A.g.my - 2) * (A.g.mx - 2) : 0);
     hipLaunchKernelGGL(k_ff1_periodic, dim3(nblk(nf)), dim3(BS), 0, c->stream, A, v3(c->rhs));

@@ -161,8 +161,9 @@ int cv_nullspace_project(cv_ctx *ctx, int level, double *x);                    
  * Requests queue in arrival order; a request is admitted when one of the
  * server's slots is free and its context (cv_ctx_bytes) fits the HBM budget
  * beside the cases in flight.  An admitted case keeps all its fields
- * resident on the device for its nsteps fractional steps (cv_flow_step),
- * on its own HIP stream, so the cases in flight overlap on the GPU.  The
+ * resident on the device for its nsteps fractional steps, either on its
+ * own HIP stream (cv_flow_step; the cases in flight overlap on the GPU) or,
+ * in CV_SERVE_BATCHED mode, as a member of a cv_batch.  The
  * server copies the inputs at submit; results (p, Ucont) and per-step
  * wall-clock latencies stay with the server until cv_server_release.
  * cv_server_destroy finishes the cases in flight and drops queued ones. */
@@ -237,8 +238,8 @@ int cv_batch_destroy(cv_batch *b);
 #define CV_PROF_RESID  1   /* 19-point operator residual r = b - A x       */
 #define CV_PROF_APPLY  2   /* 19-point operator apply y = A x (Krylov)     */
 #define CV_PROF_LINE   3   /* y-line solve of the line smoother            */
-#define CV_PROF_RCT    4   /* momentum cell kernel (Rc -> rct)             */
-#define CV_PROF_FF1    5   /* momentum face kernel (rct, p -> Rhs)         */
+#define CV_PROF_RCT    4   /* momentum face-flux pass (Visc - Conv, Rc -> rct) */
+#define CV_PROF_FF1    5   /* momentum face RHS (rct, p -> Rhs)            */
 #define CV_PROF_ORTH   6   /* Gram-Schmidt dot products                    */
 int cv_prof_select(cv_ctx *ctx, int kernel, int level, int precision);
 int cv_prof_read(cv_ctx *ctx, long *launches, double *total_ms, double *bytes, int reset);

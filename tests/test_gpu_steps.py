@@ -214,3 +214,25 @@ def test_face_flux_variants_agree(monkeypatch, walls):
     finally:
         u.close()
         o.close()
+
+
+@pytest.mark.parametrize("kernel,per_step", [("resid", None), ("apply", None), ("line", None), ("rct", 4),
+                                             ("ff1", 4), ("orth", None)])
+def test_kernel_timing_counts_launches(kernel, per_step):
+    """cv_prof_select / cv_prof_read time one kernel class with device
+    events while the context runs normally: launches, time and algorithmic
+    bytes are recorded (four RK stages give four face-flux passes)."""
+    u, _ = gpu_case(SPHERE, mglevels=3)
+    u.set_solver(rtol=1e-9, max_it=200, smoother="yline", pre_sweeps=3, post_sweeps=3, omega=0.9)
+    try:
+        u.prof_select(kernel, level=-1)
+        u.Flow_Solver()
+        launches, ms, nbytes = u.prof_read(reset=True)
+        assert launches > 0 and ms > 0 and nbytes > 0, (launches, ms, nbytes)
+        if per_step is not None:
+            assert launches == per_step, launches
+        u.prof_select("none")
+        u.Flow_Solver()
+        assert u.prof_read()[0] == 0
+    finally:
+        u.close()
