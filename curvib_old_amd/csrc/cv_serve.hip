@@ -182,7 +182,12 @@ struct Group {
   cv_batch *b = nullptr;
   bool batchable = true;
 };
-bool same_solver(const cv_solver_opts &a, const cv_solver_opts &b) { return memcmp(&a, &b, sizeof(a)) == 0; }
+// field by field: the struct has padding a caller need not clear
+bool same_solver(const cv_solver_opts &a, const cv_solver_opts &b) {
+  return a.precision == b.precision && a.restart == b.restart && a.max_it == b.max_it && a.rtol == b.rtol &&
+         a.atol == b.atol && a.pre_sweeps == b.pre_sweeps && a.post_sweeps == b.post_sweeps &&
+         a.coarse_sweeps == b.coarse_sweeps && a.smoother == b.smoother && a.omega == b.omega;
+}
 }  // namespace
 
 // record one step of a case (lock taken here)
