@@ -1143,7 +1143,8 @@ int cv_imprk_dev(cv_ctx *c, cv_step_stats *st) {
     if (pseudot == 1) normF0 = normF_bk;
     normdU = normdU_bk;
     reldU = reldU_bk;
-    if (!std::isfinite(normdU) || !std::isfinite(normF_bk)) return CV_ERR_NONFINITE;
+    // an overflowing pseudo iteration (infinite norms) is rejected by the
+    // lambda-halving test below like any growing one, as in the reference
     if (pseudot > 1 && normF_bk > normF1 && normdU_bk > normdU1 && lambda > 0.001) {
       lambda = 0.5 * lambda;
       pseudot--;
@@ -1157,6 +1158,10 @@ int cv_imprk_dev(cv_ctx *c, cv_step_stats *st) {
   (void)normF0;
   if (st) { st->imp_its = pseudot; st->imp_lambda = lambda; st->imp_normdU = normdU; st->imp_reldU = reldU; }
   CV_CHECK(hipGetLastError());
+  if (!std::isfinite(normdU)) {
+    cv_set_error("ImpRK: pseudo iterations ended on a non-finite update");
+    return CV_ERR_NONFINITE;
+  }
   return CV_OK;
 }
 
