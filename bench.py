@@ -197,7 +197,8 @@ def batch_ensemble_only(args):
     serving.poisson_arrivals(args.serving_requests, 50.0, rng)
     ens = [serving.sample_case(rng, lo=(32, 32, 32), hi=(128, 96, 96)) for _ in range(args.ensemble_cases)]
     arrays = serving.build_inputs(ens, workers=args.build_workers)
-    r = serving.run_batched(list(zip(ens, arrays)), args.ensemble_steps, args.batch_ensemble_only, log=log, **SOLVER)
+    r = serving.run_batched(list(zip(ens, arrays)), args.ensemble_steps, args.batch_ensemble_only, log=log,
+                            max_levels=args.ensemble_max_levels, **SOLVER)
     print(json.dumps({"workload": f"{len(ens)}-case ensemble, one cv_batch", **r}), flush=True)
 
 
@@ -261,6 +262,7 @@ def main():
     ap.add_argument("--omega", type=float, default=SOLVER["omega"])
     ap.add_argument("--smoother", default=SOLVER["smoother"], choices=("jacobi", "rb", "yline"))
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--ensemble-max-levels", type=int, default=8)
     ap.add_argument("--batch-ensemble-only", choices=("mixed", "fp64"), default=None,
                     help="run only the batched-ensemble leg in this precision")
     ap.add_argument("--serving-requests", type=int, default=50)

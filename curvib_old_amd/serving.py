@@ -233,7 +233,7 @@ def poisson_arrivals(n, lam, rng):
     return np.cumsum(rng.exponential(1.0 / lam, size=n))
 
 
-def run_batched(requests, nsteps, precision, warmup=1, device=0, log=None, **solver):
+def run_batched(requests, nsteps, precision, warmup=1, device=0, log=None, max_levels=8, **solver):
     """The ensemble as one cv_batch: every case [(case, arrays)] resident in
     its own context, all advanced together ``warmup`` untimed steps and then
     ``nsteps`` timed steps.  Every case finishes a step when the batch step
@@ -243,8 +243,8 @@ def run_batched(requests, nsteps, precision, warmup=1, device=0, log=None, **sol
     b = None
     try:
         for case, (xyz, nv, uc) in requests:
-            u = UserCtx(case.nodes, case.bctype, levels_for(case.cells), case.re, case.dt, semi=SEMI, device=device,
-                        coarsen_auto=COARSEN_AUTO)
+            u = UserCtx(case.nodes, case.bctype, levels_for(case.cells, max_levels=max_levels), case.re, case.dt,
+                        semi=SEMI, device=device, coarsen_auto=COARSEN_AUTO)
             u.set_grid(xyz)
             u.set_nvert(nv)
             u.put("ucont", uc)
