@@ -41,6 +41,7 @@ SYMBOLS = [
     "cv_ctx_bytes", "cv_server_create", "cv_server_submit", "cv_server_wait", "cv_server_step_ms",
     "cv_server_fetch", "cv_server_release", "cv_server_status", "cv_server_destroy",
     "cv_batch_create", "cv_batch_step", "cv_batch_poisson_solve", "cv_batch_destroy", "cv_server_create_mode",
+    "cv_inflow_flux", "cv_outflow_flux",
 ]
 CV_SERVE_STREAMS, CV_SERVE_BATCHED = 0, 1
 CV_CASE_QUEUED, CV_CASE_RUNNING, CV_CASE_DONE, CV_CASE_FAILED = 0, 1, 2, 3
@@ -51,7 +52,8 @@ CV_PROF = {"none": 0, "resid": 1, "apply": 2, "line": 3, "rct": 4, "ff1": 5, "or
 class GridDesc(C.Structure):
     _fields_ = [("IM", C.c_int), ("JM", C.c_int), ("KM", C.c_int), ("bctype", C.c_int * 6),
                 ("mglevels", C.c_int), ("isc", C.c_int), ("jsc", C.c_int), ("ksc", C.c_int),
-                ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double), ("coarsen_auto", C.c_int)]
+                ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double), ("coarsen_auto", C.c_int),
+                ("inletprofile", C.c_int), ("inlet_center", C.c_double * 3)]
 
 
 class SolverOpts(C.Structure):
@@ -67,7 +69,8 @@ class StepStats(C.Structure):
                 ("maxdiv_idx", C.c_long), ("ibm_flux0", C.c_double), ("ibm_area0", C.c_double),
                 ("ibm_flux1", C.c_double), ("ibm_area1", C.c_double), ("t_mom", C.c_double),
                 ("t_p", C.c_double), ("t_pr", C.c_double), ("imp_its", C.c_int), ("imp_lambda", C.c_double),
-                ("imp_normdU", C.c_double), ("imp_reldU", C.c_double)]
+                ("imp_normdU", C.c_double), ("imp_reldU", C.c_double), ("flux_in", C.c_double),
+                ("flux_out", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -120,6 +123,8 @@ def load():
     for nm in ("cv_contra2cart", "cv_form_bcs", "cv_rungekutta", "cv_update_pressure", "cv_projection"):
         getattr(L, nm).argtypes = [vp]
     L.cv_form_function1.argtypes = [vp, dp]
+    L.cv_inflow_flux.argtypes = [vp, dp]
+    L.cv_outflow_flux.argtypes = [vp, dp]
     L.cv_volumeflux_rev.argtypes = [vp, dp]
     L.cv_poisson_rhs.argtypes = [vp, dp]
     L.cv_poisson_apply.argtypes = [vp, C.c_int, dp, dp]

@@ -45,6 +45,8 @@ class ChannelCase:
     lx: float = 1.0
     ly: float = 1.0
     lz: float = 1.0
+    inlet: int = 2                          # -inlet profile of a bctype-5 k- face (main.c:23)
+    inlet_center: tuple = (0.0, 0.0, 0.0)   # -x_c -y_c -z_c (main.c:924-926)
 
     @property
     def nodes(self):
@@ -202,6 +204,7 @@ def initial_ucont(case: ChannelCase, xyz: np.ndarray, nvert: np.ndarray) -> np.n
         uc[:, my - 2, :, 1] = 0.0
     if bc[4] == 1:
         uc[0, :, :, 2] = 0.0
+    if bc[5] == 1:
         uc[mz - 2, :, :, 2] = 0.0
     # faces touching a non-fluid cell carry no flux
     solid = nvert > 0.1

@@ -968,12 +968,13 @@ extern "C" int cv_batch_poisson_solve(cv_batch *B, cv_step_stats *st) {
 
 static int b_finish(cv_batch *B, cv_step_stats *st, double tp);
 
-// every case runs explicit RK with kernel timing off: the whole step
-// around the solve can then run on the batch stream, one launch per kernel
+// every case runs explicit RK with kernel timing off and has no inlet or
+// outlet: the whole step around the solve can then run on the batch
+// stream, one launch per kernel
 static bool b_mom_batchable(const cv_batch *B) {
   if (!B->use_mom || !B->mom) return false;
   for (const cv_ctx *x : B->cs)
-    if (x->mom.scheme != CV_MOM_RK || x->prof.kind != CV_PROF_NONE) return false;
+    if (x->mom.scheme != CV_MOM_RK || x->prof.kind != CV_PROF_NONE || x->inlet || x->outlet) return false;
   return true;
 }
 
@@ -1079,7 +1080,11 @@ static int b_finish(cv_batch *B, cv_step_stats *st, double tp) {
   int conv_rc = CV_OK;
   for (int c = 0; c < B->n; c++) {
     cv_ctx *x = B->cs[c];
+    if (x->inlet || x->outlet)
+      CV_CHECK(hipMemcpyAsync(x->red_host + 110, x->dsc + CV_DSC_FLUXIN, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              x->stream));
     if ((rc = cv_divergence_dev(x, &st[c].maxdiv, &st[c].maxdiv_idx))) return rc;
+    if (x->inlet || x->outlet) { st[c].flux_in = x->red_host[110]; st[c].flux_out = x->red_host[111]; }
     x->last = st[c];
     x->steps_done++;
     if (!std::isfinite(st[c].maxdiv)) return CV_ERR_NONFINITE;

@@ -90,19 +90,32 @@ extern "C" int cv_set_momentum(cv_ctx *c, const cv_momentum_opts *o) {
   return CV_OK;
 }
 
-static int check_desc(const cv_grid_desc *d) {
-  if (!d) return 0;
-  if (d->IM < 4 || d->JM < 4 || d->KM < 4) return 0;
-  for (int q = 0; q < 6; q++)
-    if (d->bctype[q] != 1 && d->bctype[q] != 7) return 0;
-  if (d->bctype[2] == 7 || d->bctype[3] == 7) return 0;            // j-periodic not supported
-  if ((d->bctype[0] == 7) != (d->bctype[1] == 7)) return 0;
-  if ((d->bctype[4] == 7) != (d->bctype[5] == 7)) return 0;
-  if (d->mglevels < 1 || d->mglevels > CV_MAXLEV) return 0;
-  if (d->dt <= 0 || d->ren <= 0) return 0;
-  if ((long)(d->IM + 1) * (d->JM + 1) * (d->KM + 1) >= (1L << 31)) return 0;   // 32-bit storage indices in kernels
-  return 1;
+// the analytic inflow profiles of InflowFlux (bcs.c:158-196); 0, 3, 5, 6
+// and 8 interpolate an inlet file and are not supported
+static bool inlet_profile_ok(int p) {
+  return p == 1 || p == -1 || p == 2 || p == 4 || p == 7 || p == 10 || p == 11;
 }
+
+// what is wrong with a grid description, or nullptr
+static const char *desc_problem(const cv_grid_desc *d) {
+  if (!d) return "no description";
+  if (d->IM < 4 || d->JM < 4 || d->KM < 4) return "fewer than 4 nodes on an axis";
+  for (int q = 0; q < 4; q++)
+    if (d->bctype[q] != 1 && d->bctype[q] != 7) return "i and j faces take bctype 1 or 7";
+  if (d->bctype[4] != 1 && d->bctype[4] != 5 && d->bctype[4] != 7) return "the k- face takes bctype 1, 5 or 7";
+  if (d->bctype[5] != 1 && d->bctype[5] != 4 && d->bctype[5] != 7) return "the k+ face takes bctype 1, 4 or 7";
+  if (d->bctype[2] == 7 || d->bctype[3] == 7) return "j-periodic grids are not supported";
+  if ((d->bctype[0] == 7) != (d->bctype[1] == 7)) return "bctype 7 must pair on the i faces";
+  if ((d->bctype[4] == 7) != (d->bctype[5] == 7)) return "bctype 7 must pair on the k faces";
+  if (d->bctype[4] == 5 && !inlet_profile_ok(d->inletprofile))
+    return "inlet profile not supported (analytic profiles 1, -1, 2, 4, 7, 10, 11)";
+  if (d->mglevels < 1 || d->mglevels > CV_MAXLEV) return "mglevels out of range";
+  if (d->dt <= 0 || d->ren <= 0) return "dt and ren must be positive";
+  if ((long)(d->IM + 1) * (d->JM + 1) * (d->KM + 1) >= (1L << 31)) return "grid too large";   // 32-bit storage indices in kernels
+  return nullptr;
+}
+static int check_desc(const cv_grid_desc *d) { return desc_problem(d) == nullptr; }
+static bool desc_kio(const cv_grid_desc *d) { return d->bctype[4] == 5 || d->bctype[5] == 4; }
 
 // MGDACreate dimensions (init.c:78-99), level 0 coarsest; each coarsening
 // must keep an even cell count on coarsened axes so that fine cells pair
@@ -153,9 +166,10 @@ static int level_dims(const cv_grid_desc *desc, CvDims *dims) {
 // line factors, line scratch, x b r (18 arrays); the operator word), the finest-level
 // metrics (40 doubles), state (25 doubles: 7 vectors of 3, p phi b w) and
 // the 3n exchange staging, reduction scratch and the Krylov space of
-// FGMRES(restart)
-static size_t plan_bytes(const CvDims *dims, int nlev, int restart) {
+// FGMRES(restart); with an inlet or outlet the Ubcs of the two k planes
+static size_t plan_bytes(const CvDims *dims, int nlev, int restart, bool kio) {
   size_t b = 0;
+  if (kio) b += (size_t)6 * dims[nlev - 1].mx * dims[nlev - 1].my * 8;
   for (int l = 0; l < nlev; l++) b += (size_t)dims[l].n * (22 * 8 + 18 * 4 + 4);
   const size_t n = (size_t)dims[nlev - 1].n;
   b += n * 8 * (40 + 25 + 3);
@@ -170,14 +184,14 @@ extern "C" long cv_ctx_bytes(const cv_grid_desc *desc, int restart) {
   if (!check_desc(desc)) return -1;
   CvDims dims[CV_MAXLEV];
   if (level_dims(desc, dims)) return -1;
-  return (long)plan_bytes(dims, desc->mglevels, restart);
+  return (long)plan_bytes(dims, desc->mglevels, restart, desc_kio(desc));
 }
 
 extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out) {
   if (!out) return CV_ERR_ARG;
   *out = nullptr;
-  if (!check_desc(desc)) {
-    cv_set_error("cv_ctx_create: unsupported grid description");
+  if (const char *why = desc_problem(desc)) {
+    cv_set_error("cv_ctx_create: unsupported grid description: %s", why);
     return CV_ERR_ARG;
   }
   CvDims dims[CV_MAXLEV];
@@ -188,6 +202,11 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
   c->nlev = desc->mglevels;
   c->IM = desc->IM; c->JM = desc->JM; c->KM = desc->KM;
   memcpy(c->bctype, desc->bctype, sizeof(c->bctype));
+  c->inlet = desc->bctype[4] == 5;
+  c->outlet = desc->bctype[5] == 4;
+  c->inletprofile = desc->inletprofile;
+  for (int q = 0; q < 3; q++) c->cm_c[q] = desc->inlet_center[q];
+  c->ubk = nullptr;
   c->ren = desc->ren; c->dt = desc->dt; c->st = desc->st > 0 ? desc->st : 1.0;
   cv_default_solver(&c->opt);
   cv_default_momentum(&c->mom);
@@ -257,6 +276,14 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     c->red_len = 1 << 16;
     if ((rc = cv_alloc(c, &p, c->red_len * sizeof(double)))) goto fail; c->red = (double *)p;
     if ((rc = cv_alloc(c, &p, 128 * sizeof(double)))) goto fail; c->dsc = (double *)p;
+    CV_CHECK(hipMemsetAsync(c->dsc, 0, 128 * sizeof(double), c->stream));   // FluxInSum starts at 0
+    if (c->inlet || c->outlet) {
+      const CvDims &D = c->lev[c->nlev - 1].d;
+      const size_t nb = (size_t)6 * D.mx * D.my * sizeof(double);
+      if ((rc = cv_alloc(c, &p, nb))) goto fail;
+      c->ubk = (double *)p;
+      CV_CHECK(hipMemsetAsync(c->ubk, 0, nb, c->stream));
+    }
     if ((rc = cv_alloc(c, &p, 34 * 1024 * sizeof(double)))) goto fail; c->part = (double *)p;
     c->krylov_m = 0;
     CV_CHECK(hipHostMalloc((void **)&c->red_host, 4096 * sizeof(double), hipHostMallocDefault));

@@ -102,7 +102,18 @@ struct cv_ctx {
   std::vector<void *> allocs;
   int geometry_set, nvert_set;
   CvProf prof;
+  // inlet (bctype[4] == 5) and outlet (bctype[5] == 4) of the k faces
+  int inlet, outlet;
+  int inletprofile;           // -inlet
+  double cm_c[3];             // -x_c -y_c -z_c
+  double *ubk;                // Ubcs of the k = 0 and k = mz-1 planes, [plane][x|y|z][i + mx*j]
 };
+
+// device scalars of the inlet/outlet update (slots of cv_ctx::dsc)
+#define CV_DSC_FLUXIN  116    // FluxInSum (InflowFlux)
+#define CV_DSC_FLUXOUT 117    // FluxOutSum after FormBCS's outlet correction
+#define CV_DSC_OUTSUM  118    // outlet velocity flux and area over the last fluid cells (2)
+#define CV_DSC_OUTDIAG 121    // OutflowFlux's sum
 
 // Records device events around one launch when (kind, level, precision)
 // matches the selection; prec 0 = FP64, 1 = FP32.  bytes: algorithmic bytes
@@ -133,6 +144,8 @@ int cv_levels_setup_nvert(cv_ctx *c, const double *nvert_host);
 int cv_poisson_prepare(cv_ctx *c);
 int cv_contra2cart_dev(cv_ctx *c);
 int cv_ghost_update(cv_ctx *c, int formbcs);
+int cv_inflow_dev(cv_ctx *c);
+int cv_bc_update_dev(cv_ctx *c);   // InflowFlux + FormBCS
 int cv_poisson_solve_dev(cv_ctx *c, cv_step_stats *st);
 int cv_update_pressure_dev(cv_ctx *c);
 int cv_projection_dev(cv_ctx *c);

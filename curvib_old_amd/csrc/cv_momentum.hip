@@ -2,7 +2,9 @@
 // driver on gfx950.
 //
 //  Contra2Cart        rhs.c:25-144
-//  FormBCS            bcs.c:4436-5888  (bctype 1 wall, 7 periodic)
+//  InflowFlux         bcs.c:99-246     (bctype 5 on the k- face, analytic profiles)
+//  OutflowFlux        bcs.c:403-469    (bctype 4 on the k+ face)
+//  FormBCS            bcs.c:4436-5888  (bctype 1 wall, 7 periodic, 5 inlet, 4 outlet)
 //  GhostNodeVelocity  bcs.c:3645-4435  (same subset)
 //  FormFunction1      rhs.c:2567-3351 with Convection (QUICK, rhs.c:1463-1945)
 //                     and Viscous (laminar, rhs.c:1947-2565)
@@ -85,42 +87,55 @@ inline dim3 march_grid(int mx, int my, int mz) {
 // ------------------------------------------------------------------
 // Contra2Cart (rhs.c:25-144)
 // ------------------------------------------------------------------
+// one interior cell (i, j, k); IB cells (0.1 <= nvert <= 7) keep their value
+__device__ __forceinline__ void c2c_cell(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, int i, int j, int k) {
+  const long t = g.at(i, j, k);
+  const double nn = nv[t];
+  if (!(nn < 0.1 || nn > 7.)) return;
+  const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
+  double m00 = 0.5 * (csi.x[ti] + csi.x[t]), m01 = 0.5 * (csi.y[ti] + csi.y[t]), m02 = 0.5 * (csi.z[ti] + csi.z[t]);
+  double m10 = 0.5 * (eta.x[tj] + eta.x[t]), m11 = 0.5 * (eta.y[tj] + eta.y[t]), m12 = 0.5 * (eta.z[tj] + eta.z[t]);
+  double m20 = 0.5 * (zet.x[tk] + zet.x[t]), m21 = 0.5 * (zet.y[tk] + zet.y[t]), m22 = 0.5 * (zet.z[tk] + zet.z[t]);
+  double q0 = 0.5 * (uc.x[ti] + uc.x[t]), q1 = 0.5 * (uc.y[tj] + uc.y[t]), q2 = 0.5 * (uc.z[tk] + uc.z[t]);
+  double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+  double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
+  double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
+  double det2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
+  ucat.x[t] = det0 / det;
+  ucat.y[t] = det1 / det;
+  ucat.z[t] = det2 / det;
+}
 __device__ __forceinline__ void contra2cart_body(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n, int bx, int by, int bz) {
   (void)n;
   CV_MARCH_B(g, bx, by, bz) {
-    const long t = g.at(i, j, k);
     if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
-    const double nn = nv[t];
-    if (!(nn < 0.1 || nn > 7.)) continue;
-    const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
-    double m00 = 0.5 * (csi.x[ti] + csi.x[t]), m01 = 0.5 * (csi.y[ti] + csi.y[t]), m02 = 0.5 * (csi.z[ti] + csi.z[t]);
-    double m10 = 0.5 * (eta.x[tj] + eta.x[t]), m11 = 0.5 * (eta.y[tj] + eta.y[t]), m12 = 0.5 * (eta.z[tj] + eta.z[t]);
-    double m20 = 0.5 * (zet.x[tk] + zet.x[t]), m21 = 0.5 * (zet.y[tk] + zet.y[t]), m22 = 0.5 * (zet.z[tk] + zet.z[t]);
-    double q0 = 0.5 * (uc.x[ti] + uc.x[t]), q1 = 0.5 * (uc.y[tj] + uc.y[t]), q2 = 0.5 * (uc.z[tk] + uc.z[t]);
-    double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
-    double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
-    double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
-    double det2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
-    ucat.x[t] = det0 / det;
-    ucat.y[t] = det1 / det;
-    ucat.z[t] = det2 / det;
+    c2c_cell(g, uc, csi, eta, zet, nv, ucat, i, j, k);
   }
 }
 __global__ void k_contra2cart(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n) { contra2cart_body(g, uc, csi, eta, zet, nv, ucat, n, blockIdx.x, blockIdx.y, blockIdx.z); }
+// Contra2Cart of the interior cells of one k plane
+__global__ void k_contra2cart_plane(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, int k) {
+  const long m2 = (long)(g.mx - 2) * (g.my - 2);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < m2; t += (long)gridDim.x * blockDim.x)
+    c2c_cell(g, uc, csi, eta, zet, nv, ucat, 1 + (int)(t % (g.mx - 2)), 1 + (int)(t / (g.mx - 2)), k);
+}
 
 
 // ------------------------------------------------------------------
 // ghost planes (FormBCS bcs.c:5252-5731, GhostNodeVelocity bcs.c:4015-4415)
 // ------------------------------------------------------------------
-// wall ghosts: ucat_g = 2*ubcs - ucat_inner with ubcs = 0.  full: whole
-// tangential ranges (GhostNodeVelocity); otherwise interior ranges (FormBCS)
-__device__ __forceinline__ void wall_ghost_body(G g, int bx, int bz, int full, V3 u, GS gs) {
+// wall ghosts: ucat_g = 2*ubcs - ucat_inner.  Ubcs is 0 on the i and j
+// faces; on the k faces it is ubk (the inlet / outlet planes, 0 elsewhere)
+// or 0 when ubk is null.  full: whole tangential ranges (GhostNodeVelocity);
+// otherwise interior ranges (FormBCS)
+__device__ __forceinline__ void wall_ghost_body(G g, int bx, int bz, int full, V3 u, const double *ubk, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   const int a0 = full ? 0 : 1;
   const long nyz = (long)my * mz, nxz = (long)mx * mz, nxy = (long)mx * my;
   const long tot = nyz + nxz + nxy;
   for (long t = gs.t0; t < tot; t += gs.step) {
     long lo, hi, ln, in_lo, in_hi;
+    double lx = 0.0, ly = 0.0, lz = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;   // Ubcs of the two ghosts
     if (t < nyz) {                       // x planes
       if (bx) continue;
       int j = (int)(t % my), k = (int)(t / my);
@@ -138,12 +153,16 @@ __device__ __forceinline__ void wall_ghost_body(G g, int bx, int bz, int full, V
       int i = (int)(r % mx), j = (int)(r / mx);
       if (i < a0 || i >= (full ? mx : mx - 1) || j < a0 || j >= (full ? my : my - 1)) continue;
       lo = g.at(i, j, 0); in_lo = g.at(i, j, 1); hi = g.at(i, j, mz - 1); in_hi = g.at(i, j, mz - 2);
+      if (ubk) {
+        lx = ubk[r]; ly = ubk[nxy + r]; lz = ubk[2 * nxy + r];
+        hx = ubk[3 * nxy + r]; hy = ubk[4 * nxy + r]; hz = ubk[5 * nxy + r];
+      }
     }
-    u.x[lo] = 2 * 0.0 - u.x[in_lo]; u.y[lo] = 2 * 0.0 - u.y[in_lo]; u.z[lo] = 2 * 0.0 - u.z[in_lo];
-    u.x[hi] = 2 * 0.0 - u.x[in_hi]; u.y[hi] = 2 * 0.0 - u.y[in_hi]; u.z[hi] = 2 * 0.0 - u.z[in_hi];
+    u.x[lo] = 2 * lx - u.x[in_lo]; u.y[lo] = 2 * ly - u.y[in_lo]; u.z[lo] = 2 * lz - u.z[in_lo];
+    u.x[hi] = 2 * hx - u.x[in_hi]; u.y[hi] = 2 * hy - u.y[in_hi]; u.z[hi] = 2 * hz - u.z[in_hi];
   }
 }
-__global__ void k_wall_ghost(G g, int bx, int bz, int full, V3 u) { wall_ghost_body(g, bx, bz, full, u, GS_SELF); }
+__global__ void k_wall_ghost(G g, int bx, int bz, int full, V3 u, const double *ubk) { wall_ghost_body(g, bx, bz, full, u, ubk, GS_SELF); }
 
 
 // periodic plane copies restricted to the interior of the other axes
@@ -230,7 +249,9 @@ __global__ void k_periodic_full(G g, int axis, int x_only, V3 u, double *p, doub
 
 
 // zero wall-normal flux of bctype-1 faces after an RK stage (the build's
-// deliberate deviation: the reference never resets it)
+// deliberate deviation: the reference never resets it).  bx, bz: bit 0 the
+// lower face, bit 1 the upper face of the axis is a wall; the j faces
+// always are
 __device__ __forceinline__ void wall_flux_zero_body(G g, int bx, int bz, V3 u, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   const long nyz = (long)my * mz, nxz = (long)mx * mz, nxy = (long)mx * my;
@@ -238,7 +259,8 @@ __device__ __forceinline__ void wall_flux_zero_body(G g, int bx, int bz, V3 u, G
     if (t < nyz) {
       if (!bx) continue;
       int j = (int)(t % my), k = (int)(t / my);
-      u.x[g.at(0, j, k)] = 0.; u.x[g.at(mx - 2, j, k)] = 0.;
+      if (bx & 1) u.x[g.at(0, j, k)] = 0.;
+      if (bx & 2) u.x[g.at(mx - 2, j, k)] = 0.;
     } else if (t < nyz + nxz) {
       long r = t - nyz;
       int i = (int)(r % mx), k = (int)(r / mx);
@@ -247,11 +269,139 @@ __device__ __forceinline__ void wall_flux_zero_body(G g, int bx, int bz, V3 u, G
       if (!bz) continue;
       long r = t - nyz - nxz;
       int i = (int)(r % mx), j = (int)(r / mx);
-      u.z[g.at(i, j, 0)] = 0.; u.z[g.at(i, j, mz - 2)] = 0.;
+      if (bz & 1) u.z[g.at(i, j, 0)] = 0.;
+      if (bz & 2) u.z[g.at(i, j, mz - 2)] = 0.;
     }
   }
 }
 __global__ void k_wall_flux_zero(G g, int bx, int bz, V3 u) { wall_flux_zero_body(g, bx, bz, u, GS_SELF); }
+
+// wall masks of wall_flux_zero_body for one axis from its two bctypes
+inline int wall_mask(int lo, int hi) { return (lo == 1 ? 1 : 0) | (hi == 1 ? 2 : 0); }
+
+
+// ------------------------------------------------------------------
+// inlet (bctype 5, k = 0) and outlet (bctype 4, k = mz-1):
+// InflowFlux bcs.c:99-246, OutflowFlux bcs.c:403-469, FormBCS's outlet
+// section bcs.c:4914-5010.  Plane kernels over the interior (i, j) of one
+// k plane; sums go through per-block partials and a fixed-order final
+// pass.  ubk holds Ubcs of plane 0 (k = 0) and plane 1 (k = mz-1), planar
+// x|y|z, index i + mx*j.
+// ------------------------------------------------------------------
+// the analytic inflow profiles (bcs.c:158-196) at the offset face centre
+__device__ double inlet_uin(int prof, double xc, double yc, double zc) {
+  const double H = 4.1, Umax = 1.5;
+  const double r = sqrt(zc * zc + yc * yc);
+  double uin = 0.;
+  if (prof == 1) uin = 2.;
+  else if (prof == -1) uin = -1.;
+  else if (prof == 2) uin = 4. * Umax * yc * (H - yc) / (H * H);
+  else if (prof == 4) {            // fully developed pipe flow
+    uin = 2. * (1. - 4. * r * r);
+    if (r > 0.5) uin = 0.;
+  } else if (prof == 7) uin = 1.5 * (1. - 4. * yc * yc);
+  else if (prof == 10) {           // fully developed flow in a 2 x 2 duct, 20 terms
+    const double pi = 3.14159265358979323846264338327950288;
+    const double y2 = (yc - 1.5) * 2., x2 = xc - 0.5, A = 1.;
+    uin = 1 - y2 * y2;
+    for (int n = 0; n < 20; n++)
+      uin -= 4. * pow(2. / pi, 3) * pow(-1., n) / pow(2 * n + 1., 3.) * cosh((2 * n + 1) * pi * x2 / 2.) *
+             cos((2. * n + 1) * pi * y2 / 2.) / cosh((2 * n + 1) * pi * A / 2.);
+  } else if (prof == 11) uin = 0.185;
+  return uin;
+}
+
+struct Inlet {
+  int prof;
+  double cx, cy, cz;   // -x_c -y_c -z_c
+};
+
+// inlet fluxes and Ubcs where the first cell is fluid; the fluid inflow
+// sum to part[block]
+__global__ void k_inflow(G g, Inlet in, const double *coor, long n, C3 zet, const double *nv, double *ucz, double *ubk,
+                         double *part) {
+  __shared__ double sh[BS / 64];
+  const long nxy = (long)g.mx * g.my, m2 = (long)(g.mx - 2) * (g.my - 2);
+  const double *X = coor, *Y = coor + n, *Z = coor + 2 * n;
+  double f = 0.0;
+  for (long t = blockIdx.x * (long)BS + threadIdx.x; t < m2; t += (long)gridDim.x * BS) {
+    const int i = 1 + (int)(t % (g.mx - 2)), j = 1 + (int)(t / (g.mx - 2));
+    const long c = g.at(i, j, 0), cs = c - g.mx, cw = c - 1, csw = c - g.mx - 1;   // nodes (i,j) (i,j-1) (i-1,j) (i-1,j-1)
+    const double xc = (X[c] + X[cs] + X[cw] + X[csw]) * 0.25 - in.cx;
+    const double yc = (Y[c] + Y[cs] + Y[cw] + Y[csw]) * 0.25 - in.cy;
+    const double zc = (Z[c] + Z[cs] + Z[cw] + Z[csw]) * 0.25 - in.cz;
+    const double uin = inlet_uin(in.prof, xc, yc, zc);
+    if (nv[c + g.sk] < 0.1) {
+      const double zx = zet.x[c], zy = zet.y[c], zz = zet.z[c];
+      const double a = sqrt(zz * zz + zy * zy + zx * zx);
+      ubk[nxy + c] = uin * zy / a;
+      ubk[2 * nxy + c] = uin * zz / a;
+      ubk[c] = uin * zx / a;
+      const double q = uin * a;
+      ucz[c] = q;
+      f += q;
+    }
+  }
+  const double sf = cv_block_sum(f, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = sf;
+}
+
+// outlet: velocity flux and area over the last fluid cells (k = mz-2) to
+// part[block] and part[nb + block]
+__global__ void k_outlet_sum(G g, C3 ucat, C3 zet, const double *nv, double *part) {
+  __shared__ double sh[BS / 64];
+  const long m2 = (long)(g.mx - 2) * (g.my - 2);
+  double f = 0.0, a = 0.0;
+  for (long t = blockIdx.x * (long)BS + threadIdx.x; t < m2; t += (long)gridDim.x * BS) {
+    const long c = g.at(1 + (int)(t % (g.mx - 2)), 1 + (int)(t / (g.mx - 2)), g.mz - 2);
+    if (nv[c] < 0.1) {
+      const double zx = zet.x[c], zy = zet.y[c], zz = zet.z[c];
+      f += ucat.x[c] * zx + ucat.y[c] * zy + ucat.z[c] * zz;
+      a += sqrt(zx * zx + zy * zy + zz * zz);
+    }
+  }
+  const double sf = cv_block_sum(f, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = sf;
+  const double sa = cv_block_sum(a, sh);
+  if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = sa;
+}
+
+// outlet correction: Ubcs = Ucat of the last cell, its flux plus the
+// uniform correction (FluxInSum - flux) / area; the corrected fluid
+// outflow to part[block] when stat
+__global__ void k_outlet_fix(G g, C3 ucat, C3 zet, const double *nv, double *ucz, double *ubk, const double *dsc,
+                             int stat, double *part) {
+  __shared__ double sh[BS / 64];
+  const long nxy = (long)g.mx * g.my, m2 = (long)(g.mx - 2) * (g.my - 2), base = (long)(g.mz - 2) * nxy;
+  // FluxIn = FluxInSum + FarFluxInSum + FluxIntpSum, the last two 0 here
+  const double ratio = (dsc[CV_DSC_FLUXIN] - dsc[CV_DSC_OUTSUM]) / dsc[CV_DSC_OUTSUM + 1];
+  double f = 0.0;
+  for (long t = blockIdx.x * (long)BS + threadIdx.x; t < m2; t += (long)gridDim.x * BS) {
+    const long r = g.at(1 + (int)(t % (g.mx - 2)), 1 + (int)(t / (g.mx - 2)), 0), c = base + r;
+    if (nv[c] < 0.1) {
+      const double bx = ucat.x[c], by = ucat.y[c], bz = ucat.z[c];
+      ubk[3 * nxy + r] = bx; ubk[4 * nxy + r] = by; ubk[5 * nxy + r] = bz;
+      const double zx = zet.x[c], zy = zet.y[c], zz = zet.z[c];
+      const double q = (bx * zx + by * zy + bz * zz) + ratio * sqrt(zx * zx + zy * zy + zz * zz);
+      ucz[c] = q;
+      f += q;
+    }
+  }
+  if (!stat) return;
+  const double sf = cv_block_sum(f, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = sf;
+}
+
+// OutflowFlux: the k = mz-2 face flux over the interior, fluid or not
+__global__ void k_outflow_diag(G g, const double *ucz, double *part) {
+  __shared__ double sh[BS / 64];
+  const long m2 = (long)(g.mx - 2) * (g.my - 2);
+  double f = 0.0;
+  for (long t = blockIdx.x * (long)BS + threadIdx.x; t < m2; t += (long)gridDim.x * BS)
+    f += ucz[g.at(1 + (int)(t % (g.mx - 2)), 1 + (int)(t / (g.mx - 2)), g.mz - 2)];
+  const double sf = cv_block_sum(f, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = sf;
+}
 
 
 // ------------------------------------------------------------------
@@ -948,21 +1098,72 @@ int cv_contra2cart_dev(cv_ctx *c) {
   return CV_OK;
 }
 
-// formbcs=1: the three passes of FormBCS (Contra2Cart once: it is
-// idempotent across the passes); formbcs=0: GhostNodeVelocity
+// blocks of a plane kernel over the (mx-2) x (my-2) interior of a k plane
+static unsigned plane_nb(const G &g) {
+  const long m2 = (long)(g.mx - 2) * (g.my - 2);
+  return (unsigned)std::min(256L, std::max(1L, (m2 + 4 * BS - 1) / (4 * BS)));
+}
+
+// InflowFlux (bcs.c:99-246): inlet fluxes and Ubcs of a bctype-5 k- face,
+// FluxInSum to dsc (device); nothing without an inlet, as FluxInSum is
+// only written by the inlet branch
+int cv_inflow_dev(cv_ctx *c) {
+  if (!c->inlet) return CV_OK;
+  CvLevel &F = c->lev[c->nlev - 1];
+  const G g = gview(c);
+  const unsigned nb = plane_nb(g);
+  const Inlet in{c->inletprofile, c->cm_c[0], c->cm_c[1], c->cm_c[2]};
+  hipLaunchKernelGGL(k_inflow, dim3(nb), dim3(BS), 0, c->stream, g, in, (const double *)F.coor, F.d.n, c3(c->m.zet),
+                     (const double *)F.nvert, c->ucont[2], c->ubk, c->part);
+  hipLaunchKernelGGL(cv_k_final_sum, dim3(1), dim3(BS), 0, c->stream, (const double *)c->part, (int)nb,
+                     c->dsc + CV_DSC_FLUXIN, 1);
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+// FormBCS's outlet section (bcs.c:4914-5010) from the current Ucat; with
+// stat the corrected outflow FluxOutSum to dsc
+static void outlet_fix(cv_ctx *c, int stat) {
+  CvLevel &F = c->lev[c->nlev - 1];
+  const G g = gview(c);
+  const unsigned nb = plane_nb(g);
+  const double *nv = F.nvert;
+  hipLaunchKernelGGL(k_outlet_sum, dim3(nb), dim3(BS), 0, c->stream, g, c3(c->ucat), c3(c->m.zet), nv, c->part);
+  hipLaunchKernelGGL(cv_k_final_sum, dim3(2), dim3(BS), 0, c->stream, (const double *)c->part, (int)nb,
+                     c->dsc + CV_DSC_OUTSUM, 2);
+  hipLaunchKernelGGL(k_outlet_fix, dim3(nb), dim3(BS), 0, c->stream, g, c3(c->ucat), c3(c->m.zet), nv, c->ucont[2],
+                     c->ubk, (const double *)c->dsc, stat, c->part + 2 * nb);
+  if (stat)
+    hipLaunchKernelGGL(cv_k_final_sum, dim3(1), dim3(BS), 0, c->stream, (const double *)(c->part + 2 * nb), (int)nb,
+                       c->dsc + CV_DSC_FLUXOUT, 1);
+}
+
+// formbcs=1: the three passes of FormBCS.  Contra2Cart runs once over the
+// grid: between passes only the outlet correction changes Ucont (the
+// k = mz-2 fluxes), so later passes recompute that plane alone.
+// formbcs=0: GhostNodeVelocity
 int cv_ghost_update(cv_ctx *c, int formbcs) {
   CvLevel &F = c->lev[c->nlev - 1];
+  CvMetrics &M = c->m;
   G g = gview(c);
   const int bxper = F.d.xper, bzper = F.d.zper;
   const long planes = (long)g.my * g.mz + (long)g.mx * g.mz + (long)g.mx * g.my;
   const int passes = formbcs ? 3 : 1;
+  const bool outl = formbcs && c->outlet;
   if (formbcs) {
+    if (outl) outlet_fix(c, 0);
     int rc = cv_contra2cart_dev(c);
     if (rc) return rc;
   }
   double *p = formbcs ? c->p : nullptr, *nv = formbcs ? F.nvert : nullptr;
   for (int ps = 0; ps < passes; ps++) {
-    hipLaunchKernelGGL(k_wall_ghost, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, bxper, bzper, formbcs ? 0 : 1, v3(c->ucat));
+    if (outl && ps > 0) {
+      outlet_fix(c, ps == passes - 1);
+      hipLaunchKernelGGL(k_contra2cart_plane, dim3(plane_nb(g)), dim3(BS), 0, c->stream, g, c3(c->ucont), c3(M.csi),
+                         c3(M.eta), c3(M.zet), (const double *)F.nvert, v3(c->ucat), g.mz - 2);
+    }
+    hipLaunchKernelGGL(k_wall_ghost, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, bxper, bzper, formbcs ? 0 : 1, v3(c->ucat),
+                       (const double *)c->ubk);
     if (bxper || bzper)
       hipLaunchKernelGGL(k_periodic_planes, dim3(nblk((long)g.my * g.mz + (long)g.mx * g.my)), dim3(BS), 0, c->stream, g,
                          v3(c->ucat), p, nv);
@@ -975,6 +1176,14 @@ int cv_ghost_update(cv_ctx *c, int formbcs) {
   }
   CV_CHECK(hipGetLastError());
   return CV_OK;
+}
+
+// the boundary update before and after every RK stage: InflowFlux,
+// OutflowFlux (a diagnostic whose sum FormBCS overwrites before any use, so
+// not launched here) and FormBCS (implicitsolver.c:43-46, 79-84)
+int cv_bc_update_dev(cv_ctx *c) {
+  int rc = cv_inflow_dev(c);
+  return rc ? rc : cv_ghost_update(c, 1);
 }
 
 // rows per face-flux block (4 or 8, CV_RC_ROWS; row 0 is halo); 8 is the
@@ -1058,19 +1267,19 @@ int cv_rk_dev(cv_ctx *c) {
   CvLevel &F = c->lev[c->nlev - 1];
   G g = gview(c);
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
-  int rc = cv_ghost_update(c, 1);
+  int rc = cv_bc_update_dev(c);
   if (rc) return rc;
   for (int q = 0; q < 3; q++)
     hipLaunchKernelGGL(k_fill, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, c->rhs[q], 0.0, F.d.n);
+  const int wx = wall_mask(c->bctype[0], c->bctype[1]), wz = wall_mask(c->bctype[4], c->bctype[5]);
   const long planes = (long)g.my * g.mz + (long)g.mx * g.mz + (long)g.mx * g.my;
   for (int s = 0; s < 4; s++) {
     if ((rc = cv_ff1_dev(c))) return rc;
     double a = alfa[s] * c->dt * c->st;
     hipLaunchKernelGGL(k_rk_update, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, a, c3(c->ucont_o), c3(c->rhs),
                        v3(c->ucont), F.d.n);
-    hipLaunchKernelGGL(k_wall_flux_zero, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, c->bctype[0] == 1,
-                       c->bctype[4] == 1, v3(c->ucont));
-    if ((rc = cv_ghost_update(c, 1))) return rc;
+    hipLaunchKernelGGL(k_wall_flux_zero, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, wx, wz, v3(c->ucont));
+    if ((rc = cv_bc_update_dev(c))) return rc;
   }
   CV_CHECK(hipGetLastError());
   return CV_OK;
@@ -1114,7 +1323,7 @@ static void copy3(cv_ctx *c, double *const dst[3], double *const src[3]) {
 int cv_imprk_dev(cv_ctx *c, cv_step_stats *st) {
   CvLevel &F = c->lev[c->nlev - 1];
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
-  int rc = cv_ghost_update(c, 1);   // InflowFlux/OutflowFlux have nothing to do for bctype 1 and 7
+  int rc = cv_bc_update_dev(c);
   if (rc) return rc;
   for (int q = 0; q < 3; q++)
     hipLaunchKernelGGL(k_fill, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, c->rhs[q], 0.0, F.d.n);
@@ -1131,7 +1340,7 @@ int cv_imprk_dev(cv_ctx *c, cv_step_stats *st) {
       const double a = lambda * alfa[s] * c->dt * c->st;
       hipLaunchKernelGGL(k_rk_update, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, a, c3(c->pucont), c3(c->rhs),
                          v3(c->ucont), F.d.n);
-      if ((rc = cv_ghost_update(c, 1))) return rc;
+      if ((rc = cv_bc_update_dev(c))) return rc;
       if ((rc = cv_calcrhs_dev(c))) return rc;
     }
     normdU1 = normdU_bk;
@@ -1181,6 +1390,36 @@ extern "C" int cv_contra2cart(cv_ctx *c) {
   rc = cv_contra2cart_dev(c);
   CV_CHECK(hipStreamSynchronize(c->stream));
   return rc;
+}
+
+extern "C" int cv_inflow_flux(cv_ctx *c, double *flux_in) {
+  int rc = ready(c);
+  if (rc) return rc;
+  if ((rc = cv_inflow_dev(c))) return rc;
+  CV_CHECK(hipMemcpyAsync(c->red_host + 110, c->dsc + CV_DSC_FLUXIN, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  if (flux_in) *flux_in = c->red_host[110];
+  return CV_OK;
+}
+
+extern "C" int cv_outflow_flux(cv_ctx *c, double *flux_out) {
+  int rc = ready(c);
+  if (rc) return rc;
+  double v = 0.0;
+  if (c->outlet) {
+    const G g = gview(c);
+    const unsigned nb = plane_nb(g);
+    hipLaunchKernelGGL(k_outflow_diag, dim3(nb), dim3(BS), 0, c->stream, g, (const double *)c->ucont[2], c->part);
+    hipLaunchKernelGGL(cv_k_final_sum, dim3(1), dim3(BS), 0, c->stream, (const double *)c->part, (int)nb,
+                       c->dsc + CV_DSC_OUTDIAG, 1);
+    CV_CHECK(hipGetLastError());
+    CV_CHECK(hipMemcpyAsync(c->red_host + 112, c->dsc + CV_DSC_OUTDIAG, sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    CV_CHECK(hipStreamSynchronize(c->stream));
+    v = c->red_host[112];
+  }
+  if (flux_out) *flux_out = v;
+  return CV_OK;
 }
 
 extern "C" int cv_form_bcs(cv_ctx *c) {
@@ -1239,8 +1478,12 @@ extern "C" int cv_flow_step(cv_ctx *c, cv_step_stats *st) {
   CV_CHECK(hipEventRecord(e[2], c->stream));
   if ((rc = cv_step_post_dev(c))) return rc;
   CV_CHECK(hipEventRecord(e[3], c->stream));
+  if (c->inlet || c->outlet)
+    CV_CHECK(hipMemcpyAsync(c->red_host + 110, c->dsc + CV_DSC_FLUXIN, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
   if ((rc = cv_divergence_dev(c, &s.maxdiv, &s.maxdiv_idx))) return rc;
   CV_CHECK(hipStreamSynchronize(c->stream));
+  if (c->inlet || c->outlet) { s.flux_in = c->red_host[110]; s.flux_out = c->red_host[111]; }
   float ms[3];
   for (int q = 0; q < 3; q++) CV_CHECK(hipEventElapsedTime(&ms[q], e[q], e[q + 1]));
   s.t_mom = ms[0] * 1e-3; s.t_p = ms[1] * 1e-3; s.t_pr = ms[2] * 1e-3;
@@ -1289,7 +1532,7 @@ struct MCase {
   C3 csi, eta, zet;
   const double *aj;
   double nu, dt, st;
-  int wx, wz;               // bctype[0] == 1, bctype[4] == 1 (wall-normal flux reset)
+  int wx, wz;               // wall masks of the i and k faces (wall-normal flux reset)
   long n;
   int mgx, mgy, mgz;        // k-marching blocks
   int rgx, rgy, rgz, rkc;   // face-flux tiles (8 rows) and their k planes
@@ -1327,7 +1570,7 @@ __global__ void __launch_bounds__(64 * MARCH_TY) kbm_contra2cart(MB_ARGS) {
 
 __global__ void kbm_wall_ghost(MB_ARGS, int full) {
   MB_HEAD
-  wall_ghost_body(C.g, C.g.xper, C.g.zper, full, mv3(C.uca), MB_GSP);
+  wall_ghost_body(C.g, C.g.xper, C.g.zper, full, mv3(C.uca), nullptr, MB_GSP);   // batched cases have no inlet/outlet
 }
 
 __global__ void kbm_periodic_planes(MB_ARGS, int formbcs) {
@@ -1433,8 +1676,8 @@ int cv_mom_batch_create(cv_ctx *const *cases, int n, hipStream_t s, CvMomBatch *
     m.nu = 1. / x->ren;
     m.dt = x->dt;
     m.st = x->st;
-    m.wx = x->bctype[0] == 1;
-    m.wz = x->bctype[4] == 1;
+    m.wx = wall_mask(x->bctype[0], x->bctype[1]);
+    m.wz = wall_mask(x->bctype[4], x->bctype[5]);
     m.n = F.d.n;
     const dim3 mg = march_grid(F.d.mx, F.d.my, F.d.mz);
     m.mgx = (int)mg.x; m.mgy = (int)mg.y; m.mgz = (int)mg.z;

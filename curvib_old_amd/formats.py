@@ -131,7 +131,8 @@ def _is_number(t):
 
 def run_options(opts):
     """Grid-level options with the reference's defaults (init.c:52-62,
-    316-354): Re 3000, dt 0.01, St 1, 3 MG levels, no semi-coarsening."""
+    316-354; main.c:23-24): Re 3000, dt 0.01, St 1, 3 MG levels, no
+    semi-coarsening, inlet profile 2, inlet centre offsets 0."""
     return {
         "ren": float(opts.get("ren", 3000.0)),
         "dt": float(opts.get("dt", 0.01)),
@@ -139,6 +140,8 @@ def run_options(opts):
         "mglevels": int(opts.get("mg_level", 3)),
         "semi": tuple(int(opts.get(f"mg_{a}_semi", 0)) for a in ("i", "j", "k")),
         "cl": float(opts.get("cl", 1.0)),
+        "inlet": int(opts.get("inlet", 2)),
+        "inlet_center": tuple(float(opts.get(f"{a}_c", 0.0)) for a in ("x", "y", "z")),
     }
 
 
@@ -165,7 +168,7 @@ def load_case_dir(directory, device=0, control="control.dat", nvert=None, **ctx_
                                       scale=1.0 / ro["cl"])
     bc = read_bcs_dat(os.path.join(directory, "bcs.dat"))[0]
     u = UserCtx((IM, JM, KM), bc, ro["mglevels"], ro["ren"], ro["dt"], st=ro["st"], semi=ro["semi"],
-                device=device, **ctx_kw)
+                device=device, inlet=ro["inlet"], inlet_center=ro["inlet_center"], **ctx_kw)
     u.set_solver(**solver_options(opts))
     u.set_grid(xyz)
     u.set_nvert(np.zeros(u._shape("nvert", -1)) if nvert is None else nvert)

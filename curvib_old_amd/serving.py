@@ -87,6 +87,9 @@ def grid_desc(case, mglevels, semi=SEMI, coarsen_auto=COARSEN_AUTO):
     d.mglevels = int(mglevels)
     d.isc, d.jsc, d.ksc = (int(v) for v in semi)
     d.ren, d.dt, d.st = float(case.re), float(case.dt), 1.0
+    d.inletprofile = int(case.inlet)
+    for q in range(3):
+        d.inlet_center[q] = float(case.inlet_center[q])
     return d
 
 
@@ -244,7 +247,8 @@ def run_batched(requests, nsteps, precision, warmup=1, device=0, log=None, max_l
     try:
         for case, (xyz, nv, uc) in requests:
             u = UserCtx(case.nodes, case.bctype, levels_for(case.cells, max_levels=max_levels), case.re, case.dt,
-                        semi=SEMI, device=device, coarsen_auto=COARSEN_AUTO)
+                        semi=SEMI, device=device, coarsen_auto=COARSEN_AUTO, inlet=case.inlet,
+                        inlet_center=case.inlet_center)
             u.set_grid(xyz)
             u.set_nvert(nv)
             u.put("ucont", uc)

@@ -30,7 +30,8 @@ def _dp(a):
 
 
 class UserCtx:
-    def __init__(self, nodes, bctype, mglevels, ren, dt, st=1.0, semi=(0, 0, 0), device=0, coarsen_auto=0):
+    def __init__(self, nodes, bctype, mglevels, ren, dt, st=1.0, semi=(0, 0, 0), device=0, coarsen_auto=0,
+                 inlet=2, inlet_center=(0.0, 0.0, 0.0)):
         L = _lib.load()
         d = _lib.GridDesc()
         d.IM, d.JM, d.KM = (int(v) for v in nodes)
@@ -40,6 +41,9 @@ class UserCtx:
         d.isc, d.jsc, d.ksc = (int(v) for v in semi)
         d.ren, d.dt, d.st = float(ren), float(dt), float(st)
         d.coarsen_auto = int(coarsen_auto)
+        d.inletprofile = int(inlet)
+        for q in range(3):
+            d.inlet_center[q] = float(inlet_center[q])
         h = C.c_void_p()
         check(L.cv_ctx_create(C.byref(d), int(device), C.byref(h)), "cv_ctx_create")
         self.h = h
@@ -132,6 +136,18 @@ class UserCtx:
     # -- reference routines ----------------------------------------------
     def Contra2Cart(self):
         check(_lib.load().cv_contra2cart(self.h), "Contra2Cart")
+
+    def InflowFlux(self):
+        """Inlet fluxes and Ubcs of a bctype-5 k- face; returns FluxInSum."""
+        out = C.c_double()
+        check(_lib.load().cv_inflow_flux(self.h, C.byref(out)), "InflowFlux")
+        return out.value
+
+    def OutflowFlux(self):
+        """Flux through a bctype-4 k+ face (diagnostic)."""
+        out = C.c_double()
+        check(_lib.load().cv_outflow_flux(self.h, C.byref(out)), "OutflowFlux")
+        return out.value
 
     def FormBCS(self):
         check(_lib.load().cv_form_bcs(self.h), "FormBCS")
@@ -259,7 +275,7 @@ def from_case(case, mglevels=3, device=0, semi=(0, 0, 0), coarsen_auto=0):
     from . import cases
     xyz, nv, uc = cases.build(case)
     u = UserCtx(case.nodes, case.bctype, mglevels, case.re, case.dt, semi=semi, device=device,
-                coarsen_auto=coarsen_auto)
+                coarsen_auto=coarsen_auto, inlet=case.inlet, inlet_center=case.inlet_center)
     u.set_grid(xyz)
     u.set_nvert(nv)
     u.put("ucont", uc)

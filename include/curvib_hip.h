@@ -46,13 +46,19 @@ typedef struct cv_ctx cv_ctx;
  * MG_Initial (init.c:296-321, 350-358) and FormInitialize (init.c:52-62). */
 typedef struct {
   int IM, JM, KM;          /* grid nodes per axis */
-  int bctype[6];           /* i-, i+, j-, j+, k-, k+ ; supported: 1 wall, 7 periodic (i, k) */
+  int bctype[6];           /* i-, i+, j-, j+, k-, k+ ; supported: 1 wall, 7 periodic (i, k),
+                              5 inlet (k- only), 4 outlet (k+ only) */
   int mglevels;            /* -mg_level */
   int isc, jsc, ksc;       /* -mg_{i,j,k}_semi */
   double ren, dt, st;      /* -ren, -dt, Strouhal number */
   int coarsen_auto;        /* 0: every level coarsens the axes the semi flags allow
                               (MGDACreate); 1: a level coarsens, among those axes,
                               only the ones with the most cells (anisotropic grids) */
+  int inletprofile;        /* -inlet (main.c:23, 887): the analytic inflow profile of a
+                              bctype-5 k- face (bcs.c:158-196), one of 1, -1, 2, 4, 7,
+                              10, 11; read only when bctype[4] == 5 */
+  double inlet_center[3];  /* -x_c -y_c -z_c (main.c:24, 924-926): subtracted from the
+                              inlet face centre before the profile is evaluated */
 } cv_grid_desc;
 
 /* Poisson solver options (control.dat~:69-87 prefix ps_) */
@@ -81,6 +87,9 @@ typedef struct {
   int imp_its;              /* ImpRK: pseudo-time iterations taken ("!!norm of dU", implicitsolver.c:7190) */
   double imp_lambda;        /* ImpRK: final pseudo-time step factor */
   double imp_normdU, imp_reldU;  /* ImpRK: last ||dU||_inf and its ratio to the first */
+  double flux_in, flux_out; /* FluxInSum of InflowFlux and the corrected outlet flux
+                               FluxOutSum of FormBCS ("FluxOutRK", implicitsolver.c:82);
+                               0 without an inlet / outlet */
 } cv_step_stats;
 
 /* momentum scheme of the fractional step (-imp, main.c; Flow_Solver
@@ -132,6 +141,10 @@ int cv_put_field(cv_ctx *ctx, int field, int level, const double *in);
 
 /* ---- hot path: one entry point per reference routine ---- */
 int cv_contra2cart(cv_ctx *ctx);                         /* Contra2Cart        rhs.c:25-144      */
+int cv_inflow_flux(cv_ctx *ctx, double *flux_in);        /* InflowFlux         bcs.c:99-246: inlet Ucont
+                                                            and Ubcs of a bctype-5 k- face; FluxInSum out */
+int cv_outflow_flux(cv_ctx *ctx, double *flux_out);      /* OutflowFlux        bcs.c:403-469: flux through
+                                                            a bctype-4 k+ face (diagnostic) */
 int cv_form_bcs(cv_ctx *ctx);                            /* FormBCS            bcs.c:4436-5888   */
 int cv_form_function1(cv_ctx *ctx, double *rhs_out);     /* FormFunction1      rhs.c:2567-3351   */
 int cv_rungekutta(cv_ctx *ctx);                          /* RungeKutta         implicitsolver.c:32-119 */

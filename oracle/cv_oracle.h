@@ -68,6 +68,11 @@ typedef struct {
   /* momentum scheme: 0 RungeKutta (implicitsolver.c:32), 1 ImpRK (7094);
    * ImpRK options -imp_MAX_IT -imp_atol -imp_rtol -cfl (main.c:21-26, init.c:54) */
   int mom_scheme, imp_max_it; double imp_atol, imp_rtol, cfl;
+  /* inlet on the k- face (bctype 5): -inlet profile and the -x_c -y_c -z_c
+   * offsets (main.c:23-24, 887, 924-926); FluxInSum / FluxOutSum
+   * (main.c:9) as InflowFlux, FormBCS and OutflowFlux leave them */
+  int inletprofile; double cm_c[3];
+  double flux_in_sum, flux_out_sum, flux_out_diag;
   int imp_its; double imp_lambda, imp_normdU, imp_reldU;
   ofield b;           /* Poisson right-hand side */
   /* diagnostics */
@@ -97,6 +102,11 @@ void  o_g2l(ofield *f, const int per[3]);             /* DMGlobalToLocal ghost f
 
 /* ---- hot-path routines ---- */
 void o_contra2cart(octx *c);
+void o_set_inlet(octx *c, int profile, double x_c, double y_c, double z_c);
+double o_inlet_uin(int profile, double xc, double yc, double zc);
+int  o_inlet_profile_ok(int profile);
+void o_inflowflux(octx *c);
+void o_outflowflux(octx *c);
 void o_formbcs(octx *c);
 void o_formfunction1(octx *c, ofield *rhs);
 void o_rungekutta(octx *c);

@@ -79,6 +79,10 @@ void o_set_momentum(octx *c, int scheme, int imp_max_it, double imp_atol, double
 
 void o_set_steps_done(octx *c, int n) { c->steps_done = n; }
 
+void o_flux_stats(octx *c, double *out) {
+  out[0] = c->flux_in_sum; out[1] = c->flux_out_sum; out[2] = c->flux_out_diag;
+}
+
 void o_imp_stats(octx *c, double *out) {
   out[0] = c->imp_its; out[1] = c->imp_lambda; out[2] = c->imp_normdU; out[3] = c->imp_reldU; out[4] = c->steps_done;
 }

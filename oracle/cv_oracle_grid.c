@@ -10,7 +10,8 @@
  *   MyNvertRestriction brute-force restatement in numpy on small masks.
  * Only single-rank semantics are restated (xs=0, xe=mx, ...).
  * Supported boundary types: 1 (no-slip wall) on every axis, 7 (periodic)
- * on the i and k axes.  The cgrid option is taken as 0 (the reference
+ * on the i and k axes, 5 (inlet) on the k- face and 4 (outlet) on the k+
+ * face; metrics and the Poisson operator only tell 7 from the others.  The cgrid option is taken as 0 (the reference
  * leaves it uninitialised unless -cgrid is given, init.c:356-358).
  */
 #include "cv_oracle_int.h"
@@ -110,7 +111,10 @@ octx *o_create(int IM, int JM, int KM, const int bctype[6], int nlev,
                int isc, int jsc, int ksc, double ren, double dt, int coarsen_auto) {
   if (nlev < 1 || nlev > O_MAXLEV) return NULL;
   if (bctype[2] == 7 || bctype[3] == 7) return NULL;  /* j-periodic not restated */
-  for (int q = 0; q < 6; q++) if (bctype[q] != 1 && bctype[q] != 7) return NULL;
+  /* i and j faces: wall 1 or periodic 7; k- also inlet 5, k+ also outlet 4 */
+  for (int q = 0; q < 4; q++) if (bctype[q] != 1 && bctype[q] != 7) return NULL;
+  if (bctype[4] != 1 && bctype[4] != 5 && bctype[4] != 7) return NULL;
+  if (bctype[5] != 1 && bctype[5] != 4 && bctype[5] != 7) return NULL;
   if ((bctype[0] == 7) != (bctype[1] == 7) || (bctype[4] == 7) != (bctype[5] == 7)) return NULL;
   octx *c = (octx *)calloc(1, sizeof(octx));
   c->nlev = nlev;
@@ -147,6 +151,7 @@ octx *o_create(int IM, int JM, int KM, const int bctype[6], int nlev,
   ofield *v3[] = {&c->ucont, &c->ucont_o, &c->ucat, &c->rhs, &c->ubcs, &c->ucont_rm1, &c->pucont};
   for (size_t q = 0; q < 7; q++) of_alloc(v3[q], F->mx, F->my, F->mz, 3);
   c->mom_scheme = 0; c->imp_max_it = 50; c->imp_atol = 1e-7; c->imp_rtol = 1e-4; c->cfl = 0.2;
+  c->inletprofile = 2;                            /* -inlet default (main.c:23) */
   of_alloc(&c->p, F->mx, F->my, F->mz, 1);
   of_alloc(&c->phi, F->mx, F->my, F->mz, 1);
   of_alloc(&c->nvert_o, F->mx, F->my, F->mz, 1);

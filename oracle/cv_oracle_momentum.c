@@ -6,17 +6,23 @@
  *   convection          Convection (QUICK)   rhs.c:1463-1945
  *   viscous             Viscous (laminar)    rhs.c:1947-2565
  *   o_formfunction1     FormFunction1        rhs.c:2567-3351
- *   o_formbcs           FormBCS, bctype {1,7} subset   bcs.c:4436-5888
+ *   o_inflowflux        InflowFlux, analytic profiles  bcs.c:99-246
+ *   o_outflowflux       OutflowFlux          bcs.c:403-469
+ *   o_formbcs           FormBCS, bctype {1,4,5,7} subset   bcs.c:4436-5888
  *   o_ghostnodevelocity GhostNodeVelocity    bcs.c:3645-4435 (subset)
  *   o_rungekutta        RungeKutta           implicitsolver.c:32-119
  *   o_flow_step         Flow_Solver          solvers.c:944-1156 (+ the level
  *                       rotation Ucont_o <- Ucont of main.c:1123-1145)
  *
- * Parity pins (tests/test_oracle_momentum.py): free-stream preservation on a
+ * Parity pins (tests/test_oracle_pins.py): free-stream preservation on a
  * warped grid (RHS equals the GCL error), Poiseuille flow on a uniform grid
  * (viscous term exact for a quadratic profile, convection zero), QUICK
  * exactness for linear fields, Contra2Cart round trip, RK update linearity
- * in a constant RHS.  The stale dpdc/dpde/dpdz carry of FormFunction1
+ * in a constant RHS.  Boundaries: the inlet profiles against their closed
+ * forms (the duct series solves u_xx + u_yy = -2 with u = 0 on the duct
+ * walls), the inlet flux = uin * face area on a uniform grid, the outlet
+ * correction's mass balance (fluid outflow = FluxInSum) and a uniform
+ * stream through an inlet/outlet box left unchanged by FormBCS.  The stale dpdc/dpde/dpdz carry of FormFunction1
  * (rhs.c:2803-2861) is reproduced in loop order; its first read is of an
  * uninitialised variable in the reference and is 0 here.
  *
@@ -223,26 +229,145 @@ static void periodic_corners(octx *c, int with_p, int x_only_upper) {
   }
 }
 
-/* zero the wall-normal flux of bctype-1 faces (deviation, see header) */
+/* zero the wall-normal flux of bctype-1 faces, each face on its own
+ * (deviation, see header) */
 static void wall_flux_zero(octx *c) {
   olevel *L = &c->lev[c->nlev - 1];
   const int mx = L->mx, my = L->my, mz = L->mz;
   ofield *u = &c->ucont;
-  if (L->bctype[0] == 1)
-    for (int k = 0; k < mz; k++) for (int j = 0; j < my; j++) { V(u,k,j,0).x = 0.; V(u,k,j,mx-2).x = 0.; }
-  if (L->bctype[2] == 1)
-    for (int k = 0; k < mz; k++) for (int i = 0; i < mx; i++) { V(u,k,0,i).y = 0.; V(u,k,my-2,i).y = 0.; }
-  if (L->bctype[4] == 1)
-    for (int j = 0; j < my; j++) for (int i = 0; i < mx; i++) { V(u,0,j,i).z = 0.; V(u,mz-2,j,i).z = 0.; }
+  for (int k = 0; k < mz; k++) for (int j = 0; j < my; j++) {
+    if (L->bctype[0] == 1) V(u,k,j,0).x = 0.;
+    if (L->bctype[1] == 1) V(u,k,j,mx-2).x = 0.;
+  }
+  for (int k = 0; k < mz; k++) for (int i = 0; i < mx; i++) {
+    if (L->bctype[2] == 1) V(u,k,0,i).y = 0.;
+    if (L->bctype[3] == 1) V(u,k,my-2,i).y = 0.;
+  }
+  for (int j = 0; j < my; j++) for (int i = 0; i < mx; i++) {
+    if (L->bctype[4] == 1) V(u,0,j,i).z = 0.;
+    if (L->bctype[5] == 1) V(u,mz-2,j,i).z = 0.;
+  }
   o_g2l(u, L->per);
 }
 
-/* FormBCS for bctype {1,7} (bcs.c:4436-5888): three passes of
- * Contra2Cart -> wall ghosts -> periodic planes -> edge averages ->
- * periodic full planes. */
+/* ------------------------------------------------------------------ */
+/* InflowFlux (bcs.c:99-246) for an inlet on the k- face (bctype 5)      */
+/* ------------------------------------------------------------------ */
+void o_set_inlet(octx *c, int profile, double x_c, double y_c, double z_c) {
+  c->inletprofile = profile; c->cm_c[0] = x_c; c->cm_c[1] = y_c; c->cm_c[2] = z_c;
+}
+
+/* the analytic inlet profiles (bcs.c:158-196); 0, 3, 5, 6 and 8 read an
+ * inlet file (InletInterpolation) and are not restated */
+int o_inlet_profile_ok(int profile) {
+  return profile == 1 || profile == -1 || profile == 2 || profile == 4 || profile == 7 || profile == 10 ||
+         profile == 11;
+}
+
+double o_inlet_uin(int profile, double xc, double yc, double zc) {
+  const double H = 4.1, Umax = 1.5;
+  const double r = sqrt(zc * zc + yc * yc);
+  double uin = 0.;
+  if (profile == 1) uin = 2.;
+  else if (profile == -1) uin = -1.;
+  else if (profile == 2) uin = 4. * Umax * yc * (H - yc) / (H * H);
+  else if (profile == 4) {           /* fully developed pipe flow */
+    uin = 2. * (1. - 4. * r * r);
+    if (r > 0.5) uin = 0.;
+  } else if (profile == 7) uin = 1.5 * (1. - 4. * yc * yc);
+  else if (profile == 10) {          /* fully developed flow in a 2 x 2 duct, 20 terms */
+    const double pi = 3.14159265358979323846264338327950288;
+    const double y2 = (yc - 1.5) * 2., x2 = xc - 0.5, A = 1.;
+    uin = 1 - y2 * y2;
+    for (int n = 0; n < 20; n++)
+      uin -= 4. * pow(2. / pi, 3) * pow(-1., n) / pow(2 * n + 1., 3.) * cosh((2 * n + 1) * pi * x2 / 2.) *
+             cos((2. * n + 1) * pi * y2 / 2.) / cosh((2 * n + 1) * pi * A / 2.);
+  } else if (profile == 11) uin = 0.185;
+  return uin;
+}
+
+/* FluxInSum is a global that only the inlet branch writes (main.c:9) */
+void o_inflowflux(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  if (L->bctype[4] != 5) return;
+  const int mx = L->mx, my = L->my;
+  const ofield *co = &L->coor, *zet = &L->zet, *nv = &L->nvert;
+  ofield *uc = &c->ucont, *ub = &c->ubcs;
+  const int k = 0;
+  for (int j = 1; j < my - 1; j++)
+    for (int i = 1; i < mx - 1; i++) {
+      const double xc = (V(co,k,j,i).x + V(co,k,j-1,i).x + V(co,k,j,i-1).x + V(co,k,j-1,i-1).x) * 0.25 - c->cm_c[0];
+      const double yc = (V(co,k,j,i).y + V(co,k,j-1,i).y + V(co,k,j,i-1).y + V(co,k,j-1,i-1).y) * 0.25 - c->cm_c[1];
+      const double zc = (V(co,k,j,i).z + V(co,k,j-1,i).z + V(co,k,j,i-1).z + V(co,k,j-1,i-1).z) * 0.25 - c->cm_c[2];
+      const double uin = o_inlet_uin(c->inletprofile, xc, yc, zc);
+      if (S(nv,k+1,j,i) < 0.1) {
+        const ocmp z = V(zet,k,j,i);
+        const double a = sqrt(z.z * z.z + z.y * z.y + z.x * z.x);
+        V(ub,k,j,i).y = uin * z.y / a;
+        V(ub,k,j,i).z = uin * z.z / a;
+        V(ub,k,j,i).x = uin * z.x / a;
+        V(uc,k,j,i).z = uin * a;
+      }
+    }
+  double flux = 0.;
+  for (int j = 1; j < my - 1; j++)
+    for (int i = 1; i < mx - 1; i++)
+      if (S(nv,k+1,j,i) < 0.1) flux += V(uc,k,j,i).z;
+  c->flux_in_sum = flux;
+  o_g2l(uc, L->per);
+}
+
+/* OutflowFlux (bcs.c:403-469): the flux through the k+ outlet face,
+ * fluid or not.  FormBCS overwrites the FluxOutSum it leaves before any
+ * use, so it is a diagnostic only. */
+void o_outflowflux(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  if (L->bctype[5] != 4) return;
+  const int mx = L->mx, my = L->my, k = L->mz - 2;
+  double flux = 0.;
+  for (int j = 1; j < my - 1; j++)
+    for (int i = 1; i < mx - 1; i++) flux += V(&c->ucont,k,j,i).z;
+  c->flux_out_diag = flux;
+}
+
+/* FormBCS's outlet section for bctype 4 on the k+ face (bcs.c:4914-5010):
+ * Ubcs = Ucat of the last cell, and the outlet flux is that velocity's flux
+ * plus a uniform correction that makes the fluid outflow equal FluxInSum */
+static void outlet_kplus(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  if (L->bctype[5] != 4) return;
+  const int mx = L->mx, my = L->my, k = L->mz - 1;
+  const ofield *zet = &L->zet, *nv = &L->nvert, *ucat = &c->ucat;
+  ofield *uc = &c->ucont, *ub = &c->ubcs;
+  double fout = 0., area = 0.;
+  for (int j = 1; j < my - 1; j++)
+    for (int i = 1; i < mx - 1; i++)
+      if (S(nv,k-1,j,i) < 0.1) {
+        const ocmp u = V(ucat,k-1,j,i), z = V(zet,k-1,j,i);
+        fout += u.x * z.x + u.y * z.y + u.z * z.z;
+        area += sqrt(z.x * z.x + z.y * z.y + z.z * z.z);
+      }
+  /* FluxIn = FluxInSum + FarFluxInSum + FluxIntpSum, the last two 0 here */
+  const double ratio = (c->flux_in_sum - fout) / area;
+  double fsum = 0.;
+  for (int j = 1; j < my - 1; j++)
+    for (int i = 1; i < mx - 1; i++)
+      if (S(nv,k-1,j,i) < 0.1) {
+        V(ub,k,j,i) = V(ucat,k-1,j,i);
+        const ocmp b = V(ub,k,j,i), z = V(zet,k-1,j,i);
+        V(uc,k-1,j,i).z = (b.x * z.x + b.y * z.y + b.z * z.z) + ratio * sqrt(z.x * z.x + z.y * z.y + z.z * z.z);
+        fsum += V(uc,k-1,j,i).z;
+      }
+  c->flux_out_sum = fsum;
+}
+
+/* FormBCS for bctype {1,4,5,7} (bcs.c:4436-5888): three passes of
+ * outlet correction -> Contra2Cart -> wall ghosts -> periodic planes ->
+ * edge averages -> periodic full planes. */
 void o_formbcs(octx *c) {
   olevel *L = &c->lev[c->nlev - 1];
   for (int ttemp = 0; ttemp < 3; ttemp++) {
+    outlet_kplus(c);
     o_g2l(&c->ucont, L->per);
     o_contra2cart(c);
     wall_ghosts(c, 0);
@@ -253,7 +378,8 @@ void o_formbcs(octx *c) {
   o_g2l(&c->p, L->per); o_g2l(&c->ucat, L->per); o_g2l(&c->ucont, L->per);
 }
 
-/* GhostNodeVelocity (bcs.c:3645-4435) for bctype {1,7} */
+/* GhostNodeVelocity (bcs.c:3645-4435) for bctype {1,4,5,7}: the ghost
+ * values follow the Ubcs the last InflowFlux / FormBCS left */
 void o_ghostnodevelocity(octx *c) {
   wall_ghosts(c, 1);
   periodic_planes(c, 0);
@@ -610,10 +736,19 @@ void o_formfunction1(octx *c, ofield *rhs) {
 /* ------------------------------------------------------------------ */
 /* RungeKutta (implicitsolver.c:32-119)                                 */
 /* ------------------------------------------------------------------ */
+/* InflowFlux, OutflowFlux, FormBCS: the boundary update the reference
+ * runs before and after every RK stage (implicitsolver.c:43-46, 79-84,
+ * 7125-7127, 7161-7164) */
+static void bc_update(octx *c) {
+  o_inflowflux(c);
+  o_outflowflux(c);
+  o_formbcs(c);
+}
+
 void o_rungekutta(octx *c) {
   olevel *L = &c->lev[c->nlev - 1];
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
-  o_formbcs(c);
+  bc_update(c);
   of_zero(&c->rhs);
   const int mx = L->mx, my = L->my, mz = L->mz;
   for (int istage = 0; istage < 4; istage++) {
@@ -627,7 +762,7 @@ void o_rungekutta(octx *c) {
         }
     wall_flux_zero(c);
     o_g2l(&c->ucont, L->per);
-    o_formbcs(c);
+    bc_update(c);
   }
 }
 
@@ -738,7 +873,7 @@ static double absmax3(const octx *c, const ofield *a, const ofield *b) {
 void o_imprk(octx *c) {
   olevel *L = &c->lev[c->nlev - 1];
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
-  o_formbcs(c);                     /* InflowFlux/OutflowFlux: nothing for bctype 1 and 7 */
+  bc_update(c);
   of_zero(&c->rhs);
   of_copy(&c->pucont, &c->ucont);
   o_calcrhs(c);
@@ -759,7 +894,7 @@ void o_imprk(octx *c) {
             u->x = a * r->x + pu->x; u->y = a * r->y + pu->y; u->z = a * r->z + pu->z;
           }
       o_g2l(&c->ucont, L->per);
-      o_formbcs(c);
+      bc_update(c);
       o_calcrhs(c);
     }
     normdU1 = normdU_bk;

@@ -60,8 +60,14 @@ def lib():
         L.o_set_momentum.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
         L.o_imp_stats.argtypes = [vp, dp]
         L.o_set_steps_done.argtypes = [vp, C.c_int]
-        for nm in ("o_calcrhs", "o_imprk"):
+        for nm in ("o_calcrhs", "o_imprk", "o_inflowflux", "o_outflowflux"):
             getattr(L, nm).argtypes = [vp]
+        L.o_set_inlet.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.o_inlet_uin.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.o_inlet_uin.restype = C.c_double
+        L.o_inlet_profile_ok.argtypes = [C.c_int]
+        L.o_inlet_profile_ok.restype = C.c_int
+        L.o_flux_stats.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -156,6 +162,16 @@ class Oracle:
         lib().o_imp_stats(self.h, _dp(out))
         return dict(zip(("its", "lambda", "normdU", "reldU", "steps_done"), out.tolist()))
 
+    def set_inlet(self, profile, center=(0.0, 0.0, 0.0)):
+        """-inlet profile and the -x_c -y_c -z_c offsets of a bctype-5 inlet."""
+        lib().o_set_inlet(self.h, int(profile), float(center[0]), float(center[1]), float(center[2]))
+
+    def flux_stats(self):
+        """FluxInSum, FormBCS's FluxOutSum and OutflowFlux's sum, as last left."""
+        out = np.zeros(3)
+        lib().o_flux_stats(self.h, _dp(out))
+        return dict(zip(("flux_in", "flux_out", "flux_out_diag"), out.tolist()))
+
     def csr(self, level):
         mx, my, mz = self.dims(level)[:3]
         n = mx * my * mz
@@ -207,7 +223,13 @@ def from_case(case, nlev=3, semi=(0, 0, 0), coarsen_auto=0):
     from curvib_old_amd import cases
     xyz, nv, uc = cases.build(case)
     o = Oracle(case.nodes, case.bctype, nlev, case.re, case.dt, semi, coarsen_auto)
+    o.set_inlet(case.inlet, case.inlet_center)
     o.set_grid(xyz)
     o.set_nvert(nv)
     o.put("ucont", uc)
     return o, (xyz, nv, uc)
+
+
+def inlet_uin(profile, xc, yc, zc):
+    """The oracle's inlet velocity of one analytic profile at a face centre."""
+    return lib().o_inlet_uin(int(profile), float(xc), float(yc), float(zc))

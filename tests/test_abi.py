@@ -85,3 +85,63 @@ def test_levels_for_semi_coarsening():
     h = level_cells((512, 256, 256), (0, 1, 0), 8, 1)
     assert h[-1] == (512, 256, 256) and h[-2] == (256, 256, 256) and h[0] == (4, 256, 4)
     assert serving.levels_for((512, 256, 256), coarsen_auto=1) == 8
+
+
+def _c_layout(tmp_path):
+    """sizeof / offsetof of the ABI structs as a C compiler lays them out."""
+    src = tmp_path / "layout.c"
+    fields = {
+        "cv_grid_desc": ["IM", "bctype", "mglevels", "ren", "coarsen_auto", "inletprofile", "inlet_center"],
+        "cv_step_stats": ["ksp_its", "maxdiv_idx", "t_mom", "imp_its", "imp_reldU", "flux_in", "flux_out"],
+        "cv_solver_opts": ["precision", "rtol", "smoother", "omega"],
+        "cv_momentum_opts": ["scheme", "imp_atol", "cfl"],
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for st, fs in fields.items():
+        lines.append(f'  printf("{st} size %zu\\n", sizeof({st}));')
+        for f in fs:
+            lines.append(f'  printf("{st} {f} %zu\\n", offsetof({st}, {f}));')
+    lines.append("  return 0;\n}")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    return fields, {tuple(l.split()[:2]): int(l.split()[2]) for l in out.splitlines()}
+
+
+def test_binding_structs_match_the_c_layout(tmp_path):
+    """The ctypes mirrors of the ABI structs have the C compiler's sizes and
+    field offsets."""
+    from curvib_old_amd import _lib
+    fields, lay = _c_layout(tmp_path)
+    py = {"cv_grid_desc": _lib.GridDesc, "cv_step_stats": _lib.StepStats, "cv_solver_opts": _lib.SolverOpts,
+          "cv_momentum_opts": _lib.MomentumOpts}
+    for st, fs in fields.items():
+        cls = py[st]
+        assert ctypes.sizeof(cls) == lay[(st, "size")], st
+        for f in fs:
+            assert getattr(cls, f).offset == lay[(st, f)], (st, f)
+
+
+@pytest.mark.skipif(not os.path.exists(SO), reason="library not built")
+def test_inlet_outlet_descriptions_without_gpu():
+    """Inlet (k-, bctype 5) and outlet (k+, bctype 4) are accepted with an
+    analytic inlet profile; file-based profiles and the faces the reference
+    path does not handle here are rejected before any device call."""
+    from curvib_old_amd import _lib, cases, serving
+    la = 3
+    ok = cases.ChannelCase(cells=(16, 16, 16), bctype=(7, 7, 1, 1, 5, 4), inlet=7)
+    walls = cases.ChannelCase(cells=(16, 16, 16), bctype=(7, 7, 1, 1, 1, 1))
+    # Ubcs of the two k planes: 6 * mx * my doubles
+    assert serving.ctx_bytes(ok, la, 30) - serving.ctx_bytes(walls, la, 30) == 6 * 18 * 18 * 8
+    for bad in (dict(inlet=0), dict(inlet=3), dict(bctype=(7, 7, 1, 1, 4, 5)), dict(bctype=(5, 4, 1, 1, 1, 1)),
+                dict(bctype=(7, 7, 1, 1, 5, 7))):
+        kw = dict(cells=(16, 16, 16), bctype=(7, 7, 1, 1, 5, 4), inlet=7)
+        kw.update(bad)
+        assert serving.ctx_bytes(cases.ChannelCase(**kw), la, 30) == -1, bad
+    L = _lib.load()
+    d = serving.grid_desc(cases.ChannelCase(cells=(16, 16, 16), bctype=(7, 7, 1, 1, 5, 4), inlet=0), 3)
+    out = ctypes.c_void_p()
+    assert L.cv_ctx_create(ctypes.byref(d), 0, ctypes.byref(out)) == _lib.CV_ERR_ARG
+    assert "inlet profile" in _lib.last_error()

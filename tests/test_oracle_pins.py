@@ -18,6 +18,14 @@ ships no tests, golden vectors or fixtures for this path):
                 BDF2 coefficients, the boundary zero set
   ImpRK         the pseudo-time iteration meets its stopping test and
                 reduces its own residual
+  inlet/outlet  profile closed forms (zeros, maxima; the duct series solves
+                u_xx + u_yy = -2 with u = 0 on the walls), inlet flux =
+                uin * face area on a uniform grid, a uniform stream is a
+                fixed point of InflowFlux + FormBCS, the outlet correction
+                balances the fluid outflow with FluxInSum, and a full step
+                through an inlet/outlet box projects to a residual-scale
+                divergence (an unbalanced outlet leaves the Neumann
+                problem incompatible)
 
 Each check is chosen so that a dropped term, a wrong sign or index or a
 transposed operand fails it."""
@@ -41,6 +49,7 @@ def uniform_case(cells, **kw):
 def oracle_for(case, nlev=1, xyz=None, semi=(0, 0, 0)):
     xyz0, nv, uc = cases.build(case)
     o = po.Oracle(case.nodes, case.bctype, nlev, case.re, case.dt, semi)
+    o.set_inlet(case.inlet, case.inlet_center)
     o.set_grid(xyz0 if xyz is None else xyz)
     o.set_nvert(nv)
     o.put("ucont", uc)
@@ -326,3 +335,112 @@ def test_imprk_reduces_its_residual():
     assert st["its"] < 200
     assert st["normdU"] <= 1e-9 or st["reldU"] <= 1e-6
     assert np.abs(o.get("rhs")).max() < 1e-2 * r0
+
+
+INOUT = (7, 7, 1, 1, 5, 4)
+
+
+def test_inlet_profiles_closed_forms():
+    u = po.inlet_uin
+    assert u(1, 0.3, 0.2, 0.1) == 2.0 and u(-1, 0.3, 0.2, 0.1) == -1.0 and u(11, 0.3, 0.2, 0.1) == 0.185
+    # 2: channel parabola of height H = 4.1, peak Umax = 1.5 at mid height
+    assert abs(u(2, 0.0, 0.0, 0.0)) < 1e-15 and abs(u(2, 0.0, 4.1, 0.0)) < 1e-15
+    assert u(2, 0.0, 2.05, 0.0) == pytest.approx(1.5, rel=1e-15)
+    # 7: channel parabola over y in [-1/2, 1/2], peak 1.5
+    assert abs(u(7, 0.0, 0.5, 0.0)) < 1e-15 and abs(u(7, 0.0, -0.5, 0.0)) < 1e-15
+    assert u(7, 0.9, 0.0, 0.3) == 1.5
+    # 4: pipe of radius 1/2 in (y, z), peak 2 on the axis, zero on and outside the wall
+    assert u(4, 0.7, 0.0, 0.0) == 2.0
+    assert abs(u(4, 0.7, 0.3, 0.4)) < 1e-15 and u(4, 0.7, 0.5, 0.4) == 0.0
+    # 10: square duct xc in [-1/2, 3/2], yc in [1, 2]; in the duct's own
+    # variables (x', y') = (xc - 1/2, 2 yc - 3) the flow solves
+    # u_x'x' + u_y'y' = -2 with u = 0 on |x'| = 1 and |y'| = 1
+    for xc in (-0.2, 0.5, 1.1):
+        assert abs(u(10, xc, 1.0, 0.0)) < 1e-15 and abs(u(10, xc, 2.0, 0.0)) < 1e-15
+    for yc in (1.2, 1.5, 1.9):
+        assert abs(u(10, -0.5, yc, 0.0)) < 1e-4 and abs(u(10, 1.5, yc, 0.0)) < 1e-4
+    h = 1e-3
+    for xc, yc in ((0.5, 1.5), (0.1, 1.3), (1.2, 1.8)):
+        uxx = (u(10, xc + h, yc, 0) - 2 * u(10, xc, yc, 0) + u(10, xc - h, yc, 0)) / h**2
+        uyy = (u(10, xc, yc + h, 0) - 2 * u(10, xc, yc, 0) + u(10, xc, yc - h, 0)) / h**2
+        assert uxx + uyy / 4.0 == pytest.approx(-2.0, abs=1e-5)
+    # centre value of the square-duct (torsion) problem
+    assert u(10, 0.5, 1.5, 0.0) == pytest.approx(0.5894, abs=2e-4)
+
+
+def test_inflow_flux_uniform_grid():
+    cells = (6, 8, 5)
+    case = uniform_case(cells, bctype=INOUT, inlet=7, inlet_center=(0.0, 0.5, 0.0))
+    o, _ = oracle_for(case)
+    o.call("o_inflowflux")
+    mx, my, mz = o.dims(0)[:3]
+    dx, dy = 1 / cells[0], 1 / cells[1]
+    yc = (np.arange(1, my - 1) - 0.5) * dy - 0.5
+    uin = 1.5 * (1 - 4 * yc**2)
+    uc, ub = o.get("ucont"), o.get("ubcs")
+    np.testing.assert_allclose(uc[0, 1:-1, 1:-1, 2], np.broadcast_to((uin * dx * dy)[:, None], (my - 2, mx - 2)),
+                               rtol=1e-13)
+    np.testing.assert_allclose(ub[0, 1:-1, 1:-1, 2], np.broadcast_to(uin[:, None], (my - 2, mx - 2)), rtol=1e-13)
+    assert np.abs(ub[0, 1:-1, 1:-1, :2]).max() == 0.0
+    assert o.flux_stats()["flux_in"] == pytest.approx(uin.sum() * dx * dy * (mx - 2), rel=1e-13)
+
+
+def test_uniform_stream_is_a_fixed_point_of_the_boundary_update():
+    cells = (6, 8, 7)
+    case = uniform_case(cells, bctype=INOUT, inlet=1)
+    o, _ = oracle_for(case)
+    mx, my, mz = o.dims(0)[:3]
+    dx, dy = 1 / cells[0], 1 / cells[1]
+    uc = np.zeros((mz, my, mx, 3))
+    uc[:-1, 1:-1, 1:-1, 2] = 2.0 * dx * dy
+    o.put("ucont", uc)
+    o.call("o_contra2cart")
+    o.call("o_inflowflux")
+    o.call("o_outflowflux")
+    o.call("o_formbcs")
+    uc1, ucat, fl = o.get("ucont"), o.get("ucat"), o.flux_stats()
+    np.testing.assert_allclose(uc1[:-1, 1:-1, 1:-1, 2], uc[:-1, 1:-1, 1:-1, 2], rtol=1e-14)
+    # every cell of the interior (j) range, ghost k planes included, carries the stream
+    np.testing.assert_allclose(ucat[:, 1:-1, 1:-1, 2], 2.0, rtol=1e-14)
+    assert np.abs(ucat[:, 1:-1, 1:-1, :2]).max() < 1e-14
+    area = dx * dy * (mx - 2) * (my - 2)
+    assert fl["flux_in"] == pytest.approx(2 * area, rel=1e-14)
+    assert fl["flux_out"] == pytest.approx(2 * area, rel=1e-14)
+    assert fl["flux_out_diag"] == pytest.approx(2 * area, rel=1e-14)
+
+
+def test_outlet_correction_balances_the_inflow():
+    case = cases.ChannelCase(cells=(12, 10, 16), amp=0.1, sphere=(0.5, 0.5, 0.7, 0.4), seed=5, bctype=INOUT,
+                             inlet=7, inlet_center=(0.0, 0.5, 0.0))
+    o, nv = oracle_for(case)
+    o.call("o_contra2cart")
+    o.call("o_inflowflux")
+    o.call("o_formbcs")
+    uc, fl = o.get("ucont"), o.flux_stats()
+    fluid = nv[-2, 1:-1, 1:-1] < 0.1
+    out = uc[-2, 1:-1, 1:-1, 2][fluid].sum()
+    assert fl["flux_in"] > 0.5
+    assert out == pytest.approx(fl["flux_in"], rel=1e-12)
+    assert fl["flux_out"] == pytest.approx(out, rel=1e-14)
+    # the outlet ghost: 2 Ubcs - Ucat with Ubcs from the last pass's entry Ucat
+    ub, ucat = o.get("ubcs"), o.get("ucat")
+    np.testing.assert_allclose(ucat[-1, 1:-1, 1:-1], 2 * ub[-1, 1:-1, 1:-1] - ucat[-2, 1:-1, 1:-1], rtol=0,
+                               atol=1e-14)
+
+
+@pytest.mark.parametrize("scheme", ["rk", "imprk"])
+def test_inlet_outlet_step_projects_to_residual_divergence(scheme):
+    case = cases.ChannelCase(cells=(16, 12, 8), amp=0.1, sphere=(0.5, 0.5, 0.5, 0.4), seed=9, dt=2e-4, re=500.0,
+                             bctype=INOUT, inlet=7, inlet_center=(0.0, 0.5, 0.0))
+    o, _ = oracle_for(case, nlev=2)
+    o.set_solver(rtol=1e-12, max_it=400)
+    if scheme == "imprk":
+        o.set_momentum("imprk", imp_max_it=20, imp_atol=1e-7, imp_rtol=1e-4, cfl=0.2)
+    o.call("o_flow_step")
+    st = o.stats()
+    assert st["its"] > 0
+    assert st["rnorm"] <= 1e-12 * st["bnorm"] * 1.0001
+    assert st["maxdiv"] < 1e-6
+    fl = o.flux_stats()
+    assert fl["flux_out"] == pytest.approx(fl["flux_in"], rel=1e-12)
+
