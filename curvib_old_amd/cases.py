@@ -6,7 +6,9 @@ format (init.c:339-343, 610-720), ``bcs.dat`` (init.c:478-492) and the
 immersed-body classification that ``ibm_search_advanced_rev`` writes into
 ``Nvert`` (ibm.c:1074-1079: 0 fluid, 1.4 immersed-boundary node, 3.4 solid).
 
-Geometry (all cases are a periodic channel, bcs.dat ``7 7 1 1 7 7``):
+Geometry (by default a periodic channel, bcs.dat ``7 7 1 1 7 7``; other
+bcs.dat faces -- slip 3, far field 6, a k- inlet 5 with an analytic -inlet
+profile, a k+ outlet 4 -- keep the same grid and initial flux):
   x, z in [0, Lx] x [0, Lz] uniform; y tanh-stretched with parameter beta,
   y0 = (1 + tanh(beta*(2*eta-1))/tanh(beta))/2, eta = j/(JM-1); the lower
   wall is warped by A*sin(2*pi*x/Lx)*sin(2*pi*z/Lz), blended linearly to zero
@@ -198,9 +200,11 @@ def initial_ucont(case: ChannelCase, xyz: np.ndarray, nvert: np.ndarray) -> np.n
     bc = case.bctype
     if bc[0] == 1:
         uc[:, :, 0, 0] = 0.0
+    if bc[1] == 1:
         uc[:, :, mx - 2, 0] = 0.0
     if bc[2] == 1:
         uc[:, 0, :, 1] = 0.0
+    if bc[3] == 1:
         uc[:, my - 2, :, 1] = 0.0
     if bc[4] == 1:
         uc[0, :, :, 2] = 0.0

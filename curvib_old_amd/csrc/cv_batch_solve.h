@@ -968,13 +968,13 @@ extern "C" int cv_batch_poisson_solve(cv_batch *B, cv_step_stats *st) {
 
 static int b_finish(cv_batch *B, cv_step_stats *st, double tp);
 
-// every case runs explicit RK with kernel timing off and has no inlet or
-// outlet: the whole step around the solve can then run on the batch
-// stream, one launch per kernel
+// every case runs explicit RK with kernel timing off and has only wall and
+// periodic faces: the whole step around the solve can then run on the
+// batch stream, one launch per kernel
 static bool b_mom_batchable(const cv_batch *B) {
   if (!B->use_mom || !B->mom) return false;
   for (const cv_ctx *x : B->cs)
-    if (x->mom.scheme != CV_MOM_RK || x->prof.kind != CV_PROF_NONE || x->inlet || x->outlet) return false;
+    if (x->mom.scheme != CV_MOM_RK || x->prof.kind != CV_PROF_NONE || x->ubf) return false;
   return true;
 }
 

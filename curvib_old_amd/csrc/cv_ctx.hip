@@ -100,11 +100,16 @@ static bool inlet_profile_ok(int p) {
 static const char *desc_problem(const cv_grid_desc *d) {
   if (!d) return "no description";
   if (d->IM < 4 || d->JM < 4 || d->KM < 4) return "fewer than 4 nodes on an axis";
-  for (int q = 0; q < 4; q++)
-    if (d->bctype[q] != 1 && d->bctype[q] != 7) return "i and j faces take bctype 1 or 7";
-  if (d->bctype[4] != 1 && d->bctype[4] != 5 && d->bctype[4] != 7) return "the k- face takes bctype 1, 5 or 7";
-  if (d->bctype[5] != 1 && d->bctype[5] != 4 && d->bctype[5] != 7) return "the k+ face takes bctype 1, 4 or 7";
+  for (int q = 0; q < 6; q++) {
+    const int b = d->bctype[q];
+    if (b == 1 || b == 3 || b == 6 || b == 7) continue;
+    if ((q == 4 && b == 5) || (q == 5 && b == 4)) continue;
+    return "faces take bctype 1, 3, 6 or 7; the k- face also 5, the k+ face also 4";
+  }
   if (d->bctype[2] == 7 || d->bctype[3] == 7) return "j-periodic grids are not supported";
+  // FormMetrics leaves eta of the j = 0 face at 0 (its loop starts at j = 1,
+  // metrics.c:107-110): slip and far field divide by it
+  if (d->bctype[2] == 3 || d->bctype[2] == 6) return "slip and far field are not defined on the j- face";
   if ((d->bctype[0] == 7) != (d->bctype[1] == 7)) return "bctype 7 must pair on the i faces";
   if ((d->bctype[4] == 7) != (d->bctype[5] == 7)) return "bctype 7 must pair on the k faces";
   if (d->bctype[4] == 5 && !inlet_profile_ok(d->inletprofile))
@@ -115,7 +120,12 @@ static const char *desc_problem(const cv_grid_desc *d) {
   return nullptr;
 }
 static int check_desc(const cv_grid_desc *d) { return desc_problem(d) == nullptr; }
-static bool desc_kio(const cv_grid_desc *d) { return d->bctype[4] == 5 || d->bctype[5] == 4; }
+// a face that needs Ubcs storage: inlet, outlet, slip or far field
+static bool desc_ubf(const cv_grid_desc *d) {
+  for (int q = 0; q < 6; q++)
+    if (d->bctype[q] != 1 && d->bctype[q] != 7) return true;
+  return false;
+}
 
 // MGDACreate dimensions (init.c:78-99), level 0 coarsest; each coarsening
 // must keep an even cell count on coarsened axes so that fine cells pair
@@ -166,10 +176,12 @@ static int level_dims(const cv_grid_desc *desc, CvDims *dims) {
 // line factors, line scratch, x b r (18 arrays); the operator word), the finest-level
 // metrics (40 doubles), state (25 doubles: 7 vectors of 3, p phi b w) and
 // the 3n exchange staging, reduction scratch and the Krylov space of
-// FGMRES(restart); with an inlet or outlet the Ubcs of the two k planes
-static size_t plan_bytes(const CvDims *dims, int nlev, int restart, bool kio) {
+// FGMRES(restart); with an inlet, outlet, slip or far-field face the Ubcs
+// of the six boundary planes
+static size_t plan_bytes(const CvDims *dims, int nlev, int restart, bool ubf) {
   size_t b = 0;
-  if (kio) b += (size_t)6 * dims[nlev - 1].mx * dims[nlev - 1].my * 8;
+  const CvDims &F = dims[nlev - 1];
+  if (ubf) b += (size_t)3 * 2 * ((size_t)F.my * F.mz + (size_t)F.mx * F.mz + (size_t)F.mx * F.my) * 8;
   for (int l = 0; l < nlev; l++) b += (size_t)dims[l].n * (22 * 8 + 18 * 4 + 4);
   const size_t n = (size_t)dims[nlev - 1].n;
   b += n * 8 * (40 + 25 + 3);
@@ -184,7 +196,7 @@ extern "C" long cv_ctx_bytes(const cv_grid_desc *desc, int restart) {
   if (!check_desc(desc)) return -1;
   CvDims dims[CV_MAXLEV];
   if (level_dims(desc, dims)) return -1;
-  return (long)plan_bytes(dims, desc->mglevels, restart, desc_kio(desc));
+  return (long)plan_bytes(dims, desc->mglevels, restart, desc_ubf(desc));
 }
 
 extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out) {
@@ -206,7 +218,12 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
   c->outlet = desc->bctype[5] == 4;
   c->inletprofile = desc->inletprofile;
   for (int q = 0; q < 3; q++) c->cm_c[q] = desc->inlet_center[q];
-  c->ubk = nullptr;
+  c->slip = c->far = 0;
+  for (int q = 0; q < 6; q++) {
+    if (desc->bctype[q] == 3) c->slip |= 1 << q;
+    if (desc->bctype[q] == 6) c->far |= 1 << q;
+  }
+  c->ubf = nullptr;
   c->ren = desc->ren; c->dt = desc->dt; c->st = desc->st > 0 ? desc->st : 1.0;
   cv_default_solver(&c->opt);
   cv_default_momentum(&c->mom);
@@ -277,12 +294,12 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     if ((rc = cv_alloc(c, &p, c->red_len * sizeof(double)))) goto fail; c->red = (double *)p;
     if ((rc = cv_alloc(c, &p, 128 * sizeof(double)))) goto fail; c->dsc = (double *)p;
     CV_CHECK(hipMemsetAsync(c->dsc, 0, 128 * sizeof(double), c->stream));   // FluxInSum starts at 0
-    if (c->inlet || c->outlet) {
+    if (desc_ubf(desc)) {
       const CvDims &D = c->lev[c->nlev - 1].d;
-      const size_t nb = (size_t)6 * D.mx * D.my * sizeof(double);
+      const size_t nb = (size_t)6 * ((size_t)D.my * D.mz + (size_t)D.mx * D.mz + (size_t)D.mx * D.my) * sizeof(double);
       if ((rc = cv_alloc(c, &p, nb))) goto fail;
-      c->ubk = (double *)p;
-      CV_CHECK(hipMemsetAsync(c->ubk, 0, nb, c->stream));
+      c->ubf = (double *)p;
+      CV_CHECK(hipMemsetAsync(c->ubf, 0, nb, c->stream));
     }
     if ((rc = cv_alloc(c, &p, 34 * 1024 * sizeof(double)))) goto fail; c->part = (double *)p;
     c->krylov_m = 0;

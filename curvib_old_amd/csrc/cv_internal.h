@@ -102,11 +102,13 @@ struct cv_ctx {
   std::vector<void *> allocs;
   int geometry_set, nvert_set;
   CvProf prof;
-  // inlet (bctype[4] == 5) and outlet (bctype[5] == 4) of the k faces
+  // boundary faces other than walls and periodic pairs: inlet (bctype[4]
+  // == 5), outlet (bctype[5] == 4), slip (3) and far field (6) faces
   int inlet, outlet;
+  int slip, far;              // face masks, bit f for face f (0 i-, 1 i+, 2 j-, 3 j+, 4 k-, 5 k+)
   int inletprofile;           // -inlet
   double cm_c[3];             // -x_c -y_c -z_c
-  double *ubk;                // Ubcs of the k = 0 and k = mz-1 planes, [plane][x|y|z][i + mx*j]
+  double *ubf;                // Ubcs of the six faces (null with walls and periodic faces only)
 };
 
 // device scalars of the inlet/outlet update (slots of cv_ctx::dsc)
@@ -114,6 +116,7 @@ struct cv_ctx {
 #define CV_DSC_FLUXOUT 117    // FluxOutSum after FormBCS's outlet correction
 #define CV_DSC_OUTSUM  118    // outlet velocity flux and area over the last fluid cells (2)
 #define CV_DSC_OUTDIAG 121    // OutflowFlux's sum
+#define CV_DSC_FAR     122    // far-field inflow, outflow, |flux|, inflow area, outflow area (5)
 
 // Records device events around one launch when (kind, level, precision)
 // matches the selection; prec 0 = FP64, 1 = FP32.  bytes: algorithmic bytes

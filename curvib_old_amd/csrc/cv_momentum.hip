@@ -4,7 +4,8 @@
 //  Contra2Cart        rhs.c:25-144
 //  InflowFlux         bcs.c:99-246     (bctype 5 on the k- face, analytic profiles)
 //  OutflowFlux        bcs.c:403-469    (bctype 4 on the k+ face)
-//  FormBCS            bcs.c:4436-5888  (bctype 1 wall, 7 periodic, 5 inlet, 4 outlet)
+//  FormBCS            bcs.c:4436-5888  (bctype 1 wall, 3 slip, 6 far field, 7 periodic,
+//                                       5 inlet, 4 outlet)
 //  GhostNodeVelocity  bcs.c:3645-4435  (same subset)
 //  FormFunction1      rhs.c:2567-3351 with Convection (QUICK, rhs.c:1463-1945)
 //                     and Viscous (laminar, rhs.c:1947-2565)
@@ -124,11 +125,20 @@ __global__ void k_contra2cart_plane(G g, C3 uc, C3 csi, C3 eta, C3 zet, const do
 // ------------------------------------------------------------------
 // ghost planes (FormBCS bcs.c:5252-5731, GhostNodeVelocity bcs.c:4015-4415)
 // ------------------------------------------------------------------
-// wall ghosts: ucat_g = 2*ubcs - ucat_inner.  Ubcs is 0 on the i and j
-// faces; on the k faces it is ubk (the inlet / outlet planes, 0 elsewhere)
-// or 0 when ubk is null.  full: whole tangential ranges (GhostNodeVelocity);
-// otherwise interior ranges (FormBCS)
-__device__ __forceinline__ void wall_ghost_body(G g, int bx, int bz, int full, V3 u, const double *ubk, GS gs) {
+// Ubcs of the six boundary planes in one array: face f (0 i-, 1 i+, 2 j-,
+// 3 j+, 4 k-, 5 k+), component q (x, y, z), at the plane index r of the
+// face's ghost point (i faces j + my*k, j faces i + mx*k, k faces i + mx*j)
+__device__ __forceinline__ long ubf_at(const G &g, int f, int q, long r) {
+  const long nyz = (long)g.my * g.mz, nxz = (long)g.mx * g.mz, nxy = (long)g.mx * g.my;
+  const long sz = f < 2 ? nyz : (f < 4 ? nxz : nxy);
+  const long base = f < 2 ? 3 * f * nyz : (f < 4 ? 6 * nyz + 3 * (f - 2) * nxz : 6 * (nyz + nxz) + 3 * (f - 4) * nxy);
+  return base + q * sz + r;
+}
+
+// wall ghosts: ucat_g = 2*ubcs - ucat_inner, with Ubcs from ubf (0 on
+// walls) or 0 when ubf is null.  full: whole tangential ranges
+// (GhostNodeVelocity); otherwise interior ranges (FormBCS)
+__device__ __forceinline__ void wall_ghost_body(G g, int bx, int bz, int full, V3 u, const double *ubf, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   const int a0 = full ? 0 : 1;
   const long nyz = (long)my * mz, nxz = (long)mx * mz, nxy = (long)mx * my;
@@ -136,33 +146,36 @@ __device__ __forceinline__ void wall_ghost_body(G g, int bx, int bz, int full, V
   for (long t = gs.t0; t < tot; t += gs.step) {
     long lo, hi, ln, in_lo, in_hi;
     double lx = 0.0, ly = 0.0, lz = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;   // Ubcs of the two ghosts
+    long r;
+    int f0;
     if (t < nyz) {                       // x planes
       if (bx) continue;
+      r = t; f0 = 0;
       int j = (int)(t % my), k = (int)(t / my);
       if (j < a0 || j >= (full ? my : my - 1) || k < a0 || k >= (full ? mz : mz - 1)) continue;
       lo = g.at(0, j, k); in_lo = g.at(1, j, k); hi = g.at(mx - 1, j, k); in_hi = g.at(mx - 2, j, k);
       (void)ln;
     } else if (t < nyz + nxz) {          // y planes
-      long r = t - nyz;
+      r = t - nyz; f0 = 2;
       int i = (int)(r % mx), k = (int)(r / mx);
       if (i < a0 || i >= (full ? mx : mx - 1) || k < a0 || k >= (full ? mz : mz - 1)) continue;
       lo = g.at(i, 0, k); in_lo = g.at(i, 1, k); hi = g.at(i, my - 1, k); in_hi = g.at(i, my - 2, k);
     } else {                             // z planes
       if (bz) continue;
-      long r = t - nyz - nxz;
+      r = t - nyz - nxz; f0 = 4;
       int i = (int)(r % mx), j = (int)(r / mx);
       if (i < a0 || i >= (full ? mx : mx - 1) || j < a0 || j >= (full ? my : my - 1)) continue;
       lo = g.at(i, j, 0); in_lo = g.at(i, j, 1); hi = g.at(i, j, mz - 1); in_hi = g.at(i, j, mz - 2);
-      if (ubk) {
-        lx = ubk[r]; ly = ubk[nxy + r]; lz = ubk[2 * nxy + r];
-        hx = ubk[3 * nxy + r]; hy = ubk[4 * nxy + r]; hz = ubk[5 * nxy + r];
-      }
+    }
+    if (ubf) {
+      lx = ubf[ubf_at(g, f0, 0, r)]; ly = ubf[ubf_at(g, f0, 1, r)]; lz = ubf[ubf_at(g, f0, 2, r)];
+      hx = ubf[ubf_at(g, f0 + 1, 0, r)]; hy = ubf[ubf_at(g, f0 + 1, 1, r)]; hz = ubf[ubf_at(g, f0 + 1, 2, r)];
     }
     u.x[lo] = 2 * lx - u.x[in_lo]; u.y[lo] = 2 * ly - u.y[in_lo]; u.z[lo] = 2 * lz - u.z[in_lo];
     u.x[hi] = 2 * hx - u.x[in_hi]; u.y[hi] = 2 * hy - u.y[in_hi]; u.z[hi] = 2 * hz - u.z[in_hi];
   }
 }
-__global__ void k_wall_ghost(G g, int bx, int bz, int full, V3 u, const double *ubk) { wall_ghost_body(g, bx, bz, full, u, ubk, GS_SELF); }
+__global__ void k_wall_ghost(G g, int bx, int bz, int full, V3 u, const double *ubf) { wall_ghost_body(g, bx, bz, full, u, ubf, GS_SELF); }
 
 
 // periodic plane copies restricted to the interior of the other axes
@@ -249,10 +262,9 @@ __global__ void k_periodic_full(G g, int axis, int x_only, V3 u, double *p, doub
 
 
 // zero wall-normal flux of bctype-1 faces after an RK stage (the build's
-// deliberate deviation: the reference never resets it).  bx, bz: bit 0 the
-// lower face, bit 1 the upper face of the axis is a wall; the j faces
-// always are
-__device__ __forceinline__ void wall_flux_zero_body(G g, int bx, int bz, V3 u, GS gs) {
+// deliberate deviation: the reference never resets it).  bx, by, bz: bit 0
+// the lower face, bit 1 the upper face of the axis is a wall
+__device__ __forceinline__ void wall_flux_zero_body(G g, int bx, int by, int bz, V3 u, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
   const long nyz = (long)my * mz, nxz = (long)mx * mz, nxy = (long)mx * my;
   for (long t = gs.t0; t < nyz + nxz + nxy; t += gs.step) {
@@ -262,9 +274,11 @@ __device__ __forceinline__ void wall_flux_zero_body(G g, int bx, int bz, V3 u, G
       if (bx & 1) u.x[g.at(0, j, k)] = 0.;
       if (bx & 2) u.x[g.at(mx - 2, j, k)] = 0.;
     } else if (t < nyz + nxz) {
+      if (!by) continue;
       long r = t - nyz;
       int i = (int)(r % mx), k = (int)(r / mx);
-      u.y[g.at(i, 0, k)] = 0.; u.y[g.at(i, my - 2, k)] = 0.;
+      if (by & 1) u.y[g.at(i, 0, k)] = 0.;
+      if (by & 2) u.y[g.at(i, my - 2, k)] = 0.;
     } else {
       if (!bz) continue;
       long r = t - nyz - nxz;
@@ -274,19 +288,18 @@ __device__ __forceinline__ void wall_flux_zero_body(G g, int bx, int bz, V3 u, G
     }
   }
 }
-__global__ void k_wall_flux_zero(G g, int bx, int bz, V3 u) { wall_flux_zero_body(g, bx, bz, u, GS_SELF); }
+__global__ void k_wall_flux_zero(G g, int bx, int by, int bz, V3 u) { wall_flux_zero_body(g, bx, by, bz, u, GS_SELF); }
 
 // wall masks of wall_flux_zero_body for one axis from its two bctypes
 inline int wall_mask(int lo, int hi) { return (lo == 1 ? 1 : 0) | (hi == 1 ? 2 : 0); }
 
 
 // ------------------------------------------------------------------
-// inlet (bctype 5, k = 0) and outlet (bctype 4, k = mz-1):
-// InflowFlux bcs.c:99-246, OutflowFlux bcs.c:403-469, FormBCS's outlet
-// section bcs.c:4914-5010.  Plane kernels over the interior (i, j) of one
-// k plane; sums go through per-block partials and a fixed-order final
-// pass.  ubk holds Ubcs of plane 0 (k = 0) and plane 1 (k = mz-1), planar
-// x|y|z, index i + mx*j.
+// inlet (bctype 5, k = 0), outlet (bctype 4, k = mz-1), slip (3) and far
+// field (6): InflowFlux bcs.c:99-246, OutflowFlux bcs.c:403-469, FormBCS's
+// far-field, slip and outlet sections bcs.c:4500-5010.  Kernels over the
+// interior points of the boundary faces; sums go through per-block
+// partials and a fixed-order final pass.
 // ------------------------------------------------------------------
 // the analytic inflow profiles (bcs.c:158-196) at the offset face centre
 __device__ double inlet_uin(int prof, double xc, double yc, double zc) {
@@ -318,10 +331,10 @@ struct Inlet {
 
 // inlet fluxes and Ubcs where the first cell is fluid; the fluid inflow
 // sum to part[block]
-__global__ void k_inflow(G g, Inlet in, const double *coor, long n, C3 zet, const double *nv, double *ucz, double *ubk,
+__global__ void k_inflow(G g, Inlet in, const double *coor, long n, C3 zet, const double *nv, double *ucz, double *ubf,
                          double *part) {
   __shared__ double sh[BS / 64];
-  const long nxy = (long)g.mx * g.my, m2 = (long)(g.mx - 2) * (g.my - 2);
+  const long m2 = (long)(g.mx - 2) * (g.my - 2);
   const double *X = coor, *Y = coor + n, *Z = coor + 2 * n;
   double f = 0.0;
   for (long t = blockIdx.x * (long)BS + threadIdx.x; t < m2; t += (long)gridDim.x * BS) {
@@ -334,9 +347,9 @@ __global__ void k_inflow(G g, Inlet in, const double *coor, long n, C3 zet, cons
     if (nv[c + g.sk] < 0.1) {
       const double zx = zet.x[c], zy = zet.y[c], zz = zet.z[c];
       const double a = sqrt(zz * zz + zy * zy + zx * zx);
-      ubk[nxy + c] = uin * zy / a;
-      ubk[2 * nxy + c] = uin * zz / a;
-      ubk[c] = uin * zx / a;
+      ubf[ubf_at(g, 4, 1, c)] = uin * zy / a;
+      ubf[ubf_at(g, 4, 2, c)] = uin * zz / a;
+      ubf[ubf_at(g, 4, 0, c)] = uin * zx / a;
       const double q = uin * a;
       ucz[c] = q;
       f += q;
@@ -369,18 +382,18 @@ __global__ void k_outlet_sum(G g, C3 ucat, C3 zet, const double *nv, double *par
 // outlet correction: Ubcs = Ucat of the last cell, its flux plus the
 // uniform correction (FluxInSum - flux) / area; the corrected fluid
 // outflow to part[block] when stat
-__global__ void k_outlet_fix(G g, C3 ucat, C3 zet, const double *nv, double *ucz, double *ubk, const double *dsc,
+__global__ void k_outlet_fix(G g, C3 ucat, C3 zet, const double *nv, double *ucz, double *ubf, const double *dsc,
                              int stat, double *part) {
   __shared__ double sh[BS / 64];
   const long nxy = (long)g.mx * g.my, m2 = (long)(g.mx - 2) * (g.my - 2), base = (long)(g.mz - 2) * nxy;
-  // FluxIn = FluxInSum + FarFluxInSum + FluxIntpSum, the last two 0 here
-  const double ratio = (dsc[CV_DSC_FLUXIN] - dsc[CV_DSC_OUTSUM]) / dsc[CV_DSC_OUTSUM + 1];
+  // FluxIn = FluxInSum + FarFluxInSum + FluxIntpSum, the last 0 here
+  const double ratio = (dsc[CV_DSC_FLUXIN] + dsc[CV_DSC_FAR] - dsc[CV_DSC_OUTSUM]) / dsc[CV_DSC_OUTSUM + 1];
   double f = 0.0;
   for (long t = blockIdx.x * (long)BS + threadIdx.x; t < m2; t += (long)gridDim.x * BS) {
     const long r = g.at(1 + (int)(t % (g.mx - 2)), 1 + (int)(t / (g.mx - 2)), 0), c = base + r;
     if (nv[c] < 0.1) {
       const double bx = ucat.x[c], by = ucat.y[c], bz = ucat.z[c];
-      ubk[3 * nxy + r] = bx; ubk[4 * nxy + r] = by; ubk[5 * nxy + r] = bz;
+      ubf[ubf_at(g, 5, 0, r)] = bx; ubf[ubf_at(g, 5, 1, r)] = by; ubf[ubf_at(g, 5, 2, r)] = bz;
       const double zx = zet.x[c], zy = zet.y[c], zz = zet.z[c];
       const double q = (bx * zx + by * zy + bz * zz) + ratio * sqrt(zx * zx + zy * zy + zz * zz);
       ucz[c] = q;
@@ -390,6 +403,120 @@ __global__ void k_outlet_fix(G g, C3 ucat, C3 zet, const double *nv, double *ucz
   if (!stat) return;
   const double sf = cv_block_sum(f, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = sf;
+}
+
+// point t of the interior points of the six faces, in face order (i-, i+,
+// j-, j+, k-, k+; each face's points i fastest, then j, then k): the face
+// f, the storage indices of the ghost cell, the adjacent cell and the
+// boundary face (0 on lower faces, m-2 on upper ones), and the ghost's
+// plane index r.  Returns false past the last point.
+struct FacePt {
+  int f;
+  long gi, ai, bi, r;
+};
+__device__ inline bool face_point(const G &g, long t, FacePt &p) {
+  const long nx = g.mx - 2, ny = g.my - 2, nz = g.mz - 2;
+  const long sz[3] = {ny * nz, nx * nz, nx * ny};
+  int f = 0;
+  while (f < 6 && t >= sz[f / 2]) { t -= sz[f / 2]; f++; }
+  if (f == 6) return false;
+  const int d = f / 2, up = f & 1;
+  const int m = d == 0 ? g.mx : (d == 1 ? g.my : g.mz);
+  const int gg = up ? m - 1 : 0, aa = up ? m - 2 : 1, bb = up ? m - 2 : 0;
+  if (d == 0) {
+    const int j = 1 + (int)(t % ny), k = 1 + (int)(t / ny);
+    p.gi = g.at(gg, j, k); p.ai = g.at(aa, j, k); p.bi = g.at(bb, j, k); p.r = j + (long)g.my * k;
+  } else if (d == 1) {
+    const int i = 1 + (int)(t % nx), k = 1 + (int)(t / nx);
+    p.gi = g.at(i, gg, k); p.ai = g.at(i, aa, k); p.bi = g.at(i, bb, k); p.r = i + (long)g.mx * k;
+  } else {
+    const int i = 1 + (int)(t % nx), j = 1 + (int)(t / nx);
+    p.gi = g.at(i, j, gg); p.ai = g.at(i, j, aa); p.bi = g.at(i, j, bb); p.r = i + (long)g.mx * j;
+  }
+  p.f = f;
+  return true;
+}
+__host__ __device__ inline long face_points(int mx, int my, int mz) {
+  return 2 * ((long)(my - 2) * (mz - 2) + (long)(mx - 2) * (mz - 2) + (long)(mx - 2) * (my - 2));
+}
+
+// slip faces (bctype 3; FormBCS bcs.c:4729-4853, GhostNodeVelocity
+// bcs.c:3833-3960): Ubcs = the adjacent Ucat less its normal component;
+// with formbcs the face's normal flux is zeroed
+__global__ void k_slip(G g, int mask, int formbcs, C3 ucat, C3 csi, C3 eta, C3 zet, V3 ucont, double *ubf) {
+  const long np = face_points(g.mx, g.my, g.mz);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < np; t += (long)gridDim.x * blockDim.x) {
+    FacePt p;
+    if (!face_point(g, t, p) || !((mask >> p.f) & 1)) continue;
+    const int d = p.f / 2;
+    const C3 m = d == 0 ? csi : (d == 1 ? eta : zet);
+    const double mx_ = m.x[p.bi], my_ = m.y[p.bi], mz_ = m.z[p.bi];
+    const double A = sqrt(mz_ * mz_ + my_ * my_ + mx_ * mx_);
+    const double nx = mx_ / A, ny = my_ / A, nz = mz_ / A;
+    const double ux = ucat.x[p.ai], uy = ucat.y[p.ai], uz = ucat.z[p.ai];
+    const double Un = ux * nx + uy * ny + uz * nz;
+    ubf[ubf_at(g, p.f, 0, p.r)] = ux - Un * nx;
+    ubf[ubf_at(g, p.f, 1, p.r)] = uy - Un * ny;
+    ubf[ubf_at(g, p.f, 2, p.r)] = uz - Un * nz;
+    if (formbcs) (d == 0 ? ucont.x : (d == 1 ? ucont.y : ucont.z))[p.bi] = 0.;
+  }
+}
+
+// far-field faces (bctype 6; FormBCS bcs.c:4500-4608): Ubcs = the adjacent
+// Ucat and the face flux its normal component times the matching metric
+// component; the inflow (lower faces), outflow (upper faces), |flux| and
+// the two areas to part[q*nb + block], q = 0..4
+__global__ void k_far_sum(G g, int mask, C3 ucat, C3 csi, C3 eta, C3 zet, V3 ucont, double *ubf, double *part) {
+  __shared__ double sh[BS / 64];
+  const long np = face_points(g.mx, g.my, g.mz);
+  double fin = 0.0, fout = 0.0, fab = 0.0, ain = 0.0, aout = 0.0;
+  for (long t = blockIdx.x * (long)BS + threadIdx.x; t < np; t += (long)gridDim.x * BS) {
+    FacePt p;
+    if (!face_point(g, t, p) || !((mask >> p.f) & 1)) continue;
+    const int d = p.f / 2;
+    const double ux = ucat.x[p.ai], uy = ucat.y[p.ai], uz = ucat.z[p.ai];
+    ubf[ubf_at(g, p.f, 0, p.r)] = ux;
+    ubf[ubf_at(g, p.f, 1, p.r)] = uy;
+    ubf[ubf_at(g, p.f, 2, p.r)] = uz;
+    const double mc = d == 0 ? csi.x[p.bi] : (d == 1 ? eta.y[p.bi] : zet.z[p.bi]);
+    const double q = (d == 0 ? ux : (d == 1 ? uy : uz)) * mc;
+    (d == 0 ? ucont.x : (d == 1 ? ucont.y : ucont.z))[p.bi] = q;
+    if (p.f & 1) { fout += q; aout += mc; }
+    else { fin += q; ain += mc; }
+    fab += fabs(q);
+  }
+  const double v[5] = {fin, fout, fab, ain, aout};
+  for (int q = 0; q < 5; q++) {
+    const double sv = cv_block_sum(v[q], sh);
+    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = sv;
+  }
+}
+
+// far-field flux balance (bcs.c:4610-4724): with a far-field upper face,
+// ratio = (inflow - outflow) / sum |flux| (0 below 1e-10); upper faces
+// gain and lower faces lose ratio * |flux|, and Ubcs's normal component
+// follows the corrected flux
+__global__ void k_far_fix(G g, int mask, int has_out, C3 csi, C3 eta, C3 zet, V3 ucont, double *ubf,
+                          const double *dsc) {
+  double ratio = 0.;
+  if (has_out) {
+    const double fab = dsc[CV_DSC_FAR + 2];
+    ratio = (dsc[CV_DSC_FAR] - dsc[CV_DSC_FAR + 1]) / fab;
+    if (fabs(fab) < 1.e-10) ratio = 0.;
+  }
+  const long np = face_points(g.mx, g.my, g.mz);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < np; t += (long)gridDim.x * blockDim.x) {
+    FacePt p;
+    if (!face_point(g, t, p) || !((mask >> p.f) & 1)) continue;
+    const int d = p.f / 2;
+    double *uq = d == 0 ? ucont.x : (d == 1 ? ucont.y : ucont.z);
+    double q = uq[p.bi];
+    if (p.f & 1) q += ratio * fabs(q);
+    else q -= ratio * fabs(q);
+    uq[p.bi] = q;
+    const double mc = d == 0 ? csi.x[p.bi] : (d == 1 ? eta.y[p.bi] : zet.z[p.bi]);
+    ubf[ubf_at(g, p.f, d, p.r)] = q / mc;
+  }
 }
 
 // OutflowFlux: the k = mz-2 face flux over the interior, fluid or not
@@ -1114,7 +1241,7 @@ int cv_inflow_dev(cv_ctx *c) {
   const unsigned nb = plane_nb(g);
   const Inlet in{c->inletprofile, c->cm_c[0], c->cm_c[1], c->cm_c[2]};
   hipLaunchKernelGGL(k_inflow, dim3(nb), dim3(BS), 0, c->stream, g, in, (const double *)F.coor, F.d.n, c3(c->m.zet),
-                     (const double *)F.nvert, c->ucont[2], c->ubk, c->part);
+                     (const double *)F.nvert, c->ucont[2], c->ubf, c->part);
   hipLaunchKernelGGL(cv_k_final_sum, dim3(1), dim3(BS), 0, c->stream, (const double *)c->part, (int)nb,
                      c->dsc + CV_DSC_FLUXIN, 1);
   CV_CHECK(hipGetLastError());
@@ -1132,16 +1259,42 @@ static void outlet_fix(cv_ctx *c, int stat) {
   hipLaunchKernelGGL(cv_k_final_sum, dim3(2), dim3(BS), 0, c->stream, (const double *)c->part, (int)nb,
                      c->dsc + CV_DSC_OUTSUM, 2);
   hipLaunchKernelGGL(k_outlet_fix, dim3(nb), dim3(BS), 0, c->stream, g, c3(c->ucat), c3(c->m.zet), nv, c->ucont[2],
-                     c->ubk, (const double *)c->dsc, stat, c->part + 2 * nb);
+                     c->ubf, (const double *)c->dsc, stat, c->part + 2 * nb);
   if (stat)
     hipLaunchKernelGGL(cv_k_final_sum, dim3(1), dim3(BS), 0, c->stream, (const double *)(c->part + 2 * nb), (int)nb,
                        c->dsc + CV_DSC_FLUXOUT, 1);
 }
 
-// formbcs=1: the three passes of FormBCS.  Contra2Cart runs once over the
-// grid: between passes only the outlet correction changes Ucont (the
-// k = mz-2 fluxes), so later passes recompute that plane alone.
-// formbcs=0: GhostNodeVelocity
+// FormBCS's far-field section (bcs.c:4500-4724): Ubcs, fluxes and sums,
+// then the flux balance
+static void far_fix(cv_ctx *c) {
+  const G g = gview(c);
+  CvMetrics &M = c->m;
+  const long np = face_points(g.mx, g.my, g.mz);
+  const unsigned nb = (unsigned)std::min(256L, std::max(1L, (np + 4 * BS - 1) / (4 * BS)));
+  hipLaunchKernelGGL(k_far_sum, dim3(nb), dim3(BS), 0, c->stream, g, c->far, c3(c->ucat), c3(M.csi), c3(M.eta),
+                     c3(M.zet), v3(c->ucont), c->ubf, c->part);
+  hipLaunchKernelGGL(cv_k_final_sum, dim3(5), dim3(BS), 0, c->stream, (const double *)c->part, (int)nb,
+                     c->dsc + CV_DSC_FAR, 5);
+  const int has_out = (c->far & ((1 << 1) | (1 << 3) | (1 << 5))) != 0;
+  hipLaunchKernelGGL(k_far_fix, dim3(nb), dim3(BS), 0, c->stream, g, c->far, has_out, c3(M.csi), c3(M.eta), c3(M.zet),
+                     v3(c->ucont), c->ubf, (const double *)c->dsc);
+}
+
+// FormBCS's slip section (formbcs) or GhostNodeVelocity's
+static void slip_update(cv_ctx *c, int formbcs) {
+  const G g = gview(c);
+  CvMetrics &M = c->m;
+  hipLaunchKernelGGL(k_slip, dim3(nblk(face_points(g.mx, g.my, g.mz))), dim3(BS), 0, c->stream, g, c->slip, formbcs,
+                     c3(c->ucat), c3(M.csi), c3(M.eta), c3(M.zet), v3(c->ucont), c->ubf);
+}
+
+// formbcs=1: the three passes of FormBCS, each far field -> slip -> outlet
+// -> Contra2Cart -> ghosts.  Contra2Cart runs over the whole grid in the
+// first pass; later passes repeat it only where the pass changed Ucont:
+// the whole grid with far-field faces, the k = mz-2 plane with an outlet
+// alone (the slip faces' zero fluxes are already in place).
+// formbcs=0: GhostNodeVelocity (slip Ubcs, then the ghosts)
 int cv_ghost_update(cv_ctx *c, int formbcs) {
   CvLevel &F = c->lev[c->nlev - 1];
   CvMetrics &M = c->m;
@@ -1149,21 +1302,21 @@ int cv_ghost_update(cv_ctx *c, int formbcs) {
   const int bxper = F.d.xper, bzper = F.d.zper;
   const long planes = (long)g.my * g.mz + (long)g.mx * g.mz + (long)g.mx * g.my;
   const int passes = formbcs ? 3 : 1;
-  const bool outl = formbcs && c->outlet;
-  if (formbcs) {
-    if (outl) outlet_fix(c, 0);
-    int rc = cv_contra2cart_dev(c);
-    if (rc) return rc;
-  }
+  const bool outl = formbcs && c->outlet, far = formbcs && c->far;
   double *p = formbcs ? c->p : nullptr, *nv = formbcs ? F.nvert : nullptr;
   for (int ps = 0; ps < passes; ps++) {
-    if (outl && ps > 0) {
-      outlet_fix(c, ps == passes - 1);
+    if (far) far_fix(c);
+    if (c->slip) slip_update(c, formbcs);
+    if (outl) outlet_fix(c, ps == passes - 1);
+    if (formbcs && (ps == 0 || far)) {
+      int rc = cv_contra2cart_dev(c);
+      if (rc) return rc;
+    } else if (outl) {
       hipLaunchKernelGGL(k_contra2cart_plane, dim3(plane_nb(g)), dim3(BS), 0, c->stream, g, c3(c->ucont), c3(M.csi),
                          c3(M.eta), c3(M.zet), (const double *)F.nvert, v3(c->ucat), g.mz - 2);
     }
     hipLaunchKernelGGL(k_wall_ghost, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, bxper, bzper, formbcs ? 0 : 1, v3(c->ucat),
-                       (const double *)c->ubk);
+                       (const double *)c->ubf);
     if (bxper || bzper)
       hipLaunchKernelGGL(k_periodic_planes, dim3(nblk((long)g.my * g.mz + (long)g.mx * g.my)), dim3(BS), 0, c->stream, g,
                          v3(c->ucat), p, nv);
@@ -1271,14 +1424,15 @@ int cv_rk_dev(cv_ctx *c) {
   if (rc) return rc;
   for (int q = 0; q < 3; q++)
     hipLaunchKernelGGL(k_fill, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, c->rhs[q], 0.0, F.d.n);
-  const int wx = wall_mask(c->bctype[0], c->bctype[1]), wz = wall_mask(c->bctype[4], c->bctype[5]);
+  const int wx = wall_mask(c->bctype[0], c->bctype[1]), wy = wall_mask(c->bctype[2], c->bctype[3]),
+            wz = wall_mask(c->bctype[4], c->bctype[5]);
   const long planes = (long)g.my * g.mz + (long)g.mx * g.mz + (long)g.mx * g.my;
   for (int s = 0; s < 4; s++) {
     if ((rc = cv_ff1_dev(c))) return rc;
     double a = alfa[s] * c->dt * c->st;
     hipLaunchKernelGGL(k_rk_update, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, a, c3(c->ucont_o), c3(c->rhs),
                        v3(c->ucont), F.d.n);
-    hipLaunchKernelGGL(k_wall_flux_zero, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, wx, wz, v3(c->ucont));
+    hipLaunchKernelGGL(k_wall_flux_zero, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, wx, wy, wz, v3(c->ucont));
     if ((rc = cv_bc_update_dev(c))) return rc;
   }
   CV_CHECK(hipGetLastError());
@@ -1532,7 +1686,7 @@ struct MCase {
   C3 csi, eta, zet;
   const double *aj;
   double nu, dt, st;
-  int wx, wz;               // wall masks of the i and k faces (wall-normal flux reset)
+  int wx, wy, wz;           // wall masks of the i, j and k faces (wall-normal flux reset)
   long n;
   int mgx, mgy, mgz;        // k-marching blocks
   int rgx, rgy, rgz, rkc;   // face-flux tiles (8 rows) and their k planes
@@ -1570,7 +1724,7 @@ __global__ void __launch_bounds__(64 * MARCH_TY) kbm_contra2cart(MB_ARGS) {
 
 __global__ void kbm_wall_ghost(MB_ARGS, int full) {
   MB_HEAD
-  wall_ghost_body(C.g, C.g.xper, C.g.zper, full, mv3(C.uca), nullptr, MB_GSP);   // batched cases have no inlet/outlet
+  wall_ghost_body(C.g, C.g.xper, C.g.zper, full, mv3(C.uca), nullptr, MB_GSP);   // batched cases: walls and periodic faces
 }
 
 __global__ void kbm_periodic_planes(MB_ARGS, int formbcs) {
@@ -1621,7 +1775,7 @@ __global__ void kbm_rk_update(MB_ARGS, double alfa) {
 
 __global__ void kbm_wall_flux_zero(MB_ARGS) {
   MB_HEAD
-  wall_flux_zero_body(C.g, C.wx, C.wz, mv3(C.uc), MB_GSP);
+  wall_flux_zero_body(C.g, C.wx, C.wy, C.wz, mv3(C.uc), MB_GSP);
 }
 
 }  // namespace
@@ -1677,6 +1831,7 @@ int cv_mom_batch_create(cv_ctx *const *cases, int n, hipStream_t s, CvMomBatch *
     m.dt = x->dt;
     m.st = x->st;
     m.wx = wall_mask(x->bctype[0], x->bctype[1]);
+    m.wy = wall_mask(x->bctype[2], x->bctype[3]);
     m.wz = wall_mask(x->bctype[4], x->bctype[5]);
     m.n = F.d.n;
     const dim3 mg = march_grid(F.d.mx, F.d.my, F.d.mz);

@@ -47,7 +47,9 @@ typedef struct cv_ctx cv_ctx;
 typedef struct {
   int IM, JM, KM;          /* grid nodes per axis */
   int bctype[6];           /* i-, i+, j-, j+, k-, k+ ; supported: 1 wall, 7 periodic (i, k),
-                              5 inlet (k- only), 4 outlet (k+ only) */
+                              3 slip and 6 far field (every face but j-: the reference's
+                              FormMetrics leaves that face's eta at 0), 5 inlet (k- only),
+                              4 outlet (k+ only) */
   int mglevels;            /* -mg_level */
   int isc, jsc, ksc;       /* -mg_{i,j,k}_semi */
   double ren, dt, st;      /* -ren, -dt, Strouhal number */

@@ -73,6 +73,7 @@ typedef struct {
    * (main.c:9) as InflowFlux, FormBCS and OutflowFlux leave them */
   int inletprofile; double cm_c[3];
   double flux_in_sum, flux_out_sum, flux_out_diag;
+  double far_flux_in, far_flux_out;   /* FarFluxInSum / FarFluxOutSum of FormBCS's last pass */
   int imp_its; double imp_lambda, imp_normdU, imp_reldU;
   ofield b;           /* Poisson right-hand side */
   /* diagnostics */

@@ -81,6 +81,7 @@ void o_set_steps_done(octx *c, int n) { c->steps_done = n; }
 
 void o_flux_stats(octx *c, double *out) {
   out[0] = c->flux_in_sum; out[1] = c->flux_out_sum; out[2] = c->flux_out_diag;
+  out[3] = c->far_flux_in; out[4] = c->far_flux_out;
 }
 
 void o_imp_stats(octx *c, double *out) {

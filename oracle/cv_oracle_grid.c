@@ -10,8 +10,9 @@
  *   MyNvertRestriction brute-force restatement in numpy on small masks.
  * Only single-rank semantics are restated (xs=0, xe=mx, ...).
  * Supported boundary types: 1 (no-slip wall) on every axis, 7 (periodic)
- * on the i and k axes, 5 (inlet) on the k- face and 4 (outlet) on the k+
- * face; metrics and the Poisson operator only tell 7 from the others.  The cgrid option is taken as 0 (the reference
+ * on the i and k axes, 3 (slip) and 6 (far field) on every face but j-, 5 (inlet)
+ * on the k- face and 4 (outlet) on the k+ face; metrics and the Poisson
+ * operator only tell 7 from the others.  The cgrid option is taken as 0 (the reference
  * leaves it uninitialised unless -cgrid is given, init.c:356-358).
  */
 #include "cv_oracle_int.h"
@@ -111,10 +112,17 @@ octx *o_create(int IM, int JM, int KM, const int bctype[6], int nlev,
                int isc, int jsc, int ksc, double ren, double dt, int coarsen_auto) {
   if (nlev < 1 || nlev > O_MAXLEV) return NULL;
   if (bctype[2] == 7 || bctype[3] == 7) return NULL;  /* j-periodic not restated */
-  /* i and j faces: wall 1 or periodic 7; k- also inlet 5, k+ also outlet 4 */
-  for (int q = 0; q < 4; q++) if (bctype[q] != 1 && bctype[q] != 7) return NULL;
-  if (bctype[4] != 1 && bctype[4] != 5 && bctype[4] != 7) return NULL;
-  if (bctype[5] != 1 && bctype[5] != 4 && bctype[5] != 7) return NULL;
+  /* every face: wall 1, slip 3, far field 6, periodic 7 (not on j); k-
+   * also inlet 5, k+ also outlet 4.  Not slip or far field on j-:
+   * FormMetrics leaves eta of the j = 0 face at 0 (its loop starts at
+   * j = 1, metrics.c:107-110), so both divide by zero there. */
+  for (int q = 0; q < 6; q++) {
+    const int b = bctype[q];
+    if (q == 2 && (b == 3 || b == 6)) return NULL;
+    if (b == 1 || b == 3 || b == 6 || b == 7) continue;
+    if ((q == 4 && b == 5) || (q == 5 && b == 4)) continue;
+    return NULL;
+  }
   if ((bctype[0] == 7) != (bctype[1] == 7) || (bctype[4] == 7) != (bctype[5] == 7)) return NULL;
   octx *c = (octx *)calloc(1, sizeof(octx));
   c->nlev = nlev;

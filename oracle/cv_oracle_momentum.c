@@ -8,7 +8,7 @@
  *   o_formfunction1     FormFunction1        rhs.c:2567-3351
  *   o_inflowflux        InflowFlux, analytic profiles  bcs.c:99-246
  *   o_outflowflux       OutflowFlux          bcs.c:403-469
- *   o_formbcs           FormBCS, bctype {1,4,5,7} subset   bcs.c:4436-5888
+ *   o_formbcs           FormBCS, bctype {1,3,4,5,6,7}  bcs.c:4436-5888
  *   o_ghostnodevelocity GhostNodeVelocity    bcs.c:3645-4435 (subset)
  *   o_rungekutta        RungeKutta           implicitsolver.c:32-119
  *   o_flow_step         Flow_Solver          solvers.c:944-1156 (+ the level
@@ -330,6 +330,117 @@ void o_outflowflux(octx *c) {
   c->flux_out_diag = flux;
 }
 
+/* point (s, t) of face f (0 i-, 1 i+, 2 j-, 3 j+, 4 k-, 5 k+), s and t the
+ * outer and inner indices of the reference's face loops (i faces k, j; j
+ * faces k, i; k faces j, i): the ghost cell g, the adjacent cell a and the
+ * storage point b of the boundary face (0 on lower faces, m-2 on upper
+ * ones), each as (i, j, k) */
+static void face_pt(const olevel *L, int f, int s, int t, int g[3], int a[3], int b[3]) {
+  const int d = f / 2, up = f % 2, m = d == 0 ? L->mx : (d == 1 ? L->my : L->mz);
+  int p[3] = {0, 0, 0};
+  if (d == 0) { p[2] = s; p[1] = t; }
+  else if (d == 1) { p[2] = s; p[0] = t; }
+  else { p[1] = s; p[0] = t; }
+  for (int q = 0; q < 3; q++) g[q] = a[q] = b[q] = p[q];
+  g[d] = up ? m - 1 : 0; a[d] = up ? m - 2 : 1; b[d] = up ? m - 2 : 0;
+}
+/* extents of the outer and inner loop axes of face f */
+static void face_ext(const olevel *L, int f, int *ms, int *mt) {
+  const int d = f / 2;
+  *ms = d == 2 ? L->my : L->mz;
+  *mt = d == 0 ? L->my : L->mx;
+}
+static const ofield *face_metric(const olevel *L, int f) {
+  return f / 2 == 0 ? &L->csi : (f / 2 == 1 ? &L->eta : &L->zet);
+}
+#define VP(f, p) V(f, (p)[2], (p)[1], (p)[0])
+#define CMP(v, d) (((double *)&(v))[d])
+
+/* slip walls, bctype 3 (FormBCS bcs.c:4729-4853, GhostNodeVelocity
+ * bcs.c:3833-3960): Ubcs = the adjacent Ucat less its component along the
+ * face normal; FormBCS also zeroes the face's normal flux */
+static void slip_faces(octx *c, int formbcs) {
+  olevel *L = &c->lev[c->nlev - 1];
+  for (int f = 0; f < 6; f++) {
+    if (L->bctype[f] != 3) continue;
+    const int d = f / 2;
+    const ofield *met = face_metric(L, f);
+    int ms, mt;
+    face_ext(L, f, &ms, &mt);
+    for (int s = 1; s < ms - 1; s++)
+      for (int t = 1; t < mt - 1; t++) {
+        int g[3], a[3], b[3];
+        face_pt(L, f, s, t, g, a, b);
+        const ocmp m = VP(met, b), u = VP(&c->ucat, a);
+        const double A = sqrt(m.z * m.z + m.y * m.y + m.x * m.x);
+        const double nx = m.x / A, ny = m.y / A, nz = m.z / A;
+        const double Un = u.x * nx + u.y * ny + u.z * nz;
+        ocmp *ub = &VP(&c->ubcs, g);
+        ub->x = u.x - Un * nx;
+        ub->y = u.y - Un * ny;
+        ub->z = u.z - Un * nz;
+        if (formbcs) CMP(VP(&c->ucont, b), d) = 0.;
+      }
+  }
+}
+
+/* far field, bctype 6 (FormBCS bcs.c:4500-4724): Ubcs = the adjacent Ucat
+ * and the face flux its normal component times the matching metric
+ * component; when an upper face is far field, a correction proportional to
+ * each face's |flux| balances the far-field inflow (lower faces) and
+ * outflow (upper faces).  FarFluxInSum feeds the outlet section. */
+static void farfield(octx *c) {
+  olevel *L = &c->lev[c->nlev - 1];
+  double fin = 0., fout = 0., fabs_sum = 0., ain = 0., aout = 0.;
+  for (int f = 0; f < 6; f++) {
+    if (L->bctype[f] != 6) continue;
+    const int d = f / 2, up = f % 2;
+    const ofield *met = face_metric(L, f);
+    int ms, mt;
+    face_ext(L, f, &ms, &mt);
+    for (int s = 1; s < ms - 1; s++)
+      for (int t = 1; t < mt - 1; t++) {
+        int g[3], a[3], b[3];
+        face_pt(L, f, s, t, g, a, b);
+        const ocmp u = VP(&c->ucat, a);
+        VP(&c->ubcs, g) = u;
+        const double mc = CMP(VP(met, b), d);
+        const double q = CMP(u, d) * mc;
+        CMP(VP(&c->ucont, b), d) = q;
+        if (up) { fout += q; aout += mc; }
+        else { fin += q; ain += mc; }
+        fabs_sum += fabs(q);
+      }
+  }
+  (void)ain; (void)aout;   /* the far-field areas only enter the reference's log line */
+  c->far_flux_in = fin; c->far_flux_out = fout;
+  double ratio = 0.;
+  if (L->bctype[5] == 6 || L->bctype[3] == 6 || L->bctype[1] == 6) {
+    ratio = (fin - fout) / fabs_sum;
+    if (fabs(fabs_sum) < 1.e-10) ratio = 0.;
+  }
+  const int order[6] = {5, 3, 1, 0, 2, 4};
+  for (int o = 0; o < 6; o++) {
+    const int f = order[o];
+    if (L->bctype[f] != 6) continue;
+    const int d = f / 2, up = f % 2;
+    const ofield *met = face_metric(L, f);
+    int ms, mt;
+    face_ext(L, f, &ms, &mt);
+    for (int s = 1; s < ms - 1; s++)
+      for (int t = 1; t < mt - 1; t++) {
+        int g[3], a[3], b[3];
+        face_pt(L, f, s, t, g, a, b);
+        double *q = &CMP(VP(&c->ucont, b), d);
+        if (up) *q += ratio * fabs(*q);
+        else *q -= ratio * fabs(*q);
+        CMP(VP(&c->ubcs, g), d) = *q / CMP(VP(met, b), d);
+      }
+  }
+}
+#undef VP
+#undef CMP
+
 /* FormBCS's outlet section for bctype 4 on the k+ face (bcs.c:4914-5010):
  * Ubcs = Ucat of the last cell, and the outlet flux is that velocity's flux
  * plus a uniform correction that makes the fluid outflow equal FluxInSum */
@@ -347,8 +458,8 @@ static void outlet_kplus(octx *c) {
         fout += u.x * z.x + u.y * z.y + u.z * z.z;
         area += sqrt(z.x * z.x + z.y * z.y + z.z * z.z);
       }
-  /* FluxIn = FluxInSum + FarFluxInSum + FluxIntpSum, the last two 0 here */
-  const double ratio = (c->flux_in_sum - fout) / area;
+  /* FluxIn = FluxInSum + FarFluxInSum + FluxIntpSum, the last 0 here */
+  const double ratio = (c->flux_in_sum + c->far_flux_in - fout) / area;
   double fsum = 0.;
   for (int j = 1; j < my - 1; j++)
     for (int i = 1; i < mx - 1; i++)
@@ -361,12 +472,14 @@ static void outlet_kplus(octx *c) {
   c->flux_out_sum = fsum;
 }
 
-/* FormBCS for bctype {1,4,5,7} (bcs.c:4436-5888): three passes of
- * outlet correction -> Contra2Cart -> wall ghosts -> periodic planes ->
- * edge averages -> periodic full planes. */
+/* FormBCS for bctype {1,3,4,5,6,7} (bcs.c:4436-5888): three passes of
+ * far field -> slip -> outlet correction -> Contra2Cart -> wall ghosts ->
+ * periodic planes -> edge averages -> periodic full planes. */
 void o_formbcs(octx *c) {
   olevel *L = &c->lev[c->nlev - 1];
   for (int ttemp = 0; ttemp < 3; ttemp++) {
+    farfield(c);
+    slip_faces(c, 1);
     outlet_kplus(c);
     o_g2l(&c->ucont, L->per);
     o_contra2cart(c);
@@ -378,9 +491,11 @@ void o_formbcs(octx *c) {
   o_g2l(&c->p, L->per); o_g2l(&c->ucat, L->per); o_g2l(&c->ucont, L->per);
 }
 
-/* GhostNodeVelocity (bcs.c:3645-4435) for bctype {1,4,5,7}: the ghost
- * values follow the Ubcs the last InflowFlux / FormBCS left */
+/* GhostNodeVelocity (bcs.c:3645-4435) for bctype {1,3,4,5,6,7}: slip
+ * faces take Ubcs from the current Ucat, the others keep the Ubcs the last
+ * InflowFlux / FormBCS left */
 void o_ghostnodevelocity(octx *c) {
+  slip_faces(c, 0);
   wall_ghosts(c, 1);
   periodic_planes(c, 0);
   edge_average(c, 0);

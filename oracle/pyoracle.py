@@ -167,10 +167,11 @@ class Oracle:
         lib().o_set_inlet(self.h, int(profile), float(center[0]), float(center[1]), float(center[2]))
 
     def flux_stats(self):
-        """FluxInSum, FormBCS's FluxOutSum and OutflowFlux's sum, as last left."""
-        out = np.zeros(3)
+        """FluxInSum, FormBCS's FluxOutSum, OutflowFlux's sum and the far-field
+        in/out sums, as last left."""
+        out = np.zeros(5)
         lib().o_flux_stats(self.h, _dp(out))
-        return dict(zip(("flux_in", "flux_out", "flux_out_diag"), out.tolist()))
+        return dict(zip(("flux_in", "flux_out", "flux_out_diag", "far_in", "far_out"), out.tolist()))
 
     def csr(self, level):
         mx, my, mz = self.dims(level)[:3]

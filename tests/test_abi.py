@@ -126,17 +126,21 @@ def test_binding_structs_match_the_c_layout(tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(SO), reason="library not built")
 def test_inlet_outlet_descriptions_without_gpu():
-    """Inlet (k-, bctype 5) and outlet (k+, bctype 4) are accepted with an
-    analytic inlet profile; file-based profiles and the faces the reference
-    path does not handle here are rejected before any device call."""
+    """Inlet (k-, bctype 5), outlet (k+, bctype 4), slip (3) and far field
+    (6) faces are accepted with an analytic inlet profile; file-based
+    profiles, inlets and outlets on other faces, slip and far field on the
+    j- face (whose eta the reference's FormMetrics leaves at 0) and unpaired
+    periodic faces are rejected before any device call."""
     from curvib_old_amd import _lib, cases, serving
     la = 3
-    ok = cases.ChannelCase(cells=(16, 16, 16), bctype=(7, 7, 1, 1, 5, 4), inlet=7)
     walls = cases.ChannelCase(cells=(16, 16, 16), bctype=(7, 7, 1, 1, 1, 1))
-    # Ubcs of the two k planes: 6 * mx * my doubles
-    assert serving.ctx_bytes(ok, la, 30) - serving.ctx_bytes(walls, la, 30) == 6 * 18 * 18 * 8
+    # Ubcs of the six boundary planes: 6 * (my*mz + mx*mz + mx*my) doubles
+    for bc in ((7, 7, 1, 1, 5, 4), (3, 6, 1, 3, 6, 3), (7, 7, 1, 6, 5, 4)):
+        ok = cases.ChannelCase(cells=(16, 16, 16), bctype=bc, inlet=7)
+        assert serving.ctx_bytes(ok, la, 30) - serving.ctx_bytes(walls, la, 30) == 6 * 3 * 18 * 18 * 8, bc
     for bad in (dict(inlet=0), dict(inlet=3), dict(bctype=(7, 7, 1, 1, 4, 5)), dict(bctype=(5, 4, 1, 1, 1, 1)),
-                dict(bctype=(7, 7, 1, 1, 5, 7))):
+                dict(bctype=(7, 7, 1, 1, 5, 7)), dict(bctype=(7, 7, 3, 1, 1, 1)), dict(bctype=(7, 7, 6, 1, 1, 1)),
+                dict(bctype=(7, 7, 1, 2, 1, 1))):
         kw = dict(cells=(16, 16, 16), bctype=(7, 7, 1, 1, 5, 4), inlet=7)
         kw.update(bad)
         assert serving.ctx_bytes(cases.ChannelCase(**kw), la, 30) == -1, bad

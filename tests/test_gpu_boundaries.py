@@ -1,9 +1,10 @@
-"""Inlet (bctype 5 on the k- face) and outlet (bctype 4 on the k+ face) on
-the GPU against the oracle: InflowFlux (bcs.c:99-246) with each analytic
-inlet profile, OutflowFlux (bcs.c:403-469), FormBCS with the outlet flux
-correction (bcs.c:4914-5010) and the Ubcs ghost values, then full
-fractional steps with RK and ImpRK momentum, and a batch that mixes an
-inlet/outlet case with periodic channels.
+"""Boundary faces beyond walls and periodic pairs on the GPU against the
+oracle: inlet (bctype 5 on the k- face), outlet (4 on the k+ face), slip
+(3) and far field (6).  InflowFlux (bcs.c:99-246) with each analytic inlet
+profile, OutflowFlux (bcs.c:403-469), FormBCS with its far-field, slip and
+outlet sections (bcs.c:4500-5010) and the Ubcs ghost values, then full
+fractional steps with RK and ImpRK momentum, and a batch that mixes
+inlet/outlet cases with a periodic channel.
 
 Tolerances: single routines 1e-13 relative (the outlet sums are reduced in
 another order than the oracle's sequential loops); fields after converged
@@ -138,3 +139,57 @@ def test_batch_with_inlet_outlet_matches_alone():
         b.close()
         for u in inb + alone:
             u.close()
+
+
+# slip box (slip on the i faces and j+, k faces); far field on j+ and both k
+# faces; slip and far field beside an inlet and an outlet
+@pytest.mark.parametrize("bctype", [(3, 3, 1, 3, 3, 3), (7, 7, 1, 6, 6, 6), (3, 6, 1, 3, 5, 4)])
+def test_slip_and_farfield_formbcs_match_the_oracle(bctype):
+    case = inout_case(7, bctype=bctype)
+    u, _ = gpu_case(case, mglevels=3)
+    o, _ = oracle_case(case, nlev=3)
+    try:
+        u.Contra2Cart()
+        o.call("o_contra2cart")
+        u.InflowFlux()
+        o.call("o_inflowflux")
+        for rep in range(2):
+            u.FormBCS()
+            o.call("o_formbcs")
+            for f in ("ucont", "ucat"):
+                e = rel(u.get(f), o.get(f))
+                assert e < 1e-13, (rep, f, e)
+    finally:
+        u.close()
+        o.close()
+
+
+@pytest.mark.parametrize("scheme,bctype", [("rk", (7, 7, 1, 3, 7, 7)), ("rk", (7, 7, 1, 6, 5, 4)),
+                                           ("imprk", (3, 3, 1, 6, 6, 6))])
+def test_slip_and_farfield_steps_track_the_oracle(scheme, bctype):
+    """Fractional steps of an open channel (slip top), a far-field top over
+    an inlet/outlet channel, and a slip/far-field box on ImpRK."""
+    case = inout_case(7, bctype=bctype)
+    u, _ = gpu_case(case, mglevels=3)
+    o, _ = oracle_case(case, nlev=3)
+    if scheme == "imprk":
+        imp = dict(imp_max_it=30, imp_atol=1e-7, imp_rtol=1e-4, cfl=0.3)
+        u.set_momentum("imprk", **imp)
+        o.set_momentum("imprk", **imp)
+    u.set_solver(rtol=1e-12, max_it=300)
+    o.set_solver(rtol=1e-12, max_it=300)
+    try:
+        for s in range(3 if scheme == "rk" else 2):
+            st = u.Flow_Solver()
+            o.call("o_flow_step")
+            if scheme == "imprk":
+                so = o.imp_stats()
+                assert st["imp_its"] == int(so["its"]) and st["imp_lambda"] == so["lambda"], (s, st, so)
+            for f in ("ucont", "p", "ucat"):
+                e = rel(u.get(f), o.get(f))
+                assert e < 1e-9, (s, f, e)
+            assert st["maxdiv"] < 1e-6
+    finally:
+        u.close()
+        o.close()
+

@@ -444,3 +444,76 @@ def test_inlet_outlet_step_projects_to_residual_divergence(scheme):
     fl = o.flux_stats()
     assert fl["flux_out"] == pytest.approx(fl["flux_in"], rel=1e-12)
 
+
+
+def test_slip_wall_reflects_the_normal_velocity():
+    """Slip faces (bctype 3): the ghost Ucat mirrors the adjacent cell's
+    velocity in the wall (normal component reversed, tangential kept) and
+    FormBCS zeroes the normal flux; on a uniform grid the normal is the
+    axis.  The j- face is a no-slip wall (slip is not defined there: the
+    reference's eta of that face is 0), its ghost the negated velocity."""
+    cells = (6, 8, 5)
+    case = uniform_case(cells, bctype=(3, 3, 1, 3, 3, 3))
+    o, _ = oracle_for(case)
+    rng = np.random.default_rng(4)
+    mx, my, mz = o.dims(0)[:3]
+    o.put("ucont", rng.uniform(-1, 1, (mz, my, mx, 3)))
+    o.call("o_contra2cart")
+    o.call("o_formbcs")
+    u, uc = o.get("ucat"), o.get("ucont")
+    I = (slice(1, -1),) * 2
+    faces = ((0, np.s_[1:-1, 1:-1, 0], np.s_[1:-1, 1:-1, 1]), (0, np.s_[1:-1, 1:-1, -1], np.s_[1:-1, 1:-1, -2]),
+             (1, np.s_[1:-1, -1, 1:-1], np.s_[1:-1, -2, 1:-1]),
+             (2, np.s_[0, 1:-1, 1:-1], np.s_[1, 1:-1, 1:-1]), (2, np.s_[-1, 1:-1, 1:-1], np.s_[-2, 1:-1, 1:-1]))
+    for d, lo, hi in faces:
+        for q in range(3):
+            sgn = -1.0 if q == d else 1.0
+            np.testing.assert_allclose(u[lo][..., q], sgn * u[hi][..., q], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(u[1:-1, 0, 1:-1], -u[1:-1, 1, 1:-1], rtol=0, atol=1e-14)
+    assert np.abs(uc[:, :, 0, 0][I]).max() == 0 and np.abs(uc[:, :, -2, 0][I]).max() == 0
+    assert np.abs(uc[:, -2, :, 1][I]).max() == 0
+    assert np.abs(uc[0, :, :, 2][I]).max() == 0 and np.abs(uc[-2, :, :, 2][I]).max() == 0
+
+
+def test_farfield_stream_is_a_fixed_point_and_balances_its_flux():
+    """Far-field faces (bctype 6): a uniform stream through far-field k
+    faces beside a far-field j+ face is left unchanged (fluxes, interior
+    and far-field ghosts); a perturbed field leaves FormBCS with the
+    far-field inflow (k-) equal to the outflow (j+ and k+)."""
+    cells = (6, 8, 7)
+    case = uniform_case(cells, bctype=(7, 7, 1, 6, 6, 6))
+    o, _ = oracle_for(case)
+    mx, my, mz = o.dims(0)[:3]
+    dx, dy = 1 / cells[0], 1 / cells[1]
+    U = np.array([0.0, 0.0, 1.5])
+    uc = np.zeros((mz, my, mx, 3))
+    uc[:-1, 1:-1, 1:-1, 2] = U[2] * dx * dy
+    o.put("ucont", uc)
+    o.call("o_contra2cart")
+    o.call("o_formbcs")
+    np.testing.assert_allclose(o.get("ucont")[:-1, :-1, 1:-1], uc[:-1, :-1, 1:-1], rtol=1e-14, atol=1e-16)
+    np.testing.assert_allclose(o.get("ucat")[:, 1:, 1:-1], np.broadcast_to(U, (mz, my - 1, mx - 2, 3)), rtol=1e-14,
+                               atol=1e-15)
+    rng = np.random.default_rng(6)
+    o.put("ucont", uc + rng.uniform(-0.05, 0.05, uc.shape) * dx * dy)
+    o.call("o_contra2cart")
+    o.call("o_formbcs")
+    c = o.get("ucont")
+    fin = c[0, 1:-1, 1:-1, 2].sum()
+    fout = c[1:-1, -2, 1:-1, 1].sum() + c[-2, 1:-1, 1:-1, 2].sum()
+    assert abs(fin - fout) < 1e-13 * abs(fin)
+    fl = o.flux_stats()
+    assert fl["far_in"] != fl["far_out"]      # the sums before the last pass's correction
+
+
+@pytest.mark.parametrize("bctype", [(7, 7, 1, 3, 7, 7), (7, 7, 1, 6, 5, 4), (3, 3, 1, 6, 6, 6)])
+def test_slip_and_farfield_steps_project_to_residual_divergence(bctype):
+    case = cases.ChannelCase(cells=(16, 12, 8), amp=0.1, sphere=(0.5, 0.5, 0.5, 0.4), seed=10, dt=2e-4, re=500.0,
+                             bctype=bctype, inlet=7, inlet_center=(0.0, 0.5, 0.0))
+    o, _ = oracle_for(case, nlev=2)
+    o.set_solver(rtol=1e-12, max_it=400)
+    o.call("o_flow_step")
+    st = o.stats()
+    assert st["its"] > 0
+    assert st["rnorm"] <= 1e-12 * st["bnorm"] * 1.0001
+    assert st["maxdiv"] < 1e-6
