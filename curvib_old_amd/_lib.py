@@ -6,6 +6,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 SO_PATH = os.path.join(_HERE, "libcurvib_hip.so")
 
@@ -41,7 +43,7 @@ SYMBOLS = [
     "cv_ctx_bytes", "cv_server_create", "cv_server_submit", "cv_server_wait", "cv_server_step_ms",
     "cv_server_fetch", "cv_server_release", "cv_server_status", "cv_server_destroy",
     "cv_batch_create", "cv_batch_step", "cv_batch_poisson_solve", "cv_batch_destroy", "cv_server_create_mode",
-    "cv_inflow_flux", "cv_outflow_flux",
+    "cv_inflow_flux", "cv_outflow_flux", "cv_set_ib_nodes", "cv_ibm_interpolation",
 ]
 CV_SERVE_STREAMS, CV_SERVE_BATCHED = 0, 1
 CV_CASE_QUEUED, CV_CASE_RUNNING, CV_CASE_DONE, CV_CASE_FAILED = 0, 1, 2, 3
@@ -77,6 +79,18 @@ class StepStats(C.Structure):
 
 
 CV_MOM_RK, CV_MOM_IMPRK = 0, 1
+
+
+class IbNode(C.Structure):
+    _fields_ = [("i", C.c_int), ("j", C.c_int), ("k", C.c_int), ("i1", C.c_int), ("j1", C.c_int), ("k1", C.c_int),
+                ("i2", C.c_int), ("j2", C.c_int), ("k2", C.c_int), ("i3", C.c_int), ("j3", C.c_int), ("k3", C.c_int),
+                ("cr1", C.c_double), ("cr2", C.c_double), ("cr3", C.c_double), ("d_s", C.c_double),
+                ("d_i", C.c_double), ("ua", C.c_double * 3), ("dun", C.c_double), ("nf", C.c_double * 3)]
+
+
+# the same record as a numpy dtype: 12 ints, then the 12 doubles in field order
+IB_NODE_DTYPE = np.dtype([("idx", np.int32, 12), ("val", np.float64, 12)])
+CV_IB_NOSLIP, CV_IB_FREESLIP = 0, 1
 
 
 class MomentumOpts(C.Structure):
@@ -124,6 +138,8 @@ def load():
         getattr(L, nm).argtypes = [vp]
     L.cv_form_function1.argtypes = [vp, dp]
     L.cv_inflow_flux.argtypes = [vp, dp]
+    L.cv_set_ib_nodes.argtypes = [vp, C.c_void_p, C.c_int, C.c_int]
+    L.cv_ibm_interpolation.argtypes = [vp]
     L.cv_outflow_flux.argtypes = [vp, dp]
     L.cv_volumeflux_rev.argtypes = [vp, dp]
     L.cv_poisson_rhs.argtypes = [vp, dp]

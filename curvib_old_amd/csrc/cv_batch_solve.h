@@ -974,7 +974,7 @@ static int b_finish(cv_batch *B, cv_step_stats *st, double tp);
 static bool b_mom_batchable(const cv_batch *B) {
   if (!B->use_mom || !B->mom) return false;
   for (const cv_ctx *x : B->cs)
-    if (x->mom.scheme != CV_MOM_RK || x->prof.kind != CV_PROF_NONE || x->ubf) return false;
+    if (x->mom.scheme != CV_MOM_RK || x->prof.kind != CV_PROF_NONE || x->ubf || x->ib_n) return false;
   return true;
 }
 
@@ -1084,6 +1084,7 @@ static int b_finish(cv_batch *B, cv_step_stats *st, double tp) {
       CV_CHECK(hipMemcpyAsync(x->red_host + 110, x->dsc + CV_DSC_FLUXIN, 2 * sizeof(double), hipMemcpyDeviceToHost,
                               x->stream));
     if ((rc = cv_divergence_dev(x, &st[c].maxdiv, &st[c].maxdiv_idx))) return rc;
+    if ((rc = cv_ib_dev(x))) return rc;                 // solvers.c:1095-1101
     if (x->inlet || x->outlet) { st[c].flux_in = x->red_host[110]; st[c].flux_out = x->red_host[111]; }
     x->last = st[c];
     x->steps_done++;

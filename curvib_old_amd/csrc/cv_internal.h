@@ -109,6 +109,16 @@ struct cv_ctx {
   int inletprofile;           // -inlet
   double cm_c[3];             // -x_c -y_c -z_c
   double *ubf;                // Ubcs of the six faces (null with walls and periodic faces only)
+  // immersed-boundary nodes (cv_set_ib_nodes) and the face and cell lists
+  // of ibm_interpolation_advanced, built from nvert
+  int ib_n, ib_mode;
+  int *ib_node;               // per node: storage index of the node and of its three stencil cells
+  double *ib_w;               // per node: cr1 cr2 cr3, d_s, d_i, ua xyz, dun, nf xyz (12)
+  double *ib_new;             // per node: the reconstructed Ucat xyz and p (4)
+  int ib_nface[3];            // faces touching an IB node, per direction
+  int *ib_face;               // their cells (upper face), x then y then z
+  int ib_ncell;
+  int *ib_cell;               // the cells those faces bound (Contra2Cart)
 };
 
 // device scalars of the inlet/outlet update (slots of cv_ctx::dsc)
@@ -149,6 +159,8 @@ int cv_contra2cart_dev(cv_ctx *c);
 int cv_ghost_update(cv_ctx *c, int formbcs);
 int cv_inflow_dev(cv_ctx *c);
 int cv_bc_update_dev(cv_ctx *c);   // InflowFlux + FormBCS
+int cv_ib_dev(cv_ctx *c);          // ibm_interpolation_advanced (nothing without IB nodes)
+void cv_ib_clear(cv_ctx *c);
 int cv_poisson_solve_dev(cv_ctx *c, cv_step_stats *st);
 int cv_update_pressure_dev(cv_ctx *c);
 int cv_projection_dev(cv_ctx *c);

@@ -519,6 +519,88 @@ __global__ void k_far_fix(G g, int mask, int has_out, C3 csi, C3 eta, C3 zet, V3
   }
 }
 
+// ------------------------------------------------------------------
+// ibm_interpolation_advanced (ibm.c:3012-3254) with Add_dUndt, no wall
+// function: per IB node the velocity of the interception point (a
+// weighted sum of three cells), the no-slip (wallfunction.c:12-27) or
+// free-slip (wallfunction.c:29-46) reconstruction at the node, and p from
+// the same weights plus the dU_n/dt term.  The new values go to ib_new
+// first: every node reads the Ucat and p of the sweep's start, as the
+// reference reads lUcat and lP.
+// ------------------------------------------------------------------
+__global__ void k_ib_nodes(int n, const int *node, const double *w, int freeslip, double dt, C3 ucat, const double *p,
+                           int with_p, double *out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int *I = node + 4 * q;
+  const double *W = w + 12 * q;
+  const long a1 = I[1], a2 = I[2], a3 = I[3];
+  const double sk1 = W[0], sk2 = W[1], sk3 = W[2];
+  const double sb = W[3], sc = sb + W[4];
+  const double uax = W[5], uay = W[6], uaz = W[7];
+  const double ucx = (ucat.x[a1] * sk1 + ucat.x[a2] * sk2 + ucat.x[a3] * sk3);
+  const double ucy = (ucat.y[a1] * sk1 + ucat.y[a2] * sk2 + ucat.y[a3] * sk3);
+  const double ucz = (ucat.z[a1] * sk1 + ucat.z[a2] * sk2 + ucat.z[a3] * sk3);
+  double bx, by, bz;
+  if (freeslip) {
+    const double nx = W[9], ny = W[10], nz = W[11];
+    const double Uan = uax * nx + uay * ny + uaz * nz;
+    const double Ucn = ucx * nx + ucy * ny + ucz * nz;
+    const double Ubn = Uan + (Ucn - Uan) * sb / sc;
+    bx = ucx - Ucn * nx; by = ucy - Ucn * ny; bz = ucz - Ucn * nz;
+    bx += Ubn * nx; by += Ubn * ny; bz += Ubn * nz;
+  } else {
+    bx = sb / sc * (ucx - uax); by = sb / sc * (ucy - uay); bz = sb / sc * (ucz - uaz);
+    bx += uax; by += uay; bz += uaz;
+  }
+  double *o = out + 4 * q;
+  o[0] = bx; o[1] = by; o[2] = bz;
+  if (with_p) {
+    double pn = (p[a1] * sk1 + p[a2] * sk2 + p[a3] * sk3);
+    pn += W[8] / dt * (sc - sb);
+    o[3] = pn;
+  }
+}
+
+__global__ void k_ib_scatter(int n, const int *node, const double *in, V3 ucat, double *p, int with_p) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const long t = node[4 * q];
+  const double *v = in + 4 * q;
+  ucat.x[t] = v[0]; ucat.y[t] = v[1]; ucat.z[t] = v[2];
+  if (with_p) p[t] = v[3];
+}
+
+// the fluxes of the faces touching an IB node, from the average of the
+// two cells' Ucat (ibm.c:3166-3224); cells listed x faces, then y, then z
+__global__ void k_ib_faces(G g, int nx, int ny, int nz, const int *cell, C3 ucat, C3 csi, C3 eta, C3 zet, V3 ucont) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nx + ny + nz) return;
+  const long t = cell[q];
+  if (q < nx) {
+    const long u = t + 1;
+    const double ux = (ucat.x[t] + ucat.x[u]) * 0.5, uy = (ucat.y[t] + ucat.y[u]) * 0.5, uz = (ucat.z[t] + ucat.z[u]) * 0.5;
+    ucont.x[t] = ux * csi.x[t] + uy * csi.y[t] + uz * csi.z[t];
+  } else if (q < nx + ny) {
+    const long u = t + g.sj;
+    const double ux = (ucat.x[u] + ucat.x[t]) * 0.5, uy = (ucat.y[u] + ucat.y[t]) * 0.5, uz = (ucat.z[u] + ucat.z[t]) * 0.5;
+    ucont.y[t] = ux * eta.x[t] + uy * eta.y[t] + uz * eta.z[t];
+  } else {
+    const long u = t + g.sk;
+    const double ux = (ucat.x[u] + ucat.x[t]) * 0.5, uy = (ucat.y[u] + ucat.y[t]) * 0.5, uz = (ucat.z[u] + ucat.z[t]) * 0.5;
+    ucont.z[t] = ux * zet.x[t] + uy * zet.y[t] + uz * zet.z[t];
+  }
+}
+
+// Contra2Cart of the listed interior cells
+__global__ void k_ib_c2c(G g, int n, const int *cell, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  int i, j, k;
+  g.dec(cell[q], i, j, k);
+  c2c_cell(g, uc, csi, eta, zet, nv, ucat, i, j, k);
+}
+
 // OutflowFlux: the k = mz-2 face flux over the interior, fluid or not
 __global__ void k_outflow_diag(G g, const double *ucz, double *part) {
   __shared__ double sh[BS / 64];
@@ -1339,6 +1421,122 @@ int cv_bc_update_dev(cv_ctx *c) {
   return rc ? rc : cv_ghost_update(c, 1);
 }
 
+// ---- immersed-boundary nodes ----
+void cv_ib_clear(cv_ctx *c) {
+  void *bufs[] = {c->ib_node, c->ib_w, c->ib_new, c->ib_face, c->ib_cell};
+  for (void *b : bufs)
+    if (b) cv_free(c, b);
+  c->ib_node = nullptr; c->ib_w = nullptr; c->ib_new = nullptr; c->ib_face = nullptr; c->ib_cell = nullptr;
+  c->ib_n = 0; c->ib_ncell = 0;
+  c->ib_nface[0] = c->ib_nface[1] = c->ib_nface[2] = 0;
+}
+
+// six sweeps of node reconstruction -> face fluxes -> Contra2Cart.  p is
+// rebuilt in the first sweep only: every sweep of the reference computes
+// it from the same lP, the P at entry.
+int cv_ib_dev(cv_ctx *c) {
+  if (c->ib_n <= 0) return CV_OK;
+  CvLevel &F = c->lev[c->nlev - 1];
+  CvMetrics &M = c->m;
+  const G g = gview(c);
+  const int n = c->ib_n, nf = c->ib_nface[0] + c->ib_nface[1] + c->ib_nface[2];
+  for (int sweep = 0; sweep < 6; sweep++) {
+    const int with_p = sweep == 0;
+    hipLaunchKernelGGL(k_ib_nodes, dim3((n + BS - 1) / BS), dim3(BS), 0, c->stream, n, (const int *)c->ib_node,
+                       (const double *)c->ib_w, c->ib_mode, c->dt, c3(c->ucat), (const double *)c->p, with_p, c->ib_new);
+    hipLaunchKernelGGL(k_ib_scatter, dim3((n + BS - 1) / BS), dim3(BS), 0, c->stream, n, (const int *)c->ib_node,
+                       (const double *)c->ib_new, v3(c->ucat), c->p, with_p);
+    if (nf > 0)
+      hipLaunchKernelGGL(k_ib_faces, dim3((nf + BS - 1) / BS), dim3(BS), 0, c->stream, g, c->ib_nface[0], c->ib_nface[1],
+                         c->ib_nface[2], (const int *)c->ib_face, c3(c->ucat), c3(M.csi), c3(M.eta), c3(M.zet),
+                         v3(c->ucont));
+    if (c->ib_ncell > 0)
+      hipLaunchKernelGGL(k_ib_c2c, dim3((c->ib_ncell + BS - 1) / BS), dim3(BS), 0, c->stream, g, c->ib_ncell,
+                         (const int *)c->ib_cell, c3(c->ucont), c3(M.csi), c3(M.eta), c3(M.zet), (const double *)F.nvert,
+                         v3(c->ucat));
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
+extern "C" int cv_set_ib_nodes(cv_ctx *c, const cv_ib_node *nodes, int n, int mode) {
+  if (!c || n < 0 || (n > 0 && !nodes) || (mode != CV_IB_NOSLIP && mode != CV_IB_FREESLIP)) return CV_ERR_ARG;
+  if (!c->geometry_set || !c->nvert_set) { cv_set_error("cv_set_ib_nodes: geometry and nvert must be set first"); return CV_ERR_STATE; }
+  CV_CHECK(hipSetDevice(c->device));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  cv_ib_clear(c);
+  if (n == 0) return CV_OK;
+  CvLevel &F = c->lev[c->nlev - 1];
+  const int mx = F.d.mx, my = F.d.my, mz = F.d.mz;
+  const long N = F.d.n, sj = mx, sk = (long)mx * my;
+  auto at = [&](int i, int j, int k) { return (long)i + mx * ((long)j + (long)my * k); };
+  auto inside = [&](int i, int j, int k, int lo) { return i >= lo && i < mx - lo && j >= lo && j < my - lo && k >= lo && k < mz - lo; };
+  std::vector<int> node(4 * (size_t)n);
+  std::vector<double> w(12 * (size_t)n);
+  for (int q = 0; q < n; q++) {
+    const cv_ib_node &b = nodes[q];
+    if (!inside(b.i, b.j, b.k, 1) || !inside(b.i1, b.j1, b.k1, 0) || !inside(b.i2, b.j2, b.k2, 0) ||
+        !inside(b.i3, b.j3, b.k3, 0)) {
+      cv_set_error("cv_set_ib_nodes: node %d or its interception cells lie outside the grid", q);
+      return CV_ERR_ARG;
+    }
+    if (!(b.d_s >= 0.0) || !(b.d_s + b.d_i > 0.0)) {
+      cv_set_error("cv_set_ib_nodes: node %d needs d_s >= 0 and d_s + d_i > 0", q);
+      return CV_ERR_ARG;
+    }
+    node[4 * q] = (int)at(b.i, b.j, b.k);
+    node[4 * q + 1] = (int)at(b.i1, b.j1, b.k1);
+    node[4 * q + 2] = (int)at(b.i2, b.j2, b.k2);
+    node[4 * q + 3] = (int)at(b.i3, b.j3, b.k3);
+    const double v[12] = {b.cr1, b.cr2, b.cr3, b.d_s, b.d_i, b.ua[0], b.ua[1], b.ua[2], b.dun, b.nf[0], b.nf[1], b.nf[2]};
+    for (int e = 0; e < 12; e++) w[12 * (size_t)q + e] = v[e];
+  }
+  // faces touching an IB node ((int)(nvert + 0.5) == 1, ibm.c:3166-3224) and
+  // the cells they bound, from the device nvert
+  std::vector<double> nv(N);
+  CV_CHECK(hipMemcpyAsync(nv.data(), F.nvert, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  auto ib = [&](long t) { return (int)(nv[t] + 0.5) == 1; };
+  std::vector<int> fl[3];
+  std::vector<unsigned char> mark(N, 0);
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        const long t = at(i, j, k);
+        const bool b0 = ib(t);
+        if (i < mx - 2 && (b0 || ib(t + 1))) { fl[0].push_back((int)t); mark[t] = mark[t + 1] = 1; }
+        if (j < my - 2 && (b0 || ib(t + sj))) { fl[1].push_back((int)t); mark[t] = mark[t + sj] = 1; }
+        if (k < mz - 2 && (b0 || ib(t + sk))) { fl[2].push_back((int)t); mark[t] = mark[t + sk] = 1; }
+      }
+  std::vector<int> faces, cells;
+  for (int d = 0; d < 3; d++) faces.insert(faces.end(), fl[d].begin(), fl[d].end());
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++)
+        if (mark[at(i, j, k)]) cells.push_back((int)at(i, j, k));
+  void *p;
+  int rc;
+  if ((rc = cv_alloc(c, &p, node.size() * sizeof(int)))) return rc;
+  c->ib_node = (int *)p;
+  if ((rc = cv_alloc(c, &p, w.size() * sizeof(double)))) return rc;
+  c->ib_w = (double *)p;
+  if ((rc = cv_alloc(c, &p, 4 * (size_t)n * sizeof(double)))) return rc;
+  c->ib_new = (double *)p;
+  if ((rc = cv_alloc(c, &p, std::max<size_t>(1, faces.size()) * sizeof(int)))) return rc;
+  c->ib_face = (int *)p;
+  if ((rc = cv_alloc(c, &p, std::max<size_t>(1, cells.size()) * sizeof(int)))) return rc;
+  c->ib_cell = (int *)p;
+  CV_CHECK(hipMemcpy(c->ib_node, node.data(), node.size() * sizeof(int), hipMemcpyHostToDevice));
+  CV_CHECK(hipMemcpy(c->ib_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (!faces.empty()) CV_CHECK(hipMemcpy(c->ib_face, faces.data(), faces.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!cells.empty()) CV_CHECK(hipMemcpy(c->ib_cell, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+  for (int d = 0; d < 3; d++) c->ib_nface[d] = (int)fl[d].size();
+  c->ib_ncell = (int)cells.size();
+  c->ib_n = n;
+  c->ib_mode = mode;
+  return CV_OK;
+}
+
 // rows per face-flux block (4 or 8, CV_RC_ROWS; row 0 is halo); 8 is the
 // faster at 512x256x256
 static int rc_rows() {
@@ -1421,7 +1619,7 @@ int cv_rk_dev(cv_ctx *c) {
   G g = gview(c);
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
   int rc = cv_bc_update_dev(c);
-  if (rc) return rc;
+  if (rc || (rc = cv_ib_dev(c))) return rc;   // implicitsolver.c:47-51
   for (int q = 0; q < 3; q++)
     hipLaunchKernelGGL(k_fill, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, c->rhs[q], 0.0, F.d.n);
   const int wx = wall_mask(c->bctype[0], c->bctype[1]), wy = wall_mask(c->bctype[2], c->bctype[3]),
@@ -1435,6 +1633,7 @@ int cv_rk_dev(cv_ctx *c) {
     hipLaunchKernelGGL(k_wall_flux_zero, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, wx, wy, wz, v3(c->ucont));
     if ((rc = cv_bc_update_dev(c))) return rc;
   }
+  if ((rc = cv_ib_dev(c))) return rc;         // after the stages, implicitsolver.c:87-90
   CV_CHECK(hipGetLastError());
   return CV_OK;
 }
@@ -1478,7 +1677,7 @@ int cv_imprk_dev(cv_ctx *c, cv_step_stats *st) {
   CvLevel &F = c->lev[c->nlev - 1];
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
   int rc = cv_bc_update_dev(c);
-  if (rc) return rc;
+  if (rc || (rc = cv_ib_dev(c))) return rc;   // implicitsolver.c:7128-7131
   for (int q = 0; q < 3; q++)
     hipLaunchKernelGGL(k_fill, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, c->rhs[q], 0.0, F.d.n);
   copy3(c, c->pucont, c->ucont);
@@ -1497,6 +1696,7 @@ int cv_imprk_dev(cv_ctx *c, cv_step_stats *st) {
       if ((rc = cv_bc_update_dev(c))) return rc;
       if ((rc = cv_calcrhs_dev(c))) return rc;
     }
+    if ((rc = cv_ib_dev(c))) return rc;       // after the stages, implicitsolver.c:7170-7173
     normdU1 = normdU_bk;
     if ((rc = absmax3(c, c->ucont, c->pucont, &normdU_bk))) return rc;   // dU = pU - U
     if (pseudot == 1) { normdU0 = normdU_bk; reldU_bk = 1.; }
@@ -1542,6 +1742,14 @@ extern "C" int cv_contra2cart(cv_ctx *c) {
   int rc = ready(c);
   if (rc) return rc;
   rc = cv_contra2cart_dev(c);
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+extern "C" int cv_ibm_interpolation(cv_ctx *c) {
+  int rc = ready(c);
+  if (rc) return rc;
+  rc = cv_ib_dev(c);
   CV_CHECK(hipStreamSynchronize(c->stream));
   return rc;
 }
@@ -1636,6 +1844,7 @@ extern "C" int cv_flow_step(cv_ctx *c, cv_step_stats *st) {
     CV_CHECK(hipMemcpyAsync(c->red_host + 110, c->dsc + CV_DSC_FLUXIN, 2 * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream));
   if ((rc = cv_divergence_dev(c, &s.maxdiv, &s.maxdiv_idx))) return rc;
+  if ((rc = cv_ib_dev(c))) return rc;           // solvers.c:1095-1101
   CV_CHECK(hipStreamSynchronize(c->stream));
   if (c->inlet || c->outlet) { s.flux_in = c->red_host[110]; s.flux_out = c->red_host[111]; }
   float ms[3];

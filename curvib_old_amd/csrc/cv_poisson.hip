@@ -1023,6 +1023,8 @@ int cv_poisson_prepare(cv_ctx *c) {
 extern "C" int cv_set_nvert(cv_ctx *c, const double *nvert) {
   if (!c || !nvert) return CV_ERR_ARG;
   CV_CHECK(hipSetDevice(c->device));
+  CV_CHECK(hipStreamSynchronize(c->stream));
+  cv_ib_clear(c);                      // the IB face and cell lists follow nvert
   int rc = cv_levels_setup_nvert(c, nvert);
   if (rc) return rc;
   c->nvert_set = 1;

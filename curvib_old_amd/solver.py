@@ -137,6 +137,22 @@ class UserCtx:
     def Contra2Cart(self):
         check(_lib.load().cv_contra2cart(self.h), "Contra2Cart")
 
+    def set_ib_nodes(self, idx, val, mode="noslip"):
+        """IB node records (cv_set_ib_nodes): idx (n, 12) ints = i j k, i1 j1 k1,
+        i2 j2 k2, i3 j3 k3 and val (n, 12) doubles = cr1 cr2 cr3, d_s, d_i,
+        ua[3], dun, nf[3], e.g. from ibsearch.sphere_ib_nodes.  mode "noslip"
+        or "freeslip" (the reference's -inv)."""
+        idx = np.asarray(idx).reshape(-1, 12)
+        rec = np.zeros(len(idx), dtype=_lib.IB_NODE_DTYPE)
+        rec["idx"] = idx
+        rec["val"] = np.asarray(val, dtype=np.float64).reshape(-1, 12)
+        m = {"noslip": _lib.CV_IB_NOSLIP, "freeslip": _lib.CV_IB_FREESLIP}[mode]
+        check(_lib.load().cv_set_ib_nodes(self.h, rec.ctypes.data_as(C.c_void_p), len(rec), m), "cv_set_ib_nodes")
+
+    def ibm_interpolation(self):
+        """ibm_interpolation_advanced(user, ibm, ibi, 1) (ibm.c:3012-3254)."""
+        check(_lib.load().cv_ibm_interpolation(self.h), "ibm_interpolation_advanced")
+
     def InflowFlux(self):
         """Inlet fluxes and Ubcs of a bctype-5 k- face; returns FluxInSum."""
         out = C.c_double()

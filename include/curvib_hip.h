@@ -137,6 +137,28 @@ int cv_set_geometry(cv_ctx *ctx, const double *xyz_nodes);
  * (poisson.c:1662-1773). */
 int cv_set_nvert(cv_ctx *ctx, const double *nvert);
 
+/* Immersed-boundary nodes: the output of the immersed-body search
+ * (ibm_search_advanced with InterceptionPoint, ibm.c:1952-2420), one
+ * record per IB node (IBMInfo, variables.h:36-72).  Storage indices (i, j,
+ * k) as in the DMDA arrays.  With nodes set, the momentum scheme and
+ * cv_flow_step run ibm_interpolation_advanced(user, ibm, ibi, 1) where the
+ * reference does (implicitsolver.c:47-51, 87-90, 7128-7131, 7170-7173;
+ * solvers.c:1095-1101). */
+typedef struct {
+  int i, j, k;                  /* the IB node (ni, nj, nk) */
+  int i1, j1, k1, i2, j2, k2, i3, j3, k3;   /* the three cells of the interception triangle */
+  double cr1, cr2, cr3;         /* their interpolation weights */
+  double d_s, d_i;              /* distance to the surface and to the interception point */
+  double ua[3];                 /* surface velocity at the foot point (sum of vertex u * cs) */
+  double dun;                   /* Ua.n - Uaold.n along the node's normal (the dU_n/dt term) */
+  double nf[3];                 /* surface element normal (free-slip reconstruction) */
+} cv_ib_node;
+#define CV_IB_NOSLIP   0        /* noslip (wallfunction.c:12-27) */
+#define CV_IB_FREESLIP 1        /* freeslip, the -inv option (wallfunction.c:29-46) */
+/* n = 0 removes the nodes; cv_set_nvert removes them too (the face and cell
+ * lists follow nvert) */
+int cv_set_ib_nodes(cv_ctx *ctx, const cv_ib_node *nodes, int n, int mode);
+
 /* field exchange (Ucont_P_Binary_Input/Output, main.c:75-467, in memory) */
 int cv_get_field(cv_ctx *ctx, int field, int level, double *out);
 int cv_put_field(cv_ctx *ctx, int field, int level, const double *in);
@@ -161,6 +183,8 @@ int cv_poisson_solve(cv_ctx *ctx, cv_step_stats *st);    /* PoissonSolver_MG   p
 int cv_update_pressure(cv_ctx *ctx);                     /* UpdatePressure     poisson.c:2908-3248 */
 int cv_projection(cv_ctx *ctx);                          /* Projection         poisson.c:2281-2906 */
 int cv_divergence(cv_ctx *ctx, double *maxdiv, long *idx); /* Divergence       main.c:469-593    */
+int cv_ibm_interpolation(cv_ctx *ctx);                   /* ibm_interpolation_advanced(user, ibm, ibi, 1)
+                                                            ibm.c:3012-3254 (nothing without IB nodes) */
 int cv_flow_step(cv_ctx *ctx, cv_step_stats *st);        /* Flow_Solver        solvers.c:944-1156 + main.c:1123-1145;
                                                             momentum by the scheme of cv_set_momentum */
 

@@ -74,6 +74,10 @@ typedef struct {
   int inletprofile; double cm_c[3];
   double flux_in_sum, flux_out_sum, flux_out_diag;
   double far_flux_in, far_flux_out;   /* FarFluxInSum / FarFluxOutSum of FormBCS's last pass */
+  /* immersed-boundary nodes (IBMInfo list, variables.h:36-72): per node 12
+   * ints (i j k, i1 j1 k1, i2 j2 k2, i3 j3 k3) and 12 doubles (cr1 cr2 cr3,
+   * d_s d_i, ua[3], dun, nf[3]); mode 0 noslip, 1 freeslip (-inv) */
+  int ib_n, ib_mode; int *ib_idx; double *ib_val;
   int imp_its; double imp_lambda, imp_normdU, imp_reldU;
   ofield b;           /* Poisson right-hand side */
   /* diagnostics */
@@ -107,6 +111,8 @@ void o_set_inlet(octx *c, int profile, double x_c, double y_c, double z_c);
 double o_inlet_uin(int profile, double xc, double yc, double zc);
 int  o_inlet_profile_ok(int profile);
 void o_inflowflux(octx *c);
+void o_set_ib_nodes(octx *c, int n, const int *idx12, const double *val12, int mode);
+void o_ibm_interpolation(octx *c);
 void o_outflowflux(octx *c);
 void o_formbcs(octx *c);
 void o_formfunction1(octx *c, ofield *rhs);

@@ -173,6 +173,7 @@ void o_destroy(octx *c) {
   ofield *v[] = {&c->ucont, &c->ucont_o, &c->ucat, &c->rhs, &c->ubcs, &c->p, &c->phi,
                  &c->nvert_o, &c->b, &c->ucont_rm1, &c->pucont};
   for (size_t q = 0; q < sizeof(v) / sizeof(v[0]); q++) of_free(v[q]);
+  free(c->ib_idx); free(c->ib_val);
   free(c);
 }
 

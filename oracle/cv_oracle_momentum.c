@@ -503,6 +503,110 @@ void o_ghostnodevelocity(octx *c) {
 }
 
 /* ------------------------------------------------------------------ */
+/* ibm_interpolation_advanced (ibm.c:3012-3254), Add_dUndt = 1, without  */
+/* wall functions: six sweeps of the IB-node reconstruction, the fluxes  */
+/* of the faces touching an IB node, and Contra2Cart                      */
+/* ------------------------------------------------------------------ */
+void o_set_ib_nodes(octx *c, int n, const int *idx12, const double *val12, int mode) {
+  free(c->ib_idx); free(c->ib_val);
+  c->ib_idx = NULL; c->ib_val = NULL; c->ib_n = 0; c->ib_mode = mode;
+  if (n <= 0) return;
+  c->ib_idx = (int *)malloc((size_t)12 * n * sizeof(int));
+  c->ib_val = (double *)malloc((size_t)12 * n * sizeof(double));
+  memcpy(c->ib_idx, idx12, (size_t)12 * n * sizeof(int));
+  memcpy(c->ib_val, val12, (size_t)12 * n * sizeof(double));
+  c->ib_n = n;
+}
+
+/* noslip (wallfunction.c:12-27): linear in the distance from the wall */
+static void ib_noslip(double sc, double sb, ocmp Ua, ocmp Uc, ocmp *Ub) {
+  const double u_c = Uc.x - Ua.x, v_c = Uc.y - Ua.y, w_c = Uc.z - Ua.z;
+  Ub->x = sb / sc * u_c;
+  Ub->y = sb / sc * v_c;
+  Ub->z = sb / sc * w_c;
+  Ub->x += Ua.x;
+  Ub->y += Ua.y;
+  Ub->z += Ua.z;
+}
+
+/* freeslip (wallfunction.c:29-46): the normal component linear in the
+ * distance, the tangential one that of the interception point */
+static void ib_freeslip(double sc, double sb, ocmp Ua, ocmp Uc, ocmp *Ub, double nx, double ny, double nz) {
+  const double Uan = Ua.x * nx + Ua.y * ny + Ua.z * nz;
+  const double Ucn = Uc.x * nx + Uc.y * ny + Uc.z * nz;
+  const double Ubn = Uan + (Ucn - Uan) * sb / sc;
+  const double ut = Uc.x - Ucn * nx, vt = Uc.y - Ucn * ny, wt = Uc.z - Ucn * nz;
+  Ub->x = ut; Ub->y = vt; Ub->z = wt;
+  Ub->x += Ubn * nx;
+  Ub->y += Ubn * ny;
+  Ub->z += Ubn * nz;
+}
+
+void o_ibm_interpolation(octx *c) {
+  if (c->ib_n <= 0) return;
+  olevel *L = &c->lev[c->nlev - 1];
+  const int mx = L->mx, my = L->my, mz = L->mz;
+  const ofield *nv = &L->nvert, *icsi = &L->csi, *jeta = &L->eta, *kzet = &L->zet;
+  ofield *ucat = &c->ucat, *ucont = &c->ucont, *p = &c->p;
+  ofield lucat, lp;
+  of_alloc(&lucat, mx, my, mz, 3);
+  of_alloc(&lp, mx, my, mz, 1);
+  of_copy(&lp, p);                    /* lP: P at entry, refreshed only after the sweeps */
+  const int itr_tmp = 6;
+  for (int tmp = 0; tmp < itr_tmp; tmp++) {
+    of_copy(&lucat, ucat);            /* lUcat: Ucat at the sweep's start */
+    for (int q = 0; q < c->ib_n; q++) {
+      const int *I = c->ib_idx + 12 * q;
+      const double *W = c->ib_val + 12 * q;
+      const int i = I[0], j = I[1], k = I[2];
+      const double sk1 = W[0], sk2 = W[1], sk3 = W[2];
+      const double sb = W[3], sc = sb + W[4];
+      const ocmp Ua = {W[5], W[6], W[7]};
+      const double dun = W[8];
+      const ocmp *a1 = &V(&lucat, I[5], I[4], I[3]), *a2 = &V(&lucat, I[8], I[7], I[6]),
+                 *a3 = &V(&lucat, I[11], I[10], I[9]);
+      ocmp Uc;
+      Uc.x = (a1->x * sk1 + a2->x * sk2 + a3->x * sk3);
+      Uc.y = (a1->y * sk1 + a2->y * sk2 + a3->y * sk3);
+      Uc.z = (a1->z * sk1 + a2->z * sk2 + a3->z * sk3);
+      if (c->ib_mode == 1) ib_freeslip(sc, sb, Ua, Uc, &V(ucat, k, j, i), W[9], W[10], W[11]);
+      else ib_noslip(sc, sb, Ua, Uc, &V(ucat, k, j, i));
+      const double cv1 = S(&lp, I[5], I[4], I[3]), cv2 = S(&lp, I[8], I[7], I[6]), cv3 = S(&lp, I[11], I[10], I[9]);
+      S(p, k, j, i) = (cv1 * sk1 + cv2 * sk2 + cv3 * sk3);
+      S(p, k, j, i) += dun / c->dt * (sc - sb);
+    }
+    o_g2l(ucat, L->per);
+    of_copy(&lucat, ucat);
+    for (int k = 1; k < mz - 1; k++)
+      for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++) {
+          ocmp uc;
+          const ocmp *u0 = &V(&lucat, k, j, i), *ui = &V(&lucat, k, j, i + 1), *uj = &V(&lucat, k, j + 1, i),
+                     *uk = &V(&lucat, k + 1, j, i);
+          const ocmp *ci = &V(icsi, k, j, i), *cj = &V(jeta, k, j, i), *ck = &V(kzet, k, j, i);
+          ocmp *f = &V(ucont, k, j, i);
+          const int ib0 = (int)(S(nv, k, j, i) + 0.5) == 1;
+          if (ib0 || (int)(S(nv, k, j, i + 1) + 0.5) == 1) {
+            uc.x = (u0->x + ui->x) * 0.5; uc.y = (u0->y + ui->y) * 0.5; uc.z = (u0->z + ui->z) * 0.5;
+            if (i < mx - 2) f->x = (uc.x * ci->x + uc.y * ci->y + uc.z * ci->z) * 1.0;
+          }
+          if (ib0 || (int)(S(nv, k, j + 1, i) + 0.5) == 1) {
+            uc.x = (uj->x + u0->x) * 0.5; uc.y = (uj->y + u0->y) * 0.5; uc.z = (uj->z + u0->z) * 0.5;
+            if (j < my - 2) f->y = (uc.x * cj->x + uc.y * cj->y + uc.z * cj->z) * 1.0;
+          }
+          if (ib0 || (int)(S(nv, k + 1, j, i) + 0.5) == 1) {
+            uc.x = (uk->x + u0->x) * 0.5; uc.y = (uk->y + u0->y) * 0.5; uc.z = (uk->z + u0->z) * 0.5;
+            if (k < mz - 2) f->z = (uc.x * ck->x + uc.y * ck->y + uc.z * ck->z) * 1.0;
+          }
+        }
+    o_g2l(ucont, L->per);
+    o_contra2cart(c);
+  }
+  o_g2l(p, L->per);
+  of_free(&lucat); of_free(&lp);
+}
+
+/* ------------------------------------------------------------------ */
 /* Convection (rhs.c:1463-1945), QUICK (central/LES branches omitted)   */
 /* ------------------------------------------------------------------ */
 static inline void quick(double um, double up, const ocmp *a2, const ocmp *a1, const ocmp *a0,
@@ -864,6 +968,7 @@ void o_rungekutta(octx *c) {
   olevel *L = &c->lev[c->nlev - 1];
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
   bc_update(c);
+  o_ibm_interpolation(c);           /* implicitsolver.c:47-51 */
   of_zero(&c->rhs);
   const int mx = L->mx, my = L->my, mz = L->mz;
   for (int istage = 0; istage < 4; istage++) {
@@ -879,6 +984,7 @@ void o_rungekutta(octx *c) {
     o_g2l(&c->ucont, L->per);
     bc_update(c);
   }
+  o_ibm_interpolation(c);           /* after the stages, implicitsolver.c:87-90 */
 }
 
 /* ------------------------------------------------------------------ */
@@ -989,6 +1095,7 @@ void o_imprk(octx *c) {
   olevel *L = &c->lev[c->nlev - 1];
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
   bc_update(c);
+  o_ibm_interpolation(c);           /* implicitsolver.c:7128-7131 */
   of_zero(&c->rhs);
   of_copy(&c->pucont, &c->ucont);
   o_calcrhs(c);
@@ -1012,6 +1119,7 @@ void o_imprk(octx *c) {
       bc_update(c);
       o_calcrhs(c);
     }
+    o_ibm_interpolation(c);         /* after the stages, implicitsolver.c:7170-7173 */
     normdU1 = normdU_bk;
     normdU_bk = absmax3(c, &c->ucont, &c->pucont);   /* dU = pU - U */
     if (pseudot == 1) { normdU0 = normdU_bk; reldU_bk = 1.; }
@@ -1054,6 +1162,7 @@ void o_flow_step(octx *c) {
   o_g2l(&c->p, L->per);
   double t3 = now_s();
   o_divergence(c);
+  o_ibm_interpolation(c);           /* solvers.c:1095-1101 */
   c->t_mom = t1 - t0; c->t_p = t2 - t1; c->t_pr = t3 - t2;
   c->steps_done++;
 }

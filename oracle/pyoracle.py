@@ -68,6 +68,8 @@ def lib():
         L.o_inlet_profile_ok.argtypes = [C.c_int]
         L.o_inlet_profile_ok.restype = C.c_int
         L.o_flux_stats.argtypes = [vp, dp]
+        L.o_set_ib_nodes.argtypes = [vp, C.c_int, ip, dp, C.c_int]
+        L.o_ibm_interpolation.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -165,6 +167,14 @@ class Oracle:
     def set_inlet(self, profile, center=(0.0, 0.0, 0.0)):
         """-inlet profile and the -x_c -y_c -z_c offsets of a bctype-5 inlet."""
         lib().o_set_inlet(self.h, int(profile), float(center[0]), float(center[1]), float(center[2]))
+
+    def set_ib_nodes(self, idx, val, mode=0):
+        """IB node list: idx (n, 12) ints (i j k, i1 j1 k1, i2 j2 k2, i3 j3 k3),
+        val (n, 12) doubles (cr1..3, d_s, d_i, ua[3], dun, nf[3]); mode 0 noslip,
+        1 freeslip."""
+        a = np.ascontiguousarray(idx, dtype=np.int32).reshape(-1, 12)
+        b = np.ascontiguousarray(val, dtype=np.float64).reshape(-1, 12)
+        lib().o_set_ib_nodes(self.h, len(a), a.ctypes.data_as(C.POINTER(C.c_int)), _dp(b), int(mode))
 
     def flux_stats(self):
         """FluxInSum, FormBCS's FluxOutSum, OutflowFlux's sum and the far-field

@@ -95,6 +95,7 @@ def _c_layout(tmp_path):
         "cv_step_stats": ["ksp_its", "maxdiv_idx", "t_mom", "imp_its", "imp_reldU", "flux_in", "flux_out"],
         "cv_solver_opts": ["precision", "rtol", "smoother", "omega"],
         "cv_momentum_opts": ["scheme", "imp_atol", "cfl"],
+        "cv_ib_node": ["i", "k3", "cr1", "d_i", "ua", "dun", "nf"],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for st, fs in fields.items():
@@ -116,12 +117,15 @@ def test_binding_structs_match_the_c_layout(tmp_path):
     from curvib_old_amd import _lib
     fields, lay = _c_layout(tmp_path)
     py = {"cv_grid_desc": _lib.GridDesc, "cv_step_stats": _lib.StepStats, "cv_solver_opts": _lib.SolverOpts,
-          "cv_momentum_opts": _lib.MomentumOpts}
+          "cv_momentum_opts": _lib.MomentumOpts, "cv_ib_node": _lib.IbNode}
     for st, fs in fields.items():
         cls = py[st]
         assert ctypes.sizeof(cls) == lay[(st, "size")], st
         for f in fs:
             assert getattr(cls, f).offset == lay[(st, f)], (st, f)
+    # the numpy record the Python mirror hands to cv_set_ib_nodes
+    assert _lib.IB_NODE_DTYPE.itemsize == lay[("cv_ib_node", "size")]
+    assert _lib.IB_NODE_DTYPE.fields["val"][1] == lay[("cv_ib_node", "cr1")]
 
 
 @pytest.mark.skipif(not os.path.exists(SO), reason="library not built")

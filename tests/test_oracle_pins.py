@@ -517,3 +517,70 @@ def test_slip_and_farfield_steps_project_to_residual_divergence(bctype):
     assert st["its"] > 0
     assert st["rnorm"] <= 1e-12 * st["bnorm"] * 1.0001
     assert st["maxdiv"] < 1e-6
+
+
+def _flat_wall_case():
+    """A uniform periodic channel with an immersed flat wall: solid cells
+    j = 1, 2 (nvert 3.4), IB nodes j = 3 (1.4), fluid above; the wall at
+    y_w = 2 dy, the top of the solid cells.  Each IB node's normal is +y, so
+    its interception point is the centre of the cell above (weight 1),
+    d_s = dy / 2 and d_i = dy."""
+    cells = (6, 10, 5)
+    case = uniform_case(cells)
+    o, _ = oracle_for(case)
+    mx, my, mz = o.dims(0)[:3]
+    dy = 1 / cells[1]
+    nv = np.zeros((mz, my, mx))
+    nv[:, 1:3] = 3.4
+    nv[:, 3] = 1.4
+    o.set_nvert(nv)
+    kk, ii = np.meshgrid(np.arange(1, mz - 1), np.arange(1, mx - 1), indexing="ij")
+    kk, ii = kk.ravel(), ii.ravel()
+    n = len(kk)
+    idx = np.stack([ii, np.full(n, 3), kk, ii, np.full(n, 4), kk, np.minimum(ii + 1, mx - 2), np.full(n, 4), kk,
+                    ii, np.full(n, 4), np.minimum(kk + 1, mz - 2)], axis=1)
+    val = np.zeros((n, 12))
+    val[:, 0] = 1.0
+    val[:, 3], val[:, 4] = 0.5 * dy, dy
+    val[:, 10] = 1.0          # nf = +y
+    return case, o, idx, val, (mx, my, mz), cells
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ib_reconstruction_on_a_flat_wall(mode):
+    """ibm_interpolation_advanced: a shear flow u = s (y - y_w) with a
+    uniform w and a linear p.  No slip reconstructs u = s (y - y_w), w = 0 at
+    the IB nodes (linear in the wall distance, zero on the wall); free slip
+    keeps the interception point's tangential velocity.  p at the nodes is
+    the interception value plus dun / dt (d_i); the faces of the IB cells
+    carry the flux of the averaged velocities."""
+    case, o, idx, val, (mx, my, mz), cells = _flat_wall_case()
+    dx, dy, dz = (1 / c for c in cells)
+    s, w0, dun = 2.0, 0.7, 0.3
+    val[:, 8] = dun
+    yc = (np.arange(my) - 0.5) * dy
+    uc = np.zeros((mz, my, mx, 3))
+    uc[1:-1, 4:-1, :-1, 0] = (s * (yc[4:-1] - 2 * dy) * dy * dz)[None, :, None]
+    uc[:-1, 4:-1, 1:-1, 2] = w0 * dx * dy
+    o.put("ucont", uc)
+    p = np.broadcast_to((1.0 + 3.0 * yc)[None, :, None], (mz, my, mx)).copy()
+    o.put("p", p)
+    o.call("o_contra2cart")
+    o.set_ib_nodes(idx, val, mode)
+    o.call("o_ibm_interpolation")
+    ucat, uco, po = o.get("ucat"), o.get("ucont"), o.get("p")
+    node = ucat[1:-1, 3, 1:-1]
+    if mode == 0:
+        np.testing.assert_allclose(node[..., 0], s * 0.5 * dy, rtol=1e-14)
+        np.testing.assert_allclose(node[..., 2], w0 / 3.0, rtol=1e-14)
+    else:
+        np.testing.assert_allclose(node[..., 0], s * 1.5 * dy, rtol=1e-14)
+        np.testing.assert_allclose(node[..., 2], w0, rtol=1e-14)
+    assert np.abs(node[..., 1]).max() < 1e-15
+    np.testing.assert_allclose(po[1:-1, 3, 1:-1], 1.0 + 3.0 * yc[4] + dun / case.dt * dy, rtol=1e-14)
+    # fluxes: x faces of the IB cells from the node values, y faces through the wall and the IB layer 0
+    np.testing.assert_allclose(uco[1:-1, 3, 1:-2, 0], node[:, :-1, 0] * dy * dz, rtol=1e-13)
+    assert np.abs(uco[1:-1, 2:4, 1:-1, 1]).max() < 1e-15
+    # the fluid above keeps its velocity
+    np.testing.assert_allclose(ucat[1:-1, 4:-1, 1:-1, 0], np.broadcast_to((s * (yc[4:-1] - 2 * dy))[None, :, None],
+                                                                          (mz - 2, my - 5, mx - 2)), rtol=1e-13)
