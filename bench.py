@@ -8,7 +8,9 @@ seeded +-1e-2 perturbations (curvib_old_amd.cases.config_case(2)), with the
 explicit-RK stable dt of that grid.  One step is the reference's
 Flow_Solver (solvers.c:944-1156): 4-stage RK momentum with FormFunction1,
 PoissonRHS, the Poisson solve to relative residual 1e-9, UpdatePressure,
-Projection and Divergence.
+Projection and Divergence, with the immersed-boundary reconstruction
+(ibm_interpolation_advanced) of the sphere's IB nodes where the reference
+runs it with -imm 1 (--no-ib leaves the IB nodes held).
 
 Two contexts run the same steps from the same initial state: the FP64 path
 (FP64 V-cycle) and the mixed path (FP32 V-cycle inside the FP64 FGMRES).
@@ -69,13 +71,15 @@ SOLVER = dict(smoother="yline", omega=0.9, pre_sweeps=3, post_sweeps=3, coarse_s
               restart=30, max_it=200, rtol=1e-9, atol=0.0)
 
 
-def make_ctx(case, arrays, levels, semi, precision, coarsen_auto):
+def make_ctx(case, arrays, levels, semi, precision, coarsen_auto, ib=None):
     from curvib_old_amd import solver
     xyz, nv, uc = arrays
     u = solver.UserCtx(case.nodes, case.bctype, levels, case.re, case.dt, semi=semi, coarsen_auto=coarsen_auto)
     u.set_solver(precision=precision, **SOLVER)
     u.set_grid(xyz)
     u.set_nvert(nv)
+    if ib is not None:
+        u.set_ib_nodes(*ib)
     u.put("ucont", uc)
     return u
 
@@ -140,15 +144,17 @@ def traffic_from_profiles(kernel_tag):
     return float(e["hbm_bytes_per_launch"]) if e and "hbm_bytes_per_launch" in e else None
 
 
-def cpu_baseline(sample_cells, full_cells, threads):
+def cpu_baseline(sample_cells, full_cells, threads, ib=True):
     """One fractional step of the oracle on a reduced version of the
     workload; steps/s at the full size scaled by cell count."""
     if threads:
         os.environ["OMP_NUM_THREADS"] = str(threads)
-    from curvib_old_amd import cases
+    from curvib_old_amd import ibsearch
     import oracle.pyoracle as po
     case = workload_case(sample_cells)
-    o, _ = po.from_case(case, nlev=3)
+    o, (xyz, nv, _) = po.from_case(case, nlev=3)
+    if ib:
+        o.set_ib_nodes(*ibsearch.sphere_ib_nodes(case, xyz, nv), 0)
     o.set_solver(restart=30, max_it=500, rtol=SOLVER["rtol"], atol=0.0, smooth_it=2)
     t0 = time.perf_counter()
     o.call("o_flow_step")
@@ -255,6 +261,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-ib", action="store_true", help="no immersed-boundary reconstruction")
     ap.add_argument("--restart", type=int, default=SOLVER["restart"])
     ap.add_argument("--pre", type=int, default=SOLVER["pre_sweeps"])
     ap.add_argument("--post", type=int, default=SOLVER["post_sweeps"])
@@ -293,10 +300,15 @@ def main():
     case = workload_case(cells)
     t0 = time.perf_counter()
     arrays = cases.build(case)
-    log(f"[setup] case {cells} dt={case.dt:g} built in {time.perf_counter() - t0:.1f}s")
+    ib = None
+    if not args.no_ib:
+        from curvib_old_amd import ibsearch
+        ib = ibsearch.sphere_ib_nodes(case, arrays[0], arrays[1])
+    log(f"[setup] case {cells} dt={case.dt:g} built in {time.perf_counter() - t0:.1f}s, "
+        f"{0 if ib is None else len(ib[0])} IB nodes")
 
-    u64 = make_ctx(case, arrays, args.levels, semi, "fp64", args.coarsen_auto)
-    umx = make_ctx(case, arrays, args.levels, semi, "mixed", args.coarsen_auto)
+    u64 = make_ctx(case, arrays, args.levels, semi, "fp64", args.coarsen_auto, ib)
+    umx = make_ctx(case, arrays, args.levels, semi, "mixed", args.coarsen_auto, ib)
     log(f"[setup] device bytes per context: {u64.device_bytes() / 2**30:.1f} GiB")
 
     t64, its64, _ = timed_steps(u64, "fp64", args.warmup, args.steps)
@@ -342,7 +354,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cells_of(args.cpu_sample), cells, args.cpu_threads)
+        cpu = cpu_baseline(cells_of(args.cpu_sample), cells, args.cpu_threads, ib=not args.no_ib)
 
     out = {
         "metric": METRIC,
@@ -360,7 +372,7 @@ def main():
         "config": {
             "workload": f"warped channel + immersed sphere {'x'.join(map(str, cells))} cells, Re {case.re:g}, "
                         f"dt {case.dt:g}, Poisson rtol {SOLVER['rtol']:g}",
-            "cells": int(ncell), "storage_points": int(npts),
+            "cells": int(ncell), "storage_points": int(npts), "ib_nodes": 0 if ib is None else len(ib[0]),
             "mg_levels": args.levels, "semi_coarsening": list(semi), "coarsen_auto": args.coarsen_auto, **SOLVER,
         },
         "fp64": {"steps_per_s": sps_64, "ms_per_step": 1e3 * t64 / args.steps, "its": its64,
