@@ -178,7 +178,7 @@ def cpu_baseline(sample_cells, full_cells, threads, ib=True):
     }
 
 
-def serving_workloads(n_serve, n_ens, seed, log_workers):
+def serving_workloads(n_serve, n_ens, seed, log_workers, ib=True):
     """Requests of the serving stream (Poisson arrivals, lam 50/s, grids
     32^3..192^3) and of the ensemble (grids 32^3..128x96x96), inputs built
     in spawned worker processes before any GPU work."""
@@ -188,7 +188,7 @@ def serving_workloads(n_serve, n_ens, seed, log_workers):
     arrivals = serving.poisson_arrivals(n_serve, 50.0, rng)
     ens = [serving.sample_case(rng, lo=(32, 32, 32), hi=(128, 96, 96)) for _ in range(n_ens)]
     t0 = time.perf_counter()
-    arrays = serving.build_inputs(serve + ens, workers=log_workers)
+    arrays = serving.build_inputs(serve + ens, workers=log_workers, ib=ib)
     log(f"[serving] built {len(arrays)} case inputs in {time.perf_counter() - t0:.1f}s")
     return list(zip(serve, arrays[:n_serve])), arrivals, list(zip(ens, arrays[n_serve:]))
 
@@ -202,7 +202,7 @@ def batch_ensemble_only(args):
     [serving.sample_case(rng, lo=(32, 32, 32), hi=(192, 192, 192)) for _ in range(args.serving_requests)]
     serving.poisson_arrivals(args.serving_requests, 50.0, rng)
     ens = [serving.sample_case(rng, lo=(32, 32, 32), hi=(128, 96, 96)) for _ in range(args.ensemble_cases)]
-    arrays = serving.build_inputs(ens, workers=args.build_workers)
+    arrays = serving.build_inputs(ens, workers=args.build_workers, ib=not args.no_ib)
     r = serving.run_batched(list(zip(ens, arrays)), args.ensemble_steps, args.batch_ensemble_only, log=log,
                             max_levels=args.ensemble_max_levels, **SOLVER)
     print(json.dumps({"workload": f"{len(ens)}-case ensemble, one cv_batch", **r}), flush=True)
@@ -292,7 +292,8 @@ def main():
 
     serve_in = None
     if not args.no_serving:
-        serve_in = serving_workloads(args.serving_requests, args.ensemble_cases, 2026, args.build_workers)
+        serve_in = serving_workloads(args.serving_requests, args.ensemble_cases, 2026, args.build_workers,
+                                     ib=not args.no_ib)
 
     cells = cells_of(args.cells)
     semi = tuple(int(v) for v in args.semi.split(","))

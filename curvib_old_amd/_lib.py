@@ -43,7 +43,7 @@ SYMBOLS = [
     "cv_ctx_bytes", "cv_server_create", "cv_server_submit", "cv_server_wait", "cv_server_step_ms",
     "cv_server_fetch", "cv_server_release", "cv_server_status", "cv_server_destroy",
     "cv_batch_create", "cv_batch_step", "cv_batch_poisson_solve", "cv_batch_destroy", "cv_server_create_mode",
-    "cv_inflow_flux", "cv_outflow_flux", "cv_set_ib_nodes", "cv_ibm_interpolation",
+    "cv_inflow_flux", "cv_outflow_flux", "cv_set_ib_nodes", "cv_ibm_interpolation", "cv_server_submit_ib",
 ]
 CV_SERVE_STREAMS, CV_SERVE_BATCHED = 0, 1
 CV_CASE_QUEUED, CV_CASE_RUNNING, CV_CASE_DONE, CV_CASE_FAILED = 0, 1, 2, 3
@@ -167,6 +167,8 @@ def load():
     L.cv_server_create_mode.argtypes = [C.c_int, C.c_long, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     L.cv_server_submit.argtypes = [vp, C.POINTER(GridDesc), C.POINTER(SolverOpts), dp, dp, dp, C.c_int,
                                    C.POINTER(C.c_long)]
+    L.cv_server_submit_ib.argtypes = [vp, C.POINTER(GridDesc), C.POINTER(SolverOpts), dp, dp, dp, C.c_int, C.c_void_p,
+                                      C.c_int, C.c_int, C.POINTER(C.c_long)]
     L.cv_server_wait.argtypes = [vp, C.c_long, C.POINTER(CaseInfo)]
     L.cv_server_step_ms.argtypes = [vp, C.c_long, dp, C.POINTER(C.c_int), C.c_int]
     L.cv_server_fetch.argtypes = [vp, C.c_long, C.c_int, dp]

@@ -970,11 +970,11 @@ static int b_finish(cv_batch *B, cv_step_stats *st, double tp);
 
 // every case runs explicit RK with kernel timing off and has only wall and
 // periodic faces: the whole step around the solve can then run on the
-// batch stream, one launch per kernel
+// batch stream, one launch per kernel (IB reconstruction per IB case)
 static bool b_mom_batchable(const cv_batch *B) {
   if (!B->use_mom || !B->mom) return false;
   for (const cv_ctx *x : B->cs)
-    if (x->mom.scheme != CV_MOM_RK || x->prof.kind != CV_PROF_NONE || x->ubf || x->ib_n) return false;
+    if (x->mom.scheme != CV_MOM_RK || x->prof.kind != CV_PROF_NONE || x->ubf) return false;
   return true;
 }
 

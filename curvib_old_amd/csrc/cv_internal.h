@@ -160,6 +160,7 @@ int cv_ghost_update(cv_ctx *c, int formbcs);
 int cv_inflow_dev(cv_ctx *c);
 int cv_bc_update_dev(cv_ctx *c);   // InflowFlux + FormBCS
 int cv_ib_dev(cv_ctx *c);          // ibm_interpolation_advanced (nothing without IB nodes)
+int cv_ib_dev_on(cv_ctx *c, hipStream_t s);   // the same on another stream
 void cv_ib_clear(cv_ctx *c);
 int cv_poisson_solve_dev(cv_ctx *c, cv_step_stats *st);
 int cv_update_pressure_dev(cv_ctx *c);

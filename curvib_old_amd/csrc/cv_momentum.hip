@@ -1434,7 +1434,9 @@ void cv_ib_clear(cv_ctx *c) {
 // six sweeps of node reconstruction -> face fluxes -> Contra2Cart.  p is
 // rebuilt in the first sweep only: every sweep of the reference computes
 // it from the same lP, the P at entry.
-int cv_ib_dev(cv_ctx *c) {
+int cv_ib_dev(cv_ctx *c) { return cv_ib_dev_on(c, c->stream); }
+
+int cv_ib_dev_on(cv_ctx *c, hipStream_t stream) {
   if (c->ib_n <= 0) return CV_OK;
   CvLevel &F = c->lev[c->nlev - 1];
   CvMetrics &M = c->m;
@@ -1442,16 +1444,16 @@ int cv_ib_dev(cv_ctx *c) {
   const int n = c->ib_n, nf = c->ib_nface[0] + c->ib_nface[1] + c->ib_nface[2];
   for (int sweep = 0; sweep < 6; sweep++) {
     const int with_p = sweep == 0;
-    hipLaunchKernelGGL(k_ib_nodes, dim3((n + BS - 1) / BS), dim3(BS), 0, c->stream, n, (const int *)c->ib_node,
+    hipLaunchKernelGGL(k_ib_nodes, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, n, (const int *)c->ib_node,
                        (const double *)c->ib_w, c->ib_mode, c->dt, c3(c->ucat), (const double *)c->p, with_p, c->ib_new);
-    hipLaunchKernelGGL(k_ib_scatter, dim3((n + BS - 1) / BS), dim3(BS), 0, c->stream, n, (const int *)c->ib_node,
+    hipLaunchKernelGGL(k_ib_scatter, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, n, (const int *)c->ib_node,
                        (const double *)c->ib_new, v3(c->ucat), c->p, with_p);
     if (nf > 0)
-      hipLaunchKernelGGL(k_ib_faces, dim3((nf + BS - 1) / BS), dim3(BS), 0, c->stream, g, c->ib_nface[0], c->ib_nface[1],
+      hipLaunchKernelGGL(k_ib_faces, dim3((nf + BS - 1) / BS), dim3(BS), 0, stream, g, c->ib_nface[0], c->ib_nface[1],
                          c->ib_nface[2], (const int *)c->ib_face, c3(c->ucat), c3(M.csi), c3(M.eta), c3(M.zet),
                          v3(c->ucont));
     if (c->ib_ncell > 0)
-      hipLaunchKernelGGL(k_ib_c2c, dim3((c->ib_ncell + BS - 1) / BS), dim3(BS), 0, c->stream, g, c->ib_ncell,
+      hipLaunchKernelGGL(k_ib_c2c, dim3((c->ib_ncell + BS - 1) / BS), dim3(BS), 0, stream, g, c->ib_ncell,
                          (const int *)c->ib_cell, c3(c->ucont), c3(M.csi), c3(M.eta), c3(M.zet), (const double *)F.nvert,
                          v3(c->ucat));
   }
@@ -2111,6 +2113,8 @@ int cv_mom_batch_rk(CvMomBatch *mb, unsigned long long rot) {
   MB_LAUNCH(kbm_vec3, dim3(BS), mb->t_vec, mb->n_vec, mb->all, 1);
   int rc = cv_mom_batch_ghost(mb, 1, 1);
   if (rc) return rc;
+  for (cv_ctx *x : mb->cs)                 // implicitsolver.c:47-51
+    if ((rc = cv_ib_dev_on(x, mb->s))) return rc;
   MB_LAUNCH(kbm_vec3, dim3(BS), mb->t_vec, mb->n_vec, mb->all, 2);
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
   for (int s = 0; s < 4; s++) {
@@ -2122,6 +2126,8 @@ int cv_mom_batch_rk(CvMomBatch *mb, unsigned long long rot) {
     MB_LAUNCH(kbm_wall_flux_zero, dim3(BS), mb->t_plane, mb->n_plane, mb->all);
     if ((rc = cv_mom_batch_ghost(mb, 1, 1))) return rc;
   }
+  for (cv_ctx *x : mb->cs)                 // after the stages, implicitsolver.c:87-90
+    if ((rc = cv_ib_dev_on(x, mb->s))) return rc;
   CV_CHECK(hipGetLastError());
   return CV_OK;
 }

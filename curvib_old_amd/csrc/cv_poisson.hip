@@ -717,7 +717,19 @@ __device__ __forceinline__ void fluid_sum_body(const T *__restrict__ x, const ui
                                       int bid, int nb) {
   __shared__ double sh[BS / 64];
   double s = 0.0;
-  RED_LOOP
+  // four loads in flight per thread; the additions keep the order of a
+  // plain strided loop, so the sum is the same bits
+  const long step = (long)nb * BS;
+  long t = (long)bid * BS + threadIdx.x;
+  for (; t + 3 * step < n; t += 4 * step) {
+    const T x0 = x[t], x1 = x[t + step], x2 = x[t + 2 * step], x3 = x[t + 3 * step];
+    const uint32_t f0 = fl[t], f1 = fl[t + step], f2 = fl[t + 2 * step], f3 = fl[t + 3 * step];
+    if (f0 & 1u) s += (double)x0;
+    if (f1 & 1u) s += (double)x1;
+    if (f2 & 1u) s += (double)x2;
+    if (f3 & 1u) s += (double)x3;
+  }
+  for (; t < n; t += step)
     if (fl[t] & 1u) s += (double)x[t];
   s = cv_block_sum(s, sh);
   if (threadIdx.x == 0) part[bid] = s;

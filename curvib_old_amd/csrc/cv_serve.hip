@@ -36,6 +36,8 @@ struct Case {
   cv_grid_desc g{};
   cv_solver_opts o{};
   std::vector<double> xyz, nv, uc;
+  std::vector<cv_ib_node> ib;   // IB node records (none: IB cells held)
+  int ib_mode = CV_IB_NOSLIP;
   int nsteps = 0;
   long bytes = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
@@ -74,10 +76,12 @@ static int open_case(cv_server *S, Case *cs, cv_ctx **out) {
   if (rc == CV_OK) rc = cv_set_solver(c, &cs->o);
   if (rc == CV_OK) rc = cv_set_geometry(c, cs->xyz.data());
   if (rc == CV_OK) rc = cv_set_nvert(c, cs->nv.data());
+  if (rc == CV_OK && !cs->ib.empty()) rc = cv_set_ib_nodes(c, cs->ib.data(), (int)cs->ib.size(), cs->ib_mode);
   if (rc == CV_OK) rc = cv_put_field(c, CV_F_UCONT, -1, cs->uc.data());
   std::vector<double>().swap(cs->xyz);
   std::vector<double>().swap(cs->nv);
   std::vector<double>().swap(cs->uc);
+  std::vector<cv_ib_node>().swap(cs->ib);
   *out = c;
   return rc;
 }
@@ -323,9 +327,22 @@ extern "C" int cv_server_create(int device, long hbm_budget_bytes, int slots, cv
 
 extern "C" int cv_server_submit(cv_server *S, const cv_grid_desc *g, const cv_solver_opts *o, const double *xyz,
                                 const double *nvert, const double *ucont, int nsteps, long *case_id) {
-  if (!S || !g || !o || !xyz || !nvert || !ucont || nsteps < 1 || !case_id) return CV_ERR_ARG;
-  const long bytes = cv_ctx_bytes(g, o->restart);
+  return cv_server_submit_ib(S, g, o, xyz, nvert, ucont, nsteps, nullptr, 0, CV_IB_NOSLIP, case_id);
+}
+
+// device bytes of the IB node buffers and the face and cell lists: at most
+// 6 faces and 7 cells per node
+static long ib_bytes(int n) { return (long)n * (4 * 4 + 12 * 8 + 4 * 8 + 6 * 4 + 7 * 4); }
+
+extern "C" int cv_server_submit_ib(cv_server *S, const cv_grid_desc *g, const cv_solver_opts *o, const double *xyz,
+                                   const double *nvert, const double *ucont, int nsteps, const cv_ib_node *ib,
+                                   int n_ib, int ib_mode, long *case_id) {
+  if (!S || !g || !o || !xyz || !nvert || !ucont || nsteps < 1 || !case_id || n_ib < 0 || (n_ib > 0 && !ib) ||
+      (ib_mode != CV_IB_NOSLIP && ib_mode != CV_IB_FREESLIP))
+    return CV_ERR_ARG;
+  long bytes = cv_ctx_bytes(g, o->restart);
   if (bytes < 0) return CV_ERR_ARG;
+  bytes += ib_bytes(n_ib);
   if (bytes > S->budget) {
     cv_set_error("cv_server_submit: case needs %ld bytes, budget is %ld", bytes, S->budget);
     return CV_ERR_OOM;
@@ -337,6 +354,8 @@ extern "C" int cv_server_submit(cv_server *S, const cv_grid_desc *g, const cv_so
   cs->xyz.assign(xyz, xyz + 3 * nn);
   cs->nv.assign(nvert, nvert + ns);
   cs->uc.assign(ucont, ucont + 3 * ns);
+  if (n_ib > 0) cs->ib.assign(ib, ib + n_ib);
+  cs->ib_mode = ib_mode;
   cs->nsteps = nsteps;
   cs->bytes = bytes;
   {

@@ -11,7 +11,8 @@ aggregate cell-updates/s.
 Workloads:
   ensemble  n cases submitted at once, grids log-uniform from 32^3 to
             128x96x96 cells, wall amplitude U[0, 0.1], Re U[100, 2000],
-            a sphere of diameter U[0.1, 0.3] in half of the cases
+            a sphere of diameter U[0.1, 0.3] in half of the cases (its IB
+            nodes reconstructed when the inputs carry their records)
   serving   Poisson arrivals (rate lam per second) of cases with grids
             log-uniform from 32^3 to 192^3 cells, geometry and Re as above,
             nsteps fractional steps each, an HBM admission budget
@@ -68,14 +69,33 @@ def _build_arrays(case):
     return cases.build(case)
 
 
-def build_inputs(case_list, workers=8):
+def _build_arrays_ib(case):
+    from . import ibsearch
+    xyz, nv, uc = cases.build(case)
+    ib = ibsearch.sphere_ib_nodes(case, xyz, nv) if case.sphere is not None else None
+    return xyz, nv, uc, ib
+
+
+def build_inputs(case_list, workers=8, ib=False):
     """Node coordinates, nvert and initial fluxes of every case, built in a
-    spawn-started process pool (call before any GPU work in this process)."""
+    spawn-started process pool (call before any GPU work in this process).
+    With ib, a fourth entry per case holds the sphere's IB node records
+    (ibsearch.sphere_ib_nodes; None without a sphere)."""
+    fn = _build_arrays_ib if ib else _build_arrays
     if workers <= 1 or len(case_list) <= 1:
-        return [cases.build(c) for c in case_list]
+        return [fn(c) for c in case_list]
     ctx = mp.get_context("spawn")
     with ctx.Pool(workers) as pool:
-        return pool.map(_build_arrays, case_list, chunksize=1)
+        return pool.map(fn, case_list, chunksize=1)
+
+
+def _ib_records(ib):
+    """(idx, val) arrays as the cv_ib_node records the library takes."""
+    idx, val = ib
+    rec = np.zeros(len(idx), dtype=_lib.IB_NODE_DTYPE)
+    rec["idx"] = np.asarray(idx).reshape(-1, 12)
+    rec["val"] = np.asarray(val, dtype=np.float64).reshape(-1, 12)
+    return rec
 
 
 def grid_desc(case, mglevels, semi=SEMI, coarsen_auto=COARSEN_AUTO):
@@ -125,14 +145,19 @@ class Server:
         self.h = h
 
     def submit(self, case, arrays, nsteps, mglevels=None, semi=SEMI, **solver):
-        xyz, nv, uc = (np.ascontiguousarray(a, dtype=np.float64) for a in arrays)
+        """arrays: (xyz, nvert, ucont) or (xyz, nvert, ucont, ib) with ib the IB
+        node records (idx, val) or None."""
+        xyz, nv, uc = (np.ascontiguousarray(a, dtype=np.float64) for a in arrays[:3])
+        ib = arrays[3] if len(arrays) > 3 else None
         if mglevels is None:
             mglevels = levels_for(case.cells, semi)
         d = grid_desc(case, mglevels, semi)
         o = solver_opts(**solver)
         cid = C.c_long()
-        check(_lib.load().cv_server_submit(self.h, C.byref(d), C.byref(o), _dp(xyz), _dp(nv), _dp(uc),
-                                           int(nsteps), C.byref(cid)), "cv_server_submit")
+        rec = _ib_records(ib) if ib is not None else np.zeros(0, dtype=_lib.IB_NODE_DTYPE)
+        check(_lib.load().cv_server_submit_ib(self.h, C.byref(d), C.byref(o), _dp(xyz), _dp(nv), _dp(uc),
+                                              int(nsteps), rec.ctypes.data_as(C.c_void_p), len(rec),
+                                              _lib.CV_IB_NOSLIP, C.byref(cid)), "cv_server_submit_ib")
         return cid.value
 
     def wait(self, cid):
@@ -245,12 +270,15 @@ def run_batched(requests, nsteps, precision, warmup=1, device=0, log=None, max_l
     ctxs = []
     b = None
     try:
-        for case, (xyz, nv, uc) in requests:
+        for case, arrays in requests:
+            xyz, nv, uc = arrays[:3]
             u = UserCtx(case.nodes, case.bctype, levels_for(case.cells, max_levels=max_levels), case.re, case.dt,
                         semi=SEMI, device=device, coarsen_auto=COARSEN_AUTO, inlet=case.inlet,
                         inlet_center=case.inlet_center)
             u.set_grid(xyz)
             u.set_nvert(nv)
+            if len(arrays) > 3 and arrays[3] is not None:
+                u.set_ib_nodes(*arrays[3])
             u.put("ucont", uc)
             u.set_solver(precision=precision, **solver)
             ctxs.append(u)

@@ -237,6 +237,10 @@ int  cv_server_create(int device, long hbm_budget_bytes, int slots, cv_server **
 int  cv_server_create_mode(int device, long hbm_budget_bytes, int slots, int mode, cv_server **out);
 int  cv_server_submit(cv_server *s, const cv_grid_desc *grid, const cv_solver_opts *opts, const double *xyz_nodes,
                       const double *nvert, const double *ucont, int nsteps, long *case_id);
+/* the same with the case's IB node records (cv_set_ib_nodes), copied at submit */
+int  cv_server_submit_ib(cv_server *s, const cv_grid_desc *grid, const cv_solver_opts *opts, const double *xyz_nodes,
+                         const double *nvert, const double *ucont, int nsteps, const cv_ib_node *ib_nodes,
+                         int n_ib_nodes, int ib_mode, long *case_id);
 int  cv_server_wait(cv_server *s, long case_id, cv_case_info *info);      /* blocks until done or failed */
 int  cv_server_step_ms(cv_server *s, long case_id, double *ms, int *its, int max);  /* returns the count */
 int  cv_server_fetch(cv_server *s, long case_id, int field /* CV_F_P, CV_F_UCONT */, double *out);

@@ -113,7 +113,11 @@ def test_set_nvert_drops_the_ib_nodes():
         u.close()
 
 
-def test_batch_with_ib_matches_alone():
+@pytest.mark.parametrize("mom", ["1", "0"])
+def test_batch_with_ib_matches_alone(monkeypatch, mom):
+    """mom "1": the IB case runs its reconstruction inside the batched
+    momentum; "0": per case around the batched solve."""
+    monkeypatch.setenv("CV_BATCH_MOM", mom)
     solver = dict(rtol=1e-9, max_it=200, smoother="yline", pre_sweeps=3, post_sweeps=3, coarse_sweeps=10, omega=0.9,
                   restart=30)
     sph = cases.ChannelCase(cells=(32, 32, 32), amp=0.08, sphere=(0.5, 0.5, 0.5, 0.4), re=400.0, seed=4, dt=2e-4)

@@ -110,3 +110,42 @@ def test_batched_mode_budget(mix):
     out, st = run(mix, slots=6, budget=budget, nsteps=1, mode="batched")
     assert st["bytes_peak"] <= budget
     assert all(i["steps_done"] == 1 for _, _, i in out)
+
+
+def test_ib_cases_alone_equal_mix_in_both_modes():
+    """Cases with a sphere carry their IB node records through
+    cv_server_submit_ib: each ends with the bits cv_flow_step gives it alone
+    with the same records, streams and batched alike."""
+    from curvib_old_amd import ibsearch
+    from curvib_old_amd.solver import from_case
+    rng = np.random.default_rng(5)
+    cs = []
+    while len(cs) < 4:
+        c = serving.sample_case(rng, lo=(24, 24, 24), hi=(48, 40, 40))
+        if c.sphere is not None or len(cs) % 2:
+            cs.append(c)
+    reqs = []
+    for c in cs:
+        xyz, nv, uc = cases.build(c)
+        ib = ibsearch.sphere_ib_nodes(c, xyz, nv) if c.sphere is not None else None
+        reqs.append((c, (xyz, nv, uc, ib)))
+    assert any(a[3] is not None for _, a in reqs)
+    ref = []
+    for c, (xyz, nv, uc, ib) in reqs:
+        u, _ = from_case(c, mglevels=serving.levels_for(c.cells), semi=serving.SEMI,
+                         coarsen_auto=serving.COARSEN_AUTO)
+        try:
+            u.set_solver(**OPTS)
+            if ib is not None:
+                u.set_ib_nodes(*ib)
+            for _ in range(3):
+                u.Flow_Solver()
+            ref.append((u.get("p"), u.get("ucont")))
+        finally:
+            u.close()
+    for mode in ("streams", "batched"):
+        out, _ = run(reqs, slots=4, mode=mode)
+        for q, ((p, u, info), (pr, ur)) in enumerate(zip(out, ref)):
+            assert info["steps_done"] == 3
+            assert np.array_equal(p, pr), (mode, q)
+            assert np.array_equal(u, ur), (mode, q)
