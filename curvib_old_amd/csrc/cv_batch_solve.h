@@ -318,6 +318,7 @@ struct cv_batch {
   hipStream_t stream = nullptr;
   int n = 0;
   std::vector<cv_ctx *> cs;
+  std::vector<int> ib_gen;            // each case's IB generation at cv_batch_create
   cv_solver_opts opt{};
   int maxlev = 0;
   BCase *dcases = nullptr;
@@ -729,6 +730,7 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
   B->device = dev;
   B->n = ncases;
   B->cs.assign(cases, cases + ncases);
+  for (int c = 0; c < ncases; c++) B->ib_gen.push_back(cases[c]->ib_gen);
   B->opt = o;
   const char *ge = getenv("CV_BATCH_GRAPH");
   B->use_graph = ge ? atoi(ge) != 0 : 1;
@@ -865,6 +867,10 @@ static int b_check(cv_batch *B) {
     cv_ctx *x = B->cs[c];
     if (!same_opts(x->opt, B->opt) || x->krylov_m < B->opt.restart) {
       cv_set_error("cv_batch: solver options of case %d changed since cv_batch_create", c);
+      return CV_ERR_STATE;
+    }
+    if (x->ib_gen != B->ib_gen[c]) {
+      cv_set_error("cv_batch: IB nodes or nvert of case %d changed since cv_batch_create", c);
       return CV_ERR_STATE;
     }
   }

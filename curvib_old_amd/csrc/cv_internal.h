@@ -119,6 +119,7 @@ struct cv_ctx {
   int *ib_face;               // their cells (upper face), x then y then z
   int ib_ncell;
   int *ib_cell;               // the cells those faces bound (Contra2Cart)
+  int ib_gen;                 // bumped whenever the IB buffers are replaced (captured graphs hold them)
 };
 
 // device scalars of the inlet/outlet update (slots of cv_ctx::dsc)

@@ -1402,7 +1402,9 @@ int cv_ghost_update(cv_ctx *c, int formbcs) {
     if (bxper || bzper)
       hipLaunchKernelGGL(k_periodic_planes, dim3(nblk((long)g.my * g.mz + (long)g.mx * g.my)), dim3(BS), 0, c->stream, g,
                          v3(c->ucat), p, nv);
-    hipLaunchKernelGGL(k_edge_average, dim3(1), dim3(BS), 0, c->stream, g, v3(c->ucat), p);
+    // one block runs the twelve ordered sub-sweeps; 1024 threads cover an
+    // edge line of up to 1024 points in one pass per sub-sweep
+    hipLaunchKernelGGL(k_edge_average, dim3(1), dim3(1024), 0, c->stream, g, v3(c->ucat), p);
     if (bxper)
       hipLaunchKernelGGL(k_periodic_full, dim3(nblk((long)g.my * g.mz)), dim3(BS), 0, c->stream, g, 0, formbcs ? 1 : 0,
                          v3(c->ucat), p, nv);
@@ -1429,6 +1431,7 @@ void cv_ib_clear(cv_ctx *c) {
   c->ib_node = nullptr; c->ib_w = nullptr; c->ib_new = nullptr; c->ib_face = nullptr; c->ib_cell = nullptr;
   c->ib_n = 0; c->ib_ncell = 0;
   c->ib_nface[0] = c->ib_nface[1] = c->ib_nface[2] = 0;
+  c->ib_gen++;
 }
 
 // six sweeps of node reconstruction -> face fluxes -> Contra2Cart.  p is

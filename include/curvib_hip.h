@@ -262,7 +262,9 @@ int  cv_server_destroy(cv_server *s);
  * a hipGraph.  Each case's fields are bit-identical to cv_flow_step on that
  * case alone.  stats has one entry per case (in the order given); the call
  * returns CV_ERR_NOT_CONVERGED when any case missed its tolerance.  The
- * contexts must not be destroyed or re-optioned while the batch exists. */
+ * contexts must not be destroyed or re-optioned while the batch exists;
+ * new solver options, IB nodes or nvert make the next batch call return
+ * CV_ERR_STATE. */
 typedef struct cv_batch cv_batch;
 int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out);
 int cv_batch_step(cv_batch *b, cv_step_stats *stats);

@@ -146,3 +146,25 @@ def test_batch_with_ib_matches_alone(monkeypatch, mom):
         b.close()
         for u in inb + alone:
             u.close()
+
+
+def test_batch_refuses_new_ib_nodes_after_create():
+    """A batch's captured graphs hold each case's IB buffers: new IB nodes
+    after cv_batch_create make the next batch call fail with CV_ERR_STATE
+    instead of replaying stale pointers."""
+    from curvib_old_amd import _lib
+    sph = cases.ChannelCase(cells=(32, 32, 32), amp=0.08, sphere=(0.5, 0.5, 0.5, 0.4), re=400.0, seed=4, dt=2e-4)
+    u, _ = gpu_case(sph, mglevels=serving.levels_for(sph.cells), semi=serving.SEMI, coarsen_auto=serving.COARSEN_AUTO)
+    u.set_solver(precision="mixed", rtol=1e-9, max_it=200, smoother="yline", pre_sweeps=3, post_sweeps=3,
+                 coarse_sweeps=10, omega=0.9, restart=30)
+    u.set_ib_nodes(*_nodes(sph))
+    b = Batch([u])
+    try:
+        b.step()
+        u.set_ib_nodes(*_nodes(sph))
+        with pytest.raises(_lib.CvError) as e:
+            b.step()
+        assert e.value.code == _lib.CV_ERR_STATE
+    finally:
+        b.close()
+        u.close()
