@@ -21,6 +21,15 @@
               natural order (dof-3 fields interleaved); a dof-3 field has a
               companion .info file "-vecload_block_size 3".
 
+  nlist/elist the immersed body's triangulated surface in Icem format
+              (ibm_read_Icem, ibm_io.c:3040-3240): nlist%02d holds the
+              vertex count, then one "i i i x y z" line per vertex (the three
+              integers unused; coordinates divided by -char_length_ibm, then
+              offset by -x_c -y_c -z_c and scaled by L_dim); elist%02d holds
+              the element count, then one line of 15 integers per triangle,
+              whose 12th-14th are its 1-based vertex numbers.  Element
+              normals are (p2 - p1) x (p3 - p1), normalised.
+
 The field helpers exchange data with a curvib_old_amd.solver.UserCtx, in
 the layout its get/put use.
 """
@@ -226,3 +235,42 @@ def read_fields(u, ti, bi=0, directory="."):
             raise ValueError(f"{name}: file holds {a.size} values, the grid needs {int(np.prod(shp))}")
         u.put(name, a.reshape(shp))
     return read_petsc_vec(field_path(directory, "nvert", ti, bi)).reshape(u._shape("nvert", -1))
+
+
+# ------------------------------------------------------------ Icem surface
+def write_icem(directory, vertices, triangles, ibi=0):
+    """nlist%02d / elist%02d of a triangulated surface (0-based triangles)."""
+    v = np.asarray(vertices, dtype=np.float64).reshape(-1, 3)
+    t = np.asarray(triangles, dtype=np.int64).reshape(-1, 3)
+    with open(os.path.join(directory, f"nlist{ibi:02d}"), "w") as f:
+        f.write(f"{len(v)}\n")
+        for q, (x, y, z) in enumerate(v):
+            f.write(f"{q + 1} 0 0 {x:.17e} {y:.17e} {z:.17e}\n")
+    with open(os.path.join(directory, f"elist{ibi:02d}"), "w") as f:
+        f.write(f"{len(t)}\n")
+        for q, (a, b, c) in enumerate(t):
+            f.write(f"{q + 1} 0 0 0 0 0 0 0 0 0 0 {a + 1} {b + 1} {c + 1} 0\n")
+
+
+def read_icem(directory, ibi=0, cl=1.0, center=(0.0, 0.0, 0.0), L_dim=1.0):
+    """The surface ibm_read_Icem reads (ibm_io.c:3076-3240): vertices (n_v, 3)
+    after the -char_length_ibm, -x_c/-y_c/-z_c and L_dim transform, 0-based
+    triangles (n_e, 3) and the unit element normals (n_e, 3)."""
+    with open(os.path.join(directory, f"nlist{ibi:02d}")) as f:
+        n_v = int(f.readline().split()[0])
+        rows = [f.readline().split() for _ in range(n_v)]
+    v = np.array([[float(r[3]), float(r[4]), float(r[5])] for r in rows])
+    v = (v / cl + np.asarray(center, dtype=np.float64)) * L_dim
+    with open(os.path.join(directory, f"elist{ibi:02d}")) as f:
+        n_e = int(f.readline().split()[0])
+        rows = [f.readline().split() for _ in range(n_e)]
+    t = np.array([[int(r[11]) - 1, int(r[12]) - 1, int(r[13]) - 1] for r in rows], dtype=np.int64)
+    return v, t, element_normals(v, t)
+
+
+def element_normals(v, t):
+    """Unit normals (p2 - p1) x (p3 - p1) of the triangles (ibm_io.c:3252-3263)."""
+    p1, p2, p3 = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    nf = np.cross(p2 - p1, p3 - p1)
+    return nf / np.linalg.norm(nf, axis=1)[:, None]
+

@@ -73,3 +73,19 @@ def test_bcs_and_control(tmp_path):
     s = formats.solver_options(o)
     assert s == {"rtol": 1e-11, "atol": 1e-5, "max_it": 40, "restart": 30}
     assert formats.solver_options({})["rtol"] == 1e-5          # PETSc KSP default
+
+
+def test_icem_surface_round_trip(tmp_path):
+    """nlist/elist (ibm_read_Icem, ibm_io.c:3076-3240) written and read back:
+    vertices to the last bit, triangles, unit outward normals of a sphere;
+    -char_length_ibm, -x_c.. and L_dim transform the vertices as read."""
+    from curvib_old_amd import ibsearch
+    v, t = ibsearch.icosphere((0.5, 0.4, 0.3), 0.2, 2)
+    formats.write_icem(str(tmp_path), v, t, ibi=1)
+    v2, t2, nf = formats.read_icem(str(tmp_path), ibi=1)
+    assert np.array_equal(v, v2) and np.array_equal(t, t2)
+    np.testing.assert_allclose(np.linalg.norm(nf, axis=1), 1.0, rtol=1e-14)
+    out = v[t].mean(axis=1) - np.array([0.5, 0.4, 0.3])
+    assert (np.einsum("nc,nc->n", nf, out) > 0).all()
+    v3, _, _ = formats.read_icem(str(tmp_path), ibi=1, cl=2.0, center=(1.0, 0.0, 0.0), L_dim=3.0)
+    np.testing.assert_allclose(v3, (v / 2.0 + np.array([1.0, 0.0, 0.0])) * 3.0, rtol=1e-15)

@@ -584,3 +584,4 @@ def test_ib_reconstruction_on_a_flat_wall(mode):
     # the fluid above keeps its velocity
     np.testing.assert_allclose(ucat[1:-1, 4:-1, 1:-1, 0], np.broadcast_to((s * (yc[4:-1] - 2 * dy))[None, :, None],
                                                                           (mz - 2, my - 5, mx - 2)), rtol=1e-13)
+
