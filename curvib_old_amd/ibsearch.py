@@ -247,3 +247,87 @@ def surface_ib_nodes(case: cases.ChannelCase, xyz: np.ndarray, nvert: np.ndarray
     _intercept(cent, ii, jj, kk, p, nrm, idx, val)
     return idx, val
 
+
+
+
+def _pseudo_normals(v, t):
+    """Angle-weighted pseudo-normals of a closed, outward-oriented surface
+    (Baerentzen and Aanaes, IEEE TVCG 11(3), 2005): per facet its unit
+    normal; per facet edge (the edge opposite corner q) the sum of the two
+    facets sharing it; per vertex the sum of the incident facets' normals
+    weighted by their angles there."""
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    fn = _unit(np.cross(b - a, c - a))
+    vn = np.zeros_like(v)
+    for q in range(3):
+        e1 = v[t[:, (q + 1) % 3]] - v[t[:, q]]
+        e2 = v[t[:, (q + 2) % 3]] - v[t[:, q]]
+        ang = np.arccos(np.clip(np.einsum("nc,nc->n", _unit(e1), _unit(e2)), -1.0, 1.0))
+        np.add.at(vn, t[:, q], ang[:, None] * fn)
+    ends = np.stack([np.sort(t[:, [(q + 1) % 3, (q + 2) % 3]], axis=1) for q in range(3)], axis=1)
+    key, inv = np.unique(ends.reshape(-1, 2), axis=0, return_inverse=True)
+    en = np.zeros((len(key), 3))
+    np.add.at(en, inv.ravel(), np.repeat(fn, 3, axis=0))
+    return fn, en[inv.reshape(-1, 3)], vn
+
+
+def surface_nvert(case: cases.ChannelCase, xyz: np.ndarray, vertices, triangles, candidates=16):
+    """nvert of a closed, outward-oriented triangulated body, as
+    ibm_search_advanced (ibm.c:4197) classifies the cells: a cell whose
+    centre lies inside the body is solid (3.4), a non-solid cell with any of
+    its 26 neighbours solid is an IB node (1.4, the rule of ibm.c:4410-4433),
+    else fluid (0); periodic ghost planes repeat the interior.
+
+    Inside or outside: the reference casts a ray and counts facet crossings
+    (point_cell_advanced, ibm.c:1285).  Here the centre is inside when it
+    lies behind the surface at its nearest point, judged against that
+    point's angle-weighted pseudo-normal (facet, edge or vertex, as the
+    nearest point falls); for a closed surface the two tests agree."""
+    from scipy.spatial import cKDTree
+    v = np.asarray(vertices, dtype=np.float64)
+    t = np.asarray(triangles, dtype=np.int64)
+    mx, my, mz = case.storage
+    cent = cases.cell_centres(case, xyz)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    box = np.all((cent >= lo) & (cent <= hi), axis=-1)
+    pts = cent[box]
+    inside = np.zeros(len(pts), dtype=bool)
+    if len(pts):
+        fn, en, vn = _pseudo_normals(v, t)
+        k = min(candidates, len(t))
+        _, cand = cKDTree(v[t].mean(axis=1)).query(pts, k=k)
+        cand = cand.reshape(len(pts), k)
+        best_d = np.full(len(pts), np.inf)
+        best_q = np.zeros((len(pts), 3))
+        best_n = np.zeros((len(pts), 3))
+        for c in range(k):
+            e = cand[:, c]
+            q, w, dd = _closest_points(pts, v[t[e, 0]], v[t[e, 1]], v[t[e, 2]])
+            live = w > 0
+            n_live = live.sum(axis=1)
+            nrm = fn[e]
+            on_edge = n_live == 2
+            nrm = np.where(on_edge[:, None], en[e, np.argmin(live, axis=1)], nrm)
+            corner = t[e, np.argmax(w, axis=1)]
+            nrm = np.where((n_live == 1)[:, None], vn[corner], nrm)
+            better = dd < best_d
+            best_d = np.where(better, dd, best_d)
+            best_q = np.where(better[:, None], q, best_q)
+            best_n = np.where(better[:, None], nrm, best_n)
+        inside = np.einsum("nc,nc->n", pts - best_q, best_n) < 0
+    core = np.zeros((mz, my, mx), dtype=bool)
+    inner = np.zeros(cent.shape[:3], dtype=bool)
+    inner[box] = inside
+    core[1:-1, 1:-1, 1:-1] = inner
+    near = np.zeros_like(core)
+    for dk in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                if dk == dj == di == 0:
+                    continue
+                near[1:-1, 1:-1, 1:-1] |= core[1 + dk:mz - 1 + dk, 1 + dj:my - 1 + dj, 1 + di:mx - 1 + di]
+    nv = np.zeros((mz, my, mx))
+    nv[near & ~core] = 1.4
+    nv[core] = 3.4
+    cases._wrap_periodic_planes(case, nv)
+    return nv

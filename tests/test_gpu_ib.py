@@ -172,18 +172,22 @@ def test_batch_refuses_new_ib_nodes_after_create():
 
 def test_icem_surface_steps_track_the_oracle(tmp_path):
     """A triangulated sphere read from Icem files (nlist/elist) translating
-    rigidly: its IB records (foot points on the facets, surface velocity and
-    dU_n) drive two RK steps on the GPU and on the oracle."""
+    rigidly: nvert from the surface (inside/outside of the facets), its IB
+    records (foot points on the facets, surface velocity and dU_n) drive two
+    RK steps on the GPU and on the oracle."""
     from curvib_old_amd import formats
     vv, tt = ibsearch.icosphere((0.5, 0.5, 0.5), 0.225, 4)
     formats.write_icem(str(tmp_path), vv, tt)
     v, t, _ = formats.read_icem(str(tmp_path))
-    xyz, nv, _ = cases.build(SPHERE)
+    xyz, _, _ = cases.build(SPHERE)
+    nv = ibsearch.surface_nvert(SPHERE, xyz, v, t)
     U = np.tile([0.05, 0.0, -0.02], (len(v), 1))
     idx, val = ibsearch.surface_ib_nodes(SPHERE, xyz, nv, v, t, u=U, uold=0.5 * U)
     u, _ = gpu_case(SPHERE, mglevels=3)
     o, _ = oracle_case(SPHERE, nlev=3)
     try:
+        u.set_nvert(nv)
+        o.set_nvert(nv)
         u.set_ib_nodes(idx, val)
         o.set_ib_nodes(idx, val, 0)
         u.set_solver(rtol=1e-12, max_it=300)
