@@ -269,6 +269,8 @@ def main():
     ap.add_argument("--omega", type=float, default=SOLVER["omega"])
     ap.add_argument("--smoother", default=SOLVER["smoother"], choices=("jacobi", "rb", "yline"))
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--mixed", default="mixed16", choices=("mixed", "mixed16"),
+                    help="precision of the mixed leg: FP32 V-cycle, or with FP16 face coefficients")
     ap.add_argument("--ensemble-max-levels", type=int, default=8)
     ap.add_argument("--batch-ensemble-only", choices=("mixed", "fp64"), default=None,
                     help="run only the batched-ensemble leg in this precision")
@@ -309,11 +311,11 @@ def main():
         f"{0 if ib is None else len(ib[0])} IB nodes")
 
     u64 = make_ctx(case, arrays, args.levels, semi, "fp64", args.coarsen_auto, ib)
-    umx = make_ctx(case, arrays, args.levels, semi, "mixed", args.coarsen_auto, ib)
+    umx = make_ctx(case, arrays, args.levels, semi, args.mixed, args.coarsen_auto, ib)
     log(f"[setup] device bytes per context: {u64.device_bytes() / 2**30:.1f} GiB")
 
     t64, its64, _ = timed_steps(u64, "fp64", args.warmup, args.steps)
-    tmx, itsmx, pr = timed_steps(umx, "mixed", args.warmup, args.steps, prof=("resid", -1, 1))
+    tmx, itsmx, pr = timed_steps(umx, args.mixed, args.warmup, args.steps, prof=("resid", -1, 1))
 
     acc = [] if args.no_accuracy else accuracy_pass(u64, umx, arrays, args.steps)
     # host-array exchange of a drop-in caller that hands Ucont in and takes
@@ -378,8 +380,8 @@ def main():
         },
         "fp64": {"steps_per_s": sps_64, "ms_per_step": 1e3 * t64 / args.steps, "its": its64,
                  "cell_updates_per_s": ncell * sps_64},
-        "mixed": {"steps_per_s": sps_mx, "ms_per_step": 1e3 * tmx / args.steps, "its": itsmx,
-                  "cell_updates_per_s": ncell * sps_mx},
+        "mixed": {"precision": args.mixed, "steps_per_s": sps_mx, "ms_per_step": 1e3 * tmx / args.steps,
+                  "its": itsmx, "cell_updates_per_s": ncell * sps_mx},
         "mixed_over_fp64": sps_mx / sps_64,
         "host_exchange_ms_per_step": xchg_ms,
         "pcie_inclusive_mixed_ms_per_step": 1e3 * tmx / args.steps + xchg_ms,

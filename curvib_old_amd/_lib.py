@@ -20,6 +20,7 @@ CV_ERR_HIP = -5
 CV_ERR_STATE = -6
 CV_FP64 = 0
 CV_MIXED = 1
+CV_MIXED16 = 2
 CV_SMOOTH_JACOBI = 0
 CV_SMOOTH_RB = 1
 CV_SMOOTH_YLINE = 2

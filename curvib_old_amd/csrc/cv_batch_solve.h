@@ -104,25 +104,25 @@ __device__ inline bool bblock(const uint32_t *tab, unsigned long long pm, const 
   if (!bblock(tab, pm, ctl, c, lb)) return;      \
   const BCase &C = cs[c];
 
-template <class T, int MODE>
+template <class T, int MODE, bool HG>
 __global__ void __launch_bounds__(256) kb_stencil(BK_ARGS, int d, int sx, int sb, int sy) {
   BK_HEAD
   const OpV<T> A = bop<T>(C, d);
   const int gx = C.lev[d].sgx, gy = C.lev[d].sgy, jj = ctl->jj;
   const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
-  stencil_tile<T, MODE, 4, true>(A, bvec<T>(C, d, sx, jj), MODE >= 1 ? bvec<T>(C, d, sb, jj) : (const T *)nullptr,
+  stencil_tile<T, MODE, 4, true, HG>(A, bvec<T>(C, d, sx, jj), MODE >= 1 ? bvec<T>(C, d, sb, jj) : (const T *)nullptr,
                                  bvec<T>(C, d, sy, jj), A.dinv, (T)0, C.lev[d].kc, bx, by, bz);
 }
 
 // x += omega T^-1 r in place (zg: x = omega T^-1 r)
-template <class T, int CS>
+template <class T, int CS, bool HL = false>
 __global__ void __launch_bounds__(1024) kb_yline(BK_ARGS, int d, int sr, T omega, int zg) {
   BK_HEAD
   const OpV<T> A = bop<T>(C, d);
   const int gx = C.lev[d].lgx;
   T *x = bvec<T>(C, d, BX, 0);
-  yline_reg_tile<T, CS>(A.mx, A.my, A.mz, bvec<T>(C, d, sr, 0), A.ylo, A.yip, A.ycp, x, x, omega, zg, lb % gx,
-                        lb / gx);
+  yline_reg_tile<T, CS, HL>(A.mx, A.my, A.mz, bvec<T>(C, d, sr, 0), A.ylo, A.yip, A.ycp, x, x,
+                            HL ? (T)(omega * A.ysc) : omega, zg, lb % gx, lb / gx, A.yh);
 }
 
 // r of depth d -> b of depth d+1 (block table of depth d+1)
@@ -377,8 +377,15 @@ template <class T, int MODE>
 static void b_stencil(cv_batch *B, int d, unsigned long long pm, int sx, int sb, int sy) {
   const BTab &t = BPICK(B->st, B->st_v, d, pm);
   if (!BLAUNCH_OK(t, pm)) return;
-  hipLaunchKernelGGL((kb_stencil<T, MODE>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream, (const BCase *)B->dcases,
-                     (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, d, sx, sb, sy);
+  if constexpr (sizeof(T) == 4) {
+    if (B->opt.precision == CV_MIXED16) {
+      hipLaunchKernelGGL((kb_stencil<T, MODE, true>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream,
+                         (const BCase *)B->dcases, (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, d, sx, sb, sy);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((kb_stencil<T, MODE, false>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream,
+                     (const BCase *)B->dcases, (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, d, sx, sb, sy);
 }
 
 template <class T>
@@ -392,6 +399,17 @@ static void b_line(cv_batch *B, int d, unsigned long long pm, int sr, T om, bool
     const BCase *cs = B->dcases;
     const uint32_t *tb = BTAB(t);
     const BCtl *ctl = B->dctl;
+    if constexpr (sizeof(T) == 4) {
+      if (B->opt.precision == CV_MIXED16 && g < 2) {      // FP16 line factors (8 or 16 rows per thread)
+        if (g == 0)
+          hipLaunchKernelGGL((kb_yline<T, 8, true>), dim3((unsigned)t.len), blk, 0, B->stream, cs, tb, m, ctl, d, sr, om,
+                             zg ? 1 : 0);
+        else
+          hipLaunchKernelGGL((kb_yline<T, 16, true>), dim3((unsigned)t.len), blk, 0, B->stream, cs, tb, m, ctl, d, sr, om,
+                             zg ? 1 : 0);
+        continue;
+      }
+    }
     if (g == 0)
       hipLaunchKernelGGL((kb_yline<T, 8>), dim3((unsigned)t.len), blk, 0, B->stream, cs, tb, m, ctl, d, sr, om, zg ? 1 : 0);
     else if (g == 1)
@@ -464,7 +482,7 @@ static void b_vcycle(cv_batch *B, int d) {
 static void b_precond_apply(cv_batch *B) {
   const unsigned long long all = B->all;
   const BTab &v0 = B->vc[0];
-  if (B->opt.precision == CV_MIXED) {
+  if (B->opt.precision != CV_FP64) {
     hipLaunchKernelGGL(kb_conv<float>, dim3((unsigned)v0.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
                        (const uint32_t *)BTAB(v0), all, (const BCtl *)B->dctl, 0);
     b_vcycle<float>(B, 0);
@@ -762,6 +780,11 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
       const int l = x->nlev - 1 - d;
       CvLevel &L = x->lev[l];
       BLev &b = bc.lev[d];
+      if (o.precision == CV_MIXED16 && !L.gh) {
+        cv_set_error("cv_batch_create: case %d has no FP16 coefficients", c);
+        b_release(B);
+        return CV_ERR_STATE;
+      }
       b.Af = opview<float>(x, l);
       b.Ad = opview<double>(x, l);
       b.xf = L.xf; b.bf = L.bf; b.rf = L.rf;
@@ -775,7 +798,7 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
       b.sgx = (L.d.mx + 62) / 63; b.sgy = (L.d.my + 3) / 4; b.sgz = (L.d.mz + b.kc - 1) / b.kc;
       b.lgx = (L.d.mx + 63) / 64;
       const int rows = L.d.my - 2;
-      const int rmax = o.precision == CV_MIXED ? 512 : 256;
+      const int rmax = o.precision != CV_FP64 ? 512 : 256;
       if (rows > rmax || rows < 1) {
         cv_set_error("cv_batch_create: case %d has %d rows per y-line (the batched line solve takes 1..%d)", c, rows, rmax);
         b_release(B);

@@ -192,6 +192,15 @@ static size_t plan_bytes(const CvDims *dims, int nlev, int restart, bool ubf) {
   return b;
 }
 
+// device bytes of the CV_MIXED16 FP16 packs (32 per point and level)
+long cv_half_bytes(const cv_grid_desc *desc) {
+  CvDims dims[CV_MAXLEV];
+  if (!check_desc(desc) || level_dims(desc, dims)) return 0;
+  long b = 0;
+  for (int l = 0; l < desc->mglevels; l++) b += dims[l].n * 32;
+  return b;
+}
+
 extern "C" long cv_ctx_bytes(const cv_grid_desc *desc, int restart) {
   if (!check_desc(desc)) return -1;
   CvDims dims[CV_MAXLEV];

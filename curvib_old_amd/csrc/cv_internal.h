@@ -47,6 +47,10 @@ struct CvLevel {
   uint32_t *flags;
   double *dinv;           // 1/diag of the operator row (smoother)
   float *gf[9], *ajf, *dinvf;  // FP32 copies for the mixed-precision V-cycle
+  void *gh;               // CV_MIXED16 packs (null until first used): n x 16 bytes (g0..g7 *
+                          // 2^hexp in FP16), n x 8 bytes (g8 * 2^hexp in FP16, unused FP16,
+                          // aj * 2^-hexp in FP32), n x 8 bytes (y-line a, ip * 2^yexp, cp in FP16)
+  int hexp, yexp;
   double *ylo, *yip, *ycp;     // y-line smoother: lower coefficient, Thomas 1/pivot and c'
   float *ylof, *yipf, *ycpf;
   double *ltmp; float *ltmpf;  // line-solve forward-sweep scratch
@@ -155,6 +159,7 @@ __global__ void k_axpby(double *y, double a, const double *x, double b, long n);
 
 int cv_metrics_build(cv_ctx *c, const double *xyz_host);
 int cv_levels_setup_nvert(cv_ctx *c, const double *nvert_host);
+long cv_half_bytes(const cv_grid_desc *desc);
 int cv_poisson_prepare(cv_ctx *c);
 int cv_contra2cart_dev(cv_ctx *c);
 int cv_ghost_update(cv_ctx *c, int formbcs);

@@ -103,6 +103,15 @@ template <class T>
 struct OpV {
   const T *g[9];
   const T *aj;
+  // CV_MIXED16: per cell g0..g7 * 2^e in FP16 (16 bytes, one load) and
+  // {g8 * 2^e in FP16, unused FP16, aj * 2^-e in FP32} (8 bytes), so that
+  // aj * 2^-e * sum(g * 2^e ...) = aj * sum(g ...)
+  const uint4 *hA;
+  const uint2 *hB;
+  // CV_MIXED16 y-line factors per cell: {a in FP16 | ip * 2^e in FP16,
+  // cp in FP16 | unused}; ysc = 2^-e undoes the scaling of the solve
+  const uint2 *yh;
+  float ysc;
   const T *dinv;
   const uint32_t *fl;
   const T *ylo, *yip, *ycp;
@@ -252,6 +261,14 @@ __device__ inline T tile_flux_i(const T *M, const T *P, const T *Q, int c, int r
                       g2, Q[o], Q[o + 1], M[o], M[o + 1]);
 }
 
+// FP16 halves of packed coefficient words, widened to FP32 (exact)
+__device__ inline float half_lo(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)); }
+__device__ inline float half_hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
+__device__ inline float half_at(const uint4 &v, int q) {
+  const uint32_t w = q < 2 ? v.x : q < 4 ? v.y : q < 6 ? v.z : v.w;
+  return (q & 1) ? half_hi(w) : half_lo(w);
+}
+
 // operator word and face coefficients of one cell, loaded a plane ahead;
 // in FP32 (where registers allow it) also aj, b and dinv
 template <class T>
@@ -263,9 +280,10 @@ struct StCell {
 
 // TY: rows per block (block 64 x TY); PFB: prefetch the row-0 face below
 // the tile with the cells (keeps the row-0 wave off the barrier's critical
-// path at the cost of registers); remap: XCD-aware tile order
+// path at the cost of registers); HG: face coefficients from the FP16 copy
+// (h, ajh) instead of g, aj; remap: XCD-aware tile order
 // one tile (bx, by, bz) of the stencil: 63 x TY cells of kchunk planes
-template <class T, int MODE, int TY, bool PFB>
+template <class T, int MODE, int TY, bool PFB, bool HG = false>
 __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restrict__ x, const T *__restrict__ b,
                                     T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk, int bx,
                                     int by, int bz, int color = -1) {
@@ -313,6 +331,14 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
     if (e1 < NT) S[e1] = v1;
   };
   constexpr bool PF = sizeof(T) == 4;      // prefetch aj, b, dinv with the cell
+  auto gco = [&](int q, long t) -> T {
+    if constexpr (HG) return q < 8 ? half_at(A.hA[t], q) : half_lo(A.hB[t].x);
+    else return A.g[q][t];
+  };
+  auto ajo = [&](long t) -> T {
+    if constexpr (HG) return __uint_as_float(A.hB[t].y);
+    else return A.aj[t];
+  };
   auto fetch_cell = [&](int gk, StCell<T> &C) {
     C.w = 0u;
 #pragma unroll
@@ -323,16 +349,30 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
     if (!cell) return;
     const long t = tij + pl * gk;
     C.w = A.fl[t];
+    if constexpr (HG) {
+      const uint4 ha = A.hA[t];
+      const uint2 hb = A.hB[t];
 #pragma unroll
-    for (int q = 0; q < 9; q++) C.g[q] = A.g[q][t];
+      for (int q = 0; q < 8; q++) C.g[q] = half_at(ha, q);
+      C.g[8] = half_lo(hb.x);
+      if (out) C.aj = __uint_as_float(hb.y);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) C.g[q] = A.g[q][t];
+    }
     if (PF && out) {
-      C.aj = A.aj[t];
+      if constexpr (!HG) C.aj = A.aj[t];
       if (MODE >= 1) C.b = b[t];
       if (MODE == 2) C.dv = dinv[t];
     }
     if (PFB && row0) {
       C.wb = A.fl[t - mx];
-      C.gb[0] = A.g[3][t - mx]; C.gb[1] = A.g[4][t - mx]; C.gb[2] = A.g[5][t - mx];
+      if constexpr (HG) {
+        const uint4 ha = A.hA[t - mx];
+        C.gb[0] = half_at(ha, 3); C.gb[1] = half_at(ha, 4); C.gb[2] = half_at(ha, 5);
+      } else {
+        C.gb[0] = A.g[3][t - mx]; C.gb[1] = A.g[4][t - mx]; C.gb[2] = A.g[5][t - mx];
+      }
     }
   };
 
@@ -349,7 +389,7 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
   T fk_lo = (T)0;
   if (out && k0 >= 1) {
     const long t = tij + pl * (k0 - 1);
-    fk_lo = tile_flux_k<T>(X[(k0 - 1 + 4) & 3], X[(k0 + 4) & 3], c, r, A.fl[t], A.g[6][t], A.g[7][t], A.g[8][t]);
+    fk_lo = tile_flux_k<T>(X[(k0 - 1 + 4) & 3], X[(k0 + 4) & 3], c, r, A.fl[t], gco(6, t), gco(7, t), gco(8, t));
   }
   for (int gk = k0; gk < k1; gk++) {
     T xq0, xq1;
@@ -369,14 +409,14 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
           fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, cur.wb, cur.gb[0], cur.gb[1], cur.gb[2]);
         } else {
           const long tb = t - mx;
-          fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, A.fl[tb], A.g[3][tb], A.g[4][tb], A.g[5][tb]);
+          fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, A.fl[tb], gco(3, tb), gco(4, tb), gco(5, tb));
         }
       }
     }
     // operands of the output (FP64: loaded here, before the barrier)
     T ajc = cur.aj, bc = cur.b, dvc = cur.dv;
     if (!PF && out) {
-      ajc = A.aj[t];
+      ajc = ajo(t);
       if (MODE >= 1) bc = b[t];
       if (MODE == 2) dvc = dinv[t];
     }
@@ -415,14 +455,14 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
   }
 }
 
-template <class T, int MODE, int TY, bool PFB>
+template <class T, int MODE, int TY, bool PFB, bool HG = false>
 __global__ void __launch_bounds__(64 * TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
                                                    T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk,
                                                    int remap, int color) {
   int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
   if (remap == 1) cv_xcd_tile(bx, by, bz);
   else if (remap > 1) cv_xcd_runs(remap, bx, by, bz);
-  stencil_tile<T, MODE, TY, PFB>(A, x, b, y, dinv, omega, kchunk, bx, by, bz, color);
+  stencil_tile<T, MODE, TY, PFB, HG>(A, x, b, y, dinv, omega, kchunk, bx, by, bz, color);
 }
 
 // ------------------------------------------------------------------
@@ -538,10 +578,13 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
 // lines (bx*64 .. bx*64+63, k) of the register-resident line solve; the
 // chunks beyond the line (blockDim.y larger than the line needs) are
 // identity maps, so the result does not depend on blockDim.y
-template <class T, int CS>
+// HL: the factors from the FP16 pack yh (one 8-byte load per row, cp kept
+// in registers for the backward pass); the solution then carries 2^e and
+// the caller passes omega * 2^-e
+template <class T, int CS, bool HL = false>
 __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
                                       const T *__restrict__ ip, const T *__restrict__ cp, const T *xi, T *xo,
-                                      T omega, int zero_guess, int bx, int bk) {
+                                      T omega, int zero_guess, int bx, int bk, const uint2 *__restrict__ yh = nullptr) {
   __shared__ T SA[16][64], SB[16][64];
   const int ti = threadIdx.x, tc = threadIdx.y, nc = blockDim.y;
   const int i = bx * 64 + ti, k = bk;
@@ -550,15 +593,22 @@ __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *
   const long base = (long)i + (long)mx * my * k;
   const int j0 = 1 + tc * CS;
   const int nrow = act ? max(0, min(CS, my - 1 - j0)) : 0;
-  T av[CS], yv[CS];
+  T av[CS], yv[CS], cv[HL ? CS : 1];
   // forward composite of the chunk
   T Aa = 1, Bb = 0;
 #pragma unroll
   for (int q = 0; q < CS; q++) {
     if (q < nrow) {
       const long t = base + sj * (j0 + q);
-      av[q] = ya[t];
-      yv[q] = r[t] * ip[t];
+      if constexpr (HL) {
+        const uint2 w = yh[t];
+        av[q] = half_lo(w.x);
+        yv[q] = r[t] * half_hi(w.x);
+        cv[q] = half_lo(w.y);
+      } else {
+        av[q] = ya[t];
+        yv[q] = r[t] * ip[t];
+      }
       Aa = av[q] * Aa;
       Bb = av[q] * Bb + yv[q];
     }
@@ -576,7 +626,8 @@ __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *
 #pragma unroll
   for (int q = CS - 1; q >= 0; q--) {
     if (q < nrow) {
-      av[q] = -cp[base + sj * (j0 + q)];
+      if constexpr (HL) av[q] = -cv[q];
+      else av[q] = -cp[base + sj * (j0 + q)];
       Aa = av[q] * Aa;
       Bb = av[q] * Bb + yv[q];
     }
@@ -598,13 +649,14 @@ __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *
   if (tc == nc - 1) xo[base + sj * (my - 1)] = zero_guess ? (T)0 : xi[base + sj * (my - 1)];
 }
 
-template <class T, int CS>
+template <class T, int CS, bool HL = false>
 __global__ void __launch_bounds__(1024) k_yline_reg(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
                                                    const T *__restrict__ ip, const T *__restrict__ cp,
-                                                   const T *xi, T *xo, T omega, int zero_guess) {
+                                                   const T *xi, T *xo, T omega, int zero_guess,
+                                                   const uint2 *__restrict__ yh) {
   int bx, bk, bz;
   cv_xcd_tile(bx, bk, bz);
-  yline_reg_tile<T, CS>(mx, my, mz, r, ya, ip, cp, xi, xo, omega, zero_guess, bx, bk);
+  yline_reg_tile<T, CS, HL>(mx, my, mz, r, ya, ip, cp, xi, xo, omega, zero_guess, bx, bk, yh);
 }
 
 // ------------------------------------------------------------------
@@ -877,6 +929,7 @@ OpV<double> opview<double>(cv_ctx *c, int l) {
   CvLevel &L = c->lev[l];
   OpV<double> A;
   for (int q = 0; q < 9; q++) A.g[q] = L.g[q];
+  A.hA = nullptr; A.hB = nullptr; A.yh = nullptr; A.ysc = 1.0f;
   A.aj = L.aj; A.dinv = L.dinv; A.fl = L.flags;
   A.ylo = L.ylo; A.yip = L.yip; A.ycp = L.ycp;
   A.mx = L.d.mx; A.my = L.d.my; A.mz = L.d.mz; A.xper = L.d.xper; A.zper = L.d.zper;
@@ -887,6 +940,10 @@ OpV<float> opview<float>(cv_ctx *c, int l) {
   CvLevel &L = c->lev[l];
   OpV<float> A;
   for (int q = 0; q < 9; q++) A.g[q] = L.gf[q];
+  A.hA = L.gh ? (const uint4 *)L.gh : nullptr;
+  A.hB = L.gh ? (const uint2 *)((const char *)L.gh + 16 * L.d.n) : nullptr;
+  A.yh = L.gh ? (const uint2 *)((const char *)L.gh + 24 * L.d.n) : nullptr;
+  A.ysc = L.gh ? (float)ldexp(1.0, -L.yexp) : 1.0f;
   A.aj = L.ajf; A.dinv = L.dinvf; A.fl = L.flags;
   A.ylo = L.ylof; A.yip = L.yipf; A.ycp = L.ycpf;
   A.mx = L.d.mx; A.my = L.d.my; A.mz = L.d.mz; A.xper = L.d.xper; A.zper = L.d.zper;
@@ -944,11 +1001,16 @@ static int stencil_remap(unsigned gx) {
 }
 
 // algorithmic bytes per storage point of the stencil modes: x, 9 g, aj (+b)
-// (+dinv), y in T, and the 4-byte operator word
+// (+dinv), y in T, and the 4-byte operator word; hg: the 9 g and aj as the
+// 24-byte FP16 pack
 template <class T>
-static double stencil_bytes_pt(int mode) {
+static double stencil_bytes_pt(int mode, bool hg = false) {
+  if (hg) return (double)(mode == 0 ? 2 : mode == 1 ? 3 : 4) * sizeof(T) + 24.0 + 4.0;
   return (double)(mode == 0 ? 12 : mode == 1 ? 13 : 14) * sizeof(T) + 4.0;
 }
+
+// the FP32 V-cycle reads the FP16 face coefficients
+static bool half_coef(const cv_ctx *c) { return c->opt.precision == CV_MIXED16; }
 
 // MODE 0: y = A x; 1: y = b - A x; 2: y = x + omega D^-1 (b - A x), on the
 // cells of one (i+j+k) parity when color is 0 or 1 (the others keep x)
@@ -960,9 +1022,21 @@ static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T
   const int ty = stencil_rows<T>();
   dim3 g((unsigned)((L.d.mx + 62) / 63), (unsigned)((L.d.my + ty - 1) / ty), (unsigned)((L.d.mz + kc - 1) / kc));
   const int remap = stencil_remap(g.x);
+  const bool hg = sizeof(T) == 4 && half_coef(c) && L.gh;
   CvProfScope ps(c, MODE == 0 ? CV_PROF_APPLY : CV_PROF_RESID, l, sizeof(T) == 4 ? 1 : 0,
-                 (double)L.d.n * stencil_bytes_pt<T>(MODE));
+                 (double)L.d.n * stencil_bytes_pt<T>(MODE, hg));
   const bool pfb = sizeof(T) == 4 || stencil_pfb64();
+  if constexpr (sizeof(T) == 4) {
+    if (hg) {
+      if (ty == 4)
+        hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, true>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega,
+                           kc, remap, color);
+      else
+        hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, true>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega,
+                           kc, remap, color);
+      return;
+    }
+  }
   if (ty == 4 && pfb)
     hipLaunchKernelGGL((k_stencil<T, MODE, 4, true>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
                        color);
@@ -1005,6 +1079,89 @@ __global__ static void k_count_fluid(const uint32_t *fl, long n, double *part) {
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// ---------------- FP16 face coefficients (CV_MIXED16) --------------
+struct G9 { const double *p[9]; };
+// max |g| over the 9 coefficient arrays: non-negative doubles order as
+// their bit patterns, so an unsigned 64-bit max per wave does it
+__global__ static void k_absmax9(G9 g, long n, unsigned long long *out) {
+  double m = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int q = 0; q < 9; q++) m = fmax(m, fabs(g.p[q][t]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+__device__ inline uint32_t half_bits(double v) {
+  return (uint32_t)__builtin_bit_cast(unsigned short, (_Float16)(float)v);
+}
+// the y-line factor word of OpV::yh: (a, ip * s) and (cp, 0)
+__global__ static void k_pack_line(const double *ya, const double *ip, const double *cp, double s, long n, uint2 *yh) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    yh[t] = make_uint2(half_bits(ya[t]) | (half_bits(ip[t] * s) << 16), half_bits(cp[t]));
+}
+__global__ static void k_absmax1(const double *a, long n, unsigned long long *out) {
+  double m = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    m = fmax(m, fabs(a[t]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// the two words per cell of OpV::hA / hB
+__global__ static void k_pack_half(G9 g, const double *aj, double s, long n, uint4 *ha, uint2 *hb) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    uint32_t w[5];
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = half_bits(g.p[2 * q][t] * s) | (half_bits(g.p[2 * q + 1][t] * s) << 16);
+    w[4] = half_bits(g.p[8][t] * s);
+    ha[t] = make_uint4(w[0], w[1], w[2], w[3]);
+    hb[t] = make_uint2(w[4], __float_as_uint((float)(aj[t] / s)));
+  }
+}
+
+// per level: 2^hexp puts max |g| in [2^13, 2^14) (FP16 holds up to 65504),
+// g * 2^hexp in FP16 and aj * 2^-hexp in FP32, exact power-of-two scalings
+static int half_ops(cv_ctx *c) {
+  for (int l = 0; l < c->nlev; l++) {
+    CvLevel &L = c->lev[l];
+    const long n = L.d.n;
+    int rc;
+    void *p;
+    if (!L.gh) {
+      if ((rc = cv_alloc(c, &p, (size_t)32 * n))) return rc;
+      L.gh = p;
+    }
+    G9 g;
+    for (int q = 0; q < 9; q++) g.p[q] = L.g[q];
+    CV_CHECK(hipMemsetAsync(c->part, 0, sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_absmax9, dim3((unsigned)std::min<long>(nblk(n), 1024)), dim3(BS), 0, c->stream, g, n,
+                       (unsigned long long *)c->part);
+    CV_CHECK(hipMemcpyAsync(c->red_host, c->part, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CV_CHECK(hipStreamSynchronize(c->stream));
+    const double gmax = c->red_host[0];
+    int e = 0;
+    if (gmax > 0) frexp(gmax, &e);      // gmax < 2^e
+    L.hexp = gmax > 0 ? 14 - e : 0;
+    const double sc = ldexp(1.0, L.hexp);
+    hipLaunchKernelGGL(k_pack_half, dim3(nblk(n)), dim3(BS), 0, c->stream, g, (const double *)L.aj, sc, n, (uint4 *)L.gh,
+                       (uint2 *)((char *)L.gh + 16 * n));
+    // line factors: 2^yexp puts max |ip| in [2^13, 2^14); |a|, |cp| < 1
+    CV_CHECK(hipMemsetAsync(c->part, 0, sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_absmax1, dim3((unsigned)std::min<long>(nblk(n), 1024)), dim3(BS), 0, c->stream,
+                       (const double *)L.yip, n, (unsigned long long *)c->part);
+    CV_CHECK(hipMemcpyAsync(c->red_host, c->part, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CV_CHECK(hipStreamSynchronize(c->stream));
+    const double ipmax = c->red_host[0];
+    int ey = 0;
+    if (ipmax > 0) frexp(ipmax, &ey);
+    L.yexp = ipmax > 0 ? 14 - ey : 0;
+    hipLaunchKernelGGL(k_pack_line, dim3(nblk(n)), dim3(BS), 0, c->stream, (const double *)L.ylo, (const double *)L.yip,
+                       (const double *)L.ycp, ldexp(1.0, L.yexp), n, (uint2 *)((char *)L.gh + 24 * n));
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
 int cv_poisson_prepare(cv_ctx *c) {
   for (int l = 0; l < c->nlev; l++) {
     CvLevel &L = c->lev[l];
@@ -1029,6 +1186,11 @@ int cv_poisson_prepare(cv_ctx *c) {
     c->nfluid[l] = c->red_host[0];
   }
   CV_CHECK(hipGetLastError());
+  if (half_coef(c)) {
+    const int rc = half_ops(c);
+    if (rc) c->opt.precision = CV_MIXED;     // no FP16 copies: the FP32 V-cycle
+    return rc;
+  }
   return CV_OK;
 }
 
@@ -1094,21 +1256,35 @@ static void line_solve(cv_ctx *c, int l, const T *r, const T *xi, T *xo, T *tmp,
   OpV<T> A = opview<T>(c, l);
   const int rows = L.d.my - 2;
   const dim3 g((unsigned)((L.d.mx + 63) / 64), (unsigned)L.d.mz);
-  CvProfScope ps(c, CV_PROF_LINE, l, sizeof(T) == 4 ? 1 : 0, (double)L.d.n * 6.0 * sizeof(T));
+  // bytes per point: r, the factors (FP16 pack: 8 bytes, else a, ip, cp), x in and out
+  const bool hl = sizeof(T) == 4 && A.yh && rows <= 16 * 16;
+  CvProfScope ps(c, CV_PROF_LINE, l, sizeof(T) == 4 ? 1 : 0, (double)L.d.n * (hl ? 20.0 : 6.0 * sizeof(T)));
   // FP64 keeps at most 16 rows per thread (32 would spill at 1024 threads)
   const int csmax = sizeof(T) == 8 ? 16 : 32;
   if (rows <= 16 * csmax) {
     const int cs = rows <= 16 * 8 ? 8 : rows <= 16 * 16 ? 16 : 32;
     const dim3 b(64, (unsigned)((rows + cs - 1) / cs));
+    if constexpr (sizeof(T) == 4) {
+      if (hl) {
+        const T omh = om * A.ysc;
+        if (cs == 8)
+          hipLaunchKernelGGL((k_yline_reg<T, 8, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
+                             A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
+        else
+          hipLaunchKernelGGL((k_yline_reg<T, 16, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
+                             A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
+        return;
+      }
+    }
     if (cs == 8)
       hipLaunchKernelGGL((k_yline_reg<T, 8>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip, A.ycp, xi,
-                         xo, om, zg ? 1 : 0);
+                         xo, om, zg ? 1 : 0, (const uint2 *)nullptr);
     else if (cs == 16)
       hipLaunchKernelGGL((k_yline_reg<T, 16>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip, A.ycp, xi,
-                         xo, om, zg ? 1 : 0);
+                         xo, om, zg ? 1 : 0, (const uint2 *)nullptr);
     else
       hipLaunchKernelGGL((k_yline_reg<T, sizeof(T) == 8 ? 16 : 32>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r,
-                         A.ylo, A.yip, A.ycp, xi, xo, om, zg ? 1 : 0);
+                         A.ylo, A.yip, A.ycp, xi, xo, om, zg ? 1 : 0, (const uint2 *)nullptr);
   } else {
     hipLaunchKernelGGL(k_yline_solve<T>, g, dim3(64, YL_NC), 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
                        A.ycp, xi, xo, tmp, om, zg ? 1 : 0);
@@ -1180,7 +1356,7 @@ static void precondition(cv_ctx *c, const double *vin, double *zout) {
   const int top = c->nlev - 1;
   CvLevel &F = c->lev[top];
   const long n = F.d.n;
-  if (c->opt.precision == CV_MIXED) {
+  if (c->opt.precision != CV_FP64) {
     MgVecs<float> v = mgvecs<float>(c);
     hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, v.b[top], vin, n);
     vcycle<float>(c, v, top);
@@ -1679,12 +1855,20 @@ static int need_ready(cv_ctx *c) {
 extern "C" int cv_set_solver(cv_ctx *c, const cv_solver_opts *o) {
   if (!c || !o) return CV_ERR_ARG;
   if (o->restart < 1 || o->restart > MAXV - 2 || o->max_it < 1 || o->rtol < 0 || o->pre_sweeps < 0 ||
-      o->post_sweeps < 0 || o->coarse_sweeps < 1 || (o->precision != CV_FP64 && o->precision != CV_MIXED) ||
+      o->post_sweeps < 0 || o->coarse_sweeps < 1 ||
+      (o->precision != CV_FP64 && o->precision != CV_MIXED && o->precision != CV_MIXED16) ||
       o->omega <= 0 || (o->smoother != CV_SMOOTH_JACOBI && o->smoother != CV_SMOOTH_RB && o->smoother != CV_SMOOTH_YLINE)) {
     cv_set_error("cv_set_solver: invalid options");
     return CV_ERR_ARG;
   }
+  const cv_solver_opts prev = c->opt;
   c->opt = *o;
+  if (half_coef(c) && c->geometry_set && c->nvert_set) {
+    CV_CHECK(hipSetDevice(c->device));
+    const int rc = half_ops(c);
+    if (rc) c->opt = prev;
+    return rc;
+  }
   return CV_OK;
 }
 

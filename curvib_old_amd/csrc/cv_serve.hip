@@ -343,6 +343,7 @@ extern "C" int cv_server_submit_ib(cv_server *S, const cv_grid_desc *g, const cv
   long bytes = cv_ctx_bytes(g, o->restart);
   if (bytes < 0) return CV_ERR_ARG;
   bytes += ib_bytes(n_ib);
+  if (o->precision == CV_MIXED16) bytes += cv_half_bytes(g);
   if (bytes > S->budget) {
     cv_set_error("cv_server_submit: case needs %ld bytes, budget is %ld", bytes, S->budget);
     return CV_ERR_OOM;

@@ -79,7 +79,7 @@ class UserCtx:
     def set_solver(self, **kw):
         for k, v in kw.items():
             if k == "precision" and isinstance(v, str):
-                v = {"fp64": _lib.CV_FP64, "mixed": _lib.CV_MIXED}[v]
+                v = {"fp64": _lib.CV_FP64, "mixed": _lib.CV_MIXED, "mixed16": _lib.CV_MIXED16}[v]
             if k == "smoother" and isinstance(v, str):
                 v = {"jacobi": _lib.CV_SMOOTH_JACOBI, "rb": _lib.CV_SMOOTH_RB, "yline": _lib.CV_SMOOTH_YLINE}[v]
             setattr(self.opts, k, v)

@@ -34,6 +34,9 @@ extern "C" {
 /* Poisson preconditioner precision */
 #define CV_FP64   0   /* FP64 V-cycle */
 #define CV_MIXED  1   /* FP32 V-cycle inside the FP64 FGMRES */
+#define CV_MIXED16 2  /* as CV_MIXED, the V-cycle's stencil reading its face
+                         coefficients in FP16 (scaled by a power of two per
+                         level); FGMRES still solves the FP64 operator */
 
 /* multigrid smoothers */
 #define CV_SMOOTH_JACOBI 0    /* damped point Jacobi                     */
@@ -65,7 +68,7 @@ typedef struct {
 
 /* Poisson solver options (control.dat~:69-87 prefix ps_) */
 typedef struct {
-  int precision;           /* CV_FP64 / CV_MIXED */
+  int precision;           /* CV_FP64 / CV_MIXED / CV_MIXED16 */
   int restart;             /* FGMRES restart (-ps_ksp_gmres_restart, default 30) */
   int max_it;              /* -ps_ksp_max_it */
   double rtol, atol;       /* -ps_ksp_rtol, -ps_ksp_atol: stop when ||b-Ax|| <= max(rtol*||b||, atol) */

@@ -36,7 +36,7 @@ def _ctx(case, precision):
 # mom "1": the momentum and the rest of the step batched on the batch stream;
 # "0": per case on each case's stream (graph "1": replayed from per-case graphs)
 @pytest.mark.parametrize("precision,graph,mom", [("mixed", "1", "1"), ("fp64", "1", "1"), ("mixed", "1", "0"),
-                                                 ("mixed", "0", "0")])
+                                                 ("mixed", "0", "0"), ("mixed16", "1", "1")])
 def test_batch_matches_each_case_alone(monkeypatch, precision, graph, mom):
     monkeypatch.setenv("CV_BATCH_GRAPH", graph)
     monkeypatch.setenv("CV_BATCH_MOM", mom)

@@ -4,7 +4,8 @@ the not-converged status.
 
 Tolerances: FP64 routine outputs 1e-12 relative; fields after converged
 steps (both sides solved to rtol 1e-12) 1e-9 relative L2; mixed against
-FP64 pressure at rtol 1e-9: 1e-8 relative L2."""
+FP64 pressure at rtol 1e-9: 1e-8 relative L2 (also with the FP16
+coefficient V-cycle)."""
 import numpy as np
 import pytest
 
@@ -65,7 +66,11 @@ def test_three_steps_track_the_oracle():
         o.close()
 
 
-def test_mixed_pressure_matches_fp64():
+@pytest.mark.parametrize("precision", ["mixed", "mixed16"])
+def test_mixed_pressure_matches_fp64(precision):
+    """mixed16: the V-cycle's stencil reads FP16 face coefficients; the
+    FP64 FGMRES still drives the FP64 residual to rtol, so the pressure
+    meets the same bound, in at most two more iterations per step."""
     case = cases.ChannelCase(cells=(64, 48, 32), amp=0.1, beta=2.0, sphere=(0.5, 0.5, 0.5, 0.25), re=500.0,
                              seed=23)
     case = cases.ChannelCase(**{**case.__dict__, "dt": cases.stable_dt(case)})
@@ -73,16 +78,44 @@ def test_mixed_pressure_matches_fp64():
     a, _ = gpu_case(case, mglevels=4, semi=(0, 1, 0), coarsen_auto=1)
     b, _ = gpu_case(case, mglevels=4, semi=(0, 1, 0), coarsen_auto=1)
     a.set_solver(precision="fp64", **opts)
-    b.set_solver(precision="mixed", **opts)
+    b.set_solver(precision=precision, **opts)
     try:
         for s in range(3):
             sa, sb = a.Flow_Solver(), b.Flow_Solver()
             assert sa["ksp_rnorm"] <= 1e-9 * sa["ksp_bnorm"] and sb["ksp_rnorm"] <= 1e-9 * sb["ksp_bnorm"]
+            assert sb["ksp_its"] <= sa["ksp_its"] + 2, (s, sa["ksp_its"], sb["ksp_its"])
             e = rel(b.get("p"), a.get("p"))
             assert e < 1e-8, (s, e)
     finally:
         a.close()
         b.close()
+
+
+def test_mixed16_takes_its_own_path_and_follows_nvert():
+    """The FP16 coefficients change the preconditioner (a residual history
+    unlike mixed's); they are made when the option is set and again when
+    nvert is set, so both orders give the same solve."""
+    case = cases.ChannelCase(cells=(48, 32, 32), amp=0.1, beta=2.0, sphere=(0.5, 0.5, 0.5, 0.3), re=500.0, seed=24,
+                             dt=2e-4)
+    opts = dict(rtol=1e-9, max_it=200, smoother="yline", pre_sweeps=3, post_sweeps=3, omega=0.9)
+    ctx = [gpu_case(case, mglevels=4, semi=(0, 1, 0), coarsen_auto=1) for _ in range(3)]
+    (m, _), (h1, _), (h2, (_, nv, _)) = ctx
+    try:
+        m.set_solver(precision="mixed", **opts)
+        h1.set_solver(precision="mixed16", **opts)
+        h2.set_solver(precision="mixed16", **opts)
+        h2.set_nvert(nv)
+        for u in (m, h1, h2):
+            u.FormBCS()
+        sm, s1, s2 = m.PoissonSolver_MG(), h1.PoissonSolver_MG(), h2.PoissonSolver_MG()
+        assert sm["converged"] == s1["converged"] == s2["converged"] == 1
+        assert s1["ksp_rnorm"] != sm["ksp_rnorm"]
+        assert s1["ksp_its"] == s2["ksp_its"] and s1["ksp_rnorm"] == s2["ksp_rnorm"]
+        assert np.array_equal(h1.get("phi"), h2.get("phi"))
+        assert rel(h1.get("phi"), m.get("phi")) < 1e-7
+    finally:
+        for u, _ in ctx:
+            u.close()
 
 
 @pytest.mark.parametrize("smoother", ["jacobi", "rb", "yline"])
