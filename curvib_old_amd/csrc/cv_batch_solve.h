@@ -27,6 +27,7 @@ struct BLev {
   float *xf, *bf, *rf;
   double *xd, *bd, *rd;
   const double *nv;
+  const float *nwf;     // max(0, 1 - nvert) in FP32 (restriction weights)
   const uint32_t *fl;
   Idx I;
   int isc, jsc, ksc;
@@ -130,8 +131,8 @@ template <class T>
 __global__ void kb_restrict(BK_ARGS, int d) {
   BK_HEAD
   const BLev &F = C.lev[d], &K = C.lev[d + 1];
-  restrict_body<T>(K.I, F.I, K.isc, K.jsc, K.ksc, F.nv, K.nv, (const T *)bvec<T>(C, d, BR, 0), bvec<T>(C, d + 1, BB, 0),
-                   K.n, (long)lb * BS + threadIdx.x, (long)K.nvb * BS);
+  restrict_body<T>(K.I, F.I, K.isc, K.jsc, K.ksc, F.nv, F.nwf, K.fl, (const T *)bvec<T>(C, d, BR, 0),
+                   bvec<T>(C, d + 1, BB, 0), K.n, (long)lb * BS + threadIdx.x, (long)K.nvb * BS);
 }
 
 // x of depth d += P x of depth d+1 (block table of depth d)
@@ -139,7 +140,7 @@ template <class T>
 __global__ void kb_prolong(BK_ARGS, int d) {
   BK_HEAD
   const BLev &F = C.lev[d], &K = C.lev[d + 1];
-  prolong_body<T>(F.I, K.I, K.isc, K.jsc, K.ksc, K.nv, F.nv, (const T *)bvec<T>(C, d + 1, BX, 0), bvec<T>(C, d, BX, 0),
+  prolong_body<T>(F.I, K.I, K.isc, K.jsc, K.ksc, K.nv, F.fl, (const T *)bvec<T>(C, d + 1, BX, 0), bvec<T>(C, d, BX, 0),
                   1, F.n, (long)lb * BS + threadIdx.x, (long)F.nvb * BS);
 }
 
@@ -789,7 +790,7 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
       b.Ad = opview<double>(x, l);
       b.xf = L.xf; b.bf = L.bf; b.rf = L.rf;
       b.xd = L.x; b.bd = L.b; b.rd = L.r;
-      b.nv = L.nvert; b.fl = L.flags;
+      b.nv = L.nvert; b.nwf = L.nwf; b.fl = L.flags;
       b.I = idx_of(L);
       b.isc = L.d.isc; b.jsc = L.d.jsc; b.ksc = L.d.ksc;
       b.n = L.d.n;

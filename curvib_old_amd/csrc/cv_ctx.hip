@@ -182,7 +182,7 @@ static size_t plan_bytes(const CvDims *dims, int nlev, int restart, bool ubf) {
   size_t b = 0;
   const CvDims &F = dims[nlev - 1];
   if (ubf) b += (size_t)3 * 2 * ((size_t)F.my * F.mz + (size_t)F.mx * F.mz + (size_t)F.mx * F.my) * 8;
-  for (int l = 0; l < nlev; l++) b += (size_t)dims[l].n * (22 * 8 + 18 * 4 + 4);
+  for (int l = 0; l < nlev; l++) b += (size_t)dims[l].n * (22 * 8 + 19 * 4 + 4);
   const size_t n = (size_t)dims[nlev - 1].n;
   b += n * 8 * (40 + 25 + 3);
   b += (size_t)(1 << 16) * 8 + 128 * 8 + 34 * 1024 * 8;
@@ -256,6 +256,7 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     for (int q = 0; q < 9; q++) { if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.gf[q] = (float *)p; }
     if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.ajf = (float *)p;
     if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.dinvf = (float *)p;
+    if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.nwf = (float *)p;
     if ((rc = cv_alloc(c, &p, 3 * n * sizeof(double)))) goto fail; L.coor = (double *)p;
     if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.ylo = (double *)p;
     if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail; L.yip = (double *)p;

@@ -42,6 +42,7 @@ struct CvDims {
 struct CvLevel {
   CvDims d;
   double *nvert;          // storage nvert (level nvert hierarchy)
+  float *nwf;             // max(0, 1 - nvert) in FP32: restriction weights of the FP32 V-cycle
   double *aj;             // cell 1/J
   double *g[9];           // g11 g12 g13 (i faces) g21 g22 g23 (j faces) g31 g32 g33 (k faces)
   uint32_t *flags;

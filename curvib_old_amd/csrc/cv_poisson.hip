@@ -695,38 +695,50 @@ __global__ void k_nvrestrict2(Idx C, const double *nin, double *nout, long n) {
 // ------------------------------------------------------------------
 // transfers
 // ------------------------------------------------------------------
-// restriction of the coarse points t0, t0+step, ...
+// restriction of the coarse points t0, t0+step, ...  Fine points enter
+// weighted by max(0, 1 - nvert); in FP32 that weight comes precomputed
+// (wff, the same value rounded once).  The coarse point is an interior
+// fluid cell iff bit 0 of its operator word is set.
 template <class T>
 __device__ __forceinline__ void restrict_body(Idx C, Idx F, int isc, int jsc, int ksc, const double *__restrict__ nvf,
-                                     const double *__restrict__ nvc, const T *__restrict__ xf, T *__restrict__ xc,
-                                     long n, long t0, long step) {
+                                     const float *__restrict__ wff, const uint32_t *__restrict__ flc,
+                                     const T *__restrict__ xf, T *__restrict__ xc, long n, long t0, long step) {
   for (long t = t0; t < n; t += step) {
     int i, j, k;
     C.dec(t, i, j, k);
     T f = 0;
-    if (i > 0 && i < C.mx - 1 && j > 0 && j < C.my - 1 && k > 0 && k < C.mz - 1 && !(nvc[t] > 0.1)) {
+    if (flc[t] & 1u) {
       int ia = isc ? 0 : 1, ja = jsc ? 0 : 1, ka = ksc ? 0 : 1;
       int ih = isc ? i : 2 * i, jh = jsc ? j : 2 * j, kh = ksc ? k : 2 * k;
-#define RT(a, b, cc) (xf[F.at(a, b, cc)] * (T)fmax(0., 1 - nvf[F.at(a, b, cc)]))
+      auto RT = [&](int a, int b, int cc) -> T {
+        const long q = F.at(a, b, cc);
+        if constexpr (sizeof(T) == 4) return xf[q] * wff[q];
+        else return xf[q] * (T)fmax(0., 1 - nvf[q]);
+      };
       f = (T)0.125 * (RT(ih, jh, kh) + RT(ih - ia, jh, kh) + RT(ih, jh - ja, kh) + RT(ih, jh, kh - ka) +
                       RT(ih - ia, jh - ja, kh) + RT(ih, jh - ja, kh - ka) + RT(ih - ia, jh, kh - ka) +
                       RT(ih - ia, jh - ja, kh - ka));
-#undef RT
     }
     xc[t] = f;
   }
 }
 template <class T>
 __global__ void k_restrict(Idx C, Idx F, int isc, int jsc, int ksc, const double *__restrict__ nvf,
-                           const double *__restrict__ nvc, const T *__restrict__ xf, T *__restrict__ xc, long n) {
-  restrict_body<T>(C, F, isc, jsc, ksc, nvf, nvc, xf, xc, n, blockIdx.x * (long)blockDim.x + threadIdx.x,
+                           const float *__restrict__ wff, const uint32_t *__restrict__ flc, const T *__restrict__ xf,
+                           T *__restrict__ xc, long n) {
+  restrict_body<T>(C, F, isc, jsc, ksc, nvf, wff, flc, xf, xc, n, blockIdx.x * (long)blockDim.x + threadIdx.x,
                    (long)gridDim.x * blockDim.x);
 }
+__global__ void k_nvweight(const double *__restrict__ nv, float *__restrict__ w, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    w[t] = (float)fmax(0., 1 - nv[t]);
+}
 
-// xf (+)= P xc ; add=0 overwrites
+// xf (+)= P xc ; add=0 overwrites; fine points other than interior fluid
+// cells (bit 0 of the operator word clear) take 0
 template <class T>
 __device__ __forceinline__ void prolong_body(Idx F, Idx C, int isc, int jsc, int ksc, const double *__restrict__ nvc,
-                                    const double *__restrict__ nvf, const T *__restrict__ xc, T *__restrict__ xf,
+                                    const uint32_t *__restrict__ flf, const T *__restrict__ xc, T *__restrict__ xf,
                                     int add, long n, long t0, long step) {
   for (long t = t0; t < n; t += step) {
     int i, j, k;
@@ -748,14 +760,14 @@ __device__ __forceinline__ void prolong_body(Idx F, Idx C, int isc, int jsc, int
           (xc[C.at(ic, jc, kc + ka)] * (T)9 + xc[C.at(ic, jc + ja, kc + ka)] * (T)3 +
            xc[C.at(ic + ia, jc, kc + ka)] * (T)3 + xc[C.at(ic + ia, jc + ja, kc + ka)]) / (T)64.;
     }
-    if (nvf[t] > 0.1) f = 0;
+    if (!(flf[t] & 1u)) f = 0;
     if (add) xf[t] += f; else xf[t] = f;
   }
 }
 template <class T>
 __global__ void k_prolong(Idx F, Idx C, int isc, int jsc, int ksc, const double *__restrict__ nvc,
-                          const double *__restrict__ nvf, const T *__restrict__ xc, T *__restrict__ xf, int add, long n) {
-  prolong_body<T>(F, C, isc, jsc, ksc, nvc, nvf, xc, xf, add, n, blockIdx.x * (long)blockDim.x + threadIdx.x,
+                          const uint32_t *__restrict__ flf, const T *__restrict__ xc, T *__restrict__ xf, int add, long n) {
+  prolong_body<T>(F, C, isc, jsc, ksc, nvc, flf, xc, xf, add, n, blockIdx.x * (long)blockDim.x + threadIdx.x,
                   (long)gridDim.x * blockDim.x);
 }
 
@@ -1179,6 +1191,7 @@ int cv_poisson_prepare(cv_ctx *c) {
       hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.gf[q], (const double *)L.g[q], n);
     hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.ajf, (const double *)L.aj, n);
     hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, L.dinvf, (const double *)L.dinv, n);
+    hipLaunchKernelGGL(k_nvweight, dim3(nblk(n)), dim3(BS), 0, c->stream, (const double *)L.nvert, L.nwf, n);
     hipLaunchKernelGGL(k_count_fluid, dim3(CV_RED_GRID), dim3(BS), 0, c->stream, (const uint32_t *)L.flags, n, c->part);
     final_sum(c, c->part, CV_RED_GRID, c->dsc, 1);
     CV_CHECK(hipMemcpyAsync(c->red_host, c->dsc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1344,10 +1357,11 @@ static void vcycle(cv_ctx *c, MgVecs<T> &v, int l) {
   smooth<T>(c, v, l, c->opt.pre_sweeps, true);
   stencil<T, 1>(c, l, (const T *)v.x[l], (const T *)v.b[l], v.r[l]);
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.d.n)), dim3(BS), 0, c->stream, idx_of(C), idx_of(L), C.d.isc, C.d.jsc,
-                     C.d.ksc, (const double *)L.nvert, (const double *)C.nvert, (const T *)v.r[l], v.b[l - 1], C.d.n);
+                     C.d.ksc, (const double *)L.nvert, (const float *)L.nwf, (const uint32_t *)C.flags, (const T *)v.r[l],
+                     v.b[l - 1], C.d.n);
   vcycle<T>(c, v, l - 1);
   hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(n)), dim3(BS), 0, c->stream, idx_of(L), idx_of(C), C.d.isc, C.d.jsc, C.d.ksc,
-                     (const double *)C.nvert, (const double *)L.nvert, (const T *)v.x[l - 1], v.x[l], 1, n);
+                     (const double *)C.nvert, (const uint32_t *)L.flags, (const T *)v.x[l - 1], v.x[l], 1, n);
   smooth<T>(c, v, l, c->opt.post_sweeps, false);
 }
 
@@ -1945,7 +1959,8 @@ extern "C" int cv_mg_restrict(cv_ctx *c, int lc, const double *xf, double *xc) {
   CvLevel &C = c->lev[lc], &F = c->lev[lc + 1];
   CV_CHECK(hipMemcpyAsync(F.r, xf, F.d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(C.d.n)), dim3(BS), 0, c->stream, idx_of(C), idx_of(F), C.d.isc,
-                     C.d.jsc, C.d.ksc, (const double *)F.nvert, (const double *)C.nvert, (const double *)F.r, C.b, C.d.n);
+                     C.d.jsc, C.d.ksc, (const double *)F.nvert, (const float *)F.nwf, (const uint32_t *)C.flags,
+                     (const double *)F.r, C.b, C.d.n);
   CV_CHECK(hipMemcpyAsync(xc, C.b, C.d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CV_CHECK(hipStreamSynchronize(c->stream));
   return CV_OK;
@@ -1958,7 +1973,8 @@ extern "C" int cv_mg_prolong(cv_ctx *c, int lf, const double *xc, double *xf) {
   CvLevel &C = c->lev[lf - 1], &F = c->lev[lf];
   CV_CHECK(hipMemcpyAsync(C.r, xc, C.d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_prolong<double>, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, idx_of(F), idx_of(C), C.d.isc,
-                     C.d.jsc, C.d.ksc, (const double *)C.nvert, (const double *)F.nvert, (const double *)C.r, F.b, 0, F.d.n);
+                     C.d.jsc, C.d.ksc, (const double *)C.nvert, (const uint32_t *)F.flags, (const double *)C.r, F.b, 0,
+                     F.d.n);
   CV_CHECK(hipMemcpyAsync(xf, F.b, F.d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CV_CHECK(hipStreamSynchronize(c->stream));
   return CV_OK;
