@@ -344,7 +344,10 @@ def main():
     achieved = (byts / (ms * 1e-3)) / 1e9 if ms > 0 else 0.0
     roof = {
         "bound": "hbm",
-        "kernel": "k_stencil<float, 1> (fine-level operator residual r = b - A x of the FP32 V-cycle)",
+        "kernel": ("k_stencil<float, 1, 4, true, true> (fine-level operator residual r = b - A x of the FP32 "
+                   "V-cycle, FP16 face coefficients: 40 B per point)" if args.mixed == "mixed16" else
+                   "k_stencil<float, 1, 4, true> (fine-level operator residual r = b - A x of the FP32 V-cycle: "
+                   "56 B per point)"),
         "launches": launches,
         "avg_us": 1e3 * ms / launches if launches else None,
         "bytes_per_launch": byts / launches if launches else None,
@@ -352,7 +355,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": traffic_from_profiles("k_stencil<float, 1>"),
+        # the committed counter summary is of the default (FP16 coefficient) kernel
+        "traffic": traffic_from_profiles("k_stencil<float, 1>") if args.mixed == "mixed16" else None,
     }
 
     cpu = None
@@ -376,7 +380,8 @@ def main():
             "workload": f"warped channel + immersed sphere {'x'.join(map(str, cells))} cells, Re {case.re:g}, "
                         f"dt {case.dt:g}, Poisson rtol {SOLVER['rtol']:g}",
             "cells": int(ncell), "storage_points": int(npts), "ib_nodes": 0 if ib is None else len(ib[0]),
-            "mg_levels": args.levels, "semi_coarsening": list(semi), "coarsen_auto": args.coarsen_auto, **SOLVER,
+            "mg_levels": args.levels, "semi_coarsening": list(semi), "coarsen_auto": args.coarsen_auto,
+            "mixed_precision": args.mixed, **SOLVER,
         },
         "fp64": {"steps_per_s": sps_64, "ms_per_step": 1e3 * t64 / args.steps, "its": its64,
                  "cell_updates_per_s": ncell * sps_64},
