@@ -103,9 +103,11 @@ template <class T>
 struct OpV {
   const T *g[9];
   const T *aj;
-  // CV_MIXED16: per cell g0..g7 * 2^e in FP16 (16 bytes, one load) and
-  // {g8 * 2^e in FP16, unused FP16, aj * 2^-e in FP32} (8 bytes), so that
-  // aj * 2^-e * sum(g * 2^e ...) = aj * sum(g ...)
+  // CV_MIXED16 per cell, g scaled by 2^e in FP16 and aj by 2^-e in FP32
+  // (aj * 2^-e * sum(g * 2^e ...) = aj * sum(g ...)):
+  //   hA (16 bytes): g0 g1 | g2 g6 | g7 g8 | aj
+  //   hB (8 bytes):  g3 g4 | g5, the operator word (it uses 16 bits)
+  // so the j-face below a tile row needs hB alone
   const uint4 *hA;
   const uint2 *hB;
   // CV_MIXED16 y-line factors per cell: {a in FP16 | ip * 2^e in FP16,
@@ -264,10 +266,6 @@ __device__ inline T tile_flux_i(const T *M, const T *P, const T *Q, int c, int r
 // FP16 halves of packed coefficient words, widened to FP32 (exact)
 __device__ inline float half_lo(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)); }
 __device__ inline float half_hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
-__device__ inline float half_at(const uint4 &v, int q) {
-  const uint32_t w = q < 2 ? v.x : q < 4 ? v.y : q < 6 ? v.z : v.w;
-  return (q & 1) ? half_hi(w) : half_lo(w);
-}
 
 // operator word and face coefficients of one cell, loaded a plane ahead;
 // in FP32 (where registers allow it) also aj, b and dinv
@@ -332,12 +330,25 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
   };
   constexpr bool PF = sizeof(T) == 4;      // prefetch aj, b, dinv with the cell
   auto gco = [&](int q, long t) -> T {
-    if constexpr (HG) return q < 8 ? half_at(A.hA[t], q) : half_lo(A.hB[t].x);
-    else return A.g[q][t];
+    if constexpr (HG) {
+      if (q >= 3 && q <= 5) {
+        const uint2 hb = A.hB[t];
+        return q == 3 ? half_lo(hb.x) : q == 4 ? half_hi(hb.x) : half_lo(hb.y);
+      }
+      const uint4 ha = A.hA[t];
+      return q == 0 ? half_lo(ha.x) : q == 1 ? half_hi(ha.x) : q == 2 ? half_lo(ha.y)
+           : q == 6 ? half_hi(ha.y) : q == 7 ? half_lo(ha.z) : half_hi(ha.z);
+    } else {
+      return A.g[q][t];
+    }
   };
   auto ajo = [&](long t) -> T {
-    if constexpr (HG) return __uint_as_float(A.hB[t].y);
+    if constexpr (HG) return __uint_as_float(A.hA[t].w);
     else return A.aj[t];
+  };
+  auto wo = [&](long t) -> uint32_t {
+    if constexpr (HG) return A.hB[t].y >> 16;
+    else return A.fl[t];
   };
   auto fetch_cell = [&](int gk, StCell<T> &C) {
     C.w = 0u;
@@ -348,15 +359,16 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
     C.gb[0] = C.gb[1] = C.gb[2] = (T)0;
     if (!cell) return;
     const long t = tij + pl * gk;
-    C.w = A.fl[t];
     if constexpr (HG) {
       const uint4 ha = A.hA[t];
       const uint2 hb = A.hB[t];
-#pragma unroll
-      for (int q = 0; q < 8; q++) C.g[q] = half_at(ha, q);
-      C.g[8] = half_lo(hb.x);
-      if (out) C.aj = __uint_as_float(hb.y);
+      C.w = hb.y >> 16;
+      C.g[0] = half_lo(ha.x); C.g[1] = half_hi(ha.x); C.g[2] = half_lo(ha.y);
+      C.g[3] = half_lo(hb.x); C.g[4] = half_hi(hb.x); C.g[5] = half_lo(hb.y);
+      C.g[6] = half_hi(ha.y); C.g[7] = half_lo(ha.z); C.g[8] = half_hi(ha.z);
+      if (out) C.aj = __uint_as_float(ha.w);
     } else {
+      C.w = A.fl[t];
 #pragma unroll
       for (int q = 0; q < 9; q++) C.g[q] = A.g[q][t];
     }
@@ -366,11 +378,12 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
       if (MODE == 2) C.dv = dinv[t];
     }
     if (PFB && row0) {
-      C.wb = A.fl[t - mx];
       if constexpr (HG) {
-        const uint4 ha = A.hA[t - mx];
-        C.gb[0] = half_at(ha, 3); C.gb[1] = half_at(ha, 4); C.gb[2] = half_at(ha, 5);
+        const uint2 hb = A.hB[t - mx];
+        C.wb = hb.y >> 16;
+        C.gb[0] = half_lo(hb.x); C.gb[1] = half_hi(hb.x); C.gb[2] = half_lo(hb.y);
       } else {
+        C.wb = A.fl[t - mx];
         C.gb[0] = A.g[3][t - mx]; C.gb[1] = A.g[4][t - mx]; C.gb[2] = A.g[5][t - mx];
       }
     }
@@ -389,7 +402,7 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
   T fk_lo = (T)0;
   if (out && k0 >= 1) {
     const long t = tij + pl * (k0 - 1);
-    fk_lo = tile_flux_k<T>(X[(k0 - 1 + 4) & 3], X[(k0 + 4) & 3], c, r, A.fl[t], gco(6, t), gco(7, t), gco(8, t));
+    fk_lo = tile_flux_k<T>(X[(k0 - 1 + 4) & 3], X[(k0 + 4) & 3], c, r, wo(t), gco(6, t), gco(7, t), gco(8, t));
   }
   for (int gk = k0; gk < k1; gk++) {
     T xq0, xq1;
@@ -409,7 +422,7 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
           fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, cur.wb, cur.gb[0], cur.gb[1], cur.gb[2]);
         } else {
           const long tb = t - mx;
-          fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, A.fl[tb], gco(3, tb), gco(4, tb), gco(5, tb));
+          fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, wo(tb), gco(3, tb), gco(4, tb), gco(5, tb));
         }
       }
     }
@@ -1013,11 +1026,11 @@ static int stencil_remap(unsigned gx) {
 }
 
 // algorithmic bytes per storage point of the stencil modes: x, 9 g, aj (+b)
-// (+dinv), y in T, and the 4-byte operator word; hg: the 9 g and aj as the
-// 24-byte FP16 pack
+// (+dinv), y in T, and the 4-byte operator word; hg: the 9 g, aj and the
+// operator word as the 24-byte FP16 pack
 template <class T>
 static double stencil_bytes_pt(int mode, bool hg = false) {
-  if (hg) return (double)(mode == 0 ? 2 : mode == 1 ? 3 : 4) * sizeof(T) + 24.0 + 4.0;
+  if (hg) return (double)(mode == 0 ? 2 : mode == 1 ? 3 : 4) * sizeof(T) + 24.0;
   return (double)(mode == 0 ? 12 : mode == 1 ? 13 : 14) * sizeof(T) + 4.0;
 }
 
@@ -1119,15 +1132,14 @@ __global__ static void k_absmax1(const double *a, long n, unsigned long long *ou
   if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
-// the two words per cell of OpV::hA / hB
-__global__ static void k_pack_half(G9 g, const double *aj, double s, long n, uint4 *ha, uint2 *hb) {
+// the two words per cell of OpV::hA / hB; the operator word has 16 bits
+// (bit 0 and five per face axis)
+__global__ static void k_pack_half(G9 g, const double *aj, const uint32_t *fl, double s, long n, uint4 *ha, uint2 *hb) {
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    uint32_t w[5];
-#pragma unroll
-    for (int q = 0; q < 4; q++) w[q] = half_bits(g.p[2 * q][t] * s) | (half_bits(g.p[2 * q + 1][t] * s) << 16);
-    w[4] = half_bits(g.p[8][t] * s);
-    ha[t] = make_uint4(w[0], w[1], w[2], w[3]);
-    hb[t] = make_uint2(w[4], __float_as_uint((float)(aj[t] / s)));
+    auto h = [&](int q) { return half_bits(g.p[q][t] * s); };
+    ha[t] = make_uint4(h(0) | (h(1) << 16), h(2) | (h(6) << 16), h(7) | (h(8) << 16),
+                       __float_as_uint((float)(aj[t] / s)));
+    hb[t] = make_uint2(h(3) | (h(4) << 16), h(5) | ((fl[t] & 0xffffu) << 16));
   }
 }
 
@@ -1155,8 +1167,8 @@ static int half_ops(cv_ctx *c) {
     if (gmax > 0) frexp(gmax, &e);      // gmax < 2^e
     L.hexp = gmax > 0 ? 14 - e : 0;
     const double sc = ldexp(1.0, L.hexp);
-    hipLaunchKernelGGL(k_pack_half, dim3(nblk(n)), dim3(BS), 0, c->stream, g, (const double *)L.aj, sc, n, (uint4 *)L.gh,
-                       (uint2 *)((char *)L.gh + 16 * n));
+    hipLaunchKernelGGL(k_pack_half, dim3(nblk(n)), dim3(BS), 0, c->stream, g, (const double *)L.aj,
+                       (const uint32_t *)L.flags, sc, n, (uint4 *)L.gh, (uint2 *)((char *)L.gh + 16 * n));
     // line factors: 2^yexp puts max |ip| in [2^13, 2^14); |a|, |cp| < 1
     CV_CHECK(hipMemsetAsync(c->part, 0, sizeof(double), c->stream));
     hipLaunchKernelGGL(k_absmax1, dim3((unsigned)std::min<long>(nblk(n), 1024)), dim3(BS), 0, c->stream,
