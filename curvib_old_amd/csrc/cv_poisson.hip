@@ -994,8 +994,17 @@ static int stencil_kchunk(int mz) {
     kc = e ? std::max(1, atoi(e)) : 0;
   }
   // default: 12 planes on deep levels, 8 on shallower ones (more blocks);
-  // the fastest measured at 512x256x256
-  return kc ? kc : (mz >= 192 ? 12 : 8);
+  // the fastest measured at 512x256x256.  Levels of few planes (mz < 64)
+  // take CV_STENCIL_KCHUNK_SMALL planes per block: their launches are short
+  // and latency-bound, so a shorter plane march can pay
+  static int ks = -1, km = -1;
+  if (ks < 0) {
+    const char *e = getenv("CV_STENCIL_KCHUNK_SMALL");
+    ks = e ? std::max(1, atoi(e)) : 2;
+    e = getenv("CV_STENCIL_KCHUNK_MID");
+    km = e ? std::max(1, atoi(e)) : 8;
+  }
+  return kc ? kc : (mz >= 192 ? 12 : mz >= 64 ? km : ks);
 }
 
 // rows per stencil block (4 or 8; CV_STENCIL_ROWS64 / CV_STENCIL_ROWS32) and
