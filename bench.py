@@ -208,40 +208,43 @@ def batch_ensemble_only(args):
     print(json.dumps({"workload": f"{len(ens)}-case ensemble, one cv_batch", **r}), flush=True)
 
 
-def run_serving(serve_reqs, arrivals, ens_reqs, nsteps, ens_steps):
+def run_serving(serve_reqs, arrivals, ens_reqs, nsteps, ens_steps, mixed="mixed16"):
+    """The serving and ensemble legs in FP64 and in the mixed leg's precision
+    (reported under the "mixed" keys)."""
     from curvib_old_amd import serving
     opts = {k: v for k, v in SOLVER.items()}
     out = {}
-    for prec in ("fp64", "mixed"):
+    precs = (("fp64", "fp64"), ("mixed", mixed))
+    for key, prec in precs:
         r = serving.run_stream(serve_reqs, arrivals, nsteps, prec, budget_bytes=200e9, slots=16, log=None, **opts)
         r.pop("records")
-        out[f"serving_{prec}"] = r
+        out[f"serving_{key}"] = r
         log(f"[serving {prec}] {r['cases']} cases x {nsteps} steps: step p50 {r['step_latency'].get('p50_ms', 0):.1f} ms "
             f"p99 {r['step_latency'].get('p99_ms', 0):.1f} ms, {r['cell_updates_per_s'] / 1e6:.1f} M cell-updates/s, "
             f"peak HBM {r['bytes_peak'] / 2**30:.1f} GiB, failed {r['failed']}")
-    for prec in ("fp64", "mixed"):
+    for key, prec in precs:
         r = serving.run_stream(ens_reqs, np.zeros(len(ens_reqs)), ens_steps, prec, budget_bytes=200e9, slots=16,
                                **opts)
         r.pop("records")
-        out[f"ensemble_{prec}"] = r
+        out[f"ensemble_{key}"] = r
         log(f"[ensemble {prec}] {r['cases']} cases x {ens_steps} steps: makespan {r['makespan_s']:.2f} s, step p50 "
             f"{r['step_latency'].get('p50_ms', 0):.1f} ms p99 {r['step_latency'].get('p99_ms', 0):.1f} ms, "
             f"{r['cell_updates_per_s'] / 1e6:.1f} M cell-updates/s")
     # the serving stream again through the batched server: every admitted case
     # advances in one cv_batch, rebuilt as cases arrive and finish
-    for prec in ("fp64", "mixed"):
+    for key, prec in precs:
         r = serving.run_stream(serve_reqs, arrivals, nsteps, prec, budget_bytes=200e9, slots=64, log=None,
                                mode="batched", **opts)
         r.pop("records")
-        out[f"serving_batched_{prec}"] = r
+        out[f"serving_batched_{key}"] = r
         log(f"[serving batched {prec}] {r['cases']} cases x {nsteps} steps: step p50 "
             f"{r['step_latency'].get('p50_ms', 0):.1f} ms p99 {r['step_latency'].get('p99_ms', 0):.1f} ms, "
             f"{r['cell_updates_per_s'] / 1e6:.1f} M cell-updates/s, peak HBM {r['bytes_peak'] / 2**30:.1f} GiB, "
             f"failed {r['failed']}")
     # the same ensemble as one batch: one launch per multigrid depth for all cases
-    for prec in ("fp64", "mixed"):
+    for key, prec in precs:
         r = serving.run_batched(ens_reqs, ens_steps, prec, **opts)
-        out[f"ensemble_batched_{prec}"] = r
+        out[f"ensemble_batched_{key}"] = r
         log(f"[ensemble batched {prec}] {r['cases']} cases x {ens_steps} steps: {r['makespan_s']:.2f} s, step p50 "
             f"{r['step_latency'].get('p50_ms', 0):.1f} ms p99 {r['step_latency'].get('p99_ms', 0):.1f} ms, "
             f"{r['cell_updates_per_s'] / 1e6:.1f} M cell-updates/s, max its {r['max_its']}")
@@ -272,7 +275,7 @@ def main():
     ap.add_argument("--mixed", default="mixed16", choices=("mixed", "mixed16"),
                     help="precision of the mixed leg: FP32 V-cycle, or with FP16 face coefficients")
     ap.add_argument("--ensemble-max-levels", type=int, default=8)
-    ap.add_argument("--batch-ensemble-only", choices=("mixed", "fp64"), default=None,
+    ap.add_argument("--batch-ensemble-only", choices=("mixed", "mixed16", "fp64"), default=None,
                     help="run only the batched-ensemble leg in this precision")
     ap.add_argument("--serving-requests", type=int, default=50)
     ap.add_argument("--serving-steps", type=int, default=20)
@@ -332,7 +335,7 @@ def main():
 
     srv = None
     if serve_in is not None:
-        srv = run_serving(*serve_in, args.serving_steps, args.ensemble_steps)
+        srv = run_serving(*serve_in, args.serving_steps, args.ensemble_steps, args.mixed)
         serve_in = None
 
     ncell = float(np.prod(cells))

@@ -795,7 +795,7 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
       b.isc = L.d.isc; b.jsc = L.d.jsc; b.ksc = L.d.ksc;
       b.n = L.d.n;
       b.inv_nf = x->nfluid[l] > 0 ? 1.0 / x->nfluid[l] : 0.0;
-      b.kc = stencil_kchunk(L.d.mz);
+      b.kc = stencil_kchunk_batch(L.d.mz);
       b.sgx = (L.d.mx + 62) / 63; b.sgy = (L.d.my + 3) / 4; b.sgz = (L.d.mz + b.kc - 1) / b.kc;
       b.lgx = (L.d.mx + 63) / 64;
       const int rows = L.d.my - 2;

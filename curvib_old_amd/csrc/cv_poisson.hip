@@ -1006,6 +1006,17 @@ static int stencil_kchunk(int mz) {
   }
   return kc ? kc : (mz >= 192 ? 12 : mz >= 64 ? km : ks);
 }
+// in a batch the small levels of many cases share one launch, which is then
+// throughput-bound: 8 planes there measured best (the tile shape does not
+// change any point's arithmetic, so a case gets the same bits either way)
+static int stencil_kchunk_batch(int mz) {
+  static int kb = -1;
+  if (kb < 0) {
+    const char *e = getenv("CV_BATCH_KCHUNK_SMALL");
+    kb = e ? std::max(1, atoi(e)) : 8;
+  }
+  return mz >= 64 || getenv("CV_STENCIL_KCHUNK") ? stencil_kchunk(mz) : kb;
+}
 
 // rows per stencil block (4 or 8; CV_STENCIL_ROWS64 / CV_STENCIL_ROWS32) and
 // XCD-aware tile order (CV_STENCIL_REMAP); the defaults are the fastest

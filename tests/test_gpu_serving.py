@@ -26,7 +26,7 @@ def mix():
     return [(c, cases.build(c)) for c in cs]
 
 
-def run(reqs, slots, budget=50e9, nsteps=3, mode="streams", stagger=0.0):
+def run(reqs, slots, budget=50e9, nsteps=3, mode="streams", stagger=0.0, opts=OPTS):
     import time
     srv = serving.Server(budget, slots, mode=mode)
     try:
@@ -34,7 +34,7 @@ def run(reqs, slots, budget=50e9, nsteps=3, mode="streams", stagger=0.0):
         for q, (c, a) in enumerate(reqs):
             if stagger and q:
                 time.sleep(stagger)
-            ids.append(srv.submit(c, a, nsteps, **OPTS))
+            ids.append(srv.submit(c, a, nsteps, **opts))
         out = []
         for (c, _), cid in zip(reqs, ids):
             rc, info = srv.wait(cid)
@@ -149,3 +149,31 @@ def test_ib_cases_alone_equal_mix_in_both_modes():
             assert info["steps_done"] == 3
             assert np.array_equal(p, pr), (mode, q)
             assert np.array_equal(u, ur), (mode, q)
+
+
+def test_mixed16_budget_and_batched_equals_streams(mix):
+    """CV_MIXED16 allocates its FP16 packs (32 bytes per point and level) when
+    the option is set; the server counts them in a case's bytes, and the
+    batched server still gives each case the streams server's bits."""
+    from curvib_old_amd.solver import UserCtx
+    opts16 = {**OPTS, "precision": "mixed16"}
+    c, (xyz, nv, uc) = mix[1]
+    lev = serving.levels_for(c.cells)
+    u = UserCtx(c.nodes, c.bctype, lev, c.re, c.dt, semi=serving.SEMI, coarsen_auto=serving.COARSEN_AUTO)
+    try:
+        u.set_grid(xyz)
+        u.set_nvert(nv)
+        u.put("ucont", uc)
+        u.set_solver(**opts16)
+        u.Flow_Solver()
+        half = 32 * sum(int(np.prod(u.level_storage(q))) for q in range(lev))
+        assert u.device_bytes() == serving.ctx_bytes(c, lev, OPTS["restart"]) + half
+    finally:
+        u.close()
+    ref, _ = run(mix, slots=6, nsteps=3, opts=opts16)
+    got, _ = run(mix, slots=4, mode="batched", nsteps=3, stagger=0.005, opts=opts16)
+    for (cc, _), (p0, u0, i0), (p1, u1, i1) in zip(mix, ref, got):
+        assert i1["steps_done"] == 3
+        assert np.array_equal(p0, p1) and np.array_equal(u0, u1)
+        lv = serving.levels_for(cc.cells)
+        assert i0["bytes"] > serving.ctx_bytes(cc, lv, OPTS["restart"])
