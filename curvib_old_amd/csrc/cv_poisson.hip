@@ -205,9 +205,10 @@ __global__ void __launch_bounds__(BS) k_jacobi(OpV<T> A, const T *__restrict__ x
 }
 
 // ------------------------------------------------------------------
-// 2.5D-tiled flux-form stencil.  A block of 64 x TY threads owns 63
-// cells along i (lane L <-> cell i0-1+L; lane 0 only supplies the flux of
-// face i0-1) and TY rows along j, and marches k over one chunk.  The x
+// 2.5D-tiled flux-form stencil.  A block of TW x TY threads (TW = 64 or
+// 32 lanes per tile row) owns TW-1 cells along i (lane L <-> cell i0-1+L;
+// lane 0 only supplies the flux of face i0-1) and TY rows along j, and
+// marches k over one chunk.  The x
 // planes k-1, k, k+1 of the tile (+1 halo, periodic wrap applied at load)
 // live in LDS (four rotating slots); every face flux is formed once: the
 // lower i-face comes from the neighbouring lane (shuffle), the lower j-face
@@ -219,7 +220,6 @@ __global__ void __launch_bounds__(BS) k_jacobi(OpV<T> A, const T *__restrict__ x
 //   MODE 0: y = A x      MODE 1: y = b - A x
 //   MODE 2: y = x + omega * dinv * (b - A x)   (damped Jacobi sweep)
 // ------------------------------------------------------------------
-#define ST_TC 66
 
 // one face flux from its two cells (xl, xu) and the four neighbours of
 // each tangential axis: (+f, +e+f, -f, +e-f)
@@ -238,8 +238,8 @@ __device__ inline T tile_flux(T xl, T xu, T gdd, uint32_t w, int sh, T g0, T a0,
 }
 
 // k-face flux between planes P (k) and Q (k+1) at LDS column c, row r;
-// g6, g7, g8 of the face
-template <class T>
+// g6, g7, g8 of the face; ST_TC: the LDS row stride (tile width + 2)
+template <class T, int ST_TC>
 __device__ inline T tile_flux_k(const T *P, const T *Q, int c, int r, uint32_t w, T g6, T g7, T g8) {
   if (!(w & (1u << 11))) return (T)0;
   const int o = r * ST_TC + c;
@@ -247,7 +247,7 @@ __device__ inline T tile_flux_k(const T *P, const T *Q, int c, int r, uint32_t w
                       g7, P[o + ST_TC], Q[o + ST_TC], P[o - ST_TC], Q[o - ST_TC]);
 }
 // j-face flux between rows r and r+1 of plane P (M = k-1, Q = k+1)
-template <class T>
+template <class T, int ST_TC>
 __device__ inline T tile_flux_j(const T *M, const T *P, const T *Q, int c, int r, uint32_t w, T g3, T g4, T g5) {
   if (!(w & (1u << 6))) return (T)0;
   const int o = r * ST_TC + c, u = o + ST_TC;
@@ -255,7 +255,7 @@ __device__ inline T tile_flux_j(const T *M, const T *P, const T *Q, int c, int r
                       g5, Q[o], Q[u], M[o], M[u]);
 }
 // i-face flux between columns c and c+1 of plane P
-template <class T>
+template <class T, int ST_TC>
 __device__ inline T tile_flux_i(const T *M, const T *P, const T *Q, int c, int r, uint32_t w, T g0, T g1, T g2) {
   if (!(w & (1u << 1))) return (T)0;
   const int o = r * ST_TC + c;
@@ -276,23 +276,27 @@ struct StCell {
   T aj, b, dv;
 };
 
-// TY: rows per block (block 64 x TY); PFB: prefetch the row-0 face below
+// TY: rows per block (block TW x TY); PFB: prefetch the row-0 face below
 // the tile with the cells (keeps the row-0 wave off the barrier's critical
-// path at the cost of registers); HG: face coefficients from the FP16 copy
-// (h, ajh) instead of g, aj; remap: XCD-aware tile order
-// one tile (bx, by, bz) of the stencil: 63 x TY cells of kchunk planes
-template <class T, int MODE, int TY, bool PFB, bool HG = false>
+// path at the cost of registers); HG: face coefficients from the FP16
+// packs instead of g, aj; TW: lanes per tile row, 64 (one wavefront per
+// row) or 32 (two rows per wavefront; the i-face shuffle stays within a
+// row); remap: XCD-aware tile order
+// one tile (bx, by, bz) of the stencil: (TW-1) x TY cells of kchunk planes
+template <class T, int MODE, int TY, bool PFB, bool HG = false, int TW = 64>
 __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restrict__ x, const T *__restrict__ b,
                                     T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk, int bx,
                                     int by, int bz, int color = -1) {
-  constexpr int TR = TY + 2, NT = TR * ST_TC, NTH = 64 * TY;
+  static_assert(TW == 64 || TW == 32, "tile rows of 64 or 32 lanes");
+  constexpr int ST_TC = TW + 2;
+  constexpr int TR = TY + 2, NT = TR * ST_TC, NTH = TW * TY;
   static_assert(2 * NTH >= NT, "two tile entries per thread must cover a plane");
   __shared__ T X[4][NT];
-  __shared__ T FJ[2][TY][64];
+  __shared__ T FJ[2][TY][TW];
   const int L = threadIdx.x, R = threadIdx.y;
-  const int tid = R * 64 + L;
+  const int tid = R * TW + L;
   const int mx = A.mx, my = A.my, mz = A.mz;
-  const int i0 = 63 * bx, j0 = TY * by;
+  const int i0 = (TW - 1) * bx, j0 = TY * by;
   const int k0 = kchunk * bz, k1 = min(k0 + kchunk, mz);
   const int gi = i0 - 1 + L, gj = j0 + R;
   const bool in_i = gi >= 0 && gi < mx, in_j = gj < my;
@@ -402,7 +406,7 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
   T fk_lo = (T)0;
   if (out && k0 >= 1) {
     const long t = tij + pl * (k0 - 1);
-    fk_lo = tile_flux_k<T>(X[(k0 - 1 + 4) & 3], X[(k0 + 4) & 3], c, r, wo(t), gco(6, t), gco(7, t), gco(8, t));
+    fk_lo = tile_flux_k<T, ST_TC>(X[(k0 - 1 + 4) & 3], X[(k0 + 4) & 3], c, r, wo(t), gco(6, t), gco(7, t), gco(8, t));
   }
   for (int gk = k0; gk < k1; gk++) {
     T xq0, xq1;
@@ -412,17 +416,17 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
     const long t = tij + pl * gk;
     const T *Pm = X[(gk - 1 + 4) & 3], *P = X[(gk + 4) & 3], *Q = X[(gk + 1 + 4) & 3];
     const uint32_t w = cur.w;
-    const T fi = cell ? tile_flux_i<T>(Pm, P, Q, c, r, w, cur.g[0], cur.g[1], cur.g[2]) : (T)0;
+    const T fi = cell ? tile_flux_i<T, ST_TC>(Pm, P, Q, c, r, w, cur.g[0], cur.g[1], cur.g[2]) : (T)0;
     T fj = (T)0, fk = (T)0, fj_lo = (T)0;
     if (out) {
-      fj = tile_flux_j<T>(Pm, P, Q, c, r, w, cur.g[3], cur.g[4], cur.g[5]);
-      fk = tile_flux_k<T>(P, Q, c, r, w, cur.g[6], cur.g[7], cur.g[8]);
+      fj = tile_flux_j<T, ST_TC>(Pm, P, Q, c, r, w, cur.g[3], cur.g[4], cur.g[5]);
+      fk = tile_flux_k<T, ST_TC>(P, Q, c, r, w, cur.g[6], cur.g[7], cur.g[8]);
       if (row0) {
         if (PFB) {
-          fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, cur.wb, cur.gb[0], cur.gb[1], cur.gb[2]);
+          fj_lo = tile_flux_j<T, ST_TC>(Pm, P, Q, c, r - 1, cur.wb, cur.gb[0], cur.gb[1], cur.gb[2]);
         } else {
           const long tb = t - mx;
-          fj_lo = tile_flux_j<T>(Pm, P, Q, c, r - 1, wo(tb), gco(3, tb), gco(4, tb), gco(5, tb));
+          fj_lo = tile_flux_j<T, ST_TC>(Pm, P, Q, c, r - 1, wo(tb), gco(3, tb), gco(4, tb), gco(5, tb));
         }
       }
     }
@@ -436,7 +440,7 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
     const T xc = P[r * ST_TC + c];
     FJ[gk & 1][R][L] = fj;
     store_x(gk + 2, xn0, xn1);
-    const T fi_lo = __shfl_up(fi, 1, 64);
+    const T fi_lo = __shfl_up(fi, 1, TW);
     __syncthreads();
     if (R > 0) fj_lo = FJ[gk & 1][R - 1][L];
     if (out) {
@@ -468,14 +472,14 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
   }
 }
 
-template <class T, int MODE, int TY, bool PFB, bool HG = false>
-__global__ void __launch_bounds__(64 * TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
+template <class T, int MODE, int TY, bool PFB, bool HG = false, int TW = 64>
+__global__ void __launch_bounds__(TW * TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
                                                    T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk,
                                                    int remap, int color) {
   int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
   if (remap == 1) cv_xcd_tile(bx, by, bz);
   else if (remap > 1) cv_xcd_runs(remap, bx, by, bz);
-  stencil_tile<T, MODE, TY, PFB, HG>(A, x, b, y, dinv, omega, kchunk, bx, by, bz, color);
+  stencil_tile<T, MODE, TY, PFB, HG, TW>(A, x, b, y, dinv, omega, kchunk, bx, by, bz, color);
 }
 
 // ------------------------------------------------------------------
@@ -1059,18 +1063,59 @@ static bool half_coef(const cv_ctx *c) { return c->opt.precision == CV_MIXED16; 
 
 // MODE 0: y = A x; 1: y = b - A x; 2: y = x + omega D^-1 (b - A x), on the
 // cells of one (i+j+k) parity when color is 0 or 1 (the others keep x)
+// tile rows of 32 lanes (31 cells per tile row, two rows per wavefront) by
+// default, 4 rows per block: at 512x256x256 they waste fewer lanes on the
+// storage width (514 = 16.6 x 31 against 8.2 x 63) and measured 3-4 %
+// faster than rows of 64.  CV_STENCIL_TW=64 restores rows of 64 except on
+// levels of at most CV_STENCIL_TW32_MAXMX columns; CV_STENCIL_TW32_ROWS: 4 or 8
+static int stencil_tw(int mx) {
+  static int all = -1, maxmx = -1;
+  if (all < 0) {
+    all = env_int("CV_STENCIL_TW", 32) == 32;
+    maxmx = env_int("CV_STENCIL_TW32_MAXMX", 0);
+  }
+  return (all || mx <= maxmx) ? 32 : 64;
+}
+static int stencil_tw32_rows() {
+  static int rows = 0;
+  if (!rows) rows = env_int("CV_STENCIL_TW32_ROWS", 4) == 8 ? 8 : 4;
+  return rows;
+}
+
 template <class T, int MODE>
 static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T)0, int color = -1) {
   CvLevel &L = c->lev[l];
   OpV<T> A = opview<T>(c, l);
   const int kc = stencil_kchunk(L.d.mz);
-  const int ty = stencil_rows<T>();
-  dim3 g((unsigned)((L.d.mx + 62) / 63), (unsigned)((L.d.my + ty - 1) / ty), (unsigned)((L.d.mz + kc - 1) / kc));
+  const int tw = stencil_tw(L.d.mx);
+  const int ty = tw == 32 ? stencil_tw32_rows() : stencil_rows<T>();
+  dim3 g((unsigned)((L.d.mx + tw - 2) / (tw - 1)), (unsigned)((L.d.my + ty - 1) / ty),
+         (unsigned)((L.d.mz + kc - 1) / kc));
   const int remap = stencil_remap(g.x);
   const bool hg = sizeof(T) == 4 && half_coef(c) && L.gh;
   CvProfScope ps(c, MODE == 0 ? CV_PROF_APPLY : CV_PROF_RESID, l, sizeof(T) == 4 ? 1 : 0,
                  (double)L.d.n * stencil_bytes_pt<T>(MODE, hg));
   const bool pfb = sizeof(T) == 4 || stencil_pfb64();
+  if (tw == 32) {
+    if constexpr (sizeof(T) == 4) {
+      if (hg) {
+        if (ty == 8)
+          hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, true, 32>), g, dim3(32, 8), 0, c->stream, A, x, b, y, A.dinv,
+                             omega, kc, remap, color);
+        else
+          hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, true, 32>), g, dim3(32, 4), 0, c->stream, A, x, b, y, A.dinv,
+                             omega, kc, remap, color);
+        return;
+      }
+    }
+    if (ty == 8)
+      hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, false, 32>), g, dim3(32, 8), 0, c->stream, A, x, b, y, A.dinv,
+                         omega, kc, remap, color);
+    else
+      hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, false, 32>), g, dim3(32, 4), 0, c->stream, A, x, b, y, A.dinv,
+                         omega, kc, remap, color);
+    return;
+  }
   if constexpr (sizeof(T) == 4) {
     if (hg) {
       if (ty == 4)
