@@ -105,13 +105,13 @@ __device__ inline bool bblock(const uint32_t *tab, unsigned long long pm, const 
   if (!bblock(tab, pm, ctl, c, lb)) return;      \
   const BCase &C = cs[c];
 
-template <class T, int MODE, bool HG>
-__global__ void __launch_bounds__(256) kb_stencil(BK_ARGS, int d, int sx, int sb, int sy) {
+template <class T, int MODE, bool HG, int TW>
+__global__ void __launch_bounds__(TW * 4) kb_stencil(BK_ARGS, int d, int sx, int sb, int sy) {
   BK_HEAD
   const OpV<T> A = bop<T>(C, d);
   const int gx = C.lev[d].sgx, gy = C.lev[d].sgy, jj = ctl->jj;
   const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
-  stencil_tile<T, MODE, 4, true, HG>(A, bvec<T>(C, d, sx, jj), MODE >= 1 ? bvec<T>(C, d, sb, jj) : (const T *)nullptr,
+  stencil_tile<T, MODE, 4, true, HG, TW>(A, bvec<T>(C, d, sx, jj), MODE >= 1 ? bvec<T>(C, d, sb, jj) : (const T *)nullptr,
                                  bvec<T>(C, d, sy, jj), A.dinv, (T)0, C.lev[d].kc, bx, by, bz);
 }
 
@@ -336,6 +336,7 @@ struct cv_batch {
   std::vector<hipGraphExec_t> gpre, gpost;   // per-case graphs of the work around the solve
   CvMomBatch *mom = nullptr;                  // the momentum of all cases on the batch stream
   int use_mom = 1;
+  int tw = 32;                                // lanes per stencil tile row (32 or 64)
   unsigned long long xper = 0, zper = 0;
   std::vector<BFg> fg;
   std::vector<int> rc;
@@ -378,15 +379,25 @@ template <class T, int MODE>
 static void b_stencil(cv_batch *B, int d, unsigned long long pm, int sx, int sb, int sy) {
   const BTab &t = BPICK(B->st, B->st_v, d, pm);
   if (!BLAUNCH_OK(t, pm)) return;
-  if constexpr (sizeof(T) == 4) {
-    if (B->opt.precision == CV_MIXED16) {
-      hipLaunchKernelGGL((kb_stencil<T, MODE, true>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream,
-                         (const BCase *)B->dcases, (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, d, sx, sb, sy);
-      return;
-    }
+  const bool hg = sizeof(T) == 4 && B->opt.precision == CV_MIXED16;
+  const BCase *cs = B->dcases;
+  const uint32_t *tb = BTAB(t);
+  const BCtl *ctl = B->dctl;
+  if (B->tw == 32) {
+    if (hg)
+      hipLaunchKernelGGL((kb_stencil<T, MODE, sizeof(T) == 4, 32>), dim3((unsigned)t.len), dim3(32, 4), 0, B->stream, cs,
+                         tb, pm, ctl, d, sx, sb, sy);
+    else
+      hipLaunchKernelGGL((kb_stencil<T, MODE, false, 32>), dim3((unsigned)t.len), dim3(32, 4), 0, B->stream, cs, tb, pm,
+                         ctl, d, sx, sb, sy);
+  } else {
+    if (hg)
+      hipLaunchKernelGGL((kb_stencil<T, MODE, sizeof(T) == 4, 64>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream, cs,
+                         tb, pm, ctl, d, sx, sb, sy);
+    else
+      hipLaunchKernelGGL((kb_stencil<T, MODE, false, 64>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream, cs, tb, pm,
+                         ctl, d, sx, sb, sy);
   }
-  hipLaunchKernelGGL((kb_stencil<T, MODE, false>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream,
-                     (const BCase *)B->dcases, (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, d, sx, sb, sy);
 }
 
 template <class T>
@@ -755,6 +766,8 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
   B->use_graph = ge ? atoi(ge) != 0 : 1;
   const char *gm = getenv("CV_BATCH_MOM");
   B->use_mom = gm ? atoi(gm) != 0 : 1;
+  const char *gt = getenv("CV_BATCH_TW");       // stencil tile row width: 32 (measured best) or 64 lanes
+  B->tw = gt && atoi(gt) == 64 ? 64 : 32;
   int rc = CV_OK;
   std::vector<BCase> hc(ncases);
   std::vector<uint32_t> tabs;
@@ -796,7 +809,7 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
       b.n = L.d.n;
       b.inv_nf = x->nfluid[l] > 0 ? 1.0 / x->nfluid[l] : 0.0;
       b.kc = stencil_kchunk_batch(L.d.mz);
-      b.sgx = (L.d.mx + 62) / 63; b.sgy = (L.d.my + 3) / 4; b.sgz = (L.d.mz + b.kc - 1) / b.kc;
+      b.sgx = (L.d.mx + B->tw - 2) / (B->tw - 1); b.sgy = (L.d.my + 3) / 4; b.sgz = (L.d.mz + b.kc - 1) / b.kc;
       b.lgx = (L.d.mx + 63) / 64;
       const int rows = L.d.my - 2;
       const int rmax = o.precision != CV_FP64 ? 512 : 256;
