@@ -348,7 +348,7 @@ def main():
     roof = {
         "bound": "hbm",
         "kernel": ("k_stencil<float, 1, 4, true, true> (fine-level operator residual r = b - A x of the FP32 "
-                   "V-cycle, FP16 face coefficients: 40 B per point)" if args.mixed == "mixed16" else
+                   "V-cycle, FP16 face coefficients: 36 B per point)" if args.mixed == "mixed16" else
                    "k_stencil<float, 1, 4, true> (fine-level operator residual r = b - A x of the FP32 V-cycle: "
                    "56 B per point)"),
         "launches": launches,
