@@ -2,9 +2,10 @@
 different multigrid depths and y-line lengths -- ends each fractional step
 with the same bits in every field, and the same FGMRES iteration count, as
 the same case stepped alone by cv_flow_step (Flow_Solver, solvers.c:944-1156),
-in mixed and FP64 precision, with the captured hipGraph and with direct
-launches, with the momentum batched and per case.  Tolerance: exact
-equality."""
+in mixed (FP32 and FP16 coefficients) and FP64 precision, with the captured
+hipGraph and with direct launches, with the momentum batched and per case,
+and with stencil tiles of another row width than the case alone uses.
+Tolerance: exact equality."""
 import numpy as np
 import pytest
 
@@ -35,11 +36,16 @@ def _ctx(case, precision):
 
 # mom "1": the momentum and the rest of the step batched on the batch stream;
 # "0": per case on each case's stream (graph "1": replayed from per-case graphs)
-@pytest.mark.parametrize("precision,graph,mom", [("mixed", "1", "1"), ("fp64", "1", "1"), ("mixed", "1", "0"),
-                                                 ("mixed", "0", "0"), ("mixed16", "1", "1")])
-def test_batch_matches_each_case_alone(monkeypatch, precision, graph, mom):
+# tw: lanes per stencil tile row in the batch (each case alone uses 32); the
+# tile shape must not change any bit
+@pytest.mark.parametrize("precision,graph,mom,tw", [("mixed", "1", "1", "32"), ("fp64", "1", "1", "32"),
+                                                    ("mixed", "1", "0", "32"), ("mixed", "0", "0", "32"),
+                                                    ("mixed16", "1", "1", "32"), ("mixed16", "1", "1", "64"),
+                                                    ("fp64", "1", "1", "64")])
+def test_batch_matches_each_case_alone(monkeypatch, precision, graph, mom, tw):
     monkeypatch.setenv("CV_BATCH_GRAPH", graph)
     monkeypatch.setenv("CV_BATCH_MOM", mom)
+    monkeypatch.setenv("CV_BATCH_TW", tw)
     cl = _cases()
     levels = {serving.levels_for(c.cells) for c in cl}
     assert len(levels) > 1, "the batch must mix multigrid depths"
