@@ -37,6 +37,19 @@ def test_binding_symbol_list_matches_header():
     assert sorted(_lib.SYMBOLS) == declared()
 
 
+def test_binding_constants_match_the_header():
+    """Every integer #define CV_* of the header that the binding names has
+    the header's value (precision modes, smoothers, error codes, ...)."""
+    from curvib_old_amd import _lib
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    consts = {k: int(v) for k, v in re.findall(r"^\s*#define\s+(CV_\w+)\s+\(?(-?\d+)\)?\s*$", txt, flags=re.M)}
+    shared = [k for k in consts if hasattr(_lib, k)]
+    for must in ("CV_FP64", "CV_MIXED", "CV_MIXED16", "CV_OK", "CV_ERR_ARG", "CV_ERR_STATE"):
+        assert must in shared, must
+    bad = {k: (consts[k], getattr(_lib, k)) for k in shared if getattr(_lib, k) != consts[k]}
+    assert not bad, bad
+
+
 @pytest.mark.skipif(not os.path.exists(SO), reason="library not built")
 def test_error_paths_without_gpu():
     """Argument errors are reported before any device call."""
