@@ -528,10 +528,8 @@ __global__ void k_far_fix(G g, int mask, int has_out, C3 csi, C3 eta, C3 zet, V3
 // first: every node reads the Ucat and p of the sweep's start, as the
 // reference reads lUcat and lP.
 // ------------------------------------------------------------------
-__global__ void k_ib_nodes(int n, const int *node, const double *w, int freeslip, double dt, C3 ucat, const double *p,
-                           int with_p, double *out) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
+__device__ __forceinline__ void ib_node_body(int q, const int *node, const double *w, int freeslip, double dt, C3 ucat,
+                                             const double *p, int with_p, double *out) {
   const int *I = node + 4 * q;
   const double *W = w + 12 * q;
   const long a1 = I[1], a2 = I[2], a3 = I[3];
@@ -561,21 +559,27 @@ __global__ void k_ib_nodes(int n, const int *node, const double *w, int freeslip
     o[3] = pn;
   }
 }
-
-__global__ void k_ib_scatter(int n, const int *node, const double *in, V3 ucat, double *p, int with_p) {
+__global__ void k_ib_nodes(int n, const int *node, const double *w, int freeslip, double dt, C3 ucat, const double *p,
+                           int with_p, double *out) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
+  if (q < n) ib_node_body(q, node, w, freeslip, dt, ucat, p, with_p, out);
+}
+
+__device__ __forceinline__ void ib_scatter_body(int q, const int *node, const double *in, V3 ucat, double *p, int with_p) {
   const long t = node[4 * q];
   const double *v = in + 4 * q;
   ucat.x[t] = v[0]; ucat.y[t] = v[1]; ucat.z[t] = v[2];
   if (with_p) p[t] = v[3];
 }
+__global__ void k_ib_scatter(int n, const int *node, const double *in, V3 ucat, double *p, int with_p) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) ib_scatter_body(q, node, in, ucat, p, with_p);
+}
 
 // the fluxes of the faces touching an IB node, from the average of the
 // two cells' Ucat (ibm.c:3166-3224); cells listed x faces, then y, then z
-__global__ void k_ib_faces(G g, int nx, int ny, int nz, const int *cell, C3 ucat, C3 csi, C3 eta, C3 zet, V3 ucont) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nx + ny + nz) return;
+__device__ __forceinline__ void ib_face_body(int q, G g, int nx, int ny, const int *cell, C3 ucat, C3 csi, C3 eta,
+                                             C3 zet, V3 ucont) {
   const long t = cell[q];
   if (q < nx) {
     const long u = t + 1;
@@ -590,6 +594,10 @@ __global__ void k_ib_faces(G g, int nx, int ny, int nz, const int *cell, C3 ucat
     const double ux = (ucat.x[u] + ucat.x[t]) * 0.5, uy = (ucat.y[u] + ucat.y[t]) * 0.5, uz = (ucat.z[u] + ucat.z[t]) * 0.5;
     ucont.z[t] = ux * zet.x[t] + uy * zet.y[t] + uz * zet.z[t];
   }
+}
+__global__ void k_ib_faces(G g, int nx, int ny, int nz, const int *cell, C3 ucat, C3 csi, C3 eta, C3 zet, V3 ucont) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nx + ny + nz) ib_face_body(q, g, nx, ny, cell, ucat, csi, eta, zet, ucont);
 }
 
 // Contra2Cart of the listed interior cells
@@ -1994,12 +2002,56 @@ __global__ void kbm_wall_flux_zero(MB_ARGS) {
 
 }  // namespace
 
+// one case's IB reconstruction buffers, for the batched sweeps
+struct BIbCase {
+  G g;
+  int n, nfx, nfy, nface, ncell, freeslip;
+  double dt;
+  const int *node, *face, *cell;
+  const double *w, *nv;
+  double *out, *p;
+  double *ucat[3], *ucont[3];
+  C3 csi, eta, zet;
+};
+
+// the four IB kernels of a sweep for every case of a batch: blockIdx.y is
+// the case, and each item runs the body the single-case kernels run
+__global__ void kbi_nodes(const BIbCase *cs, int with_p) {
+  const BIbCase &C = cs[blockIdx.y];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < C.n)
+    ib_node_body(q, C.node, C.w, C.freeslip, C.dt, C3{C.ucat[0], C.ucat[1], C.ucat[2]}, C.p, with_p, C.out);
+}
+__global__ void kbi_scatter(const BIbCase *cs, int with_p) {
+  const BIbCase &C = cs[blockIdx.y];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < C.n) ib_scatter_body(q, C.node, C.out, V3{C.ucat[0], C.ucat[1], C.ucat[2]}, C.p, with_p);
+}
+__global__ void kbi_faces(const BIbCase *cs) {
+  const BIbCase &C = cs[blockIdx.y];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < C.nface)
+    ib_face_body(q, C.g, C.nfx, C.nfy, C.face, C3{C.ucat[0], C.ucat[1], C.ucat[2]}, C.csi, C.eta, C.zet,
+                 V3{C.ucont[0], C.ucont[1], C.ucont[2]});
+}
+__global__ void kbi_c2c(const BIbCase *cs) {
+  const BIbCase &C = cs[blockIdx.y];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= C.ncell) return;
+  int i, j, k;
+  C.g.dec(C.cell[q], i, j, k);
+  c2c_cell(C.g, C3{C.ucont[0], C.ucont[1], C.ucont[2]}, C.csi, C.eta, C.zet, C.nv, V3{C.ucat[0], C.ucat[1], C.ucat[2]},
+           i, j, k);
+}
+
 struct CvMomBatch {
   int n = 0;
   hipStream_t s = nullptr;
   std::vector<cv_ctx *> cs;
   MCase *d = nullptr;
   uint32_t *tab = nullptr;
+  BIbCase *ib = nullptr;          // the cases with IB nodes
+  int nib = 0, ib_maxn = 0, ib_maxface = 0, ib_maxcell = 0;
   long t_march = 0, t_tile = 0, t_vec = 0, t_plane = 0, t_one = 0;
   int n_march = 0, n_tile = 0, n_vec = 0, n_plane = 0, n_one = 0;
   unsigned long long all = 0, xper = 0, zper = 0, per = 0;
@@ -2009,6 +2061,7 @@ void cv_mom_batch_destroy(CvMomBatch *mb) {
   if (!mb) return;
   if (mb->d) (void)hipFree(mb->d);
   if (mb->tab) (void)hipFree(mb->tab);
+  if (mb->ib) (void)hipFree(mb->ib);
   delete mb;
 }
 
@@ -2073,8 +2126,36 @@ int cv_mom_batch_create(cv_ctx *const *cases, int n, hipStream_t s, CvMomBatch *
       for (int b = 0; b < cnt[k][c]; b++) tabs.push_back(((uint32_t)c << 24) | (uint32_t)b);
     *len[k] = (int)((long)tabs.size() - *off[k]);
   }
+  std::vector<BIbCase> hib;
+  for (int c = 0; c < n; c++) {
+    cv_ctx *x = cases[c];
+    if (x->ib_n <= 0) continue;
+    CvMetrics &M = x->m;
+    BIbCase b;
+    memset(&b, 0, sizeof(b));
+    b.g = gview(x);
+    b.n = x->ib_n;
+    b.nfx = x->ib_nface[0]; b.nfy = x->ib_nface[1];
+    b.nface = x->ib_nface[0] + x->ib_nface[1] + x->ib_nface[2];
+    b.ncell = x->ib_ncell;
+    b.freeslip = x->ib_mode;
+    b.dt = x->dt;
+    b.node = x->ib_node; b.face = x->ib_face; b.cell = x->ib_cell;
+    b.w = x->ib_w; b.nv = x->lev[x->nlev - 1].nvert;
+    b.out = x->ib_new; b.p = x->p;
+    for (int q = 0; q < 3; q++) { b.ucat[q] = x->ucat[q]; b.ucont[q] = x->ucont[q]; }
+    b.csi = c3(M.csi); b.eta = c3(M.eta); b.zet = c3(M.zet);
+    mb->ib_maxn = std::max(mb->ib_maxn, b.n);
+    mb->ib_maxface = std::max(mb->ib_maxface, b.nface);
+    mb->ib_maxcell = std::max(mb->ib_maxcell, b.ncell);
+    hib.push_back(b);
+  }
+  mb->nib = (int)hib.size();
   hipError_t e = hipMalloc((void **)&mb->d, sizeof(MCase) * n);
   if (e == hipSuccess) e = hipMalloc((void **)&mb->tab, sizeof(uint32_t) * tabs.size());
+  if (e == hipSuccess && mb->nib > 0) e = hipMalloc((void **)&mb->ib, sizeof(BIbCase) * mb->nib);
+  if (e == hipSuccess && mb->nib > 0)
+    e = hipMemcpy(mb->ib, hib.data(), sizeof(BIbCase) * mb->nib, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(mb->d, hc.data(), sizeof(MCase) * n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(mb->tab, tabs.data(), sizeof(uint32_t) * tabs.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -2109,6 +2190,25 @@ int cv_mom_batch_ghost(CvMomBatch *mb, int formbcs, int cart) {
   return CV_OK;
 }
 
+// ibm_interpolation_advanced of every case with IB nodes: the six sweeps of
+// cv_ib_dev_on, each kernel launched once for all those cases
+static int cv_mom_batch_ib(CvMomBatch *mb) {
+  if (mb->nib == 0) return CV_OK;
+  const unsigned nc = (unsigned)mb->nib;
+  const dim3 gn((unsigned)((mb->ib_maxn + BS - 1) / BS), nc);
+  const dim3 gf((unsigned)((mb->ib_maxface + BS - 1) / BS), nc);
+  const dim3 gc((unsigned)((mb->ib_maxcell + BS - 1) / BS), nc);
+  for (int sweep = 0; sweep < 6; sweep++) {
+    const int with_p = sweep == 0;
+    hipLaunchKernelGGL(kbi_nodes, gn, dim3(BS), 0, mb->s, (const BIbCase *)mb->ib, with_p);
+    hipLaunchKernelGGL(kbi_scatter, gn, dim3(BS), 0, mb->s, (const BIbCase *)mb->ib, with_p);
+    if (mb->ib_maxface > 0) hipLaunchKernelGGL(kbi_faces, gf, dim3(BS), 0, mb->s, (const BIbCase *)mb->ib);
+    if (mb->ib_maxcell > 0) hipLaunchKernelGGL(kbi_c2c, gc, dim3(BS), 0, mb->s, (const BIbCase *)mb->ib);
+  }
+  CV_CHECK(hipGetLastError());
+  return CV_OK;
+}
+
 // the level rotation (rot: the cases with a step history) and the explicit
 // RK step of cv_rk_dev for every case
 int cv_mom_batch_rk(CvMomBatch *mb, unsigned long long rot) {
@@ -2116,8 +2216,7 @@ int cv_mom_batch_rk(CvMomBatch *mb, unsigned long long rot) {
   MB_LAUNCH(kbm_vec3, dim3(BS), mb->t_vec, mb->n_vec, mb->all, 1);
   int rc = cv_mom_batch_ghost(mb, 1, 1);
   if (rc) return rc;
-  for (cv_ctx *x : mb->cs)                 // implicitsolver.c:47-51
-    if ((rc = cv_ib_dev_on(x, mb->s))) return rc;
+  if ((rc = cv_mom_batch_ib(mb))) return rc;   // implicitsolver.c:47-51
   MB_LAUNCH(kbm_vec3, dim3(BS), mb->t_vec, mb->n_vec, mb->all, 2);
   const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
   for (int s = 0; s < 4; s++) {
@@ -2129,8 +2228,7 @@ int cv_mom_batch_rk(CvMomBatch *mb, unsigned long long rot) {
     MB_LAUNCH(kbm_wall_flux_zero, dim3(BS), mb->t_plane, mb->n_plane, mb->all);
     if ((rc = cv_mom_batch_ghost(mb, 1, 1))) return rc;
   }
-  for (cv_ctx *x : mb->cs)                 // after the stages, implicitsolver.c:87-90
-    if ((rc = cv_ib_dev_on(x, mb->s))) return rc;
+  if ((rc = cv_mom_batch_ib(mb))) return rc;   // after the stages, implicitsolver.c:87-90
   CV_CHECK(hipGetLastError());
   return CV_OK;
 }
