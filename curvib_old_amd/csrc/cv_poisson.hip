@@ -1228,6 +1228,10 @@ static int half_ops(cv_ctx *c) {
     CV_CHECK(hipMemcpyAsync(c->red_host, c->part, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CV_CHECK(hipStreamSynchronize(c->stream));
     const double gmax = c->red_host[0];
+    if (!std::isfinite(gmax)) {
+      cv_set_error("CV_MIXED16: level %d has a non-finite face coefficient", l);
+      return CV_ERR_NONFINITE;
+    }
     int e = 0;
     if (gmax > 0) frexp(gmax, &e);      // gmax < 2^e
     L.hexp = gmax > 0 ? 14 - e : 0;
@@ -1241,6 +1245,10 @@ static int half_ops(cv_ctx *c) {
     CV_CHECK(hipMemcpyAsync(c->red_host, c->part, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CV_CHECK(hipStreamSynchronize(c->stream));
     const double ipmax = c->red_host[0];
+    if (!std::isfinite(ipmax)) {
+      cv_set_error("CV_MIXED16: level %d has a non-finite y-line factor", l);
+      return CV_ERR_NONFINITE;
+    }
     int ey = 0;
     if (ipmax > 0) frexp(ipmax, &ey);
     L.yexp = ipmax > 0 ? 14 - ey : 0;
