@@ -1009,7 +1009,7 @@ extern "C" int cv_batch_poisson_solve(cv_batch *B, cv_step_stats *st) {
   return CV_OK;
 }
 
-static int b_finish(cv_batch *B, cv_step_stats *st, double tp);
+static int b_finish(cv_batch *B, cv_step_stats *st, double tp, bool batched_ib = false);
 
 // every case runs explicit RK with kernel timing off and has only wall and
 // periodic faces: the whole step around the solve can then run on the
@@ -1091,7 +1091,7 @@ extern "C" int cv_batch_step(cv_batch *B, cv_step_stats *st) {
     float ms = 0;
     CV_CHECK(hipEventElapsedTime(&ms, B->ev_t0, B->ev_t1));
     tp = ms * 1e-3;
-    return b_finish(B, st, tp);
+    return b_finish(B, st, tp, true);
   }
   // momentum, VolumeFlux_rev and PoissonRHS per case on its own stream
   for (int c = 0; c < B->n; c++) {
@@ -1117,7 +1117,11 @@ extern "C" int cv_batch_step(cv_batch *B, cv_step_stats *st) {
 }
 
 // statistics, Divergence and the step count of every case
-static int b_finish(cv_batch *B, cv_step_stats *st, double tp) {
+// batched_ib: the IB reconstruction after Divergence (solvers.c:1095-1101)
+// runs for every case in one batched sweep on the batch stream once all the
+// divergences are read (each was synchronised), and the case streams then
+// wait for it; otherwise per case on its own stream
+static int b_finish(cv_batch *B, cv_step_stats *st, double tp, bool batched_ib) {
   int rc;
   if ((rc = b_stats(B, st, tp))) return rc;
   int conv_rc = CV_OK;
@@ -1127,12 +1131,17 @@ static int b_finish(cv_batch *B, cv_step_stats *st, double tp) {
       CV_CHECK(hipMemcpyAsync(x->red_host + 110, x->dsc + CV_DSC_FLUXIN, 2 * sizeof(double), hipMemcpyDeviceToHost,
                               x->stream));
     if ((rc = cv_divergence_dev(x, &st[c].maxdiv, &st[c].maxdiv_idx))) return rc;
-    if ((rc = cv_ib_dev(x))) return rc;                 // solvers.c:1095-1101
+    if (!batched_ib && (rc = cv_ib_dev(x))) return rc;  // solvers.c:1095-1101
     if (x->inlet || x->outlet) { st[c].flux_in = x->red_host[110]; st[c].flux_out = x->red_host[111]; }
     x->last = st[c];
     x->steps_done++;
     if (!std::isfinite(st[c].maxdiv)) return CV_ERR_NONFINITE;
     if (B->rc[c] && !conv_rc) conv_rc = B->rc[c];
+  }
+  if (batched_ib) {
+    if ((rc = cv_mom_batch_ib(B->mom))) return rc;
+    CV_CHECK(hipEventRecord(B->ev_solved, B->stream));
+    for (int c = 0; c < B->n; c++) CV_CHECK(hipStreamWaitEvent(B->cs[c]->stream, B->ev_solved, 0));
   }
   return conv_rc;
 }

@@ -188,6 +188,7 @@ int cv_mom_batch_create(cv_ctx *const *cases, int n, hipStream_t s, CvMomBatch *
 void cv_mom_batch_destroy(CvMomBatch *mb);
 int cv_mom_batch_rk(CvMomBatch *mb, unsigned long long rot);
 int cv_mom_batch_ghost(CvMomBatch *mb, int formbcs, int cart);
+int cv_mom_batch_ib(CvMomBatch *mb);   // the IB reconstruction of every case with IB nodes, on the batch stream
 
 __device__ inline int cv_wrap(int i, int m, int per) {
   return per ? (i == m - 1 ? 1 : (i == 0 ? m - 2 : i)) : i;

@@ -2192,7 +2192,7 @@ int cv_mom_batch_ghost(CvMomBatch *mb, int formbcs, int cart) {
 
 // ibm_interpolation_advanced of every case with IB nodes: the six sweeps of
 // cv_ib_dev_on, each kernel launched once for all those cases
-static int cv_mom_batch_ib(CvMomBatch *mb) {
+int cv_mom_batch_ib(CvMomBatch *mb) {
   if (mb->nib == 0) return CV_OK;
   const unsigned nc = (unsigned)mb->nib;
   const dim3 gn((unsigned)((mb->ib_maxn + BS - 1) / BS), nc);
