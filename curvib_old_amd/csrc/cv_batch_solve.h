@@ -314,6 +314,45 @@ struct BFg {
   int its = 0, jn = 0;
 };
 
+// Divergence (main.c:469-593) of every case at once: block (b, c) reduces
+// case c's share b of nb into the case's partials, then one block per case
+// reduces those with the same order rule (largest value, then smallest
+// index) into out[2c], out[2c+1]; the result equals cv_divergence_dev's
+struct BDiv {
+  Idx I;
+  const double *ux, *uy, *uz, *aj, *nv;
+  long n;
+  double *part;
+};
+__global__ void kbd_divmax(const BDiv *cs, int nb) {
+  const BDiv &C = cs[blockIdx.y];
+  divmax_block(C.I, C.ux, C.uy, C.uz, C.aj, C.nv, C.n, C.part, C.part + nb, blockIdx.x, nb);
+}
+__global__ void kbd_final(const BDiv *cs, int nb, double *out) {
+  __shared__ double sv[BS];
+  __shared__ long si[BS];
+  const BDiv &C = cs[blockIdx.x];
+  double best = -1.0;
+  long bi = 0;
+  for (int b = threadIdx.x; b < nb; b += BS) {
+    const double v = C.part[b];
+    const long ii = (long)C.part[nb + b];
+    if (v > best || (v == best && ii < bi)) { best = v; bi = ii; }
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double v2 = sv[threadIdx.x + s];
+      const long i2 = si[threadIdx.x + s];
+      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[2 * blockIdx.x] = sv[0]; out[2 * blockIdx.x + 1] = (double)si[0]; }
+}
+
 struct cv_batch {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -325,6 +364,7 @@ struct cv_batch {
   BCase *dcases = nullptr;
   BCtl *dctl = nullptr, *hctl = nullptr;
   double *dbsc = nullptr, *hbsc = nullptr, *dgat = nullptr, *hgat = nullptr;
+  BDiv *ddiv = nullptr;                       // per-case Divergence operands
   uint32_t *dtab = nullptr;
   BTab st[CV_MAXLEV], vc[CV_MAXLEV], rd[CV_MAXLEV], ln[CV_MAXLEV][3];
   // the same tables restricted to the cases that go deeper [0] or bottom out [1] at the depth
@@ -717,6 +757,7 @@ static void b_release(cv_batch *B) {
   if (B->dcases) (void)hipFree(B->dcases);
   if (B->dctl) (void)hipFree(B->dctl);
   if (B->dbsc) (void)hipFree(B->dbsc);
+  if (B->ddiv) (void)hipFree(B->ddiv);
   if (B->dgat) (void)hipFree(B->dgat);
   if (B->dtab) (void)hipFree(B->dtab);
   if (B->hctl) (void)hipHostFree(B->hctl);
@@ -881,6 +922,20 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
   if (!tabs.empty()) CV_CHECK(hipMemcpy(B->dtab, tabs.data(), sizeof(uint32_t) * tabs.size(), hipMemcpyHostToDevice));
   CV_CHECK(hipMemcpy(B->dbsc, B->hbsc, sizeof(double) * BSC * ncases, hipMemcpyHostToDevice));
   if ((rc = cv_mom_batch_create(cases, ncases, B->stream, &B->mom))) { b_release(B); return rc; }
+  {
+    std::vector<BDiv> hd(ncases);
+    for (int c = 0; c < ncases; c++) {
+      cv_ctx *x = cases[c];
+      CvLevel &F = x->lev[x->nlev - 1];
+      hd[c] = BDiv{idx_of(F), x->ucont[0], x->ucont[1], x->ucont[2], F.aj, F.nvert, F.d.n, x->part};
+    }
+    if (hipMalloc((void **)&B->ddiv, sizeof(BDiv) * ncases) != hipSuccess) {
+      b_release(B);
+      cv_set_error("cv_batch_create: out of device memory");
+      return CV_ERR_OOM;
+    }
+    CV_CHECK(hipMemcpy(B->ddiv, hd.data(), sizeof(BDiv) * ncases, hipMemcpyHostToDevice));
+  }
   B->gpre.assign(ncases, nullptr);
   B->gpost.assign(ncases, nullptr);
   B->ev_case.resize(ncases);
@@ -1009,7 +1064,7 @@ extern "C" int cv_batch_poisson_solve(cv_batch *B, cv_step_stats *st) {
   return CV_OK;
 }
 
-static int b_finish(cv_batch *B, cv_step_stats *st, double tp, bool batched_ib = false);
+static int b_finish(cv_batch *B, cv_step_stats *st, double tp, bool batched = false);
 
 // every case runs explicit RK with kernel timing off and has only wall and
 // periodic faces: the whole step around the solve can then run on the
@@ -1117,28 +1172,39 @@ extern "C" int cv_batch_step(cv_batch *B, cv_step_stats *st) {
 }
 
 // statistics, Divergence and the step count of every case
-// batched_ib: the IB reconstruction after Divergence (solvers.c:1095-1101)
-// runs for every case in one batched sweep on the batch stream once all the
-// divergences are read (each was synchronised), and the case streams then
-// wait for it; otherwise per case on its own stream
-static int b_finish(cv_batch *B, cv_step_stats *st, double tp, bool batched_ib) {
+// batched (the whole step ran on the batch stream): Divergence of every
+// case in two launches and one read, then the IB reconstruction after it
+// (solvers.c:1095-1101) in one batched sweep, and the case streams wait for
+// it; otherwise both per case on the case's stream
+static int b_finish(cv_batch *B, cv_step_stats *st, double tp, bool batched) {
   int rc;
   if ((rc = b_stats(B, st, tp))) return rc;
+  if (batched) {
+    hipLaunchKernelGGL(kbd_divmax, dim3(CV_RED_GRID, (unsigned)B->n), dim3(BS), 0, B->stream, (const BDiv *)B->ddiv,
+                       CV_RED_GRID);
+    hipLaunchKernelGGL(kbd_final, dim3((unsigned)B->n), dim3(BS), 0, B->stream, (const BDiv *)B->ddiv, CV_RED_GRID,
+                       B->dgat);
+    CV_CHECK(hipMemcpyAsync(B->hgat, B->dgat, 2 * (size_t)B->n * sizeof(double), hipMemcpyDeviceToHost, B->stream));
+    CV_CHECK(hipStreamSynchronize(B->stream));
+    for (int c = 0; c < B->n; c++) { st[c].maxdiv = B->hgat[2 * c]; st[c].maxdiv_idx = (long)B->hgat[2 * c + 1]; }
+  }
   int conv_rc = CV_OK;
   for (int c = 0; c < B->n; c++) {
     cv_ctx *x = B->cs[c];
-    if (x->inlet || x->outlet)
+    if (x->inlet || x->outlet) {
       CV_CHECK(hipMemcpyAsync(x->red_host + 110, x->dsc + CV_DSC_FLUXIN, 2 * sizeof(double), hipMemcpyDeviceToHost,
                               x->stream));
-    if ((rc = cv_divergence_dev(x, &st[c].maxdiv, &st[c].maxdiv_idx))) return rc;
-    if (!batched_ib && (rc = cv_ib_dev(x))) return rc;  // solvers.c:1095-1101
+      if (batched) CV_CHECK(hipStreamSynchronize(x->stream));
+    }
+    if (!batched && (rc = cv_divergence_dev(x, &st[c].maxdiv, &st[c].maxdiv_idx))) return rc;
+    if (!batched && (rc = cv_ib_dev(x))) return rc;     // solvers.c:1095-1101
     if (x->inlet || x->outlet) { st[c].flux_in = x->red_host[110]; st[c].flux_out = x->red_host[111]; }
     x->last = st[c];
     x->steps_done++;
     if (!std::isfinite(st[c].maxdiv)) return CV_ERR_NONFINITE;
     if (B->rc[c] && !conv_rc) conv_rc = B->rc[c];
   }
-  if (batched_ib) {
+  if (batched) {
     if ((rc = cv_mom_batch_ib(B->mom))) return rc;
     CV_CHECK(hipEventRecord(B->ev_solved, B->stream));
     for (int c = 0; c < B->n; c++) CV_CHECK(hipStreamWaitEvent(B->cs[c]->stream, B->ev_solved, 0));

@@ -1890,14 +1890,18 @@ int cv_projection_dev(cv_ctx *c) {
 // ====================================================================
 // Divergence (main.c:469-593): max and first index of the maximum
 // ====================================================================
-__global__ static void k_divmax(Idx I, const double *ux, const double *uy, const double *uz, const double *aj,
-                                const double *nv, long n, double *pv, double *pi) {
+// block bid of nb: the largest |div| of its grid-stride share and its first
+// index (ties go to the smaller index, so the result does not depend on
+// the partition)
+__device__ __forceinline__ void divmax_block(Idx I, const double *ux, const double *uy, const double *uz,
+                                             const double *aj, const double *nv, long n, double *pv, double *pi,
+                                             int bid, int nb) {
   __shared__ double sv[BS];
   __shared__ long si[BS];
   double best = -1.0;
   long bi = 0;
   const long sj = I.mx, sk = (long)I.mx * I.my;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+  for (long t = bid * (long)BS + threadIdx.x; t < n; t += (long)nb * BS) {
     int i, j, k;
     I.dec(t, i, j, k);
     double dv = 0.0;
@@ -1918,7 +1922,11 @@ __global__ static void k_divmax(Idx I, const double *ux, const double *uy, const
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { pv[blockIdx.x] = sv[0]; pi[blockIdx.x] = (double)si[0]; }
+  if (threadIdx.x == 0) { pv[bid] = sv[0]; pi[bid] = (double)si[0]; }
+}
+__global__ static void k_divmax(Idx I, const double *ux, const double *uy, const double *uz, const double *aj,
+                                const double *nv, long n, double *pv, double *pi) {
+  divmax_block(I, ux, uy, uz, aj, nv, n, pv, pi, blockIdx.x, gridDim.x);
 }
 
 int cv_divergence_dev(cv_ctx *c, double *maxdiv, long *idx) {
