@@ -26,13 +26,13 @@
  * (rhs.c:2803-2861) is reproduced in loop order; its first read is of an
  * uninitialised variable in the reference and is 0 here.
  *
- * Deliberate deviations (each also in DESIGN.md):
+ * Deliberate deviations (the GPU library makes the same ones):
  *  - no-slip walls (bctype 1): the reference never resets the wall-normal
  *    face flux, so FormFunction1's interior loop drives the flux through the
  *    upper wall face (j=my-2 etc.) while the lower one keeps its initial
  *    value, which makes the Poisson right-hand side incompatible.  After each
- *    RK stage the build sets the wall-normal flux of bctype-1 faces to the
- *    wall value 0, as FormBCS does for the slip wall (bcs.c:4745-4853).
+ *    RK stage the wall-normal flux of bctype-1 faces is set to the wall
+ *    value 0, as FormBCS does for the slip wall (bcs.c:4745-4853).
  *  - GhostNodeVelocity's two periodic copies that read ".x" from k+2
  *    (bcs.c:4099, 4159) are taken as full copies from the wrapped plane.
  */
