@@ -3,11 +3,17 @@
  * fields, grid hierarchy, FormMetrics + Update_Metrics_PBC, nvert
  * restriction.
  *
- * Parity pins (tests/test_oracle_*.py):
- *   FormMetrics        closed forms on uniform / affine grids, GCL
- *                      (MetricsDivergence, metrics.c:1551-1625) <= 1e-13,
- *                      2nd-order convergence against the analytic mapping.
- *   MyNvertRestriction brute-force restatement in numpy on small masks.
+ * Parity pins:
+ *   FormMetrics        closed forms on uniform and affine grids (face vectors
+ *                      = lattice cofactors, 1/J = triple product;
+ *                      tests/test_oracle_pins.py); the geometric conservation
+ *                      law of MetricsDivergence (metrics.c:1551-1625), face
+ *                      vectors of every cell summing to 0 within 1e-13 on the
+ *                      warped channel and on jittered nodes
+ *                      (tests/test_oracle_grid.py).
+ *   MyNvertRestriction a direct numpy evaluation of the two passes on random
+ *                      masks, bit for bit on every level, full and
+ *                      semi-coarsened (tests/test_oracle_grid.py).
  * Only single-rank semantics are restated (xs=0, xe=mx, ...).
  * Supported boundary types: 1 (no-slip wall) on every axis, 7 (periodic)
  * on the i and k axes, 3 (slip) and 6 (far field) on every face but j-, 5 (inlet)

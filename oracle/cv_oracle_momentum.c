@@ -14,17 +14,32 @@
  *   o_flow_step         Flow_Solver          solvers.c:944-1156 (+ the level
  *                       rotation Ucont_o <- Ucont of main.c:1123-1145)
  *
- * Parity pins (tests/test_oracle_pins.py): free-stream preservation on a
- * warped grid (RHS equals the GCL error), Poiseuille flow on a uniform grid
- * (viscous term exact for a quadratic profile, convection zero), QUICK
- * exactness for linear fields, Contra2Cart round trip, RK update linearity
- * in a constant RHS.  Boundaries: the inlet profiles against their closed
- * forms (the duct series solves u_xx + u_yy = -2 with u = 0 on the duct
- * walls), the inlet flux = uin * face area on a uniform grid, the outlet
- * correction's mass balance (fluid outflow = FluxInSum) and a uniform
- * stream through an inlet/outlet box left unchanged by FormBCS.  The stale dpdc/dpde/dpdz carry of FormFunction1
- * (rhs.c:2803-2861) is reproduced in loop order; its first read is of an
- * uninitialised variable in the reference and is 0 here.
+ * Parity pins (closed forms; tests/test_oracle_momentum.py unless noted):
+ *  - Convection: QUICK face values of a cubic profile carried by a flux that
+ *    changes sign (exact for the quadratic part, the Lagrange remainder
+ *    (3/8) e h^3 sign(F) for the cubic part);
+ *  - Viscous: a quadratic velocity on a sheared lattice gives the viscous
+ *    right-hand side (nu1 - nu2) A.(lap u + grad div u) exactly; Poiseuille
+ *    flow on a uniform grid (test_oracle_pins.py);
+ *  - FormFunction1: -(face area).grad p for a linear p on uniform and
+ *    sheared lattices; a divergence-free shear wave on a lattice whose j
+ *    face areas vary along j (the averaged basis of rct); free-stream
+ *    preservation on the warped channel;
+ *  - RungeKutta: a viscous shear-wave mode is multiplied by
+ *    1 + s + s^2/2 + s^3/6 + s^4/24;
+ *  - Contra2Cart: constant flow round trip on a warped grid with a body;
+ *    CalcRHS/ImpRK: BDF1/BDF2 exactness, the boundary zero set, the stopping
+ *    test (test_oracle_pins.py).
+ *  Boundaries (test_oracle_pins.py): the inlet profiles against their
+ *  closed forms (the duct series solves u_xx + u_yy = -2 with u = 0 on the
+ *  duct walls), the inlet flux = uin * face area on a uniform grid, the
+ *  outlet correction's mass balance (fluid outflow = FluxInSum) and a
+ *  uniform stream through an inlet/outlet box left unchanged by FormBCS.
+ *  tests/test_oracle_mutations.py plants a mistake in each of these
+ *  routines and checks that a pin fails.  The stale dpdc/dpde/dpdz carry of
+ *  FormFunction1 (rhs.c:2803-2861) is reproduced in loop order; its first
+ *  read is of an uninitialised variable in the reference and is 0 here;
+ *  no closed form reaches it (parity unpinned).
  *
  * Deliberate deviations (the GPU library makes the same ones):
  *  - no-slip walls (bctype 1): the reference never resets the wall-normal

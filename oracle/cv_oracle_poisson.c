@@ -20,11 +20,19 @@
  *   o_divergence       Divergence         main.c:469-593
  *
  * Parity pins: the operator against the textbook 7-point Laplacian on
- * uniform grids (cross coefficients vanish), A*1 = 0 on fluid rows, a dense
- * numpy solve on small grids (tests/test_oracle_poisson.py); the solve by
- * its true residual; projection/divergence by Maxdiv after a converged
- * solve.  The converged Phi is normalised by the reference's nullspace rule
- * (zero mean over interior fluid cells, zero on boundary/solid cells).
+ * uniform grids (cross coefficients vanish) with its Neumann eigenpairs,
+ * A*1 = 0 on fluid rows of a warped grid with a body, R.1 = P.1 = 1, the
+ * nullspace rule in closed form, the solve against a dense least-squares
+ * solution, a residual-scale divergence after a step
+ * (tests/test_oracle_pins.py); PoissonRHS of a known divergence, Projection
+ * subtracting (face area).grad phi for a linear phi on uniform and sheared
+ * lattices, UpdatePressure p += phi, VolumeFlux_rev cancelling the net
+ * fluid-to-IB flux with one correction per unit face area, Divergence on
+ * planted maxima and ties (tests/test_oracle_poisson.py).
+ * tests/test_oracle_mutations.py plants a mistake in each of these routines
+ * and checks that a pin fails.  The converged Phi is normalised by the
+ * reference's nullspace rule (zero mean over interior fluid cells, zero on
+ * boundary/solid cells).
  */
 #include "cv_oracle_int.h"
 #include <stdio.h>
