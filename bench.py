@@ -12,13 +12,15 @@ Projection and Divergence, with the immersed-boundary reconstruction
 (ibm_interpolation_advanced) of the sphere's IB nodes where the reference
 runs it with -imm 1 (--no-ib leaves the IB nodes held).
 
-Two contexts run the same steps from the same initial state: the FP64 path
-(FP64 V-cycle) and the mixed path (FP32 V-cycle inside the FP64 FGMRES).
-``value`` is the mixed path's steps/s; the FP64 figures, their ratio, the
-per-step pressure error of the mixed path against the FP64 path and the
-per-step Poisson residuals sit beside it.  All inputs are resident in HBM
-before the timed region starts; every step call returns after a
-synchronisation of the library's stream.
+Three contexts run the same steps from the same initial state: the FP64
+path (FP64 V-cycle), the mixed path (the FP32 V-cycle with FP16-stored face
+coefficients inside the FP64 FGMRES, CV_MIXED16) and, under "mixed_fp32",
+the FP32 V-cycle with FP32 coefficients (CV_MIXED).  ``value`` is the mixed
+path's steps/s; the FP64 figures, the ratios, the per-step pressure error of
+each mixed path against the FP64 path and the per-step Poisson residuals
+sit beside it.  All inputs are resident in HBM before the timed region
+starts; every step call returns after a synchronisation of the library's
+stream.
 
 ``roofline``: the fine-level operator residual kernel of the V-cycle (the
 most launched stencil of the step), its algorithmic bytes per launch over
@@ -26,10 +28,17 @@ its average device-event duration on the library stream, measured during
 the timed steps.  ``traffic``: HBM bytes per launch of the same kernel from
 the rocprofv3 FETCH_SIZE/WRITE_SIZE summary under profiles/, when present.
 
+``step_roofline``: the whole step against the same peak, from the byte
+model of the surveyed design (per storage point 4 x 428 + 65 +
+sum_j (173 + V + 16 j) + 8 N + 16 + 307 bytes with the counted FGMRES
+iterations N) over the measured step time.
+
 ``cpu_baseline``: the C oracle (the reference algorithm restated: assembled
 19-point CSR operator, FGMRES(30) with a 3-level V-cycle and block-Jacobi
-ILU(0) Richardson smoothing, RK momentum) timed for one fractional step of
-the same workload at 256x128x128 cells, scaled by cell count.
+ILU(0) Richardson smoothing, one block per thread, RK momentum) timed for
+one fractional step of the same workload at 256x128x128 cells on every host
+thread, scaled by cell count; ``cpu_baseline_1core`` the same on one thread
+at 128x64x64 cells.  Both run before any GPU work.
 """
 from __future__ import annotations
 
@@ -146,11 +155,11 @@ def traffic_from_profiles(kernel_tag):
 
 def cpu_baseline(sample_cells, full_cells, threads, ib=True):
     """One fractional step of the oracle on a reduced version of the
-    workload; steps/s at the full size scaled by cell count."""
-    if threads:
-        os.environ["OMP_NUM_THREADS"] = str(threads)
+    workload on ``threads`` host threads (the block-Jacobi smoother takes
+    one block per thread); steps/s at the full size scaled by cell count."""
     from curvib_old_amd import ibsearch
     import oracle.pyoracle as po
+    po.set_threads(threads)
     case = workload_case(sample_cells)
     o, (xyz, nv, _) = po.from_case(case, nlev=3)
     if ib:
@@ -178,15 +187,33 @@ def cpu_baseline(sample_cells, full_cells, threads, ib=True):
     }
 
 
+def step_bytes(its, vcycle):
+    """Algorithmic HBM bytes per storage point of one fractional step, the
+    byte model of the surveyed design with the counted FGMRES iterations N:
+    4 x 428 (FormFunction1 fused with the RK stage update) + 65 (PoissonRHS,
+    VolumeFlux_rev) + sum_j (173 + V + 16 j) (apply, V-cycle,
+    orthogonalisation, normalisation, nullspace of iteration j)
+    + (8 N + 16) (x += Z y) + 307 (Projection, UpdatePressure, Divergence,
+    Contra2Cart), V = 652 B for the FP64 V-cycle, 363 B for the FP32 one
+    with its conversions."""
+    n = int(its)
+    return 4 * 428 + 65 + sum(173 + vcycle + 16 * j for j in range(1, n + 1)) + (8 * n + 16) + 307
+
+
+def step_roofline(npts, its, seconds, vcycle):
+    b = float(sum(step_bytes(n, vcycle) for n in its)) * npts
+    achieved = b / seconds / 1e9
+    return {"bound": "hbm", "model": "4*428 + 65 + sum_j(173 + V + 16j) + (8N + 16) + 307 B per storage point",
+            "V": vcycle, "its": list(its), "bytes_per_step": b / len(its), "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS}
+
+
 def serving_workloads(n_serve, n_ens, seed, log_workers, ib=True):
     """Requests of the serving stream (Poisson arrivals, lam 50/s, grids
     32^3..192^3) and of the ensemble (grids 32^3..128x96x96), inputs built
     in spawned worker processes before any GPU work."""
     from curvib_old_amd import serving
-    rng = np.random.default_rng(seed)
-    serve = [serving.sample_case(rng, lo=(32, 32, 32), hi=(192, 192, 192)) for _ in range(n_serve)]
-    arrivals = serving.poisson_arrivals(n_serve, 50.0, rng)
-    ens = [serving.sample_case(rng, lo=(32, 32, 32), hi=(128, 96, 96)) for _ in range(n_ens)]
+    serve, arrivals, ens = serving.stated_workloads(n_serve, n_ens, seed)
     t0 = time.perf_counter()
     arrays = serving.build_inputs(serve + ens, workers=log_workers, ib=ib)
     log(f"[serving] built {len(arrays)} case inputs in {time.perf_counter() - t0:.1f}s")
@@ -198,17 +225,14 @@ def batch_ensemble_only(args):
     run): one untimed step, then --ensemble-steps timed steps; one JSON line
     with its latency and throughput (for profiling that leg alone)."""
     from curvib_old_amd import serving
-    rng = np.random.default_rng(2026)
-    [serving.sample_case(rng, lo=(32, 32, 32), hi=(192, 192, 192)) for _ in range(args.serving_requests)]
-    serving.poisson_arrivals(args.serving_requests, 50.0, rng)
-    ens = [serving.sample_case(rng, lo=(32, 32, 32), hi=(128, 96, 96)) for _ in range(args.ensemble_cases)]
+    _, _, ens = serving.stated_workloads(args.serving_requests, args.ensemble_cases, 2026)
     arrays = serving.build_inputs(ens, workers=args.build_workers, ib=not args.no_ib)
     r = serving.run_batched(list(zip(ens, arrays)), args.ensemble_steps, args.batch_ensemble_only, log=log,
                             max_levels=args.ensemble_max_levels, **SOLVER)
     print(json.dumps({"workload": f"{len(ens)}-case ensemble, one cv_batch", **r}), flush=True)
 
 
-def run_serving(serve_reqs, arrivals, ens_reqs, nsteps, ens_steps, mixed="mixed16"):
+def run_serving(serve_reqs, arrivals, ens_reqs, nsteps, ens_steps, mixed="mixed16", batch_steps=100):
     """The serving and ensemble legs in FP64 and in the mixed leg's precision
     (reported under the "mixed" keys)."""
     from curvib_old_amd import serving
@@ -243,9 +267,9 @@ def run_serving(serve_reqs, arrivals, ens_reqs, nsteps, ens_steps, mixed="mixed1
             f"failed {r['failed']}")
     # the same ensemble as one batch: one launch per multigrid depth for all cases
     for key, prec in precs:
-        r = serving.run_batched(ens_reqs, ens_steps, prec, **opts)
+        r = serving.run_batched(ens_reqs, batch_steps, prec, **opts)
         out[f"ensemble_batched_{key}"] = r
-        log(f"[ensemble batched {prec}] {r['cases']} cases x {ens_steps} steps: {r['makespan_s']:.2f} s, step p50 "
+        log(f"[ensemble batched {prec}] {r['cases']} cases x {batch_steps} steps: {r['makespan_s']:.2f} s, step p50 "
             f"{r['step_latency'].get('p50_ms', 0):.1f} ms p99 {r['step_latency'].get('p99_ms', 0):.1f} ms, "
             f"{r['cell_updates_per_s'] / 1e6:.1f} M cell-updates/s, max its {r['max_its']}")
     return out
@@ -261,7 +285,9 @@ def main():
     ap.add_argument("--coarsen-auto", type=int, default=1)
     ap.add_argument("--semi", default="0,1,0")
     ap.add_argument("--cpu-sample", default="256x128x128")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS, else every core")
+    ap.add_argument("--cpu1-sample", default="128x64x64", help="the one-core CPU sample")
+    ap.add_argument("--no-fp32-leg", action="store_true", help="skip the CV_MIXED (FP32 coefficient) leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-ib", action="store_true", help="no immersed-boundary reconstruction")
@@ -281,6 +307,8 @@ def main():
     ap.add_argument("--serving-steps", type=int, default=20)
     ap.add_argument("--ensemble-cases", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=5)
+    ap.add_argument("--ensemble-batch-steps", type=int, default=100,
+                    help="timed steps of the batched ensemble (its p99 needs about 100 samples)")
     ap.add_argument("--build-workers", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     SOLVER.update(restart=args.restart, pre_sweeps=args.pre, post_sweeps=args.post, coarse_sweeps=args.coarse,
@@ -301,6 +329,12 @@ def main():
                                      ib=not args.no_ib)
 
     cells = cells_of(args.cells)
+    # the CPU legs run before any GPU work: all host threads, then one
+    cpu = cpu1 = None
+    if not args.no_cpu:
+        nthr = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        cpu = cpu_baseline(cells_of(args.cpu_sample), cells, nthr, ib=not args.no_ib)
+        cpu1 = cpu_baseline(cells_of(args.cpu1_sample), cells, 1, ib=not args.no_ib)
     semi = tuple(int(v) for v in args.semi.split(","))
     from curvib_old_amd import cases
     case = workload_case(cells)
@@ -319,6 +353,16 @@ def main():
 
     t64, its64, _ = timed_steps(u64, "fp64", args.warmup, args.steps)
     tmx, itsmx, pr = timed_steps(umx, args.mixed, args.warmup, args.steps, prof=("resid", -1, 1))
+    # the north star's FP32 V-cycle (FP32 coefficients) as a leg of its own
+    leg32 = None
+    if args.mixed != "mixed" and not args.no_fp32_leg:
+        u32 = make_ctx(case, arrays, args.levels, semi, "mixed", args.coarsen_auto, ib)
+        t32, its32, _ = timed_steps(u32, "mixed", args.warmup, args.steps)
+        acc32 = [] if args.no_accuracy else accuracy_pass(u64, u32, arrays, args.steps)
+        u32.close()
+        leg32 = {"precision": "mixed", "steps_per_s": args.steps / t32, "ms_per_step": 1e3 * t32 / args.steps,
+                 "its": its32, "cell_updates_per_s": float(np.prod(cells)) * args.steps / t32,
+                 "over_fp64": t64 / t32, "accuracy": acc32}
 
     acc = [] if args.no_accuracy else accuracy_pass(u64, umx, arrays, args.steps)
     # host-array exchange of a drop-in caller that hands Ucont in and takes
@@ -335,7 +379,7 @@ def main():
 
     srv = None
     if serve_in is not None:
-        srv = run_serving(*serve_in, args.serving_steps, args.ensemble_steps, args.mixed)
+        srv = run_serving(*serve_in, args.serving_steps, args.ensemble_steps, args.mixed, args.ensemble_batch_steps)
         serve_in = None
 
     ncell = float(np.prod(cells))
@@ -362,9 +406,9 @@ def main():
         "traffic": traffic_from_profiles("k_stencil<float, 1>") if args.mixed == "mixed16" else None,
     }
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(cells_of(args.cpu_sample), cells, args.cpu_threads, ib=not args.no_ib)
+    step_roof = {"fp64": step_roofline(npts, its64, t64, 652), "mixed": step_roofline(npts, itsmx, tmx, 363)}
+    if leg32:
+        step_roof["mixed_fp32"] = step_roofline(npts, leg32["its"], args.steps / leg32["steps_per_s"], 363)
 
     out = {
         "metric": METRIC,
@@ -391,6 +435,7 @@ def main():
         "mixed": {"precision": args.mixed, "steps_per_s": sps_mx, "ms_per_step": 1e3 * tmx / args.steps,
                   "its": itsmx, "cell_updates_per_s": ncell * sps_mx},
         "mixed_over_fp64": sps_mx / sps_64,
+        "mixed_fp32": leg32,
         "host_exchange_ms_per_step": xchg_ms,
         "pcie_inclusive_mixed_ms_per_step": 1e3 * tmx / args.steps + xchg_ms,
         "accuracy": acc,
@@ -412,8 +457,11 @@ def main():
         }),
         "serving": srv,
         "roofline": roof,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_1core": cpu1,
         "gpu_mixed_over_cpu": (sps_mx / cpu["value"]) if cpu else None,
+        "gpu_mixed_over_cpu_1core": (sps_mx / cpu1["value"]) if cpu1 else None,
     }
     print(json.dumps(out), flush=True)
 

@@ -337,7 +337,7 @@ __global__ void kbd_final(const BDiv *cs, int nb, double *out) {
   for (int b = threadIdx.x; b < nb; b += BS) {
     const double v = C.part[b];
     const long ii = (long)C.part[nb + b];
-    if (v > best || (v == best && ii < bi)) { best = v; bi = ii; }
+    if (div_beats(v, ii, best, bi)) { best = v; bi = ii; }
   }
   sv[threadIdx.x] = best;
   si[threadIdx.x] = bi;
@@ -346,7 +346,7 @@ __global__ void kbd_final(const BDiv *cs, int nb, double *out) {
     if (threadIdx.x < s) {
       const double v2 = sv[threadIdx.x + s];
       const long i2 = si[threadIdx.x + s];
-      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
+      if (div_beats(v2, i2, sv[threadIdx.x], si[threadIdx.x])) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
     }
     __syncthreads();
   }
@@ -626,6 +626,9 @@ static int b_fgmres(cv_batch *B) {
     f.bnorm = std::sqrt(G[c * BSC + 40]);
     f.tol = std::fmax(B->opt.rtol * f.bnorm, B->opt.atol);
     f.beta = f.rnorm = f.bnorm;
+    // a case whose right-hand side is not finite leaves the solve; the
+    // others go on (their arithmetic does not depend on it)
+    if (!std::isfinite(f.bnorm)) { B->rc[c] = CV_ERR_NONFINITE; continue; }
     if (f.beta > f.tol && f.its < B->opt.max_it) cyc |= bit(c);
   }
   b_zero<double>(B, 0, B->all, BPHI);
@@ -674,7 +677,12 @@ static int b_fgmres(cv_batch *B) {
         BFg &f = B->fg[c];
         const double hn = std::sqrt(std::fmax(f.hn2, 0.0));
         f.H[(jj + 1) * m + jj] = hn;
-        if (!std::isfinite(hn)) { B->rc[c] = CV_ERR_NONFINITE; return CV_ERR_NONFINITE; }
+        if (!std::isfinite(hn)) {   // this case leaves the solve
+          B->rc[c] = CV_ERR_NONFINITE;
+          keep &= ~bit(c);
+          cyc &= ~bit(c);
+          continue;
+        }
         if (hn > 0) { B->hbsc[c * BSC + 1] = 1.0 / hn; scal |= bit(c); }
         std::vector<double> &H = f.H;
         for (int q = 0; q < jj; q++) {
@@ -730,12 +738,13 @@ static int b_fgmres(cv_batch *B) {
       BFg &f = B->fg[c];
       f.beta = std::sqrt(G[c * BSC + 40]);
       f.rnorm = f.beta;
-      if (!std::isfinite(f.beta)) { B->rc[c] = CV_ERR_NONFINITE; return CV_ERR_NONFINITE; }
+      if (!std::isfinite(f.beta)) { B->rc[c] = CV_ERR_NONFINITE; continue; }
       if (f.beta > f.tol && f.its < B->opt.max_it) next |= bit(c);
     }
     cyc = next;
   }
-  for (int c = 0; c < nc; c++) B->rc[c] = B->fg[c].rnorm <= B->fg[c].tol ? CV_OK : CV_ERR_NOT_CONVERGED;
+  for (int c = 0; c < nc; c++)
+    if (B->rc[c] != CV_ERR_NONFINITE) B->rc[c] = B->fg[c].rnorm <= B->fg[c].tol ? CV_OK : CV_ERR_NOT_CONVERGED;
   // the nullspace rule on phi (as after the single-case solve)
   b_upload_ctl(B, B->all, 0);
   b_nullspace<double>(B, 0, B->all, BPHI);
@@ -1059,9 +1068,10 @@ extern "C" int cv_batch_poisson_solve(cv_batch *B, cv_step_stats *st) {
   double tp = 0;
   if ((rc = b_solve(B, &tp))) return rc;
   if ((rc = b_stats(B, st, tp))) return rc;
+  int worst = CV_OK;
   for (int c = 0; c < B->n; c++)
-    if (B->rc[c]) return B->rc[c];
-  return CV_OK;
+    if (B->rc[c] == CV_ERR_NONFINITE || (B->rc[c] && !worst)) worst = B->rc[c];
+  return worst;
 }
 
 static int b_finish(cv_batch *B, cv_step_stats *st, double tp, bool batched = false);
@@ -1201,8 +1211,11 @@ static int b_finish(cv_batch *B, cv_step_stats *st, double tp, bool batched) {
     if (x->inlet || x->outlet) { st[c].flux_in = x->red_host[110]; st[c].flux_out = x->red_host[111]; }
     x->last = st[c];
     x->steps_done++;
-    if (!std::isfinite(st[c].maxdiv)) return CV_ERR_NONFINITE;
-    if (B->rc[c] && !conv_rc) conv_rc = B->rc[c];
+    // every case's record is complete before a failure is reported; a
+    // non-finite case outranks an unconverged one
+    const int rcc = !std::isfinite(st[c].maxdiv) ? CV_ERR_NONFINITE : B->rc[c];
+    if (rcc == CV_ERR_NONFINITE) { st[c].converged = 0; conv_rc = CV_ERR_NONFINITE; }
+    else if (rcc && !conv_rc) conv_rc = rcc;
   }
   if (batched) {
     if ((rc = cv_mom_batch_ib(B->mom))) return rc;

@@ -243,6 +243,17 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     return CV_ERR_HIP;
   }
   int rc;
+  // a failing HIP call inside the set-up frees what was allocated so far
+#define CV_TRY(expr)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) {                                                                       \
+      cv_set_error("%s:%d %s -> %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_));           \
+      rc = CV_ERR_HIP;                                                                            \
+      goto fail;                                                                                  \
+    }                                                                                             \
+  } while (0)
+  for (int q = 0; q < 4; q++) CV_TRY(hipEventCreate(&c->step_ev[q]));
   // per-level operator storage
   for (int l = 0; l < c->nlev; l++) {
     CvLevel &L = c->lev[l];
@@ -272,7 +283,7 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.xf = (float *)p;
     if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.bf = (float *)p;
     if ((rc = cv_alloc(c, &p, n * sizeof(float)))) goto fail; L.rf = (float *)p;
-    CV_CHECK(hipMemsetAsync(L.nvert, 0, n * sizeof(double), c->stream));
+    CV_TRY(hipMemsetAsync(L.nvert, 0, n * sizeof(double), c->stream));
   }
   {
     const long n = c->lev[c->nlev - 1].d.n;
@@ -290,7 +301,7 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
       for (int q = 0; q < 3; q++) {
         if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail;
         v[q] = (double *)p;
-        CV_CHECK(hipMemsetAsync(v[q], 0, n * sizeof(double), c->stream));
+        CV_TRY(hipMemsetAsync(v[q], 0, n * sizeof(double), c->stream));
       }
     if ((rc = cv_alloc(c, &p, 3 * n * sizeof(double)))) goto fail;
     c->stage = (double *)p;
@@ -298,29 +309,34 @@ extern "C" int cv_ctx_create(const cv_grid_desc *desc, int device, cv_ctx **out)
     for (auto s : st1) {
       if ((rc = cv_alloc(c, &p, n * sizeof(double)))) goto fail;
       *s = (double *)p;
-      CV_CHECK(hipMemsetAsync(*s, 0, n * sizeof(double), c->stream));
+      CV_TRY(hipMemsetAsync(*s, 0, n * sizeof(double), c->stream));
     }
     c->red_len = 1 << 16;
     if ((rc = cv_alloc(c, &p, c->red_len * sizeof(double)))) goto fail; c->red = (double *)p;
     if ((rc = cv_alloc(c, &p, 128 * sizeof(double)))) goto fail; c->dsc = (double *)p;
-    CV_CHECK(hipMemsetAsync(c->dsc, 0, 128 * sizeof(double), c->stream));   // FluxInSum starts at 0
+    CV_TRY(hipMemsetAsync(c->dsc, 0, 128 * sizeof(double), c->stream));   // FluxInSum starts at 0
     if (desc_ubf(desc)) {
       const CvDims &D = c->lev[c->nlev - 1].d;
       const size_t nb = (size_t)6 * ((size_t)D.my * D.mz + (size_t)D.mx * D.mz + (size_t)D.mx * D.my) * sizeof(double);
       if ((rc = cv_alloc(c, &p, nb))) goto fail;
       c->ubf = (double *)p;
-      CV_CHECK(hipMemsetAsync(c->ubf, 0, nb, c->stream));
+      CV_TRY(hipMemsetAsync(c->ubf, 0, nb, c->stream));
     }
     if ((rc = cv_alloc(c, &p, 34 * 1024 * sizeof(double)))) goto fail; c->part = (double *)p;
     c->krylov_m = 0;
-    CV_CHECK(hipHostMalloc((void **)&c->red_host, 4096 * sizeof(double), hipHostMallocDefault));
+    CV_TRY(hipHostMalloc((void **)&c->red_host, 4096 * sizeof(double), hipHostMallocDefault));
     c->V = nullptr; c->Z = nullptr;
   }
-  CV_CHECK(hipStreamSynchronize(c->stream));
+  CV_TRY(hipStreamSynchronize(c->stream));
   *out = c;
   return CV_OK;
 fail:
+#undef CV_TRY
+  (void)hipStreamSynchronize(c->stream);
   for (void *p : c->allocs) (void)hipFree(p);
+  if (c->red_host) (void)hipHostFree(c->red_host);
+  for (auto e : c->step_ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return rc;
@@ -333,6 +349,8 @@ extern "C" int cv_ctx_destroy(cv_ctx *c) {
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->red_host) (void)hipHostFree(c->red_host);
   for (auto e : c->prof.ev) (void)hipEventDestroy(e);
+  for (auto e : c->step_ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return CV_OK;

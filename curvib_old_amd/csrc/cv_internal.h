@@ -125,6 +125,7 @@ struct cv_ctx {
   int ib_ncell;
   int *ib_cell;               // the cells those faces bound (Contra2Cart)
   int ib_gen;                 // bumped whenever the IB buffers are replaced (captured graphs hold them)
+  hipEvent_t step_ev[4];      // cv_flow_step's phase timing (momentum, solve, projection)
 };
 
 // device scalars of the inlet/outlet update (slots of cv_ctx::dsc)
@@ -161,6 +162,11 @@ __global__ void k_axpby(double *y, double a, const double *x, double b, long n);
 int cv_metrics_build(cv_ctx *c, const double *xyz_host);
 int cv_levels_setup_nvert(cv_ctx *c, const double *nvert_host);
 long cv_half_bytes(const cv_grid_desc *desc);
+// cv_set_geometry's transient device scratch: 17 padded Cmpnts fields and 4
+// padded scalars of the finest level, plus its node coordinates
+#define CV_METRIC_SCRATCH_VEC 17
+#define CV_METRIC_SCRATCH_SC 4
+long cv_geometry_scratch_bytes(const cv_grid_desc *desc);
 int cv_poisson_prepare(cv_ctx *c);
 int cv_contra2cart_dev(cv_ctx *c);
 int cv_ghost_update(cv_ctx *c, int formbcs);

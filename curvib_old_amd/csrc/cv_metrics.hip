@@ -92,10 +92,19 @@ static int level_metrics(cv_ctx *c, int l, const double *fine_xyz_host) {
   const long np = g.np();
   const int xper = L.d.xper, zper = L.d.zper;
   const int bct0 = c->bctype[0], bct4 = c->bctype[4];
-  // padded temporaries
-  const int nvec = 17, nsc = 4;
-  double *buf = nullptr;
-  CV_CHECK(hipMalloc((void **)&buf, (size_t)(3 * nvec + nsc) * np * sizeof(double)));
+  // padded temporaries, freed on every return path (an early return waits
+  // for the stream first, so no queued kernel still uses them)
+  struct Scratch {
+    double *p = nullptr;
+    hipStream_t s;
+    ~Scratch() {
+      if (p) { (void)hipStreamSynchronize(s); (void)hipFree(p); }
+    }
+  };
+  const int nvec = CV_METRIC_SCRATCH_VEC, nsc = CV_METRIC_SCRATCH_SC;
+  Scratch sbuf{nullptr, s}, sxyz{nullptr, s};
+  CV_CHECK(hipMalloc((void **)&sbuf.p, (size_t)(3 * nvec + nsc) * np * sizeof(double)));
+  double *buf = sbuf.p;
   CV_CHECK(hipMemsetAsync(buf, 0, (size_t)(3 * nvec + nsc) * np * sizeof(double), s));
   MetricSet M;
   PV *pv[] = {&M.coor, &M.csi, &M.eta, &M.zet, &M.icsi, &M.ieta, &M.izet, &M.jcsi, &M.jeta, &M.jzet,
@@ -111,17 +120,18 @@ static int level_metrics(cv_ctx *c, int l, const double *fine_xyz_host) {
   // ---- node coordinates (storage index IM keeps 0, init.c:666) ----
   PV co = M.coor;
   if (l == c->nlev - 1) {
-    double *d_xyz = nullptr;
     long nn = (long)IM * JM * KM;
-    CV_CHECK(hipMalloc((void **)&d_xyz, 3 * nn * sizeof(double)));
-    CV_CHECK(hipMemcpyAsync(d_xyz, fine_xyz_host, 3 * nn * sizeof(double), hipMemcpyHostToDevice, s));
+    CV_CHECK(hipMalloc((void **)&sxyz.p, 3 * nn * sizeof(double)));
+    const double *d_xyz = sxyz.p;
+    CV_CHECK(hipMemcpyAsync(sxyz.p, fine_xyz_host, 3 * nn * sizeof(double), hipMemcpyHostToDevice, s));
     box(s, 0, IM, 0, JM, 0, KM, [=] __device__(int i, int j, int k) {
       long n = (long)i + (long)IM * (j + (long)JM * k);
       long p = g.ix(i, j, k);
       co.x[p] = d_xyz[3 * n]; co.y[p] = d_xyz[3 * n + 1]; co.z[p] = d_xyz[3 * n + 2];
     });
     CV_CHECK(hipStreamSynchronize(s));
-    CV_CHECK(hipFree(d_xyz));
+    CV_CHECK(hipFree(sxyz.p));
+    sxyz.p = nullptr;
   } else {
     // coarse coordinates = fine coordinates at (2i,2j,2k) (GridRestriction)
     const CvLevel &H = c->lev[l + 1];
@@ -504,7 +514,16 @@ static int level_metrics(cv_ctx *c, int l, const double *fine_xyz_host) {
   }
   CV_CHECK(hipStreamSynchronize(s));
   CV_CHECK(hipFree(buf));
+  sbuf.p = nullptr;
   return CV_OK;
+}
+
+// the device scratch cv_set_geometry holds while it builds the finest level
+// (the largest): the padded temporaries plus the node coordinates
+long cv_geometry_scratch_bytes(const cv_grid_desc *d) {
+  const long px = d->IM + 1 + 2 * PG, py = d->JM + 1 + 2 * PG, pz = d->KM + 1 + 2 * PG;
+  return (long)(3 * CV_METRIC_SCRATCH_VEC + CV_METRIC_SCRATCH_SC) * px * py * pz * (long)sizeof(double) +
+         3L * d->IM * d->JM * d->KM * (long)sizeof(double);
 }
 
 int cv_metrics_build(cv_ctx *c, const double *xyz_host) {

@@ -1278,7 +1278,7 @@ __global__ void __launch_bounds__(BS) k_absmax3(C3 a, C3 b, int diff, long n, do
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     double v0 = a.x[t], v1 = a.y[t], v2 = a.z[t];
     if (diff) { v0 = b.x[t] - v0; v1 = b.y[t] - v1; v2 = b.z[t] - v2; }
-    m = fmax(m, fmax(fabs(v0), fmax(fabs(v1), fabs(v2))));
+    m = cv_nmax(m, cv_nmax(fabs(v0), cv_nmax(fabs(v1), fabs(v2))));
   }
   m = cv_block_max(m, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = m;
@@ -1287,7 +1287,7 @@ __global__ void __launch_bounds__(BS) k_absmax3(C3 a, C3 b, int diff, long n, do
 __global__ void k_final_max(const double *part, int nb, double *out) {
   __shared__ double sh[BS / 64];
   double m = 0.0;
-  for (int q = threadIdx.x; q < nb; q += blockDim.x) m = fmax(m, part[q]);
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) m = cv_nmax(m, part[q]);
   m = cv_block_max(m, sh);
   if (threadIdx.x == 0) out[0] = m;
 }
@@ -1842,8 +1842,7 @@ extern "C" int cv_flow_step(cv_ctx *c, cv_step_stats *st) {
   int rc = ready(c);
   if (rc) return rc;
   cv_step_stats s{};
-  hipEvent_t e[4];
-  for (int q = 0; q < 4; q++) CV_CHECK(hipEventCreate(&e[q]));
+  hipEvent_t *e = c->step_ev;
   CV_CHECK(hipEventRecord(e[0], c->stream));
   if ((rc = cv_step_pre_dev(c, &s))) return rc;
   CV_CHECK(hipEventRecord(e[1], c->stream));
@@ -1863,7 +1862,6 @@ extern "C" int cv_flow_step(cv_ctx *c, cv_step_stats *st) {
   float ms[3];
   for (int q = 0; q < 3; q++) CV_CHECK(hipEventElapsedTime(&ms[q], e[q], e[q + 1]));
   s.t_mom = ms[0] * 1e-3; s.t_p = ms[1] * 1e-3; s.t_pr = ms[2] * 1e-3;
-  for (int q = 0; q < 4; q++) (void)hipEventDestroy(e[q]);
   c->last = s;
   c->steps_done++;
   if (st) *st = s;

@@ -1177,8 +1177,8 @@ __global__ static void k_absmax9(G9 g, long n, unsigned long long *out) {
   double m = 0.0;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
 #pragma unroll
-    for (int q = 0; q < 9; q++) m = fmax(m, fabs(g.p[q][t]));
-  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    for (int q = 0; q < 9; q++) m = cv_nmax(m, fabs(g.p[q][t]));
+  for (int o = 32; o > 0; o >>= 1) m = cv_nmax(m, __shfl_xor(m, o, 64));
   if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 __device__ inline uint32_t half_bits(double v) {
@@ -1192,8 +1192,8 @@ __global__ static void k_pack_line(const double *ya, const double *ip, const dou
 __global__ static void k_absmax1(const double *a, long n, unsigned long long *out) {
   double m = 0.0;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
-    m = fmax(m, fabs(a[t]));
-  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    m = cv_nmax(m, fabs(a[t]));
+  for (int o = 32; o > 0; o >>= 1) m = cv_nmax(m, __shfl_xor(m, o, 64));
   if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
@@ -1522,6 +1522,10 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
   CV_CHECK(hipStreamSynchronize(c->stream));
   const double bnorm = std::sqrt(hh[0]);
   *bnorm_out = bnorm;
+  if (!std::isfinite(bnorm)) {
+    cv_set_error("FGMRES: the right-hand side is not finite");
+    return CV_ERR_NONFINITE;
+  }
   const double tol = std::fmax(c->opt.rtol * bnorm, c->opt.atol);
   hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(BS), 0, c->stream, x, n);
   int its = 0;
@@ -1890,12 +1894,24 @@ int cv_projection_dev(cv_ctx *c) {
 // ====================================================================
 // Divergence (main.c:469-593): max and first index of the maximum
 // ====================================================================
+// (v2, i2) beats (v1, i1): larger, or equal with the smaller index; a NaN
+// beats every number, so a field holding a NaN reports NaN (and the first
+// NaN's index) instead of hiding it
+__host__ __device__ inline bool div_beats(double v2, long i2, double v1, long i1) {
+  const bool n2 = v2 != v2, n1 = v1 != v1;
+  if (n2 != n1) return n2;
+  if (!n2 && v2 != v1) return v2 > v1;
+  return i2 < i1;
+}
 // block bid of nb: the largest |div| of its grid-stride share and its first
 // index (ties go to the smaller index, so the result does not depend on
 // the partition)
-__device__ __forceinline__ void divmax_block(Idx I, const double *ux, const double *uy, const double *uz,
-                                             const double *aj, const double *nv, long n, double *pv, double *pi,
-                                             int bid, int nb) {
+// S: element stride of the flux components (1: planar arrays, 3: the
+// interleaved Cmpnts of the DMDA layout, ux = u, uy = u + 1, uz = u + 2)
+template <int S>
+__device__ __forceinline__ void divmax_block_s(Idx I, const double *ux, const double *uy, const double *uz,
+                                               const double *aj, const double *nv, long n, double *pv, double *pi,
+                                               int bid, int nb) {
   __shared__ double sv[BS];
   __shared__ long si[BS];
   double best = -1.0;
@@ -1906,10 +1922,10 @@ __device__ __forceinline__ void divmax_block(Idx I, const double *ux, const doub
     I.dec(t, i, j, k);
     double dv = 0.0;
     if (i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1) {
-      dv = fabs((ux[t] - ux[t - 1] + uy[t] - uy[t - sj] + uz[t] - uz[t - sk]) * aj[t]);
+      dv = fabs((ux[S * t] - ux[S * (t - 1)] + uy[S * t] - uy[S * (t - sj)] + uz[S * t] - uz[S * (t - sk)]) * aj[t]);
       if (nv[t] + nv[t + sk] + nv[t - sk] + nv[t + sj] + nv[t - sj] + nv[t + 1] + nv[t - 1] > 0.1) dv = 0.;
     }
-    if (dv > best) { best = dv; bi = t; }
+    if (div_beats(dv, t, best, bi)) { best = dv; bi = t; }
   }
   sv[threadIdx.x] = best;
   si[threadIdx.x] = bi;
@@ -1918,11 +1934,16 @@ __device__ __forceinline__ void divmax_block(Idx I, const double *ux, const doub
     if (threadIdx.x < s) {
       double v2 = sv[threadIdx.x + s];
       long i2 = si[threadIdx.x + s];
-      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
+      if (div_beats(v2, i2, sv[threadIdx.x], si[threadIdx.x])) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) { pv[bid] = sv[0]; pi[bid] = (double)si[0]; }
+}
+__device__ __forceinline__ void divmax_block(Idx I, const double *ux, const double *uy, const double *uz,
+                                             const double *aj, const double *nv, long n, double *pv, double *pi,
+                                             int bid, int nb) {
+  divmax_block_s<1>(I, ux, uy, uz, aj, nv, n, pv, pi, bid, nb);
 }
 __global__ static void k_divmax(Idx I, const double *ux, const double *uy, const double *uz, const double *aj,
                                 const double *nv, long n, double *pv, double *pi) {
@@ -1942,7 +1963,7 @@ int cv_divergence_dev(cv_ctx *c, double *maxdiv, long *idx) {
   long bi = 0;
   for (int b = 0; b < nb; b++) {
     long ii = (long)hv[nb + b];
-    if (hv[b] > best || (hv[b] == best && ii < bi)) { best = hv[b]; bi = ii; }
+    if (div_beats(hv[b], ii, best, bi)) { best = hv[b]; bi = ii; }
   }
   if (maxdiv) *maxdiv = best;
   if (idx) *idx = bi;

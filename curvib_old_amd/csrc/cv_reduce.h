@@ -26,9 +26,13 @@ __device__ inline double cv_block_sum(double v, double *sh) {
   return s;
 }
 
+// max that keeps NaN: a NaN operand wins, so a norm or maximum over a field
+// holding a NaN is NaN (fmax would drop it and hide a blown-up step)
+__device__ inline double cv_nmax(double a, double b) { return (b > a || b != b) ? b : a; }
+
 __device__ inline double cv_wave_max(double v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = cv_nmax(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -42,7 +46,7 @@ __device__ inline double cv_block_max(double v, double *sh) {
   double m = v;
   if (threadIdx.x == 0) {
     const int nw = (blockDim.x + 63) >> 6;
-    for (int q = 0; q < nw; q++) m = fmax(m, sh[q]);
+    for (int q = 0; q < nw; q++) m = cv_nmax(m, sh[q]);
   }
   return m;
 }

@@ -18,6 +18,7 @@
 // single-case bits).  Each step's wall-clock latency is recorded.
 #include "cv_internal.h"
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -39,7 +40,8 @@ struct Case {
   std::vector<cv_ib_node> ib;   // IB node records (none: IB cells held)
   int ib_mode = CV_IB_NOSLIP;
   int nsteps = 0;
-  long bytes = 0;
+  long bytes = 0;     // resident device bytes of the case's context
+  long scratch = 0;   // cv_set_geometry's transient scratch, held from admission until the metrics are built
   double t_submit = 0, t_admit = 0, t_done = 0;
   int status = CV_CASE_QUEUED;
   int rc = CV_OK;
@@ -68,6 +70,17 @@ struct cv_server {
   double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
 };
 
+// the admission reserved the geometry scratch beside the context; once the
+// metrics are built (or failed) it is free again
+static void release_scratch(cv_server *S, Case *cs) {
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->in_use -= cs->scratch;
+    cs->scratch = 0;
+  }
+  S->cv_work.notify_all();
+}
+
 // the case's device context from its inputs (which are then dropped)
 static int open_case(cv_server *S, Case *cs, cv_ctx **out) {
   (void)hipSetDevice(S->device);
@@ -75,6 +88,7 @@ static int open_case(cv_server *S, Case *cs, cv_ctx **out) {
   int rc = cv_ctx_create(&cs->g, S->device, &c);
   if (rc == CV_OK) rc = cv_set_solver(c, &cs->o);
   if (rc == CV_OK) rc = cv_set_geometry(c, cs->xyz.data());
+  release_scratch(S, cs);
   if (rc == CV_OK) rc = cv_set_nvert(c, cs->nv.data());
   if (rc == CV_OK && !cs->ib.empty()) rc = cv_set_ib_nodes(c, cs->ib.data(), (int)cs->ib.size(), cs->ib_mode);
   if (rc == CV_OK) rc = cv_put_field(c, CV_F_UCONT, -1, cs->uc.data());
@@ -132,9 +146,9 @@ static void retire(cv_server *S, Case *cs, int rc) {
 static Case *admit_one(cv_server *S) {
   if (S->pending.empty() || S->running >= S->slots) return nullptr;
   Case *h = S->cases[S->pending.front()].get();
-  if (S->in_use + h->bytes > S->budget) return nullptr;
+  if (S->in_use + h->bytes + h->scratch > S->budget) return nullptr;
   S->pending.pop_front();
-  S->in_use += h->bytes;
+  S->in_use += h->bytes + h->scratch;
   if (S->in_use > S->peak) S->peak = S->in_use;
   S->running++;
   h->status = CV_CASE_RUNNING;
@@ -151,13 +165,13 @@ static void worker(cv_server *S) {
         if (S->stop) return true;
         if (S->pending.empty()) return false;
         const Case *h = S->cases[S->pending.front()].get();
-        return S->in_use + h->bytes <= S->budget;
+        return S->in_use + h->bytes + h->scratch <= S->budget;
       });
       if (S->stop) return;
       const long id = S->pending.front();
       S->pending.pop_front();
       cs = S->cases[id].get();
-      S->in_use += cs->bytes;
+      S->in_use += cs->bytes + cs->scratch;
       if (S->in_use > S->peak) S->peak = S->in_use;
       S->running++;
       cs->status = CV_CASE_RUNNING;
@@ -215,7 +229,8 @@ static void scheduler(cv_server *S) {
       S->cv_work.wait(lk, [&] {
         if (S->stop || !live.empty()) return true;
         if (S->pending.empty()) return false;
-        return S->in_use + S->cases[S->pending.front()]->bytes <= S->budget;
+        const Case *h = S->cases[S->pending.front()].get();
+        return S->in_use + h->bytes + h->scratch <= S->budget;
       });
       if (S->stop && live.empty()) break;
       if (!S->stop)   // at destroy: finish the cases in flight, admit no more
@@ -261,12 +276,21 @@ static void scheduler(cv_server *S) {
         const auto a = clk::now();
         const int r = cv_batch_step(g.b, st.data());
         const double ms = 1e3 * std::chrono::duration<double>(clk::now() - a).count();
-        if (r != CV_OK && r != CV_ERR_NOT_CONVERGED) {
+        if (r != CV_OK && r != CV_ERR_NOT_CONVERGED && r != CV_ERR_NONFINITE) {
           for (Live *l : g.members) { l->cs->rc = r; failed.push_back(l); }
           continue;
         }
-        for (size_t q = 0; q < g.members.size(); q++)
-          record(S, g.members[q], ms, st[q], st[q].converged ? CV_OK : CV_ERR_NOT_CONVERGED);
+        // a case that went non-finite fails alone; the batch carried the
+        // others through the step with the bits they get without it
+        for (size_t q = 0; q < g.members.size(); q++) {
+          const cv_step_stats &x = st[q];
+          if (!std::isfinite(x.maxdiv) || !std::isfinite(x.ksp_bnorm) || !std::isfinite(x.ksp_rnorm)) {
+            g.members[q]->cs->rc = CV_ERR_NONFINITE;
+            failed.push_back(g.members[q]);
+            continue;
+          }
+          record(S, g.members[q], ms, x, x.converged ? CV_OK : CV_ERR_NOT_CONVERGED);
+        }
       } else {
         for (Live *l : g.members) {
           cv_step_stats st{};
@@ -344,8 +368,10 @@ extern "C" int cv_server_submit_ib(cv_server *S, const cv_grid_desc *g, const cv
   if (bytes < 0) return CV_ERR_ARG;
   bytes += ib_bytes(n_ib);
   if (o->precision == CV_MIXED16) bytes += cv_half_bytes(g);
-  if (bytes > S->budget) {
-    cv_set_error("cv_server_submit: case needs %ld bytes, budget is %ld", bytes, S->budget);
+  const long scratch = cv_geometry_scratch_bytes(g);
+  if (bytes + scratch > S->budget) {
+    cv_set_error("cv_server_submit: case needs %ld bytes (%ld while its metrics are built), budget is %ld", bytes,
+                 bytes + scratch, S->budget);
     return CV_ERR_OOM;
   }
   auto cs = std::make_unique<Case>();
@@ -359,6 +385,7 @@ extern "C" int cv_server_submit_ib(cv_server *S, const cv_grid_desc *g, const cv
   cs->ib_mode = ib_mode;
   cs->nsteps = nsteps;
   cs->bytes = bytes;
+  cs->scratch = scratch;
   {
     std::lock_guard<std::mutex> lk(S->mu);
     cs->id = S->next_id++;

@@ -143,12 +143,19 @@ class Server:
         check(L.cv_server_create_mode(int(device), int(budget_bytes), int(slots), m, C.byref(h)),
               "cv_server_create_mode")
         self.h = h
+        self._points = {}       # storage points of each submitted case
 
     def submit(self, case, arrays, nsteps, mglevels=None, semi=SEMI, **solver):
         """arrays: (xyz, nvert, ucont) or (xyz, nvert, ucont, ib) with ib the IB
         node records (idx, val) or None."""
         xyz, nv, uc = (np.ascontiguousarray(a, dtype=np.float64) for a in arrays[:3])
         ib = arrays[3] if len(arrays) > 3 else None
+        IM, JM, KM = case.nodes
+        ns = (IM + 1) * (JM + 1) * (KM + 1)
+        # the library copies exactly these many doubles from each pointer
+        for name, a, want in (("xyz", xyz, 3 * IM * JM * KM), ("nvert", nv, ns), ("ucont", uc, 3 * ns)):
+            if a.size != want:
+                raise ValueError(f"submit: {name} holds {a.size} values, the case needs {want}")
         if mglevels is None:
             mglevels = levels_for(case.cells, semi)
         d = grid_desc(case, mglevels, semi)
@@ -158,6 +165,7 @@ class Server:
         check(_lib.load().cv_server_submit_ib(self.h, C.byref(d), C.byref(o), _dp(xyz), _dp(nv), _dp(uc),
                                               int(nsteps), rec.ctypes.data_as(C.c_void_p), len(rec),
                                               _lib.CV_IB_NOSLIP, C.byref(cid)), "cv_server_submit_ib")
+        self._points[cid.value] = ns
         return cid.value
 
     def wait(self, cid):
@@ -176,7 +184,12 @@ class Server:
         return ms[:n], its[:n]
 
     def fetch(self, cid, field, shape):
+        """p or ucont of a finished case; shape must hold the case's storage
+        points (mz, my, mx) for p, (mz, my, mx, 3) for ucont."""
         out = np.empty(shape)
+        want = self._points[int(cid)] * (3 if field == "ucont" else 1)
+        if out.size != want:
+            raise ValueError(f"fetch: {field} of case {cid} holds {want} values, the buffer {out.size}")
         check(_lib.load().cv_server_fetch(self.h, int(cid), _lib.FIELD_ID[field], _dp(out)), "cv_server_fetch")
         return out
 
@@ -259,6 +272,20 @@ def run_stream(requests, arrivals, nsteps, precision, budget_bytes=200e9, slots=
 
 def poisson_arrivals(n, lam, rng):
     return np.cumsum(rng.exponential(1.0 / lam, size=n))
+
+
+def stated_workloads(n_serve=50, n_ens=64, seed=2026):
+    """The serving stream and the ensemble of the benchmark, drawn from one
+    seeded generator: ``n_serve`` cases with grids log-uniform in
+    32^3..192^3 and Poisson arrival times at 50 cases/s, then ``n_ens``
+    cases with grids in 32^3..128x96x96 (wall amplitude U[0, 0.1], Re
+    U[100, 2000], a sphere of diameter U[0.1, 0.3] in half the cases).
+    Returns (serve cases, arrival offsets in s, ensemble cases)."""
+    rng = np.random.default_rng(seed)
+    serve = [sample_case(rng, lo=(32, 32, 32), hi=(192, 192, 192)) for _ in range(n_serve)]
+    arrivals = poisson_arrivals(n_serve, 50.0, rng)
+    ens = [sample_case(rng, lo=(32, 32, 32), hi=(128, 96, 96)) for _ in range(n_ens)]
+    return serve, arrivals, ens
 
 
 def run_batched(requests, nsteps, precision, warmup=1, device=0, log=None, max_levels=8, **solver):

@@ -135,6 +135,7 @@ void o_ghostnodevelocity(octx *c);
 double o_divergence(octx *c);
 void o_flow_step(octx *c);
 void o_csr_matvec(const olevel *L, const double *x, double *y);
+void o_set_threads(int n);
 
 #ifdef __cplusplus
 }

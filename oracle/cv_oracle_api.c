@@ -4,6 +4,9 @@
  */
 #include "cv_oracle_int.h"
 #include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static ofield *lookup(octx *c, const char *name, int level) {
   if (level < 0) level = c->nlev - 1;
@@ -101,3 +104,13 @@ void o_rhs_into_b(octx *c) { o_poisson_rhs(c, &c->b); }
 void o_ff1_into_rhs(octx *c) { o_formfunction1(c, &c->rhs); }
 void o_lhs_all(octx *c) { for (int l = 0; l < c->nlev; l++) o_poisson_lhs(c, l); }
 void o_matvec_level(octx *c, int level, const double *x, double *y) { o_csr_matvec(&c->lev[level], x, y); }
+
+/* OpenMP threads of the following parallel regions (the block-Jacobi
+ * smoother takes one block per thread, as PCBJACOBI takes one per rank) */
+void o_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}

@@ -69,6 +69,7 @@ def lib():
         L.o_inlet_profile_ok.restype = C.c_int
         L.o_flux_stats.argtypes = [vp, dp]
         L.o_set_ib_nodes.argtypes = [vp, C.c_int, ip, dp, C.c_int]
+        L.o_set_threads.argtypes = [C.c_int]
         L.o_ibm_interpolation.argtypes = [vp]
         _lib = L
     return _lib
@@ -239,6 +240,11 @@ def from_case(case, nlev=3, semi=(0, 0, 0), coarsen_auto=0):
     o.set_nvert(nv)
     o.put("ucont", uc)
     return o, (xyz, nv, uc)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's later parallel regions."""
+    lib().o_set_threads(int(n))
 
 
 def inlet_uin(profile, xc, yc, zc):

@@ -1,12 +1,13 @@
 """GPU parity: every hot-path routine of the gfx950 library against the CPU
 oracle on identical seeded inputs (small grids the oracle finishes in
 seconds).  Tolerances: FP64 stencils 1e-12 relative (different summation
-order only); converged Poisson fields 1e-9 relative L2 (both solved to
-rtol 1e-13)."""
+order only); fields after a converged step 1e-10 relative L2 (both solved
+to rtol 1e-13), printed."""
 import numpy as np
 import pytest
 
 from curvib_old_amd import cases
+from conftest import assert_fields_match
 
 pytestmark = pytest.mark.gpu
 
@@ -102,9 +103,7 @@ def test_one_step(pair):
     o.call("o_flow_step")
     st = u.Flow_Solver()
     so = o.stats()
-    for f, tol in (("ucont", 1e-10), ("p", 1e-9), ("phi", 1e-9), ("ucat", 1e-10)):
-        a, b = u.get(f), o.get(f)
-        assert rel(a, b) < tol, (name, f, rel(a, b))
+    assert_fields_match(f"one step {name}", u, o, ("ucont", "ucat", "p", "phi"))
     # Maxdiv (main.c:469-593) after a solve to rtol 1e-13 sits at roundoff
     # level; both sides must be there, and agree to the accuracy the face
     # fluxes agree to, in units of one flux term's divergence (|Ucont| / J)

@@ -177,3 +177,93 @@ def test_mixed16_budget_and_batched_equals_streams(mix):
         assert np.array_equal(p0, p1) and np.array_equal(u0, u1)
         lv = serving.levels_for(cc.cells)
         assert i0["bytes"] > serving.ctx_bytes(cc, lv, OPTS["restart"])
+
+
+def _with_nan(arrays):
+    xyz, nv, uc = arrays[:3]
+    uc = uc.copy()
+    mz, my, mx, _ = uc.shape
+    uc[mz // 2, my // 2, mx // 2, 0] = np.nan
+    return (xyz, nv, uc)
+
+
+def test_nonfinite_step_is_reported(mix):
+    """A NaN in the face fluxes reaches the Poisson right-hand side: the
+    step returns CV_ERR_NONFINITE instead of a converged-looking status."""
+    from curvib_old_amd.solver import UserCtx
+    c, a = mix[0]
+    xyz, nv, uc = _with_nan(a)
+    u = UserCtx(c.nodes, c.bctype, serving.levels_for(c.cells), c.re, c.dt, semi=serving.SEMI,
+                coarsen_auto=serving.COARSEN_AUTO)
+    try:
+        u.set_grid(xyz)
+        u.set_nvert(nv)
+        u.put("ucont", uc)
+        u.set_solver(**OPTS)
+        with pytest.raises(_lib.CvError) as e:
+            u.Flow_Solver()
+        assert e.value.code == _lib.CV_ERR_NONFINITE, e.value
+    finally:
+        u.close()
+
+
+@pytest.mark.parametrize("mode", ["streams", "batched"])
+def test_server_fails_only_the_nonfinite_case(mix, mode):
+    """A case that goes non-finite fails with CV_ERR_NONFINITE; the cases
+    beside it (in the batched server: in the same batch) finish with the
+    bits they get alone."""
+    reqs = [mix[0], (mix[1][0], _with_nan(mix[1][1])), mix[2]]
+    ref, _ = run([mix[0], mix[2]], slots=2, nsteps=2)
+    srv = serving.Server(50e9, 4, mode=mode)
+    try:
+        ids = [srv.submit(c, a, 2, **OPTS) for c, a in reqs]
+        res = [srv.wait(cid) for cid in ids]
+        assert res[1][0] == _lib.CV_ERR_NONFINITE and res[1][1]["status"] == 3, res[1]
+        for q, r in ((0, 0), (2, 1)):
+            assert res[q][0] == 0 and res[q][1]["steps_done"] == 2, (q, res[q])
+            c = reqs[q][0]
+            mx, my, mz = c.storage
+            assert np.array_equal(srv.fetch(ids[q], "p", (mz, my, mx)), ref[r][0]), (mode, q)
+    finally:
+        srv.close()
+
+
+def _hip_used_bytes():
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return total.value - free.value
+
+
+def test_device_memory_stays_within_the_budget(mix):
+    """hipMemGetInfo sampled every millisecond while the server admits,
+    builds (metrics scratch included) and steps cases concurrently: the
+    device memory beyond what was in use before stays below the budget plus
+    a fixed allowance for the runtime's own state."""
+    import threading
+    import time
+    run(mix[:1], slots=1, nsteps=1)                 # load the code objects first
+    base = _hip_used_bytes()
+    peak = [0]
+    done = threading.Event()
+
+    def sample():
+        while not done.is_set():
+            peak[0] = max(peak[0], _hip_used_bytes() - base)
+            time.sleep(1e-3)
+
+    b = sorted(serving.ctx_bytes(c, serving.levels_for(c.cells), OPTS["restart"]) for c, _ in mix)
+    budget = int(2.5 * b[-1])
+    th = threading.Thread(target=sample)
+    th.start()
+    try:
+        out, st = run(mix, slots=6, budget=budget, nsteps=2)
+    finally:
+        done.set()
+        th.join()
+    allowance = 256 << 20
+    print(f"device memory: peak {peak[0] / 2**20:.0f} MiB above the start, budget {budget / 2**20:.0f} MiB, "
+          f"admitted peak {st['bytes_peak'] / 2**20:.0f} MiB", flush=True)
+    assert st["bytes_peak"] <= budget
+    assert peak[0] <= budget + allowance, (peak[0], budget)
