@@ -129,6 +129,21 @@ def ctx_bytes(case, mglevels, restart=30, semi=SEMI):
     return _lib.load().cv_ctx_bytes(C.byref(grid_desc(case, mglevels, semi)), int(restart))
 
 
+def geometry_scratch_bytes(case):
+    """Device bytes cv_set_geometry holds while it builds the finest level's
+    metrics (the server reserves them beside the context at admission):
+    55 padded doubles per point (3 ghost layers per side) and the node
+    coordinates."""
+    IM, JM, KM = case.nodes
+    return 55 * (IM + 7) * (JM + 7) * (KM + 7) * 8 + 3 * IM * JM * KM * 8
+
+
+def admission_bytes(case, mglevels, restart=30, semi=SEMI):
+    """The most device bytes a case holds at once: its context while the
+    metrics scratch is held beside it."""
+    return ctx_bytes(case, mglevels, restart, semi) + geometry_scratch_bytes(case)
+
+
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 

@@ -58,9 +58,32 @@ def test_alone_equals_mix(mix):
     assert st["bytes_peak"] > max(i["bytes"] for _, _, i in together)
 
 
-def test_budget_bounds_residency(mix):
+def test_admission_counts_the_metrics_scratch(mix):
+    """A case is admitted when its context and cv_set_geometry's transient
+    scratch fit the budget together, and refused at submit one byte below."""
+    c, a = mix[0]
+    need = serving.admission_bytes(c, serving.levels_for(c.cells), OPTS["restart"])
+    srv = serving.Server(budget_bytes=need - 1, slots=1)
+    try:
+        with pytest.raises(_lib.CvError) as e:
+            srv.submit(c, a, 1, **OPTS)
+        assert e.value.code == _lib.CV_ERR_OOM
+    finally:
+        srv.close()
+    out, st = run([(c, a)], slots=1, budget=need, nsteps=1)
+    assert out[0][2]["steps_done"] == 1 and st["bytes_peak"] == need
+
+
+def _two_largest_budget(mix):
+    """Room for the two largest contexts, and for any one case's admission
+    (context plus metrics scratch)."""
     b = sorted(serving.ctx_bytes(c, serving.levels_for(c.cells), OPTS["restart"]) for c, _ in mix)
-    budget = b[-1] + b[-2]           # at most two of the largest fit together
+    adm = max(serving.admission_bytes(c, serving.levels_for(c.cells), OPTS["restart"]) for c, _ in mix)
+    return max(b[-1] + b[-2], adm)
+
+
+def test_budget_bounds_residency(mix):
+    budget = _two_largest_budget(mix)
     out, st = run(mix, slots=6, budget=budget, nsteps=1)
     assert st["bytes_peak"] <= budget
     assert all(i["steps_done"] == 1 for _, _, i in out)
@@ -105,8 +128,7 @@ def test_batched_mode_equals_streams(mix):
 
 
 def test_batched_mode_budget(mix):
-    b = sorted(serving.ctx_bytes(c, serving.levels_for(c.cells), OPTS["restart"]) for c, _ in mix)
-    budget = b[-1] + b[-2]
+    budget = _two_largest_budget(mix)
     out, st = run(mix, slots=6, budget=budget, nsteps=1, mode="batched")
     assert st["bytes_peak"] <= budget
     assert all(i["steps_done"] == 1 for _, _, i in out)
@@ -253,8 +275,8 @@ def test_device_memory_stays_within_the_budget(mix):
             peak[0] = max(peak[0], _hip_used_bytes() - base)
             time.sleep(1e-3)
 
-    b = sorted(serving.ctx_bytes(c, serving.levels_for(c.cells), OPTS["restart"]) for c, _ in mix)
-    budget = int(2.5 * b[-1])
+    budget = int(2.5 * max(serving.admission_bytes(c, serving.levels_for(c.cells), OPTS["restart"])
+                           for c, _ in mix))
     th = threading.Thread(target=sample)
     th.start()
     try:
