@@ -45,6 +45,7 @@ SYMBOLS = [
     "cv_server_fetch", "cv_server_release", "cv_server_status", "cv_server_destroy",
     "cv_batch_create", "cv_batch_step", "cv_batch_poisson_solve", "cv_batch_destroy", "cv_server_create_mode",
     "cv_inflow_flux", "cv_outflow_flux", "cv_set_ib_nodes", "cv_ibm_interpolation", "cv_server_submit_ib",
+    "cv_divergence_noalloc",
 ]
 CV_SERVE_STREAMS, CV_SERVE_BATCHED = 0, 1
 CV_CASE_QUEUED, CV_CASE_RUNNING, CV_CASE_DONE, CV_CASE_FAILED = 0, 1, 2, 3
@@ -148,6 +149,7 @@ def load():
     L.cv_poisson_solve.argtypes = [vp, C.POINTER(StepStats)]
     L.cv_flow_step.argtypes = [vp, C.POINTER(StepStats)]
     L.cv_divergence.argtypes = [vp, dp, C.POINTER(C.c_long)]
+    L.cv_divergence_noalloc.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp]
     L.cv_mg_restrict.argtypes = [vp, C.c_int, dp, dp]
     L.cv_mg_prolong.argtypes = [vp, C.c_int, dp, dp]
     L.cv_nullspace_project.argtypes = [vp, C.c_int, dp]

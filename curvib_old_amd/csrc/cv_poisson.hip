@@ -16,6 +16,7 @@
 #include "cv_reduce.h"
 #include <cmath>
 #include <chrono>
+#include <mutex>
 
 #define BS 256
 
@@ -1967,6 +1968,89 @@ int cv_divergence_dev(cv_ctx *c, double *maxdiv, long *idx) {
   }
   if (maxdiv) *maxdiv = best;
   if (idx) *idx = bi;
+  return CV_OK;
+}
+
+// ====================================================================
+// Divergence without device allocation (cv_divergence_noalloc): the block
+// partials live in a file-scope device array, one copy per GPU, that every
+// call on that GPU shares
+// ====================================================================
+#define CV_NOALLOC_PARTS CV_RED_GRID       // (value, index) pairs: 16 KiB per GPU
+#define CV_NOALLOC_DEVICES 64
+static __device__ __align__(256) double scratch[2 * CV_NOALLOC_PARTS];
+static double *g_scratch[CV_NOALLOC_DEVICES];   // device address of each GPU's copy
+static std::mutex g_scratch_mu;
+
+// the address of this GPU's copy of scratch, looked up on the first call
+// on the device (which is current); in host code the array's name is only
+// a stand-in for it
+static int noalloc_scratch(int device, double **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      cv_set_error("cv_divergence_noalloc: hipGetSymbolAddress(scratch) -> %s", hipGetErrorString(e));
+      return CV_ERR_HIP;
+    }
+    g_scratch[device] = (double *)p;
+  }
+  *out = g_scratch[device];
+  return CV_OK;
+}
+
+// block partials of Divergence over the DMDA layout (Cmpnts interleaved)
+__global__ static void k_divmax_dmda(Idx I, const double *u, const double *aj, const double *nv, long n,
+                                     double *pv, double *pi) {
+  divmax_block_s<3>(I, u, u + 1, u + 2, aj, nv, n, pv, pi, blockIdx.x, gridDim.x);
+}
+// the largest partial and its first index (the order cv_divergence_dev
+// applies on the host) into out[0], out[1]
+__global__ static void k_divmax_final(const double *pv, const double *pi, int nb, double *out) {
+  __shared__ double sv[BS];
+  __shared__ long si[BS];
+  double best = -1.0;
+  long bi = 0;
+  for (int b = threadIdx.x; b < nb; b += BS) {
+    const long ii = (long)pi[b];
+    if (div_beats(pv[b], ii, best, bi)) { best = pv[b]; bi = ii; }
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s && div_beats(sv[threadIdx.x + s], si[threadIdx.x + s], sv[threadIdx.x], si[threadIdx.x])) {
+      sv[threadIdx.x] = sv[threadIdx.x + s];
+      si[threadIdx.x] = si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = sv[0]; out[1] = (double)si[0]; }
+}
+
+extern "C" int cv_divergence_noalloc(int device, const double *ucont, const double *aj, const double *nvert, int mx,
+                                     int my, int mz, void *stream, double *out) {
+  if (!ucont || !aj || !nvert || !out || device < 0 || device >= CV_NOALLOC_DEVICES) {
+    cv_set_error("cv_divergence_noalloc: null pointer or device outside 0..%d", CV_NOALLOC_DEVICES - 1);
+    return CV_ERR_ARG;
+  }
+  if (mx < 3 || my < 3 || mz < 3 || (long)mx * my * mz >= (1L << 31)) {
+    cv_set_error("cv_divergence_noalloc: storage %d x %d x %d outside 3^3 .. 2^31 points", mx, my, mz);
+    return CV_ERR_ARG;
+  }
+  CV_CHECK(hipSetDevice(device));
+  double *part = nullptr;
+  int rc = noalloc_scratch(device, &part);
+  if (rc) return rc;
+  const long n = (long)mx * my * mz;
+  hipStream_t s = (hipStream_t)stream;
+  // every block writes its own pair, so nothing needs clearing first
+  hipLaunchKernelGGL(k_divmax_dmda, dim3(CV_NOALLOC_PARTS), dim3(BS), 0, s, Idx{mx, my, mz}, ucont, aj, nvert, n,
+                     part, part + CV_NOALLOC_PARTS);
+  hipLaunchKernelGGL(k_divmax_final, dim3(1), dim3(BS), 0, s, (const double *)part,
+                     (const double *)(part + CV_NOALLOC_PARTS), CV_NOALLOC_PARTS, out);
+  CV_CHECK(hipGetLastError());
   return CV_OK;
 }
 

@@ -186,6 +186,21 @@ int cv_poisson_solve(cv_ctx *ctx, cv_step_stats *st);    /* PoissonSolver_MG   p
 int cv_update_pressure(cv_ctx *ctx);                     /* UpdatePressure     poisson.c:2908-3248 */
 int cv_projection(cv_ctx *ctx);                          /* Projection         poisson.c:2281-2906 */
 int cv_divergence(cv_ctx *ctx, double *maxdiv, long *idx); /* Divergence       main.c:469-593    */
+/* cv_divergence on device-resident fields, allocating no device memory.  It
+ * has no counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  ucont:
+ * 3*mx*my*mz doubles (Cmpnts interleaved), aj and nvert: mx*my*mz doubles,
+ * all device pointers in the DMDA layout; out: a device pointer to 2
+ * doubles, the maximum |div| and its first storage index (the result of
+ * cv_divergence on the same fields, bit for bit).  Runs on `stream` (a
+ * hipStream_t; 0 for the null stream) of GPU `device` and returns without
+ * waiting.  Its block partials live in a fixed device array of 1024
+ * (value, index) pairs (16 KiB) per GPU, held until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU race.
+ * Storage of 2^31 points or more, or below 3 per axis, is refused with
+ * CV_ERR_ARG. */
+int cv_divergence_noalloc(int device, const double *ucont, const double *aj, const double *nvert,
+                          int mx, int my, int mz, void *stream, double *out);
 int cv_ibm_interpolation(cv_ctx *ctx);                   /* ibm_interpolation_advanced(user, ibm, ibi, 1)
                                                             ibm.c:3012-3254 (nothing without IB nodes) */
 int cv_flow_step(cv_ctx *ctx, cv_step_stats *st);        /* Flow_Solver        solvers.c:944-1156 + main.c:1123-1145;
