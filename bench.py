@@ -94,11 +94,15 @@ def make_ctx(case, arrays, levels, semi, precision, coarsen_auto, ib=None):
 
 
 def reset(u, arrays):
+    """The state of a fresh context: initial fluxes, zero p, phi and Ucat
+    (Ucat at immersed-boundary nodes carries over from step to step, and the
+    first reconstruction of a step starts from it)."""
     _, _, uc = arrays
     u.put("ucont", uc)
     z = np.zeros(u._shape("p", -1))
     u.put("p", z)
     u.put("phi", z)
+    u.put("ucat", np.zeros(u._shape("ucat", -1)))
 
 
 def timed_steps(u, tag, warmup, steps, prof=None):

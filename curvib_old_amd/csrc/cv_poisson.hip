@@ -1463,10 +1463,19 @@ static void precondition(cv_ctx *c, const double *vin, double *zout) {
     hipLaunchKernelGGL(k_f2d, dim3(nblk(n)), dim3(BS), 0, c->stream, zout, (const float *)v.x[top], n);
     for (int l = 0; l < c->nlev; l++) { c->lev[l].xf = v.x[l]; c->lev[l].rf = v.r[l]; }
   } else {
+    // the finest level reads its right-hand side from vin and starts its
+    // x/r ping-pong in zout: after an even number of sweeps the result is
+    // in zout already (the V-cycle only reads b, and the arithmetic does
+    // not depend on which buffer holds x)
     MgVecs<double> v = mgvecs<double>(c);
-    hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(BS), 0, c->stream, v.b[top], vin, n);
+    double *const own_x = v.x[top], *const own_r = v.r[top];
+    v.b[top] = const_cast<double *>(vin);
+    v.x[top] = zout;
     vcycle<double>(c, v, top);
-    hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(BS), 0, c->stream, zout, (const double *)v.x[top], n);
+    if (v.x[top] != zout)
+      hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(BS), 0, c->stream, zout, (const double *)v.x[top], n);
+    v.x[top] = own_x;
+    v.r[top] = own_r;
     for (int l = 0; l < c->nlev; l++) { c->lev[l].x = v.x[l]; c->lev[l].r = v.r[l]; }
   }
   nullspace_dev<double>(c, top, zout);
