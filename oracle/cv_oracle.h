@@ -136,6 +136,7 @@ double o_divergence(octx *c);
 void o_flow_step(octx *c);
 void o_csr_matvec(const olevel *L, const double *x, double *y);
 void o_set_threads(int n);
+int  o_get_threads(void);
 
 #ifdef __cplusplus
 }

@@ -114,3 +114,11 @@ void o_set_threads(int n) {
   (void)n;
 #endif
 }
+
+int o_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -70,6 +70,8 @@ def lib():
         L.o_flux_stats.argtypes = [vp, dp]
         L.o_set_ib_nodes.argtypes = [vp, C.c_int, ip, dp, C.c_int]
         L.o_set_threads.argtypes = [C.c_int]
+        L.o_get_threads.argtypes = []
+        L.o_get_threads.restype = C.c_int
         L.o_ibm_interpolation.argtypes = [vp]
         _lib = L
     return _lib
@@ -245,6 +247,12 @@ def from_case(case, nlev=3, semi=(0, 0, 0), coarsen_auto=0):
 def set_threads(n):
     """OpenMP threads of the oracle's later parallel regions."""
     lib().o_set_threads(int(n))
+
+
+def get_threads():
+    """OpenMP threads the oracle's next parallel region uses (the block
+    count of its block-Jacobi smoother)."""
+    return int(lib().o_get_threads())
 
 
 def inlet_uin(profile, xc, yc, zc):
