@@ -19,6 +19,11 @@
 namespace {
 
 #define BMAXC 64      // cases per batch (one bit each in a 64-bit mask)
+#ifndef CV_BATCH_LEGS_DEFAULT
+// first depth of the persistent V-cycle legs; off (-1): measured slower than
+// one launch per operation on the 64-case ensemble at every starting depth
+#define CV_BATCH_LEGS_DEFAULT -1
+#endif
 #define BSC 48        // per-case host-set scalars: [0] 1/beta, [1] 1/hn, [2] m, [8..] y
 
 struct BLev {
@@ -353,6 +358,8 @@ __global__ void kbd_final(const BDiv *cs, int nb, double *out) {
   if (threadIdx.x == 0) { out[2 * blockIdx.x] = sv[0]; out[2 * blockIdx.x + 1] = (double)si[0]; }
 }
 
+struct BLegs;
+
 struct cv_batch {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -382,6 +389,8 @@ struct cv_batch {
   std::vector<int> rc;
   std::vector<hipEvent_t> ev_case;
   hipEvent_t ev_solved = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+  BLegs *legs = nullptr;                      // the V-cycle from a depth down as persistent legs
+  int *herr = nullptr;                        // their error flag, read back with each gather
 };
 
 static void b_upload_ctl(cv_batch *B, unsigned long long act, int jj) {
@@ -393,10 +402,17 @@ static void b_upload_bsc(cv_batch *B) {
   (void)hipMemcpyAsync(B->dbsc, B->hbsc, (size_t)B->n * BSC * sizeof(double), hipMemcpyHostToDevice, B->stream);
 }
 // dsc[0..BSC) of every case to the host, then wait
+static int b_legs_error(cv_batch *B);
 static int b_gather(cv_batch *B) {
   hipLaunchKernelGGL(kb_gather, dim3((unsigned)B->n), dim3(64), 0, B->stream, (const BCase *)B->dcases, B->dgat);
   CV_CHECK(hipMemcpyAsync(B->hgat, B->dgat, (size_t)B->n * BSC * sizeof(double), hipMemcpyDeviceToHost, B->stream));
+  int rc = b_legs_error(B);
+  if (rc) return rc;
   CV_CHECK(hipStreamSynchronize(B->stream));
+  if (B->legs && *B->herr) {
+    cv_set_error("cv_batch: a persistent V-cycle leg stopped at its wait limit");
+    return CV_ERR_HIP;
+  }
   return CV_OK;
 }
 
@@ -496,10 +512,16 @@ static void b_nullspace(cv_batch *B, int d, unsigned long long pm, int sx) {
                      (const uint32_t *)BTAB(tv), pm, (const BCtl *)B->dctl, d, sx);
 }
 
+#include "cv_batch_legs.h"
+
 // the V-cycle of depth d for every case that has it; the cases whose
 // coarsest level is at depth d run the coarse sweeps and the nullspace rule
 template <class T>
 static void b_vcycle(cv_batch *B, int d) {
+  if (B->legs && d == B->legs->d0) {
+    b_vcycle_legs<T>(B);
+    return;
+  }
   const cv_solver_opts &o = B->opt;
   const T om = (T)o.omega;
   const unsigned long long in = B->inner[d], co = B->coarse[d];
@@ -763,6 +785,8 @@ static void b_release(cv_batch *B) {
     if (g) (void)hipGraphExecDestroy(g);
   for (auto g : B->gpost)
     if (g) (void)hipGraphExecDestroy(g);
+  legs_free(B->legs);
+  if (B->herr) (void)hipHostFree(B->herr);
   if (B->dcases) (void)hipFree(B->dcases);
   if (B->dctl) (void)hipFree(B->dctl);
   if (B->dbsc) (void)hipFree(B->dbsc);
@@ -930,6 +954,27 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
   CV_CHECK(hipMemcpy(B->dcases, hc.data(), sizeof(BCase) * ncases, hipMemcpyHostToDevice));
   if (!tabs.empty()) CV_CHECK(hipMemcpy(B->dtab, tabs.data(), sizeof(uint32_t) * tabs.size(), hipMemcpyHostToDevice));
   CV_CHECK(hipMemcpy(B->dbsc, B->hbsc, sizeof(double) * BSC * ncases, hipMemcpyHostToDevice));
+  // the V-cycle from depth CV_BATCH_LEGS down as persistent legs ("off" or a
+  // negative depth: one launch per operation throughout); the legs' line
+  // items take up to 16 rows per thread
+  {
+    const char *el = getenv("CV_BATCH_LEGS");
+    int d0 = el ? (strcmp(el, "off") == 0 ? -1 : atoi(el)) : CV_BATCH_LEGS_DEFAULT;
+    for (int c = 0; c < ncases && d0 >= 0; c++)
+      for (int d = 0; d < hc[c].nlev; d++)
+        if (hc[c].lev[d].cs > 16) d0 = -1;
+    if (d0 >= 0 && d0 < B->maxlev) {
+      const char *ek = getenv("CV_BATCH_LEG_KC");
+      const int kc = ek ? std::max(1, atoi(ek)) : 4;
+      if ((rc = legs_build(B, hc, d0, kc, &B->legs))) { b_release(B); return rc; }
+      if (hipHostMalloc((void **)&B->herr, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        b_release(B);
+        cv_set_error("cv_batch_create: out of host memory");
+        return CV_ERR_OOM;
+      }
+      *B->herr = 0;
+    }
+  }
   if ((rc = cv_mom_batch_create(cases, ncases, B->stream, &B->mom))) { b_release(B); return rc; }
   {
     std::vector<BDiv> hd(ncases);

@@ -794,15 +794,15 @@ __global__ void k_prolong(Idx F, Idx C, int isc, int jsc, int ksc, const double 
 // ------------------------------------------------------------------
 // partition of a reduction: block bid of nb, BS threads each
 #define RED_LOOP for (long t = (long)bid * BS + threadIdx.x; t < n; t += (long)nb * BS)
+// the fluid-cell sum of thread lt (of BS) of reduction block bid of nb
 template <class T>
-__device__ __forceinline__ void fluid_sum_body(const T *__restrict__ x, const uint32_t *__restrict__ fl, long n, double *part,
-                                      int bid, int nb) {
-  __shared__ double sh[BS / 64];
+__device__ __forceinline__ double fluid_sum_thread(const T *__restrict__ x, const uint32_t *__restrict__ fl, long n,
+                                                   int bid, int nb, int lt) {
   double s = 0.0;
   // four loads in flight per thread; the additions keep the order of a
   // plain strided loop, so the sum is the same bits
   const long step = (long)nb * BS;
-  long t = (long)bid * BS + threadIdx.x;
+  long t = (long)bid * BS + lt;
   for (; t + 3 * step < n; t += 4 * step) {
     const T x0 = x[t], x1 = x[t + step], x2 = x[t + 2 * step], x3 = x[t + 3 * step];
     const uint32_t f0 = fl[t], f1 = fl[t + step], f2 = fl[t + 2 * step], f3 = fl[t + 3 * step];
@@ -813,6 +813,13 @@ __device__ __forceinline__ void fluid_sum_body(const T *__restrict__ x, const ui
   }
   for (; t < n; t += step)
     if (fl[t] & 1u) s += (double)x[t];
+  return s;
+}
+template <class T>
+__device__ __forceinline__ void fluid_sum_body(const T *__restrict__ x, const uint32_t *__restrict__ fl, long n, double *part,
+                                      int bid, int nb) {
+  __shared__ double sh[BS / 64];
+  double s = fluid_sum_thread<T>(x, fl, n, bid, nb, threadIdx.x);
   s = cv_block_sum(s, sh);
   if (threadIdx.x == 0) part[bid] = s;
 }

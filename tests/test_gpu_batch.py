@@ -38,14 +38,21 @@ def _ctx(case, precision):
 # "0": per case on each case's stream (graph "1": replayed from per-case graphs)
 # tw: lanes per stencil tile row in the batch (each case alone uses 32); the
 # tile shape must not change any bit
-@pytest.mark.parametrize("precision,graph,mom,tw", [("mixed", "1", "1", "32"), ("fp64", "1", "1", "32"),
-                                                    ("mixed", "1", "0", "32"), ("mixed", "0", "0", "32"),
-                                                    ("mixed16", "1", "1", "32"), ("mixed16", "1", "1", "64"),
-                                                    ("fp64", "1", "1", "64")])
-def test_batch_matches_each_case_alone(monkeypatch, precision, graph, mom, tw):
+# legs: first depth of the V-cycle run as persistent per-depth launches
+# ("off": one launch per operation at every depth); kc: k planes per stencil item
+@pytest.mark.parametrize("precision,graph,mom,tw,legs,kc", [
+    ("mixed", "1", "1", "32", "0", "4"), ("fp64", "1", "1", "32", "0", "4"),
+    ("mixed", "1", "0", "32", "0", "4"), ("mixed", "0", "0", "32", "0", "4"),
+    ("mixed16", "1", "1", "32", "0", "4"), ("mixed16", "1", "1", "64", "off", "4"),
+    ("fp64", "1", "1", "64", "off", "4"), ("mixed16", "1", "1", "32", "2", "1"),
+    ("mixed16", "0", "1", "32", "1", "8"), ("fp64", "0", "1", "32", "3", "2")])
+def test_batch_matches_each_case_alone(monkeypatch, precision, graph, mom, tw, legs, kc):
     monkeypatch.setenv("CV_BATCH_GRAPH", graph)
     monkeypatch.setenv("CV_BATCH_MOM", mom)
     monkeypatch.setenv("CV_BATCH_TW", tw)
+    monkeypatch.setenv("CV_BATCH_LEGS", legs)
+    monkeypatch.setenv("CV_BATCH_LEG_KC", kc)
+    print(f"batch vs alone: {precision} graph {graph} mom {mom} tw {tw} legs {legs} kc {kc}", flush=True)
     cl = _cases()
     levels = {serving.levels_for(c.cells) for c in cl}
     assert len(levels) > 1, "the batch must mix multigrid depths"
@@ -68,6 +75,36 @@ def test_batch_matches_each_case_alone(monkeypatch, precision, graph, mom, tw):
     finally:
         b.close()
         for u in inb + alone:
+            u.close()
+
+
+def test_persistent_leg_wait_failure_is_reported(monkeypatch):
+    """The persistent V-cycle legs bound every wait; a wait that runs past
+    its limit makes every work-group leave and the step returns an error.
+    The failure is forced from the host (a negative limit fails every wait at
+    once), so every kernel still ends normally."""
+    monkeypatch.setenv("CV_BATCH_LEGS", "0")
+    monkeypatch.setenv("CV_BATCH_LEG_SPIN", "-1")
+    cl = _cases()[:2]
+    inb = [_ctx(c, "mixed16") for c in cl]
+    b = Batch(inb)
+    try:
+        with pytest.raises(_lib.CvError) as e:
+            b.step()
+        assert e.value.code == _lib.CV_ERR_HIP and "wait limit" in str(e.value), str(e.value)
+    finally:
+        b.close()
+        for u in inb:
+            u.close()
+    # a batch built without the forced failure steps normally afterwards
+    monkeypatch.setenv("CV_BATCH_LEG_SPIN", str(1 << 22))
+    inb = [_ctx(c, "mixed16") for c in cl]
+    b = Batch(inb)
+    try:
+        assert all(x["converged"] == 1 for x in b.step())
+    finally:
+        b.close()
+        for u in inb:
             u.close()
 
 
