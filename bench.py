@@ -15,7 +15,10 @@ runs it with -imm 1 (--no-ib leaves the IB nodes held).
 Three contexts run the same steps from the same initial state: the FP64
 path (FP64 V-cycle), the mixed path (the FP32 V-cycle with FP16-stored face
 coefficients inside the FP64 FGMRES, CV_MIXED16) and, under "mixed_fp32",
-the FP32 V-cycle with FP32 coefficients (CV_MIXED).  ``value`` is the mixed
+the FP32 V-cycle with FP32 coefficients (CV_MIXED); under
+"mixed_fp32_4level_rb" a fourth context runs CV_MIXED with the
+preconditioner the mixed workload is stated with (4 fully coarsened
+levels, 2 red-black pre- and post-sweeps).  ``value`` is the mixed
 path's steps/s; the FP64 figures, the ratios, the per-step pressure error of
 each mixed path against the FP64 path and the per-step Poisson residuals
 sit beside it.  All inputs are resident in HBM before the timed region
@@ -80,11 +83,18 @@ SOLVER = dict(smoother="yline", omega=0.9, pre_sweeps=3, post_sweeps=3, coarse_s
               restart=30, max_it=200, rtol=1e-9, atol=0.0)
 
 
-def make_ctx(case, arrays, levels, semi, precision, coarsen_auto, ib=None):
+# the preconditioner the mixed-precision workload is stated with: an FP32
+# V-cycle of 4 fully coarsened levels, 2 red-black pre- and post-sweeps
+# (omega 1; 40 sweeps on the coarsest level, 64x32x32 at 512x256x256)
+STATED_PC = dict(levels=4, semi=(0, 0, 0), coarsen_auto=0,
+                 solver=dict(smoother="rb", omega=1.0, pre_sweeps=2, post_sweeps=2, coarse_sweeps=40, max_it=400))
+
+
+def make_ctx(case, arrays, levels, semi, precision, coarsen_auto, ib=None, solver_over=None):
     from curvib_old_amd import solver
     xyz, nv, uc = arrays
     u = solver.UserCtx(case.nodes, case.bctype, levels, case.re, case.dt, semi=semi, coarsen_auto=coarsen_auto)
-    u.set_solver(precision=precision, **SOLVER)
+    u.set_solver(precision=precision, **{**SOLVER, **(solver_over or {})})
     u.set_grid(xyz)
     u.set_nvert(nv)
     if ib is not None:
@@ -292,6 +302,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS, else every core")
     ap.add_argument("--cpu1-sample", default="128x64x64", help="the one-core CPU sample")
     ap.add_argument("--no-fp32-leg", action="store_true", help="skip the CV_MIXED (FP32 coefficient) leg")
+    ap.add_argument("--no-stated-leg", action="store_true",
+                    help="skip the leg with the stated preconditioner (FP32, 4 levels, 2+2 red-black sweeps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-ib", action="store_true", help="no immersed-boundary reconstruction")
@@ -368,6 +380,19 @@ def main():
                  "its": its32, "cell_updates_per_s": float(np.prod(cells)) * args.steps / t32,
                  "over_fp64": t64 / t32, "accuracy": acc32}
 
+    # the stated preconditioner point: FP32 V-cycle, 4 levels, 2+2 red-black sweeps
+    leg_pc = None
+    if not args.no_stated_leg:
+        upc = make_ctx(case, arrays, STATED_PC["levels"], STATED_PC["semi"], "mixed", STATED_PC["coarsen_auto"], ib,
+                       STATED_PC["solver"])
+        tpc, itspc, _ = timed_steps(upc, "mixed 4-level rb", args.warmup, args.steps)
+        accpc = [] if args.no_accuracy else accuracy_pass(u64, upc, arrays, args.steps)
+        upc.close()
+        leg_pc = {"precision": "mixed", "mg_levels": STATED_PC["levels"], "semi_coarsening": list(STATED_PC["semi"]),
+                  **STATED_PC["solver"], "steps_per_s": args.steps / tpc, "ms_per_step": 1e3 * tpc / args.steps,
+                  "its": itspc, "cell_updates_per_s": float(np.prod(cells)) * args.steps / tpc,
+                  "over_fp64": t64 / tpc, "accuracy": accpc}
+
     acc = [] if args.no_accuracy else accuracy_pass(u64, umx, arrays, args.steps)
     # host-array exchange of a drop-in caller that hands Ucont in and takes
     # Ucont and P back every step (pageable numpy buffers, PCIe)
@@ -440,6 +465,7 @@ def main():
                   "its": itsmx, "cell_updates_per_s": ncell * sps_mx},
         "mixed_over_fp64": sps_mx / sps_64,
         "mixed_fp32": leg32,
+        "mixed_fp32_4level_rb": leg_pc,
         "host_exchange_ms_per_step": xchg_ms,
         "pcie_inclusive_mixed_ms_per_step": 1e3 * tmx / args.steps + xchg_ms,
         "accuracy": acc,

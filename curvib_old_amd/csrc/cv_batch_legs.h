@@ -116,13 +116,12 @@ __global__ void __launch_bounds__(LEG_NT) kb_leg(const BCase *__restrict__ cs, u
         break;
       }
       case PK_LINE: {
-        const int gx = (A.mx + 63) / 64;
         const T om = HG ? (T)(omega * A.ysc) : omega;
         const T *r = bvec<T>(C, d, P.src, 0);
         if (Lv.cs == 8)
-          yline_reg_tile<T, 8, HG>(A.mx, A.my, A.mz, r, A.ylo, A.yip, A.ycp, x, x, om, P.zg, it % gx, it / gx, A.yh);
+          yline_reg_tile<T, 8, HG>(A.mx, A.my, A.mz, r, A.ylo, A.yip, A.ycp, x, x, om, P.zg, it, A.yh);
         else
-          yline_reg_tile<T, 16, HG>(A.mx, A.my, A.mz, r, A.ylo, A.yip, A.ycp, x, x, om, P.zg, it % gx, it / gx, A.yh);
+          yline_reg_tile<T, 16, HG>(A.mx, A.my, A.mz, r, A.ylo, A.yip, A.ycp, x, x, om, P.zg, it, A.yh);
         break;
       }
       case PK_RESTRICT: {
@@ -224,7 +223,7 @@ static int leg_items(const BCase &C, int d, int kind, int kc) {
       const int gx = (b.I.mx + LEG_TX - 2) / (LEG_TX - 1), gy = (b.I.my + LEG_TY - 1) / LEG_TY, gz = (b.I.mz + kc - 1) / kc;
       return gx * gy * gz;
     }
-    case PK_LINE: return ((b.I.mx + 63) / 64) * b.I.mz;
+    case PK_LINE: return b.lgx;
     case PK_RESTRICT: return (int)((C.lev[d + 1].n + LEG_CHUNK - 1) / LEG_CHUNK);
     case PK_FSUM: return (b.nrd + LEG_NT / BS - 1) / (LEG_NT / BS);
     case PK_FFINAL: return 1;

@@ -40,7 +40,7 @@ struct BLev {
   double inv_nf;
   int sgx, sgy, sgz;   // stencil tiles in i, j, k
   int kc;              // k planes per stencil tile
-  int lgx, cs;         // line-solve blocks in i; rows per thread of the line solve
+  int lgx, cs;         // line-solve blocks (64 lines each); rows per thread of the line solve
   int nvb, nrd;        // blocks of an elementwise pass and of a reduction
 };
 
@@ -125,10 +125,9 @@ template <class T, int CS, bool HL = false>
 __global__ void __launch_bounds__(1024) kb_yline(BK_ARGS, int d, int sr, T omega, int zg) {
   BK_HEAD
   const OpV<T> A = bop<T>(C, d);
-  const int gx = C.lev[d].lgx;
   T *x = bvec<T>(C, d, BX, 0);
   yline_reg_tile<T, CS, HL>(A.mx, A.my, A.mz, bvec<T>(C, d, sr, 0), A.ylo, A.yip, A.ycp, x, x,
-                            HL ? (T)(omega * A.ysc) : omega, zg, lb % gx, lb / gx, A.yh);
+                            HL ? (T)(omega * A.ysc) : omega, zg, lb, A.yh);
 }
 
 // r of depth d -> b of depth d+1 (block table of depth d+1)
@@ -884,7 +883,7 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
       b.inv_nf = x->nfluid[l] > 0 ? 1.0 / x->nfluid[l] : 0.0;
       b.kc = stencil_kchunk_batch(L.d.mz);
       b.sgx = (L.d.mx + B->tw - 2) / (B->tw - 1); b.sgy = (L.d.my + 3) / 4; b.sgz = (L.d.mz + b.kc - 1) / b.kc;
-      b.lgx = (L.d.mx + 63) / 64;
+      b.lgx = (int)(((long)L.d.mx * L.d.mz + 63) / 64);
       const int rows = L.d.my - 2;
       const int rmax = o.precision != CV_FP64 ? 512 : 256;
       if (rows > rmax || rows < 1) {
@@ -930,7 +929,7 @@ extern "C" int cv_batch_create(cv_ctx *const *cases, int ncases, cv_batch **out)
         vc.push_back({c, b.nvb});
         rd.push_back({c, b.nrd});
         const int g = b.cs == 8 ? 0 : b.cs == 16 ? 1 : 2;
-        ln[g].push_back({c, b.lgx * b.I.mz});
+        ln[g].push_back({c, b.lgx});
       }
       add_tab(v < 0 ? B->st[d] : B->st_v[d][v], st);
       add_tab(v < 0 ? B->vc[d] : B->vc_v[d][v], vc);

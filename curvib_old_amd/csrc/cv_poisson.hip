@@ -593,20 +593,23 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
 // factors and x are each read once and x written once.  Forward recurrence
 // y_j = r_j ip_j + a_j y_{j-1} (a = -lo ip, stored), backward
 // e_j = y_j - cp_j e_{j+1}; chunk carries resolved through LDS.
-// lines (bx*64 .. bx*64+63, k) of the register-resident line solve; the
-// chunks beyond the line (blockDim.y larger than the line needs) are
-// identity maps, so the result does not depend on blockDim.y
+// lines lg*64 .. lg*64+63 of the register-resident line solve, counted
+// i fastest over the (i, k) pairs of the level (a group may span two k
+// planes, so no lane idles where mx is not a multiple of 64); the chunks
+// beyond the line (blockDim.y larger than the line needs) are identity
+// maps, so the result does not depend on blockDim.y
 // HL: the factors from the FP16 pack yh (one 8-byte load per row, cp kept
 // in registers for the backward pass); the solution then carries 2^e and
 // the caller passes omega * 2^-e
 template <class T, int CS, bool HL = false>
 __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
                                       const T *__restrict__ ip, const T *__restrict__ cp, const T *xi, T *xo,
-                                      T omega, int zero_guess, int bx, int bk, const uint2 *__restrict__ yh = nullptr) {
+                                      T omega, int zero_guess, int lg, const uint2 *__restrict__ yh = nullptr) {
   __shared__ T SA[16][64], SB[16][64];
   const int ti = threadIdx.x, tc = threadIdx.y, nc = blockDim.y;
-  const int i = bx * 64 + ti, k = bk;
-  const bool act = i < mx;
+  const long q0 = (long)lg * 64 + ti;
+  const bool act = q0 < (long)mx * mz;
+  const int i = act ? (int)(q0 % mx) : 0, k = act ? (int)(q0 / mx) : 0;
   const long sj = mx;
   const long base = (long)i + (long)mx * my * k;
   const int j0 = 1 + tc * CS;
@@ -672,9 +675,9 @@ __global__ void __launch_bounds__(1024) k_yline_reg(int mx, int my, int mz, cons
                                                    const T *__restrict__ ip, const T *__restrict__ cp,
                                                    const T *xi, T *xo, T omega, int zero_guess,
                                                    const uint2 *__restrict__ yh) {
-  int bx, bk, bz;
-  cv_xcd_tile(bx, bk, bz);
-  yline_reg_tile<T, CS, HL>(mx, my, mz, r, ya, ip, cp, xi, xo, omega, zero_guess, bx, bk, yh);
+  int lg, by, bz;
+  cv_xcd_tile(lg, by, bz);
+  yline_reg_tile<T, CS, HL>(mx, my, mz, r, ya, ip, cp, xi, xo, omega, zero_guess, lg, yh);
 }
 
 // ------------------------------------------------------------------
@@ -1361,7 +1364,8 @@ static void line_solve(cv_ctx *c, int l, const T *r, const T *xi, T *xo, T *tmp,
   CvLevel &L = c->lev[l];
   OpV<T> A = opview<T>(c, l);
   const int rows = L.d.my - 2;
-  const dim3 g((unsigned)((L.d.mx + 63) / 64), (unsigned)L.d.mz);
+  const dim3 g((unsigned)(((long)L.d.mx * L.d.mz + 63) / 64));
+  const dim3 gs((unsigned)((L.d.mx + 63) / 64), (unsigned)L.d.mz);   // k_yline_solve
   // bytes per point: r, the factors (FP16 pack: 8 bytes, else a, ip, cp), x in and out
   const bool hl = sizeof(T) == 4 && A.yh && rows <= 16 * 16;
   CvProfScope ps(c, CV_PROF_LINE, l, sizeof(T) == 4 ? 1 : 0, (double)L.d.n * (hl ? 20.0 : 6.0 * sizeof(T)));
@@ -1392,7 +1396,7 @@ static void line_solve(cv_ctx *c, int l, const T *r, const T *xi, T *xo, T *tmp,
       hipLaunchKernelGGL((k_yline_reg<T, sizeof(T) == 8 ? 16 : 32>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r,
                          A.ylo, A.yip, A.ycp, xi, xo, om, zg ? 1 : 0, (const uint2 *)nullptr);
   } else {
-    hipLaunchKernelGGL(k_yline_solve<T>, g, dim3(64, YL_NC), 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
+    hipLaunchKernelGGL(k_yline_solve<T>, gs, dim3(64, YL_NC), 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
                        A.ycp, xi, xo, tmp, om, zg ? 1 : 0);
   }
 }
