@@ -36,6 +36,12 @@ model of the surveyed design (per storage point 4 x 428 + 65 +
 sum_j (173 + V + 16 j) + 8 N + 16 + 307 bytes with the counted FGMRES
 iterations N) over the measured step time.
 
+``shard_share_8way``: one GPU's share of an 8-way k-slab decomposition of
+the workload (512x256x32 cells at the same spacing, no halo exchange), its
+momentum and per-FGMRES-iteration times to set beside 1/8 of the full
+step's (a slab solved alone needs more iterations than the coupled solve,
+so its whole step is not the shard's).
+
 ``cpu_baseline``: the C oracle (the reference algorithm restated: assembled
 19-point CSR operator, FGMRES(30) with a 3-level V-cycle and block-Jacobi
 ILU(0) Richardson smoothing, one block per thread, RK momentum) timed for
@@ -115,18 +121,21 @@ def reset(u, arrays):
     u.put("ucat", np.zeros(u._shape("ucat", -1)))
 
 
-def timed_steps(u, tag, warmup, steps, prof=None):
+def timed_steps(u, tag, warmup, steps, prof=None, stats_out=None):
     for s in range(warmup):
         st = u.Flow_Solver()
         log(f"[{tag}] warmup {s}: its={st['ksp_its']} t_p={st['t_p']:.4f}s")
     if prof is not None:
         u.prof_select(*prof)
-    its = []
+    its, stats = [], []
     t0 = time.perf_counter()
     for s in range(steps):
         st = u.Flow_Solver()
         its.append(st["ksp_its"])
+        stats.append(st)
     t1 = time.perf_counter()
+    if stats_out is not None:
+        stats_out.extend(stats)
     pr = u.prof_read() if prof is not None else None
     if prof is not None:
         u.prof_select("none")
@@ -222,6 +231,39 @@ def step_roofline(npts, its, seconds, vcycle):
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS}
 
 
+def shard_share(full_case, steps, warmup, mixed):
+    """One GPU's share of an 8-way k-slab decomposition of the workload: the
+    512x256x32 slab at the same grid spacing (z length 1/8, periodic; the
+    wall warp scaled by 1/8 with its z period, so its slope is the full
+    grid's), no halo exchange and no body, stepped alone in the mixed
+    precision."""
+    from curvib_old_amd import cases, serving
+    cells = (full_case.cells[0], full_case.cells[1], full_case.cells[2] // 8)
+    case = cases.config_case(2, cells=cells, lz=full_case.lz / 8, amp=full_case.amp / 8, sphere=None)
+    case = cases.config_case(2, cells=cells, lz=full_case.lz / 8, amp=full_case.amp / 8, sphere=None,
+                             dt=min(full_case.dt, cases.stable_dt(case)))
+    arrays = cases.build(case)
+    levels = serving.levels_for(cells, semi=(0, 1, 0), coarsen_auto=1)
+    u = make_ctx(case, arrays, levels, (0, 1, 0), mixed, 1)
+    stats = []
+    try:
+        t, its, _ = timed_steps(u, f"{mixed} shard share", warmup, steps, stats_out=stats)
+    finally:
+        u.close()
+    return {"cells": list(cells), "mg_levels": levels, "precision": mixed, "dt": case.dt, "ms_per_step": 1e3 * t / steps,
+            "its": its, "cell_updates_per_s": float(np.prod(cells)) * steps / t, **phase_ms(stats)}
+
+
+def phase_ms(stats):
+    """Mean momentum and pressure-solve milliseconds per step, and the solve's
+    milliseconds per FGMRES iteration, from the library's step statistics."""
+    n = max(len(stats), 1)
+    t_mom = 1e3 * sum(x["t_mom"] for x in stats) / n
+    t_p = 1e3 * sum(x["t_p"] for x in stats) / n
+    its = sum(x["ksp_its"] for x in stats) / n
+    return {"momentum_ms": t_mom, "poisson_ms": t_p, "poisson_ms_per_iteration": t_p / its if its else None}
+
+
 def serving_workloads(n_serve, n_ens, seed, log_workers, ib=True):
     """Requests of the serving stream (Poisson arrivals, lam 50/s, grids
     32^3..192^3) and of the ensemble (grids 32^3..128x96x96), inputs built
@@ -304,6 +346,8 @@ def main():
     ap.add_argument("--no-fp32-leg", action="store_true", help="skip the CV_MIXED (FP32 coefficient) leg")
     ap.add_argument("--no-stated-leg", action="store_true",
                     help="skip the leg with the stated preconditioner (FP32, 4 levels, 2+2 red-black sweeps)")
+    ap.add_argument("--no-shard-leg", action="store_true",
+                    help="skip the one-GPU share of an 8-way k-slab decomposition (512x256x32)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-ib", action="store_true", help="no immersed-boundary reconstruction")
@@ -368,7 +412,8 @@ def main():
     log(f"[setup] device bytes per context: {u64.device_bytes() / 2**30:.1f} GiB")
 
     t64, its64, _ = timed_steps(u64, "fp64", args.warmup, args.steps)
-    tmx, itsmx, pr = timed_steps(umx, args.mixed, args.warmup, args.steps, prof=("resid", -1, 1))
+    stats_mx = []
+    tmx, itsmx, pr = timed_steps(umx, args.mixed, args.warmup, args.steps, prof=("resid", -1, 1), stats_out=stats_mx)
     # the north star's FP32 V-cycle (FP32 coefficients) as a leg of its own
     leg32 = None
     if args.mixed != "mixed" and not args.no_fp32_leg:
@@ -394,6 +439,7 @@ def main():
                   "over_fp64": t64 / tpc, "accuracy": accpc}
 
     acc = [] if args.no_accuracy else accuracy_pass(u64, umx, arrays, args.steps)
+    shard = None if args.no_shard_leg else shard_share(case, args.steps, args.warmup, args.mixed)
     # host-array exchange of a drop-in caller that hands Ucont in and takes
     # Ucont and P back every step (pageable numpy buffers, PCIe)
     t0 = time.perf_counter()
@@ -462,10 +508,11 @@ def main():
         "fp64": {"steps_per_s": sps_64, "ms_per_step": 1e3 * t64 / args.steps, "its": its64,
                  "cell_updates_per_s": ncell * sps_64},
         "mixed": {"precision": args.mixed, "steps_per_s": sps_mx, "ms_per_step": 1e3 * tmx / args.steps,
-                  "its": itsmx, "cell_updates_per_s": ncell * sps_mx},
+                  "its": itsmx, "cell_updates_per_s": ncell * sps_mx, **phase_ms(stats_mx)},
         "mixed_over_fp64": sps_mx / sps_64,
         "mixed_fp32": leg32,
         "mixed_fp32_4level_rb": leg_pc,
+        "shard_share_8way": shard,
         "host_exchange_ms_per_step": xchg_ms,
         "pcie_inclusive_mixed_ms_per_step": 1e3 * tmx / args.steps + xchg_ms,
         "accuracy": acc,
