@@ -70,8 +70,9 @@ enum { BX = 0, BB = 1, BR = 2, BZJ = 3, BW = 4, BPHI = 5, BRHS = 6, BVJ = 7, BVJ
 template <class T>
 __device__ inline T *bvec(const BCase &C, int d, int s, int jj);
 template <>
-__device__ inline float *bvec<float>(const BCase &C, int d, int s, int) {
-  return s == BX ? C.lev[d].xf : s == BB ? C.lev[d].bf : C.lev[d].rf;
+__device__ inline float *bvec<float>(const BCase &C, int d, int s, int jj) {
+  // BZJ: the FP32 preconditioned vector Z_jj of the mixed solve
+  return s == BX ? C.lev[d].xf : s == BB ? C.lev[d].bf : s == BZJ ? (float *)C.Z + (long)jj * C.n : C.lev[d].rf;
 }
 template <>
 __device__ inline double *bvec<double>(const BCase &C, int d, int s, int jj) {
@@ -110,14 +111,17 @@ __device__ inline bool bblock(const uint32_t *tab, unsigned long long pm, const 
   if (!bblock(tab, pm, ctl, c, lb)) return;      \
   const BCase &C = cs[c];
 
-template <class T, int MODE, bool HG, int TW>
+// TX: element type of the input vector (FP32 for the mixed solve's Z_jj
+// under the FP64 apply)
+template <class T, int MODE, bool HG, int TW, class TX = T>
 __global__ void __launch_bounds__(TW * 4) kb_stencil(BK_ARGS, int d, int sx, int sb, int sy) {
   BK_HEAD
   const OpV<T> A = bop<T>(C, d);
   const int gx = C.lev[d].sgx, gy = C.lev[d].sgy, jj = ctl->jj;
   const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
-  stencil_tile<T, MODE, 4, true, HG, TW>(A, bvec<T>(C, d, sx, jj), MODE >= 1 ? bvec<T>(C, d, sb, jj) : (const T *)nullptr,
-                                 bvec<T>(C, d, sy, jj), A.dinv, (T)0, C.lev[d].kc, bx, by, bz);
+  stencil_tile<T, MODE, 4, true, HG, TW, TX>(A, (const TX *)bvec<TX>(C, d, sx, jj),
+                                     MODE >= 1 ? bvec<T>(C, d, sb, jj) : (const T *)nullptr, bvec<T>(C, d, sy, jj),
+                                     A.dinv, (T)0, C.lev[d].kc, bx, by, bz);
 }
 
 // x += omega T^-1 r in place (zg: x = omega T^-1 r)
@@ -189,10 +193,10 @@ __global__ void kb_conv(BK_ARGS, int dir) {
     const double *s = bvec<double>(C, 0, BVJ, jj);
     T *o = bvec<T>(C, 0, BB, jj);
     for (long t = (long)lb * BS + threadIdx.x; t < n; t += st) o[t] = (T)s[t];
-  } else {
+  } else {   // the V-cycle's result to Z_jj (FP32 in the mixed solve)
     const T *s = bvec<T>(C, 0, BX, jj);
-    double *o = bvec<double>(C, 0, BZJ, jj);
-    for (long t = (long)lb * BS + threadIdx.x; t < n; t += st) o[t] = (double)s[t];
+    T *o = bvec<T>(C, 0, BZJ, jj);
+    for (long t = (long)lb * BS + threadIdx.x; t < n; t += st) o[t] = s[t];
   }
 }
 
@@ -211,9 +215,11 @@ __global__ void kb_scale(BK_ARGS, int sdst, int ssrc, int bslot) {
                 (long)lb * BS + threadIdx.x, (long)C.lev[0].nvb * BS);
 }
 
+template <class TZ>
 __global__ void kb_xupdate(BK_ARGS) {
   BK_HEAD
-  xupdate_body(C.phi, C.Z, C.n, C.bsc + 8, (int)C.bsc[2], C.n, (long)lb * BS + threadIdx.x, (long)C.lev[0].nvb * BS);
+  xupdate_body<TZ>(C.phi, (const TZ *)C.Z, C.n, C.bsc + 8, (int)C.bsc[2], C.n, (long)lb * BS + threadIdx.x,
+                   (long)C.lev[0].nvb * BS);
 }
 
 __global__ void kb_sumsq(BK_ARGS, int sx) {
@@ -430,7 +436,7 @@ static int b_variant(const cv_batch *B, int d, unsigned long long pm) {
 #define BPICK(full, var, d, pm) (b_variant(B, d, pm) < 0 ? (full)[d] : (var)[d][b_variant(B, d, pm)])
 #define BLAUNCH_OK(t, pm) ((t).len > 0 && (pm) != 0)
 
-template <class T, int MODE>
+template <class T, int MODE, class TX = T>
 static void b_stencil(cv_batch *B, int d, unsigned long long pm, int sx, int sb, int sy) {
   const BTab &t = BPICK(B->st, B->st_v, d, pm);
   if (!BLAUNCH_OK(t, pm)) return;
@@ -440,17 +446,17 @@ static void b_stencil(cv_batch *B, int d, unsigned long long pm, int sx, int sb,
   const BCtl *ctl = B->dctl;
   if (B->tw == 32) {
     if (hg)
-      hipLaunchKernelGGL((kb_stencil<T, MODE, sizeof(T) == 4, 32>), dim3((unsigned)t.len), dim3(32, 4), 0, B->stream, cs,
+      hipLaunchKernelGGL((kb_stencil<T, MODE, sizeof(T) == 4, 32, TX>), dim3((unsigned)t.len), dim3(32, 4), 0, B->stream, cs,
                          tb, pm, ctl, d, sx, sb, sy);
     else
-      hipLaunchKernelGGL((kb_stencil<T, MODE, false, 32>), dim3((unsigned)t.len), dim3(32, 4), 0, B->stream, cs, tb, pm,
+      hipLaunchKernelGGL((kb_stencil<T, MODE, false, 32, TX>), dim3((unsigned)t.len), dim3(32, 4), 0, B->stream, cs, tb, pm,
                          ctl, d, sx, sb, sy);
   } else {
     if (hg)
-      hipLaunchKernelGGL((kb_stencil<T, MODE, sizeof(T) == 4, 64>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream, cs,
+      hipLaunchKernelGGL((kb_stencil<T, MODE, sizeof(T) == 4, 64, TX>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream, cs,
                          tb, pm, ctl, d, sx, sb, sy);
     else
-      hipLaunchKernelGGL((kb_stencil<T, MODE, false, 64>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream, cs, tb, pm,
+      hipLaunchKernelGGL((kb_stencil<T, MODE, false, 64, TX>), dim3((unsigned)t.len), dim3(64, 4), 0, B->stream, cs, tb, pm,
                          ctl, d, sx, sb, sy);
   }
 }
@@ -556,20 +562,23 @@ static void b_precond_apply(cv_batch *B) {
   const unsigned long long all = B->all;
   const BTab &v0 = B->vc[0];
   if (B->opt.precision != CV_FP64) {
+    // Z_jj in FP32, as in the single-case solve
     hipLaunchKernelGGL(kb_conv<float>, dim3((unsigned)v0.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
                        (const uint32_t *)BTAB(v0), all, (const BCtl *)B->dctl, 0);
     b_vcycle<float>(B, 0);
     hipLaunchKernelGGL(kb_conv<float>, dim3((unsigned)v0.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
                        (const uint32_t *)BTAB(v0), all, (const BCtl *)B->dctl, 1);
+    b_nullspace<float>(B, 0, all, BZJ);
+    b_stencil<double, 0, float>(B, 0, all, BZJ, BZJ, BW);
   } else {
     hipLaunchKernelGGL(kb_conv<double>, dim3((unsigned)v0.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
                        (const uint32_t *)BTAB(v0), all, (const BCtl *)B->dctl, 0);
     b_vcycle<double>(B, 0);
     hipLaunchKernelGGL(kb_conv<double>, dim3((unsigned)v0.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
                        (const uint32_t *)BTAB(v0), all, (const BCtl *)B->dctl, 1);
+    b_nullspace<double>(B, 0, all, BZJ);
+    b_stencil<double, 0>(B, 0, all, BZJ, BZJ, BW);
   }
-  b_nullspace<double>(B, 0, all, BZJ);
-  b_stencil<double, 0>(B, 0, all, BZJ, BZJ, BW);
 }
 
 static void b_vec(cv_batch *B, void (*k)(BK_ARGS, int, int), unsigned long long pm, int a0, int a1) {
@@ -747,7 +756,8 @@ static int b_fgmres(cv_batch *B) {
     b_upload_ctl(B, cyc, 0);
     {
       const BTab &t = B->vc[0];
-      hipLaunchKernelGGL(kb_xupdate, dim3((unsigned)t.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
+      hipLaunchKernelGGL(B->opt.precision != CV_FP64 ? kb_xupdate<float> : kb_xupdate<double>, dim3((unsigned)t.len),
+                         dim3(BS), 0, B->stream, (const BCase *)B->dcases,
                          (const uint32_t *)BTAB(t), cyc, (const BCtl *)B->dctl);
     }
     b_stencil<double, 1>(B, 0, cyc, BPHI, BRHS, BW);

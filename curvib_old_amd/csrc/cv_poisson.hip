@@ -284,8 +284,10 @@ struct StCell {
 // row) or 32 (two rows per wavefront; the i-face shuffle stays within a
 // row); remap: XCD-aware tile order
 // one tile (bx, by, bz) of the stencil: (TW-1) x TY cells of kchunk planes
-template <class T, int MODE, int TY, bool PFB, bool HG = false, int TW = 64>
-__device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restrict__ x, const T *__restrict__ b,
+// TX: element type of x (FP32 preconditioned vectors under the FP64 apply
+// of the mixed solve; widened exactly on load)
+template <class T, int MODE, int TY, bool PFB, bool HG = false, int TW = 64, class TX = T>
+__device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restrict__ x, const T *__restrict__ b,
                                     T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk, int bx,
                                     int by, int bz, int color = -1) {
   static_assert(TW == 64 || TW == 32, "tile rows of 64 or 32 lanes");
@@ -325,8 +327,8 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
   auto fetch_x = [&](int gk, T &v0, T &v1) {
     const bool kin = gk >= 0 && gk < mz;
     const long kb = kin ? pl * cv_wrap(gk, mz, A.zper) : 0;
-    v0 = (kin && ok0) ? x[kb + off0] : (T)0;
-    v1 = (kin && ok1) ? x[kb + off1] : (T)0;
+    v0 = (kin && ok0) ? (T)x[kb + off0] : (T)0;
+    v1 = (kin && ok1) ? (T)x[kb + off1] : (T)0;
   };
   auto store_x = [&](int gk, T v0, T v1) {
     T *S = X[(gk + 4) & 3];
@@ -449,7 +451,7 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
       // MODE 2 with a colour: cells of the other (i+j+k) parity keep x
       const bool keep = MODE == 2 && color >= 0 && ((gi + gj + gk) & 1) != color;
       if (!(w & 1u)) {
-        const T xv = x[t];
+        const T xv = (T)x[t];
         if (MODE == 0) res = xv;
         else if (MODE == 1) res = bc - xv;
         else res = keep ? xv : xv + omega * dvc * (bc - xv);
@@ -473,14 +475,14 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const T *__restric
   }
 }
 
-template <class T, int MODE, int TY, bool PFB, bool HG = false, int TW = 64>
-__global__ void __launch_bounds__(TW * TY) k_stencil(OpV<T> A, const T *__restrict__ x, const T *__restrict__ b,
+template <class T, int MODE, int TY, bool PFB, bool HG = false, int TW = 64, class TX = T>
+__global__ void __launch_bounds__(TW * TY) k_stencil(OpV<T> A, const TX *__restrict__ x, const T *__restrict__ b,
                                                    T *__restrict__ y, const T *__restrict__ dinv, T omega, int kchunk,
                                                    int remap, int color) {
   int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
   if (remap == 1) cv_xcd_tile(bx, by, bz);
   else if (remap > 1) cv_xcd_runs(remap, bx, by, bz);
-  stencil_tile<T, MODE, TY, PFB, HG, TW>(A, x, b, y, dinv, omega, kchunk, bx, by, bz, color);
+  stencil_tile<T, MODE, TY, PFB, HG, TW, TX>(A, x, b, y, dinv, omega, kchunk, bx, by, bz, color);
 }
 
 // ------------------------------------------------------------------
@@ -913,21 +915,28 @@ __global__ void k_scale_to(double *__restrict__ dst, const double *__restrict__ 
   scale_to_body(dst, src, inv[0], n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
 }
 
-// x += sum_q y[q] Z_q
-__device__ __forceinline__ void xupdate_body(double *__restrict__ x, const double *__restrict__ Z, long stride,
+// x += sum_q y[q] Z_q (Z in FP64, or the FP32 preconditioned vectors of
+// the mixed solve, widened exactly)
+template <class TZ>
+__device__ __forceinline__ void xupdate_body(double *__restrict__ x, const TZ *__restrict__ Z, long stride,
                                     const double *__restrict__ y, int m, long n, long t0, long step) {
   __shared__ double ys[MAXV];
   if (threadIdx.x < m) ys[threadIdx.x] = y[threadIdx.x];
   __syncthreads();
   for (long t = t0; t < n; t += step) {
     double v = x[t];
-    for (int q = 0; q < m; q++) v += ys[q] * Z[q * stride + t];
+    for (int q = 0; q < m; q++) v += ys[q] * (double)Z[q * stride + t];
     x[t] = v;
   }
 }
-__global__ void k_xupdate(double *__restrict__ x, const double *__restrict__ Z, long stride, const double *__restrict__ y,
+template <class TZ>
+__global__ void k_xupdate(double *__restrict__ x, const TZ *__restrict__ Z, long stride, const double *__restrict__ y,
                           int m, long n) {
-  xupdate_body(x, Z, stride, y, m, n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
+  xupdate_body<TZ>(x, Z, stride, y, m, n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
+}
+template <class T>
+__global__ void k_copy_t(T *__restrict__ dst, const T *__restrict__ src, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = src[t];
 }
 
 __device__ __forceinline__ void sumsq_body(const double *__restrict__ x, long n, double *part, int bid, int nb) {
@@ -1093,8 +1102,8 @@ static int stencil_tw32_rows() {
   return rows;
 }
 
-template <class T, int MODE>
-static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T)0, int color = -1) {
+template <class T, int MODE, class TX = T>
+static void stencil(cv_ctx *c, int l, const TX *x, const T *b, T *y, T omega = (T)0, int color = -1) {
   CvLevel &L = c->lev[l];
   OpV<T> A = opview<T>(c, l);
   const int kc = stencil_kchunk(L.d.mz);
@@ -1105,50 +1114,50 @@ static void stencil(cv_ctx *c, int l, const T *x, const T *b, T *y, T omega = (T
   const int remap = stencil_remap(g.x);
   const bool hg = sizeof(T) == 4 && half_coef(c) && L.gh;
   CvProfScope ps(c, MODE == 0 ? CV_PROF_APPLY : CV_PROF_RESID, l, sizeof(T) == 4 ? 1 : 0,
-                 (double)L.d.n * stencil_bytes_pt<T>(MODE, hg));
+                 (double)L.d.n * (stencil_bytes_pt<T>(MODE, hg) - (double)(sizeof(T) - sizeof(TX))));
   const bool pfb = sizeof(T) == 4 || stencil_pfb64();
   if (tw == 32) {
     if constexpr (sizeof(T) == 4) {
       if (hg) {
         if (ty == 8)
-          hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, true, 32>), g, dim3(32, 8), 0, c->stream, A, x, b, y, A.dinv,
+          hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, true, 32, TX>), g, dim3(32, 8), 0, c->stream, A, x, b, y, A.dinv,
                              omega, kc, remap, color);
         else
-          hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, true, 32>), g, dim3(32, 4), 0, c->stream, A, x, b, y, A.dinv,
+          hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, true, 32, TX>), g, dim3(32, 4), 0, c->stream, A, x, b, y, A.dinv,
                              omega, kc, remap, color);
         return;
       }
     }
     if (ty == 8)
-      hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, false, 32>), g, dim3(32, 8), 0, c->stream, A, x, b, y, A.dinv,
+      hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, false, 32, TX>), g, dim3(32, 8), 0, c->stream, A, x, b, y, A.dinv,
                          omega, kc, remap, color);
     else
-      hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, false, 32>), g, dim3(32, 4), 0, c->stream, A, x, b, y, A.dinv,
+      hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, false, 32, TX>), g, dim3(32, 4), 0, c->stream, A, x, b, y, A.dinv,
                          omega, kc, remap, color);
     return;
   }
   if constexpr (sizeof(T) == 4) {
     if (hg) {
       if (ty == 4)
-        hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, true>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega,
+        hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, true, 64, TX>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega,
                            kc, remap, color);
       else
-        hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, true>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega,
+        hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, true, 64, TX>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega,
                            kc, remap, color);
       return;
     }
   }
   if (ty == 4 && pfb)
-    hipLaunchKernelGGL((k_stencil<T, MODE, 4, true>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
+    hipLaunchKernelGGL((k_stencil<T, MODE, 4, true, false, 64, TX>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
                        color);
   else if (ty == 4)
-    hipLaunchKernelGGL((k_stencil<T, MODE, 4, false>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
+    hipLaunchKernelGGL((k_stencil<T, MODE, 4, false, false, 64, TX>), g, dim3(64, 4), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
                        color);
   else if (pfb)
-    hipLaunchKernelGGL((k_stencil<T, MODE, 8, true>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
+    hipLaunchKernelGGL((k_stencil<T, MODE, 8, true, false, 64, TX>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
                        color);
   else
-    hipLaunchKernelGGL((k_stencil<T, MODE, 8, false>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
+    hipLaunchKernelGGL((k_stencil<T, MODE, 8, false, false, 64, TX>), g, dim3(64, 8), 0, c->stream, A, x, b, y, A.dinv, omega, kc, remap,
                        color);
 }
 
@@ -1463,33 +1472,46 @@ static void vcycle(cv_ctx *c, MgVecs<T> &v, int l) {
 }
 
 // z = M^{-1} v on the finest level (FP64 or FP32 V-cycle), then the nullspace rule
-static void precondition(cv_ctx *c, const double *vin, double *zout) {
+// Z = M^-1 vin with the nullspace rule.  Mixed precisions: the FP32 V-cycle
+// writes its result straight into zout, which then holds the FP32
+// preconditioned vector (the FP64 apply and the solution update widen it
+// exactly), and the nullspace rule runs on it in FP32 with FP64 sums
+static void precondition(cv_ctx *c, const double *vin, void *zout) {
   const int top = c->nlev - 1;
   CvLevel &F = c->lev[top];
   const long n = F.d.n;
   if (c->opt.precision != CV_FP64) {
+    float *const zf = (float *)zout;
     MgVecs<float> v = mgvecs<float>(c);
+    float *const own_x = v.x[top], *const own_r = v.r[top];
     hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, v.b[top], vin, n);
+    v.x[top] = zf;
     vcycle<float>(c, v, top);
-    hipLaunchKernelGGL(k_f2d, dim3(nblk(n)), dim3(BS), 0, c->stream, zout, (const float *)v.x[top], n);
+    if (v.x[top] != zf)
+      hipLaunchKernelGGL(k_copy_t<float>, dim3(nblk(n)), dim3(BS), 0, c->stream, zf, (const float *)v.x[top], n);
+    v.x[top] = own_x;
+    v.r[top] = own_r;
     for (int l = 0; l < c->nlev; l++) { c->lev[l].xf = v.x[l]; c->lev[l].rf = v.r[l]; }
+    nullspace_dev<float>(c, top, zf);
+    return;
   } else {
     // the finest level reads its right-hand side from vin and starts its
     // x/r ping-pong in zout: after an even number of sweeps the result is
     // in zout already (the V-cycle only reads b, and the arithmetic does
     // not depend on which buffer holds x)
+    double *const zd = (double *)zout;
     MgVecs<double> v = mgvecs<double>(c);
     double *const own_x = v.x[top], *const own_r = v.r[top];
     v.b[top] = const_cast<double *>(vin);
-    v.x[top] = zout;
+    v.x[top] = zd;
     vcycle<double>(c, v, top);
-    if (v.x[top] != zout)
-      hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(BS), 0, c->stream, zout, (const double *)v.x[top], n);
+    if (v.x[top] != zd)
+      hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(BS), 0, c->stream, zd, (const double *)v.x[top], n);
     v.x[top] = own_x;
     v.r[top] = own_r;
     for (int l = 0; l < c->nlev; l++) { c->lev[l].x = v.x[l]; c->lev[l].r = v.r[l]; }
   }
-  nullspace_dev<double>(c, top, zout);
+  nullspace_dev<double>(c, top, (double *)zout);
 }
 
 static int ensure_krylov(cv_ctx *c) {
@@ -1534,6 +1556,9 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
   if (rc) return rc;
   const int m = c->opt.restart;
   double *V = c->V, *Z = c->Z, *w = c->w;
+  // the preconditioned vectors of the mixed solve are the FP32 V-cycle's
+  // output, kept in FP32 (packed in the first half of Z's allocation)
+  const bool zf32 = c->opt.precision != CV_FP64;
   double *hd = c->dsc;              // device: h[0..33]
   double *hh = c->red_host;         // pinned host
   // ||b||
@@ -1563,9 +1588,16 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
     g[0] = beta;
     int jj = 0;
     for (; jj < m && its < c->opt.max_it; jj++) {
-      double *vj = V + (size_t)jj * n, *zj = Z + (size_t)jj * n;
-      precondition(c, vj, zj);
-      stencil<double, 0>(c, top, (const double *)zj, (const double *)nullptr, w);
+      double *vj = V + (size_t)jj * n;
+      if (zf32) {
+        float *zj = (float *)Z + (size_t)jj * n;
+        precondition(c, vj, zj);
+        stencil<double, 0, float>(c, top, (const float *)zj, (const double *)nullptr, w);
+      } else {
+        double *zj = Z + (size_t)jj * n;
+        precondition(c, vj, zj);
+        stencil<double, 0>(c, top, (const double *)zj, (const double *)nullptr, w);
+      }
       const int nv = jj + 1;
       // first CGS pass
       gram_schmidt_pass(c, w, V, nv, n);
@@ -1613,7 +1645,12 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
     }
     for (int q = 0; q < jj; q++) hh[q] = y[q];
     CV_CHECK(hipMemcpyAsync(hd, hh, jj * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_xupdate, dim3(nblk(n)), dim3(BS), 0, c->stream, x, (const double *)Z, (long)n, (const double *)hd, jj, n);
+    if (zf32)
+      hipLaunchKernelGGL(k_xupdate<float>, dim3(nblk(n)), dim3(BS), 0, c->stream, x, (const float *)Z, (long)n,
+                         (const double *)hd, jj, n);
+    else
+      hipLaunchKernelGGL(k_xupdate<double>, dim3(nblk(n)), dim3(BS), 0, c->stream, x, (const double *)Z, (long)n,
+                         (const double *)hd, jj, n);
     // true residual
     stencil<double, 1>(c, top, (const double *)x, b, w);
     hipLaunchKernelGGL(k_sumsq, dim3(nred(n)), dim3(BS), 0, c->stream, (const double *)w, n, c->part);
