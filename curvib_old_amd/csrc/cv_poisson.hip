@@ -914,6 +914,17 @@ __device__ __forceinline__ void scale_to_body(double *__restrict__ dst, const do
 __global__ void k_scale_to(double *__restrict__ dst, const double *__restrict__ src, const double *inv, long n) {
   scale_to_body(dst, src, inv[0], n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
 }
+// the same, also writing the FP32 copy the mixed V-cycle takes as its
+// right-hand side (the conversion it would otherwise make on entry)
+__global__ void k_scale_to_f32(double *__restrict__ dst, const double *__restrict__ src, const double *inv,
+                               float *__restrict__ f32, long n) {
+  const double a = inv[0];
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const double v = src[t] * a;
+    dst[t] = v;
+    f32[t] = (float)v;
+  }
+}
 
 // x += sum_q y[q] Z_q (Z in FP64, or the FP32 preconditioned vectors of
 // the mixed solve, widened exactly)
@@ -1476,7 +1487,8 @@ static void vcycle(cv_ctx *c, MgVecs<T> &v, int l) {
 // writes its result straight into zout, which then holds the FP32
 // preconditioned vector (the FP64 apply and the solution update widen it
 // exactly), and the nullspace rule runs on it in FP32 with FP64 sums
-static void precondition(cv_ctx *c, const double *vin, void *zout) {
+// b_ready: the FP32 right-hand side of the mixed V-cycle already holds vin
+static void precondition(cv_ctx *c, const double *vin, void *zout, bool b_ready = false) {
   const int top = c->nlev - 1;
   CvLevel &F = c->lev[top];
   const long n = F.d.n;
@@ -1484,7 +1496,7 @@ static void precondition(cv_ctx *c, const double *vin, void *zout) {
     float *const zf = (float *)zout;
     MgVecs<float> v = mgvecs<float>(c);
     float *const own_x = v.x[top], *const own_r = v.r[top];
-    hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, v.b[top], vin, n);
+    if (!b_ready) hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BS), 0, c->stream, v.b[top], vin, n);
     v.x[top] = zf;
     vcycle<float>(c, v, top);
     if (v.x[top] != zf)
@@ -1583,15 +1595,21 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
   while (beta > tol && its < c->opt.max_it) {
     hh[0] = 1.0 / beta;
     CV_CHECK(hipMemcpyAsync(hd + 48, hh, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_scale_to, dim3(nblk(n)), dim3(BS), 0, c->stream, V, (const double *)w, (const double *)(hd + 48), n);
+    if (zf32)
+      hipLaunchKernelGGL(k_scale_to_f32, dim3(nblk(n)), dim3(BS), 0, c->stream, V, (const double *)w,
+                         (const double *)(hd + 48), c->lev[top].bf, n);
+    else
+      hipLaunchKernelGGL(k_scale_to, dim3(nblk(n)), dim3(BS), 0, c->stream, V, (const double *)w,
+                         (const double *)(hd + 48), n);
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
     int jj = 0;
     for (; jj < m && its < c->opt.max_it; jj++) {
       double *vj = V + (size_t)jj * n;
       if (zf32) {
+        // V_jj's FP32 copy was written with V_jj itself (k_scale_to_f32)
         float *zj = (float *)Z + (size_t)jj * n;
-        precondition(c, vj, zj);
+        precondition(c, vj, zj, true);
         stencil<double, 0, float>(c, top, (const float *)zj, (const double *)nullptr, w);
       } else {
         double *zj = Z + (size_t)jj * n;
@@ -1618,8 +1636,12 @@ static int fgmres(cv_ctx *c, const double *b, double *x, int *its_out, double *r
       if (hn > 0) {
         hh[0] = 1.0 / hn;
         CV_CHECK(hipMemcpyAsync(hd + 48, hh, sizeof(double), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_scale_to, dim3(nblk(n)), dim3(BS), 0, c->stream, V + (size_t)(jj + 1) * n,
-                           (const double *)w, (const double *)(hd + 48), n);
+        if (zf32)
+          hipLaunchKernelGGL(k_scale_to_f32, dim3(nblk(n)), dim3(BS), 0, c->stream, V + (size_t)(jj + 1) * n,
+                             (const double *)w, (const double *)(hd + 48), c->lev[top].bf, n);
+        else
+          hipLaunchKernelGGL(k_scale_to, dim3(nblk(n)), dim3(BS), 0, c->stream, V + (size_t)(jj + 1) * n,
+                             (const double *)w, (const double *)(hd + 48), n);
       }
       for (int q = 0; q < jj; q++) {
         double a = H[q * m + jj], bb = H[(q + 1) * m + jj];
