@@ -324,16 +324,19 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restri
     ok1 = e1 < NT && ii >= 0 && ii < mx && jj >= 0 && jj < my;
     if (ok1) off1 = (long)mx * jj + cv_wrap(ii, mx, A.xper);
   }
-  auto fetch_x = [&](int gk, T &v0, T &v1) {
+  // x values travel in their own type until they are stored to LDS, so a
+  // narrower x (FP32 under the FP64 apply) is widened only when its
+  // prefetch is consumed, not when it is issued
+  auto fetch_x = [&](int gk, TX &v0, TX &v1) {
     const bool kin = gk >= 0 && gk < mz;
     const long kb = kin ? pl * cv_wrap(gk, mz, A.zper) : 0;
-    v0 = (kin && ok0) ? (T)x[kb + off0] : (T)0;
-    v1 = (kin && ok1) ? (T)x[kb + off1] : (T)0;
+    v0 = (kin && ok0) ? x[kb + off0] : (TX)0;
+    v1 = (kin && ok1) ? x[kb + off1] : (TX)0;
   };
-  auto store_x = [&](int gk, T v0, T v1) {
+  auto store_x = [&](int gk, TX v0, TX v1) {
     T *S = X[(gk + 4) & 3];
-    if (tid < NT) S[tid] = v0;
-    if (e1 < NT) S[e1] = v1;
+    if (tid < NT) S[tid] = (T)v0;
+    if (e1 < NT) S[e1] = (T)v1;
   };
   constexpr bool PF = sizeof(T) == 4;      // prefetch aj, b, dinv with the cell
   auto gco = [&](int q, long t) -> T {
@@ -397,11 +400,11 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restri
   };
 
   // prologue: planes k0-1 .. k0+1 in LDS, plane k0+2 and the cells of k0 in registers
-  T xa, xb;
+  TX xa, xb;
   fetch_x(k0 - 1, xa, xb); store_x(k0 - 1, xa, xb);
   fetch_x(k0, xa, xb); store_x(k0, xa, xb);
   fetch_x(k0 + 1, xa, xb); store_x(k0 + 1, xa, xb);
-  T xn0, xn1;
+  TX xn0, xn1;
   fetch_x(k0 + 2, xn0, xn1);
   StCell<T> cur;
   fetch_cell(k0, cur);
@@ -412,7 +415,7 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restri
     fk_lo = tile_flux_k<T, ST_TC>(X[(k0 - 1 + 4) & 3], X[(k0 + 4) & 3], c, r, wo(t), gco(6, t), gco(7, t), gco(8, t));
   }
   for (int gk = k0; gk < k1; gk++) {
-    T xq0, xq1;
+    TX xq0, xq1;
     fetch_x(gk + 3, xq0, xq1);
     StCell<T> nxt;
     if (gk + 1 < k1) fetch_cell(gk + 1, nxt);
