@@ -275,6 +275,10 @@ struct StCell {
   uint32_t w, wb;        // operator words of the cell and (row 0) of the cell below the tile
   T g[9], gb[3];
   T aj, b, dv;
+  // FP16 packs as loaded (HG): decoded into w, g, aj, wb, gb only when the
+  // plane is computed, so the prefetch a plane ahead stays in flight
+  uint4 ha;
+  uint2 hb, hbb;
 };
 
 // TY: rows per block (block TW x TY); PFB: prefetch the row-0 face below
@@ -367,16 +371,15 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restri
     C.aj = C.b = C.dv = (T)0;
     C.wb = 0u;
     C.gb[0] = C.gb[1] = C.gb[2] = (T)0;
+    if constexpr (HG) {
+      C.ha = make_uint4(0u, 0u, 0u, 0u);
+      C.hb = C.hbb = make_uint2(0u, 0u);
+    }
     if (!cell) return;
     const long t = tij + pl * gk;
     if constexpr (HG) {
-      const uint4 ha = A.hA[t];
-      const uint2 hb = A.hB[t];
-      C.w = hb.y >> 16;
-      C.g[0] = half_lo(ha.x); C.g[1] = half_hi(ha.x); C.g[2] = half_lo(ha.y);
-      C.g[3] = half_lo(hb.x); C.g[4] = half_hi(hb.x); C.g[5] = half_lo(hb.y);
-      C.g[6] = half_hi(ha.y); C.g[7] = half_lo(ha.z); C.g[8] = half_hi(ha.z);
-      if (out) C.aj = __uint_as_float(ha.w);
+      C.ha = A.hA[t];
+      C.hb = A.hB[t];
     } else {
       C.w = A.fl[t];
 #pragma unroll
@@ -389,12 +392,28 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restri
     }
     if (PFB && row0) {
       if constexpr (HG) {
-        const uint2 hb = A.hB[t - mx];
-        C.wb = hb.y >> 16;
-        C.gb[0] = half_lo(hb.x); C.gb[1] = half_hi(hb.x); C.gb[2] = half_lo(hb.y);
+        C.hbb = A.hB[t - mx];
       } else {
         C.wb = A.fl[t - mx];
         C.gb[0] = A.g[3][t - mx]; C.gb[1] = A.g[4][t - mx]; C.gb[2] = A.g[5][t - mx];
+      }
+    }
+  };
+
+  // the FP16 packs of a fetched cell into its operands (zero where the cell
+  // lies outside the grid: the packs were cleared then)
+  auto decode_cell = [&](StCell<T> &C) {
+    if constexpr (HG) {
+      const uint4 ha = C.ha;
+      const uint2 hb = C.hb, hbb = C.hbb;
+      C.w = hb.y >> 16;
+      C.g[0] = half_lo(ha.x); C.g[1] = half_hi(ha.x); C.g[2] = half_lo(ha.y);
+      C.g[3] = half_lo(hb.x); C.g[4] = half_hi(hb.x); C.g[5] = half_lo(hb.y);
+      C.g[6] = half_hi(ha.y); C.g[7] = half_lo(ha.z); C.g[8] = half_hi(ha.z);
+      C.aj = out ? __uint_as_float(ha.w) : (T)0;
+      if (PFB && row0) {
+        C.wb = hbb.y >> 16;
+        C.gb[0] = half_lo(hbb.x); C.gb[1] = half_hi(hbb.x); C.gb[2] = half_lo(hbb.y);
       }
     }
   };
@@ -421,6 +440,7 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restri
     if (gk + 1 < k1) fetch_cell(gk + 1, nxt);
     const long t = tij + pl * gk;
     const T *Pm = X[(gk - 1 + 4) & 3], *P = X[(gk + 4) & 3], *Q = X[(gk + 1 + 4) & 3];
+    decode_cell(cur);
     const uint32_t w = cur.w;
     const T fi = cell ? tile_flux_i<T, ST_TC>(Pm, P, Q, c, r, w, cur.g[0], cur.g[1], cur.g[2]) : (T)0;
     T fj = (T)0, fk = (T)0, fj_lo = (T)0;
