@@ -106,11 +106,36 @@ __device__ __forceinline__ void c2c_cell(G g, C3 uc, C3 csi, C3 eta, C3 zet, con
   ucat.y[t] = det1 / det;
   ucat.z[t] = det2 / det;
 }
+// the k-march of the interior: the k-1 plane's zet and Ucont.z a cell reads
+// are the previous iteration's own values, carried in registers (a whole
+// plane of the XCD's blocks outgrows its L2 between the two reads)
 __device__ __forceinline__ void contra2cart_body(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n, int bx, int by, int bz) {
   (void)n;
-  CV_MARCH_B(g, bx, by, bz) {
-    if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
-    c2c_cell(g, uc, csi, eta, zet, nv, ucat, i, j, k);
+  const int i = bx * 64 + threadIdx.x, j = by * MARCH_TY + threadIdx.y;
+  if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1)) return;
+  const int k0 = max(bz * MARCH_KC, 1), k1 = min(bz * MARCH_KC + MARCH_KC, g.mz - 1);
+  if (k0 >= k1) return;
+  long t = g.at(i, j, k0 - 1);
+  double zx = zet.x[t], zy = zet.y[t], zz = zet.z[t], wz = uc.z[t];
+  for (int k = k0; k < k1; k++) {
+    t += g.sk;
+    const double zx1 = zet.x[t], zy1 = zet.y[t], zz1 = zet.z[t], wz1 = uc.z[t];
+    const double nn = nv[t];
+    if (nn < 0.1 || nn > 7.) {
+      const long ti = t - 1, tj = t - g.sj;
+      double m00 = 0.5 * (csi.x[ti] + csi.x[t]), m01 = 0.5 * (csi.y[ti] + csi.y[t]), m02 = 0.5 * (csi.z[ti] + csi.z[t]);
+      double m10 = 0.5 * (eta.x[tj] + eta.x[t]), m11 = 0.5 * (eta.y[tj] + eta.y[t]), m12 = 0.5 * (eta.z[tj] + eta.z[t]);
+      double m20 = 0.5 * (zx + zx1), m21 = 0.5 * (zy + zy1), m22 = 0.5 * (zz + zz1);
+      double q0 = 0.5 * (uc.x[ti] + uc.x[t]), q1 = 0.5 * (uc.y[tj] + uc.y[t]), q2 = 0.5 * (wz + wz1);
+      double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+      double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
+      double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
+      double det2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
+      ucat.x[t] = det0 / det;
+      ucat.y[t] = det1 / det;
+      ucat.z[t] = det2 / det;
+    }
+    zx = zx1; zy = zy1; zz = zz1; wz = wz1;
   }
 }
 __global__ void k_contra2cart(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n) { contra2cart_body(g, uc, csi, eta, zet, nv, ucat, n, blockIdx.x, blockIdx.y, blockIdx.z); }
@@ -240,17 +265,35 @@ __global__ void k_edge_average(G g, V3 u, double *p) { edge_average_body(g, u, p
 
 // periodic full-plane copies (bcs.c:5567-5731); axis 0 then 2.  x_only:
 // the reference copies only .x on the upper x plane in FormBCS.
+// Only the ring of the plane (a ghost index on one of the other two axes)
+// is copied: periodic_planes_body copied the rest of the plane earlier in
+// the same pass (all three components, p and nvert alike), and between the
+// two neither those cells nor their sources change (the edge averages write
+// points with two ghost indices, the x copy writes i = 0 and mx-1 only), so
+// a second copy of the plane's interior rewrites the values it holds.
+__device__ __forceinline__ long periodic_ring_size(const G &g, int axis) {
+  const int na = axis == 0 ? g.my : g.mx, nb = axis == 0 ? g.mz : g.my;   // plane extents, a fastest
+  return 2L * nb + 2L * max(na - 2, 0);
+}
 __device__ __forceinline__ void periodic_full_body(G g, int axis, int x_only, V3 u, double *p, double *nv, GS gs) {
   const int mx = g.mx, my = g.my, mz = g.mz;
-  const long n = axis == 0 ? (long)my * mz : (long)mx * my;
+  const int na = axis == 0 ? my : mx, nb = axis == 0 ? mz : my;
+  const long n = periodic_ring_size(g, axis);
   for (long t = gs.t0; t < n; t += gs.step) {
+    int a, b;   // a = 0 or na-1 over every b, then b = 0 or nb-1 over the inner a
+    if (t < 2L * nb) {
+      a = (t & 1) ? na - 1 : 0;
+      b = (int)(t >> 1);
+    } else {
+      const long r = t - 2L * nb;
+      b = (r & 1) ? nb - 1 : 0;
+      a = 1 + (int)(r >> 1);
+    }
     long d0, s0, d1, s1;
     if (axis == 0) {
-      int j = (int)(t % my), k = (int)(t / my);
-      d0 = g.at(0, j, k); s0 = g.at(mx - 2, j, k); d1 = g.at(mx - 1, j, k); s1 = g.at(1, j, k);
+      d0 = g.at(0, a, b); s0 = g.at(mx - 2, a, b); d1 = g.at(mx - 1, a, b); s1 = g.at(1, a, b);
     } else {
-      int i = (int)(t % mx), j = (int)(t / mx);
-      d0 = g.at(i, j, 0); s0 = g.at(i, j, mz - 2); d1 = g.at(i, j, mz - 1); s1 = g.at(i, j, 1);
+      d0 = g.at(a, b, 0); s0 = g.at(a, b, mz - 2); d1 = g.at(a, b, mz - 1); s1 = g.at(a, b, 1);
     }
     u.x[d0] = u.x[s0]; u.y[d0] = u.y[s0]; u.z[d0] = u.z[s0];
     u.x[d1] = u.x[s1];
@@ -1414,10 +1457,10 @@ int cv_ghost_update(cv_ctx *c, int formbcs) {
     // edge line of up to 1024 points in one pass per sub-sweep
     hipLaunchKernelGGL(k_edge_average, dim3(1), dim3(1024), 0, c->stream, g, v3(c->ucat), p);
     if (bxper)
-      hipLaunchKernelGGL(k_periodic_full, dim3(nblk((long)g.my * g.mz)), dim3(BS), 0, c->stream, g, 0, formbcs ? 1 : 0,
+      hipLaunchKernelGGL(k_periodic_full, dim3(nblk(2L * g.mz + 2L * (g.my - 2))), dim3(BS), 0, c->stream, g, 0, formbcs ? 1 : 0,
                          v3(c->ucat), p, nv);
     if (bzper)
-      hipLaunchKernelGGL(k_periodic_full, dim3(nblk((long)g.mx * g.my)), dim3(BS), 0, c->stream, g, 2, 0, v3(c->ucat), p, nv);
+      hipLaunchKernelGGL(k_periodic_full, dim3(nblk(2L * g.my + 2L * (g.mx - 2))), dim3(BS), 0, c->stream, g, 2, 0, v3(c->ucat), p, nv);
   }
   CV_CHECK(hipGetLastError());
   return CV_OK;
@@ -1960,8 +2003,9 @@ __global__ void kbm_edge_average(MB_ARGS, int formbcs) {
 
 __global__ void kbm_periodic_full(MB_ARGS, int axis, int formbcs) {
   MB_HEAD
+  (void)lb;   // one block per case: the ring is at most 2 (my + mz) or 2 (mx + my) points
   periodic_full_body(C.g, axis, axis == 0 ? formbcs : 0, mv3(C.uca), formbcs ? C.p : nullptr,
-                     formbcs ? C.nv : nullptr, MB_GSP);
+                     formbcs ? C.nv : nullptr, GS{(long)threadIdx.x, (long)blockDim.x});
 }
 
 __global__ void __launch_bounds__(64 * 8) kbm_rc_tiled(MB_ARGS) {
@@ -2181,8 +2225,8 @@ int cv_mom_batch_ghost(CvMomBatch *mb, int formbcs, int cart) {
     MB_LAUNCH(kbm_wall_ghost, dim3(BS), mb->t_plane, mb->n_plane, mb->all, formbcs ? 0 : 1);
     MB_LAUNCH(kbm_periodic_planes, dim3(BS), mb->t_plane, mb->n_plane, mb->per, formbcs);
     MB_LAUNCH(kbm_edge_average, dim3(BS), mb->t_one, mb->n_one, mb->all, formbcs);
-    MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_plane, mb->n_plane, mb->xper, 0, formbcs);
-    MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_plane, mb->n_plane, mb->zper, 2, formbcs);
+    MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_one, mb->n_one, mb->xper, 0, formbcs);
+    MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_one, mb->n_one, mb->zper, 2, formbcs);
   }
   CV_CHECK(hipGetLastError());
   return CV_OK;
