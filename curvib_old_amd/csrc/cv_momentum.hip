@@ -1427,6 +1427,11 @@ static void slip_update(cv_ctx *c, int formbcs) {
 // first pass; later passes repeat it only where the pass changed Ucont:
 // the whole grid with far-field faces, the k = mz-2 plane with an outlet
 // alone (the slip faces' zero fluxes are already in place).
+// Only a far-field, slip or outlet face changes Ucont or Ubcs inside a
+// pass; without one, the first pass leaves a fixed point of the pass (wall
+// ghosts and the periodic plane copies take interior values, the edge
+// averages those face ghosts, and each corner's last write edges of the same
+// pass), so one pass gives the three passes' bits.
 // formbcs=0: GhostNodeVelocity (slip Ubcs, then the ghosts)
 int cv_ghost_update(cv_ctx *c, int formbcs) {
   CvLevel &F = c->lev[c->nlev - 1];
@@ -1434,7 +1439,7 @@ int cv_ghost_update(cv_ctx *c, int formbcs) {
   G g = gview(c);
   const int bxper = F.d.xper, bzper = F.d.zper;
   const long planes = (long)g.my * g.mz + (long)g.mx * g.mz + (long)g.mx * g.my;
-  const int passes = formbcs ? 3 : 1;
+  const int passes = formbcs && (c->far || c->slip || c->outlet) ? 3 : 1;
   const bool outl = formbcs && c->outlet, far = formbcs && c->far;
   double *p = formbcs ? c->p : nullptr, *nv = formbcs ? F.nvert : nullptr;
   for (int ps = 0; ps < passes; ps++) {
@@ -2216,18 +2221,17 @@ int cv_mom_batch_create(cv_ctx *const *cases, int n, hipStream_t s, CvMomBatch *
                          (const uint32_t *)(mb->tab + (t)), (unsigned long long)(pm), ##__VA_ARGS__);            \
   } while (0)
 
-// Contra2Cart (formbcs) and the ghost passes of cv_ghost_update for every case
+// Contra2Cart (formbcs) and the ghost pass of cv_ghost_update for every
+// case: the batched cases have only wall and periodic faces, so FormBCS's
+// later passes would rewrite the first pass's bits (cv_ghost_update)
 int cv_mom_batch_ghost(CvMomBatch *mb, int formbcs, int cart) {
   if (cart)
     MB_LAUNCH(kbm_contra2cart, dim3(64, MARCH_TY), mb->t_march, mb->n_march, mb->all);
-  const int passes = formbcs ? 3 : 1;
-  for (int ps = 0; ps < passes; ps++) {
-    MB_LAUNCH(kbm_wall_ghost, dim3(BS), mb->t_plane, mb->n_plane, mb->all, formbcs ? 0 : 1);
-    MB_LAUNCH(kbm_periodic_planes, dim3(BS), mb->t_plane, mb->n_plane, mb->per, formbcs);
-    MB_LAUNCH(kbm_edge_average, dim3(BS), mb->t_one, mb->n_one, mb->all, formbcs);
-    MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_one, mb->n_one, mb->xper, 0, formbcs);
-    MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_one, mb->n_one, mb->zper, 2, formbcs);
-  }
+  MB_LAUNCH(kbm_wall_ghost, dim3(BS), mb->t_plane, mb->n_plane, mb->all, formbcs ? 0 : 1);
+  MB_LAUNCH(kbm_periodic_planes, dim3(BS), mb->t_plane, mb->n_plane, mb->per, formbcs);
+  MB_LAUNCH(kbm_edge_average, dim3(BS), mb->t_one, mb->n_one, mb->all, formbcs);
+  MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_one, mb->n_one, mb->xper, 0, formbcs);
+  MB_LAUNCH(kbm_periodic_full, dim3(BS), mb->t_one, mb->n_one, mb->zper, 2, formbcs);
   CV_CHECK(hipGetLastError());
   return CV_OK;
 }

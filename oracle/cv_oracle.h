@@ -87,6 +87,7 @@ typedef struct {
   int ksp_its; double ksp_rnorm, ksp_bnorm;
   long drop_count;    /* DROP cross-term modes met on unmasked faces */
   int nthreads;
+  int formbcs_passes; /* FormBCS passes: 3 as the reference (bcs.c:4483); other counts are for tests */
   /* solver options (reference: control.dat~:69-87, poisson.c:1828-1983) */
   int restart, max_it; double rtol, atol;
   int smooth_it;      /* Richardson iterations per level smoother */
@@ -115,6 +116,7 @@ void o_set_ib_nodes(octx *c, int n, const int *idx12, const double *val12, int m
 void o_ibm_interpolation(octx *c);
 void o_outflowflux(octx *c);
 void o_formbcs(octx *c);
+void o_set_formbcs_passes(octx *c, int n);  /* test hook: FormBCS with n passes instead of 3 */
 void o_formfunction1(octx *c, ofield *rhs);
 void o_rungekutta(octx *c);
 void o_calcrhs(octx *c);

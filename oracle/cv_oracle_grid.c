@@ -134,7 +134,7 @@ octx *o_create(int IM, int JM, int KM, const int bctype[6], int nlev,
   c->nlev = nlev;
   c->ren = ren; c->dt = dt; c->st = 1.0;          /* init.c:52-62 */
   c->restart = 30; c->max_it = 500; c->rtol = 1e-10; c->atol = 0.0;
-  c->smooth_it = 2; c->nthreads = 1;
+  c->smooth_it = 2; c->nthreads = 1; c->formbcs_passes = 3;
   int im = IM, jm = JM, km = KM;
   for (int l = nlev - 1; l >= 0; l--) {
     olevel *L = &c->lev[l];

@@ -492,7 +492,7 @@ static void outlet_kplus(octx *c) {
  * periodic planes -> edge averages -> periodic full planes. */
 void o_formbcs(octx *c) {
   olevel *L = &c->lev[c->nlev - 1];
-  for (int ttemp = 0; ttemp < 3; ttemp++) {
+  for (int ttemp = 0; ttemp < c->formbcs_passes; ttemp++) {
     farfield(c);
     slip_faces(c, 1);
     outlet_kplus(c);
@@ -505,6 +505,9 @@ void o_formbcs(octx *c) {
   }
   o_g2l(&c->p, L->per); o_g2l(&c->ucat, L->per); o_g2l(&c->ucont, L->per);
 }
+
+/* test hook: the number of FormBCS passes (the reference makes 3) */
+void o_set_formbcs_passes(octx *c, int n) { c->formbcs_passes = n; }
 
 /* GhostNodeVelocity (bcs.c:3645-4435) for bctype {1,3,4,5,6,7}: slip
  * faces take Ubcs from the current Ucat, the others keep the Ubcs the last

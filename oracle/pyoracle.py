@@ -60,6 +60,7 @@ def lib():
         L.o_set_momentum.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
         L.o_imp_stats.argtypes = [vp, dp]
         L.o_set_steps_done.argtypes = [vp, C.c_int]
+        L.o_set_formbcs_passes.argtypes = [vp, C.c_int]
         for nm in ("o_calcrhs", "o_imprk", "o_inflowflux", "o_outflowflux"):
             getattr(L, nm).argtypes = [vp]
         L.o_set_inlet.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
@@ -157,6 +158,10 @@ class Oracle:
         """scheme "rk" (RungeKutta) or "imprk" (ImpRK dual time stepping)."""
         lib().o_set_momentum(self.h, {"rk": 0, "imprk": 1}[scheme], int(imp_max_it), float(imp_atol),
                              float(imp_rtol), float(cfl))
+
+    def set_formbcs_passes(self, n):
+        """FormBCS with n passes instead of the reference's 3 (a test hook)."""
+        lib().o_set_formbcs_passes(self.h, int(n))
 
     def lib_set_steps(self, n):
         """Steps since the fluxes were loaded (0: BDF1 in CalcRHS)."""
