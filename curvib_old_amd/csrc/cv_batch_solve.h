@@ -124,14 +124,15 @@ __global__ void __launch_bounds__(TW * 4) kb_stencil(BK_ARGS, int d, int sx, int
                                      A.dinv, (T)0, C.lev[d].kc, bx, by, bz);
 }
 
-// x += omega T^-1 r in place (zg: x = omega T^-1 r)
-template <class T, int CS, bool HL = false>
+// x += omega T^-1 r in place (zg: x = omega T^-1 r); PX: x loaded with r
+// (launches of at most one block per CU)
+template <class T, int CS, bool HL = false, bool PX = false>
 __global__ void __launch_bounds__(1024) kb_yline(BK_ARGS, int d, int sr, T omega, int zg) {
   BK_HEAD
   const OpV<T> A = bop<T>(C, d);
   T *x = bvec<T>(C, d, BX, 0);
-  yline_reg_tile<T, CS, HL>(A.mx, A.my, A.mz, bvec<T>(C, d, sr, 0), A.ylo, A.yip, A.ycp, x, x,
-                            HL ? (T)(omega * A.ysc) : omega, zg, lb, A.yh);
+  yline_reg_tile<T, CS, HL, PX>(A.mx, A.my, A.mz, bvec<T>(C, d, sr, 0), A.ylo, A.yip, A.ycp, x, x,
+                                HL ? (T)(omega * A.ysc) : omega, zg, lb, A.yh);
 }
 
 // r of depth d -> b of depth d+1 (block table of depth d+1)
@@ -474,9 +475,16 @@ static void b_line(cv_batch *B, int d, unsigned long long pm, int sr, T om, bool
     const BCtl *ctl = B->dctl;
     if constexpr (sizeof(T) == 4) {
       if (B->opt.precision == CV_MIXED16 && g < 2) {      // FP16 line factors (8 or 16 rows per thread)
-        if (g == 0)
+        const bool px = t.len <= CV_PX_BLOCKS;
+        if (g == 0 && px)
+          hipLaunchKernelGGL((kb_yline<T, 8, true, true>), dim3((unsigned)t.len), blk, 0, B->stream, cs, tb, m, ctl, d,
+                             sr, om, zg ? 1 : 0);
+        else if (g == 0)
           hipLaunchKernelGGL((kb_yline<T, 8, true>), dim3((unsigned)t.len), blk, 0, B->stream, cs, tb, m, ctl, d, sr, om,
                              zg ? 1 : 0);
+        else if (px)
+          hipLaunchKernelGGL((kb_yline<T, 16, true, true>), dim3((unsigned)t.len), blk, 0, B->stream, cs, tb, m, ctl, d,
+                             sr, om, zg ? 1 : 0);
         else
           hipLaunchKernelGGL((kb_yline<T, 16, true>), dim3((unsigned)t.len), blk, 0, B->stream, cs, tb, m, ctl, d, sr, om,
                              zg ? 1 : 0);

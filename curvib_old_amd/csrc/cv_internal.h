@@ -200,6 +200,10 @@ __device__ inline int cv_wrap(int i, int m, int per) {
   return per ? (i == m - 1 ? 1 : (i == 0 ? m - 2 : i)) : i;
 }
 
+// line-solve launches of at most this many blocks (one per CU of the
+// MI355X's 256) load x with the residual (yline_reg_tile's PX)
+#define CV_PX_BLOCKS 256
+
 // XCD-aware tile order.  Work-groups are handed to the 8 XCDs of the chip
 // round-robin by linear id; this maps the blocks one XCD receives onto a
 // contiguous run of tiles (x fastest), so tiles that share halo lines share

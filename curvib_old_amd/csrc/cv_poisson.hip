@@ -626,7 +626,11 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
 // HL: the factors from the FP16 pack yh (one 8-byte load per row, cp kept
 // in registers for the backward pass); the solution then carries 2^e and
 // the caller passes omega * 2^-e
-template <class T, int CS, bool HL = false>
+// PX: x is loaded with r at the start (each thread reads and writes only
+// its own rows, so xi may be xo), which takes one dependent memory round
+// trip off a launch of few blocks at the cost of CS registers (launches of
+// more blocks than CUs keep PX off)
+template <class T, int CS, bool HL = false, bool PX = false>
 __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
                                       const T *__restrict__ ip, const T *__restrict__ cp, const T *xi, T *xo,
                                       T omega, int zero_guess, int lg, const uint2 *__restrict__ yh = nullptr) {
@@ -639,7 +643,12 @@ __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *
   const long base = (long)i + (long)mx * my * k;
   const int j0 = 1 + tc * CS;
   const int nrow = act ? max(0, min(CS, my - 1 - j0)) : 0;
-  T av[CS], yv[CS], cv[HL ? CS : 1];
+  T av[CS], yv[CS], cv[HL ? CS : 1], xv[PX ? CS : 1];
+  if constexpr (PX) {
+#pragma unroll
+    for (int q = 0; q < CS; q++)
+      xv[q] = q < nrow && !zero_guess ? xi[base + sj * (j0 + q)] : (T)0;
+  }
   // forward composite of the chunk
   T Aa = 1, Bb = 0;
 #pragma unroll
@@ -688,21 +697,22 @@ __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *
     if (q < nrow) {
       const long t = base + sj * (j0 + q);
       e = yv[q] + av[q] * e;
-      xo[t] = (zero_guess ? (T)0 : xi[t]) + omega * e;
+      if constexpr (PX) xo[t] = xv[q] + omega * e;
+      else xo[t] = (zero_guess ? (T)0 : xi[t]) + omega * e;
     }
   }
   if (tc == 0) xo[base] = zero_guess ? (T)0 : xi[base];
   if (tc == nc - 1) xo[base + sj * (my - 1)] = zero_guess ? (T)0 : xi[base + sj * (my - 1)];
 }
 
-template <class T, int CS, bool HL = false>
+template <class T, int CS, bool HL = false, bool PX = false>
 __global__ void __launch_bounds__(1024) k_yline_reg(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
                                                    const T *__restrict__ ip, const T *__restrict__ cp,
                                                    const T *xi, T *xo, T omega, int zero_guess,
                                                    const uint2 *__restrict__ yh) {
   int lg, by, bz;
   cv_xcd_tile(lg, by, bz);
-  yline_reg_tile<T, CS, HL>(mx, my, mz, r, ya, ip, cp, xi, xo, omega, zero_guess, lg, yh);
+  yline_reg_tile<T, CS, HL, PX>(mx, my, mz, r, ya, ip, cp, xi, xo, omega, zero_guess, lg, yh);
 }
 
 // ------------------------------------------------------------------
@@ -1420,9 +1430,17 @@ static void line_solve(cv_ctx *c, int l, const T *r, const T *xi, T *xo, T *tmp,
     if constexpr (sizeof(T) == 4) {
       if (hl) {
         const T omh = om * A.ysc;
-        if (cs == 8)
+        // at most one block per CU: x loaded up front
+        const bool px = g.x <= CV_PX_BLOCKS;
+        if (cs == 8 && px)
+          hipLaunchKernelGGL((k_yline_reg<T, 8, true, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo,
+                             A.yip, A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
+        else if (cs == 8)
           hipLaunchKernelGGL((k_yline_reg<T, 8, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
                              A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
+        else if (px)
+          hipLaunchKernelGGL((k_yline_reg<T, 16, true, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo,
+                             A.yip, A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
         else
           hipLaunchKernelGGL((k_yline_reg<T, 16, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
                              A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
