@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(LEG_NT) kb_leg(const BCase *__restrict__ cs, u
     const PPhase P = lg.ph[q];
     const BLev &Lv = C.lev[d];
     const OpV<T> A = bop<T>(C, d);
-    T *x = bvec<T>(C, d, BX, 0);
+    T *x = bvec<T>(C, d, BX, ctl->jj);
     const long c0 = (long)it * LEG_CHUNK, c1 = c0 + LEG_CHUNK;
     switch (P.kind) {
       case PK_STEN: {
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(LEG_NT) kb_leg(const BCase *__restrict__ cs, u
       }
       case PK_PROLONG: {
         const BLev &K = C.lev[d + 1];
-        prolong_body<T>(Lv.I, K.I, K.isc, K.jsc, K.ksc, K.nv, Lv.fl, (const T *)bvec<T>(C, d + 1, BX, 0), x, 1,
+        prolong_body<T>(Lv.I, K.I, K.isc, K.jsc, K.ksc, K.nv, Lv.fl, (const T *)bvec<T>(C, d + 1, BX, ctl->jj), x, 1,
                         c1 < Lv.n ? c1 : Lv.n, c0 + tid, LEG_NT);
         break;
       }

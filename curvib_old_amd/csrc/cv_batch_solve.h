@@ -71,8 +71,11 @@ template <class T>
 __device__ inline T *bvec(const BCase &C, int d, int s, int jj);
 template <>
 __device__ inline float *bvec<float>(const BCase &C, int d, int s, int jj) {
-  // BZJ: the FP32 preconditioned vector Z_jj of the mixed solve
-  return s == BX ? C.lev[d].xf : s == BB ? C.lev[d].bf : s == BZJ ? (float *)C.Z + (long)jj * C.n : C.lev[d].rf;
+  // BZJ: the FP32 preconditioned vector Z_jj of the mixed solve, which is
+  // also the x of the finest V-cycle level (the V-cycle writes M^-1 V_jj in
+  // place; only the mixed solve runs an FP32 V-cycle)
+  float *const zj = (float *)C.Z + (long)jj * C.n;
+  return s == BX ? (d == 0 ? zj : C.lev[d].xf) : s == BB ? C.lev[d].bf : s == BZJ ? zj : C.lev[d].rf;
 }
 template <>
 __device__ inline double *bvec<double>(const BCase &C, int d, int s, int jj) {
@@ -130,7 +133,7 @@ template <class T, int CS, bool HL = false, bool PX = false>
 __global__ void __launch_bounds__(1024) kb_yline(BK_ARGS, int d, int sr, T omega, int zg) {
   BK_HEAD
   const OpV<T> A = bop<T>(C, d);
-  T *x = bvec<T>(C, d, BX, 0);
+  T *x = bvec<T>(C, d, BX, ctl->jj);
   yline_reg_tile<T, CS, HL, PX>(A.mx, A.my, A.mz, bvec<T>(C, d, sr, 0), A.ylo, A.yip, A.ycp, x, x,
                                 HL ? (T)(omega * A.ysc) : omega, zg, lb, A.yh);
 }
@@ -149,8 +152,8 @@ template <class T>
 __global__ void kb_prolong(BK_ARGS, int d) {
   BK_HEAD
   const BLev &F = C.lev[d], &K = C.lev[d + 1];
-  prolong_body<T>(F.I, K.I, K.isc, K.jsc, K.ksc, K.nv, F.fl, (const T *)bvec<T>(C, d + 1, BX, 0), bvec<T>(C, d, BX, 0),
-                  1, F.n, (long)lb * BS + threadIdx.x, (long)F.nvb * BS);
+  prolong_body<T>(F.I, K.I, K.isc, K.jsc, K.ksc, K.nv, F.fl, (const T *)bvec<T>(C, d + 1, BX, ctl->jj),
+                  bvec<T>(C, d, BX, ctl->jj), 1, F.n, (long)lb * BS + threadIdx.x, (long)F.nvb * BS);
 }
 
 template <class T>
@@ -185,6 +188,7 @@ __global__ void kb_zero(BK_ARGS, int d, int sx) {
 }
 
 // dir 0: V_jj -> b of the finest V-cycle level; dir 1: x of that level -> Z_jj
+// (the FP64 solve; the mixed solve's scale writes that b, its V-cycle Z_jj)
 template <class T>
 __global__ void kb_conv(BK_ARGS, int dir) {
   BK_HEAD
@@ -209,11 +213,25 @@ __global__ void kb_copy(BK_ARGS, int sdst, int ssrc) {
   for (long t = (long)lb * BS + threadIdx.x; t < C.n; t += (long)C.lev[0].nvb * BS) o[t] = s[t];
 }
 
-__global__ void kb_scale(BK_ARGS, int sdst, int ssrc, int bslot) {
+// dst = src * bsc[bslot]; f32: also the FP32 copy the mixed V-cycle takes
+// as its right-hand side (the b of the finest level)
+__global__ void kb_scale(BK_ARGS, int sdst, int ssrc, int bslot, int f32) {
   BK_HEAD
   const int jj = ctl->jj;
-  scale_to_body(bvec<double>(C, 0, sdst, jj), bvec<double>(C, 0, ssrc, jj), C.bsc[bslot], C.n,
-                (long)lb * BS + threadIdx.x, (long)C.lev[0].nvb * BS);
+  double *const dst = bvec<double>(C, 0, sdst, jj);
+  const double *const src = bvec<double>(C, 0, ssrc, jj);
+  const double a = C.bsc[bslot];
+  const long t0 = (long)lb * BS + threadIdx.x, st = (long)C.lev[0].nvb * BS;
+  if (!f32) {
+    scale_to_body(dst, src, a, C.n, t0, st);
+    return;
+  }
+  float *const bf = C.lev[0].bf;
+  for (long t = t0; t < C.n; t += st) {
+    const double v = src[t] * a;
+    dst[t] = v;
+    bf[t] = (float)v;
+  }
 }
 
 template <class TZ>
@@ -570,12 +588,10 @@ static void b_precond_apply(cv_batch *B) {
   const unsigned long long all = B->all;
   const BTab &v0 = B->vc[0];
   if (B->opt.precision != CV_FP64) {
-    // Z_jj in FP32, as in the single-case solve
-    hipLaunchKernelGGL(kb_conv<float>, dim3((unsigned)v0.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
-                       (const uint32_t *)BTAB(v0), all, (const BCtl *)B->dctl, 0);
+    // Z_jj in FP32, as in the single-case solve: the scale that wrote V_jj
+    // wrote its FP32 copy (the V-cycle's b), and the V-cycle's finest x is Z_jj
+    (void)v0;
     b_vcycle<float>(B, 0);
-    hipLaunchKernelGGL(kb_conv<float>, dim3((unsigned)v0.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
-                       (const uint32_t *)BTAB(v0), all, (const BCtl *)B->dctl, 1);
     b_nullspace<float>(B, 0, all, BZJ);
     b_stencil<double, 0, float>(B, 0, all, BZJ, BZJ, BW);
   } else {
@@ -607,7 +623,8 @@ static void b_scale(cv_batch *B, unsigned long long pm, int dst, int src, int sl
   const BTab &t = B->vc[0];
   if (!BLAUNCH_OK(t, pm)) return;
   hipLaunchKernelGGL(kb_scale, dim3((unsigned)t.len), dim3(BS), 0, B->stream, (const BCase *)B->dcases,
-                     (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, dst, src, slot);
+                     (const uint32_t *)BTAB(t), pm, (const BCtl *)B->dctl, dst, src, slot,
+                     B->opt.precision != CV_FP64 ? 1 : 0);
 }
 // one classical Gram-Schmidt pass: h and ||w||^2 to dsc[0..nv], ||w - V h||^2 to dsc[36]
 static void b_gs(cv_batch *B, unsigned long long pm, int nv) {

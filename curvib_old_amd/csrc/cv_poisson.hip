@@ -995,9 +995,6 @@ __global__ void k_sumsq(const double *__restrict__ x, long n, double *part) { su
 __global__ void k_d2f(float *dst, const double *src, long n) {
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = (float)src[t];
 }
-__global__ void k_f2d(double *dst, const float *src, long n) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = (double)src[t];
-}
 template <class T>
 __global__ void k_zero(T *dst, long n) {
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = (T)0;
