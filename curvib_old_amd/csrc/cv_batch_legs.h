@@ -119,9 +119,9 @@ __global__ void __launch_bounds__(LEG_NT) kb_leg(const BCase *__restrict__ cs, u
         const T om = HG ? (T)(omega * A.ysc) : omega;
         const T *r = bvec<T>(C, d, P.src, 0);
         if (Lv.cs == 8)
-          yline_reg_tile<T, 8, HG>(A.mx, A.my, A.mz, r, A.ylo, A.yip, A.ycp, x, x, om, P.zg, it, A.yh);
+          yline_reg_tile<T, 8, HG>(A.mx, A.my, A.mz, r, A.ylo, A.yip, A.ycp, x, x, om, P.zg, it, A.yh, A.yc);
         else
-          yline_reg_tile<T, 16, HG>(A.mx, A.my, A.mz, r, A.ylo, A.yip, A.ycp, x, x, om, P.zg, it, A.yh);
+          yline_reg_tile<T, 16, HG>(A.mx, A.my, A.mz, r, A.ylo, A.yip, A.ycp, x, x, om, P.zg, it, A.yh, A.yc);
         break;
       }
       case PK_RESTRICT: {

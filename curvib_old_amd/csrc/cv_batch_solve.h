@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(1024) kb_yline(BK_ARGS, int d, int sr, T omega
   const OpV<T> A = bop<T>(C, d);
   T *x = bvec<T>(C, d, BX, ctl->jj);
   yline_reg_tile<T, CS, HL, PX>(A.mx, A.my, A.mz, bvec<T>(C, d, sr, 0), A.ylo, A.yip, A.ycp, x, x,
-                                HL ? (T)(omega * A.ysc) : omega, zg, lb, A.yh);
+                                HL ? (T)(omega * A.ysc) : omega, zg, lb, A.yh, A.yc);
 }
 
 // r of depth d -> b of depth d+1 (block table of depth d+1)

@@ -111,9 +111,11 @@ struct OpV {
   // so the j-face below a tile row needs hB alone
   const uint4 *hA;
   const uint2 *hB;
-  // CV_MIXED16 y-line factors per cell: {a in FP16 | ip * 2^e in FP16,
-  // cp in FP16 | unused}; ysc = 2^-e undoes the scaling of the solve
-  const uint2 *yh;
+  // CV_MIXED16 y-line factors per cell: yh {a in FP16 | ip * 2^e in FP16}
+  // and yc {cp in FP16} (6 bytes a row); ysc = 2^-e undoes the scaling of
+  // the solve
+  const uint32_t *yh;
+  const uint16_t *yc;
   float ysc;
   const T *dinv;
   const uint32_t *fl;
@@ -623,9 +625,9 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
 // planes, so no lane idles where mx is not a multiple of 64); the chunks
 // beyond the line (blockDim.y larger than the line needs) are identity
 // maps, so the result does not depend on blockDim.y
-// HL: the factors from the FP16 pack yh (one 8-byte load per row, cp kept
-// in registers for the backward pass); the solution then carries 2^e and
-// the caller passes omega * 2^-e
+// HL: the factors from the FP16 packs yh and yc (a 4-byte and a 2-byte
+// load per row, cp kept in registers for the backward pass); the solution
+// then carries 2^e and the caller passes omega * 2^-e
 // PX: x is loaded with r at the start (each thread reads and writes only
 // its own rows, so xi may be xo), which takes one dependent memory round
 // trip off a launch of few blocks at the cost of CS registers (launches of
@@ -633,7 +635,8 @@ __global__ void __launch_bounds__(64 * YL_NC) k_yline_solve(int mx, int my, int 
 template <class T, int CS, bool HL = false, bool PX = false>
 __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
                                       const T *__restrict__ ip, const T *__restrict__ cp, const T *xi, T *xo,
-                                      T omega, int zero_guess, int lg, const uint2 *__restrict__ yh = nullptr) {
+                                      T omega, int zero_guess, int lg, const uint32_t *__restrict__ yh = nullptr,
+                                      const uint16_t *__restrict__ yc = nullptr) {
   __shared__ T SA[16][64], SB[16][64];
   const int ti = threadIdx.x, tc = threadIdx.y, nc = blockDim.y;
   const long q0 = (long)lg * 64 + ti;
@@ -656,10 +659,10 @@ __device__ __forceinline__ void yline_reg_tile(int mx, int my, int mz, const T *
     if (q < nrow) {
       const long t = base + sj * (j0 + q);
       if constexpr (HL) {
-        const uint2 w = yh[t];
-        av[q] = half_lo(w.x);
-        yv[q] = r[t] * half_hi(w.x);
-        cv[q] = half_lo(w.y);
+        const uint32_t w = yh[t];
+        av[q] = half_lo(w);
+        yv[q] = r[t] * half_hi(w);
+        cv[q] = half_lo((uint32_t)yc[t]);
       } else {
         av[q] = ya[t];
         yv[q] = r[t] * ip[t];
@@ -709,10 +712,10 @@ template <class T, int CS, bool HL = false, bool PX = false>
 __global__ void __launch_bounds__(1024) k_yline_reg(int mx, int my, int mz, const T *r, const T *__restrict__ ya,
                                                    const T *__restrict__ ip, const T *__restrict__ cp,
                                                    const T *xi, T *xo, T omega, int zero_guess,
-                                                   const uint2 *__restrict__ yh) {
+                                                   const uint32_t *__restrict__ yh, const uint16_t *__restrict__ yc) {
   int lg, by, bz;
   cv_xcd_tile(lg, by, bz);
-  yline_reg_tile<T, CS, HL, PX>(mx, my, mz, r, ya, ip, cp, xi, xo, omega, zero_guess, lg, yh);
+  yline_reg_tile<T, CS, HL, PX>(mx, my, mz, r, ya, ip, cp, xi, xo, omega, zero_guess, lg, yh, yc);
 }
 
 // ------------------------------------------------------------------
@@ -1019,7 +1022,7 @@ OpV<double> opview<double>(cv_ctx *c, int l) {
   CvLevel &L = c->lev[l];
   OpV<double> A;
   for (int q = 0; q < 9; q++) A.g[q] = L.g[q];
-  A.hA = nullptr; A.hB = nullptr; A.yh = nullptr; A.ysc = 1.0f;
+  A.hA = nullptr; A.hB = nullptr; A.yh = nullptr; A.yc = nullptr; A.ysc = 1.0f;
   A.aj = L.aj; A.dinv = L.dinv; A.fl = L.flags;
   A.ylo = L.ylo; A.yip = L.yip; A.ycp = L.ycp;
   A.mx = L.d.mx; A.my = L.d.my; A.mz = L.d.mz; A.xper = L.d.xper; A.zper = L.d.zper;
@@ -1032,7 +1035,8 @@ OpV<float> opview<float>(cv_ctx *c, int l) {
   for (int q = 0; q < 9; q++) A.g[q] = L.gf[q];
   A.hA = L.gh ? (const uint4 *)L.gh : nullptr;
   A.hB = L.gh ? (const uint2 *)((const char *)L.gh + 16 * L.d.n) : nullptr;
-  A.yh = L.gh ? (const uint2 *)((const char *)L.gh + 24 * L.d.n) : nullptr;
+  A.yh = L.gh ? (const uint32_t *)((const char *)L.gh + 24 * L.d.n) : nullptr;
+  A.yc = L.gh ? (const uint16_t *)((const char *)L.gh + 28 * L.d.n) : nullptr;
   A.ysc = L.gh ? (float)ldexp(1.0, -L.yexp) : 1.0f;
   A.aj = L.ajf; A.dinv = L.dinvf; A.fl = L.flags;
   A.ylo = L.ylof; A.yip = L.yipf; A.ycp = L.ycpf;
@@ -1245,10 +1249,13 @@ __global__ static void k_absmax9(G9 g, long n, unsigned long long *out) {
 __device__ inline uint32_t half_bits(double v) {
   return (uint32_t)__builtin_bit_cast(unsigned short, (_Float16)(float)v);
 }
-// the y-line factor word of OpV::yh: (a, ip * s) and (cp, 0)
-__global__ static void k_pack_line(const double *ya, const double *ip, const double *cp, double s, long n, uint2 *yh) {
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
-    yh[t] = make_uint2(half_bits(ya[t]) | (half_bits(ip[t] * s) << 16), half_bits(cp[t]));
+// the y-line factors of OpV::yh (a, ip * s) and OpV::yc (cp)
+__global__ static void k_pack_line(const double *ya, const double *ip, const double *cp, double s, long n, uint32_t *yh,
+                                   uint16_t *yc) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    yh[t] = half_bits(ya[t]) | (half_bits(ip[t] * s) << 16);
+    yc[t] = (uint16_t)half_bits(cp[t]);
+  }
 }
 __global__ static void k_absmax1(const double *a, long n, unsigned long long *out) {
   double m = 0.0;
@@ -1314,7 +1321,8 @@ static int half_ops(cv_ctx *c) {
     if (ipmax > 0) frexp(ipmax, &ey);
     L.yexp = ipmax > 0 ? 14 - ey : 0;
     hipLaunchKernelGGL(k_pack_line, dim3(nblk(n)), dim3(BS), 0, c->stream, (const double *)L.ylo, (const double *)L.yip,
-                       (const double *)L.ycp, ldexp(1.0, L.yexp), n, (uint2 *)((char *)L.gh + 24 * n));
+                       (const double *)L.ycp, ldexp(1.0, L.yexp), n, (uint32_t *)((char *)L.gh + 24 * n),
+                       (uint16_t *)((char *)L.gh + 28 * n));
   }
   CV_CHECK(hipGetLastError());
   return CV_OK;
@@ -1416,9 +1424,9 @@ static void line_solve(cv_ctx *c, int l, const T *r, const T *xi, T *xo, T *tmp,
   const int rows = L.d.my - 2;
   const dim3 g((unsigned)(((long)L.d.mx * L.d.mz + 63) / 64));
   const dim3 gs((unsigned)((L.d.mx + 63) / 64), (unsigned)L.d.mz);   // k_yline_solve
-  // bytes per point: r, the factors (FP16 pack: 8 bytes, else a, ip, cp), x in and out
+  // bytes per point: r, the factors (FP16 packs: 6 bytes, else a, ip, cp), x in and out
   const bool hl = sizeof(T) == 4 && A.yh && rows <= 16 * 16;
-  CvProfScope ps(c, CV_PROF_LINE, l, sizeof(T) == 4 ? 1 : 0, (double)L.d.n * (hl ? 20.0 : 6.0 * sizeof(T)));
+  CvProfScope ps(c, CV_PROF_LINE, l, sizeof(T) == 4 ? 1 : 0, (double)L.d.n * (hl ? 18.0 : 6.0 * sizeof(T)));
   // FP64 keeps at most 16 rows per thread (32 would spill at 1024 threads)
   const int csmax = sizeof(T) == 8 ? 16 : 32;
   if (rows <= 16 * csmax) {
@@ -1431,28 +1439,28 @@ static void line_solve(cv_ctx *c, int l, const T *r, const T *xi, T *xo, T *tmp,
         const bool px = g.x <= CV_PX_BLOCKS;
         if (cs == 8 && px)
           hipLaunchKernelGGL((k_yline_reg<T, 8, true, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo,
-                             A.yip, A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
+                             A.yip, A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh, A.yc);
         else if (cs == 8)
           hipLaunchKernelGGL((k_yline_reg<T, 8, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
-                             A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
+                             A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh, A.yc);
         else if (px)
           hipLaunchKernelGGL((k_yline_reg<T, 16, true, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo,
-                             A.yip, A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
+                             A.yip, A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh, A.yc);
         else
           hipLaunchKernelGGL((k_yline_reg<T, 16, true>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
-                             A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh);
+                             A.ycp, xi, xo, omh, zg ? 1 : 0, A.yh, A.yc);
         return;
       }
     }
     if (cs == 8)
       hipLaunchKernelGGL((k_yline_reg<T, 8>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip, A.ycp, xi,
-                         xo, om, zg ? 1 : 0, (const uint2 *)nullptr);
+                         xo, om, zg ? 1 : 0, (const uint32_t *)nullptr, (const uint16_t *)nullptr);
     else if (cs == 16)
       hipLaunchKernelGGL((k_yline_reg<T, 16>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip, A.ycp, xi,
-                         xo, om, zg ? 1 : 0, (const uint2 *)nullptr);
+                         xo, om, zg ? 1 : 0, (const uint32_t *)nullptr, (const uint16_t *)nullptr);
     else
       hipLaunchKernelGGL((k_yline_reg<T, sizeof(T) == 8 ? 16 : 32>), g, b, 0, c->stream, L.d.mx, L.d.my, L.d.mz, r,
-                         A.ylo, A.yip, A.ycp, xi, xo, om, zg ? 1 : 0, (const uint2 *)nullptr);
+                         A.ylo, A.yip, A.ycp, xi, xo, om, zg ? 1 : 0, (const uint32_t *)nullptr, (const uint16_t *)nullptr);
   } else {
     hipLaunchKernelGGL(k_yline_solve<T>, gs, dim3(64, YL_NC), 0, c->stream, L.d.mx, L.d.my, L.d.mz, r, A.ylo, A.yip,
                        A.ycp, xi, xo, tmp, om, zg ? 1 : 0);
