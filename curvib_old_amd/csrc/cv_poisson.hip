@@ -767,16 +767,17 @@ __device__ __forceinline__ void restrict_body(Idx C, Idx F, int isc, int jsc, in
     C.dec(t, i, j, k);
     T f = 0;
     if (flc[t] & 1u) {
-      int ia = isc ? 0 : 1, ja = jsc ? 0 : 1, ka = ksc ? 0 : 1;
-      int ih = isc ? i : 2 * i, jh = jsc ? j : 2 * j, kh = ksc ? k : 2 * k;
-      auto RT = [&](int a, int b, int cc) -> T {
-        const long q = F.at(a, b, cc);
+      // 32-bit indices (storage sizes stay below 2^31): the fine point and
+      // its lower neighbours along the coarsened axes at fixed strides
+      const int ia = isc ? 0 : 1, ja = jsc ? 0 : F.mx, ka = ksc ? 0 : F.mx * F.my;
+      const int ih = isc ? i : 2 * i, jh = jsc ? j : 2 * j, kh = ksc ? k : 2 * k;
+      const int q0 = ih + F.mx * (jh + F.my * kh);
+      auto RT = [&](int q) -> T {
         if constexpr (sizeof(T) == 4) return xf[q] * wff[q];
         else return xf[q] * (T)fmax(0., 1 - nvf[q]);
       };
-      f = (T)0.125 * (RT(ih, jh, kh) + RT(ih - ia, jh, kh) + RT(ih, jh - ja, kh) + RT(ih, jh, kh - ka) +
-                      RT(ih - ia, jh - ja, kh) + RT(ih, jh - ja, kh - ka) + RT(ih - ia, jh, kh - ka) +
-                      RT(ih - ia, jh - ja, kh - ka));
+      f = (T)0.125 * (RT(q0) + RT(q0 - ia) + RT(q0 - ja) + RT(q0 - ka) + RT(q0 - ia - ja) + RT(q0 - ja - ka) +
+                      RT(q0 - ia - ka) + RT(q0 - ia - ja - ka));
     }
     xc[t] = f;
   }
@@ -808,16 +809,16 @@ __device__ __forceinline__ void prolong_body(Idx F, Idx C, int isc, int jsc, int
       if (isc) { ic = i; ia = 0; } else { ic = (i + 1) / 2; ia = (i - 2 * ic) == 0 ? 1 : -1; if (i == 1 || i == F.mx - 2) ia = 0; }
       if (jsc) { jc = j; ja = 0; } else { jc = (j + 1) / 2; ja = (j - 2 * jc) == 0 ? 1 : -1; if (j == 1 || j == F.my - 2) ja = 0; }
       if (ksc) { kc = k; ka = 0; } else { kc = (k + 1) / 2; ka = (k - 2 * kc) == 0 ? 1 : -1; if (k == 1 || k == F.mz - 2) ka = 0; }
-      if (ka == -1 && nvc[C.at(ic, jc, kc - 1)] > 0.1) ka = 0;
-      else if (ka == 1 && nvc[C.at(ic, jc, kc + 1)] > 0.1) ka = 0;
-      if (ja == -1 && nvc[C.at(ic, jc - 1, kc)] > 0.1) ja = 0;
-      else if (ja == 1 && nvc[C.at(ic, jc + 1, kc)] > 0.1) ja = 0;
-      if (ia == -1 && nvc[C.at(ic - 1, jc, kc)] > 0.1) ia = 0;
-      else if (ia == 1 && nvc[C.at(ic + 1, jc, kc)] > 0.1) ia = 0;
-      f = (xc[C.at(ic, jc, kc)] * (T)9 + xc[C.at(ic, jc + ja, kc)] * (T)3 + xc[C.at(ic + ia, jc, kc)] * (T)3 +
-           xc[C.at(ic + ia, jc + ja, kc)]) * (T)(3. / 64.) +
-          (xc[C.at(ic, jc, kc + ka)] * (T)9 + xc[C.at(ic, jc + ja, kc + ka)] * (T)3 +
-           xc[C.at(ic + ia, jc, kc + ka)] * (T)3 + xc[C.at(ic + ia, jc + ja, kc + ka)]) / (T)64.;
+      // 32-bit indices (storage sizes stay below 2^31): the coarse point
+      // and its neighbours at fixed strides
+      const int c0 = ic + C.mx * (jc + C.my * kc);
+      const int di = ia, dj = ja * C.mx, dk = ka * (C.mx * C.my);
+      if (ka != 0 && nvc[c0 + dk] > 0.1) ka = 0;
+      if (ja != 0 && nvc[c0 + dj] > 0.1) ja = 0;
+      if (ia != 0 && nvc[c0 + di] > 0.1) ia = 0;
+      const int oi = ia != 0 ? di : 0, oj = ja != 0 ? dj : 0, ok = ka != 0 ? dk : 0;
+      f = (xc[c0] * (T)9 + xc[c0 + oj] * (T)3 + xc[c0 + oi] * (T)3 + xc[c0 + oi + oj]) * (T)(3. / 64.) +
+          (xc[c0 + ok] * (T)9 + xc[c0 + oj + ok] * (T)3 + xc[c0 + oi + ok] * (T)3 + xc[c0 + oi + oj + ok]) / (T)64.;
     }
     if (!(flf[t] & 1u)) f = 0;
     if (add) xf[t] += f; else xf[t] = f;
