@@ -1947,26 +1947,21 @@ struct FaceMet {
   const double *g[3];   // (xi_d . xi_t) / J of one face family d, t = 0..2 (the operator's g[3d + t])
 };
 
-__device__ static inline double tang_p(const double *p, Idx I, int d, int t, int i, int j, int k, int mode) {
-  int o[3] = {0, 0, 0}; o[d] = 1;
-  int f[3] = {0, 0, 0}; f[t] = 1;
-#define PP(a, b, cc) p[I.at(i + (a), j + (b), k + (cc))]
-  if (mode == XM_LOWER)
-    return (PP(0, 0, 0) + PP(o[0], o[1], o[2]) - PP(-f[0], -f[1], -f[2]) - PP(o[0] - f[0], o[1] - f[1], o[2] - f[2])) * 0.5;
-  if (mode == XM_UPPER)
-    return (PP(f[0], f[1], f[2]) + PP(o[0] + f[0], o[1] + f[1], o[2] + f[2]) - PP(0, 0, 0) - PP(o[0], o[1], o[2])) * 0.5;
-  if (mode == XM_CENTRAL)
-    return (PP(f[0], f[1], f[2]) + PP(o[0] + f[0], o[1] + f[1], o[2] + f[2]) - PP(-f[0], -f[1], -f[2]) -
-            PP(o[0] - f[0], o[1] - f[1], o[2] - f[2])) * 0.25;
-#undef PP
+// the tangential difference of p along axis t on the face of axis d of the
+// cell at storage index c (strides of the axes: 1, mx, mx*my)
+__device__ static inline double tang_p(const double *p, Idx I, int d, int t, long c, int mode) {
+  const long so[3] = {1, (long)I.mx, (long)I.mx * I.my};
+  const long o = so[d], f = so[t];
+  if (mode == XM_LOWER) return (p[c] + p[c + o] - p[c - f] - p[c + o - f]) * 0.5;
+  if (mode == XM_UPPER) return (p[c + f] + p[c + o + f] - p[c] - p[c + o]) * 0.5;
+  if (mode == XM_CENTRAL) return (p[c + f] + p[c + o + f] - p[c - f] - p[c + o - f]) * 0.25;
   return 0.0;
 }
 
 __device__ static inline void project_face(Idx I, int d, int i, int j, int k, const double *phi, const double *nv,
                                            const uint32_t *fl, FaceMet M, double *u, double dt, double st) {
-  long t = I.at(i, j, k);
-  int o[3] = {0, 0, 0}; o[d] = 1;
-  long up = I.at(i + o[0], j + o[1], k + o[2]);
+  const long t = I.at(i, j, k);
+  const long up = t + (d == 0 ? 1 : d == 1 ? (long)I.mx : (long)I.mx * I.my);
   if (nv[t] + nv[up] != 0.0) return;
   uint32_t w = fl[t];
   double dp[3];
@@ -1974,7 +1969,7 @@ __device__ static inline void project_face(Idx I, int d, int i, int j, int k, co
   for (int tt = 0; tt < 3; tt++) {
     if (tt == d) { dp[tt] = phi[up] - phi[t]; continue; }
     int mode = (w >> (2 + 5 * d + 2 * q)) & 3u; q++;
-    dp[tt] = tang_p(phi, I, d, tt, i, j, k, mode);
+    dp[tt] = tang_p(phi, I, d, tt, t, mode);
   }
   // the coefficients of poisson.c:2420-2431 (dot products of the face
   // metrics times the face 1/J) are the operator's g of this face
