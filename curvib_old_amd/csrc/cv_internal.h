@@ -169,7 +169,7 @@ long cv_half_bytes(const cv_grid_desc *desc);
 long cv_geometry_scratch_bytes(const cv_grid_desc *desc);
 int cv_poisson_prepare(cv_ctx *c);
 int cv_contra2cart_dev(cv_ctx *c);
-int cv_ghost_update(cv_ctx *c, int formbcs);
+int cv_ghost_update(cv_ctx *c, int formbcs, bool cart = true);
 int cv_inflow_dev(cv_ctx *c);
 int cv_bc_update_dev(cv_ctx *c);   // InflowFlux + FormBCS
 int cv_ib_dev(cv_ctx *c);          // ibm_interpolation_advanced (nothing without IB nodes)

@@ -88,16 +88,11 @@ inline dim3 march_grid(int mx, int my, int mz) {
 // ------------------------------------------------------------------
 // Contra2Cart (rhs.c:25-144)
 // ------------------------------------------------------------------
-// one interior cell (i, j, k); IB cells (0.1 <= nvert <= 7) keep their value
-__device__ __forceinline__ void c2c_cell(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, int i, int j, int k) {
-  const long t = g.at(i, j, k);
-  const double nn = nv[t];
-  if (!(nn < 0.1 || nn > 7.)) return;
-  const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
-  double m00 = 0.5 * (csi.x[ti] + csi.x[t]), m01 = 0.5 * (csi.y[ti] + csi.y[t]), m02 = 0.5 * (csi.z[ti] + csi.z[t]);
-  double m10 = 0.5 * (eta.x[tj] + eta.x[t]), m11 = 0.5 * (eta.y[tj] + eta.y[t]), m12 = 0.5 * (eta.z[tj] + eta.z[t]);
-  double m20 = 0.5 * (zet.x[tk] + zet.x[t]), m21 = 0.5 * (zet.y[tk] + zet.y[t]), m22 = 0.5 * (zet.z[tk] + zet.z[t]);
-  double q0 = 0.5 * (uc.x[ti] + uc.x[t]), q1 = 0.5 * (uc.y[tj] + uc.y[t]), q2 = 0.5 * (uc.z[tk] + uc.z[t]);
+// Cramer's rule for Ucat of one cell from its averaged face vectors (rows
+// m0, m1, m2) and averaged fluxes q (rhs.c:25-144)
+__device__ __forceinline__ void c2c_solve(double m00, double m01, double m02, double m10, double m11, double m12,
+                                          double m20, double m21, double m22, double q0, double q1, double q2, V3 ucat,
+                                          long t) {
   double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
   double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
   double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
@@ -105,6 +100,18 @@ __device__ __forceinline__ void c2c_cell(G g, C3 uc, C3 csi, C3 eta, C3 zet, con
   ucat.x[t] = det0 / det;
   ucat.y[t] = det1 / det;
   ucat.z[t] = det2 / det;
+}
+
+// one interior cell (i, j, k); IB cells (0.1 <= nvert <= 7) keep their value
+__device__ __forceinline__ void c2c_cell(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, int i, int j, int k) {
+  const long t = g.at(i, j, k);
+  const double nn = nv[t];
+  if (!(nn < 0.1 || nn > 7.)) return;
+  const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
+  c2c_solve(0.5 * (csi.x[ti] + csi.x[t]), 0.5 * (csi.y[ti] + csi.y[t]), 0.5 * (csi.z[ti] + csi.z[t]),
+            0.5 * (eta.x[tj] + eta.x[t]), 0.5 * (eta.y[tj] + eta.y[t]), 0.5 * (eta.z[tj] + eta.z[t]),
+            0.5 * (zet.x[tk] + zet.x[t]), 0.5 * (zet.y[tk] + zet.y[t]), 0.5 * (zet.z[tk] + zet.z[t]),
+            0.5 * (uc.x[ti] + uc.x[t]), 0.5 * (uc.y[tj] + uc.y[t]), 0.5 * (uc.z[tk] + uc.z[t]), ucat, t);
 }
 // the k-march of the interior: the k-1 plane's zet and Ucont.z a cell reads
 // are the previous iteration's own values, carried in registers (a whole
@@ -123,22 +130,72 @@ __device__ __forceinline__ void contra2cart_body(G g, C3 uc, C3 csi, C3 eta, C3 
     const double nn = nv[t];
     if (nn < 0.1 || nn > 7.) {
       const long ti = t - 1, tj = t - g.sj;
-      double m00 = 0.5 * (csi.x[ti] + csi.x[t]), m01 = 0.5 * (csi.y[ti] + csi.y[t]), m02 = 0.5 * (csi.z[ti] + csi.z[t]);
-      double m10 = 0.5 * (eta.x[tj] + eta.x[t]), m11 = 0.5 * (eta.y[tj] + eta.y[t]), m12 = 0.5 * (eta.z[tj] + eta.z[t]);
-      double m20 = 0.5 * (zx + zx1), m21 = 0.5 * (zy + zy1), m22 = 0.5 * (zz + zz1);
-      double q0 = 0.5 * (uc.x[ti] + uc.x[t]), q1 = 0.5 * (uc.y[tj] + uc.y[t]), q2 = 0.5 * (wz + wz1);
-      double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
-      double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
-      double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
-      double det2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
-      ucat.x[t] = det0 / det;
-      ucat.y[t] = det1 / det;
-      ucat.z[t] = det2 / det;
+      c2c_solve(0.5 * (csi.x[ti] + csi.x[t]), 0.5 * (csi.y[ti] + csi.y[t]), 0.5 * (csi.z[ti] + csi.z[t]),
+                0.5 * (eta.x[tj] + eta.x[t]), 0.5 * (eta.y[tj] + eta.y[t]), 0.5 * (eta.z[tj] + eta.z[t]),
+                0.5 * (zx + zx1), 0.5 * (zy + zy1), 0.5 * (zz + zz1),
+                0.5 * (uc.x[ti] + uc.x[t]), 0.5 * (uc.y[tj] + uc.y[t]), 0.5 * (wz + wz1), ucat, t);
     }
     zx = zx1; zy = zy1; zz = zz1; wz = wz1;
   }
 }
+
+// One k-march for an RK stage on faces without an inlet, far-field, slip
+// or outlet (FormBCS then changes only ghosts): U = U_o + a Rhs on every
+// point, the wall-normal flux reset of wall_flux_zero_body (masks wx, wy,
+// wz), and Contra2Cart of the updated U on the interior.  The updated
+// fluxes of the i-1 and j-1 neighbours are formed again from U_o and Rhs
+// with the same expression, the k-1 plane's is carried, so the bits are
+// those of the three passes in turn.
+__device__ __forceinline__ void rk_c2c_body(G g, double a, C3 uo, C3 r, V3 u, int wx, int wy, int wz, C3 csi, C3 eta,
+                                            C3 zet, const double *nv, V3 ucat, int bx, int by, int bz) {
+  const int mx = g.mx, my = g.my, mz = g.mz;
+  const int i = bx * 64 + threadIdx.x, j = by * MARCH_TY + threadIdx.y;
+  if (!(i < mx && j < my)) return;
+  const int k0 = bz * MARCH_KC, k1 = min(k0 + MARCH_KC, mz);
+  auto ux = [&](long q, int ii) -> double {
+    const double v = a * r.x[q] + uo.x[q];
+    return (((wx & 1) && ii == 0) || ((wx & 2) && ii == mx - 2)) ? 0. : v;
+  };
+  auto uy = [&](long q, int jj) -> double {
+    const double v = a * r.y[q] + uo.y[q];
+    return (((wy & 1) && jj == 0) || ((wy & 2) && jj == my - 2)) ? 0. : v;
+  };
+  auto uz = [&](long q, int kk) -> double {
+    const double v = a * r.z[q] + uo.z[q];
+    return (((wz & 1) && kk == 0) || ((wz & 2) && kk == mz - 2)) ? 0. : v;
+  };
+  const bool in_ij = i > 0 && i < mx - 1 && j > 0 && j < my - 1;
+  long t = g.at(i, j, k0);
+  // the k-1 plane's zet and updated Ucont.z (carried from one plane to the next)
+  double zx = 0., zy = 0., zz = 0., pz = 0.;
+  if (in_ij && k0 >= 1) {
+    const long q = t - g.sk;
+    zx = zet.x[q]; zy = zet.y[q]; zz = zet.z[q];
+    pz = uz(q, k0 - 1);
+  }
+  for (int k = k0; k < k1; k++, t += g.sk) {
+    const double vx = ux(t, i), vy = uy(t, j), vz = uz(t, k);
+    u.x[t] = vx; u.y[t] = vy; u.z[t] = vz;
+    if (!in_ij) continue;
+    const double zx1 = zet.x[t], zy1 = zet.y[t], zz1 = zet.z[t];
+    if (k > 0 && k < mz - 1) {
+      const double nn = nv[t];
+      if (nn < 0.1 || nn > 7.) {
+        const long ti = t - 1, tj = t - g.sj;
+        c2c_solve(0.5 * (csi.x[ti] + csi.x[t]), 0.5 * (csi.y[ti] + csi.y[t]), 0.5 * (csi.z[ti] + csi.z[t]),
+                  0.5 * (eta.x[tj] + eta.x[t]), 0.5 * (eta.y[tj] + eta.y[t]), 0.5 * (eta.z[tj] + eta.z[t]),
+                  0.5 * (zx + zx1), 0.5 * (zy + zy1), 0.5 * (zz + zz1),
+                  0.5 * (ux(ti, i - 1) + vx), 0.5 * (uy(tj, j - 1) + vy), 0.5 * (pz + vz), ucat, t);
+      }
+    }
+    zx = zx1; zy = zy1; zz = zz1; pz = vz;
+  }
+}
 __global__ void k_contra2cart(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, long n) { contra2cart_body(g, uc, csi, eta, zet, nv, ucat, n, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ void __launch_bounds__(64 * MARCH_TY) k_rk_c2c(G g, double a, C3 uo, C3 r, V3 u, int wx, int wy, int wz, C3 csi,
+                                                         C3 eta, C3 zet, const double *nv, V3 ucat) {
+  rk_c2c_body(g, a, uo, r, u, wx, wy, wz, csi, eta, zet, nv, ucat, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 // Contra2Cart of the interior cells of one k plane
 __global__ void k_contra2cart_plane(G g, C3 uc, C3 csi, C3 eta, C3 zet, const double *nv, V3 ucat, int k) {
   const long m2 = (long)(g.mx - 2) * (g.my - 2);
@@ -1433,7 +1490,9 @@ static void slip_update(cv_ctx *c, int formbcs) {
 // averages those face ghosts, and each corner's last write edges of the same
 // pass), so one pass gives the three passes' bits.
 // formbcs=0: GhostNodeVelocity (slip Ubcs, then the ghosts)
-int cv_ghost_update(cv_ctx *c, int formbcs) {
+// cart=false: the interior Ucat of the first pass is already in place
+// (k_rk_c2c formed it with the RK stage update)
+int cv_ghost_update(cv_ctx *c, int formbcs, bool cart) {
   CvLevel &F = c->lev[c->nlev - 1];
   CvMetrics &M = c->m;
   G g = gview(c);
@@ -1446,7 +1505,7 @@ int cv_ghost_update(cv_ctx *c, int formbcs) {
     if (far) far_fix(c);
     if (c->slip) slip_update(c, formbcs);
     if (outl) outlet_fix(c, ps == passes - 1);
-    if (formbcs && (ps == 0 || far)) {
+    if (formbcs && ((ps == 0 && cart) || far)) {
       int rc = cv_contra2cart_dev(c);
       if (rc) return rc;
     } else if (outl) {
@@ -1621,6 +1680,13 @@ static int rc_kchunk() {
 
 // face-flux pass variant (CV_RC_KERNEL: 1 LDS-tiled, 0 global gathers),
 // read at every call so one process can run both
+// CV_RK_FUSED=0: the RK stage update, the wall flux reset and Contra2Cart
+// as three passes also where k_rk_c2c applies (read per call)
+static bool rk_fused() {
+  const char *e = getenv("CV_RK_FUSED");
+  return e ? atoi(e) != 0 : true;
+}
+
 static int rc_kernel() {
   const char *e = getenv("CV_RC_KERNEL");
   return e ? atoi(e) != 0 : 1;
@@ -1686,9 +1752,21 @@ int cv_rk_dev(cv_ctx *c) {
   const int wx = wall_mask(c->bctype[0], c->bctype[1]), wy = wall_mask(c->bctype[2], c->bctype[3]),
             wz = wall_mask(c->bctype[4], c->bctype[5]);
   const long planes = (long)g.my * g.mz + (long)g.mx * g.mz + (long)g.mx * g.my;
+  // faces without an inlet, far-field, slip or outlet: FormBCS changes
+  // only ghosts, so the stage update, the wall flux reset and Contra2Cart
+  // run as one k-march and FormBCS makes its ghost pass alone
+  const bool fused = !c->inlet && !c->far && !c->slip && !c->outlet && rk_fused();
   for (int s = 0; s < 4; s++) {
     if ((rc = cv_ff1_dev(c))) return rc;
     double a = alfa[s] * c->dt * c->st;
+    if (fused) {
+      CvMetrics &M = c->m;
+      hipLaunchKernelGGL(k_rk_c2c, march_grid(F.d.mx, F.d.my, F.d.mz), MARCH_BLOCK, 0, c->stream, g, a, c3(c->ucont_o),
+                         c3(c->rhs), v3(c->ucont), wx, wy, wz, c3(M.csi), c3(M.eta), c3(M.zet), (const double *)F.nvert,
+                         v3(c->ucat));
+      if ((rc = cv_ghost_update(c, 1, false))) return rc;
+      continue;
+    }
     hipLaunchKernelGGL(k_rk_update, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, a, c3(c->ucont_o), c3(c->rhs),
                        v3(c->ucont), F.d.n);
     hipLaunchKernelGGL(k_wall_flux_zero, dim3(nblk(planes)), dim3(BS), 0, c->stream, g, wx, wy, wz, v3(c->ucont));
@@ -2047,6 +2125,15 @@ __global__ void kbm_wall_flux_zero(MB_ARGS) {
   wall_flux_zero_body(C.g, C.wx, C.wy, C.wz, mv3(C.uc), MB_GSP);
 }
 
+// k_rk_c2c of every case (the batched cases have only wall and periodic faces)
+__global__ void __launch_bounds__(64 * MARCH_TY) kbm_rk_c2c(MB_ARGS, double alfa) {
+  MB_HEAD
+  const double a = alfa * C.dt * C.st;
+  const int bx = lb % C.mgx, by = (lb / C.mgx) % C.mgy, bz = lb / (C.mgx * C.mgy);
+  rk_c2c_body(C.g, a, mc3(C.uo), mc3(C.rhs), mv3(C.uc), C.wx, C.wy, C.wz, C.csi, C.eta, C.zet, C.nv, mv3(C.uca), bx, by,
+              bz);
+}
+
 }  // namespace
 
 // one case's IB reconstruction buffers, for the batched sweeps
@@ -2270,6 +2357,11 @@ int cv_mom_batch_rk(CvMomBatch *mb, unsigned long long rot) {
     MB_LAUNCH(kbm_rct_periodic, dim3(BS), mb->t_plane, mb->n_plane, mb->per);
     MB_LAUNCH(kbm_ff1_faces, dim3(64, MARCH_TY), mb->t_march, mb->n_march, mb->all);
     MB_LAUNCH(kbm_ff1_periodic, dim3(BS), mb->t_plane, mb->n_plane, mb->per);
+    if (rk_fused()) {   // the stage update, the wall flux reset and Contra2Cart in one k-march
+      MB_LAUNCH(kbm_rk_c2c, dim3(64, MARCH_TY), mb->t_march, mb->n_march, mb->all, alfa[s]);
+      if ((rc = cv_mom_batch_ghost(mb, 1, 0))) return rc;
+      continue;
+    }
     MB_LAUNCH(kbm_rk_update, dim3(BS), mb->t_vec, mb->n_vec, mb->all, alfa[s]);
     MB_LAUNCH(kbm_wall_flux_zero, dim3(BS), mb->t_plane, mb->n_plane, mb->all);
     if ((rc = cv_mom_batch_ghost(mb, 1, 1))) return rc;

@@ -318,3 +318,32 @@ def test_kernel_timing_counts_launches(kernel, per_step):
         assert u.prof_read()[0] == 0
     finally:
         u.close()
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_fused_rk_stage_keeps_the_bits(monkeypatch, walls):
+    """On faces without an inlet, far-field, slip or outlet, each RK stage
+    runs the update U = U_o + a Rhs, the wall-normal flux reset and
+    Contra2Cart as one k-march (implicitsolver.c:32-119, rhs.c:25-144;
+    CV_RK_FUSED, on by default).  Two steps with it off and on end with the
+    same bits in every field, on a periodic channel and on a box of walls
+    (wall-flux masks on all three axes), both around an immersed sphere."""
+    kw = dict(bctype=(1, 1, 1, 1, 1, 1)) if walls else {}
+    case = cases.ChannelCase(cells=(72, 20, 36), amp=0.1, beta=2.0, sphere=(0.5, 0.5, 0.5, 0.35), re=400.0,
+                             seed=17, **kw)
+    u0, _ = gpu_case(case, mglevels=2)
+    u1, _ = gpu_case(case, mglevels=2)
+    try:
+        for u in (u0, u1):
+            u.set_solver(rtol=1e-10, max_it=300, smoother="yline", pre_sweeps=3, post_sweeps=3, omega=0.9)
+        for s in range(2):
+            monkeypatch.setenv("CV_RK_FUSED", "0")
+            s0 = u0.Flow_Solver()
+            monkeypatch.setenv("CV_RK_FUSED", "1")
+            s1 = u1.Flow_Solver()
+            assert s0["ksp_its"] == s1["ksp_its"], (s, s0["ksp_its"], s1["ksp_its"])
+            for f in ("ucont", "ucat", "p", "phi"):
+                assert np.array_equal(u0.get(f), u1.get(f)), (s, f)
+    finally:
+        u0.close()
+        u1.close()
