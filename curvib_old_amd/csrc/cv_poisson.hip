@@ -50,6 +50,25 @@ struct Idx {
   __device__ long at(int i, int j, int k) const { return (long)i + (long)mx * ((long)j + (long)my * k); }
 };
 
+// (i, j, k) of a grid-stride walk t0, t0 + step, ...: one decode of t0 and
+// of step, then carries per advance instead of two divisions per point
+struct IdxWalk {
+  int i, j, k, di, dj, dk, mx, my;
+  __device__ IdxWalk(const Idx &I, long t0, long step) : mx(I.mx), my(I.my) {
+    I.dec(t0, i, j, k);
+    I.dec(step, di, dj, dk);
+  }
+  __device__ void next() {
+    i += di;
+    int cj = 0;
+    if (i >= mx) { i -= mx; cj = 1; }
+    j += dj + cj;
+    int ck = 0;
+    if (j >= my) { j -= my; ck = 1; }
+    k += dk + ck;
+  }
+};
+
 // ------------------------------------------------------------------
 // cross-derivative mode (poisson.c:800-837 and siblings)
 // ------------------------------------------------------------------
@@ -762,9 +781,9 @@ template <class T>
 __device__ __forceinline__ void restrict_body(Idx C, Idx F, int isc, int jsc, int ksc, const double *__restrict__ nvf,
                                      const float *__restrict__ wff, const uint32_t *__restrict__ flc,
                                      const T *__restrict__ xf, T *__restrict__ xc, long n, long t0, long step) {
-  for (long t = t0; t < n; t += step) {
-    int i, j, k;
-    C.dec(t, i, j, k);
+  IdxWalk w(C, t0, step);
+  for (long t = t0; t < n; t += step, w.next()) {
+    const int i = w.i, j = w.j, k = w.k;
     T f = 0;
     if (flc[t] & 1u) {
       // 32-bit indices (storage sizes stay below 2^31): the fine point and
@@ -800,9 +819,9 @@ template <class T>
 __device__ __forceinline__ void prolong_body(Idx F, Idx C, int isc, int jsc, int ksc, const double *__restrict__ nvc,
                                     const uint32_t *__restrict__ flf, const T *__restrict__ xc, T *__restrict__ xf,
                                     int add, long n, long t0, long step) {
-  for (long t = t0; t < n; t += step) {
-    int i, j, k;
-    F.dec(t, i, j, k);
+  IdxWalk w(F, t0, step);
+  for (long t = t0; t < n; t += step, w.next()) {
+    int i = w.i, j = w.j, k = w.k;
     T f = 0;
     if (i > 0 && i < F.mx - 1 && j > 0 && j < F.my - 1 && k > 0 && k < F.mz - 1) {
       int ic, jc, kc, ia, ja, ka;
@@ -1981,9 +2000,9 @@ __device__ __forceinline__ static void projection_body(Idx I, int xper, int zper
                                     FaceMet Mi, FaceMet Mj, FaceMet Mk, double *ux, double *uy, double *uz, double dt,
                                     double st, long n, long t0_, long st_) {
   const int i_end = xper ? I.mx - 1 : I.mx - 2, j_end = I.my - 2, k_end = zper ? I.mz - 1 : I.mz - 2;
-  for (long t = t0_; t < n; t += st_) {
-    int i, j, k;
-    I.dec(t, i, j, k);
+  IdxWalk w(I, t0_, st_);
+  for (long t = t0_; t < n; t += st_, w.next()) {
+    const int i = w.i, j = w.j, k = w.k;
     bool jk = j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1, ij = i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1;
     if (jk && i > 0 && i < I.mx - 1) {
       if (i < i_end) project_face(I, 0, i, j, k, phi, nv, fl, Mi, ux, dt, st);
