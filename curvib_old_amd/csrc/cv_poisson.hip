@@ -795,8 +795,13 @@ __device__ __forceinline__ void restrict_body(Idx C, Idx F, int isc, int jsc, in
         if constexpr (sizeof(T) == 4) return xf[q] * wff[q];
         else return xf[q] * (T)fmax(0., 1 - nvf[q]);
       };
-      f = (T)0.125 * (RT(q0) + RT(q0 - ia) + RT(q0 - ja) + RT(q0 - ka) + RT(q0 - ia - ja) + RT(q0 - ja - ka) +
-                      RT(q0 - ia - ka) + RT(q0 - ia - ja - ka));
+      if (ja == 0) {   // y kept (semi-coarsening): four distinct terms, summed in the same order
+        const T a = RT(q0), b = RT(q0 - ia), c = RT(q0 - ka), d = RT(q0 - ia - ka);
+        f = (T)0.125 * (a + b + a + c + b + c + d + d);
+      } else {
+        f = (T)0.125 * (RT(q0) + RT(q0 - ia) + RT(q0 - ja) + RT(q0 - ka) + RT(q0 - ia - ja) + RT(q0 - ja - ka) +
+                        RT(q0 - ia - ka) + RT(q0 - ia - ja - ka));
+      }
     }
     xc[t] = f;
   }
@@ -836,8 +841,14 @@ __device__ __forceinline__ void prolong_body(Idx F, Idx C, int isc, int jsc, int
       if (ja != 0 && nvc[c0 + dj] > 0.1) ja = 0;
       if (ia != 0 && nvc[c0 + di] > 0.1) ia = 0;
       const int oi = ia != 0 ? di : 0, oj = ja != 0 ? dj : 0, ok = ka != 0 ? dk : 0;
-      f = (xc[c0] * (T)9 + xc[c0 + oj] * (T)3 + xc[c0 + oi] * (T)3 + xc[c0 + oi + oj]) * (T)(3. / 64.) +
-          (xc[c0 + ok] * (T)9 + xc[c0 + oj + ok] * (T)3 + xc[c0 + oi + ok] * (T)3 + xc[c0 + oi + oj + ok]) / (T)64.;
+      if (oj == 0) {   // no j neighbour (y kept, or dropped): four distinct values, same expression
+        const T v0 = xc[c0], vi = xc[c0 + oi], vk = xc[c0 + ok], vik = xc[c0 + oi + ok];
+        f = (v0 * (T)9 + v0 * (T)3 + vi * (T)3 + vi) * (T)(3. / 64.) +
+            (vk * (T)9 + vk * (T)3 + vik * (T)3 + vik) / (T)64.;
+      } else {
+        f = (xc[c0] * (T)9 + xc[c0 + oj] * (T)3 + xc[c0 + oi] * (T)3 + xc[c0 + oi + oj]) * (T)(3. / 64.) +
+            (xc[c0 + ok] * (T)9 + xc[c0 + oj + ok] * (T)3 + xc[c0 + oi + ok] * (T)3 + xc[c0 + oi + oj + ok]) / (T)64.;
+      }
     }
     if (!(flf[t] & 1u)) f = 0;
     if (add) xf[t] += f; else xf[t] = f;
