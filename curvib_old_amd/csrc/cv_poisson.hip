@@ -1093,8 +1093,9 @@ static int stencil_kchunk(int mz) {
     const char *e = getenv("CV_STENCIL_KCHUNK");
     kc = e ? std::max(1, atoi(e)) : 0;
   }
-  // default: 12 planes on deep levels, 8 on shallower ones (more blocks);
-  // the fastest measured at 512x256x256.  Levels of few planes (mz < 64)
+  // default: 12 planes on deep levels, 6 on shallower ones (more blocks);
+  // the fastest measured at 512x256x256 (6 against 8 on the middle tier:
+  // 0.3 ms per step).  Levels of few planes (mz < 64)
   // take CV_STENCIL_KCHUNK_SMALL planes per block: their launches are short
   // and latency-bound, so a shorter plane march can pay
   static int ks = -1, km = -1;
@@ -1102,21 +1103,25 @@ static int stencil_kchunk(int mz) {
     const char *e = getenv("CV_STENCIL_KCHUNK_SMALL");
     ks = e ? std::max(1, atoi(e)) : 2;
     e = getenv("CV_STENCIL_KCHUNK_MID");
-    km = e ? std::max(1, atoi(e)) : 8;
+    km = e ? std::max(1, atoi(e)) : 6;
   }
   return kc ? kc : (mz >= 192 ? 12 : mz >= 64 ? km : ks);
 }
 // in a batch the small levels of many cases share one launch: 5 planes
 // there measured best for the 64-case ensemble (102.7 ms per step against
-// 103.0 with 4 and 104.1 with 8; the tile shape does not change any point's
-// arithmetic, so a case gets the same bits either way)
+// 103.0 with 4 and 104.1 with 8), and 8 on the middle tier (6 there cost
+// 0.4 ms).  The tile shape does not change any point's arithmetic, so a
+// case gets the same bits either way
 static int stencil_kchunk_batch(int mz) {
-  static int kb = -1;
+  static int kb = -1, kbm = -1;
   if (kb < 0) {
     const char *e = getenv("CV_BATCH_KCHUNK_SMALL");
     kb = e ? std::max(1, atoi(e)) : 5;
+    e = getenv("CV_BATCH_KCHUNK_MID");
+    kbm = e ? std::max(1, atoi(e)) : 8;
   }
-  return mz >= 64 || getenv("CV_STENCIL_KCHUNK") ? stencil_kchunk(mz) : kb;
+  if (mz >= 192 || getenv("CV_STENCIL_KCHUNK")) return stencil_kchunk(mz);
+  return mz >= 64 ? kbm : kb;
 }
 
 // rows per stencil block (4 or 8; CV_STENCIL_ROWS64 / CV_STENCIL_ROWS32) and
