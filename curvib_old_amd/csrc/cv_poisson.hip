@@ -439,13 +439,16 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restri
     }
   };
 
-  // prologue: planes k0-1 .. k0+1 in LDS, plane k0+2 and the cells of k0 in registers
+  // prologue: planes k0-1 .. k0+1 in LDS, plane k0+2 and the cells of k0 in
+  // registers.  The chunk reads planes k0-1 .. k1 only (plane gk+1 is the
+  // last neighbour of plane gk), so planes past k1 are neither loaded nor
+  // stored: the next chunk loads them itself
   TX xa, xb;
   fetch_x(k0 - 1, xa, xb); store_x(k0 - 1, xa, xb);
   fetch_x(k0, xa, xb); store_x(k0, xa, xb);
   fetch_x(k0 + 1, xa, xb); store_x(k0 + 1, xa, xb);
-  TX xn0, xn1;
-  fetch_x(k0 + 2, xn0, xn1);
+  TX xn0 = (TX)0, xn1 = (TX)0;
+  if (k0 + 2 <= k1) fetch_x(k0 + 2, xn0, xn1);
   StCell<T> cur;
   fetch_cell(k0, cur);
   __syncthreads();
@@ -455,8 +458,8 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restri
     fk_lo = tile_flux_k<T, ST_TC>(X[(k0 - 1 + 4) & 3], X[(k0 + 4) & 3], c, r, wo(t), gco(6, t), gco(7, t), gco(8, t));
   }
   for (int gk = k0; gk < k1; gk++) {
-    TX xq0, xq1;
-    fetch_x(gk + 3, xq0, xq1);
+    TX xq0 = (TX)0, xq1 = (TX)0;
+    if (gk + 3 <= k1) fetch_x(gk + 3, xq0, xq1);
     StCell<T> nxt;
     if (gk + 1 < k1) fetch_cell(gk + 1, nxt);
     const long t = tij + pl * gk;
@@ -486,7 +489,7 @@ __device__ __forceinline__ void stencil_tile(const OpV<T> &A, const TX *__restri
     }
     const T xc = P[r * ST_TC + c];
     FJ[gk & 1][R][L] = fj;
-    store_x(gk + 2, xn0, xn1);
+    if (gk + 2 <= k1) store_x(gk + 2, xn0, xn1);
     const T fi_lo = __shfl_up(fi, 1, TW);
     __syncthreads();
     if (R > 0) fj_lo = FJ[gk & 1][R - 1][L];
