@@ -1096,19 +1096,23 @@ static int stencil_kchunk(int mz) {
     const char *e = getenv("CV_STENCIL_KCHUNK");
     kc = e ? std::max(1, atoi(e)) : 0;
   }
-  // default: 12 planes on deep levels, 6 on shallower ones (more blocks);
-  // the fastest measured at 512x256x256 (6 against 8 on the middle tier:
-  // 0.3 ms per step).  Levels of few planes (mz < 64)
+  // default: 8 planes on deep levels (CV_STENCIL_KCHUNK_DEEP), 6 on
+  // shallower ones (more blocks); the fastest measured at 512x256x256 (6
+  // against 8 on the middle tier: 0.3 ms per step; 8 against 12 on the deep
+  // tier: 0.35 ms, once a chunk stopped loading planes past its end).
+  // Levels of few planes (mz < 64)
   // take CV_STENCIL_KCHUNK_SMALL planes per block: their launches are short
   // and latency-bound, so a shorter plane march can pay
-  static int ks = -1, km = -1;
+  static int ks = -1, km = -1, kd = -1;
   if (ks < 0) {
     const char *e = getenv("CV_STENCIL_KCHUNK_SMALL");
     ks = e ? std::max(1, atoi(e)) : 2;
     e = getenv("CV_STENCIL_KCHUNK_MID");
     km = e ? std::max(1, atoi(e)) : 6;
+    e = getenv("CV_STENCIL_KCHUNK_DEEP");
+    kd = e ? std::max(1, atoi(e)) : 8;
   }
-  return kc ? kc : (mz >= 192 ? 12 : mz >= 64 ? km : ks);
+  return kc ? kc : (mz >= 192 ? kd : mz >= 64 ? km : ks);
 }
 // in a batch the small levels of many cases share one launch: 5 planes
 // there measured best for the 64-case ensemble (102.7 ms per step against
