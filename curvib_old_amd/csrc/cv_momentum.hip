@@ -1151,6 +1151,16 @@ __device__ inline long prev_cell(const G &g, int i, int j, int k) {
   return -1;
 }
 
+// ptan from loaded values: p0 = p[t], po = p[t+o], pf and pof = p[t+f],
+// p[t+o+f], pmf and pomf = p[t-f], p[t+o-f]; the same terms in the same
+// order as ptan, so the same bits
+__device__ inline double ptan_v(int mode, double p0, double po, double pf, double pof, double pmf, double pomf) {
+  const bool lo = mode == XM_LOWER, up = mode == XM_UPPER;
+  const double a = lo ? p0 : pf, b = lo ? po : pof;
+  const double c = up ? p0 : pmf, d = up ? po : pomf;
+  return (a + b - c - d) * (lo || up ? 0.5 : 0.25);
+}
+
 __device__ __forceinline__ void ff1_faces_body(FFArgs A, V3 rhs, long n, int bx, int by, int bz) {
   const G &g = A.g;
   (void)n;
@@ -1158,36 +1168,49 @@ __device__ __forceinline__ void ff1_faces_body(FFArgs A, V3 rhs, long n, int bx,
     const long t = g.at(i, j, k);
     if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
     const double *p = A.p;
+    const long sj = g.sj, sk = g.sk;
+    const uint32_t w = A.fl[t];
+    // mode of the tangential derivative q (0, 1: the two other axes in
+    // order) on the face of axis d, as face_mode reads it
+    auto fm = [w](int d, int q) { return (int)((w >> (2 + 5 * d + 2 * q)) & 3u); };
+    // the 16 points of p the three faces' rules read, loaded before any
+    // mode test so that their loads overlap (all lie inside the grid for an
+    // interior cell)
+    const double p0 = p[t], pe = p[t + 1], pn = p[t + sj], pt = p[t + sk];
+    const double pw = p[t - 1], ps = p[t - sj], pb = p[t - sk];
+    const double pen = p[t + 1 + sj], pes = p[t + 1 - sj], pet = p[t + 1 + sk], peb = p[t + 1 - sk];
+    const double pnw = p[t + sj - 1], pnt = p[t + sj + sk], pnb = p[t + sj - sk];
+    const double ptw = p[t + sk - 1], pts = p[t + sk - sj];
     // carried values from the previous cell in loop order
-    double pe = 0.0, pz = 0.0;
-    int mxe = face_mode(A.fl, t, 0, 1), mxz = face_mode(A.fl, t, 0, 2);
+    double cde = 0.0, cdz = 0.0;
+    const int mxe = fm(0, 0), mxz = fm(0, 1);
     if (mxe == XM_DROP || mxz == XM_DROP) {
       long pc = prev_cell(g, i, j, k);
-      if (pc >= 0) zface_state(A, pc, &pe, &pz);
+      if (pc >= 0) zface_state(A, pc, &cde, &cdz);
     }
     // x face: dpdc set, dpde rule, dpdz rule
-    double xdc = p[t + 1] - p[t];
-    double xde = mxe == XM_DROP ? pe : ptan(g, p, 0, 1, t, mxe);
-    double xdz = mxz == XM_DROP ? pz : ptan(g, p, 0, 2, t, mxz);
+    double xdc = pe - p0;
+    double xde = mxe == XM_DROP ? cde : ptan_v(mxe, p0, pe, pn, pen, ps, pes);
+    double xdz = mxz == XM_DROP ? cdz : ptan_v(mxz, p0, pe, pt, pet, pb, peb);
     double rx = face_rhs_value(A, 0, t, xdc, xde, xdz);
     // y face: dpdc rule, dpde set, dpdz rule
-    int myc = face_mode(A.fl, t, 1, 0), myz = face_mode(A.fl, t, 1, 2);
-    double ydc = myc == XM_DROP ? xdc : ptan(g, p, 1, 0, t, myc);
-    double yde = p[t + g.sj] - p[t];
-    double ydz = myz == XM_DROP ? xdz : ptan(g, p, 1, 2, t, myz);
+    const int myc = fm(1, 0), myz = fm(1, 1);
+    double ydc = myc == XM_DROP ? xdc : ptan_v(myc, p0, pn, pe, pen, pw, pnw);
+    double yde = pn - p0;
+    double ydz = myz == XM_DROP ? xdz : ptan_v(myz, p0, pn, pt, pnt, pb, pnb);
     double ry = face_rhs_value(A, 1, t, ydc, yde, ydz);
     // z face: dpdc rule, dpde rule, dpdz set
-    int mzc = face_mode(A.fl, t, 2, 0), mze = face_mode(A.fl, t, 2, 1);
-    double zdc = mzc == XM_DROP ? ydc : ptan(g, p, 2, 0, t, mzc);
-    double zde = mze == XM_DROP ? yde : ptan(g, p, 2, 1, t, mze);
-    double zdz = p[t + g.sk] - p[t];
+    const int mzc = fm(2, 0), mze = fm(2, 1);
+    double zdc = mzc == XM_DROP ? ydc : ptan_v(mzc, p0, pt, pe, pet, pw, ptw);
+    double zde = mze == XM_DROP ? yde : ptan_v(mze, p0, pt, pn, pnt, ps, pts);
+    double zdz = pt - p0;
     double rz = face_rhs_value(A, 2, t, zdc, zde, zdz);
     // masks (rhs.c:3280-3306)
     const double *nv = A.nv;
     if (nv[t] > 0.1) { rx = 0; ry = 0; rz = 0; }
     if (nv[t + 1] > 0.1) rx = 0;
-    if (nv[t + g.sj] > 0.1) ry = 0;
-    if (nv[t + g.sk] > 0.1) rz = 0;
+    if (nv[t + sj] > 0.1) ry = 0;
+    if (nv[t + sk] > 0.1) rz = 0;
     rhs.x[t] = rx; rhs.y[t] = ry; rhs.z[t] = rz;
   }
 }
