@@ -2020,6 +2020,59 @@ __device__ static inline void project_face(Idx I, int d, int i, int j, int k, co
   u[t] -= (dp[0] * g0 + dp[1] * g1 + dp[2] * g2) * dt * st / 1.5;
 }
 
+// tang_p from loaded values: p0 = p[c], po = p[c+o], pf and pof = p[c+f],
+// p[c+o+f], pmf and pomf = p[c-f], p[c+o-f]; the same terms in the same
+// order as tang_p, so the same bits
+__device__ static inline double tang_v(int mode, double p0, double po, double pf, double pof, double pmf,
+                                       double pomf) {
+  const bool lo = mode == XM_LOWER, up = mode == XM_UPPER;
+  const double a = lo ? p0 : pf, b = lo ? po : pof;
+  const double c = up ? p0 : pmf, d = up ? po : pomf;
+  const double v = (a + b - c - d) * (lo || up ? 0.5 : 0.25);
+  return mode == XM_DROP ? 0.0 : v;
+}
+
+// the three faces of an interior cell t (1 <= i, j, k <= m-2) as three
+// project_face calls (fx, fy, fz: which of them to update), with every
+// operand loaded before the fluid and mode tests so that the loads overlap:
+// the 16 points of phi the faces' rules read (all inside the grid), the
+// nvert pairs, the face coefficients and the fluxes
+__device__ static inline void project_cell(Idx I, long t, bool fx, bool fy, bool fz, const double *phi,
+                                           const double *nv, const uint32_t *fl, const FaceMet &Mi,
+                                           const FaceMet &Mj, const FaceMet &Mk, double *ux, double *uy, double *uz,
+                                           double dt, double st) {
+  const long sj = (long)I.mx, sk = (long)I.mx * I.my;
+  const double p0 = phi[t], pe = phi[t + 1], pn = phi[t + sj], pt = phi[t + sk];
+  const double pw = phi[t - 1], ps = phi[t - sj], pb = phi[t - sk];
+  const double pen = phi[t + 1 + sj], pes = phi[t + 1 - sj], pet = phi[t + 1 + sk], peb = phi[t + 1 - sk];
+  const double pnw = phi[t + sj - 1], pnt = phi[t + sj + sk], pnb = phi[t + sj - sk];
+  const double ptw = phi[t + sk - 1], pts = phi[t + sk - sj];
+  const double n0 = nv[t], ne = nv[t + 1], nn = nv[t + sj], nt = nv[t + sk];
+  const uint32_t w = fl[t];
+  auto md = [w](int d, int q) { return (int)((w >> (2 + 5 * d + 2 * q)) & 3u); };
+  if (fx && n0 + ne == 0.0) {
+    const double dp0 = pe - p0;
+    const double dp1 = tang_v(md(0, 0), p0, pe, pn, pen, ps, pes);
+    const double dp2 = tang_v(md(0, 1), p0, pe, pt, pet, pb, peb);
+    const double g0 = Mi.g[0][t], g1 = Mi.g[1][t], g2 = Mi.g[2][t];
+    ux[t] -= (dp0 * g0 + dp1 * g1 + dp2 * g2) * dt * st / 1.5;
+  }
+  if (fy && n0 + nn == 0.0) {
+    const double dp0 = tang_v(md(1, 0), p0, pn, pe, pen, pw, pnw);
+    const double dp1 = pn - p0;
+    const double dp2 = tang_v(md(1, 1), p0, pn, pt, pnt, pb, pnb);
+    const double g0 = Mj.g[0][t], g1 = Mj.g[1][t], g2 = Mj.g[2][t];
+    uy[t] -= (dp0 * g0 + dp1 * g1 + dp2 * g2) * dt * st / 1.5;
+  }
+  if (fz && n0 + nt == 0.0) {
+    const double dp0 = tang_v(md(2, 0), p0, pt, pe, pet, pw, ptw);
+    const double dp1 = tang_v(md(2, 1), p0, pt, pn, pnt, ps, pts);
+    const double dp2 = pt - p0;
+    const double g0 = Mk.g[0][t], g1 = Mk.g[1][t], g2 = Mk.g[2][t];
+    uz[t] -= (dp0 * g0 + dp1 * g1 + dp2 * g2) * dt * st / 1.5;
+  }
+}
+
 __device__ __forceinline__ static void projection_body(Idx I, int xper, int zper, const double *phi, const double *nv, const uint32_t *fl,
                                     FaceMet Mi, FaceMet Mj, FaceMet Mk, double *ux, double *uy, double *uz, double dt,
                                     double st, long n, long t0_, long st_) {
@@ -2028,11 +2081,8 @@ __device__ __forceinline__ static void projection_body(Idx I, int xper, int zper
   for (long t = t0_; t < n; t += st_, w.next()) {
     const int i = w.i, j = w.j, k = w.k;
     bool jk = j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1, ij = i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1;
-    if (jk && i > 0 && i < I.mx - 1) {
-      if (i < i_end) project_face(I, 0, i, j, k, phi, nv, fl, Mi, ux, dt, st);
-      if (j < j_end) project_face(I, 1, i, j, k, phi, nv, fl, Mj, uy, dt, st);
-      if (k < k_end) project_face(I, 2, i, j, k, phi, nv, fl, Mk, uz, dt, st);
-    }
+    if (jk && i > 0 && i < I.mx - 1)
+      project_cell(I, t, i < i_end, j < j_end, k < k_end, phi, nv, fl, Mi, Mj, Mk, ux, uy, uz, dt, st);
     if (xper && i == 0 && jk) project_face(I, 0, 0, j, k, phi, nv, fl, Mi, ux, dt, st);
     if (zper && k == 0 && ij) project_face(I, 2, i, j, 0, phi, nv, fl, Mk, uz, dt, st);
   }
