@@ -914,6 +914,30 @@ __device__ inline void rct_out(const G &g, long t, const double *d0, const doubl
   rct.z[t] = A * (0.5 * (zet.x[t] + zet.x[tk]) * rcx + 0.5 * (zet.y[t] + zet.y[tk]) * rcy + 0.5 * (zet.z[t] + zet.z[tk]) * rcz);
 }
 
+// rct_out in two halves: the face-averaged basis vectors and aj of cell t
+// (loaded ahead, e.g. before a barrier), then the products with Rc.  The
+// terms and their order are rct_out's, so the bits are the same
+struct RctOps {
+  double A, c[3], e[3], z[3];
+};
+__device__ inline RctOps rct_load(const G &g, long t, C3 csi, C3 eta, C3 zet, const double *aj) {
+  const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
+  RctOps o;
+  o.A = aj[t];
+  o.c[0] = 0.5 * (csi.x[t] + csi.x[ti]); o.c[1] = 0.5 * (csi.y[t] + csi.y[ti]); o.c[2] = 0.5 * (csi.z[t] + csi.z[ti]);
+  o.e[0] = 0.5 * (eta.x[t] + eta.x[tj]); o.e[1] = 0.5 * (eta.y[t] + eta.y[tj]); o.e[2] = 0.5 * (eta.z[t] + eta.z[tj]);
+  o.z[0] = 0.5 * (zet.x[t] + zet.x[tk]); o.z[1] = 0.5 * (zet.y[t] + zet.y[tk]); o.z[2] = 0.5 * (zet.z[t] + zet.z[tk]);
+  return o;
+}
+__device__ inline void rct_store(const RctOps &o, long t, const double *d0, const double *d1, const double *d2, V3 rct) {
+  const double rcx = d0[0] + d1[0] + d2[0];
+  const double rcy = d0[1] + d1[1] + d2[1];
+  const double rcz = d0[2] + d1[2] + d2[2];
+  rct.x[t] = o.A * (o.c[0] * rcx + o.c[1] * rcy + o.c[2] * rcz);
+  rct.y[t] = o.A * (o.e[0] * rcx + o.e[1] * rcy + o.e[2] * rcz);
+  rct.z[t] = o.A * (o.z[0] * rcx + o.z[1] * rcy + o.z[2] * rcz);
+}
+
 // Rc = Visc - Conv as the divergence of the three face families and rct
 // at the interior cells, in one pass with no face arrays in HBM.  Block
 // 64 x TY owns an (i, j) tile and marches RC_KC planes of k: lane 0 is the
@@ -1052,12 +1076,15 @@ __device__ __forceinline__ void rc_tiled_body(const G &g, C3 uc, C3 u, const dou
       }
       face_rc<2>(g, i, j, k, t, uc.z[t], a, mk, nu, a2);
     }
+    // rct's operands are loaded before barrier B, so their latency overlaps the wait
+    RctOps ro{};
+    if (out && kin) ro = rct_load(g, t, csi, eta, zet, aj);
     __syncthreads();   // B
     if (out && kin) {
       double d1[3], d2[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) { d1[c] = a1[c] - SF[c][R - 1][L]; d2[c] = a2[c] - p2[c]; }
-      rct_out(g, t, d0, d1, d2, csi, eta, zet, aj, rct);
+      rct_store(ro, t, d0, d1, d2, rct);
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) p2[c] = a2[c];
