@@ -920,13 +920,15 @@ __device__ inline void rct_out(const G &g, long t, const double *d0, const doubl
 struct RctOps {
   double A, c[3], e[3], z[3];
 };
-__device__ inline RctOps rct_load(const G &g, long t, C3 csi, C3 eta, C3 zet, const double *aj) {
-  const long ti = t - 1, tj = t - g.sj, tk = t - g.sk;
+// zt, zb: zet at t and at t - sk (the k-marching pass carries the latter)
+__device__ inline RctOps rct_load(const G &g, long t, C3 csi, C3 eta, const double *zt, const double *zb,
+                                  const double *aj) {
+  const long ti = t - 1, tj = t - g.sj;
   RctOps o;
   o.A = aj[t];
   o.c[0] = 0.5 * (csi.x[t] + csi.x[ti]); o.c[1] = 0.5 * (csi.y[t] + csi.y[ti]); o.c[2] = 0.5 * (csi.z[t] + csi.z[ti]);
   o.e[0] = 0.5 * (eta.x[t] + eta.x[tj]); o.e[1] = 0.5 * (eta.y[t] + eta.y[tj]); o.e[2] = 0.5 * (eta.z[t] + eta.z[tj]);
-  o.z[0] = 0.5 * (zet.x[t] + zet.x[tk]); o.z[1] = 0.5 * (zet.y[t] + zet.y[tk]); o.z[2] = 0.5 * (zet.z[t] + zet.z[tk]);
+  o.z[0] = 0.5 * (zt[0] + zb[0]); o.z[1] = 0.5 * (zt[1] + zb[1]); o.z[2] = 0.5 * (zt[2] + zb[2]);
   return o;
 }
 __device__ inline void rct_store(const RctOps &o, long t, const double *d0, const double *d1, const double *d2, V3 rct) {
@@ -1044,6 +1046,9 @@ __device__ __forceinline__ void rc_tiled_body(const G &g, C3 uc, C3 u, const dou
   load_plane(ks - 1);
   load_plane(ks);
   double p2[3] = {0., 0., 0.};
+  // zet of the previous plane's cell, carried: rct's k average reads it, and
+  // a plane of the XCD's blocks outgrows its L2 between the two reads
+  double zb[3] = {0., 0., 0.};
   for (int k = ks; k < k1; k++) {
     load_plane(k + 1);
     __syncthreads();   // A
@@ -1078,7 +1083,11 @@ __device__ __forceinline__ void rc_tiled_body(const G &g, C3 uc, C3 u, const dou
     }
     // rct's operands are loaded before barrier B, so their latency overlaps the wait
     RctOps ro{};
-    if (out && kin) ro = rct_load(g, t, csi, eta, zet, aj);
+    double zt[3] = {0., 0., 0.};
+    if (out) { zt[0] = zet.x[t]; zt[1] = zet.y[t]; zt[2] = zet.z[t]; }
+    if (out && kin) ro = rct_load(g, t, csi, eta, zt, zb, aj);
+#pragma unroll
+    for (int c = 0; c < 3; c++) zb[c] = zt[c];
     __syncthreads();   // B
     if (out && kin) {
       double d1[3], d2[3];
@@ -1158,6 +1167,14 @@ __device__ inline double face_rhs_value(const FFArgs &A, int d, long t, double d
   return r;
 }
 
+// face_rhs_value with the two rct values of the face already loaded
+__device__ inline double face_rhs_v(const FFArgs &A, int d, long t, double rlo, double rhi, double dpdc, double dpde,
+                                    double dpdz) {
+  double r = 0.5 * (rlo + rhi);
+  r -= dpdc * A.gc[3 * d][t] + dpde * A.gc[3 * d + 1][t] + dpdz * A.gc[3 * d + 2][t];
+  return r;
+}
+
 // the z-face (dpde, dpdz) state a cell leaves behind in loop order, used by
 // the next cell's DROP carry
 __device__ inline void zface_state(const FFArgs &A, long t, double *dpde, double *dpdz) {
@@ -1191,23 +1208,37 @@ __device__ inline double ptan_v(int mode, double p0, double po, double pf, doubl
 __device__ __forceinline__ void ff1_faces_body(FFArgs A, V3 rhs, long n, int bx, int by, int bz) {
   const G &g = A.g;
   (void)n;
+  // what the cell below (k-1) loaded that this cell reads again: its plane
+  // k+1 points (this cell's p, pe, pn, pw, ps), its own p, pe, pn (this
+  // cell's k-1 points), rct.z and nvert at k+1; valid after an interior cell
+  bool have = false;
+  double c_pt = 0., c_pet = 0., c_pnt = 0., c_ptw = 0., c_pts = 0., c_p0 = 0., c_pe = 0., c_pn = 0.;
+  double c_rz = 0., c_nv = 0.;
   CV_MARCH_B(g, bx, by, bz) {
     const long t = g.at(i, j, k);
-    if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) continue;
+    if (!(i > 0 && i < g.mx - 1 && j > 0 && j < g.my - 1 && k > 0 && k < g.mz - 1)) { have = false; continue; }
     const double *p = A.p;
     const long sj = g.sj, sk = g.sk;
     const uint32_t w = A.fl[t];
     // mode of the tangential derivative q (0, 1: the two other axes in
     // order) on the face of axis d, as face_mode reads it
     auto fm = [w](int d, int q) { return (int)((w >> (2 + 5 * d + 2 * q)) & 3u); };
-    // the 16 points of p the three faces' rules read, loaded before any
-    // mode test so that their loads overlap (all lie inside the grid for an
-    // interior cell)
-    const double p0 = p[t], pe = p[t + 1], pn = p[t + sj], pt = p[t + sk];
-    const double pw = p[t - 1], ps = p[t - sj], pb = p[t - sk];
-    const double pen = p[t + 1 + sj], pes = p[t + 1 - sj], pet = p[t + 1 + sk], peb = p[t + 1 - sk];
-    const double pnw = p[t + sj - 1], pnt = p[t + sj + sk], pnb = p[t + sj - sk];
-    const double ptw = p[t + sk - 1], pts = p[t + sk - sj];
+    // the 16 points of p the three faces' rules read (all inside the grid
+    // for an interior cell), loaded before any mode test so that their
+    // loads overlap; 8 of them come from the cell below when it was interior
+    double p0, pe, pn, pw, ps, pb, pnb, peb, rz0, n0;
+    if (have) {
+      p0 = c_pt; pe = c_pet; pn = c_pnt; pw = c_ptw; ps = c_pts;
+      pb = c_p0; pnb = c_pn; peb = c_pe;
+      rz0 = c_rz; n0 = c_nv;
+    } else {
+      p0 = p[t]; pe = p[t + 1]; pn = p[t + sj]; pw = p[t - 1]; ps = p[t - sj];
+      pb = p[t - sk]; pnb = p[t + sj - sk]; peb = p[t + 1 - sk];
+      rz0 = A.rct.z[t]; n0 = A.nv[t];
+    }
+    const double pt = p[t + sk], pnt = p[t + sj + sk], pet = p[t + 1 + sk], ptw = p[t + sk - 1], pts = p[t + sk - sj];
+    const double pen = p[t + 1 + sj], pes = p[t + 1 - sj], pnw = p[t + sj - 1];
+    const double rz1 = A.rct.z[t + sk], nvt = A.nv[t + sk];
     // carried values from the previous cell in loop order
     double cde = 0.0, cdz = 0.0;
     const int mxe = fm(0, 0), mxz = fm(0, 1);
@@ -1231,14 +1262,18 @@ __device__ __forceinline__ void ff1_faces_body(FFArgs A, V3 rhs, long n, int bx,
     double zdc = mzc == XM_DROP ? ydc : ptan_v(mzc, p0, pt, pe, pet, pw, ptw);
     double zde = mze == XM_DROP ? yde : ptan_v(mze, p0, pt, pn, pnt, ps, pts);
     double zdz = pt - p0;
-    double rz = face_rhs_value(A, 2, t, zdc, zde, zdz);
+    double rz = face_rhs_v(A, 2, t, rz0, rz1, zdc, zde, zdz);
     // masks (rhs.c:3280-3306)
     const double *nv = A.nv;
-    if (nv[t] > 0.1) { rx = 0; ry = 0; rz = 0; }
+    if (n0 > 0.1) { rx = 0; ry = 0; rz = 0; }
     if (nv[t + 1] > 0.1) rx = 0;
     if (nv[t + sj] > 0.1) ry = 0;
-    if (nv[t + sk] > 0.1) rz = 0;
+    if (nvt > 0.1) rz = 0;
     rhs.x[t] = rx; rhs.y[t] = ry; rhs.z[t] = rz;
+    have = true;
+    c_pt = pt; c_pet = pet; c_pnt = pnt; c_ptw = ptw; c_pts = pts;
+    c_p0 = p0; c_pe = pe; c_pn = pn;
+    c_rz = rz1; c_nv = nvt;
   }
 }
 __global__ void __launch_bounds__(BS) k_ff1_faces(FFArgs A, V3 rhs, long n) { ff1_faces_body(A, rhs, n, blockIdx.x, blockIdx.y, blockIdx.z); }
