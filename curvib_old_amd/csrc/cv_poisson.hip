@@ -2032,45 +2032,64 @@ __device__ static inline double tang_v(int mode, double p0, double po, double pf
   return mode == XM_DROP ? 0.0 : v;
 }
 
+// the 16 points of phi the three faces' rules read around an interior cell
+// (all inside the grid) and the nvert of the cell and its +i, +j, +k
+// neighbours
+struct ProjPts {
+  double p0, pe, pn, pt, pw, ps, pb, pen, pes, pet, peb, pnw, pnt, pnb, ptw, pts;
+  double n0, ne, nn, nt;
+};
+
 // the three faces of an interior cell t (1 <= i, j, k <= m-2) as three
-// project_face calls (fx, fy, fz: which of them to update), with every
-// operand loaded before the fluid and mode tests so that the loads overlap:
-// the 16 points of phi the faces' rules read (all inside the grid), the
-// nvert pairs, the face coefficients and the fluxes
+// project_face calls (fx, fy, fz: which of them to update), from points
+// loaded before the fluid and mode tests so that the loads overlap
+__device__ static inline void project_faces(long t, bool fx, bool fy, bool fz, const ProjPts &q, uint32_t w,
+                                            const FaceMet &Mi, const FaceMet &Mj, const FaceMet &Mk, double *ux,
+                                            double *uy, double *uz, double dt, double st) {
+  auto md = [w](int d, int k) { return (int)((w >> (2 + 5 * d + 2 * k)) & 3u); };
+  if (fx && q.n0 + q.ne == 0.0) {
+    const double dp0 = q.pe - q.p0;
+    const double dp1 = tang_v(md(0, 0), q.p0, q.pe, q.pn, q.pen, q.ps, q.pes);
+    const double dp2 = tang_v(md(0, 1), q.p0, q.pe, q.pt, q.pet, q.pb, q.peb);
+    const double g0 = Mi.g[0][t], g1 = Mi.g[1][t], g2 = Mi.g[2][t];
+    ux[t] -= (dp0 * g0 + dp1 * g1 + dp2 * g2) * dt * st / 1.5;
+  }
+  if (fy && q.n0 + q.nn == 0.0) {
+    const double dp0 = tang_v(md(1, 0), q.p0, q.pn, q.pe, q.pen, q.pw, q.pnw);
+    const double dp1 = q.pn - q.p0;
+    const double dp2 = tang_v(md(1, 1), q.p0, q.pn, q.pt, q.pnt, q.pb, q.pnb);
+    const double g0 = Mj.g[0][t], g1 = Mj.g[1][t], g2 = Mj.g[2][t];
+    uy[t] -= (dp0 * g0 + dp1 * g1 + dp2 * g2) * dt * st / 1.5;
+  }
+  if (fz && q.n0 + q.nt == 0.0) {
+    const double dp0 = tang_v(md(2, 0), q.p0, q.pt, q.pe, q.pet, q.pw, q.ptw);
+    const double dp1 = tang_v(md(2, 1), q.p0, q.pt, q.pn, q.pnt, q.ps, q.pts);
+    const double dp2 = q.pt - q.p0;
+    const double g0 = Mk.g[0][t], g1 = Mk.g[1][t], g2 = Mk.g[2][t];
+    uz[t] -= (dp0 * g0 + dp1 * g1 + dp2 * g2) * dt * st / 1.5;
+  }
+}
+
+// the points of cell t that the cell below (t - sk) did not load: its plane
+// k+1 points are this cell's own plane, its own plane this cell's k-1
+__device__ static inline void proj_load_top(const double *phi, const double *nv, long t, long sj, long sk, ProjPts &q) {
+  q.pt = phi[t + sk]; q.pnt = phi[t + sj + sk]; q.pet = phi[t + 1 + sk]; q.ptw = phi[t + sk - 1];
+  q.pts = phi[t + sk - sj];
+  q.pen = phi[t + 1 + sj]; q.pes = phi[t + 1 - sj]; q.pnw = phi[t + sj - 1];
+  q.ne = nv[t + 1]; q.nn = nv[t + sj]; q.nt = nv[t + sk];
+}
+
 __device__ static inline void project_cell(Idx I, long t, bool fx, bool fy, bool fz, const double *phi,
                                            const double *nv, const uint32_t *fl, const FaceMet &Mi,
                                            const FaceMet &Mj, const FaceMet &Mk, double *ux, double *uy, double *uz,
                                            double dt, double st) {
   const long sj = (long)I.mx, sk = (long)I.mx * I.my;
-  const double p0 = phi[t], pe = phi[t + 1], pn = phi[t + sj], pt = phi[t + sk];
-  const double pw = phi[t - 1], ps = phi[t - sj], pb = phi[t - sk];
-  const double pen = phi[t + 1 + sj], pes = phi[t + 1 - sj], pet = phi[t + 1 + sk], peb = phi[t + 1 - sk];
-  const double pnw = phi[t + sj - 1], pnt = phi[t + sj + sk], pnb = phi[t + sj - sk];
-  const double ptw = phi[t + sk - 1], pts = phi[t + sk - sj];
-  const double n0 = nv[t], ne = nv[t + 1], nn = nv[t + sj], nt = nv[t + sk];
-  const uint32_t w = fl[t];
-  auto md = [w](int d, int q) { return (int)((w >> (2 + 5 * d + 2 * q)) & 3u); };
-  if (fx && n0 + ne == 0.0) {
-    const double dp0 = pe - p0;
-    const double dp1 = tang_v(md(0, 0), p0, pe, pn, pen, ps, pes);
-    const double dp2 = tang_v(md(0, 1), p0, pe, pt, pet, pb, peb);
-    const double g0 = Mi.g[0][t], g1 = Mi.g[1][t], g2 = Mi.g[2][t];
-    ux[t] -= (dp0 * g0 + dp1 * g1 + dp2 * g2) * dt * st / 1.5;
-  }
-  if (fy && n0 + nn == 0.0) {
-    const double dp0 = tang_v(md(1, 0), p0, pn, pe, pen, pw, pnw);
-    const double dp1 = pn - p0;
-    const double dp2 = tang_v(md(1, 1), p0, pn, pt, pnt, pb, pnb);
-    const double g0 = Mj.g[0][t], g1 = Mj.g[1][t], g2 = Mj.g[2][t];
-    uy[t] -= (dp0 * g0 + dp1 * g1 + dp2 * g2) * dt * st / 1.5;
-  }
-  if (fz && n0 + nt == 0.0) {
-    const double dp0 = tang_v(md(2, 0), p0, pt, pe, pet, pw, ptw);
-    const double dp1 = tang_v(md(2, 1), p0, pt, pn, pnt, ps, pts);
-    const double dp2 = pt - p0;
-    const double g0 = Mk.g[0][t], g1 = Mk.g[1][t], g2 = Mk.g[2][t];
-    uz[t] -= (dp0 * g0 + dp1 * g1 + dp2 * g2) * dt * st / 1.5;
-  }
+  ProjPts q;
+  q.p0 = phi[t]; q.pe = phi[t + 1]; q.pn = phi[t + sj]; q.pw = phi[t - 1]; q.ps = phi[t - sj];
+  q.pb = phi[t - sk]; q.pnb = phi[t + sj - sk]; q.peb = phi[t + 1 - sk];
+  q.n0 = nv[t];
+  proj_load_top(phi, nv, t, sj, sk, q);
+  project_faces(t, fx, fy, fz, q, fl[t], Mi, Mj, Mk, ux, uy, uz, dt, st);
 }
 
 __device__ __forceinline__ static void projection_body(Idx I, int xper, int zper, const double *phi, const double *nv, const uint32_t *fl,
@@ -2087,19 +2106,57 @@ __device__ __forceinline__ static void projection_body(Idx I, int xper, int zper
     if (zper && k == 0 && ij) project_face(I, 2, i, j, 0, phi, nv, fl, Mk, uz, dt, st);
   }
 }
-__global__ static void k_projection(Idx I, int xper, int zper, const double *phi, const double *nv, const uint32_t *fl,
-                                    FaceMet Mi, FaceMet Mj, FaceMet Mk, double *ux, double *uy, double *uz, double dt,
-                                    double st, long n) { projection_body(I, xper, zper, phi, nv, fl, Mi, Mj, Mk, ux, uy, uz, dt, st, n, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x); }
 
+
+// The projection as a k-march (64 x PJ_TY threads walk PJ_KC planes): the
+// same faces as projection_body, each cell taking 8 of its phi points and
+// its nvert from the cell below it in registers
+#define PJ_TY 4
+#define PJ_KC 16
+__global__ static void __launch_bounds__(64 * PJ_TY)
+    k_projection_march(Idx I, int xper, int zper, const double *phi, const double *nv, const uint32_t *fl, FaceMet Mi,
+                       FaceMet Mj, FaceMet Mk, double *ux, double *uy, double *uz, double dt, double st) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * PJ_TY + threadIdx.y;
+  if (i >= I.mx || j >= I.my) return;
+  const int k0 = blockIdx.z * PJ_KC, k1 = min(k0 + PJ_KC, I.mz);
+  const int i_end = xper ? I.mx - 1 : I.mx - 2, j_end = I.my - 2, k_end = zper ? I.mz - 1 : I.mz - 2;
+  const bool iin = i > 0 && i < I.mx - 1, jin = j > 0 && j < I.my - 1;
+  const long sj = (long)I.mx, sk = (long)I.mx * I.my;
+  bool have = false;
+  ProjPts c{};   // the cell below's points
+  for (int k = k0; k < k1; k++) {
+    const bool kin = k > 0 && k < I.mz - 1;
+    if (xper && i == 0 && jin && kin) project_face(I, 0, 0, j, k, phi, nv, fl, Mi, ux, dt, st);
+    if (zper && k == 0 && iin && jin) project_face(I, 2, i, j, 0, phi, nv, fl, Mk, uz, dt, st);
+    if (!(iin && jin && kin)) { have = false; continue; }
+    const long t = I.at(i, j, k);
+    ProjPts q;
+    if (have) {
+      q.p0 = c.pt; q.pe = c.pet; q.pn = c.pnt; q.pw = c.ptw; q.ps = c.pts;
+      q.pb = c.p0; q.pnb = c.pn; q.peb = c.pe;
+      q.n0 = c.nt;
+    } else {
+      q.p0 = phi[t]; q.pe = phi[t + 1]; q.pn = phi[t + sj]; q.pw = phi[t - 1]; q.ps = phi[t - sj];
+      q.pb = phi[t - sk]; q.pnb = phi[t + sj - sk]; q.peb = phi[t + 1 - sk];
+      q.n0 = nv[t];
+    }
+    proj_load_top(phi, nv, t, sj, sk, q);
+    project_faces(t, i < i_end, j < j_end, k < k_end, q, fl[t], Mi, Mj, Mk, ux, uy, uz, dt, st);
+    c = q;
+    have = true;
+  }
+}
 
 int cv_projection_dev(cv_ctx *c) {
   CvLevel &F = c->lev[c->nlev - 1];
   FaceMet Mi{{F.g[0], F.g[1], F.g[2]}};
   FaceMet Mj{{F.g[3], F.g[4], F.g[5]}};
   FaceMet Mk{{F.g[6], F.g[7], F.g[8]}};
-  hipLaunchKernelGGL(k_projection, dim3(nblk(F.d.n)), dim3(BS), 0, c->stream, idx_of(F), F.d.xper, F.d.zper,
+  const dim3 grid((unsigned)((F.d.mx + 63) / 64), (unsigned)((F.d.my + PJ_TY - 1) / PJ_TY),
+                  (unsigned)((F.d.mz + PJ_KC - 1) / PJ_KC));
+  hipLaunchKernelGGL(k_projection_march, grid, dim3(64, PJ_TY), 0, c->stream, idx_of(F), F.d.xper, F.d.zper,
                      (const double *)c->phi, (const double *)F.nvert, (const uint32_t *)F.flags, Mi, Mj, Mk,
-                     c->ucont[0], c->ucont[1], c->ucont[2], c->dt, c->st, F.d.n);
+                     c->ucont[0], c->ucont[1], c->ucont[2], c->dt, c->st);
   CV_CHECK(hipGetLastError());
   int rc = cv_contra2cart_dev(c);
   if (rc) return rc;
