@@ -2217,17 +2217,70 @@ __device__ __forceinline__ void divmax_block(Idx I, const double *ux, const doub
                                              int bid, int nb) {
   divmax_block_s<1>(I, ux, uy, uz, aj, nv, n, pv, pi, bid, nb);
 }
-__global__ static void k_divmax(Idx I, const double *ux, const double *uy, const double *uz, const double *aj,
-                                const double *nv, long n, double *pv, double *pi) {
-  divmax_block(I, ux, uy, uz, aj, nv, n, pv, pi, blockIdx.x, gridDim.x);
+
+// divmax_block_s<1> as k-marches: block b takes tiles b, b + nb, ... of 64 x
+// DV_TY columns by DV_KC planes, and each column's cell takes uz and two
+// nvert planes from the cell below it.  Every point is visited once, and
+// the maximum with its first index does not depend on the visiting order,
+// so the partials reduce to the same (value, index) as the flat pass
+#define DV_TY 4
+#define DV_KC 16
+__global__ static void __launch_bounds__(64 * DV_TY)
+    k_divmax_march(Idx I, const double *ux, const double *uy, const double *uz, const double *aj, const double *nv,
+                   double *pv, double *pi) {
+  static_assert(64 * DV_TY == BS, "one reduction slot per thread");
+  __shared__ double sv[BS];
+  __shared__ long si[BS];
+  const int tx = (I.mx + 63) / 64, ty = (I.my + DV_TY - 1) / DV_TY, tz = (I.mz + DV_KC - 1) / DV_KC;
+  const long ntile = (long)tx * ty * tz;
+  const long sj = I.mx, sk = (long)I.mx * I.my;
+  double best = -1.0;
+  long bi = 0;
+  for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
+    const int bx = (int)(tb % tx), by = (int)((tb / tx) % ty), bz = (int)(tb / ((long)tx * ty));
+    const int i = bx * 64 + threadIdx.x, j = by * DV_TY + threadIdx.y;
+    if (i >= I.mx || j >= I.my) continue;
+    const int k0 = bz * DV_KC, k1 = min(k0 + DV_KC, I.mz);
+    const bool ij = i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1;
+    bool have = false;
+    double c_uz = 0., c_n0 = 0., c_nt = 0.;   // the cell below's uz[t], nv[t], nv[t + sk]
+    for (int k = k0; k < k1; k++) {
+      const long t = I.at(i, j, k);
+      double dv = 0.0;
+      if (ij && k > 0 && k < I.mz - 1) {
+        const double uzb = have ? c_uz : uz[t - sk], nb_ = have ? c_n0 : nv[t - sk], n0 = have ? c_nt : nv[t];
+        const double uzt = uz[t], nt = nv[t + sk];
+        dv = fabs((ux[t] - ux[t - 1] + uy[t] - uy[t - sj] + uzt - uzb) * aj[t]);
+        if (n0 + nt + nb_ + nv[t + sj] + nv[t - sj] + nv[t + 1] + nv[t - 1] > 0.1) dv = 0.;
+        c_uz = uzt; c_n0 = n0; c_nt = nt;
+        have = true;
+      } else {
+        have = false;
+      }
+      if (div_beats(dv, t, best, bi)) { best = dv; bi = t; }
+    }
+  }
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  sv[tid] = best;
+  si[tid] = bi;
+  __syncthreads();
+  for (int s2 = BS / 2; s2 > 0; s2 >>= 1) {
+    if (tid < s2) {
+      double v2 = sv[tid + s2];
+      long i2 = si[tid + s2];
+      if (div_beats(v2, i2, sv[tid], si[tid])) { sv[tid] = v2; si[tid] = i2; }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) { pv[blockIdx.x] = sv[0]; pi[blockIdx.x] = (double)si[0]; }
 }
 
 int cv_divergence_dev(cv_ctx *c, double *maxdiv, long *idx) {
   CvLevel &F = c->lev[c->nlev - 1];
   const int nb = CV_RED_GRID;
-  hipLaunchKernelGGL(k_divmax, dim3(nb), dim3(BS), 0, c->stream, idx_of(F), (const double *)c->ucont[0],
+  hipLaunchKernelGGL(k_divmax_march, dim3(nb), dim3(64, DV_TY), 0, c->stream, idx_of(F), (const double *)c->ucont[0],
                      (const double *)c->ucont[1], (const double *)c->ucont[2], (const double *)F.aj,
-                     (const double *)F.nvert, F.d.n, c->part, c->part + nb);
+                     (const double *)F.nvert, c->part, c->part + nb);
   std::vector<double> hv(2 * nb);
   CV_CHECK(hipMemcpyAsync(hv.data(), c->part, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CV_CHECK(hipStreamSynchronize(c->stream));
