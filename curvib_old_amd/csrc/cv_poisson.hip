@@ -1837,10 +1837,11 @@ __device__ __forceinline__ static void ibm_flux_body(Idx I, const double *nv, do
   const double corr = mode ? corr_p[0] : 0.0;
   double f = 0.0, a = 0.0;
   const long sj = I.mx, sk = (long)I.mx * I.my;
-  for (long t = (long)bid * BS + threadIdx.x; t < n; t += (long)nb * BS) {
+  // one point as the reference's loop body visits it
+  auto visit = [&](long t) {
     int i, j, k;
     I.dec(t, i, j, k);
-    if (!(i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1)) continue;
+    if (!(i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1)) return;
     const double n0 = nv[t];
     const long nb[3] = {t + 1, t + sj, t + sk};
     const bool lim[3] = {i < I.mx - 2, j < I.my - 2, k < I.mz - 2};
@@ -1861,6 +1862,32 @@ __device__ __forceinline__ static void ibm_flux_body(Idx I, const double *nv, do
         }
       }
     }
+  };
+  auto band = [ibmval](double v) { return v > ibmval - 0.4 && v < ibmval; };
+  // the thread's points t, t + S, ... in order, four at a time: their
+  // nvert loads are issued together and only the points beside the body
+  // (a superset of those visit changes anything for) are visited, in the
+  // same order as one at a time, so the sums keep their bits
+  const long S = (long)nb * BS;
+  constexpr int U = 4;
+  for (long t0 = (long)bid * BS + threadIdx.x; t0 < n; t0 += U * S) {
+    unsigned cand = 0u;
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const long t = t0 + q * S;
+      if (t < n) {
+        int i, j, k;
+        I.dec(t, i, j, k);
+        if (i > 0 && i < I.mx - 1 && j > 0 && j < I.my - 1 && k > 0 && k < I.mz - 1) {
+          const double n0 = nv[t], a1 = nv[t + 1], a2 = nv[t + sj], a3 = nv[t + sk];
+          if ((n0 < 0.1 && (band(a1) || band(a2) || band(a3))) || (band(n0) && (a1 < 0.1 || a2 < 0.1 || a3 < 0.1)))
+            cand |= 1u << q;
+        }
+      }
+    }
+#pragma unroll 1
+    for (int q = 0; q < U; q++)
+      if ((cand >> q) & 1u) visit(t0 + q * S);
   }
   if (mode) return;
   double sf = cv_block_sum(f, sh);
